@@ -1,0 +1,282 @@
+/*
+ * lfpm.h — C ABI of the MI355X-native micro-image PatchMatch depth solver.
+ *
+ * Conventions
+ *  - Every entry point returns an int status: LFPM_OK (0) or a negative LFPM_E* code;
+ *    lfpm_last_error() gives the message of the last failure on the calling thread.
+ *  - "_dev" pointers are device (HBM) pointers on the current HIP device; all other pointers
+ *    are host pointers.  `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *    Device entry points are asynchronous on `stream` unless stated otherwise.
+ *  - A micro-image (MI) is a crop of crop x crop pixels of the raw frame.  Per-pixel arrays
+ *    are MI-major: element (v, y, x) sits at v*crop*crop + y*crop + x.
+ *  - Planes are float4 (nx, ny, nz, w) where w is the disparity at the pixel that owns the
+ *    plane; disparity d at local pixel q is w + (-nx/nz)*(q.x-p.x) + (-ny/nz)*(q.y-p.y).
+ *  - Correspondence between a reference MI r and neighbour n: local pixel p maps to
+ *    p + (c_r - c_n) * d / base, with c the MI centres in raw pixels.
+ *
+ * Each declaration cites the reference interface it replaces (paths relative to the
+ * reference repository root).
+ */
+#ifndef LFPM_H
+#define LFPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LFPM_OK 0
+#define LFPM_E_INVALID (-1)   /* bad argument / shape */
+#define LFPM_E_HIP (-2)       /* HIP runtime error */
+#define LFPM_E_NOMEM (-3)     /* allocation / budget failure */
+#define LFPM_E_UNSUPPORTED (-4)
+
+#define LFPM_MAX_NEIGHBORS 32
+#define LFPM_MAX_CROP 32
+#define LFPM_MAX_RADIUS 10
+
+/* Filter reject reasons (same codes as MIDisparityFilterCPU::PixelRejectReason,
+ * CheckModule/MIDisparityFilter.h:102-109). */
+#define LFPM_PR_KEEP 0
+#define LFPM_PR_COST 80
+#define LFPM_PR_NOSUPPORT 140
+#define LFPM_PR_CYCLE 200
+#define LFPM_PR_SPIKE 255
+
+/* ---------------------------------------------------------------------------------------
+ * Lens layout (hex micro-lens grid) and neighbour tables.
+ * Replaces DepthSolver::CreateProblemsAndDisNormals_Frame crop/abandon rule
+ * (DepthSolver.cpp:1880-1915), LightFieldParams::ComputeMIA_Math_Info (Common/Common.cpp:34-38),
+ * SelectNeighbors::CollectMIANeighImagesForMatch + SortNeighScoreForMatch
+ * (MINeighbor/SelectNeighbors.cpp:113-193; MVStereo/LFDepthInfo.cpp:214-241) and
+ * AdaptMIPMFrame::BuildViewIndexMap (MIStereo/AdaptMIPMFrame.cpp:145-222).
+ * ------------------------------------------------------------------------------------- */
+typedef struct lfpm_layout_params {
+    float pitch;               /* lens pitch = MI diameter in raw pixels (reference Base/baseline) */
+    int crop;                  /* MI crop side; <= 0 selects floor(pitch * 0.707106781186) */
+    int raw_w, raw_h;          /* raw frame size in pixels */
+    int max_neighbors;         /* K: row stride of the neighbour table (<= LFPM_MAX_NEIGHBORS) */
+    int lens_types;            /* 1 = single lens type, 3 = multi-focus 3-type hex pattern */
+    int neighbors_per_type[3]; /* neighbour cap per lens type (each <= max_neighbors) */
+} lfpm_layout_params;
+
+typedef struct lfpm_layout_info {
+    int num_mis;               /* V: kept micro-images */
+    int crop;                  /* resolved crop side */
+    int lattice_cols, lattice_rows;
+    float row_step;            /* pitch * sqrt(3)/2 */
+} lfpm_layout_info;
+
+/* Sizes of a layout (host only, no GPU). */
+int lfpm_layout_query(const lfpm_layout_params* p, lfpm_layout_info* out);
+
+/* Fill the layout arrays (host only).  Sizes: centers 2V floats, origins 2V ints,
+ * tilekeys 2V ints (lattice col,row), lens_type V ints, neighbor_ids V*K ints (-1 padded),
+ * neighbor_counts V ints, lattice_to_mi cols*rows ints (-1 where no kept MI).
+ * Any output pointer may be NULL.  MI order = (row, col), the QuadTreeTileKey order
+ * (Common/QuadTree.h:137-148).  Neighbours are ordered by distance, then angle. */
+int lfpm_layout_build(const lfpm_layout_params* p, float* centers, int* origins, int* tilekeys,
+                      int* lens_type, int* neighbor_ids, int* neighbor_counts,
+                      int* lattice_to_mi);
+
+/* ---------------------------------------------------------------------------------------
+ * Synthetic focused-plenoptic frame (host only; test and benchmark workload generator — the
+ * reference ships no data).  Seeded slanted planes in virtual space on a Voronoi partition,
+ * band-limited value-noise texture, Gaussian sensor noise.
+ * ------------------------------------------------------------------------------------- */
+typedef struct lfpm_synth_params {
+    int raw_w, raw_h;
+    float pitch;
+    int num_planes;
+    float vmin, vmax;          /* virtual depth range of the scene */
+    uint32_t scene_seed;
+    uint32_t noise_seed;
+    float noise_sigma;         /* in 0..255 units */
+} lfpm_synth_params;
+
+/* raw: raw_h*raw_w uint8.  gt_virtual_depth (optional, may be NULL): raw_h*raw_w float,
+ * 0 outside micro-lens discs. */
+int lfpm_synth_frame(const lfpm_synth_params* p, uint8_t* raw, float* gt_virtual_depth);
+
+/* Philox4x32-10 block (Salmon et al., SC'11) exposed for known-answer tests. */
+void lfpm_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ---------------------------------------------------------------------------------------
+ * PatchMatch solve parameters.  Mirrors PatchMatchParamsLF (MVStereo/LFDepthInfo.h:19-46)
+ * as filled by AdaptMIPMFrame::InitGPUParamsFromCPUParams (MIStereo/AdaptMIPMFrame.cpp:126-143).
+ * ------------------------------------------------------------------------------------- */
+typedef struct lfpm_pm_params {
+    int crop;                  /* MLA_Mask_Width_Cuda == MLA_Mask_Height_Cuda */
+    float base;                /* Base: disparity unit (lens pitch in px) */
+    int radius;                /* patch_size / 2 */
+    int radius_increment;      /* window stride (reference radius_increment) */
+    float sigma_spatial;
+    float sigma_color;         /* on [0,1] intensities */
+    float min_var;             /* variance gate (reference kMinVar=1e-5 on 0..255 -> 1e-5/255^2) */
+    int top_k;
+    float depth_min, depth_max;/* disparity search range */
+    int iterations;            /* red-black iterations */
+    int max_neighbors;         /* K: neighbour table row stride */
+    uint32_t seed;             /* Philox key word 0 */
+    uint32_t frame_uid;        /* Philox key word 1: results depend on (seed, frame_uid) only */
+    int storage;               /* 0 = FP32 tiles in LDS, 1 = FP16 tiles in LDS (FP32 accumulate) */
+    float cos_max_slant;       /* planes with |nz| < cos_max_slant score 2 (0 = no gate, the
+                                  reference behaviour; default 0.5 = 60 degrees) */
+} lfpm_pm_params;
+
+/* Fill defaults for a given crop/base (reference defaults + config conventions). */
+void lfpm_pm_params_default(lfpm_pm_params* p, int crop, float base);
+
+/* Gather MI crops from a raw uint8 frame into FP32 tiles in [0,1].
+ * Replaces DepthSolver::Slice_RawMLAImage (DepthSolver.cpp:1951-1997) and the uint8->float
+ * convertTo of AdaptMIPMFrame::BuildViewIndexMap (MIStereo/AdaptMIPMFrame.cpp:173-179).
+ * raw_dev: raw_h rows of raw_pitch bytes.  origins_dev: 2V ints.  tiles_dev: V*crop*crop. */
+int lfpm_slice_tiles(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pitch,
+                     const int* origins_dev, int num_mis, int crop, float* tiles_dev,
+                     void* stream);
+
+/* Same gather, intensities kept on the reference's 0..255 scale (the filter's gray input,
+ * MIDisparityFilterCPU::BuildTileInputsFromMaps, CheckModule/MIDisparityFilter.cpp:362-421). */
+int lfpm_slice_gray(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pitch,
+                    const int* origins_dev, int num_mis, int crop, float* gray_dev, void* stream);
+
+/* Device workspace (bytes) needed by lfpm_patchmatch_frame / _init / _phase. */
+size_t lfpm_patchmatch_workspace_size(int num_mis, const lfpm_pm_params* p);
+
+/* Full single-frame solve: random init, `iterations` x {red phase, black phase}, then the
+ * per-pixel disparity/baseline record for the final plane.
+ * Replaces RunPatchMatchCUDAForFrame_Impl (MIStereo/AdaptMIPMFrame.cu:949-995, declared at
+ * MIStereo/AdaptMIPMFrame.cpp:337-341) and its kernels RandomInitializationForFrame
+ * (:474-518) and CheckerboardUpdateFrame (:520-947).
+ * tiles_dev V*crop^2; centers_dev 2V; neighbor_ids_dev V*K; neighbor_counts_dev V.
+ * Outputs (device, caller-owned): plane 4*V*crop^2 (float4), cost V*crop^2,
+ * disp 4*V*crop^2 (float4: d, d*|b|/base, |b|, |b|/(d_real+1e-7) of the lowest-cost view),
+ * sel V*crop^2 (uint32 bitmask of selected views).
+ * sync_each_phase != 0 synchronises the stream after every launch (the reference's launch
+ * pattern); 0 leaves all launches queued. */
+int lfpm_patchmatch_frame(const float* tiles_dev, const float* centers_dev,
+                          const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                          int num_mis, const lfpm_pm_params* p,
+                          float* plane_dev, float* cost_dev, float* disp_dev, uint32_t* sel_dev,
+                          void* workspace_dev, size_t workspace_bytes, int sync_each_phase,
+                          void* stream);
+
+/* Building blocks of lfpm_patchmatch_frame for callers that drive the loop themselves.
+ * _init writes the random-initialised state; _phase reads (plane,cost,sel)_in and writes
+ * (plane,cost,sel)_out for one red/black phase; _finalize writes disp for the final planes. */
+int lfpm_patchmatch_init(const float* tiles_dev, const float* centers_dev,
+                         const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                         int num_mis, const lfpm_pm_params* p,
+                         float* plane_dev, float* cost_dev, uint32_t* sel_dev,
+                         void* workspace_dev, size_t workspace_bytes, void* stream);
+int lfpm_patchmatch_phase(const float* tiles_dev, const float* centers_dev,
+                          const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                          int num_mis, const lfpm_pm_params* p, int iter, int phase,
+                          const float* plane_in_dev, const float* cost_in_dev,
+                          const uint32_t* sel_in_dev,
+                          float* plane_out_dev, float* cost_out_dev, uint32_t* sel_out_dev,
+                          void* workspace_dev, size_t workspace_bytes, void* stream);
+int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
+                             const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                             int num_mis, const lfpm_pm_params* p, const float* plane_dev,
+                             float* disp_dev, void* stream);
+
+/* Cost of fixed plane hypotheses.  hyp_dev: n records of {int mi, int pixel, float4 plane}
+ * packed as 6 x 32-bit words (mi, pixel as int32; then nx, ny, nz, w as float).
+ * cost_out_dev: n*K floats (per-view bilateral NCC cost, 2 for views beyond the MI's count).
+ * Replaces ComputeMultiViewCostVector_Frame / ComputeBilateralNCC_Frame
+ * (MIStereo/AdaptMIPMFrame.cu:73-225). */
+int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,
+                    const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                    int num_mis, const lfpm_pm_params* p, const void* hyp_dev, int n,
+                    float* cost_out_dev, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Fused CheckModule pass.  Replaces MIDisparityFilterCPU::FilterGlobal/FilterTile
+ * (CheckModule/MIDisparityFilter.cpp:423-693, 962-1009) with configuration
+ * MIStereoMatch::FilterAndStatic (MIStereo/MIStereoMatch.cpp:759-778).  Reads a snapshot of
+ * the solve (all reads see pre-filter values) and writes filtered copies + reason mask.
+ * gray_dev: V*crop^2 floats in 0..255 (the uint8 crops as float).
+ * ------------------------------------------------------------------------------------- */
+typedef struct lfpm_filter_params {
+    int max_triplet;
+    float min_valid_disp;
+    int use_selected_views_only;
+    int clear_selected_views_when_invalid;
+    int enable_cost_filter;
+    float max_cost;
+    int enable_cycle_geo_filter;
+    float max_geo_err_px;
+    int enable_cycle_photo_filter;
+    float max_photo_err_u;
+    int min_good_neighbors;
+    int enable_spike_filter;
+    float spike_abs_diff;
+    int spike_min_neighbors;
+} lfpm_filter_params;
+
+void lfpm_filter_params_default(lfpm_filter_params* f);
+
+int lfpm_filter_frame(const float* gray_dev, const float* centers_dev,
+                      const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                      int num_mis, int crop, int max_neighbors, float base,
+                      const lfpm_filter_params* f,
+                      const float* plane_in_dev, const float* cost_in_dev,
+                      const float* disp_in_dev, const uint32_t* sel_in_dev,
+                      float* plane_out_dev, float* cost_out_dev, float* disp_out_dev,
+                      uint32_t* sel_out_dev, uint8_t* reason_dev, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Focal-stack refocus: S slices at virtual depths v_k = vmin + (vmax-vmin)*k/(S-1).
+ * Output pixel X (raw grid) of slice k = mean over MIs i with local coordinate
+ * u = (X - c_i)/v_k + (crop-1)/2 inside [0,crop-1]^2 of bilinear(tile_i, u); 0 if none.
+ * Built from LFRefocus::ComputeVirtual3D / BackProject2MI (FocalStack/LFRefocus.cpp:2100-2125)
+ * and the AIF mapping (FocalStack/LFRefocus.cpp:244-281).
+ * lattice_to_mi_dev: cols*rows ints from lfpm_layout_build.  out_dev: S*out_h*out_w floats.
+ * ------------------------------------------------------------------------------------- */
+int lfpm_refocus(const float* tiles_dev, const float* centers_dev, const int* lattice_to_mi_dev,
+                 int lattice_cols, int lattice_rows, float pitch, int crop,
+                 int out_w, int out_h, int num_slices, float vmin, float vmax,
+                 float* out_dev, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Virtual-to-real depth behavioural model, FP64 on the GPU.
+ * lfpm_fit_v2r replaces VirtualToRealDepthFunc::BehavioralModel
+ * (FocalStack/VirtualToRealDepthFunc.cpp:3390-3418): least squares of a_L on [a_L*v, v, 1].
+ * a_l_dev, v_dev: n floats (device).  c_out: 3 doubles (host), written after a stream sync.
+ * lfpm_v2r_convert replaces convertVirtualToRealDepth (:3421-3451):
+ * z = (c1*v + c2)/(1 - c0*v), non-finite or negative -> 0.
+ * ------------------------------------------------------------------------------------- */
+int lfpm_fit_v2r(const float* a_l_dev, const float* v_dev, int n, double c_out[3], void* stream);
+int lfpm_v2r_convert(const float* v_dev, int n, const double c[3], double* z_dev, void* stream);
+
+/* Work accounting for roofline reports (no reference counterpart).  While counters_dev (two
+ * device uint64: evaluated NCC window samples, evaluated (pixel, view, hypothesis) triples) is
+ * set, solver launches from the calling thread use counting kernel variants that accumulate into
+ * it; results are identical to the non-counting kernels.  NULL switches counting off. */
+int lfpm_set_work_counters(unsigned long long* counters_dev);
+
+/* ---------------------------------------------------------------------------------------
+ * Context API: frame-level drop-in used by the C++ facades (DepthSolver, AdaptMIPMFrame…).
+ * Replaces AdaptMIPMFrame::{Initialize, InitializeGPU, RunPatchMatchCUDAForFrame,
+ * WriteBackResults, ReleaseMemory} (MIStereo/AdaptMIPMFrame.cpp:48-124, 278-384).
+ * ------------------------------------------------------------------------------------- */
+typedef struct lfpm_ctx lfpm_ctx;
+
+int lfpm_create(int device, lfpm_ctx** out);
+void lfpm_destroy(lfpm_ctx* ctx);
+const char* lfpm_last_error(void);
+
+/* Upload a layout and a raw host frame, solve it, filter it, and copy results to host.
+ * Host outputs (any may be NULL): plane 4*V*crop^2, cost, disp 4*V*crop^2, sel, reason
+ * (filter reason mask; NULL skips the filter pass). */
+int lfpm_ctx_solve_frame(lfpm_ctx* ctx, const lfpm_layout_params* lp, const uint8_t* raw_host,
+                         const lfpm_pm_params* p, const lfpm_filter_params* f,
+                         float* plane, float* cost, float* disp, uint32_t* sel, uint8_t* reason);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LFPM_H */

@@ -1,0 +1,1222 @@
+// lfpm_oracle.cpp — TEST INFRASTRUCTURE ONLY.
+//
+// Deterministic CPU restatement (FP64, with an FP32 twin) of the micro-image PatchMatch path of
+// fengyunxuan6/plenoptic-for-depth, used as the parity checker for the HIP product library and as
+// the CPU baseline ("port").  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// may load it.  It shares no code with the product: its own Philox, its own brute-force lens
+// layout, its own NCC, solver, filter, refocus and fit.
+//
+// What each function follows (paths relative to the reference repository root):
+//   philox()                : Philox4x32-10, Salmon et al., "Parallel Random Numbers: As Easy as
+//                             1, 2, 3", SC'11 (replaces curand, MIStereo/AdaptMIPMFrame.cu:501).
+//   layout()                : crop rule Common/Common.cpp:34-38, abandon rule
+//                             DepthSolver.cpp:1880-1915, view order Common/QuadTree.h:137-148,
+//                             neighbour sort by baseline MVStereo/LFDepthInfo.cpp:214-241
+//                             (brute force over all MI pairs, ties broken by angle).
+//   ncc()                   : ComputeBilateralNCC_Frame, MIStereo/AdaptMIPMFrame.cu:73-179, with
+//                             ComputeBilateralWeight MIStereo/CudaUtil.cu:145-150.
+//   cost_vector()           : ComputeMultiViewCostVector_Frame, AdaptMIPMFrame.cu:181-225
+//                             (blur term off: the synthetic workloads carry no blur maps).
+//   topk()                  : ComputeSelectedViewsTopK_Frame, AdaptMIPMFrame.cu:227-282.
+//   weighted_cost()         : ComputeWeightedCost_Frame, AdaptMIPMFrame.cu:284-317.
+//   random_normal()         : GenerateRandomNormal_MIPM, MIStereo/AdaptMIPMUtil.cu:463-486
+//                             (Marsaglia 1972), rejection test done exactly in integers.
+//   perturbed_normal()      : GeneratePerturbedNormalLF, MIStereo/CudaUtil.cu:105-142.
+//   cross_view_proposal()   : BuildCrossViewPlaneProposal, AdaptMIPMFrame.cu:350-398, and
+//                             EncodePlaneHypothesisFromDispPlaneDevice :36-53.
+//   refine()                : PlaneHypothesisRefinement_Frame, AdaptMIPMFrame.cu:400-471, with
+//                             the candidate set fixed to the ACMP intent of
+//                             MIStereo/AdaptMIPMUtil.cu:181-183 (depths[] x normals[]).
+//   init_pixel()            : RandomInitializationForFrame, AdaptMIPMFrame.cu:474-518.
+//   update_pixel()          : CheckerboardUpdateFrame, AdaptMIPMFrame.cu:520-947.
+//   solve()                 : RunPatchMatchCUDAForFrame_Impl, AdaptMIPMFrame.cu:949-995.
+//   filter_f32()            : MIDisparityFilterCPU::FilterTile, CheckModule/MIDisparityFilter.cpp:
+//                             423-693 (+ helpers :25-174) with snapshot reads (every read sees
+//                             the pre-filter state), configuration MIStereoMatch.cpp:759-778.
+//   refocus()               : projection u = (X - c)/v + c_mi of LFRefocus::BackProject2MI,
+//                             FocalStack/LFRefocus.cpp:2100-2125 and :244-281.
+//   fit_v2r()/v2r_convert() : VirtualToRealDepthFunc::BehavioralModel / convertVirtualToRealDepth,
+//                             FocalStack/VirtualToRealDepthFunc.cpp:3390-3451 (Householder QR
+//                             least squares in FP64 instead of cv::solve(DECOMP_SVD)).
+//
+// Semantics the reference leaves undefined and that this restatement fixes (the product does
+// the same): planes whose normal is more than acos(cos_max_slant) from the optical axis score the
+// maximum cost 2 (cos_max_slant = 0 is the reference behaviour); hypothesis w is the disparity at the owning pixel (kept as the reference does);
+// refinement uses the ACMP depth x normal combination; disp record = (d, |d|*|b|/base, |b|,
+// |b|/(d_real+1e-7)) of the lowest-cost view; bilinear reads clamp to the crop edge; the
+// cross-view proposal denominator 1 + a*du + b*dv is kept exactly as the reference writes it.
+//
+// Parity pins: tests/test_oracle_pins.py (Philox KATs, closed-form NCC cases, hex-ring closed
+// forms, plane identities, exact-consistency filter scenes, constant-v refocus, exact-model fit).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+static const int KMAX = 32;
+
+// Filter reject reasons (CheckModule/MIDisparityFilter.h:102-109).
+enum : uint8_t { PR_KEEP = 0, PR_COST = 80, PR_NOSUPPORT = 140, PR_CYCLE = 200, PR_SPIKE = 255 };
+
+// ------------------------------------------------------------------------------------------
+// Philox4x32-10
+// ------------------------------------------------------------------------------------------
+static inline void philox(const uint32_t cin[4], const uint32_t kin[2], uint32_t out[4]) {
+    uint32_t c0 = cin[0], c1 = cin[1], c2 = cin[2], c3 = cin[3];
+    uint32_t k0 = kin[0], k1 = kin[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Per-(pixel, step) draw stream.  Slot s -> Philox block s/4, word s%4 of
+// ctr = (pixel_uid, step, s/4, 0), key = (seed, frame_uid).
+struct Stream {
+    uint32_t key[2];
+    uint32_t pix, step;
+    uint32_t cached_block;
+    uint32_t words[4];
+    Stream(uint32_t seed, uint32_t uid, uint32_t pix_, uint32_t step_)
+        : pix(pix_), step(step_), cached_block(0xFFFFFFFFu) {
+        key[0] = seed; key[1] = uid;
+    }
+    uint32_t raw(uint32_t slot) {
+        const uint32_t b = slot >> 2;
+        if (b != cached_block) {
+            const uint32_t c[4] = {pix, step, b, 0u};
+            philox(c, key, words);
+            cached_block = b;
+        }
+        return words[slot & 3u];
+    }
+};
+
+enum : uint32_t {
+    SLOT_INIT_DEPTH = 0,
+    SLOT_INIT_MARSAGLIA = 4,     // attempt a uses SLOT+2a, SLOT+2a+1
+    SLOT_VIEW_DRAW = 0,          // draws 0..14
+    SLOT_REF_DEPTH_RAND = 16,
+    SLOT_REF_DEPTH_PERT = 17,
+    SLOT_REF_ANGLE = 20,         // 20, 21, 22
+    SLOT_REF_MARSAGLIA = 24,
+    MARSAGLIA_ATTEMPTS = 16,
+    VIEW_DRAWS = 15,
+};
+
+template <typename T> static inline T uniform(uint32_t x) {
+    // ((x >> 8) + 0.5) * 2^-24: exact in FP32 and FP64, in (0, 1)
+    return ((T)(x >> 8) + (T)0.5) * (T)5.9604644775390625e-08;
+}
+
+// ------------------------------------------------------------------------------------------
+// Parameters (field-for-field identical to lfpm_pm_params / lfpm_filter_params /
+// lfpm_layout_params of the product ABI)
+// ------------------------------------------------------------------------------------------
+struct PmParams {
+    int crop; float base; int radius; int radius_increment;
+    float sigma_spatial; float sigma_color; float min_var; int top_k;
+    float depth_min, depth_max; int iterations; int max_neighbors;
+    uint32_t seed; uint32_t frame_uid; int storage;
+    float cos_max_slant;
+};
+
+struct FilterParams {
+    int max_triplet; float min_valid_disp; int use_selected_views_only;
+    int clear_selected_views_when_invalid; int enable_cost_filter; float max_cost;
+    int enable_cycle_geo_filter; float max_geo_err_px; int enable_cycle_photo_filter;
+    float max_photo_err_u; int min_good_neighbors; int enable_spike_filter;
+    float spike_abs_diff; int spike_min_neighbors;
+};
+
+struct LayoutParams {
+    float pitch; int crop; int raw_w, raw_h; int max_neighbors; int lens_types;
+    int neighbors_per_type[3];
+};
+
+// ------------------------------------------------------------------------------------------
+// Solver
+// ------------------------------------------------------------------------------------------
+template <typename T> struct P4 { T x, y, z, w; };
+
+template <typename T> struct Frame {
+    int V, W, K;
+    std::vector<T> img;           // V*W*W intensities in [0,1]
+    const float* centers;         // 2V
+    const int* nbr;               // V*K
+    const int* cnt;               // V
+    PmParams p;
+    T inv_base;
+
+    T I(int v, int x, int y) const { return img[(size_t)v * W * W + (size_t)y * W + x]; }
+
+    // Bilinear with clamp-to-edge; (x, y) in [0, W) x [0, W).
+    T bilinear(int v, T x, T y) const {
+        const T fx0 = std::floor(x), fy0 = std::floor(y);
+        const int x0 = (int)fx0, y0 = (int)fy0;
+        const int x1 = std::min(x0 + 1, W - 1), y1 = std::min(y0 + 1, W - 1);
+        const T fx = x - fx0, fy = y - fy0;
+        const T a = I(v, x0, y0), b = I(v, x1, y0), c = I(v, x0, y1), d = I(v, x1, y1);
+        const T top = a + fx * (b - a);
+        const T bot = c + fx * (d - c);
+        return top + fy * (bot - top);
+    }
+    int count(int v) const { return std::min(cnt[v], K); }
+};
+
+// Bilateral weights of one reference window (depend on the reference pixel only, so they are
+// computed once per pixel and reused for every hypothesis and view).
+template <typename T> struct Win {
+    T w[(2 * 10 + 1) * (2 * 10 + 1)];
+    T rc;
+};
+
+template <typename T>
+static void window_weights(const Frame<T>& f, int v, int px, int py, Win<T>& ww) {
+    const int r = f.p.radius, n = 2 * r + 1;
+    ww.rc = f.I(v, px, py);
+    const T ss = (T)2 * (T)f.p.sigma_spatial * (T)f.p.sigma_spatial;
+    const T sc = (T)2 * (T)f.p.sigma_color * (T)f.p.sigma_color;
+    for (int j = -r; j <= r; ++j)
+        for (int i = -r; i <= r; ++i) {
+            const int rx = px + i, ry = py + j;
+            T wv = 0;
+            if (rx >= 0 && rx < f.W && ry >= 0 && ry < f.W) {
+                const T spatial = std::sqrt((T)(i * i + j * j));
+                const T color = std::fabs(f.I(v, rx, ry) - ww.rc);
+                wv = std::exp(-spatial / ss - color / sc);
+            }
+            ww.w[(j + r) * n + (i + r)] = wv;
+        }
+}
+
+template <typename T>
+static T ncc(const Frame<T>& f, const Win<T>& ww, int v, int k, int px, int py, const P4<T>& h) {
+    const int n = f.nbr[(size_t)v * f.K + k];
+    if (n < 0) return (T)2;
+    const T sx = ((T)f.centers[2 * v] - (T)f.centers[2 * n]) * f.inv_base;
+    const T sy = ((T)f.centers[2 * v + 1] - (T)f.centers[2 * n + 1]) * f.inv_base;
+    const T W = (T)f.W;
+    const T ax = (T)px + sx * h.w, ay = (T)py + sy * h.w;
+    if (!(ax >= (T)0 && ax < W && ay >= (T)0 && ay < W)) return (T)2;
+    // slant gate: planes steeper than acos(cos_max_slant) are not scored
+    if (!(std::fabs(h.z) >= (T)f.p.cos_max_slant)) return (T)2;
+    const T A = -h.x / h.z, B = -h.y / h.z;
+    const int r = f.p.radius, inc = f.p.radius_increment, nn = 2 * r + 1;
+    T s_w = 0, s_x = 0, s_xx = 0, s_y = 0, s_yy = 0, s_xy = 0;
+    for (int i = -r; i <= r; i += inc) {
+        for (int j = -r; j <= r; j += inc) {
+            const int rx = px + i, ry = py + j;
+            if (rx < 0 || rx >= f.W || ry < 0 || ry >= f.W) continue;
+            const T d = h.w + A * (T)i + B * (T)j;
+            const T qx = (T)rx + sx * d, qy = (T)ry + sy * d;
+            if (!(qx >= (T)0 && qx < W && qy >= (T)0 && qy < W)) continue;
+            const T wgt = ww.w[(j + r) * nn + (i + r)];
+            const T x = f.I(v, rx, ry) - ww.rc;
+            const T y = f.bilinear(n, qx, qy) - ww.rc;
+            s_w += wgt;
+            s_x += wgt * x;
+            s_xx += wgt * x * x;
+            s_y += wgt * y;
+            s_yy += wgt * y * y;
+            s_xy += wgt * x * y;
+        }
+    }
+    if (s_w <= (T)1e-6) return (T)2;
+    const T inv = (T)1 / s_w;
+    const T mx = s_x * inv, my = s_y * inv;
+    const T vx = s_xx * inv - mx * mx;
+    const T vy = s_yy * inv - my * my;
+    if (vx < (T)f.p.min_var || vy < (T)f.p.min_var) return (T)2;
+    const T cov = s_xy * inv - mx * my;
+    const T c = (T)1 - cov / std::sqrt(vx * vy);
+    return std::min((T)2, std::max((T)0, c));
+}
+
+template <typename T>
+static void cost_vector(const Frame<T>& f, const Win<T>& ww, int v, int px, int py,
+                        const P4<T>& h, T* cv) {
+    for (int k = 0; k < KMAX; ++k) cv[k] = (T)2;
+    const int c = f.count(v);
+    for (int k = 0; k < c; ++k) cv[k] = ncc(f, ww, v, k, px, py, h);
+}
+
+template <typename T> static P4<T> disp_record(const Frame<T>& f, int v, int k, const P4<T>& h) {
+    const int n = f.nbr[(size_t)v * f.K + k];
+    const T bx = (T)f.centers[2 * v] - (T)f.centers[2 * n];
+    const T by = (T)f.centers[2 * v + 1] - (T)f.centers[2 * n + 1];
+    const T b = std::sqrt(bx * bx + by * by);
+    const T d = h.w;
+    const T dr = std::fabs(d) * b * f.inv_base;
+    return P4<T>{d, dr, b, b / (dr + (T)1e-7)};
+}
+
+template <typename T>
+static unsigned topk(const T* cv, int cnt, int top_k, T& mean, int& best) {
+    best = -1;
+    int valid = 0;
+    for (int i = 0; i < cnt; ++i)
+        if (cv[i] < (T)2) {
+            ++valid;
+            if (best < 0 || cv[i] < cv[best]) best = i;
+        }
+    if (valid <= 0) { mean = (T)2; return 0u; }
+    const int k = std::min(valid, top_k);
+    T tmp[KMAX];
+    for (int i = 0; i < cnt; ++i) tmp[i] = cv[i];
+    for (int i = 0; i < k; ++i)
+        for (int j = i + 1; j < cnt; ++j)
+            if (tmp[j] < tmp[i]) std::swap(tmp[i], tmp[j]);
+    mean = 0;
+    for (int i = 0; i < k; ++i) mean += tmp[i];
+    mean /= (T)k;
+    const T thr = tmp[k - 1];
+    unsigned m = 0;
+    for (int i = 0; i < cnt && i < 32; ++i)
+        if (cv[i] <= thr) m |= 1u << i;
+    return m;
+}
+
+template <typename T>
+static T weighted_cost(const T* cv, const T* vw, int cnt, int& argmin) {
+    T wn = 0, tot = 0, mc = (T)2;
+    argmin = -1;
+    for (int i = 0; i < cnt; ++i) {
+        if (vw[i] > (T)0) { wn += vw[i]; tot += vw[i] * cv[i]; }
+        if (cv[i] < mc) { mc = cv[i]; argmin = i; }
+    }
+    if (wn <= (T)1e-6) return (T)2;
+    return tot / wn;
+}
+
+template <typename T> static void normalize3(T& x, T& y, T& z) {
+    const T n = std::sqrt(x * x + y * y + z * z);
+    x = x / n; y = y / n; z = z / n;
+}
+
+template <typename T> static void random_normal(Stream& st, uint32_t base, T n[3]) {
+    int64_t A = 0, B = 0;
+    for (uint32_t a = 0; a < MARSAGLIA_ATTEMPTS; ++a) {
+        const int64_t ka = (int64_t)(st.raw(base + 2 * a) >> 8);
+        const int64_t kb = (int64_t)(st.raw(base + 2 * a + 1) >> 8);
+        const int64_t ta = 2 * ka + 1 - (int64_t(1) << 24);
+        const int64_t tb = 2 * kb + 1 - (int64_t(1) << 24);
+        if (ta * ta + tb * tb < (int64_t(1) << 48)) { A = ta; B = tb; break; }
+    }
+    const T q1 = (T)A * (T)5.9604644775390625e-08, q2 = (T)B * (T)5.9604644775390625e-08;
+    const T s = q1 * q1 + q2 * q2;
+    const T sq = std::sqrt((T)1 - s);
+    n[0] = (T)2 * q1 * sq;
+    n[1] = (T)2 * q2 * sq;
+    n[2] = (T)1 - (T)2 * s;
+    normalize3(n[0], n[1], n[2]);
+}
+
+template <typename T>
+static P4<T> perturbed_normal(Stream& st, int px, int py, const P4<T>& h, T perturbation) {
+    T vx = (T)px, vy = (T)py, vz = (T)1;
+    const T vn = std::sqrt(vx * vx + vy * vy + vz * vz);
+    vx = vx / vn; vy = vy / vn; vz = vz / vn;
+    const T a1 = (uniform<T>(st.raw(SLOT_REF_ANGLE)) - (T)0.5) * perturbation;
+    const T a2 = (uniform<T>(st.raw(SLOT_REF_ANGLE + 1)) - (T)0.5) * perturbation;
+    const T a3 = (uniform<T>(st.raw(SLOT_REF_ANGLE + 2)) - (T)0.5) * perturbation;
+    const T s1 = std::sin(a1), s2 = std::sin(a2), s3 = std::sin(a3);
+    const T c1 = std::cos(a1), c2 = std::cos(a2), c3 = std::cos(a3);
+    T R[9];
+    R[0] = c2 * c3;
+    R[1] = c3 * s1 * s2 - c1 * s3;
+    R[2] = s1 * s3 + c1 * c3 * s2;
+    R[3] = c2 * s3;
+    R[4] = c1 * c3 + s1 * s2 * s3;
+    R[5] = c1 * s2 * s3 - c3 * s1;
+    R[6] = -s2;
+    R[7] = c2 * s1;
+    R[8] = c1 * c2;
+    P4<T> o;
+    o.x = R[0] * h.x + R[1] * h.y + R[2] * h.z;
+    o.y = R[3] * h.x + R[4] * h.y + R[5] * h.z;
+    o.z = R[6] * h.x + R[7] * h.y + R[8] * h.z;
+    o.w = h.w;
+    if (o.x * vx + o.y * vy + o.z * vz >= (T)0) { o.x = h.x; o.y = h.y; o.z = h.z; }
+    normalize3(o.x, o.y, o.z);
+    return o;
+}
+
+// Neighbour plane hn (owned by neighbour pixel q) re-expressed as a reference-MI hypothesis at p
+// (BuildCrossViewPlaneProposal, AdaptMIPMFrame.cu:380-396, denominator as the reference has it).
+template <typename T>
+static P4<T> reencode_plane(const P4<T>& hn, int qx, int qy, T du, T dv, int px, int py) {
+    const T al = -hn.x / hn.z, be = -hn.y / hn.z;
+    const T ga = hn.w - al * (T)qx - be * (T)qy;
+    T den = (T)1 + al * du + be * dv;
+    if (std::fabs(den) < (T)1e-6) den = (den >= (T)0) ? (T)1e-6 : (T)-1e-6;
+    const T aR = al / den, bR = be / den, cR = ga / den;
+    T nx = -aR, ny = -bR, nz = (T)1;
+    normalize3(nx, ny, nz);
+    return P4<T>{nx, ny, nz, aR * (T)px + bR * (T)py + cR};
+}
+
+template <typename T>
+static bool cross_view_proposal(const Frame<T>& f, const P4<T>* plane_prev, int v, int n, int px,
+                                int py, const P4<T>& cur, P4<T>& out) {
+    const int W = f.W;
+    const T sx = ((T)f.centers[2 * v] - (T)f.centers[2 * n]) * f.inv_base;
+    const T sy = ((T)f.centers[2 * v + 1] - (T)f.centers[2 * n + 1]) * f.inv_base;
+    const T qfx = (T)px + sx * cur.w, qfy = (T)py + sy * cur.w;
+    const T fqx = std::floor(qfx + (T)0.5), fqy = std::floor(qfy + (T)0.5);
+    if (!(fqx >= (T)0 && fqx < (T)W && fqy >= (T)0 && fqy < (T)W)) return false;
+    const int qx = (int)fqx, qy = (int)fqy;
+    const P4<T>& hn = plane_prev[(size_t)n * W * W + (size_t)qy * W + qx];
+    const T du = (T)f.centers[2 * v] - (T)f.centers[2 * n];
+    const T dv = (T)f.centers[2 * v + 1] - (T)f.centers[2 * n + 1];
+    out = reencode_plane(hn, qx, qy, du, dv, px, py);
+    return true;
+}
+
+template <typename T> struct State {
+    P4<T>* plane; T* cost; P4<T>* disp; uint32_t* sel;
+};
+
+template <typename T>
+static void init_pixel(const Frame<T>& f, State<T>& s, int v, int x, int y) {
+    const int W = f.W;
+    const size_t idx = (size_t)v * W * W + (size_t)y * W + x;
+    Stream st(f.p.seed, f.p.frame_uid, (uint32_t)idx, 0u);
+    P4<T> h;
+    const T depth = uniform<T>(st.raw(SLOT_INIT_DEPTH)) * ((T)f.p.depth_max - (T)f.p.depth_min) +
+                    (T)f.p.depth_min;
+    T n[3];
+    random_normal<T>(st, SLOT_INIT_MARSAGLIA, n);
+    h.x = n[0]; h.y = n[1]; h.z = n[2]; h.w = depth;
+    Win<T> ww;
+    window_weights(f, v, x, y, ww);
+    T cv[KMAX];
+    cost_vector(f, ww, v, x, y, h, cv);
+    T mean; int best;
+    const unsigned m = topk(cv, f.count(v), f.p.top_k, mean, best);
+    s.plane[idx] = h;
+    s.cost[idx] = mean;
+    s.disp[idx] = best >= 0 ? disp_record(f, v, best, h) : P4<T>{0, 0, 0, 0};
+    s.sel[idx] = m;
+}
+
+template <typename T>
+static void update_pixel(const Frame<T>& f, const State<T>& prev, State<T>& next, int v, int x,
+                         int y, int iter, int phase) {
+    const int W = f.W, H = f.W;
+    const size_t ppv = (size_t)W * W;
+    const int center = y * W + x;
+    const size_t idx = (size_t)v * ppv + center;
+    const int cnt = f.count(v);
+    if (((x + y) & 1) != phase || cnt <= 0) {
+        next.plane[idx] = prev.plane[idx];
+        next.cost[idx] = prev.cost[idx];
+        next.disp[idx] = prev.disp[idx];
+        next.sel[idx] = prev.sel[idx];
+        return;
+    }
+    auto cprev = [&](int c) { return prev.cost[(size_t)v * ppv + c]; };
+    auto pprev = [&](int c) { return prev.plane[(size_t)v * ppv + c]; };
+
+    Win<T> ww;
+    window_weights(f, v, x, y, ww);
+
+    T ca[8][KMAX];
+    bool flag[8] = {false, false, false, false, false, false, false, false};
+    int pos[8];
+    for (int a = 0; a < 8; ++a) {
+        pos[a] = center;
+        for (int b = 0; b < KMAX; ++b) ca[a][b] = (T)2;
+    }
+    auto pick = [&](int& best, T& cm, int cand) {
+        if (cprev(cand) < cm) { cm = cprev(cand); best = cand; }
+    };
+    // far candidates: ±3 (±4) along the axis
+    if (y > 2) {
+        flag[1] = true; int b = center - 3 * W; T cm = cprev(b);
+        if (y > 3) pick(b, cm, center - 4 * W);
+        pos[1] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[1]);
+    }
+    if (y < H - 3) {
+        flag[3] = true; int b = center + 3 * W; T cm = cprev(b);
+        if (y < H - 4) pick(b, cm, center + 4 * W);
+        pos[3] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[3]);
+    }
+    if (x > 2) {
+        flag[5] = true; int b = center - 3; T cm = cprev(b);
+        if (x > 3) pick(b, cm, center - 4);
+        pos[5] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[5]);
+    }
+    if (x < W - 3) {
+        flag[7] = true; int b = center + 3; T cm = cprev(b);
+        if (x < W - 4) pick(b, cm, center + 4);
+        pos[7] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[7]);
+    }
+    // near candidates: ±1 plus two alternatives per step i (nerDis = 2)
+    if (y > 0) {
+        flag[0] = true; const int un = center - W; int b = un; T cm = cprev(b);
+        for (int i = 0; i < 2; ++i) {
+            if (y > 1 + i && x > i) pick(b, cm, un - (1 + i) * W - i);
+            if (y > 1 + i && x < W - 1 - i) pick(b, cm, un - (1 + i) * W + i);
+        }
+        pos[0] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[0]);
+    }
+    if (y <= 2 && y > 0) {
+        flag[1] = true; pos[1] = pos[0];
+        for (int j = 0; j < KMAX; ++j) ca[1][j] = ca[0][j];
+    }
+    if (y < H - 1) {
+        flag[2] = true; const int dn = center + W; int b = dn; T cm = cprev(b);
+        for (int i = 0; i < 2; ++i) {
+            if (y < H - 2 - i && x > i) pick(b, cm, dn + (1 + i) * W - i);
+            if (y < H - 2 - i && x < W - 1 - i) pick(b, cm, dn + (1 + i) * W + i);
+        }
+        pos[2] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[2]);
+    }
+    if (y >= H - 3 && y < H - 1) {
+        flag[3] = true; pos[3] = pos[2];
+        for (int j = 0; j < KMAX; ++j) ca[3][j] = ca[2][j];
+    }
+    if (x > 0) {
+        flag[4] = true; const int ln = center - 1; int b = ln; T cm = cprev(b);
+        for (int i = 0; i < 2; ++i) {
+            if (x > 1 + i && y > i) pick(b, cm, ln - (1 + i) - i * W);
+            if (x > 1 + i && y < H - 1 - i) pick(b, cm, ln - (1 + i) + i * W);
+        }
+        pos[4] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[4]);
+    }
+    if (x <= 2 && x > 0) {
+        flag[5] = true; pos[5] = pos[4];
+        for (int j = 0; j < KMAX; ++j) ca[5][j] = ca[4][j];
+    }
+    if (x < W - 1) {
+        flag[6] = true; const int rn = center + 1; int b = rn; T cm = cprev(b);
+        for (int i = 0; i < 2; ++i) {
+            if (x < W - 2 - i && y > i) pick(b, cm, rn + (1 + i) - i * W);
+            if (x < W - 2 - i && y < H - 1 - i) pick(b, cm, rn + (1 + i) + i * W);
+        }
+        pos[6] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[6]);
+    }
+    if (x >= W - 3 && x < W - 1) {
+        flag[7] = true; pos[7] = pos[6];
+        for (int j = 0; j < KMAX; ++j) ca[7][j] = ca[6][j];
+    }
+
+    // view-selection priors from the 4 direct neighbours' previous selections
+    T prior[KMAX];
+    for (int j = 0; j < KMAX; ++j) prior[j] = 0;
+    const int direct[4] = {y > 0 ? center - W : -1, y < H - 1 ? center + W : -1,
+                           x > 0 ? center - 1 : -1, x < W - 1 ? center + 1 : -1};
+    const int same_flag[4] = {0, 2, 4, 6};
+    for (int ii = 0; ii < 4; ++ii) {
+        if (direct[ii] < 0 || !flag[same_flag[ii]]) continue;
+        const uint32_t sp = prev.sel[(size_t)v * ppv + direct[ii]];
+        for (int j = 0; j < cnt; ++j) prior[j] += ((sp >> j) & 1u) ? (T)0.9 : (T)0.1;
+    }
+
+    // sampling probabilities -> CDF -> 15 draws
+    T prob[KMAX];
+    const T thr = (T)0.8 * std::exp((T)(iter * iter) / (T)-90.0);
+    for (int i = 0; i < cnt; ++i) {
+        T count = 0, tmpw = 0;
+        int cf = 0;
+        for (int j = 0; j < 8; ++j) {
+            if (!flag[j]) continue;
+            const T c = ca[j][i];
+            if (c < thr) { tmpw += std::exp(c * c / (T)-0.18); count += (T)1; }
+            if (c > (T)1.2) ++cf;
+        }
+        T pr = 0;
+        if (count > (T)2 && cf < 3) pr = tmpw / count;
+        else if (cf < 3) pr = std::exp(thr * thr / (T)-0.32);
+        prob[i] = pr * prior[i];
+    }
+    T psum = 0;
+    for (int i = 0; i < cnt; ++i) psum += prob[i];
+    T vw[KMAX];
+    for (int i = 0; i < KMAX; ++i) vw[i] = 0;
+    Stream st(f.p.seed, f.p.frame_uid, (uint32_t)idx, (uint32_t)(1 + 2 * iter + phase));
+    if (psum > (T)0) {
+        const T inv = (T)1 / psum;
+        T cum = 0;
+        for (int i = 0; i < cnt; ++i) { cum += prob[i] * inv; prob[i] = cum; }
+        for (uint32_t sdraw = 0; sdraw < VIEW_DRAWS; ++sdraw) {
+            const T r = uniform<T>(st.raw(SLOT_VIEW_DRAW + sdraw)) - (T)1.1920928955078125e-07;
+            for (int id = 0; id < cnt; ++id)
+                if (prob[id] > r) { vw[id] += (T)1; break; }
+        }
+    }
+    uint32_t tsel = 0;
+    T wn = 0;
+    for (int i = 0; i < cnt; ++i)
+        if (vw[i] > (T)0) { tsel |= 1u << i; wn += vw[i]; }
+    if (wn <= (T)0) {
+        for (int i = 0; i < cnt; ++i) { vw[i] = (T)1; tsel |= 1u << i; }
+    }
+
+    P4<T> best = prev.plane[idx];
+    T bcost;
+    P4<T> bdisp{0, 0, 0, 0};
+    int am;
+    {
+        T cv[KMAX];
+        cost_vector(f, ww, v, x, y, best, cv);
+        bcost = weighted_cost(cv, vw, cnt, am);
+        if (am >= 0) bdisp = disp_record(f, v, am, best);
+    }
+    for (int i = 0; i < 8; ++i) {
+        if (!flag[i]) continue;
+        const T c = weighted_cost(ca[i], vw, cnt, am);
+        if (c < bcost) {
+            bcost = c;
+            best = pprev(pos[i]);
+            bdisp = am >= 0 ? disp_record(f, v, am, best) : P4<T>{0, 0, 0, 0};
+        }
+    }
+    // cross-view proposals through every selected view
+    for (int k = 0; k < cnt; ++k) {
+        if (!((tsel >> k) & 1u)) continue;
+        const int n = f.nbr[(size_t)v * f.K + k];
+        if (n < 0) continue;
+        P4<T> prop;
+        if (!cross_view_proposal(f, prev.plane, v, n, x, y, best, prop)) continue;
+        T cv[KMAX];
+        cost_vector(f, ww, v, x, y, prop, cv);
+        const T c = weighted_cost(cv, vw, cnt, am);
+        if (c < bcost) {
+            bcost = c;
+            best = prop;
+            bdisp = am >= 0 ? disp_record(f, v, am, best) : P4<T>{0, 0, 0, 0};
+        }
+    }
+    // refinement (ACMP depth x normal candidates)
+    {
+        const T pert = (T)0.02;
+        const T dnow = best.w;
+        T drand;
+        if (bcost < (T)0.1)
+            drand = (uniform<T>(st.raw(SLOT_REF_DEPTH_RAND)) - (T)0.5) * (T)2 + dnow;
+        else
+            drand = uniform<T>(st.raw(SLOT_REF_DEPTH_RAND)) *
+                        ((T)f.p.depth_max - (T)f.p.depth_min) + (T)f.p.depth_min;
+        T nr[3];
+        random_normal<T>(st, SLOT_REF_MARSAGLIA, nr);
+        const P4<T> np = perturbed_normal<T>(st, x, y, best, pert * (T)M_PI);
+        const T lo = ((T)1 - pert) * dnow, hi = ((T)1 + pert) * dnow;
+        const T dpert = uniform<T>(st.raw(SLOT_REF_DEPTH_PERT)) * (hi - lo) + lo;
+        P4<T> cand[5];
+        cand[0] = P4<T>{best.x, best.y, best.z, drand};
+        cand[1] = P4<T>{nr[0], nr[1], nr[2], dnow};
+        cand[2] = P4<T>{nr[0], nr[1], nr[2], drand};
+        cand[3] = P4<T>{np.x, np.y, np.z, dnow};
+        cand[4] = P4<T>{best.x, best.y, best.z, dpert};
+        for (int i = 0; i < 5; ++i) {
+            T cv[KMAX];
+            cost_vector(f, ww, v, x, y, cand[i], cv);
+            const T c = weighted_cost(cv, vw, cnt, am);
+            if (c < bcost) {
+                bcost = c;
+                best = cand[i];
+                bdisp = am >= 0 ? disp_record(f, v, am, best) : P4<T>{0, 0, 0, 0};
+            }
+        }
+    }
+    next.plane[idx] = best;
+    next.cost[idx] = bcost;
+    next.disp[idx] = bdisp;
+    next.sel[idx] = tsel;
+}
+
+template <typename T>
+static void build_frame(Frame<T>& f, const uint8_t* tiles_u8, const float* centers, const int* nbr,
+                        const int* cnt, int V, const PmParams& p) {
+    f.V = V; f.W = p.crop; f.K = p.max_neighbors;
+    f.centers = centers; f.nbr = nbr; f.cnt = cnt; f.p = p;
+    f.inv_base = (T)1 / std::max((T)p.base, (T)1e-6);
+    const size_t n = (size_t)V * p.crop * p.crop;
+    f.img.resize(n);
+    for (size_t i = 0; i < n; ++i) f.img[i] = (T)tiles_u8[i] / (T)255;
+}
+
+template <typename T>
+static void solve(const uint8_t* tiles_u8, const float* centers, const int* nbr, const int* cnt,
+                  int V, const PmParams& p, T* plane_out, T* cost_out, T* disp_out,
+                  uint32_t* sel_out, int nthreads) {
+    Frame<T> f;
+    build_frame(f, tiles_u8, centers, nbr, cnt, V, p);
+    const size_t n = (size_t)V * p.crop * p.crop;
+    std::vector<P4<T>> pa(n), pb(n), da(n), db(n);
+    std::vector<T> ca(n), cb(n);
+    std::vector<uint32_t> sa(n), sb(n);
+    State<T> A{pa.data(), ca.data(), da.data(), sa.data()};
+    State<T> B{pb.data(), cb.data(), db.data(), sb.data()};
+    const int W = p.crop;
+    if (nthreads <= 0) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+    for (int v = 0; v < V; ++v)
+        for (int y = 0; y < W; ++y)
+            for (int x = 0; x < W; ++x) init_pixel(f, A, v, x, y);
+    for (int it = 0; it < p.iterations; ++it) {
+        for (int ph = 0; ph < 2; ++ph) {
+            const State<T>& src = ph == 0 ? A : B;
+            State<T>& dst = ph == 0 ? B : A;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+            for (int v = 0; v < V; ++v)
+                for (int y = 0; y < W; ++y)
+                    for (int x = 0; x < W; ++x) update_pixel(f, src, dst, v, x, y, it, ph);
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        plane_out[4 * i] = pa[i].x; plane_out[4 * i + 1] = pa[i].y;
+        plane_out[4 * i + 2] = pa[i].z; plane_out[4 * i + 3] = pa[i].w;
+        cost_out[i] = ca[i];
+        disp_out[4 * i] = da[i].x; disp_out[4 * i + 1] = da[i].y;
+        disp_out[4 * i + 2] = da[i].z; disp_out[4 * i + 3] = da[i].w;
+        sel_out[i] = sa[i];
+    }
+}
+
+template <typename T>
+static void eval_costs(const uint8_t* tiles_u8, const float* centers, const int* nbr,
+                       const int* cnt, int V, const PmParams& p, const int32_t* hyp, int n,
+                       T* out, int nthreads) {
+    Frame<T> f;
+    build_frame(f, tiles_u8, centers, nbr, cnt, V, p);
+    const int K = p.max_neighbors, W = p.crop;
+    if (nthreads <= 0) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+    for (int i = 0; i < n; ++i) {
+        const int32_t* r = hyp + 6 * (size_t)i;
+        const int v = r[0], pix = r[1];
+        float fl[4];
+        std::memcpy(fl, r + 2, sizeof(fl));
+        const P4<T> h{(T)fl[0], (T)fl[1], (T)fl[2], (T)fl[3]};
+        Win<T> ww;
+        const int x = pix % W, y = pix / W;
+        window_weights(f, v, x, y, ww);
+        T cv[KMAX];
+        cost_vector(f, ww, v, x, y, h, cv);
+        for (int k = 0; k < K; ++k) out[(size_t)i * K + k] = cv[k];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Layout (brute force)
+// ------------------------------------------------------------------------------------------
+static int resolve_crop(const LayoutParams& lp) {
+    return lp.crop > 0 ? lp.crop : (int)(lp.pitch * 0.707106781186f);
+}
+
+// Ideal hex centre snapped so that the crop origin c - (crop-1)/2 is an integer.
+static void lattice_center(float pitch, int crop, int i, int j, float& cx, float& cy) {
+    const double p = (double)pitch, h = (crop - 1) * 0.5;
+    const double ix = p * ((double)i + 0.5 * (double)(1 + (j & 1)));
+    const double iy = p * (0.5 + 0.8660254037844386 * (double)j);
+    cx = (float)(std::floor(ix - h + 0.5) + h);
+    cy = (float)(std::floor(iy - h + 0.5) + h);
+}
+
+static int layout(const LayoutParams& lp, int cap, float* centers, int* origins, int* tilekeys,
+                  int* lens_type, int* nbr, int* cnt) {
+    const int crop = resolve_crop(lp);
+    const double rs = (double)lp.pitch * 0.8660254037844386;
+    const int cols = (int)std::ceil(lp.raw_w / (double)lp.pitch) + 1;
+    const int rows = (int)std::ceil(lp.raw_h / rs) + 1;
+    struct MI { int i, j; float cx, cy; int ox, oy; };
+    std::vector<MI> mis;
+    const double half = (crop - 1) * 0.5;
+    for (int j = 0; j < rows; ++j)
+        for (int i = 0; i < cols; ++i) {
+            float cx, cy;
+            lattice_center(lp.pitch, crop, i, j, cx, cy);
+            const double lx = (double)cx - half, ly = (double)cy - half;
+            if (lx < 0 || ly < 0) continue;
+            const int ox = (int)std::floor(lx), oy = (int)std::floor(ly);
+            if (ox + crop > lp.raw_w || oy + crop > lp.raw_h) continue;
+            mis.push_back(MI{i, j, cx, cy, ox, oy});
+        }
+    const int V = (int)mis.size();
+    if (cap < V) return V;
+    const int K = lp.max_neighbors;
+    for (int v = 0; v < V; ++v) {
+        const MI& m = mis[v];
+        if (centers) { centers[2 * v] = m.cx; centers[2 * v + 1] = m.cy; }
+        if (origins) { origins[2 * v] = m.ox; origins[2 * v + 1] = m.oy; }
+        if (tilekeys) { tilekeys[2 * v] = m.i; tilekeys[2 * v + 1] = m.j; }
+        const int q = m.i - (m.j - (m.j & 1)) / 2;
+        const int t = lp.lens_types == 3 ? (((q - m.j) % 3) + 3) % 3 : 0;
+        if (lens_type) lens_type[v] = t;
+        if (!nbr && !cnt) continue;
+        struct Cand { int d4; double ang; int id; };
+        std::vector<Cand> cs;
+        int capn = lp.lens_types == 3 ? lp.neighbors_per_type[t] : K;
+        capn = std::min(capn, K);
+        // distance class of the capn-th neighbour of an interior MI: 6 at 4d^2=4 (ring 1),
+        // 6 at 4d^2=12 and 6 at 4d^2=16 (ring 2)
+        const int d4max = capn <= 6 ? 4 : (capn <= 12 ? 12 : 16);
+        for (int u = 0; u < V; ++u) {
+            if (u == v) continue;
+            const int dx2 = 2 * (mis[u].i - m.i) + ((mis[u].j & 1) - (m.j & 1));
+            const int dy = mis[u].j - m.j;
+            const int d4 = dx2 * dx2 + 3 * dy * dy;
+            if (d4 > d4max) continue;
+            double a = std::atan2(std::sqrt(3.0) * dy, (double)dx2);
+            if (a < 0) a += 2.0 * M_PI;
+            cs.push_back(Cand{d4, a, u});
+        }
+        std::sort(cs.begin(), cs.end(), [](const Cand& a, const Cand& b) {
+            return a.d4 != b.d4 ? a.d4 < b.d4 : a.ang < b.ang;
+        });
+        const int c = std::min((int)cs.size(), capn);
+        if (cnt) cnt[v] = c;
+        if (nbr)
+            for (int k = 0; k < K; ++k) nbr[(size_t)v * K + k] = k < c ? cs[k].id : -1;
+    }
+    return V;
+}
+
+// ------------------------------------------------------------------------------------------
+// Fused CheckModule filter, FP32 twin (bit-exact contract with the GPU pass)
+// ------------------------------------------------------------------------------------------
+struct FilterCtx {
+    int V, W, K;
+    const float* gray; const float* centers; const int* nbr; const int* cnt;
+    float base;
+    const float* plane; const float* cost;
+    FilterParams fp;
+    float G(int v, int x, int y) const { return gray[(size_t)v * W * W + (size_t)y * W + x]; }
+    float D(int v, int i) const { return plane[4 * ((size_t)v * W * W + i) + 3]; }
+    float C(int v, int i) const { return cost[(size_t)v * W * W + i]; }
+};
+
+static float f_nan() { return std::nanf(""); }
+
+static float bilin_disp(const FilterCtx& c, int n, float x, float y) {
+    const int W = c.W;
+    if (x < 0 || y < 0 || x > W - 1 || y > W - 1) return f_nan();
+    const int x0 = (int)std::floor(x), y0 = (int)std::floor(y);
+    const int x1 = std::min(x0 + 1, W - 1), y1 = std::min(y0 + 1, W - 1);
+    const float ax = x - (float)x0, ay = y - (float)y0;
+    auto dat = [&](int xi, int yi) {
+        const float d = c.D(n, yi * W + xi);
+        if (!std::isfinite(d) || d <= c.fp.min_valid_disp) return f_nan();
+        return d;
+    };
+    const float d00 = dat(x0, y0), d01 = dat(x0, y1), d10 = dat(x1, y0), d11 = dat(x1, y1);
+    if (!std::isfinite(d00) || !std::isfinite(d01) || !std::isfinite(d10) || !std::isfinite(d11))
+        return f_nan();
+    return (1.0f - ax) * (1.0f - ay) * d00 + (1.0f - ax) * ay * d01 + ax * (1.0f - ay) * d10 +
+           ax * ay * d11;
+}
+
+static float bilin_cost(const FilterCtx& c, int n, float x, float y) {
+    const int W = c.W;
+    if (x < 0 || y < 0 || x > W - 1 || y > W - 1) return f_nan();
+    const int x0 = (int)std::floor(x), y0 = (int)std::floor(y);
+    const int x1 = std::min(x0 + 1, W - 1), y1 = std::min(y0 + 1, W - 1);
+    const float ax = x - (float)x0, ay = y - (float)y0;
+    const float c00 = c.C(n, y0 * W + x0), c01 = c.C(n, y1 * W + x0);
+    const float c10 = c.C(n, y0 * W + x1), c11 = c.C(n, y1 * W + x1);
+    if (!std::isfinite(c00) || !std::isfinite(c01) || !std::isfinite(c10) || !std::isfinite(c11))
+        return f_nan();
+    return (1.0f - ax) * (1.0f - ay) * c00 + (1.0f - ax) * ay * c01 + ax * (1.0f - ay) * c10 +
+           ax * ay * c11;
+}
+
+static bool bilin_gray(const FilterCtx& c, int v, float x, float y, float& out) {
+    const int W = c.W;
+    if (x < 0 || y < 0 || x > W - 1 || y > W - 1) return false;
+    const int x0 = (int)std::floor(x), y0 = (int)std::floor(y);
+    const int x1 = std::min(x0 + 1, W - 1), y1 = std::min(y0 + 1, W - 1);
+    const float ax = x - (float)x0, ay = y - (float)y0;
+    const float v00 = c.G(v, x0, y0), v01 = c.G(v, x0, y1), v10 = c.G(v, x1, y0),
+                v11 = c.G(v, x1, y1);
+    out = (1.0f - ax) * (1.0f - ay) * v00 + (1.0f - ax) * ay * v01 + ax * (1.0f - ay) * v10 +
+          ax * ay * v11;
+    return true;
+}
+
+static bool mean_std3(const FilterCtx& c, int v, float fx, float fy, float& mean, float& sd) {
+    const int cx = (int)std::round(fx), cy = (int)std::round(fy);
+    if (cx < 1 || cy < 1 || cx >= c.W - 1 || cy >= c.W - 1) return false;
+    double s1 = 0, s2 = 0;
+    for (int yy = cy - 1; yy <= cy + 1; ++yy)
+        for (int xx = cx - 1; xx <= cx + 1; ++xx) {
+            const double g = (double)c.G(v, xx, yy);
+            s1 += g;
+            s2 += g * g;
+        }
+    const double scale = 1.0 / 9.0;
+    const double m = s1 * scale;
+    const double var = std::max(s2 * scale - m * m, 0.0);
+    mean = (float)m;
+    sd = (float)std::sqrt(var);
+    return std::isfinite(mean) && std::isfinite(sd) && sd > 1.0e-3f;
+}
+
+static float photo_residual(const FilterCtx& c, int v, int n, float px, float py, float qx,
+                            float qy) {
+    float Ir = 0, In = 0;
+    if (!bilin_gray(c, v, px, py, Ir)) return f_nan();
+    if (!bilin_gray(c, n, qx, qy, In)) return f_nan();
+    float mr, sr, mn, sn;
+    if (mean_std3(c, v, px, py, mr, sr) && mean_std3(c, n, qx, qy, mn, sn)) {
+        const float a = (Ir - mr) / sr;
+        const float b = (In - mn) / sn;
+        return std::fabs(a - b);
+    }
+    return std::fabs(In - Ir);
+}
+
+static void filter_f32(const FilterCtx& c, const float* disp_in, const uint32_t* sel_in,
+                       float* plane_out, float* cost_out, float* disp_out, uint32_t* sel_out,
+                       uint8_t* reason_out, int nthreads) {
+    const int W = c.W, N = W * W;
+    const FilterParams& fp = c.fp;
+    if (nthreads <= 0) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthreads)
+    for (int v = 0; v < c.V; ++v) {
+        std::vector<uint8_t> k1(N, 0), kf(N, 0), reason(N, PR_KEEP);
+        const int cntv = std::min(c.cnt[v], c.K);
+        for (int y = 0; y < W; ++y)
+            for (int x = 0; x < W; ++x) {
+                const int i = y * W + x;
+                const size_t g = (size_t)v * N + i;
+                const float d = c.D(v, i);
+                if (!std::isfinite(d) || d <= fp.min_valid_disp) continue;
+                const float cost = c.C(v, i);
+                if (fp.enable_cost_filter) {
+                    const bool invalid = !std::isfinite(cost) || cost < 0.0f;
+                    const bool high = std::isfinite(cost) && cost > fp.max_cost;
+                    if (invalid || high) { reason[i] = PR_COST; continue; }
+                }
+                int vids[32];
+                int nv = 0;
+                if (fp.use_selected_views_only) {
+                    uint32_t m = sel_in[g];
+                    for (int b = 0; m && nv < fp.max_triplet; ++b) {
+                        if (m & 1u) vids[nv++] = b;
+                        m >>= 1u;
+                    }
+                } else {
+                    const int lim = std::min(cntv, fp.max_triplet);
+                    for (int b = 0; b < lim; ++b) vids[nv++] = b;
+                }
+                if (nv == 0) { reason[i] = PR_NOSUPPORT; continue; }
+                const float px = (float)x, py = (float)y;
+                int support = 0, good = 0;
+                for (int vi = 0; vi < nv; ++vi) {
+                    const int b = vids[vi];
+                    if (b < 0 || b >= cntv) continue;
+                    const int n = c.nbr[(size_t)v * c.K + b];
+                    if (n < 0) continue;
+                    const float Bx = c.centers[2 * v] - c.centers[2 * n];
+                    const float By = c.centers[2 * v + 1] - c.centers[2 * n + 1];
+                    const float bl = std::sqrt(Bx * Bx + By * By);
+                    if (bl < 1.0e-6f) continue;
+                    const float dx = Bx / bl, dy = By / bl;
+                    const float scale = bl / c.base;
+                    const float qx = px + dx * d * scale;
+                    const float qy = py + dy * d * scale;
+                    if (qx < 0 || qy < 0 || qx > W - 1 || qy > W - 1) continue;
+                    const float dn = bilin_disp(c, n, qx, qy);
+                    if (!std::isfinite(dn)) continue;
+                    if (fp.enable_cost_filter) {
+                        const float nc = bilin_cost(c, n, qx, qy);
+                        const bool inv = !std::isfinite(nc) || nc < 0.0f;
+                        const bool hi = std::isfinite(nc) && nc > fp.max_cost;
+                        if (inv || hi) continue;
+                    }
+                    ++support;
+                    bool pg = true, pp = true;
+                    if (fp.enable_cycle_geo_filter) {
+                        const float phx = qx - dx * dn * scale;
+                        const float phy = qy - dy * dn * scale;
+                        const float ex = phx - px, ey = phy - py;
+                        const float ge = std::sqrt(ex * ex + ey * ey);
+                        pg = std::isfinite(ge) && ge <= fp.max_geo_err_px;
+                    }
+                    if (fp.enable_cycle_photo_filter) {
+                        const float pe = photo_residual(c, v, n, px, py, qx, qy);
+                        pp = std::isfinite(pe) && pe <= fp.max_photo_err_u;
+                    }
+                    if (pg && pp) ++good;
+                }
+                if (support < fp.min_good_neighbors) { reason[i] = PR_NOSUPPORT; continue; }
+                if (good < fp.min_good_neighbors) { reason[i] = PR_CYCLE; continue; }
+                k1[i] = 1; kf[i] = 1; reason[i] = PR_KEEP;
+            }
+        if (fp.enable_spike_filter) {
+            for (int y = 0; y < W; ++y)
+                for (int x = 0; x < W; ++x) {
+                    const int i = y * W + x;
+                    if (!k1[i]) continue;
+                    float vals[8];
+                    int nvals = 0;
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            if (dx == 0 && dy == 0) continue;
+                            const int xn = x + dx, yn = y + dy;
+                            if (xn < 0 || yn < 0 || xn >= W || yn >= W) continue;
+                            const int ni = yn * W + xn;
+                            if (!k1[ni]) continue;
+                            const float dv = c.D(v, ni);
+                            if (!std::isfinite(dv) || dv <= fp.min_valid_disp) continue;
+                            vals[nvals++] = dv;
+                        }
+                    if (nvals < fp.spike_min_neighbors) continue;
+                    std::sort(vals, vals + nvals);
+                    const float med = vals[nvals / 2];
+                    const float cur = c.D(v, i);
+                    if (!std::isfinite(cur)) continue;
+                    if (std::fabs(cur - med) > fp.spike_abs_diff) {
+                        kf[i] = 0;
+                        reason[i] = PR_SPIKE;
+                    }
+                }
+        }
+        for (int i = 0; i < N; ++i) {
+            const size_t g = (size_t)v * N + i;
+            const float d = c.D(v, i);
+            const bool drop = std::isfinite(d) && d > fp.min_valid_disp && !kf[i];
+            for (int q = 0; q < 4; ++q) {
+                plane_out[4 * g + q] = c.plane[4 * g + q];
+                disp_out[4 * g + q] = disp_in[4 * g + q];
+            }
+            cost_out[g] = c.cost[g];
+            sel_out[g] = sel_in[g];
+            if (drop) {
+                plane_out[4 * g] = 0.0f; plane_out[4 * g + 1] = 0.0f;
+                plane_out[4 * g + 2] = 0.0f; plane_out[4 * g + 3] = -1.0f;
+                cost_out[g] = 2.0f;
+                disp_out[4 * g] = -1.0f; disp_out[4 * g + 1] = -1.0f;
+                disp_out[4 * g + 2] = 0.0f; disp_out[4 * g + 3] = 0.0f;
+                if (fp.clear_selected_views_when_invalid) sel_out[g] = 0u;
+            }
+            reason_out[g] = reason[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Refocus (FP64 samples; MI inclusion decided with the FP32 expression the product uses)
+// ------------------------------------------------------------------------------------------
+static void refocus(const uint8_t* tiles_u8, const float* centers, int V, float pitch, int crop,
+                    int out_w, int out_h, int S, float vmin, float vmax, double* out,
+                    int nthreads) {
+    const int W = crop;
+    // uniform bucket grid of MI centres (cell = pitch)
+    const int gw = (int)std::ceil(out_w / (double)pitch) + 4;
+    const int gh = (int)std::ceil(out_h / (double)pitch) + 4;
+    std::vector<std::vector<int>> cell((size_t)gw * gh);
+    for (int v = 0; v < V; ++v) {
+        int cx = (int)std::floor(centers[2 * v] / pitch) + 2;
+        int cy = (int)std::floor(centers[2 * v + 1] / pitch) + 2;
+        cx = std::min(std::max(cx, 0), gw - 1);
+        cy = std::min(std::max(cy, 0), gh - 1);
+        cell[(size_t)cy * gw + cx].push_back(v);
+    }
+    const float halff = (float)(crop - 1) * 0.5f;
+    const double half = (crop - 1) * 0.5;
+    if (nthreads <= 0) nthreads = 1;
+    for (int k = 0; k < S; ++k) {
+        const float vk = S > 1 ? vmin + (vmax - vmin) * (float)k / (float)(S - 1) : vmin;
+        const double reach = (double)vk * half + 1.0;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+        for (int y = 0; y < out_h; ++y)
+            for (int x = 0; x < out_w; ++x) {
+                const int gx0 = std::max((int)std::floor((x - reach) / pitch) + 2, 0);
+                const int gx1 = std::min((int)std::floor((x + reach) / pitch) + 2, gw - 1);
+                const int gy0 = std::max((int)std::floor((y - reach) / pitch) + 2, 0);
+                const int gy1 = std::min((int)std::floor((y + reach) / pitch) + 2, gh - 1);
+                double acc = 0;
+                int nacc = 0;
+                for (int gy = gy0; gy <= gy1; ++gy)
+                    for (int gx = gx0; gx <= gx1; ++gx)
+                        for (int v : cell[(size_t)gy * gw + gx]) {
+                            const float cx = centers[2 * v], cy = centers[2 * v + 1];
+                            const float ufx = ((float)x - cx) / vk + halff;
+                            const float ufy = ((float)y - cy) / vk + halff;
+                            if (!(ufx >= 0.0f && ufx <= (float)(W - 1) && ufy >= 0.0f &&
+                                  ufy <= (float)(W - 1)))
+                                continue;
+                            const double ux = ((double)x - cx) / (double)vk + half;
+                            const double uy = ((double)y - cy) / (double)vk + half;
+                            // clamp the FP64 position to the decided inclusion box
+                            const double sx = std::min(std::max(ux, 0.0), (double)(W - 1));
+                            const double sy = std::min(std::max(uy, 0.0), (double)(W - 1));
+                            const double fx0 = std::floor(sx), fy0 = std::floor(sy);
+                            const int x0 = (int)fx0, y0 = (int)fy0;
+                            const int x1 = std::min(x0 + 1, W - 1), y1 = std::min(y0 + 1, W - 1);
+                            const double fx = sx - fx0, fy = sy - fy0;
+                            const uint8_t* t = tiles_u8 + (size_t)v * W * W;
+                            const double a = t[y0 * W + x0] / 255.0, b = t[y0 * W + x1] / 255.0;
+                            const double c = t[y1 * W + x0] / 255.0, d = t[y1 * W + x1] / 255.0;
+                            const double top = a + fx * (b - a), bot = c + fx * (d - c);
+                            acc += top + fy * (bot - top);
+                            ++nacc;
+                        }
+                out[((size_t)k * out_h + y) * out_w + x] = nacc ? acc / nacc : 0.0;
+            }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// V2R behavioural model fit: Householder QR least squares, FP64
+// ------------------------------------------------------------------------------------------
+static int fit_v2r(const float* aL, const float* v, int n, double c[3]) {
+    if (n < 3) return -1;
+    std::vector<double> X((size_t)n * 3), y(n);
+    for (int i = 0; i < n; ++i) {
+        const double a = aL[i], vv = v[i];
+        X[(size_t)i * 3] = a * vv; X[(size_t)i * 3 + 1] = vv; X[(size_t)i * 3 + 2] = 1.0;
+        y[i] = a;
+    }
+    for (int k = 0; k < 3; ++k) {
+        double nrm = 0;
+        for (int i = k; i < n; ++i) nrm += X[(size_t)i * 3 + k] * X[(size_t)i * 3 + k];
+        nrm = std::sqrt(nrm);
+        if (nrm == 0) return -2;
+        const double x0 = X[(size_t)k * 3 + k];
+        const double alpha = x0 > 0 ? -nrm : nrm;
+        std::vector<double> u(n - k);
+        for (int i = k; i < n; ++i) u[i - k] = X[(size_t)i * 3 + k];
+        u[0] -= alpha;
+        double un = 0;
+        for (double e : u) un += e * e;
+        if (un == 0) continue;
+        for (int j = k; j < 3; ++j) {
+            double dot = 0;
+            for (int i = k; i < n; ++i) dot += u[i - k] * X[(size_t)i * 3 + j];
+            const double s = 2.0 * dot / un;
+            for (int i = k; i < n; ++i) X[(size_t)i * 3 + j] -= s * u[i - k];
+        }
+        double dot = 0;
+        for (int i = k; i < n; ++i) dot += u[i - k] * y[i];
+        const double s = 2.0 * dot / un;
+        for (int i = k; i < n; ++i) y[i] -= s * u[i - k];
+    }
+    for (int k = 2; k >= 0; --k) {
+        double s = y[k];
+        for (int j = k + 1; j < 3; ++j) s -= X[(size_t)k * 3 + j] * c[j];
+        c[k] = s / X[(size_t)k * 3 + k];
+    }
+    return 0;
+}
+
+} // namespace orc
+
+// ------------------------------------------------------------------------------------------
+// C entry points (ctypes)
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    orc::philox(ctr, key, out);
+}
+
+int orc_layout(const orc::LayoutParams* lp, int cap, float* centers, int* origins, int* tilekeys,
+               int* lens_type, int* nbr, int* cnt) {
+    return orc::layout(*lp, cap, centers, origins, tilekeys, lens_type, nbr, cnt);
+}
+
+void orc_slice_u8(const uint8_t* raw, int raw_w, int raw_h, const int* origins, int V, int crop,
+                  uint8_t* tiles) {
+    (void)raw_h;
+    for (int v = 0; v < V; ++v)
+        for (int y = 0; y < crop; ++y)
+            for (int x = 0; x < crop; ++x)
+                tiles[((size_t)v * crop + y) * crop + x] =
+                    raw[(size_t)(origins[2 * v + 1] + y) * raw_w + origins[2 * v] + x];
+}
+
+void orc_solve_f64(const uint8_t* tiles, const float* centers, const int* nbr, const int* cnt,
+                   int V, const orc::PmParams* p, double* plane, double* cost, double* disp,
+                   uint32_t* sel, int nthreads) {
+    orc::solve<double>(tiles, centers, nbr, cnt, V, *p, plane, cost, disp, sel, nthreads);
+}
+
+void orc_solve_f32(const uint8_t* tiles, const float* centers, const int* nbr, const int* cnt,
+                   int V, const orc::PmParams* p, float* plane, float* cost, float* disp,
+                   uint32_t* sel, int nthreads) {
+    orc::solve<float>(tiles, centers, nbr, cnt, V, *p, plane, cost, disp, sel, nthreads);
+}
+
+void orc_eval_costs_f64(const uint8_t* tiles, const float* centers, const int* nbr,
+                        const int* cnt, int V, const orc::PmParams* p, const int32_t* hyp, int n,
+                        double* out, int nthreads) {
+    orc::eval_costs<double>(tiles, centers, nbr, cnt, V, *p, hyp, n, out, nthreads);
+}
+
+void orc_eval_costs_f32(const uint8_t* tiles, const float* centers, const int* nbr,
+                        const int* cnt, int V, const orc::PmParams* p, const int32_t* hyp, int n,
+                        float* out, int nthreads) {
+    orc::eval_costs<float>(tiles, centers, nbr, cnt, V, *p, hyp, n, out, nthreads);
+}
+
+void orc_filter_f32(const float* gray, const float* centers, const int* nbr, const int* cnt,
+                    int V, int crop, int K, float base, const orc::FilterParams* fp,
+                    const float* plane_in, const float* cost_in, const float* disp_in,
+                    const uint32_t* sel_in, float* plane_out, float* cost_out, float* disp_out,
+                    uint32_t* sel_out, uint8_t* reason, int nthreads) {
+    orc::FilterCtx c;
+    c.V = V; c.W = crop; c.K = K; c.gray = gray; c.centers = centers; c.nbr = nbr; c.cnt = cnt;
+    c.base = base; c.plane = plane_in; c.cost = cost_in; c.fp = *fp;
+    orc::filter_f32(c, disp_in, sel_in, plane_out, cost_out, disp_out, sel_out, reason, nthreads);
+}
+
+void orc_refocus_f64(const uint8_t* tiles, const float* centers, int V, float pitch, int crop,
+                     int out_w, int out_h, int S, float vmin, float vmax, double* out,
+                     int nthreads) {
+    orc::refocus(tiles, centers, V, pitch, crop, out_w, out_h, S, vmin, vmax, out, nthreads);
+}
+
+unsigned orc_topk_f64(const double* cv, int cnt, int top_k, double* mean, int* best) {
+    return orc::topk<double>(cv, cnt, top_k, *mean, *best);
+}
+
+double orc_weighted_f64(const double* cv, const double* vw, int cnt, int* argmin) {
+    return orc::weighted_cost<double>(cv, vw, cnt, *argmin);
+}
+
+void orc_reencode_plane_f64(const double hn[4], int qx, int qy, double du, double dv, int px,
+                            int py, double out[4]) {
+    const orc::P4<double> h{hn[0], hn[1], hn[2], hn[3]};
+    const orc::P4<double> o = orc::reencode_plane<double>(h, qx, qy, du, dv, px, py);
+    out[0] = o.x; out[1] = o.y; out[2] = o.z; out[3] = o.w;
+}
+
+void orc_random_normals_f64(uint32_t seed, uint32_t uid, uint32_t step, int n, double* out) {
+    for (int i = 0; i < n; ++i) {
+        orc::Stream st(seed, uid, (uint32_t)i, step);
+        orc::random_normal<double>(st, orc::SLOT_INIT_MARSAGLIA, out + 3 * (size_t)i);
+    }
+}
+
+int orc_fit_v2r(const float* aL, const float* v, int n, double* c) {
+    return orc::fit_v2r(aL, v, n, c);
+}
+
+void orc_v2r_convert(const float* v, int n, const double* c, double* z) {
+    for (int i = 0; i < n; ++i) {
+        const double vv = v[i];
+        const double zz = (vv * c[1] + c[2]) / (1.0 - vv * c[0]);
+        z[i] = (std::isnan(zz) || std::isinf(zz) || zz < 0) ? 0.0 : zz;
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,206 @@
+"""ctypes binding of the CPU oracle (oracle/liblfpm_oracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module; the
+product path never does.  Structures mirror the product ABI field for field.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblfpm_oracle.so")
+
+
+class PmParams(C.Structure):
+    _fields_ = [("crop", C.c_int), ("base", C.c_float), ("radius", C.c_int),
+                ("radius_increment", C.c_int), ("sigma_spatial", C.c_float),
+                ("sigma_color", C.c_float), ("min_var", C.c_float), ("top_k", C.c_int),
+                ("depth_min", C.c_float), ("depth_max", C.c_float), ("iterations", C.c_int),
+                ("max_neighbors", C.c_int), ("seed", C.c_uint32), ("frame_uid", C.c_uint32),
+                ("storage", C.c_int), ("cos_max_slant", C.c_float)]
+
+
+class FilterParams(C.Structure):
+    _fields_ = [("max_triplet", C.c_int), ("min_valid_disp", C.c_float),
+                ("use_selected_views_only", C.c_int),
+                ("clear_selected_views_when_invalid", C.c_int),
+                ("enable_cost_filter", C.c_int), ("max_cost", C.c_float),
+                ("enable_cycle_geo_filter", C.c_int), ("max_geo_err_px", C.c_float),
+                ("enable_cycle_photo_filter", C.c_int), ("max_photo_err_u", C.c_float),
+                ("min_good_neighbors", C.c_int), ("enable_spike_filter", C.c_int),
+                ("spike_abs_diff", C.c_float), ("spike_min_neighbors", C.c_int)]
+
+
+class LayoutParams(C.Structure):
+    _fields_ = [("pitch", C.c_float), ("crop", C.c_int), ("raw_w", C.c_int),
+                ("raw_h", C.c_int), ("max_neighbors", C.c_int), ("lens_types", C.c_int),
+                ("neighbors_per_type", C.c_int * 3)]
+
+
+def build():
+    """Compile the oracle with its Makefile (g++ only)."""
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def nthreads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox(c, k, o)
+    return list(o)
+
+
+def layout(lp):
+    L = lib()
+    L.orc_layout.restype = C.c_int
+    V = L.orc_layout(C.byref(lp), 0, None, None, None, None, None, None)
+    K = lp.max_neighbors
+    centers = np.zeros((V, 2), np.float32)
+    origins = np.zeros((V, 2), np.int32)
+    keys = np.zeros((V, 2), np.int32)
+    ltype = np.zeros(V, np.int32)
+    nbr = np.zeros((V, K), np.int32)
+    cnt = np.zeros(V, np.int32)
+    L.orc_layout(C.byref(lp), V, _p(centers), _p(origins), _p(keys), _p(ltype), _p(nbr),
+                 _p(cnt))
+    return dict(V=V, centers=centers, origins=origins, tilekeys=keys, lens_type=ltype,
+                nbr=nbr, cnt=cnt)
+
+
+def slice_u8(raw, origins, crop):
+    raw = np.ascontiguousarray(raw, np.uint8)
+    V = origins.shape[0]
+    tiles = np.zeros((V, crop, crop), np.uint8)
+    lib().orc_slice_u8(_p(raw), raw.shape[1], raw.shape[0], _p(np.ascontiguousarray(origins)),
+                       V, crop, _p(tiles))
+    return tiles
+
+
+def solve(tiles_u8, centers, nbr, cnt, pm, dtype=np.float64, threads=None):
+    V, W, _ = tiles_u8.shape
+    n = V * W * W
+    plane = np.zeros((n, 4), dtype)
+    cost = np.zeros(n, dtype)
+    disp = np.zeros((n, 4), dtype)
+    sel = np.zeros(n, np.uint32)
+    f = lib().orc_solve_f64 if dtype == np.float64 else lib().orc_solve_f32
+    f(_p(np.ascontiguousarray(tiles_u8)), _p(np.ascontiguousarray(centers, np.float32)),
+      _p(np.ascontiguousarray(nbr, np.int32)), _p(np.ascontiguousarray(cnt, np.int32)), V,
+      C.byref(pm), _p(plane), _p(cost), _p(disp), _p(sel), threads or nthreads())
+    return dict(plane=plane, cost=cost, disp=disp, sel=sel)
+
+
+def eval_costs(tiles_u8, centers, nbr, cnt, pm, hyp, dtype=np.float64, threads=None):
+    """hyp: structured (n, 6) int32 view: mi, pixel, then 4 float32 bit patterns."""
+    V = tiles_u8.shape[0]
+    n = hyp.shape[0]
+    out = np.zeros((n, pm.max_neighbors), dtype)
+    f = lib().orc_eval_costs_f64 if dtype == np.float64 else lib().orc_eval_costs_f32
+    f(_p(np.ascontiguousarray(tiles_u8)), _p(np.ascontiguousarray(centers, np.float32)),
+      _p(np.ascontiguousarray(nbr, np.int32)), _p(np.ascontiguousarray(cnt, np.int32)), V,
+      C.byref(pm), _p(np.ascontiguousarray(hyp, np.int32)), n, _p(out),
+      threads or nthreads())
+    return out
+
+
+def filter_f32(gray, centers, nbr, cnt, crop, K, base, fp, plane, cost, disp, sel, threads=None):
+    V = centers.shape[0]
+    n = V * crop * crop
+    po = np.zeros((n, 4), np.float32)
+    co = np.zeros(n, np.float32)
+    do = np.zeros((n, 4), np.float32)
+    so = np.zeros(n, np.uint32)
+    reason = np.zeros(n, np.uint8)
+    lib().orc_filter_f32(
+        _p(np.ascontiguousarray(gray, np.float32)), _p(np.ascontiguousarray(centers, np.float32)),
+        _p(np.ascontiguousarray(nbr, np.int32)), _p(np.ascontiguousarray(cnt, np.int32)), V,
+        crop, K, C.c_float(base), C.byref(fp), _p(np.ascontiguousarray(plane, np.float32)),
+        _p(np.ascontiguousarray(cost, np.float32)), _p(np.ascontiguousarray(disp, np.float32)),
+        _p(np.ascontiguousarray(sel, np.uint32)), _p(po), _p(co), _p(do), _p(so), _p(reason),
+        threads or nthreads())
+    return dict(plane=po, cost=co, disp=do, sel=so, reason=reason)
+
+
+def refocus(tiles_u8, centers, pitch, crop, out_w, out_h, S, vmin, vmax, threads=None):
+    V = centers.shape[0]
+    out = np.zeros((S, out_h, out_w), np.float64)
+    lib().orc_refocus_f64(_p(np.ascontiguousarray(tiles_u8)),
+                          _p(np.ascontiguousarray(centers, np.float32)), V, C.c_float(pitch),
+                          crop, out_w, out_h, S, C.c_float(vmin), C.c_float(vmax), _p(out),
+                          threads or nthreads())
+    return out
+
+
+def fit_v2r(aL, v):
+    aL = np.ascontiguousarray(aL, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    c = np.zeros(3, np.float64)
+    L = lib()
+    L.orc_fit_v2r.restype = C.c_int
+    rc = L.orc_fit_v2r(_p(aL), _p(v), aL.size, _p(c))
+    if rc != 0:
+        raise RuntimeError("oracle fit failed: %d" % rc)
+    return c
+
+
+def v2r_convert(v, c):
+    v = np.ascontiguousarray(v, np.float32)
+    z = np.zeros(v.size, np.float64)
+    lib().orc_v2r_convert(_p(v), v.size, _p(np.ascontiguousarray(c, np.float64)), _p(z))
+    return z
+
+
+def topk(cv, top_k):
+    cv = np.ascontiguousarray(cv, np.float64)
+    mean = C.c_double()
+    best = C.c_int()
+    L = lib()
+    L.orc_topk_f64.restype = C.c_uint
+    m = L.orc_topk_f64(_p(cv), cv.size, top_k, C.byref(mean), C.byref(best))
+    return m, mean.value, best.value
+
+
+def weighted(cv, vw):
+    cv = np.ascontiguousarray(cv, np.float64)
+    vw = np.ascontiguousarray(vw, np.float64)
+    am = C.c_int()
+    L = lib()
+    L.orc_weighted_f64.restype = C.c_double
+    c = L.orc_weighted_f64(_p(cv), _p(vw), cv.size, C.byref(am))
+    return c, am.value
+
+
+def reencode_plane(hn, q, du, dv, p):
+    hn = np.ascontiguousarray(hn, np.float64)
+    out = np.zeros(4)
+    lib().orc_reencode_plane_f64(_p(hn), int(q[0]), int(q[1]), C.c_double(du), C.c_double(dv),
+                                 int(p[0]), int(p[1]), _p(out))
+    return out
+
+
+def random_normals(seed, uid, step, n):
+    out = np.zeros((n, 3))
+    lib().orc_random_normals_f64(C.c_uint32(seed), C.c_uint32(uid), C.c_uint32(step), n, _p(out))
+    return out

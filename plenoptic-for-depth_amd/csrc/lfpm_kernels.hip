@@ -1,0 +1,1532 @@
+// lfpm_kernels.hip — hand-written HIP kernels (gfx950 / CDNA4) of the micro-image PatchMatch
+// solver and the C-ABI entry points that launch them.
+//
+// Work decomposition.  One workgroup owns one micro-image (MI) at a time and loops over MIs
+// (grid = resident slots, each XCD walks a contiguous MI range so ring-1/ring-2 neighbour
+// tiles stay in that XCD's L2).  The MI's reference tile and its K neighbour tiles are staged
+// in LDS; one lane owns one pixel of the active red/black colour and evaluates all of that
+// pixel's hypotheses; every bilinear gather of the bilateral NCC is an LDS read.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/lfpm.h"
+#include "lfpm_common.h"
+#include "lfpm_internal.h"
+
+namespace lfpm {
+
+// ------------------------------------------------------------------------------------------
+// Parameters as the kernels see them
+// ------------------------------------------------------------------------------------------
+struct DevParams {
+    int W, WW, K, r, inc, top_k, V;
+    float inv_base;      // 1 / max(Base, 1e-6)
+    float inv_ss;        // 1 / (2 sigma_s^2)
+    float inv_sc;        // 1 / (2 sigma_c^2)
+    float min_var;
+    float dmin, dmax;
+    float cos_max_slant;
+    uint32_t seed, uid;
+    int slots_per_xcd;   // persistent grid: workgroups per XCD
+};
+
+static DevParams make_dev_params(const lfpm_pm_params* p, int V) {
+    DevParams d;
+    d.W = p->crop;
+    d.WW = p->crop * p->crop;
+    d.K = p->max_neighbors;
+    d.r = p->radius;
+    d.inc = std::max(1, p->radius_increment);
+    d.top_k = p->top_k;
+    d.V = V;
+    d.inv_base = 1.0f / std::max(p->base, 1e-6f);
+    d.inv_ss = 1.0f / (2.0f * p->sigma_spatial * p->sigma_spatial);
+    d.inv_sc = 1.0f / (2.0f * p->sigma_color * p->sigma_color);
+    d.min_var = p->min_var;
+    d.dmin = p->depth_min;
+    d.dmax = p->depth_max;
+    d.cos_max_slant = p->cos_max_slant;
+    d.seed = p->seed;
+    d.uid = p->frame_uid;
+    d.slots_per_xcd = 1;
+    return d;
+}
+
+static int check_pm_params(const lfpm_pm_params* p, int V) {
+    if (!p) return set_error(LFPM_E_INVALID, "null params");
+    if (V <= 0) return set_error(LFPM_E_INVALID, "num_mis must be > 0");
+    if (p->crop < 3 || p->crop > LFPM_MAX_CROP)
+        return set_error(LFPM_E_INVALID, "crop %d outside [3, %d]", p->crop, LFPM_MAX_CROP);
+    if (p->max_neighbors < 1 || p->max_neighbors > LFPM_MAX_NEIGHBORS)
+        return set_error(LFPM_E_INVALID, "max_neighbors %d outside [1, %d]", p->max_neighbors,
+                         LFPM_MAX_NEIGHBORS);
+    if (p->radius < 0 || p->radius > LFPM_MAX_RADIUS)
+        return set_error(LFPM_E_INVALID, "radius %d outside [0, %d]", p->radius, LFPM_MAX_RADIUS);
+    if (p->top_k < 1) return set_error(LFPM_E_INVALID, "top_k must be >= 1");
+    if (!(p->sigma_spatial > 0.0f) || !(p->sigma_color > 0.0f))
+        return set_error(LFPM_E_INVALID, "sigmas must be > 0");
+    if (p->storage != 0) return set_error(LFPM_E_UNSUPPORTED, "storage mode %d", p->storage);
+    return LFPM_OK;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return set_error(LFPM_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));        \
+    } while (0)
+
+static int launch_check(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(LFPM_E_HIP, "%s launch: %s", what, hipGetErrorString(e));
+    return LFPM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Device helpers
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float u01_dev(uint32_t x) {
+    return ((float)(x >> 8) + 0.5f) * 5.9604644775390625e-08f;
+}
+
+// Per-(pixel, step) Philox draw stream with a one-block cache.
+struct DevStream {
+    uint32_t pix, step, k0, k1, blk;
+    uint32_t w[4];
+    __device__ DevStream(uint32_t p, uint32_t s, uint32_t a, uint32_t b)
+        : pix(p), step(s), k0(a), k1(b), blk(0xFFFFFFFFu) {}
+    __device__ uint32_t raw(uint32_t slot) {
+        const uint32_t b = slot >> 2;
+        if (b != blk) {
+            philox4x32_10(pix, step, b, 0u, k0, k1, w);
+            blk = b;
+        }
+        const uint32_t q = slot & 3u;
+        return q == 0 ? w[0] : (q == 1 ? w[1] : (q == 2 ? w[2] : w[3]));
+    }
+};
+
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
+    const float n = sqrtf(x * x + y * y + z * z);
+    x = x / n; y = y / n; z = z / n;
+}
+
+// Marsaglia (1972) sphere point; rejection decided exactly in 64-bit integers.
+__device__ void random_normal(DevStream& st, uint32_t base, float& nx, float& ny, float& nz) {
+    long long A = 0, B = 0;
+    for (uint32_t a = 0; a < MARSAGLIA_ATTEMPTS; ++a) {
+        const long long ka = (long long)(st.raw(base + 2 * a) >> 8);
+        const long long kb = (long long)(st.raw(base + 2 * a + 1) >> 8);
+        const long long ta = 2 * ka + 1 - (1LL << 24);
+        const long long tb = 2 * kb + 1 - (1LL << 24);
+        if (ta * ta + tb * tb < (1LL << 48)) { A = ta; B = tb; break; }
+    }
+    const float q1 = (float)A * 5.9604644775390625e-08f, q2 = (float)B * 5.9604644775390625e-08f;
+    const float s = q1 * q1 + q2 * q2;
+    const float sq = sqrtf(1.0f - s);
+    nx = 2.0f * q1 * sq;
+    ny = 2.0f * q2 * sq;
+    nz = 1.0f - 2.0f * s;
+    normalize3(nx, ny, nz);
+}
+
+// Bilateral NCC between the reference tile and one neighbour tile (both in LDS).
+// Follows ComputeBilateralNCC_Frame (MIStereo/AdaptMIPMFrame.cu:73-179): window (2r+1)^2 with
+// stride inc, samples whose reference or source point leaves the crop are skipped, weights
+// exp(-|dxy|/(2 s_s^2) - |dI|/(2 s_c^2)), moments taken about the window centre intensity.
+// Work counters of the counting kernel variants: evaluated window samples and evaluated
+// (pixel, view, hypothesis) triples — the units of the roofline FLOP count.
+struct WorkCount {
+    unsigned int samples, triples;
+};
+
+template <bool COUNT>
+__device__ __forceinline__ float ncc_eval(const float* __restrict__ ref,
+                                          const float* __restrict__ src,
+                                          const float* __restrict__ spat, const DevParams& P,
+                                          int px, int py, float rc, float4 h, float sx, float sy,
+                                          WorkCount& wc) {
+    const int W = P.W;
+    const float Wf = (float)W;
+    const float ax = (float)px + sx * h.w, ay = (float)py + sy * h.w;
+    if (!(ax >= 0.0f && ax < Wf && ay >= 0.0f && ay < Wf)) return 2.0f;
+    if (!(fabsf(h.z) >= P.cos_max_slant)) return 2.0f;  // slant gate
+    if (COUNT) ++wc.triples;
+    const float A = -h.x / h.z, B = -h.y / h.z;
+    const int r = P.r, inc = P.inc, n = 2 * r + 1;
+    float s_w = 0.0f, s_x = 0.0f, s_xx = 0.0f, s_y = 0.0f, s_yy = 0.0f, s_xy = 0.0f;
+    for (int i = -r; i <= r; i += inc) {
+        const int rx = px + i;
+        if (rx < 0 || rx >= W) continue;
+        const float di = h.w + A * (float)i;
+        const float* spat_i = spat + (i + r);
+        for (int j = -r; j <= r; j += inc) {
+            const int ry = py + j;
+            if (ry < 0 || ry >= W) continue;
+            const float d = di + B * (float)j;
+            const float qx = (float)rx + sx * d, qy = (float)ry + sy * d;
+            if (!(qx >= 0.0f && qx < Wf && qy >= 0.0f && qy < Wf)) continue;
+            const float rv = ref[ry * W + rx];
+            const float x = rv - rc;
+            const float wgt = __expf(spat_i[(j + r) * n] - fabsf(x) * P.inv_sc);
+            const float fx0 = floorf(qx), fy0 = floorf(qy);
+            const int x0 = (int)fx0, y0 = (int)fy0;
+            const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, W - 1);
+            const float fx = qx - fx0, fy = qy - fy0;
+            const float a = src[y0 * W + x0], b = src[y0 * W + x1];
+            const float c = src[y1 * W + x0], e = src[y1 * W + x1];
+            const float top = a + fx * (b - a);
+            const float bot = c + fx * (e - c);
+            const float y = (top + fy * (bot - top)) - rc;
+            const float wx = wgt * x;
+            if (COUNT) ++wc.samples;
+            s_w += wgt;
+            s_x += wx;
+            s_xx += wx * x;
+            s_y += wgt * y;
+            s_yy += wgt * y * y;
+            s_xy += wx * y;
+        }
+    }
+    if (s_w <= 1e-6f) return 2.0f;
+    const float inv = 1.0f / s_w;
+    const float mx = s_x * inv, my = s_y * inv;
+    const float vx = s_xx * inv - mx * mx;
+    const float vy = s_yy * inv - my * my;
+    if (vx < P.min_var || vy < P.min_var) return 2.0f;
+    const float cov = s_xy * inv - mx * my;
+    const float c = 1.0f - cov / sqrtf(vx * vy);
+    return fminf(2.0f, fmaxf(0.0f, c));
+}
+
+// LDS carve-up shared by the solver kernels.
+struct Lds {
+    float* tiles;     // (K+1)*WW: slot 0 = reference, slot 1+k = neighbour k
+    float* costp;     // WW (phase kernel: previous cost of this MI)
+    float* spat;      // (2r+1)^2 spatial log-weights -|dxy| / (2 s_s^2)
+    float2* step;     // K: (c_ref - c_nei) / Base
+    float2* cdiff;    // K: c_ref - c_nei
+    int* nid;         // K
+    uint8_t* wts;     // K*NT view weights (phase kernel)
+    uint8_t* slist;   // K*NT selected-view lists (phase kernel)
+};
+
+static size_t lds_bytes(const DevParams& P, int NT, bool phase) {
+    const int n = 2 * P.r + 1;
+    size_t b = 0;
+    b += (size_t)(P.K + 1) * P.WW * 4;
+    b += (size_t)P.WW * 4;
+    b += (size_t)n * n * 4;
+    b += (size_t)P.K * 8 * 2;
+    b += (size_t)P.K * 4;
+    if (phase) b += (size_t)P.K * NT * 2;
+    return (b + 15) & ~(size_t)15;
+}
+
+__device__ __forceinline__ Lds carve(unsigned char* base, const DevParams& P, int NT) {
+    Lds L;
+    const int n = 2 * P.r + 1;
+    unsigned char* q = base;
+    L.tiles = (float*)q; q += (size_t)(P.K + 1) * P.WW * 4;
+    L.costp = (float*)q; q += (size_t)P.WW * 4;
+    L.spat = (float*)q; q += (size_t)n * n * 4;
+    L.step = (float2*)q; q += (size_t)P.K * 8;
+    L.cdiff = (float2*)q; q += (size_t)P.K * 8;
+    L.nid = (int*)q; q += (size_t)P.K * 4;
+    L.wts = (uint8_t*)q; q += (size_t)P.K * NT;
+    L.slist = (uint8_t*)q;
+    return L;
+}
+
+__device__ __forceinline__ void fill_spatial(Lds& L, const DevParams& P) {
+    const int n = 2 * P.r + 1;
+    for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
+        const int j = idx / n - P.r, i = idx % n - P.r;
+        L.spat[idx] = -sqrtf((float)(i * i + j * j)) * P.inv_ss;
+    }
+}
+
+// Stage the MI's tables and tiles (reference + its neighbours) into LDS.  Caller syncs.
+__device__ __forceinline__ int stage_mi(Lds& L, const DevParams& P, int mi,
+                                        const float* __restrict__ tiles,
+                                        const float2* __restrict__ centers,
+                                        const int* __restrict__ nbr, const int* __restrict__ cnt) {
+    const int K = P.K, WW = P.WW;
+    const int c = min(cnt[mi], K);
+    const float2 cr = centers[mi];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const int n = k < c ? nbr[(size_t)mi * K + k] : -1;
+        L.nid[k] = n;
+        if (n >= 0) {
+            const float2 cn = centers[n];
+            const float dx = cr.x - cn.x, dy = cr.y - cn.y;
+            L.cdiff[k] = make_float2(dx, dy);
+            L.step[k] = make_float2(dx * P.inv_base, dy * P.inv_base);
+        } else {
+            L.cdiff[k] = make_float2(0.0f, 0.0f);
+            L.step[k] = make_float2(0.0f, 0.0f);
+        }
+    }
+    const int total = (c + 1) * WW;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int s = idx / WW, o = idx - s * WW;
+        int src = mi;
+        if (s > 0) {
+            src = nbr[(size_t)mi * K + s - 1];
+            if (src < 0) src = mi;  // never read: ncc() skips views with nid < 0
+        }
+        L.tiles[idx] = tiles[(size_t)src * WW + o];
+    }
+    return c;
+}
+
+__device__ __forceinline__ float4 disp_record(const Lds& L, const DevParams& P, int k, float w) {
+    const float2 cd = L.cdiff[k];
+    const float b = sqrtf(cd.x * cd.x + cd.y * cd.y);
+    const float dr = fabsf(w) * b * P.inv_base;
+    return make_float4(w, dr, b, b / (dr + 1e-7f));
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void flush_counts(unsigned long long* counters, const WorkCount& wc) {
+    if (COUNT && counters) {
+        atomicAdd(&counters[0], (unsigned long long)wc.samples);
+        atomicAdd(&counters[1], (unsigned long long)wc.triples);
+    }
+}
+
+// MI assignment of the persistent grid: workgroup g sits on XCD g % 8; XCD x walks the
+// contiguous MI range [x*V/8, (x+1)*V/8) with stride slots_per_xcd.
+__device__ __forceinline__ void mi_range(const DevParams& P, int& first, int& end, int& stride) {
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int lo = (int)(((long long)P.V * xcd) >> 3);
+    const int hi = (int)(((long long)P.V * (xcd + 1)) >> 3);
+    first = lo + local;
+    end = hi;
+    stride = P.slots_per_xcd;
+}
+
+// ------------------------------------------------------------------------------------------
+// Tile gather
+// ------------------------------------------------------------------------------------------
+__global__ void k_slice(const uint8_t* __restrict__ raw, size_t pitch, const int2* __restrict__ org,
+                        int V, int W, float scale_div, float* __restrict__ out) {
+    const size_t WW = (size_t)W * W;
+    const size_t total = (size_t)V * WW;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int v = (int)(i / WW);
+        const int o = (int)(i - (size_t)v * WW);
+        const int y = o / W, x = o - y * W;
+        const int2 g = org[v];
+        const float u = (float)raw[(size_t)(g.y + y) * pitch + (size_t)(g.x + x)];
+        out[i] = scale_div > 0.0f ? u / scale_div : u;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Random initialisation (RandomInitializationForFrame, AdaptMIPMFrame.cu:474-518)
+// ------------------------------------------------------------------------------------------
+struct InitArgs {
+    const float* tiles; const float2* centers; const int* nbr; const int* cnt;
+    float4* plane; float* cost; uint32_t* sel;
+    float* scratch;   // [grid][K][NT]
+    unsigned long long* counters;  // counting variant: {samples, triples}
+    DevParams P;
+};
+
+template <bool COUNT>
+__global__ void __launch_bounds__(1024) k_init(InitArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const DevParams& P = a.P;
+    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW, K = P.K;
+    Lds L = carve(lds_raw, P, NT);
+    fill_spatial(L, P);
+    float* scr = a.scratch + (size_t)blockIdx.x * K * NT;
+    int first, end, stride;
+    mi_range(P, first, end, stride);
+    WorkCount wc{0u, 0u};
+    for (int mi = first; mi < end; mi += stride) {
+        __syncthreads();
+        const int c = stage_mi(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
+        __syncthreads();
+        if (t >= WW) continue;
+        const int x = t % W, y = t / W;
+        const size_t idx = (size_t)mi * WW + t;
+        DevStream st((uint32_t)idx, 0u, P.seed, P.uid);
+        float4 h;
+        h.w = u01_dev(st.raw(SLOT_INIT_DEPTH)) * (P.dmax - P.dmin) + P.dmin;
+        random_normal(st, SLOT_INIT_MARSAGLIA, h.x, h.y, h.z);
+        const float rc = L.tiles[t];
+        int valid = 0, best = -1;
+        float bestc = 2.0f;
+        for (int k = 0; k < c; ++k) {
+            float cv = 2.0f;
+            if (L.nid[k] >= 0)
+                cv = ncc_eval<COUNT>(L.tiles, L.tiles + (size_t)(k + 1) * WW, L.spat, P, x, y, rc, h,
+                              L.step[k].x, L.step[k].y, wc);
+            scr[(size_t)k * NT + t] = cv;
+            if (cv < 2.0f) {
+                ++valid;
+                if (best < 0 || cv < bestc) { best = k; bestc = cv; }
+            }
+        }
+        float mean = 2.0f;
+        uint32_t mask = 0u;
+        if (valid > 0) {
+            // k smallest costs in ascending order (ComputeSelectedViewsTopK_Frame :227-282)
+            const int kk = min(valid, P.top_k);
+            uint32_t taken = 0u;
+            float sum = 0.0f, thr = 2.0f;
+            for (int m = 0; m < kk; ++m) {
+                int bi = -1;
+                float bv = 0.0f;
+                for (int k = 0; k < c; ++k) {
+                    if ((taken >> k) & 1u) continue;
+                    const float cv = scr[(size_t)k * NT + t];
+                    if (bi < 0 || cv < bv) { bi = k; bv = cv; }
+                }
+                taken |= 1u << bi;
+                sum += bv;
+                thr = bv;
+            }
+            mean = sum / (float)kk;
+            for (int k = 0; k < c; ++k)
+                if (scr[(size_t)k * NT + t] <= thr) mask |= 1u << k;
+        }
+        a.plane[idx] = h;
+        a.cost[idx] = mean;
+        a.sel[idx] = mask;
+    }
+    flush_counts<COUNT>(a.counters, wc);
+}
+
+// ------------------------------------------------------------------------------------------
+// One red/black phase (CheckerboardUpdateFrame, AdaptMIPMFrame.cu:520-947)
+// ------------------------------------------------------------------------------------------
+struct PhaseArgs {
+    const float* tiles; const float2* centers; const int* nbr; const int* cnt;
+    const float4* plane_in; const float* cost_in; const uint32_t* sel_in;
+    float4* plane_out; float* cost_out; uint32_t* sel_out;
+    float* scratch;   // [grid][9*K][NT]: 8 candidate cost vectors + sampling probabilities
+    unsigned long long* counters;
+    DevParams P;
+    int iter, phase;
+};
+
+// Propagation candidate positions (index into the MI, row-major) — the 8 slots of the
+// reference: 0/2/4/6 = near up/down/left/right, 1/3/5/7 = far up/down/left/right.
+// src[c] = c for an evaluated slot, the slot it duplicates near the border, or -1 if absent.
+__device__ __forceinline__ void select_candidates(const float* __restrict__ costp, int W, int x,
+                                                  int y, int pos[8], int src[8]) {
+    const int H = W, center = y * W + x;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) { pos[i] = center; src[i] = -1; }
+    auto pick = [&](int& b, float& cm, int cand) {
+        const float cc = costp[cand];
+        if (cc < cm) { cm = cc; b = cand; }
+    };
+    if (y > 2) {
+        int b = center - 3 * W; float cm = costp[b];
+        if (y > 3) pick(b, cm, center - 4 * W);
+        pos[1] = b; src[1] = 1;
+    }
+    if (y < H - 3) {
+        int b = center + 3 * W; float cm = costp[b];
+        if (y < H - 4) pick(b, cm, center + 4 * W);
+        pos[3] = b; src[3] = 3;
+    }
+    if (x > 2) {
+        int b = center - 3; float cm = costp[b];
+        if (x > 3) pick(b, cm, center - 4);
+        pos[5] = b; src[5] = 5;
+    }
+    if (x < W - 3) {
+        int b = center + 3; float cm = costp[b];
+        if (x < W - 4) pick(b, cm, center + 4);
+        pos[7] = b; src[7] = 7;
+    }
+    if (y > 0) {
+        const int un = center - W; int b = un; float cm = costp[b];
+        for (int i = 0; i < 2; ++i) {
+            if (y > 1 + i && x > i) pick(b, cm, un - (1 + i) * W - i);
+            if (y > 1 + i && x < W - 1 - i) pick(b, cm, un - (1 + i) * W + i);
+        }
+        pos[0] = b; src[0] = 0;
+    }
+    if (y <= 2 && y > 0) { pos[1] = pos[0]; src[1] = 0; }
+    if (y < H - 1) {
+        const int dn = center + W; int b = dn; float cm = costp[b];
+        for (int i = 0; i < 2; ++i) {
+            if (y < H - 2 - i && x > i) pick(b, cm, dn + (1 + i) * W - i);
+            if (y < H - 2 - i && x < W - 1 - i) pick(b, cm, dn + (1 + i) * W + i);
+        }
+        pos[2] = b; src[2] = 2;
+    }
+    if (y >= H - 3 && y < H - 1) { pos[3] = pos[2]; src[3] = 2; }
+    if (x > 0) {
+        const int ln = center - 1; int b = ln; float cm = costp[b];
+        for (int i = 0; i < 2; ++i) {
+            if (x > 1 + i && y > i) pick(b, cm, ln - (1 + i) - i * W);
+            if (x > 1 + i && y < H - 1 - i) pick(b, cm, ln - (1 + i) + i * W);
+        }
+        pos[4] = b; src[4] = 4;
+    }
+    if (x <= 2 && x > 0) { pos[5] = pos[4]; src[5] = 4; }
+    if (x < W - 1) {
+        const int rn = center + 1; int b = rn; float cm = costp[b];
+        for (int i = 0; i < 2; ++i) {
+            if (x < W - 2 - i && y > i) pick(b, cm, rn + (1 + i) - i * W);
+            if (x < W - 2 - i && y < H - 1 - i) pick(b, cm, rn + (1 + i) + i * W);
+        }
+        pos[6] = b; src[6] = 6;
+    }
+    if (x >= W - 3 && x < W - 1) { pos[7] = pos[6]; src[7] = 6; }
+}
+
+__device__ __forceinline__ void active_pixel(int W, int a, int phase, int& x, int& y) {
+    if (W & 1) {
+        const int i = 2 * a + phase;
+        y = i / W;
+        x = i - y * W;
+    } else {
+        const int half = W >> 1;
+        y = a / half;
+        x = 2 * (a - y * half) + ((y + phase) & 1);
+    }
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_phase(PhaseArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const DevParams& P = a.P;
+    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW, K = P.K;
+    Lds L = carve(lds_raw, P, NT);
+    fill_spatial(L, P);
+    float* scr = a.scratch + (size_t)blockIdx.x * 9 * K * NT;
+    int first, end, stride;
+    mi_range(P, first, end, stride);
+    WorkCount wc{0u, 0u};
+    const int phase = a.phase;
+    const int nact = (W & 1) ? (phase == 0 ? (WW + 1) / 2 : WW / 2) : WW / 2;
+    for (int mi = first; mi < end; mi += stride) {
+        __syncthreads();
+        const int c = stage_mi(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
+        for (int i = t; i < WW; i += NT) L.costp[i] = a.cost_in[(size_t)mi * WW + i];
+        __syncthreads();
+        const size_t base = (size_t)mi * WW;
+        // pass-through of the inactive colour (and of every pixel of an MI without neighbours)
+        for (int i = t; i < WW; i += NT) {
+            const int x = i % W, y = i / W;
+            if (((x + y) & 1) != phase || c <= 0) {
+                a.plane_out[base + i] = a.plane_in[base + i];
+                a.cost_out[base + i] = a.cost_in[base + i];
+                a.sel_out[base + i] = a.sel_in[base + i];
+            }
+        }
+        if (c <= 0 || t >= nact) continue;
+
+        int x, y;
+        active_pixel(W, t, phase, x, y);
+        const int center = y * W + x;
+        const size_t idx = base + center;
+        const float rc = L.tiles[center];
+        int pos[8], src[8];
+        select_candidates(L.costp, W, x, y, pos, src);
+
+        // cost vectors of the propagation candidates over every view
+        for (int cd = 0; cd < 8; ++cd) {
+            if (src[cd] != cd) continue;
+            const float4 h = a.plane_in[base + pos[cd]];
+            for (int k = 0; k < c; ++k) {
+                float cv = 2.0f;
+                if (L.nid[k] >= 0)
+                    cv = ncc_eval<COUNT>(L.tiles, L.tiles + (size_t)(k + 1) * WW, L.spat, P, x, y, rc, h,
+                                  L.step[k].x, L.step[k].y, wc);
+                scr[(size_t)(cd * K + k) * NT + t] = cv;
+            }
+        }
+
+        // view-selection priors from the 4 direct neighbours' previous selections
+        uint32_t dsel[4];
+        bool dval[4];
+        dval[0] = y > 0; dval[1] = y < W - 1; dval[2] = x > 0; dval[3] = x < W - 1;
+        dsel[0] = dval[0] ? a.sel_in[idx - W] : 0u;
+        dsel[1] = dval[1] ? a.sel_in[idx + W] : 0u;
+        dsel[2] = dval[2] ? a.sel_in[idx - 1] : 0u;
+        dsel[3] = dval[3] ? a.sel_in[idx + 1] : 0u;
+
+        // sampling probabilities
+        const float thr = 0.8f * expf((float)(a.iter * a.iter) / (-90.0f));
+        float psum = 0.0f;
+        for (int k = 0; k < c; ++k) {
+            float count = 0.0f, tmpw = 0.0f;
+            int cf = 0;
+            for (int cd = 0; cd < 8; ++cd) {
+                if (src[cd] < 0) continue;
+                const float cv = scr[(size_t)(src[cd] * K + k) * NT + t];
+                if (cv < thr) { tmpw += expf(cv * cv / (-0.18f)); count += 1.0f; }
+                if (cv > 1.2f) ++cf;
+            }
+            float pr = 0.0f;
+            if (count > 2.0f && cf < 3) pr = tmpw / count;
+            else if (cf < 3) pr = expf(thr * thr / (-0.32f));
+            float prior = 0.0f;
+            for (int ii = 0; ii < 4; ++ii)
+                if (dval[ii]) prior += ((dsel[ii] >> k) & 1u) ? 0.9f : 0.1f;
+            pr *= prior;
+            scr[(size_t)(8 * K + k) * NT + t] = pr;
+            psum += pr;
+        }
+
+        // 15 draws through the CDF (first view whose cumulative probability exceeds the draw)
+        DevStream st((uint32_t)idx, (uint32_t)(1 + 2 * a.iter + phase), P.seed, P.uid);
+        float draw[VIEW_DRAWS];
+#pragma unroll
+        for (int s = 0; s < (int)VIEW_DRAWS; ++s)
+            draw[s] = u01_dev(st.raw(SLOT_VIEW_DRAW + s)) - 1.1920928955078125e-07f;
+        uint32_t tsel = 0u;
+        int nsel = 0;
+        if (psum > 0.0f) {
+            const float inv = 1.0f / psum;
+            float cum = 0.0f;
+            uint32_t taken = 0u;
+            for (int k = 0; k < c; ++k) {
+                cum += scr[(size_t)(8 * K + k) * NT + t] * inv;
+                int wk = 0;
+#pragma unroll
+                for (int s = 0; s < (int)VIEW_DRAWS; ++s)
+                    if (!((taken >> s) & 1u) && cum > draw[s]) { ++wk; taken |= 1u << s; }
+                L.wts[k * NT + t] = (uint8_t)wk;
+                if (wk > 0) { tsel |= 1u << k; L.slist[nsel * NT + t] = (uint8_t)k; ++nsel; }
+            }
+        } else {
+            for (int k = 0; k < c; ++k) L.wts[k * NT + t] = 0;
+        }
+        if (nsel == 0) {
+            for (int k = 0; k < c; ++k) {
+                L.wts[k * NT + t] = 1;
+                L.slist[k * NT + t] = (uint8_t)k;
+                tsel |= 1u << k;
+            }
+            nsel = c;
+        }
+
+        // weighted cost over the selected views (ComputeWeightedCost_Frame :284-317)
+        auto wcost = [&](const float4& h) -> float {
+            float wn = 0.0f, tot = 0.0f;
+            for (int j = 0; j < nsel; ++j) {
+                const int k = L.slist[j * NT + t];
+                const float w = (float)L.wts[k * NT + t];
+                wn += w;
+                float cv = 2.0f;
+                if (L.nid[k] >= 0)
+                    cv = ncc_eval<COUNT>(L.tiles, L.tiles + (size_t)(k + 1) * WW, L.spat, P, x, y, rc, h,
+                                  L.step[k].x, L.step[k].y, wc);
+                tot += w * cv;
+            }
+            return wn <= 1e-6f ? 2.0f : tot / wn;
+        };
+
+        float4 best = a.plane_in[idx];
+        float bcost = wcost(best);
+        for (int cd = 0; cd < 8; ++cd) {
+            if (src[cd] < 0) continue;
+            float wn = 0.0f, tot = 0.0f;
+            for (int j = 0; j < nsel; ++j) {
+                const int k = L.slist[j * NT + t];
+                const float w = (float)L.wts[k * NT + t];
+                wn += w;
+                tot += w * scr[(size_t)(src[cd] * K + k) * NT + t];
+            }
+            const float cc = wn <= 1e-6f ? 2.0f : tot / wn;
+            if (cc < bcost) { bcost = cc; best = a.plane_in[base + pos[cd]]; }
+        }
+
+        // cross-view proposals through each selected view (BuildCrossViewPlaneProposal :350-398)
+        for (int j = 0; j < nsel; ++j) {
+            const int k = L.slist[j * NT + t];
+            const int n = L.nid[k];
+            if (n < 0) continue;
+            const float2 sk = L.step[k];
+            const float qfx = (float)x + sk.x * best.w, qfy = (float)y + sk.y * best.w;
+            const float fqx = floorf(qfx + 0.5f), fqy = floorf(qfy + 0.5f);
+            if (!(fqx >= 0.0f && fqx < (float)W && fqy >= 0.0f && fqy < (float)W)) continue;
+            const int qx = (int)fqx, qy = (int)fqy;
+            const float4 hn = a.plane_in[(size_t)n * WW + qy * W + qx];
+            const float al = -hn.x / hn.z, be = -hn.y / hn.z;
+            const float ga = hn.w - al * (float)qx - be * (float)qy;
+            const float2 cdv = L.cdiff[k];
+            float den = 1.0f + al * cdv.x + be * cdv.y;
+            if (fabsf(den) < 1e-6f) den = den >= 0.0f ? 1e-6f : -1e-6f;
+            const float aR = al / den, bR = be / den, cR = ga / den;
+            float nx = -aR, ny = -bR, nz = 1.0f;
+            normalize3(nx, ny, nz);
+            const float4 prop = make_float4(nx, ny, nz, aR * (float)x + bR * (float)y + cR);
+            const float cc = wcost(prop);
+            if (cc < bcost) { bcost = cc; best = prop; }
+        }
+
+        // refinement: ACMP depth x normal candidates (AdaptMIPMUtil.cu:181-183)
+        {
+            const float pert = 0.02f;
+            const float dnow = best.w;
+            float drand;
+            if (bcost < 0.1f)
+                drand = (u01_dev(st.raw(SLOT_REF_DEPTH_RAND)) - 0.5f) * 2.0f + dnow;
+            else
+                drand = u01_dev(st.raw(SLOT_REF_DEPTH_RAND)) * (P.dmax - P.dmin) + P.dmin;
+            float rnx, rny, rnz;
+            random_normal(st, SLOT_REF_MARSAGLIA, rnx, rny, rnz);
+            // perturbed normal (GeneratePerturbedNormalLF, CudaUtil.cu:105-142)
+            float vx = (float)x, vy = (float)y, vz = 1.0f;
+            normalize3(vx, vy, vz);
+            const float pamp = pert * 3.14159265358979323846f;
+            const float a1 = (u01_dev(st.raw(SLOT_REF_ANGLE)) - 0.5f) * pamp;
+            const float a2 = (u01_dev(st.raw(SLOT_REF_ANGLE + 1)) - 0.5f) * pamp;
+            const float a3 = (u01_dev(st.raw(SLOT_REF_ANGLE + 2)) - 0.5f) * pamp;
+            const float s1 = sinf(a1), s2 = sinf(a2), s3 = sinf(a3);
+            const float c1 = cosf(a1), c2 = cosf(a2), c3 = cosf(a3);
+            float px_ = (c2 * c3) * best.x + (c3 * s1 * s2 - c1 * s3) * best.y +
+                        (s1 * s3 + c1 * c3 * s2) * best.z;
+            float py_ = (c2 * s3) * best.x + (c1 * c3 + s1 * s2 * s3) * best.y +
+                        (c1 * s2 * s3 - c3 * s1) * best.z;
+            float pz_ = (-s2) * best.x + (c2 * s1) * best.y + (c1 * c2) * best.z;
+            if (px_ * vx + py_ * vy + pz_ * vz >= 0.0f) { px_ = best.x; py_ = best.y; pz_ = best.z; }
+            normalize3(px_, py_, pz_);
+            const float lo = (1.0f - pert) * dnow, hi = (1.0f + pert) * dnow;
+            const float dpert = u01_dev(st.raw(SLOT_REF_DEPTH_PERT)) * (hi - lo) + lo;
+            float4 cand[5];
+            cand[0] = make_float4(best.x, best.y, best.z, drand);
+            cand[1] = make_float4(rnx, rny, rnz, dnow);
+            cand[2] = make_float4(rnx, rny, rnz, drand);
+            cand[3] = make_float4(px_, py_, pz_, dnow);
+            cand[4] = make_float4(best.x, best.y, best.z, dpert);
+            for (int i = 0; i < 5; ++i) {
+                const float cc = wcost(cand[i]);
+                if (cc < bcost) { bcost = cc; best = cand[i]; }
+            }
+        }
+        a.plane_out[idx] = best;
+        a.cost_out[idx] = bcost;
+        a.sel_out[idx] = tsel;
+    }
+    flush_counts<COUNT>(a.counters, wc);
+}
+
+// ------------------------------------------------------------------------------------------
+// Final disparity/baseline record: lowest-cost view of the final plane over every view
+// (the disp argument of ComputeWeightedCost_Frame, AdaptMIPMFrame.cu:303-306).
+// ------------------------------------------------------------------------------------------
+struct FinalArgs {
+    const float* tiles; const float2* centers; const int* nbr; const int* cnt;
+    const float4* plane; float4* disp;
+    unsigned long long* counters;
+    DevParams P;
+};
+
+template <bool COUNT>
+__global__ void __launch_bounds__(1024) k_finalize(FinalArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const DevParams& P = a.P;
+    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW;
+    Lds L = carve(lds_raw, P, NT);
+    fill_spatial(L, P);
+    int first, end, stride;
+    mi_range(P, first, end, stride);
+    WorkCount wc{0u, 0u};
+    for (int mi = first; mi < end; mi += stride) {
+        __syncthreads();
+        const int c = stage_mi(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
+        __syncthreads();
+        if (t >= WW) continue;
+        const int x = t % W, y = t / W;
+        const size_t idx = (size_t)mi * WW + t;
+        const float4 h = a.plane[idx];
+        const float rc = L.tiles[t];
+        float mc = 2.0f;
+        int am = -1;
+        for (int k = 0; k < c; ++k) {
+            if (L.nid[k] < 0) continue;
+            const float cv = ncc_eval<COUNT>(L.tiles, L.tiles + (size_t)(k + 1) * WW, L.spat, P, x, y, rc,
+                                      h, L.step[k].x, L.step[k].y, wc);
+            if (cv < mc) { mc = cv; am = k; }
+        }
+        a.disp[idx] = am >= 0 ? disp_record(L, P, am, h.w) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    flush_counts<COUNT>(a.counters, wc);
+}
+
+// ------------------------------------------------------------------------------------------
+// Fixed-plane cost evaluation (hypotheses sorted by MI; one workgroup per MI)
+// ------------------------------------------------------------------------------------------
+struct EvalArgs {
+    const float* tiles; const float2* centers; const int* nbr; const int* cnt;
+    const int* hyp;   // n x 6 words
+    const int* seg;   // V+1 CSR offsets into hyp
+    int n;
+    float* out;       // n x K
+    unsigned long long* counters;
+    DevParams P;
+};
+
+__global__ void k_segments(const int* __restrict__ hyp, int n, int V, int* __restrict__ seg) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > V) return;
+    int lo = 0, hi = n;  // first index with hyp.mi >= v
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (hyp[(size_t)mid * 6] < v) lo = mid + 1; else hi = mid;
+    }
+    seg[v] = lo;
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_eval(EvalArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const DevParams& P = a.P;
+    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW, K = P.K;
+    Lds L = carve(lds_raw, P, NT);
+    fill_spatial(L, P);
+    int first, end, stride;
+    mi_range(P, first, end, stride);
+    WorkCount wc{0u, 0u};
+    for (int mi = first; mi < end; mi += stride) {
+        const int h0 = a.seg[mi], h1 = a.seg[mi + 1];
+        if (h0 >= h1) continue;  // uniform across the workgroup
+        __syncthreads();
+        const int c = stage_mi(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
+        __syncthreads();
+        for (int hi = h0 + t; hi < h1; hi += NT) {
+            const int* rec = a.hyp + (size_t)hi * 6;
+            const int pix = rec[1];
+            if (pix < 0 || pix >= WW || rec[0] != mi) {  // malformed record: no score
+                for (int k = 0; k < K; ++k) a.out[(size_t)hi * K + k] = 2.0f;
+                continue;
+            }
+            const float4 h = make_float4(__int_as_float(rec[2]), __int_as_float(rec[3]),
+                                         __int_as_float(rec[4]), __int_as_float(rec[5]));
+            const int x = pix % W, y = pix / W;
+            const float rc = L.tiles[pix];
+            for (int k = 0; k < K; ++k) {
+                float cv = 2.0f;
+                if (k < c && L.nid[k] >= 0)
+                    cv = ncc_eval<COUNT>(L.tiles, L.tiles + (size_t)(k + 1) * WW, L.spat, P, x, y, rc, h,
+                                  L.step[k].x, L.step[k].y, wc);
+                a.out[(size_t)hi * K + k] = cv;
+            }
+        }
+    }
+    flush_counts<COUNT>(a.counters, wc);
+}
+
+// ------------------------------------------------------------------------------------------
+// Fused CheckModule pass (MIDisparityFilterCPU::FilterTile, MIDisparityFilter.cpp:423-693).
+// FP32 with a fixed operation order and no contraction, so the reason mask is bit-identical to
+// the CPU twin given identical inputs.  All reads see the pre-filter snapshot.
+// ------------------------------------------------------------------------------------------
+struct FilterArgs {
+    const float* gray; const float2* centers; const int* nbr; const int* cnt;
+    int V, W, K;
+    float base;
+    lfpm_filter_params f;
+    const float4* plane_in; const float* cost_in; const float4* disp_in; const uint32_t* sel_in;
+    float4* plane_out; float* cost_out; float4* disp_out; uint32_t* sel_out; uint8_t* reason;
+};
+
+__device__ float f_bilin_disp(const FilterArgs& a, int n, float x, float y) {
+#pragma clang fp contract(off)
+    const int W = a.W;
+    if (x < 0 || y < 0 || x > W - 1 || y > W - 1) return __int_as_float(0x7fc00000);
+    const int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, W - 1);
+    const float ax = x - (float)x0, ay = y - (float)y0;
+    const size_t b = (size_t)n * W * W;
+    float d[4];
+    const int xi[4] = {x0, x0, x1, x1}, yi[4] = {y0, y1, y0, y1};
+    for (int q = 0; q < 4; ++q) {
+        float v = a.plane_in[b + yi[q] * W + xi[q]].w;
+        if (!isfinite(v) || v <= a.f.min_valid_disp) return __int_as_float(0x7fc00000);
+        d[q] = v;
+    }
+    return (1.0f - ax) * (1.0f - ay) * d[0] + (1.0f - ax) * ay * d[1] + ax * (1.0f - ay) * d[2] +
+           ax * ay * d[3];
+}
+
+__device__ float f_bilin_cost(const FilterArgs& a, int n, float x, float y) {
+#pragma clang fp contract(off)
+    const int W = a.W;
+    if (x < 0 || y < 0 || x > W - 1 || y > W - 1) return __int_as_float(0x7fc00000);
+    const int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, W - 1);
+    const float ax = x - (float)x0, ay = y - (float)y0;
+    const size_t b = (size_t)n * W * W;
+    const float c00 = a.cost_in[b + y0 * W + x0], c01 = a.cost_in[b + y1 * W + x0];
+    const float c10 = a.cost_in[b + y0 * W + x1], c11 = a.cost_in[b + y1 * W + x1];
+    if (!isfinite(c00) || !isfinite(c01) || !isfinite(c10) || !isfinite(c11))
+        return __int_as_float(0x7fc00000);
+    return (1.0f - ax) * (1.0f - ay) * c00 + (1.0f - ax) * ay * c01 + ax * (1.0f - ay) * c10 +
+           ax * ay * c11;
+}
+
+__device__ bool f_bilin_gray(const FilterArgs& a, int v, float x, float y, float& out) {
+#pragma clang fp contract(off)
+    const int W = a.W;
+    if (x < 0 || y < 0 || x > W - 1 || y > W - 1) return false;
+    const int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, W - 1);
+    const float ax = x - (float)x0, ay = y - (float)y0;
+    const float* g = a.gray + (size_t)v * W * W;
+    const float v00 = g[y0 * W + x0], v01 = g[y1 * W + x0], v10 = g[y0 * W + x1],
+                v11 = g[y1 * W + x1];
+    out = (1.0f - ax) * (1.0f - ay) * v00 + (1.0f - ax) * ay * v01 + ax * (1.0f - ay) * v10 +
+          ax * ay * v11;
+    return true;
+}
+
+__device__ bool f_mean_std3(const FilterArgs& a, int v, float fx, float fy, float& mean,
+                            float& sd) {
+#pragma clang fp contract(off)
+    const int W = a.W;
+    const int cx = (int)roundf(fx), cy = (int)roundf(fy);
+    if (cx < 1 || cy < 1 || cx >= W - 1 || cy >= W - 1) return false;
+    const float* g = a.gray + (size_t)v * W * W;
+    double s1 = 0.0, s2 = 0.0;
+    for (int yy = cy - 1; yy <= cy + 1; ++yy)
+        for (int xx = cx - 1; xx <= cx + 1; ++xx) {
+            const double q = (double)g[yy * W + xx];
+            s1 += q;
+            s2 += q * q;
+        }
+    const double scale = 1.0 / 9.0;
+    const double m = s1 * scale;
+    const double var = fmax(s2 * scale - m * m, 0.0);
+    mean = (float)m;
+    sd = (float)sqrt(var);
+    return isfinite(mean) && isfinite(sd) && sd > 1.0e-3f;
+}
+
+__device__ float f_photo(const FilterArgs& a, int v, int n, float px, float py, float qx,
+                         float qy) {
+#pragma clang fp contract(off)
+    float Ir = 0.0f, In = 0.0f;
+    if (!f_bilin_gray(a, v, px, py, Ir)) return __int_as_float(0x7fc00000);
+    if (!f_bilin_gray(a, n, qx, qy, In)) return __int_as_float(0x7fc00000);
+    float mr, sr, mn, sn;
+    if (f_mean_std3(a, v, px, py, mr, sr) && f_mean_std3(a, n, qx, qy, mn, sn)) {
+        const float ra = (Ir - mr) / sr;
+        const float rb = (In - mn) / sn;
+        return fabsf(ra - rb);
+    }
+    return fabsf(In - Ir);
+}
+
+__global__ void __launch_bounds__(1024) k_filter(FilterArgs a) {
+#pragma clang fp contract(off)
+    __shared__ uint8_t keep1[LFPM_MAX_CROP * LFPM_MAX_CROP];
+    const int v = blockIdx.x;
+    const int W = a.W, WW = W * W, t = threadIdx.x;
+    const lfpm_filter_params& fp = a.f;
+    const int cntv = min(a.cnt[v], a.K);
+    const size_t base = (size_t)v * WW;
+    uint8_t reason = LFPM_PR_KEEP;
+    bool k1 = false;
+    float d = 0.0f;
+    if (t < WW) {
+        const int x = t % W, y = t / W;
+        d = a.plane_in[base + t].w;
+        bool done = !isfinite(d) || d <= fp.min_valid_disp;
+        if (!done && fp.enable_cost_filter) {
+            const float cost = a.cost_in[base + t];
+            const bool invalid = !isfinite(cost) || cost < 0.0f;
+            const bool high = isfinite(cost) && cost > fp.max_cost;
+            if (invalid || high) { reason = LFPM_PR_COST; done = true; }
+        }
+        int vids[32];
+        int nv = 0;
+        if (!done) {
+            if (fp.use_selected_views_only) {
+                uint32_t m = a.sel_in[base + t];
+                for (int b = 0; m && nv < fp.max_triplet; ++b) {
+                    if (m & 1u) vids[nv++] = b;
+                    m >>= 1u;
+                }
+            } else {
+                const int lim = min(cntv, fp.max_triplet);
+                for (int b = 0; b < lim; ++b) vids[nv++] = b;
+            }
+            if (nv == 0) { reason = LFPM_PR_NOSUPPORT; done = true; }
+        }
+        if (!done) {
+            const float px = (float)x, py = (float)y;
+            int support = 0, good = 0;
+            const float2 cv = a.centers[v];
+            for (int vi = 0; vi < nv; ++vi) {
+                const int b = vids[vi];
+                if (b < 0 || b >= cntv) continue;
+                const int n = a.nbr[(size_t)v * a.K + b];
+                if (n < 0) continue;
+                const float2 cn = a.centers[n];
+                const float Bx = cv.x - cn.x, By = cv.y - cn.y;
+                const float bl = sqrtf(Bx * Bx + By * By);
+                if (bl < 1.0e-6f) continue;
+                const float dx = Bx / bl, dy = By / bl;
+                const float scale = bl / a.base;
+                const float qx = px + dx * d * scale;
+                const float qy = py + dy * d * scale;
+                if (qx < 0 || qy < 0 || qx > W - 1 || qy > W - 1) continue;
+                const float dn = f_bilin_disp(a, n, qx, qy);
+                if (!isfinite(dn)) continue;
+                if (fp.enable_cost_filter) {
+                    const float nc = f_bilin_cost(a, n, qx, qy);
+                    const bool inv = !isfinite(nc) || nc < 0.0f;
+                    const bool hi = isfinite(nc) && nc > fp.max_cost;
+                    if (inv || hi) continue;
+                }
+                ++support;
+                bool pg = true, pp = true;
+                if (fp.enable_cycle_geo_filter) {
+                    const float phx = qx - dx * dn * scale;
+                    const float phy = qy - dy * dn * scale;
+                    const float ex = phx - px, ey = phy - py;
+                    const float ge = sqrtf(ex * ex + ey * ey);
+                    pg = isfinite(ge) && ge <= fp.max_geo_err_px;
+                }
+                if (fp.enable_cycle_photo_filter) {
+                    const float pe = f_photo(a, v, n, px, py, qx, qy);
+                    pp = isfinite(pe) && pe <= fp.max_photo_err_u;
+                }
+                if (pg && pp) ++good;
+            }
+            if (support < fp.min_good_neighbors) reason = LFPM_PR_NOSUPPORT;
+            else if (good < fp.min_good_neighbors) reason = LFPM_PR_CYCLE;
+            else { k1 = true; reason = LFPM_PR_KEEP; }
+        }
+        keep1[t] = k1 ? 1 : 0;
+    }
+    __syncthreads();
+    if (t >= WW) return;
+    bool kf = k1;
+    if (k1 && fp.enable_spike_filter) {
+        const int x = t % W, y = t / W;
+        float vals[8];
+        int nvals = 0;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (dx == 0 && dy == 0) continue;
+                const int xn = x + dx, yn = y + dy;
+                if (xn < 0 || yn < 0 || xn >= W || yn >= W) continue;
+                const int ni = yn * W + xn;
+                if (!keep1[ni]) continue;
+                const float dv = a.plane_in[base + ni].w;
+                if (!isfinite(dv) || dv <= fp.min_valid_disp) continue;
+                vals[nvals++] = dv;
+            }
+        if (nvals >= fp.spike_min_neighbors) {
+            for (int i = 1; i < nvals; ++i) {  // insertion sort (<= 8 values)
+                const float key = vals[i];
+                int j = i - 1;
+                while (j >= 0 && vals[j] > key) { vals[j + 1] = vals[j]; --j; }
+                vals[j + 1] = key;
+            }
+            const float med = vals[nvals / 2];
+            if (isfinite(d) && fabsf(d - med) > fp.spike_abs_diff) {
+                kf = false;
+                reason = LFPM_PR_SPIKE;
+            }
+        }
+    }
+    const size_t g = base + t;
+    float4 po = a.plane_in[g];
+    float co = a.cost_in[g];
+    float4 dout = a.disp_in[g];
+    uint32_t so = a.sel_in[g];
+    if (isfinite(d) && d > fp.min_valid_disp && !kf) {
+        po = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        co = 2.0f;
+        dout = make_float4(-1.0f, -1.0f, 0.0f, 0.0f);
+        if (fp.clear_selected_views_when_invalid) so = 0u;
+    }
+    a.plane_out[g] = po;
+    a.cost_out[g] = co;
+    a.disp_out[g] = dout;
+    a.sel_out[g] = so;
+    a.reason[g] = reason;
+}
+
+// ------------------------------------------------------------------------------------------
+// Focal-stack refocus
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_refocus(const float* __restrict__ tiles,
+                                                 const float2* __restrict__ centers,
+                                                 const int* __restrict__ l2m, int cols, int rows,
+                                                 float pitch, int W, int out_w, int out_h, int S,
+                                                 float vmin, float vmax, float* __restrict__ out) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int k = blockIdx.z;
+    if (x >= out_w || y >= out_h) return;
+    const float vk = S > 1 ? vmin + (vmax - vmin) * (float)k / (float)(S - 1) : vmin;
+    const float halff = (float)(W - 1) * 0.5f;
+    const float reach = vk * halff + 1.0f;
+    const float rs = pitch * 0.8660254037844386f;
+    const int j0 = max(0, (int)floorf((y - reach - 0.5f * pitch) / rs) - 1);
+    const int j1 = min(rows - 1, (int)ceilf((y + reach - 0.5f * pitch) / rs) + 1);
+    const int WW = W * W;
+    float acc = 0.0f;
+    int nacc = 0;
+    for (int j = j0; j <= j1; ++j) {
+        const float off = 0.5f * pitch * (float)(1 + (j & 1));
+        const int i0 = max(0, (int)floorf((x - reach - off) / pitch) - 1);
+        const int i1 = min(cols - 1, (int)ceilf((x + reach - off) / pitch) + 1);
+        for (int i = i0; i <= i1; ++i) {
+            const int v = l2m[j * cols + i];
+            if (v < 0) continue;
+            const float2 c = centers[v];
+            const float ux = ((float)x - c.x) / vk + halff;
+            const float uy = ((float)y - c.y) / vk + halff;
+            if (!(ux >= 0.0f && ux <= (float)(W - 1) && uy >= 0.0f && uy <= (float)(W - 1)))
+                continue;
+            const float fx0 = floorf(ux), fy0 = floorf(uy);
+            const int x0 = (int)fx0, y0 = (int)fy0;
+            const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, W - 1);
+            const float fx = ux - fx0, fy = uy - fy0;
+            const float* tl = tiles + (size_t)v * WW;
+            const float a = tl[y0 * W + x0], b = tl[y0 * W + x1];
+            const float cc = tl[y1 * W + x0], d = tl[y1 * W + x1];
+            const float top = a + fx * (b - a), bot = cc + fx * (d - cc);
+            acc += top + fy * (bot - top);
+            ++nacc;
+        }
+    }
+    out[((size_t)k * out_h + y) * out_w + x] = nacc ? acc / (float)nacc : 0.0f;
+}
+
+// ------------------------------------------------------------------------------------------
+// V2R behavioural model fit, FP64: Gram-matrix partials, then a fixed-order sum and a
+// column-scaled 3x3 solve with partial pivoting on one lane.
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fit_partial(const float* __restrict__ aL,
+                                                     const float* __restrict__ v, int n,
+                                                     double* __restrict__ part) {
+    __shared__ double red[9][256];
+    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double a = aL[i], vv = v[i];
+        const double x0 = a * vv, x1 = vv, x2 = 1.0, yy = a;
+        s[0] += x0 * x0; s[1] += x0 * x1; s[2] += x0 * x2;
+        s[3] += x1 * x1; s[4] += x1 * x2; s[5] += x2 * x2;
+        s[6] += x0 * yy; s[7] += x1 * yy; s[8] += x2 * yy;
+    }
+    for (int q = 0; q < 9; ++q) red[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int q = 0; q < 9; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 9; ++q) part[(size_t)blockIdx.x * 9 + q] = red[q][0];
+}
+
+__global__ void k_fit_solve(const double* __restrict__ part, int nblocks, double* __restrict__ c) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nblocks; ++b)
+        for (int q = 0; q < 9; ++q) s[q] += part[(size_t)b * 9 + q];
+    double M[3][4] = {{s[0], s[1], s[2], s[6]}, {s[1], s[3], s[4], s[7]}, {s[2], s[4], s[5], s[8]}};
+    double sc[3];
+    for (int i = 0; i < 3; ++i) sc[i] = M[i][i] > 0 ? 1.0 / sqrt(M[i][i]) : 1.0;
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) M[i][j] *= sc[i] * sc[j];
+        M[i][3] *= sc[i];
+    }
+    for (int col = 0; col < 3; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 3; ++r)
+            if (fabs(M[r][col]) > fabs(M[piv][col])) piv = r;
+        if (piv != col)
+            for (int q = 0; q < 4; ++q) { const double tmp = M[col][q]; M[col][q] = M[piv][q]; M[piv][q] = tmp; }
+        for (int r = col + 1; r < 3; ++r) {
+            const double f = M[r][col] / M[col][col];
+            for (int q = col; q < 4; ++q) M[r][q] -= f * M[col][q];
+        }
+    }
+    double z[3];
+    for (int i = 2; i >= 0; --i) {
+        double acc = M[i][3];
+        for (int j = i + 1; j < 3; ++j) acc -= M[i][j] * z[j];
+        z[i] = acc / M[i][i];
+    }
+    for (int i = 0; i < 3; ++i) c[i] = z[i] * sc[i];
+}
+
+__global__ void k_v2r_convert(const float* __restrict__ v, int n, double c0, double c1, double c2,
+                              double* __restrict__ z) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double vv = v[i];
+        const double zz = (vv * c1 + c2) / (1.0 - vv * c0);
+        z[i] = (isnan(zz) || isinf(zz) || zz < 0.0) ? 0.0 : zz;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Launch plumbing
+// ------------------------------------------------------------------------------------------
+static int g_num_cus = 0;
+
+static int device_cus() {
+    if (g_num_cus > 0) return g_num_cus;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    g_num_cus = cus;
+    return cus;
+}
+
+static int phase_threads(const DevParams& P) {
+    const int nact = (P.WW + 1) / 2;
+    return std::min(256, 64 * ((nact + 63) / 64));
+}
+
+static int full_threads(const DevParams& P) { return 64 * ((P.WW + 63) / 64); }
+
+// Persistent grid: a multiple of 8 workgroups (one share per XCD), at most V rounded up.
+static int persistent_grid(const DevParams& P, int per_cu) {
+    int g = device_cus() * per_cu;
+    const int vv = ((P.V + 7) / 8) * 8;
+    g = std::min(g, std::max(8, vv));
+    g = (g / 8) * 8;
+    return std::max(8, g);
+}
+
+static thread_local unsigned long long* g_counters = nullptr;
+
+static const int kPhasePerCu = 8;
+static const int kInitPerCu = 4;
+
+static size_t init_scratch_bytes(const DevParams& P) {
+    return (size_t)persistent_grid(P, kInitPerCu) * P.K * full_threads(P) * 4;
+}
+static size_t phase_scratch_bytes(const DevParams& P) {
+    return (size_t)persistent_grid(P, kPhasePerCu) * 9 * P.K * phase_threads(P) * 4;
+}
+static size_t state_bytes(const DevParams& P) {
+    return (size_t)P.V * P.WW * (16 + 4 + 4);
+}
+
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+} // namespace lfpm
+
+using namespace lfpm;
+
+extern "C" {
+
+static int slice_impl(const uint8_t* raw, int raw_w, int raw_h, size_t raw_pitch, const int* org,
+                      int V, int crop, float* out, float div, void* stream) {
+    if (!raw || !org || !out || V <= 0 || crop < 1 || raw_w < crop || raw_h < crop ||
+        raw_pitch < (size_t)raw_w)
+        return set_error(LFPM_E_INVALID, "slice: bad arguments");
+    const size_t total = (size_t)V * crop * crop;
+    const int threads = 256;
+    const int blocks = (int)std::min<size_t>((total + threads - 1) / threads, 65536);
+    hipLaunchKernelGGL(k_slice, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, raw,
+                       raw_pitch, (const int2*)org, V, crop, div, out);
+    return launch_check("slice");
+}
+
+int lfpm_slice_tiles(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pitch,
+                     const int* origins_dev, int num_mis, int crop, float* tiles_dev,
+                     void* stream) {
+    return slice_impl(raw_dev, raw_w, raw_h, raw_pitch, origins_dev, num_mis, crop, tiles_dev,
+                      255.0f, stream);
+}
+
+int lfpm_slice_gray(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pitch,
+                    const int* origins_dev, int num_mis, int crop, float* gray_dev, void* stream) {
+    return slice_impl(raw_dev, raw_w, raw_h, raw_pitch, origins_dev, num_mis, crop, gray_dev,
+                      0.0f, stream);
+}
+
+size_t lfpm_patchmatch_workspace_size(int num_mis, const lfpm_pm_params* p) {
+    if (check_pm_params(p, num_mis)) return 0;
+    const DevParams P = make_dev_params(p, num_mis);
+    return align_up(state_bytes(P)) +
+           align_up(std::max(init_scratch_bytes(P), phase_scratch_bytes(P)));
+}
+
+static int check_solver_ptrs(const float* tiles, const float* centers, const int* nbr,
+                             const int* cnt) {
+    if (!tiles || !centers || !nbr || !cnt) return set_error(LFPM_E_INVALID, "null input");
+    return LFPM_OK;
+}
+
+int lfpm_patchmatch_init(const float* tiles_dev, const float* centers_dev,
+                         const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                         int num_mis, const lfpm_pm_params* p, float* plane_dev, float* cost_dev,
+                         uint32_t* sel_dev, void* workspace_dev, size_t workspace_bytes,
+                         void* stream) {
+    int rc = check_pm_params(p, num_mis);
+    if (rc) return rc;
+    if ((rc = check_solver_ptrs(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev)))
+        return rc;
+    if (!plane_dev || !cost_dev || !sel_dev) return set_error(LFPM_E_INVALID, "null output");
+    DevParams P = make_dev_params(p, num_mis);
+    const int grid = persistent_grid(P, kInitPerCu);
+    P.slots_per_xcd = grid / 8;
+    const int NT = full_threads(P);
+    if (NT > 1024) return set_error(LFPM_E_INVALID, "crop too large for init kernel");
+    if (!workspace_dev || workspace_bytes < init_scratch_bytes(P))
+        return set_error(LFPM_E_INVALID, "init: workspace too small (%zu < %zu)", workspace_bytes,
+                         init_scratch_bytes(P));
+    InitArgs a;
+    a.tiles = tiles_dev; a.centers = (const float2*)centers_dev; a.nbr = neighbor_ids_dev;
+    a.cnt = neighbor_counts_dev; a.plane = (float4*)plane_dev; a.cost = cost_dev; a.sel = sel_dev;
+    a.scratch = (float*)workspace_dev; a.P = P; a.counters = g_counters;
+    if (g_counters)
+        hipLaunchKernelGGL(k_init<true>, dim3(grid), dim3(NT), lds_bytes(P, NT, false),
+                           (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL(k_init<false>, dim3(grid), dim3(NT), lds_bytes(P, NT, false),
+                           (hipStream_t)stream, a);
+    return launch_check("init");
+}
+
+int lfpm_patchmatch_phase(const float* tiles_dev, const float* centers_dev,
+                          const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                          int num_mis, const lfpm_pm_params* p, int iter, int phase,
+                          const float* plane_in_dev, const float* cost_in_dev,
+                          const uint32_t* sel_in_dev, float* plane_out_dev, float* cost_out_dev,
+                          uint32_t* sel_out_dev, void* workspace_dev, size_t workspace_bytes,
+                          void* stream) {
+    int rc = check_pm_params(p, num_mis);
+    if (rc) return rc;
+    if ((rc = check_solver_ptrs(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev)))
+        return rc;
+    if (!plane_in_dev || !cost_in_dev || !sel_in_dev || !plane_out_dev || !cost_out_dev ||
+        !sel_out_dev)
+        return set_error(LFPM_E_INVALID, "null state buffer");
+    if (phase != 0 && phase != 1) return set_error(LFPM_E_INVALID, "phase must be 0 or 1");
+    if (iter < 0) return set_error(LFPM_E_INVALID, "iter must be >= 0");
+    DevParams P = make_dev_params(p, num_mis);
+    const int grid = persistent_grid(P, kPhasePerCu);
+    P.slots_per_xcd = grid / 8;
+    const int NT = phase_threads(P);
+    if ((P.WW + 1) / 2 > NT) return set_error(LFPM_E_INVALID, "crop too large for phase kernel");
+    if (!workspace_dev || workspace_bytes < phase_scratch_bytes(P))
+        return set_error(LFPM_E_INVALID, "phase: workspace too small (%zu < %zu)",
+                         workspace_bytes, phase_scratch_bytes(P));
+    PhaseArgs a;
+    a.tiles = tiles_dev; a.centers = (const float2*)centers_dev; a.nbr = neighbor_ids_dev;
+    a.cnt = neighbor_counts_dev;
+    a.plane_in = (const float4*)plane_in_dev; a.cost_in = cost_in_dev; a.sel_in = sel_in_dev;
+    a.plane_out = (float4*)plane_out_dev; a.cost_out = cost_out_dev; a.sel_out = sel_out_dev;
+    a.scratch = (float*)workspace_dev; a.P = P; a.iter = iter; a.phase = phase;
+    a.counters = g_counters;
+    if (g_counters)
+        hipLaunchKernelGGL(k_phase<true>, dim3(grid), dim3(NT), lds_bytes(P, NT, true),
+                           (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL(k_phase<false>, dim3(grid), dim3(NT), lds_bytes(P, NT, true),
+                           (hipStream_t)stream, a);
+    return launch_check("phase");
+}
+
+int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
+                             const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                             int num_mis, const lfpm_pm_params* p, const float* plane_dev,
+                             float* disp_dev, void* stream) {
+    int rc = check_pm_params(p, num_mis);
+    if (rc) return rc;
+    if ((rc = check_solver_ptrs(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev)))
+        return rc;
+    if (!plane_dev || !disp_dev) return set_error(LFPM_E_INVALID, "null buffer");
+    DevParams P = make_dev_params(p, num_mis);
+    const int grid = persistent_grid(P, kInitPerCu);
+    P.slots_per_xcd = grid / 8;
+    const int NT = full_threads(P);
+    FinalArgs a;
+    a.tiles = tiles_dev; a.centers = (const float2*)centers_dev; a.nbr = neighbor_ids_dev;
+    a.cnt = neighbor_counts_dev; a.plane = (const float4*)plane_dev; a.disp = (float4*)disp_dev;
+    a.P = P; a.counters = g_counters;
+    if (g_counters)
+        hipLaunchKernelGGL(k_finalize<true>, dim3(grid), dim3(NT), lds_bytes(P, NT, false),
+                           (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL(k_finalize<false>, dim3(grid), dim3(NT), lds_bytes(P, NT, false),
+                           (hipStream_t)stream, a);
+    return launch_check("finalize");
+}
+
+int lfpm_patchmatch_frame(const float* tiles_dev, const float* centers_dev,
+                          const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                          int num_mis, const lfpm_pm_params* p, float* plane_dev, float* cost_dev,
+                          float* disp_dev, uint32_t* sel_dev, void* workspace_dev,
+                          size_t workspace_bytes, int sync_each_phase, void* stream) {
+    int rc = check_pm_params(p, num_mis);
+    if (rc) return rc;
+    const DevParams P = make_dev_params(p, num_mis);
+    const size_t need = lfpm_patchmatch_workspace_size(num_mis, p);
+    if (!workspace_dev || workspace_bytes < need)
+        return set_error(LFPM_E_INVALID, "frame: workspace too small (%zu < %zu)", workspace_bytes,
+                         need);
+    const size_t npx = (size_t)P.V * P.WW;
+    unsigned char* ws = (unsigned char*)workspace_dev;
+    float* plane_b = (float*)ws;
+    float* cost_b = (float*)(ws + npx * 16);
+    uint32_t* sel_b = (uint32_t*)(ws + npx * 20);
+    unsigned char* scratch = ws + align_up(state_bytes(P));
+    const size_t scratch_bytes = workspace_bytes - align_up(state_bytes(P));
+    hipStream_t s = (hipStream_t)stream;
+    auto sync = [&]() -> int {
+        if (!sync_each_phase) return LFPM_OK;
+        HIP_TRY(hipStreamSynchronize(s));
+        return LFPM_OK;
+    };
+    if ((rc = lfpm_patchmatch_init(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev,
+                                   num_mis, p, plane_dev, cost_dev, sel_dev, scratch,
+                                   scratch_bytes, stream)))
+        return rc;
+    if ((rc = sync())) return rc;
+    for (int it = 0; it < p->iterations; ++it) {
+        if ((rc = lfpm_patchmatch_phase(tiles_dev, centers_dev, neighbor_ids_dev,
+                                        neighbor_counts_dev, num_mis, p, it, 0, plane_dev,
+                                        cost_dev, sel_dev, plane_b, cost_b, sel_b, scratch,
+                                        scratch_bytes, stream)))
+            return rc;
+        if ((rc = sync())) return rc;
+        if ((rc = lfpm_patchmatch_phase(tiles_dev, centers_dev, neighbor_ids_dev,
+                                        neighbor_counts_dev, num_mis, p, it, 1, plane_b, cost_b,
+                                        sel_b, plane_dev, cost_dev, sel_dev, scratch,
+                                        scratch_bytes, stream)))
+            return rc;
+        if ((rc = sync())) return rc;
+    }
+    if (disp_dev) {
+        if ((rc = lfpm_patchmatch_finalize(tiles_dev, centers_dev, neighbor_ids_dev,
+                                           neighbor_counts_dev, num_mis, p, plane_dev, disp_dev,
+                                           stream)))
+            return rc;
+        if ((rc = sync())) return rc;
+    }
+    return LFPM_OK;
+}
+
+int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,
+                    const int* neighbor_ids_dev, const int* neighbor_counts_dev, int num_mis,
+                    const lfpm_pm_params* p, const void* hyp_dev, int n, float* cost_out_dev,
+                    void* stream) {
+    int rc = check_pm_params(p, num_mis);
+    if (rc) return rc;
+    if ((rc = check_solver_ptrs(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev)))
+        return rc;
+    if (n < 0 || (n > 0 && (!hyp_dev || !cost_out_dev)))
+        return set_error(LFPM_E_INVALID, "eval: bad hypothesis buffer");
+    if (n == 0) return LFPM_OK;
+    DevParams P = make_dev_params(p, num_mis);
+    const int grid = persistent_grid(P, kPhasePerCu);
+    P.slots_per_xcd = grid / 8;
+    int* seg = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMallocAsync((void**)&seg, sizeof(int) * (num_mis + 1), s));
+    hipLaunchKernelGGL(k_segments, dim3((num_mis + 256) / 256), dim3(256), 0, s,
+                       (const int*)hyp_dev, n, num_mis, seg);
+    EvalArgs a;
+    a.tiles = tiles_dev; a.centers = (const float2*)centers_dev; a.nbr = neighbor_ids_dev;
+    a.cnt = neighbor_counts_dev; a.hyp = (const int*)hyp_dev; a.seg = seg; a.n = n;
+    a.out = cost_out_dev; a.P = P; a.counters = g_counters;
+    const int NT = 256;
+    if (g_counters)
+        hipLaunchKernelGGL(k_eval<true>, dim3(grid), dim3(NT), lds_bytes(P, NT, false), s, a);
+    else
+        hipLaunchKernelGGL(k_eval<false>, dim3(grid), dim3(NT), lds_bytes(P, NT, false), s, a);
+    rc = launch_check("eval");
+    HIP_TRY(hipFreeAsync(seg, s));
+    return rc;
+}
+
+int lfpm_filter_frame(const float* gray_dev, const float* centers_dev,
+                      const int* neighbor_ids_dev, const int* neighbor_counts_dev, int num_mis,
+                      int crop, int max_neighbors, float base, const lfpm_filter_params* f,
+                      const float* plane_in_dev, const float* cost_in_dev,
+                      const float* disp_in_dev, const uint32_t* sel_in_dev, float* plane_out_dev,
+                      float* cost_out_dev, float* disp_out_dev, uint32_t* sel_out_dev,
+                      uint8_t* reason_dev, void* stream) {
+    if (!f || num_mis <= 0 || crop < 3 || crop > LFPM_MAX_CROP || max_neighbors < 1 ||
+        max_neighbors > LFPM_MAX_NEIGHBORS || !(base > 0.0f))
+        return set_error(LFPM_E_INVALID, "filter: bad parameters");
+    if (f->max_triplet < 0 || f->max_triplet > 32)
+        return set_error(LFPM_E_INVALID, "filter: max_triplet outside [0, 32]");
+    if (!gray_dev || !centers_dev || !neighbor_ids_dev || !neighbor_counts_dev || !plane_in_dev ||
+        !cost_in_dev || !disp_in_dev || !sel_in_dev || !plane_out_dev || !cost_out_dev ||
+        !disp_out_dev || !sel_out_dev || !reason_dev)
+        return set_error(LFPM_E_INVALID, "filter: null buffer");
+    FilterArgs a;
+    a.gray = gray_dev; a.centers = (const float2*)centers_dev; a.nbr = neighbor_ids_dev;
+    a.cnt = neighbor_counts_dev; a.V = num_mis; a.W = crop; a.K = max_neighbors; a.base = base;
+    a.f = *f;
+    a.plane_in = (const float4*)plane_in_dev; a.cost_in = cost_in_dev;
+    a.disp_in = (const float4*)disp_in_dev; a.sel_in = sel_in_dev;
+    a.plane_out = (float4*)plane_out_dev; a.cost_out = cost_out_dev;
+    a.disp_out = (float4*)disp_out_dev; a.sel_out = sel_out_dev; a.reason = reason_dev;
+    const int NT = 64 * ((crop * crop + 63) / 64);
+    hipLaunchKernelGGL(k_filter, dim3(num_mis), dim3(NT), 0, (hipStream_t)stream, a);
+    return launch_check("filter");
+}
+
+int lfpm_refocus(const float* tiles_dev, const float* centers_dev, const int* lattice_to_mi_dev,
+                 int lattice_cols, int lattice_rows, float pitch, int crop, int out_w, int out_h,
+                 int num_slices, float vmin, float vmax, float* out_dev, void* stream) {
+    if (!tiles_dev || !centers_dev || !lattice_to_mi_dev || !out_dev || lattice_cols <= 0 ||
+        lattice_rows <= 0 || !(pitch > 0.0f) || crop < 2 || out_w <= 0 || out_h <= 0 ||
+        num_slices <= 0 || !(vmin > 0.0f) || !(vmax >= vmin))
+        return set_error(LFPM_E_INVALID, "refocus: bad arguments");
+    dim3 grid((out_w + 15) / 16, (out_h + 15) / 16, num_slices);
+    hipLaunchKernelGGL(k_refocus, grid, dim3(256), 0, (hipStream_t)stream, tiles_dev,
+                       (const float2*)centers_dev, lattice_to_mi_dev, lattice_cols, lattice_rows,
+                       pitch, crop, out_w, out_h, num_slices, vmin, vmax, out_dev);
+    return launch_check("refocus");
+}
+
+int lfpm_fit_v2r(const float* a_l_dev, const float* v_dev, int n, double c_out[3], void* stream) {
+    if (!a_l_dev || !v_dev || !c_out || n < 3)
+        return set_error(LFPM_E_INVALID, "fit: need >= 3 samples");
+    hipStream_t s = (hipStream_t)stream;
+    const int blocks = std::min(1024, (n + 255) / 256);
+    double* part = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&part, sizeof(double) * (blocks * 9 + 3), s));
+    hipLaunchKernelGGL(k_fit_partial, dim3(blocks), dim3(256), 0, s, a_l_dev, v_dev, n, part);
+    hipLaunchKernelGGL(k_fit_solve, dim3(1), dim3(64), 0, s, part, blocks, part + blocks * 9);
+    int rc = launch_check("fit");
+    if (rc == LFPM_OK) {
+        HIP_TRY(hipMemcpyAsync(c_out, part + blocks * 9, sizeof(double) * 3,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    HIP_TRY(hipFreeAsync(part, s));
+    return rc;
+}
+
+int lfpm_set_work_counters(unsigned long long* counters_dev) {
+    g_counters = counters_dev;
+    return LFPM_OK;
+}
+
+int lfpm_v2r_convert(const float* v_dev, int n, const double c[3], double* z_dev, void* stream) {
+    if (!v_dev || !z_dev || !c || n < 0) return set_error(LFPM_E_INVALID, "convert: bad args");
+    if (n == 0) return LFPM_OK;
+    const int blocks = std::min(65535, (n + 255) / 256);
+    hipLaunchKernelGGL(k_v2r_convert, dim3(blocks), dim3(256), 0, (hipStream_t)stream, v_dev, n,
+                       c[0], c[1], c[2], z_dev);
+    return launch_check("v2r_convert");
+}
+
+} // extern "C"

@@ -1,0 +1,357 @@
+"""Python mirror of the lfpm C ABI (include/lfpm.h) over ctypes.
+
+Host entry points take numpy arrays; device entry points take torch CUDA tensors (PyTorch is
+only the device-memory and stream plumbing here) and run on the current torch stream.  The
+HIP library must be built (``make -C plenoptic-for-depth_amd``); there is no CPU fallback:
+every device call goes through liblfpm.so and raises LfpmError on a non-zero status.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblfpm.so")
+
+
+class LfpmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("lfpm error %d: %s" % (code, msg))
+        self.code = code
+
+
+class LayoutParams(C.Structure):
+    _fields_ = [("pitch", C.c_float), ("crop", C.c_int), ("raw_w", C.c_int),
+                ("raw_h", C.c_int), ("max_neighbors", C.c_int), ("lens_types", C.c_int),
+                ("neighbors_per_type", C.c_int * 3)]
+
+
+class LayoutInfo(C.Structure):
+    _fields_ = [("num_mis", C.c_int), ("crop", C.c_int), ("lattice_cols", C.c_int),
+                ("lattice_rows", C.c_int), ("row_step", C.c_float)]
+
+
+class SynthParams(C.Structure):
+    _fields_ = [("raw_w", C.c_int), ("raw_h", C.c_int), ("pitch", C.c_float),
+                ("num_planes", C.c_int), ("vmin", C.c_float), ("vmax", C.c_float),
+                ("scene_seed", C.c_uint32), ("noise_seed", C.c_uint32),
+                ("noise_sigma", C.c_float)]
+
+
+class PmParams(C.Structure):
+    _fields_ = [("crop", C.c_int), ("base", C.c_float), ("radius", C.c_int),
+                ("radius_increment", C.c_int), ("sigma_spatial", C.c_float),
+                ("sigma_color", C.c_float), ("min_var", C.c_float), ("top_k", C.c_int),
+                ("depth_min", C.c_float), ("depth_max", C.c_float), ("iterations", C.c_int),
+                ("max_neighbors", C.c_int), ("seed", C.c_uint32), ("frame_uid", C.c_uint32),
+                ("storage", C.c_int), ("cos_max_slant", C.c_float)]
+
+
+class FilterParams(C.Structure):
+    _fields_ = [("max_triplet", C.c_int), ("min_valid_disp", C.c_float),
+                ("use_selected_views_only", C.c_int),
+                ("clear_selected_views_when_invalid", C.c_int),
+                ("enable_cost_filter", C.c_int), ("max_cost", C.c_float),
+                ("enable_cycle_geo_filter", C.c_int), ("max_geo_err_px", C.c_float),
+                ("enable_cycle_photo_filter", C.c_int), ("max_photo_err_u", C.c_float),
+                ("min_good_neighbors", C.c_int), ("enable_spike_filter", C.c_int),
+                ("spike_abs_diff", C.c_float), ("spike_min_neighbors", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load liblfpm.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LfpmError(-4, "liblfpm.so not built (run make -C plenoptic-for-depth_amd)")
+        L = C.CDLL(LIB_PATH)
+        L.lfpm_last_error.restype = C.c_char_p
+        L.lfpm_patchmatch_workspace_size.restype = C.c_size_t
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise LfpmError(rc, lib().lfpm_last_error().decode())
+
+
+def _np(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dp(t):
+    """Device pointer of a torch tensor (must be contiguous)."""
+    assert t.is_contiguous()
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+# --------------------------------------------------------------------------------------
+# Host entry points
+# --------------------------------------------------------------------------------------
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().lfpm_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def layout_params(pitch, raw_w, raw_h, max_neighbors, crop=0, lens_types=1, per_type=None):
+    per = per_type or (max_neighbors, max_neighbors, max_neighbors)
+    return LayoutParams(float(pitch), int(crop), int(raw_w), int(raw_h), int(max_neighbors),
+                        int(lens_types), (C.c_int * 3)(*per))
+
+
+def layout(lp):
+    info = LayoutInfo()
+    _check(lib().lfpm_layout_query(C.byref(lp), C.byref(info)))
+    V, K = info.num_mis, lp.max_neighbors
+    out = dict(V=V, crop=info.crop, cols=info.lattice_cols, rows=info.lattice_rows,
+               row_step=info.row_step, pitch=lp.pitch, K=K,
+               centers=np.zeros((V, 2), np.float32), origins=np.zeros((V, 2), np.int32),
+               tilekeys=np.zeros((V, 2), np.int32), lens_type=np.zeros(V, np.int32),
+               nbr=np.zeros((V, K), np.int32), cnt=np.zeros(V, np.int32),
+               lattice_to_mi=np.zeros((info.lattice_rows, info.lattice_cols), np.int32))
+    _check(lib().lfpm_layout_build(C.byref(lp), _np(out["centers"]), _np(out["origins"]),
+                                   _np(out["tilekeys"]), _np(out["lens_type"]), _np(out["nbr"]),
+                                   _np(out["cnt"]), _np(out["lattice_to_mi"])))
+    return out
+
+
+def synth(sp, want_gt=True):
+    raw = np.zeros((sp.raw_h, sp.raw_w), np.uint8)
+    gt = np.zeros((sp.raw_h, sp.raw_w), np.float32) if want_gt else None
+    _check(lib().lfpm_synth_frame(C.byref(sp), _np(raw), _np(gt) if want_gt else None))
+    return raw, gt
+
+
+def pm_default(crop, base):
+    p = PmParams()
+    lib().lfpm_pm_params_default(C.byref(p), int(crop), C.c_float(base))
+    return p
+
+
+def filter_default():
+    f = FilterParams()
+    lib().lfpm_filter_params_default(C.byref(f))
+    return f
+
+
+def workspace_size(V, pm):
+    n = lib().lfpm_patchmatch_workspace_size(int(V), C.byref(pm))
+    if n == 0:
+        _check(-1)
+    return n
+
+
+# --------------------------------------------------------------------------------------
+# Workloads (synthetic focused-plenoptic frames)
+# --------------------------------------------------------------------------------------
+class Workload:
+    """One synthetic frame with its lens layout and solve/filter/refocus settings."""
+
+    def __init__(self, raw_w, raw_h, pitch, K, radius, sigma_s, sigma_c, iterations, planes,
+                 vmin, vmax, seed, scene_seed, noise_seed, noise_sigma=2.0, lens_types=1,
+                 per_type=None, slices=32, frame_uid=0):
+        self.lp = layout_params(pitch, raw_w, raw_h, K, lens_types=lens_types, per_type=per_type)
+        self.sp = SynthParams(raw_w, raw_h, float(pitch), planes, float(vmin), float(vmax),
+                              scene_seed, noise_seed, float(noise_sigma))
+        self.L = layout(self.lp)
+        pm = pm_default(self.L["crop"], float(pitch))
+        pm.radius = radius
+        pm.sigma_spatial = float(sigma_s)
+        pm.sigma_color = float(sigma_c)
+        pm.iterations = iterations
+        pm.max_neighbors = K
+        pm.seed = seed
+        pm.frame_uid = frame_uid
+        self.pm = pm
+        self.fp = filter_default()
+        self.slices = slices
+        self.vmin, self.vmax = float(vmin), float(vmax)
+        self._raw = None
+        self._gt = None
+
+    @property
+    def V(self):
+        return self.L["V"]
+
+    @property
+    def crop(self):
+        return self.L["crop"]
+
+    def raw(self):
+        if self._raw is None:
+            self._raw, self._gt = synth(self.sp)
+        return self._raw
+
+    def gt_virtual_depth(self):
+        self.raw()
+        return self._gt
+
+
+def workload_c1(**kw):
+    """1024x800 raw, pitch 23, 6 ring-1 neighbours, 11x11 window (s_s=3, s_c=0.1), 3 iterations,
+    4 planes with virtual depth in [2.5, 6], Philox seed 1234."""
+    args = dict(raw_w=1024, raw_h=800, pitch=23.0, K=6, radius=5, sigma_s=3.0, sigma_c=0.1,
+                iterations=3, planes=4, vmin=2.5, vmax=6.0, seed=1234, scene_seed=1234,
+                noise_seed=1235)
+    args.update(kw)
+    return Workload(**args)
+
+
+def workload_c2(**kw):
+    """4080x3068 raw, pitch 23 (~27k MIs), 18 ring-1+ring-2 neighbours, 15x15 window (s_s=5,
+    s_c=0.12), 6 iterations, 12 planes with virtual depth in [2, 8], cycle check 0.5 px,
+    32 refocus slices."""
+    args = dict(raw_w=4080, raw_h=3068, pitch=23.0, K=18, radius=7, sigma_s=5.0, sigma_c=0.12,
+                iterations=6, planes=12, vmin=2.0, vmax=8.0, seed=4321, scene_seed=4321,
+                noise_seed=4322)
+    args.update(kw)
+    return Workload(**args)
+
+
+# --------------------------------------------------------------------------------------
+# Device entry points (torch tensors on the current device)
+# --------------------------------------------------------------------------------------
+class DeviceFrame:
+    """Device-resident inputs of one frame: raw, layout tables and tiles."""
+
+    def __init__(self, wl, raw=None, device="cuda"):
+        import torch
+        self.wl = wl
+        L = wl.L
+        raw = wl.raw() if raw is None else raw
+        self.raw = torch.from_numpy(np.ascontiguousarray(raw)).to(device)
+        self.centers = torch.from_numpy(L["centers"]).to(device)
+        self.origins = torch.from_numpy(L["origins"]).to(device)
+        self.nbr = torch.from_numpy(L["nbr"]).to(device)
+        self.cnt = torch.from_numpy(L["cnt"]).to(device)
+        self.l2m = torch.from_numpy(L["lattice_to_mi"]).to(device)
+        V, W = L["V"], L["crop"]
+        self.tiles = torch.empty((V, W, W), dtype=torch.float32, device=device)
+        self.gray = torch.empty((V, W, W), dtype=torch.float32, device=device)
+        self.slice()
+
+    def slice(self, stream=None):
+        L = self.wl.L
+        h, w = self.raw.shape
+        _check(lib().lfpm_slice_tiles(_dp(self.raw), w, h, C.c_size_t(w), _dp(self.origins),
+                                      L["V"], L["crop"], _dp(self.tiles), _stream(stream)))
+        _check(lib().lfpm_slice_gray(_dp(self.raw), w, h, C.c_size_t(w), _dp(self.origins),
+                                     L["V"], L["crop"], _dp(self.gray), _stream(stream)))
+
+
+class SolveBuffers:
+    def __init__(self, V, W, pm, device="cuda"):
+        import torch
+        n = V * W * W
+        self.plane = torch.empty((n, 4), dtype=torch.float32, device=device)
+        self.cost = torch.empty(n, dtype=torch.float32, device=device)
+        self.disp = torch.empty((n, 4), dtype=torch.float32, device=device)
+        self.sel = torch.empty(n, dtype=torch.int32, device=device)
+        self.ws_bytes = workspace_size(V, pm)
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=device)
+
+
+def patchmatch_frame(df, pm=None, bufs=None, sync_each_phase=False, stream=None):
+    pm = pm or df.wl.pm
+    V, W = df.wl.V, df.wl.crop
+    bufs = bufs or SolveBuffers(V, W, pm, df.tiles.device)
+    _check(lib().lfpm_patchmatch_frame(
+        _dp(df.tiles), _dp(df.centers), _dp(df.nbr), _dp(df.cnt), V, C.byref(pm),
+        _dp(bufs.plane), _dp(bufs.cost), _dp(bufs.disp), _dp(bufs.sel), _dp(bufs.ws),
+        C.c_size_t(bufs.ws_bytes), 1 if sync_each_phase else 0, _stream(stream)))
+    return bufs
+
+
+def eval_costs(df, hyp, pm=None, stream=None):
+    """hyp: torch int32 (n, 6) on device, sorted by MI.  Returns (n, K) float32 costs."""
+    import torch
+    pm = pm or df.wl.pm
+    n = hyp.shape[0]
+    out = torch.empty((n, pm.max_neighbors), dtype=torch.float32, device=hyp.device)
+    _check(lib().lfpm_eval_costs(_dp(df.tiles), _dp(df.centers), _dp(df.nbr), _dp(df.cnt),
+                                 df.wl.V, C.byref(pm), _dp(hyp), n, _dp(out), _stream(stream)))
+    return out
+
+
+def filter_frame(df, bufs, fp=None, stream=None):
+    import torch
+    fp = fp or df.wl.fp
+    V, W = df.wl.V, df.wl.crop
+    n = V * W * W
+    dev = bufs.plane.device
+    out = dict(plane=torch.empty((n, 4), dtype=torch.float32, device=dev),
+               cost=torch.empty(n, dtype=torch.float32, device=dev),
+               disp=torch.empty((n, 4), dtype=torch.float32, device=dev),
+               sel=torch.empty(n, dtype=torch.int32, device=dev),
+               reason=torch.empty(n, dtype=torch.uint8, device=dev))
+    _check(lib().lfpm_filter_frame(
+        _dp(df.gray), _dp(df.centers), _dp(df.nbr), _dp(df.cnt), V, W, df.wl.pm.max_neighbors,
+        C.c_float(df.wl.pm.base), C.byref(fp), _dp(bufs.plane), _dp(bufs.cost), _dp(bufs.disp),
+        _dp(bufs.sel), _dp(out["plane"]), _dp(out["cost"]), _dp(out["disp"]), _dp(out["sel"]),
+        _dp(out["reason"]), _stream(stream)))
+    return out
+
+
+def refocus(df, slices=None, vmin=None, vmax=None, out=None, stream=None):
+    import torch
+    wl = df.wl
+    S = slices or wl.slices
+    vmin = wl.vmin if vmin is None else vmin
+    vmax = wl.vmax if vmax is None else vmax
+    h, w = df.raw.shape
+    if out is None:
+        out = torch.empty((S, h, w), dtype=torch.float32, device=df.tiles.device)
+    _check(lib().lfpm_refocus(_dp(df.tiles), _dp(df.centers), _dp(df.l2m), wl.L["cols"],
+                              wl.L["rows"], C.c_float(wl.L["pitch"]), wl.crop, w, h, S,
+                              C.c_float(vmin), C.c_float(vmax), _dp(out), _stream(stream)))
+    return out
+
+
+def fit_v2r(a_l, v, stream=None):
+    c = (C.c_double * 3)()
+    _check(lib().lfpm_fit_v2r(_dp(a_l), _dp(v), a_l.numel(), c, _stream(stream)))
+    return np.array(list(c))
+
+
+def v2r_convert(v, c, stream=None):
+    import torch
+    z = torch.empty(v.numel(), dtype=torch.float64, device=v.device)
+    cc = (C.c_double * 3)(*[float(x) for x in c])
+    _check(lib().lfpm_v2r_convert(_dp(v), v.numel(), cc, _dp(z), _stream(stream)))
+    return z
+
+
+def ncc_sample_counts(pm):
+    """FLOP accounting constants: 22 FLOP per evaluated window sample, 16 per finalize."""
+    return 22, 16
+
+
+def hypotheses_from_arrays(mi, pix, planes):
+    """Pack (n,) mi, (n,) pixel, (n,4) float32 planes into the (n,6) int32 record layout."""
+    n = mi.shape[0]
+    rec = np.zeros((n, 6), np.int32)
+    rec[:, 0] = mi
+    rec[:, 1] = pix
+    rec[:, 2:] = np.ascontiguousarray(planes, np.float32).view(np.int32)
+    return rec
+
+
+def version():
+    return "lfpm-0.1"
+
+
+_ = math

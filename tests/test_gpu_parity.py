@@ -1,0 +1,210 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the CPU oracle on the same seeded
+inputs.  Bars: bit-exact for gathers, masks and determinism; fixed-plane costs within the
+stated relative tolerance of the FP64 oracle; full-solve disparities within 0.05 px of the FP64
+oracle on >= 99.5 % of valid pixels (valid = oracle final cost <= 1.8, the CheckModule cost gate).
+"""
+import ctypes as C
+import time
+
+import numpy as np
+import pytest
+
+from _helpers import cone_hypotheses, fp_to_oracle, pm_to_oracle, rel_err
+from conftest import stats_dump
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch
+
+
+@pytest.fixture(scope="module")
+def c1(lfpm, torch_cuda):
+    wl = lfpm.workload_c1()
+    df = lfpm.DeviceFrame(wl)
+    return wl, df
+
+
+def test_slice_exact(lfpm, orc, c1, torch_cuda):
+    wl, df = c1
+    tiles_u8 = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    ref = tiles_u8.astype(np.float32) / np.float32(255.0)
+    torch_cuda.cuda.synchronize()
+    assert np.array_equal(df.tiles.cpu().numpy(), ref)
+    assert np.array_equal(df.gray.cpu().numpy(), tiles_u8.astype(np.float32))
+
+
+@pytest.mark.parametrize("cfg", ["c1", "c2params"])
+def test_eval_costs_vs_fp64(lfpm, orc, torch_cuda, cfg):
+    """Fixed-plane bilateral NCC (ComputeBilateralNCC_Frame) vs the FP64 oracle.  Relative error
+    |c - c*| / max(|c*|, 1e-3); both sides return exactly 2 for unscored views."""
+    _say("case eval_costs", cfg)
+    if cfg == "c1":
+        wl = lfpm.workload_c1()
+    else:  # C2 window/neighbour parameters on a 1024x800 frame
+        wl = lfpm.workload_c2(raw_w=1024, raw_h=800)
+    df = lfpm.DeviceFrame(wl)
+    hyp = cone_hypotheses(lfpm, wl, 40000, seed=7)
+    out = lfpm.eval_costs(df, torch_cuda.from_numpy(hyp).cuda()).cpu().numpy()
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    ref = orc.eval_costs(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"],
+                         pm_to_oracle(orc, wl.pm), hyp)
+    both = (out < 2) & (ref < 2)
+    flip = ((out < 2) != (ref < 2)).mean()
+    e = rel_err(out[both].astype(np.float64), ref[both])
+    st = dict(cfg=cfg, n=int(both.sum()), flip=float(flip), max=float(e.max()),
+              p999=float(np.quantile(e, 0.999)), p99=float(np.quantile(e, 0.99)),
+              median=float(np.median(e)), frac_le_1e5=float((e <= 1e-5).mean()),
+              frac_le_1e4=float((e <= 1e-4).mean()))
+    _say("eval_costs", st)
+    stats_dump("eval_costs_" + cfg, st)
+    assert flip < 1e-3
+    assert st["median"] <= 1e-5
+    assert st["p99"] <= 1e-3
+
+
+def _solve_gpu(lfpm, df, torch_cuda, pm=None):
+    bufs = lfpm.patchmatch_frame(df, pm=pm)
+    torch_cuda.cuda.synchronize()
+    return dict(plane=bufs.plane.cpu().numpy(), cost=bufs.cost.cpu().numpy(),
+                disp=bufs.disp.cpu().numpy(), sel=bufs.sel.cpu().numpy().view(np.uint32)), bufs
+
+
+def test_solve_c1_vs_fp64(lfpm, orc, c1, torch_cuda):
+    """Full solve (init + 3 red-black iterations + refinement) on the 1024x800 frame."""
+    wl, df = c1
+    _say("case solve c1 V=%d" % wl.V)
+    g, _ = _solve_gpu(lfpm, df, torch_cuda)
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    t0 = time.time()
+    o = orc.solve(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm_to_oracle(orc, wl.pm))
+    t_orc = time.time() - t0
+    dg, do = g["plane"][:, 3].astype(np.float64), o["plane"][:, 3]
+    valid = o["cost"] <= 1.8
+    err = np.abs(dg - do)
+    frac = float((err[valid] <= 0.05).mean())
+    # ground truth (synthetic scene): GT disparity = pitch / virtual depth
+    W = wl.crop
+    gt = wl.gt_virtual_depth()
+    gtv = np.stack([gt[y:y + W, x:x + W] for x, y in wl.L["origins"]]).reshape(-1)
+    gtd = wl.L["pitch"] / np.maximum(gtv, 1e-6)
+    st = dict(V=wl.V, frac_le_005=frac, valid=float(valid.mean()),
+              frac_all_le_005=float((err <= 0.05).mean()),
+              sel_equal=float((g["sel"] == o["sel"]).mean()),
+              cost_med_abs=float(np.median(np.abs(g["cost"] - o["cost"]))),
+              gt_med_err_gpu=float(np.median(np.abs(dg - gtd))),
+              gt_med_err_orc=float(np.median(np.abs(do - gtd))),
+              gt_frac05_gpu=float((np.abs(dg - gtd) < 0.5).mean()),
+              oracle_s=t_orc)
+    _say("solve_c1", st)
+    stats_dump("solve_c1", st)
+    assert frac >= 0.995
+
+
+def test_solve_deterministic(lfpm, c1, torch_cuda):
+    wl, df = c1
+    a, _ = _solve_gpu(lfpm, df, torch_cuda)
+    b, _ = _solve_gpu(lfpm, df, torch_cuda)
+    for k in a:
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+
+
+def test_solve_seed_changes_result(lfpm, c1, torch_cuda):
+    wl, df = c1
+    import copy
+    pm = copy.copy(wl.pm)
+    pm.frame_uid = 99
+    a, _ = _solve_gpu(lfpm, df, torch_cuda)
+    b, _ = _solve_gpu(lfpm, df, torch_cuda, pm=pm)
+    assert not np.array_equal(a["plane"], b["plane"])
+
+
+def test_filter_bitexact(lfpm, orc, c1, torch_cuda):
+    """Fused CheckModule pass vs the FP32 CPU twin on the GPU solve's outputs: identical reason
+    masks and outputs, bit for bit."""
+    wl, df = c1
+    g, bufs = _solve_gpu(lfpm, df, torch_cuda)
+    fo = lfpm.filter_frame(df, bufs)
+    torch_cuda.cuda.synchronize()
+    gf = {k: v.cpu().numpy() for k, v in fo.items()}
+    gray = df.gray.cpu().numpy()
+    of = orc.filter_f32(gray, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], wl.crop, wl.pm.max_neighbors,
+                        wl.pm.base, fp_to_oracle(orc, wl.fp), g["plane"], g["cost"], g["disp"],
+                        g["sel"])
+    reasons = {int(r): int((gf["reason"] == r).sum()) for r in (0, 80, 140, 200, 255)}
+    _say("filter reasons", reasons)
+    stats_dump("filter_c1", dict(reasons=reasons))
+    assert np.array_equal(gf["reason"], of["reason"])
+    for k in ("plane", "cost", "disp"):
+        assert np.array_equal(gf[k].view(np.uint32), of[k].view(np.uint32)), k
+    assert np.array_equal(gf["sel"].view(np.uint32), of["sel"])
+
+
+def test_refocus_vs_fp64(lfpm, orc, torch_cuda):
+    wl = lfpm.workload_c1(raw_w=320, raw_h=256)
+    df = lfpm.DeviceFrame(wl)
+    out = lfpm.refocus(df, slices=5, vmin=2.0, vmax=8.0)
+    torch_cuda.cuda.synchronize()
+    g = out.cpu().numpy().astype(np.float64)
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    o = orc.refocus(tiles, wl.L["centers"], wl.L["pitch"], wl.crop, 320, 256, 5, 2.0, 8.0)
+    err = np.abs(g - o)
+    _say("refocus max abs err", float(err.max()))
+    stats_dump("refocus", dict(max_abs=float(err.max())))
+    assert err.max() <= 1e-5
+
+
+def test_fit_v2r_fp64(lfpm, orc, torch_cuda):
+    rng = np.random.default_rng(3)
+    c_true = np.array([0.05, 1.7, -0.4])
+    v = (2 + 6 * rng.random(200000)).astype(np.float32)
+    aL = ((c_true[1] * v + c_true[2]) / (1 - c_true[0] * v)).astype(np.float32)
+    aL_noisy = (aL * (1 + 1e-3 * rng.standard_normal(aL.size))).astype(np.float32)
+    cg = lfpm.fit_v2r(torch_cuda.from_numpy(aL_noisy).cuda(), torch_cuda.from_numpy(v).cuda())
+    co = orc.fit_v2r(aL_noisy, v)
+    st = dict(gpu=cg.tolist(), oracle=co.tolist(), rel=float(np.max(np.abs(cg - co) / np.abs(co))))
+    _say("fit", st)
+    stats_dump("fit_v2r", st)
+    assert np.max(np.abs(cg - co) / np.abs(co)) <= 1e-8
+    z = lfpm.v2r_convert(torch_cuda.from_numpy(v).cuda(), cg).cpu().numpy()
+    zo = orc.v2r_convert(v, cg)
+    assert np.array_equal(z, zo)
+
+
+def test_ctx_api_matches_device_path(lfpm, c1, torch_cuda):
+    """lfpm_ctx_solve_frame (host in/out) gives the device path's results bit for bit."""
+    wl, df = c1
+    g, bufs = _solve_gpu(lfpm, df, torch_cuda)
+    fo = lfpm.filter_frame(df, bufs)
+    torch_cuda.cuda.synchronize()
+    L = lfpm.lib()
+    ctx = C.c_void_p()
+    assert L.lfpm_create(0, C.byref(ctx)) == 0
+    try:
+        n = wl.V * wl.crop ** 2
+        plane = np.zeros((n, 4), np.float32)
+        cost = np.zeros(n, np.float32)
+        disp = np.zeros((n, 4), np.float32)
+        sel = np.zeros(n, np.uint32)
+        reason = np.zeros(n, np.uint8)
+        raw = np.ascontiguousarray(wl.raw())
+        rc = L.lfpm_ctx_solve_frame(ctx, C.byref(wl.lp), raw.ctypes.data_as(C.c_void_p),
+                                    C.byref(wl.pm), C.byref(wl.fp),
+                                    plane.ctypes.data_as(C.c_void_p),
+                                    cost.ctypes.data_as(C.c_void_p),
+                                    disp.ctypes.data_as(C.c_void_p),
+                                    sel.ctypes.data_as(C.c_void_p),
+                                    reason.ctypes.data_as(C.c_void_p))
+        assert rc == 0, L.lfpm_last_error()
+    finally:
+        L.lfpm_destroy(ctx)
+    assert np.array_equal(reason, fo["reason"].cpu().numpy())
+    assert np.array_equal(plane.view(np.uint32), fo["plane"].cpu().numpy().view(np.uint32))
