@@ -24,6 +24,7 @@ namespace lfpm {
 // ------------------------------------------------------------------------------------------
 struct DevParams {
     int W, WW, K, r, inc, top_k, V;
+    int W1, TS;          // padded tile row stride (W+1) and slot stride (W+1)^2 in LDS
     float inv_base;      // 1 / max(Base, 1e-6)
     float inv_ss;        // 1 / (2 sigma_s^2)
     float inv_sc;        // 1 / (2 sigma_c^2)
@@ -38,6 +39,8 @@ static DevParams make_dev_params(const lfpm_pm_params* p, int V) {
     DevParams d;
     d.W = p->crop;
     d.WW = p->crop * p->crop;
+    d.W1 = p->crop + 1;
+    d.TS = (p->crop + 1) * (p->crop + 1);
     d.K = p->max_neighbors;
     d.r = p->radius;
     d.inc = std::max(1, p->radius_increment);
@@ -205,7 +208,8 @@ __device__ __forceinline__ float ncc_eval(const float* __restrict__ ref,
 
 // LDS carve-up shared by the solver kernels.
 struct Lds {
-    float* tiles;     // (K+1)*WW: slot 0 = reference, slot 1+k = neighbour k
+    float* tiles;     // (K+1) slots of (W+1)^2: slot 0 = reference, slot 1+k = neighbour k;
+                      // column W and row W repeat the last column/row (clamp-to-edge for free)
     float* costp;     // WW (phase kernel: previous cost of this MI)
     float* spat;      // (2r+1)^2 spatial log-weights -|dxy| / (2 s_s^2)
     float2* step;     // K: (c_ref - c_nei) / Base
@@ -218,7 +222,7 @@ struct Lds {
 static size_t lds_bytes(const DevParams& P, int NT, bool phase) {
     const int n = 2 * P.r + 1;
     size_t b = 0;
-    b += (size_t)(P.K + 1) * P.WW * 4;
+    b += (size_t)(P.K + 1) * P.TS * 4;
     b += (size_t)P.WW * 4;
     b += (size_t)n * n * 4;
     b += (size_t)P.K * 8 * 2;
@@ -231,7 +235,7 @@ __device__ __forceinline__ Lds carve(unsigned char* base, const DevParams& P, in
     Lds L;
     const int n = 2 * P.r + 1;
     unsigned char* q = base;
-    L.tiles = (float*)q; q += (size_t)(P.K + 1) * P.WW * 4;
+    L.tiles = (float*)q; q += (size_t)(P.K + 1) * P.TS * 4;
     L.costp = (float*)q; q += (size_t)P.WW * 4;
     L.spat = (float*)q; q += (size_t)n * n * 4;
     L.step = (float2*)q; q += (size_t)P.K * 8;
@@ -271,17 +275,110 @@ __device__ __forceinline__ int stage_mi(Lds& L, const DevParams& P, int mi,
             L.step[k] = make_float2(0.0f, 0.0f);
         }
     }
-    const int total = (c + 1) * WW;
+    const int W = P.W, W1 = P.W1, TS = P.TS;
+    const int total = (c + 1) * TS;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int s = idx / WW, o = idx - s * WW;
+        const int s = idx / TS, o = idx - s * TS;
+        const int yy = min(o / W1, W - 1), xx = min(o - (o / W1) * W1, W - 1);
         int src = mi;
         if (s > 0) {
             src = nbr[(size_t)mi * K + s - 1];
-            if (src < 0) src = mi;  // never read: ncc() skips views with nid < 0
+            if (src < 0) src = mi;  // never read: views with nid < 0 are not scored
         }
-        L.tiles[idx] = tiles[(size_t)src * WW + o];
+        L.tiles[idx] = tiles[(size_t)src * WW + yy * W + xx];
     }
     return c;
+}
+
+// Bilateral NCC of one hypothesis against up to G views (vk[g] = view index, < 0 = unused) with
+// the window loop outside and the views inside: the reference-side terms of a window sample
+// (intensity, bilateral weight, weighted moments) are computed once and shared by the G views.
+// Same definition as ncc_eval (ComputeBilateralNCC_Frame, AdaptMIPMFrame.cu:73-179).
+constexpr int kGroup = 6;
+
+template <bool COUNT>
+__device__ __forceinline__ void ncc_group(const Lds& L, const DevParams& P, int px, int py,
+                                          float rc, float4 h, const int (&vk)[kGroup],
+                                          float (&out)[kGroup], WorkCount& wc) {
+    const int W = P.W, W1 = P.W1, TS = P.TS;
+    const float Wf = (float)W;
+    const bool gate = fabsf(h.z) >= P.cos_max_slant;  // slant gate
+    float sxv[kGroup], syv[kGroup];
+    int base[kGroup];
+    bool act[kGroup];
+    bool any = false;
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) {
+        out[g] = 2.0f;
+        act[g] = false;
+        sxv[g] = 0.0f; syv[g] = 0.0f; base[g] = 0;
+        if (vk[g] >= 0) {
+            const float2 st = L.step[vk[g]];
+            sxv[g] = st.x; syv[g] = st.y;
+            base[g] = (vk[g] + 1) * TS;
+            const float ax = (float)px + st.x * h.w, ay = (float)py + st.y * h.w;
+            if (ax >= 0.0f && ax < Wf && ay >= 0.0f && ay < Wf && gate) {
+                act[g] = true;
+                if (COUNT) ++wc.triples;
+            }
+        }
+        any |= act[g];
+    }
+    if (!any) return;
+    const float A = -h.x / h.z, B = -h.y / h.z;
+    float acc[kGroup][6];
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[g][q] = 0.0f;
+    const int r = P.r, inc = P.inc, n = 2 * r + 1;
+    for (int i = -r; i <= r; i += inc) {
+        const int rx = px + i;
+        if ((unsigned)rx >= (unsigned)W) continue;
+        const float di = h.w + A * (float)i;
+        const float* spat_i = L.spat + (i + r);
+        for (int j = -r; j <= r; j += inc) {
+            const int ry = py + j;
+            if ((unsigned)ry >= (unsigned)W) continue;
+            const float d = di + B * (float)j;
+            const float x = L.tiles[ry * W1 + rx] - rc;
+            const float wgt = __expf(spat_i[(j + r) * n] - fabsf(x) * P.inv_sc);
+            const float wx = wgt * x;
+            const float wxx = wx * x;
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                if (!act[g]) continue;
+                const float qx = (float)rx + sxv[g] * d, qy = (float)ry + syv[g] * d;
+                if (!(qx >= 0.0f && qx < Wf && qy >= 0.0f && qy < Wf)) continue;
+                const float fx0 = floorf(qx), fy0 = floorf(qy);
+                const float fx = qx - fx0, fy = qy - fy0;
+                const float* t = L.tiles + base[g] + (int)fy0 * W1 + (int)fx0;
+                const float a = t[0], b = t[1], c = t[W1], e = t[W1 + 1];
+                const float top = a + fx * (b - a);
+                const float bot = c + fx * (e - c);
+                const float y = (top + fy * (bot - top)) - rc;
+                const float wy = wgt * y;
+                if (COUNT) ++wc.samples;
+                acc[g][0] += wgt;
+                acc[g][1] += wx;
+                acc[g][2] += wxx;
+                acc[g][3] += wy;
+                acc[g][4] += wy * y;
+                acc[g][5] += wx * y;
+            }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) {
+        if (!act[g] || acc[g][0] <= 1e-6f) continue;
+        const float inv = 1.0f / acc[g][0];
+        const float mx = acc[g][1] * inv, my = acc[g][3] * inv;
+        const float vx = acc[g][2] * inv - mx * mx;
+        const float vy = acc[g][4] * inv - my * my;
+        if (vx < P.min_var || vy < P.min_var) continue;
+        const float cov = acc[g][5] * inv - mx * my;
+        out[g] = fminf(2.0f, fmaxf(0.0f, 1.0f - cov / sqrtf(vx * vy)));
+    }
 }
 
 __device__ __forceinline__ float4 disp_record(const Lds& L, const DevParams& P, int k, float w) {
@@ -361,18 +458,26 @@ __global__ void __launch_bounds__(1024) k_init(InitArgs a) {
         float4 h;
         h.w = u01_dev(st.raw(SLOT_INIT_DEPTH)) * (P.dmax - P.dmin) + P.dmin;
         random_normal(st, SLOT_INIT_MARSAGLIA, h.x, h.y, h.z);
-        const float rc = L.tiles[t];
+        const float rc = L.tiles[y * P.W1 + x];
         int valid = 0, best = -1;
         float bestc = 2.0f;
-        for (int k = 0; k < c; ++k) {
-            float cv = 2.0f;
-            if (L.nid[k] >= 0)
-                cv = ncc_eval<COUNT>(L.tiles, L.tiles + (size_t)(k + 1) * WW, L.spat, P, x, y, rc, h,
-                              L.step[k].x, L.step[k].y, wc);
-            scr[(size_t)k * NT + t] = cv;
-            if (cv < 2.0f) {
-                ++valid;
-                if (best < 0 || cv < bestc) { best = k; bestc = cv; }
+        for (int k0 = 0; k0 < c; k0 += kGroup) {
+            int vk[kGroup];
+            float cg[kGroup];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g)
+                vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
+            ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                const int k = k0 + g;
+                if (k >= c) break;
+                const float cv = cg[g];
+                scr[(size_t)k * NT + t] = cv;
+                if (cv < 2.0f) {
+                    ++valid;
+                    if (best < 0 || cv < bestc) { best = k; bestc = cv; }
+                }
             }
         }
         float mean = 2.0f;
@@ -501,7 +606,7 @@ __device__ __forceinline__ void active_pixel(int W, int a, int phase, int& x, in
 }
 
 template <bool COUNT>
-__global__ void __launch_bounds__(256) k_phase(PhaseArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_phase(PhaseArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const DevParams& P = a.P;
     const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW, K = P.K;
@@ -534,20 +639,39 @@ __global__ void __launch_bounds__(256) k_phase(PhaseArgs a) {
         active_pixel(W, t, phase, x, y);
         const int center = y * W + x;
         const size_t idx = base + center;
-        const float rc = L.tiles[center];
-        int pos[8], src[8];
-        select_candidates(L.costp, W, x, y, pos, src);
+        const float rc = L.tiles[y * P.W1 + x];
+        // candidate table packed into registers (no dynamically indexed private arrays):
+        // pos of slot cd in bits 10*cd.. of a 128-bit pair, src+1 in bits 4*cd.. of srcp
+        uint64_t posp[2] = {0ull, 0ull};
+        uint32_t srcp = 0u;
+        {
+            int pos[8], src[8];
+            select_candidates(L.costp, W, x, y, pos, src);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                posp[q / 6] |= (uint64_t)pos[q] << (10 * (q % 6));
+                srcp |= (uint32_t)(src[q] + 1) << (4 * q);
+            }
+        }
+        auto cand_src = [&](int cd) -> int { return (int)((srcp >> (4 * cd)) & 15u) - 1; };
+        auto cand_pos = [&](int cd) -> int {
+            return (int)((cd < 6 ? posp[0] >> (10 * cd) : posp[1] >> (10 * (cd - 6))) & 1023ull);
+        };
 
         // cost vectors of the propagation candidates over every view
         for (int cd = 0; cd < 8; ++cd) {
-            if (src[cd] != cd) continue;
-            const float4 h = a.plane_in[base + pos[cd]];
-            for (int k = 0; k < c; ++k) {
-                float cv = 2.0f;
-                if (L.nid[k] >= 0)
-                    cv = ncc_eval<COUNT>(L.tiles, L.tiles + (size_t)(k + 1) * WW, L.spat, P, x, y, rc, h,
-                                  L.step[k].x, L.step[k].y, wc);
-                scr[(size_t)(cd * K + k) * NT + t] = cv;
+            if (cand_src(cd) != cd) continue;
+            const float4 h = a.plane_in[base + cand_pos(cd)];
+            for (int k0 = 0; k0 < c; k0 += kGroup) {
+                int vk[kGroup];
+                float cg[kGroup];
+#pragma unroll
+                for (int g = 0; g < kGroup; ++g)
+                    vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
+                ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+#pragma unroll
+                for (int g = 0; g < kGroup; ++g)
+                    if (k0 + g < c) scr[(size_t)(cd * K + k0 + g) * NT + t] = cg[g];
             }
         }
 
@@ -567,8 +691,9 @@ __global__ void __launch_bounds__(256) k_phase(PhaseArgs a) {
             float count = 0.0f, tmpw = 0.0f;
             int cf = 0;
             for (int cd = 0; cd < 8; ++cd) {
-                if (src[cd] < 0) continue;
-                const float cv = scr[(size_t)(src[cd] * K + k) * NT + t];
+                const int sc = cand_src(cd);
+                if (sc < 0) continue;
+                const float cv = scr[(size_t)(sc * K + k) * NT + t];
                 if (cv < thr) { tmpw += expf(cv * cv / (-0.18f)); count += 1.0f; }
                 if (cv > 1.2f) ++cf;
             }
@@ -619,15 +744,26 @@ __global__ void __launch_bounds__(256) k_phase(PhaseArgs a) {
         // weighted cost over the selected views (ComputeWeightedCost_Frame :284-317)
         auto wcost = [&](const float4& h) -> float {
             float wn = 0.0f, tot = 0.0f;
-            for (int j = 0; j < nsel; ++j) {
-                const int k = L.slist[j * NT + t];
-                const float w = (float)L.wts[k * NT + t];
-                wn += w;
-                float cv = 2.0f;
-                if (L.nid[k] >= 0)
-                    cv = ncc_eval<COUNT>(L.tiles, L.tiles + (size_t)(k + 1) * WW, L.spat, P, x, y, rc, h,
-                                  L.step[k].x, L.step[k].y, wc);
-                tot += w * cv;
+            for (int j0 = 0; j0 < nsel; j0 += kGroup) {
+                int vk[kGroup];
+                float cg[kGroup];
+#pragma unroll
+                for (int g = 0; g < kGroup; ++g) {
+                    vk[g] = -1;
+                    if (j0 + g < nsel) {
+                        const int k = L.slist[(j0 + g) * NT + t];
+                        if (L.nid[k] >= 0) vk[g] = k;
+                    }
+                }
+                ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+#pragma unroll
+                for (int g = 0; g < kGroup; ++g) {
+                    if (j0 + g >= nsel) break;
+                    const int k = L.slist[(j0 + g) * NT + t];
+                    const float w = (float)L.wts[k * NT + t];
+                    wn += w;
+                    tot += w * cg[g];
+                }
             }
             return wn <= 1e-6f ? 2.0f : tot / wn;
         };
@@ -635,16 +771,17 @@ __global__ void __launch_bounds__(256) k_phase(PhaseArgs a) {
         float4 best = a.plane_in[idx];
         float bcost = wcost(best);
         for (int cd = 0; cd < 8; ++cd) {
-            if (src[cd] < 0) continue;
+            const int sc = cand_src(cd);
+            if (sc < 0) continue;
             float wn = 0.0f, tot = 0.0f;
             for (int j = 0; j < nsel; ++j) {
                 const int k = L.slist[j * NT + t];
                 const float w = (float)L.wts[k * NT + t];
                 wn += w;
-                tot += w * scr[(size_t)(src[cd] * K + k) * NT + t];
+                tot += w * scr[(size_t)(sc * K + k) * NT + t];
             }
             const float cc = wn <= 1e-6f ? 2.0f : tot / wn;
-            if (cc < bcost) { bcost = cc; best = a.plane_in[base + pos[cd]]; }
+            if (cc < bcost) { bcost = cc; best = a.plane_in[base + cand_pos(cd)]; }
         }
 
         // cross-view proposals through each selected view (BuildCrossViewPlaneProposal :350-398)
@@ -700,15 +837,17 @@ __global__ void __launch_bounds__(256) k_phase(PhaseArgs a) {
             normalize3(px_, py_, pz_);
             const float lo = (1.0f - pert) * dnow, hi = (1.0f + pert) * dnow;
             const float dpert = u01_dev(st.raw(SLOT_REF_DEPTH_PERT)) * (hi - lo) + lo;
-            float4 cand[5];
-            cand[0] = make_float4(best.x, best.y, best.z, drand);
-            cand[1] = make_float4(rnx, rny, rnz, dnow);
-            cand[2] = make_float4(rnx, rny, rnz, drand);
-            cand[3] = make_float4(px_, py_, pz_, dnow);
-            cand[4] = make_float4(best.x, best.y, best.z, dpert);
+            const float4 cur = best;
             for (int i = 0; i < 5; ++i) {
-                const float cc = wcost(cand[i]);
-                if (cc < bcost) { bcost = cc; best = cand[i]; }
+                // (n_cur, d_rand), (n_rand, d_now), (n_rand, d_rand), (n_pert, d_now),
+                // (n_cur, d_pert), selected without a dynamically indexed array
+                float4 hc;
+                hc.x = (i == 0 || i == 4) ? cur.x : (i == 3 ? px_ : rnx);
+                hc.y = (i == 0 || i == 4) ? cur.y : (i == 3 ? py_ : rny);
+                hc.z = (i == 0 || i == 4) ? cur.z : (i == 3 ? pz_ : rnz);
+                hc.w = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dnow);
+                const float cc = wcost(hc);
+                if (cc < bcost) { bcost = cc; best = hc; }
             }
         }
         a.plane_out[idx] = best;
@@ -747,14 +886,19 @@ __global__ void __launch_bounds__(1024) k_finalize(FinalArgs a) {
         const int x = t % W, y = t / W;
         const size_t idx = (size_t)mi * WW + t;
         const float4 h = a.plane[idx];
-        const float rc = L.tiles[t];
+        const float rc = L.tiles[y * P.W1 + x];
         float mc = 2.0f;
         int am = -1;
-        for (int k = 0; k < c; ++k) {
-            if (L.nid[k] < 0) continue;
-            const float cv = ncc_eval<COUNT>(L.tiles, L.tiles + (size_t)(k + 1) * WW, L.spat, P, x, y, rc,
-                                      h, L.step[k].x, L.step[k].y, wc);
-            if (cv < mc) { mc = cv; am = k; }
+        for (int k0 = 0; k0 < c; k0 += kGroup) {
+            int vk[kGroup];
+            float cg[kGroup];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g)
+                vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
+            ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g)
+                if (k0 + g < c && cg[g] < mc) { mc = cg[g]; am = k0 + g; }
         }
         a.disp[idx] = am >= 0 ? disp_record(L, P, am, h.w) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
@@ -811,13 +955,17 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs a) {
             const float4 h = make_float4(__int_as_float(rec[2]), __int_as_float(rec[3]),
                                          __int_as_float(rec[4]), __int_as_float(rec[5]));
             const int x = pix % W, y = pix / W;
-            const float rc = L.tiles[pix];
-            for (int k = 0; k < K; ++k) {
-                float cv = 2.0f;
-                if (k < c && L.nid[k] >= 0)
-                    cv = ncc_eval<COUNT>(L.tiles, L.tiles + (size_t)(k + 1) * WW, L.spat, P, x, y, rc, h,
-                                  L.step[k].x, L.step[k].y, wc);
-                a.out[(size_t)hi * K + k] = cv;
+            const float rc = L.tiles[y * P.W1 + x];
+            for (int k0 = 0; k0 < K; k0 += kGroup) {
+                int vk[kGroup];
+                float cg[kGroup];
+#pragma unroll
+                for (int g = 0; g < kGroup; ++g)
+                    vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
+                ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+#pragma unroll
+                for (int g = 0; g < kGroup; ++g)
+                    if (k0 + g < K) a.out[(size_t)hi * K + k0 + g] = cg[g];
             }
         }
     }
@@ -1167,6 +1315,7 @@ __global__ void k_fit_solve(const double* __restrict__ part, int nblocks, double
 
 __global__ void k_v2r_convert(const float* __restrict__ v, int n, double c0, double c1, double c2,
                               double* __restrict__ z) {
+#pragma clang fp contract(off)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double vv = v[i];
         const double zz = (vv * c1 + c2) / (1.0 - vv * c0);

@@ -66,9 +66,11 @@ def test_eval_costs_vs_fp64(lfpm, orc, torch_cuda, cfg):
               frac_le_1e4=float((e <= 1e-4).mean()))
     _say("eval_costs", st)
     stats_dump("eval_costs_" + cfg, st)
+    # FP32 vs FP64: >= 99.5 % of scored views within 1e-5 relative, >= 99.9 % within 1e-4; the
+    # tail are windows whose edge samples fall on opposite sides of the crop boundary
     assert flip < 1e-3
-    assert st["median"] <= 1e-5
-    assert st["p99"] <= 1e-3
+    assert st["frac_le_1e5"] >= 0.995
+    assert st["frac_le_1e4"] >= 0.999
 
 
 def _solve_gpu(lfpm, df, torch_cuda, pm=None):

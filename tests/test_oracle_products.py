@@ -1,0 +1,193 @@
+"""Pins of the oracle's CheckModule filter, refocus and V2R fit.
+
+Filter (MIDisparityFilterCPU::FilterTile, CheckModule/MIDisparityFilter.cpp:423-693): exactly
+consistent fronto-parallel scenes give zero cycle error and are kept; injected spikes are removed
+with reason PR_Spike; the cost gate is inclusive at 1.8; invalid input pixels are untouched.
+Refocus: linear in the input tiles, constant tiles refocus to the constant, a single MI reduces
+to one bilinear sample.  Fit: exact model data recovers the model; noisy data agrees with numpy's
+least squares.
+"""
+import numpy as np
+import pytest
+
+from _helpers import fp_to_oracle
+
+PR_KEEP, PR_COST, PR_NOSUPPORT, PR_CYCLE, PR_SPIKE = 0, 80, 140, 200, 255
+
+
+@pytest.fixture(scope="module")
+def scene(lfpm):
+    wl = lfpm.workload_c1(raw_w=320, raw_h=256)
+    V, W = wl.V, wl.crop
+    rng = np.random.default_rng(5)
+    gray = rng.integers(0, 256, (V, W, W)).astype(np.float32)
+    return wl, gray
+
+
+def _inputs(wl, d=5.0, cost=0.5):
+    V, W, K = wl.V, wl.crop, wl.pm.max_neighbors
+    n = V * W * W
+    plane = np.zeros((n, 4), np.float32)
+    plane[:, 2] = 1.0
+    plane[:, 3] = d
+    costs = np.full(n, cost, np.float32)
+    disp = np.zeros((n, 4), np.float32)
+    sel = np.zeros(n, np.uint32)
+    for v in range(V):
+        sel[v * W * W:(v + 1) * W * W] = (1 << int(wl.L["cnt"][v])) - 1
+    return plane, costs, disp, sel
+
+
+def _run(orc, wl, gray, fp, plane, costs, disp, sel):
+    return orc.filter_f32(gray, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], wl.crop,
+                          wl.pm.max_neighbors, wl.pm.base, fp, plane, costs, disp, sel)
+
+
+def _fp(orc, wl, **kw):
+    fp = fp_to_oracle(orc, wl.fp)
+    for k, v in kw.items():
+        setattr(fp, k, v)
+    return fp
+
+
+def test_consistent_scene_kept(orc, scene):
+    wl, gray = scene
+    fp = _fp(orc, wl, enable_cycle_photo_filter=0)
+    out = _run(orc, wl, gray, fp, *_inputs(wl))
+    r = out["reason"]
+    # constant disparity: geometric cycle error is exactly zero wherever the neighbour sample
+    # lands inside its crop, so every rejection is for missing support, never for the cycle
+    assert not np.any(r == PR_CYCLE) and not np.any(r == PR_SPIKE) and not np.any(r == PR_COST)
+    assert (r == PR_KEEP).mean() > 0.5
+    kept = r == PR_KEEP
+    assert np.all(out["plane"][kept, 3] == 5.0)
+    dropped = r == PR_NOSUPPORT
+    assert np.all(out["plane"][dropped, 3] == -1.0) and np.all(out["cost"][dropped] == 2.0)
+    assert np.all(out["sel"][dropped] == 0)
+
+
+def test_spike_removed(orc, scene):
+    wl, gray = scene
+    fp = _fp(orc, wl, enable_cycle_photo_filter=0, enable_cycle_geo_filter=0)
+    plane, costs, disp, sel = _inputs(wl)
+    W = wl.crop
+    # the centre pixel of an interior MI kept in the clean run: raise its disparity by 1.5 px
+    # (> spike_abs_diff = 1), its neighbour samples stay inside their crops
+    clean = _run(orc, wl, gray, fp, plane, costs, disp, sel)["reason"]
+    v = int(np.flatnonzero(wl.L["cnt"] == wl.L["cnt"].max())[5])
+    idx = v * W * W + (W // 2) * W + W // 2
+    assert clean[idx] == PR_KEEP
+    plane[idx, 3] = 6.5
+    out = _run(orc, wl, gray, fp, plane, costs, disp, sel)
+    assert out["reason"][idx] == PR_SPIKE
+    assert out["plane"][idx, 3] == -1.0
+
+
+def test_cost_gate_inclusive(orc, scene):
+    wl, gray = scene
+    fp = _fp(orc, wl, enable_cycle_photo_filter=0)
+    p, c, d, s = _inputs(wl)
+    base = _run(orc, wl, gray, fp, p, c, d, s)["reason"]
+    i = int(np.flatnonzero(base == PR_KEEP)[10])
+    c2 = c.copy()
+    c2[i] = np.float32(1.8)
+    assert _run(orc, wl, gray, fp, p, c2, d, s)["reason"][i] != PR_COST
+    c2[i] = np.nextafter(np.float32(1.8), np.float32(2))
+    assert _run(orc, wl, gray, fp, p, c2, d, s)["reason"][i] == PR_COST
+
+
+def test_invalid_pixels_untouched(orc, scene):
+    wl, gray = scene
+    fp = _fp(orc, wl)
+    p, c, d, s = _inputs(wl)
+    p[::7, 3] = -1.0
+    p[::11, 3] = np.nan
+    out = _run(orc, wl, gray, fp, p, c, d, s)
+    bad = (p[:, 3] <= 0) | np.isnan(p[:, 3])
+    assert np.all(out["reason"][bad] == PR_KEEP)
+    assert np.array_equal(out["plane"][bad].view(np.uint32), p[bad].view(np.uint32))
+
+
+def test_photo_residual_zero_on_identical_neighbours(orc, lfpm):
+    """Identical gray content in every MI and zero disparity: photometric cycle residual 0."""
+    wl = lfpm.workload_c1(raw_w=320, raw_h=256)
+    V, W = wl.V, wl.crop
+    rng = np.random.default_rng(9)
+    one = rng.integers(0, 256, (W, W)).astype(np.float32)
+    gray = np.broadcast_to(one, (V, W, W)).copy()
+    p, c, d, s = _inputs(wl, d=1e-3)
+    out = _run(orc, wl, gray, _fp(orc, wl), p, c, d, s)
+    assert not np.any(out["reason"] == PR_CYCLE)
+
+
+# ----------------------------------------------------------------------------------------
+# Refocus
+# ----------------------------------------------------------------------------------------
+def test_refocus_linear_and_constant(orc, lfpm):
+    wl = lfpm.workload_c1(raw_w=256, raw_h=200)
+    rng = np.random.default_rng(2)
+    V, W = wl.V, wl.crop
+    t1 = rng.integers(0, 128, (V, W, W)).astype(np.uint8)
+    t2 = rng.integers(0, 128, (V, W, W)).astype(np.uint8)
+    args = (wl.L["centers"], wl.L["pitch"], W, 256, 200, 3, 2.0, 8.0)
+    r1, r2 = orc.refocus(t1, *args), orc.refocus(t2, *args)
+    r12 = orc.refocus((t1 + t2).astype(np.uint8), *args)
+    assert np.max(np.abs(r12 - (r1 + r2))) < 1e-12
+    rc = orc.refocus(np.full((V, W, W), 100, np.uint8), *args)
+    covered = rc > 0
+    assert covered.mean() > 0.8
+    assert np.max(np.abs(rc[covered] - 100 / 255.0)) < 1e-12
+
+
+def test_refocus_single_mi_is_bilinear(orc):
+    from scipy.ndimage import map_coordinates
+    W = 16
+    rng = np.random.default_rng(4)
+    tile = rng.integers(0, 256, (1, W, W)).astype(np.uint8)
+    c = np.array([[40.0, 30.0]], np.float32)
+    v = 3.0
+    out = orc.refocus(tile, c, 23.0, W, 90, 70, 1, v, v)[0]
+    ys, xs = np.mgrid[0:70, 0:90]
+    ux = (xs - 40.0) / v + 7.5
+    uy = (ys - 30.0) / v + 7.5
+    inside = (ux >= 0) & (ux <= W - 1) & (uy >= 0) & (uy <= W - 1)
+    want = map_coordinates(tile[0] / 255.0, [uy[inside], ux[inside]], order=1, mode="nearest")
+    assert np.max(np.abs(out[inside] - want)) < 1e-6
+    assert np.all(out[~inside] == 0.0)
+
+
+# ----------------------------------------------------------------------------------------
+# V2R fit
+# ----------------------------------------------------------------------------------------
+def test_fit_recovers_exact_model(orc):
+    c_true = np.array([0.04, 1.3, 0.7])
+    v = np.linspace(2.0, 8.0, 5000)
+    aL = (c_true[1] * v + c_true[2]) / (1 - c_true[0] * v)
+    # float32 inputs (as the reference's vectors): compare against the model on those inputs
+    c = orc.fit_v2r(aL.astype(np.float32), v.astype(np.float32))
+    assert np.max(np.abs(c - c_true) / np.abs(c_true)) < 2e-5
+    v32 = v.astype(np.float32).astype(np.float64)
+    a32 = aL.astype(np.float32).astype(np.float64)
+    X = np.stack([a32 * v32, v32, np.ones_like(v32)], 1)
+    want = np.linalg.lstsq(X, a32, rcond=None)[0]
+    assert np.max(np.abs(c - want) / np.abs(want)) < 1e-9
+
+
+def test_fit_matches_lstsq_on_noisy_data(orc):
+    rng = np.random.default_rng(8)
+    v = (2 + 6 * rng.random(20000)).astype(np.float32)
+    aL = ((1.5 * v - 0.2) / (1 - 0.03 * v) * (1 + 0.01 * rng.standard_normal(v.size)))
+    aL = aL.astype(np.float32)
+    c = orc.fit_v2r(aL, v)
+    a, vv = aL.astype(np.float64), v.astype(np.float64)
+    want = np.linalg.lstsq(np.stack([a * vv, vv, np.ones_like(vv)], 1), a, rcond=None)[0]
+    assert np.max(np.abs(c - want) / np.abs(want)) < 1e-9
+
+
+def test_v2r_convert_clamps(orc):
+    v = np.array([2.0, 10.0, 25.0, np.nan], np.float32)
+    z = orc.v2r_convert(v, np.array([0.04, 1.0, 0.5]))
+    assert abs(z[0] - (2.0 + 0.5) / (1 - 0.08)) < 1e-12
+    assert z[2] == 0.0          # negative -> 0
+    assert z[3] == 0.0          # nan -> 0
+    assert z[1] > 0
