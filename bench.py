@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the work-counting pass")
     ap.add_argument("--raw", default="4080x3068")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the steps, run one 1 GiB device-to-device copy (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -169,6 +172,22 @@ def count_work(lfpm, df, bufs, fo, rf, torch):
     return out
 
 
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (scaled by the copy-kernel calibration), or None if absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_plain_4080x3068.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0].startswith("lfpm::" + kernel):
+            try:
+                return float(cells[4])
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(lfpm):
     """FP32 CPU restatement (oracle, OpenMP) of the same solve on a bounded sample: the same
     parameters on a 400x320 sub-frame, scaled to a 4080x3068 frame by MI count."""
@@ -227,6 +246,13 @@ def main():
         log("bench: step %d %.1f ms" % (i, lat[-1]))
     torch.cuda.synchronize()
     t_all = time.perf_counter() - t_all0
+    if args.calib_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     fps = args.steps / t_all
     kt = {k: float(np.mean(timer.ms(k))) for k in ("init", "phase", "finalize", "filter",
                                                     "refocus")}
@@ -241,7 +267,7 @@ def main():
         achieved = flop_per_launch / (kt["phase"] * 1e-3) / 1e12
         roof = {"bound": "valu", "kernel": "k_phase", "achieved": achieved,
                 "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / (PEAK_FP32 / 1e12),
-                "traffic": None, "avg_launch_ms": kt["phase"],
+                "traffic": measured_traffic("k_phase"), "avg_launch_ms": kt["phase"],
                 "flop_per_launch": flop_per_launch}
     cpu = None
     if not args.no_cpu_baseline:

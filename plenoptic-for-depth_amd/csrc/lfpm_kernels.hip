@@ -324,14 +324,16 @@ __device__ __forceinline__ void ncc_group(const Lds& L, const DevParams& P, int 
         }
         any |= act[g];
     }
-    if (!any) return;
     const float A = -h.x / h.z, B = -h.y / h.z;
+    // non-finite slopes make every window sample invalid (the result is 2 for every view)
+    if (!any || !(isfinite(A) && isfinite(B))) return;
     float acc[kGroup][6];
 #pragma unroll
     for (int g = 0; g < kGroup; ++g)
 #pragma unroll
         for (int q = 0; q < 6; ++q) acc[g][q] = 0.0f;
     const int r = P.r, inc = P.inc, n = 2 * r + 1;
+    const float W1f = (float)W1, Wm1 = (float)(W - 1);
     for (int i = -r; i <= r; i += inc) {
         const int rx = px + i;
         if ((unsigned)rx >= (unsigned)W) continue;
@@ -345,26 +347,31 @@ __device__ __forceinline__ void ncc_group(const Lds& L, const DevParams& P, int 
             const float wgt = __expf(spat_i[(j + r) * n] - fabsf(x) * P.inv_sc);
             const float wx = wgt * x;
             const float wxx = wx * x;
+            // branchless view block: invalid samples get weight 0 and read a clamped in-crop
+            // address, so the six views' LDS reads can be in flight together
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
-                if (!act[g]) continue;
                 const float qx = (float)rx + sxv[g] * d, qy = (float)ry + syv[g] * d;
-                if (!(qx >= 0.0f && qx < Wf && qy >= 0.0f && qy < Wf)) continue;
+                const bool ok = act[g] && qx >= 0.0f && qx < Wf && qy >= 0.0f && qy < Wf;
                 const float fx0 = floorf(qx), fy0 = floorf(qy);
                 const float fx = qx - fx0, fy = qy - fy0;
-                const float* t = L.tiles + base[g] + (int)fy0 * W1 + (int)fx0;
+                const float xa = __builtin_amdgcn_fmed3f(fx0, 0.0f, Wm1);
+                const float ya = __builtin_amdgcn_fmed3f(fy0, 0.0f, Wm1);
+                const float* t = L.tiles + base[g] + (int)(ya * W1f + xa);
                 const float a = t[0], b = t[1], c = t[W1], e = t[W1 + 1];
                 const float top = a + fx * (b - a);
                 const float bot = c + fx * (e - c);
                 const float y = (top + fy * (bot - top)) - rc;
-                const float wy = wgt * y;
-                if (COUNT) ++wc.samples;
-                acc[g][0] += wgt;
-                acc[g][1] += wx;
-                acc[g][2] += wxx;
+                const float m = ok ? 1.0f : 0.0f;
+                const float wm = wgt * m;
+                const float wy = wm * y;
+                if (COUNT && ok) ++wc.samples;
+                acc[g][0] += wm;
+                acc[g][1] += wx * m;
+                acc[g][2] += wxx * m;
                 acc[g][3] += wy;
                 acc[g][4] += wy * y;
-                acc[g][5] += wx * y;
+                acc[g][5] += wy * x;
             }
         }
     }
