@@ -252,6 +252,69 @@ int lfpm_refocus(const float* tiles_dev, const float* centers_dev, const int* la
 int lfpm_fit_v2r(const float* a_l_dev, const float* v_dev, int n, double c_out[3], void* stream);
 int lfpm_v2r_convert(const float* v_dev, int n, const double c[3], double* z_dev, void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Batched and served forms (no counterpart in the reference, which solves one frame per
+ * device reset).  Frames are grouped into parameter classes (equal lfpm_pm_params apart from
+ * seed/frame_uid); each class runs as persistent k_stage launches over the concatenated MIs of
+ * its frames.  Every frame's results equal its single-frame solve bit for bit (the random
+ * stream is keyed by (seed, frame_uid), never by batch position).
+ * ------------------------------------------------------------------------------------- */
+#define LFPM_MAX_BATCH 64
+
+#define LFPM_STAGE_IDLE 0
+#define LFPM_STAGE_INIT 1
+#define LFPM_STAGE_PHASE 2
+#define LFPM_STAGE_FINAL 3    /* disparity record, then filter and refocus when requested */
+
+typedef struct lfpm_batch_frame {
+    /* inputs (device) */
+    const float* tiles_dev;           /* V*crop^2 in [0,1] */
+    const float* gray_dev;            /* V*crop^2 in 0..255 (filter input), may be NULL */
+    const float* centers_dev;         /* 2V */
+    const int* neighbor_ids_dev;      /* V*K */
+    const int* neighbor_counts_dev;   /* V */
+    const int* lattice_to_mi_dev;     /* cols*rows (refocus), may be NULL */
+    int num_mis, lattice_cols, lattice_rows, raw_w, raw_h;
+    const lfpm_pm_params* params;     /* solve parameters incl. seed and frame_uid */
+    const lfpm_filter_params* filter; /* NULL: no filter pass */
+    /* state: A ends with the result, B is the red/black ping-pong partner (device) */
+    float* plane_dev; float* cost_dev; uint32_t* sel_dev; float* disp_dev;
+    float* plane_b_dev; float* cost_b_dev; uint32_t* sel_b_dev;
+    /* filter outputs (device), used when filter != NULL */
+    float* f_plane_dev; float* f_cost_dev; float* f_disp_dev; uint32_t* f_sel_dev;
+    uint8_t* reason_dev;
+    /* focal stack (device), used when refocus_slices > 0 */
+    int refocus_slices;
+    float refocus_vmin, refocus_vmax;
+    float* refocus_dev;               /* slices*raw_h*raw_w */
+} lfpm_batch_frame;
+
+/* Workspace for lfpm_batch_run / lfpm_batch_graph_create / lfpm_batch_step on these frames. */
+size_t lfpm_batch_workspace_size(const lfpm_batch_frame* frames, int n);
+
+/* Whole batch, stream-ordered: init, iterations x 2 phases, final record (+ filter, refocus). */
+int lfpm_batch_run(const lfpm_batch_frame* frames, int n, void* workspace_dev,
+                   size_t workspace_bytes, void* stream);
+
+/* The same schedule captured once as a hipGraph: each parameter class on its own branch, so
+ * classes run concurrently; replay with lfpm_graph_launch.  Frame pointers are baked in. */
+typedef struct lfpm_graph lfpm_graph;
+int lfpm_batch_graph_create(const lfpm_batch_frame* frames, int n, void* workspace_dev,
+                            size_t workspace_bytes, lfpm_graph** out);
+int lfpm_graph_launch(lfpm_graph* g, void* stream);
+void lfpm_graph_destroy(lfpm_graph* g);
+
+/* One continuous-batching step: every frame advances by its own stage (stages[i], iters[i],
+ * phases[i]); LFPM_STAGE_PHASE reads A and writes B for phase 0, B to A for phase 1. */
+int lfpm_batch_step(const lfpm_batch_frame* frames, int n, const int* stages, const int* iters,
+                    const int* phases, void* workspace_dev, size_t workspace_bytes, void* stream);
+
+/* Autotuner: time k_stage phase launches of this frame's parameter class at each persistent
+ * occupancy in {2,4,6,8} workgroups per CU, keep the fastest for the class (process-wide), and
+ * return it.  Results do not depend on the choice.  Uses and overwrites B state of `frame`. */
+int lfpm_autotune_class(const lfpm_batch_frame* frame, void* workspace_dev,
+                        size_t workspace_bytes, int* best_per_cu, float* best_ms);
+
 /* Work accounting for roofline reports (no reference counterpart).  While counters_dev (two
  * device uint64: evaluated NCC window samples, evaluated (pixel, view, hypothesis) triples) is
  * set, solver launches from the calling thread use counting kernel variants that accumulate into

@@ -6,4 +6,26 @@ namespace lfpm {
 int set_error(int code, const char* fmt, ...);
 const char* last_error();
 int resolve_crop(const lfpm_layout_params* p);
+
+// One frame of a k_stage launch (host description; device pointers).
+struct StageFrame {
+    const float* tiles; const float* centers; const int* nbr; const int* cnt;
+    int V;
+    int stage, iter, phase;  // STAGE_* below
+    uint32_t seed, uid;
+    const float* plane_in; const float* cost_in; const uint32_t* sel_in;
+    float* plane_out; float* cost_out; uint32_t* sel_out;
+    float* disp;
+};
+enum : int { kStageIdle = 0, kStageInit = 1, kStagePhase = 2, kStageFinal = 3 };
+
+// Scratch bytes one k_stage launch over `total_mis` MIs of parameter class `p` needs.
+size_t stage_scratch_size(const lfpm_pm_params* p, int total_mis);
+// Launch k_stage for `n` frames of one parameter class (chunks of 16 frames per launch).
+int stage_launch(const lfpm_pm_params* p, const StageFrame* frames, int n, void* scratch,
+                 size_t scratch_bytes, void* stream);
+// Workgroups per CU used by k_stage for a parameter class (tuned value or the default), and
+// the setter the autotuner uses.
+int stage_slots_per_cu(const lfpm_pm_params* p);
+void set_stage_slots_per_cu(const lfpm_pm_params* p, int per_cu);
 }  // namespace lfpm

@@ -433,101 +433,32 @@ __global__ void k_slice(const uint8_t* __restrict__ raw, size_t pitch, const int
 }
 
 // ------------------------------------------------------------------------------------------
-// Random initialisation (RandomInitializationForFrame, AdaptMIPMFrame.cu:474-518)
+// Frame table.  One persistent launch serves up to kMaxFrames frames of one parameter class
+// (crop, neighbours, window), each at its own stage: random init (RandomInitializationForFrame,
+// AdaptMIPMFrame.cu:474-518), one red/black phase (CheckerboardUpdateFrame, :520-947) or the final
+// disparity record.  The frames' MIs form one index space; a single-frame table is the plain
+// per-frame path (the reference's launch-per-phase pattern, :949-995), a multi-frame table the
+// batched and served forms.  Descriptors travel by value in the kernel arguments, so a captured
+// hipGraph replays them without memory nodes.
 // ------------------------------------------------------------------------------------------
-struct InitArgs {
+enum : int { STAGE_IDLE = 0, STAGE_INIT = 1, STAGE_PHASE = 2, STAGE_FINAL = 3 };
+constexpr int kMaxFrames = 16;
+
+struct FrameDesc {
     const float* tiles; const float2* centers; const int* nbr; const int* cnt;
-    float4* plane; float* cost; uint32_t* sel;
-    float* scratch;   // [grid][K][NT]
-    unsigned long long* counters;  // counting variant: {samples, triples}
-    DevParams P;
+    const float4* plane_in; const float* cost_in; const uint32_t* sel_in;   // phase / final input
+    float4* plane_out; float* cost_out; uint32_t* sel_out;                  // init / phase output
+    float4* disp;                                                           // final output
+    int V, mi_begin, stage, iter, phase;
+    uint32_t seed, uid;
 };
 
-template <bool COUNT>
-__global__ void __launch_bounds__(1024) k_init(InitArgs a) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const DevParams& P = a.P;
-    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW, K = P.K;
-    Lds L = carve(lds_raw, P, NT);
-    fill_spatial(L, P);
-    float* scr = a.scratch + (size_t)blockIdx.x * K * NT;
-    int first, end, stride;
-    mi_range(P, first, end, stride);
-    WorkCount wc{0u, 0u};
-    for (int mi = first; mi < end; mi += stride) {
-        __syncthreads();
-        const int c = stage_mi(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
-        __syncthreads();
-        if (t >= WW) continue;
-        const int x = t % W, y = t / W;
-        const size_t idx = (size_t)mi * WW + t;
-        DevStream st((uint32_t)idx, 0u, P.seed, P.uid);
-        float4 h;
-        h.w = u01_dev(st.raw(SLOT_INIT_DEPTH)) * (P.dmax - P.dmin) + P.dmin;
-        random_normal(st, SLOT_INIT_MARSAGLIA, h.x, h.y, h.z);
-        const float rc = L.tiles[y * P.W1 + x];
-        int valid = 0, best = -1;
-        float bestc = 2.0f;
-        for (int k0 = 0; k0 < c; k0 += kGroup) {
-            int vk[kGroup];
-            float cg[kGroup];
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g)
-                vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
-            ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g) {
-                const int k = k0 + g;
-                if (k >= c) break;
-                const float cv = cg[g];
-                scr[(size_t)k * NT + t] = cv;
-                if (cv < 2.0f) {
-                    ++valid;
-                    if (best < 0 || cv < bestc) { best = k; bestc = cv; }
-                }
-            }
-        }
-        float mean = 2.0f;
-        uint32_t mask = 0u;
-        if (valid > 0) {
-            // k smallest costs in ascending order (ComputeSelectedViewsTopK_Frame :227-282)
-            const int kk = min(valid, P.top_k);
-            uint32_t taken = 0u;
-            float sum = 0.0f, thr = 2.0f;
-            for (int m = 0; m < kk; ++m) {
-                int bi = -1;
-                float bv = 0.0f;
-                for (int k = 0; k < c; ++k) {
-                    if ((taken >> k) & 1u) continue;
-                    const float cv = scr[(size_t)k * NT + t];
-                    if (bi < 0 || cv < bv) { bi = k; bv = cv; }
-                }
-                taken |= 1u << bi;
-                sum += bv;
-                thr = bv;
-            }
-            mean = sum / (float)kk;
-            for (int k = 0; k < c; ++k)
-                if (scr[(size_t)k * NT + t] <= thr) mask |= 1u << k;
-        }
-        a.plane[idx] = h;
-        a.cost[idx] = mean;
-        a.sel[idx] = mask;
-    }
-    flush_counts<COUNT>(a.counters, wc);
-}
-
-// ------------------------------------------------------------------------------------------
-// One red/black phase (CheckerboardUpdateFrame, AdaptMIPMFrame.cu:520-947)
-// ------------------------------------------------------------------------------------------
-struct PhaseArgs {
-    const float* tiles; const float2* centers; const int* nbr; const int* cnt;
-    const float4* plane_in; const float* cost_in; const uint32_t* sel_in;
-    float4* plane_out; float* cost_out; uint32_t* sel_out;
+struct StageArgs {
+    FrameDesc f[kMaxFrames];
+    int nf, total;
     float* scratch;   // [grid][9*K][NT]: 8 candidate cost vectors + sampling probabilities
-    unsigned long long* counters;
+    unsigned long long* counters;  // counting variant: {samples, triples}
     DevParams P;
-    int iter, phase;
 };
 
 // Propagation candidate positions (index into the MI, row-major) — the 8 slots of the
@@ -612,287 +543,80 @@ __device__ __forceinline__ void active_pixel(int W, int a, int phase, int& x, in
     }
 }
 
+
+// Random initialisation of every pixel of MI v (RandomInitializationForFrame :474-518).
 template <bool COUNT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_phase(PhaseArgs a) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const DevParams& P = a.P;
-    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW, K = P.K;
-    Lds L = carve(lds_raw, P, NT);
-    fill_spatial(L, P);
-    float* scr = a.scratch + (size_t)blockIdx.x * 9 * K * NT;
-    int first, end, stride;
-    mi_range(P, first, end, stride);
-    WorkCount wc{0u, 0u};
-    const int phase = a.phase;
-    const int nact = (W & 1) ? (phase == 0 ? (WW + 1) / 2 : WW / 2) : WW / 2;
-    for (int mi = first; mi < end; mi += stride) {
-        __syncthreads();
-        const int c = stage_mi(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
-        for (int i = t; i < WW; i += NT) L.costp[i] = a.cost_in[(size_t)mi * WW + i];
-        __syncthreads();
-        const size_t base = (size_t)mi * WW;
-        // pass-through of the inactive colour (and of every pixel of an MI without neighbours)
-        for (int i = t; i < WW; i += NT) {
-            const int x = i % W, y = i / W;
-            if (((x + y) & 1) != phase || c <= 0) {
-                a.plane_out[base + i] = a.plane_in[base + i];
-                a.cost_out[base + i] = a.cost_in[base + i];
-                a.sel_out[base + i] = a.sel_in[base + i];
-            }
-        }
-        if (c <= 0 || t >= nact) continue;
-
-        int x, y;
-        active_pixel(W, t, phase, x, y);
-        const int center = y * W + x;
-        const size_t idx = base + center;
+__device__ __forceinline__ void init_mi(const Lds& L, const DevParams& P, const FrameDesc& F,
+                                        int v, int c, float* __restrict__ scr, WorkCount& wc) {
+    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW;
+    for (int pix = t; pix < WW; pix += NT) {
+        const int x = pix % W, y = pix / W;
+        const size_t idx = (size_t)v * WW + pix;
+        DevStream st((uint32_t)idx, 0u, F.seed, F.uid);
+        float4 h;
+        h.w = u01_dev(st.raw(SLOT_INIT_DEPTH)) * (P.dmax - P.dmin) + P.dmin;
+        random_normal(st, SLOT_INIT_MARSAGLIA, h.x, h.y, h.z);
         const float rc = L.tiles[y * P.W1 + x];
-        // candidate table packed into registers (no dynamically indexed private arrays):
-        // pos of slot cd in bits 10*cd.. of a 128-bit pair, src+1 in bits 4*cd.. of srcp
-        uint64_t posp[2] = {0ull, 0ull};
-        uint32_t srcp = 0u;
-        {
-            int pos[8], src[8];
-            select_candidates(L.costp, W, x, y, pos, src);
+        int valid = 0, best = -1;
+        float bestc = 2.0f;
+        for (int k0 = 0; k0 < c; k0 += kGroup) {
+            int vk[kGroup];
+            float cg[kGroup];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                posp[q / 6] |= (uint64_t)pos[q] << (10 * (q % 6));
-                srcp |= (uint32_t)(src[q] + 1) << (4 * q);
+            for (int g = 0; g < kGroup; ++g)
+                vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
+            ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                const int k = k0 + g;
+                if (k >= c) break;
+                const float cv = cg[g];
+                scr[(size_t)k * NT + t] = cv;
+                if (cv < 2.0f) {
+                    ++valid;
+                    if (best < 0 || cv < bestc) { best = k; bestc = cv; }
+                }
             }
         }
-        auto cand_src = [&](int cd) -> int { return (int)((srcp >> (4 * cd)) & 15u) - 1; };
-        auto cand_pos = [&](int cd) -> int {
-            return (int)((cd < 6 ? posp[0] >> (10 * cd) : posp[1] >> (10 * (cd - 6))) & 1023ull);
-        };
-
-        // cost vectors of the propagation candidates over every view
-        for (int cd = 0; cd < 8; ++cd) {
-            if (cand_src(cd) != cd) continue;
-            const float4 h = a.plane_in[base + cand_pos(cd)];
-            for (int k0 = 0; k0 < c; k0 += kGroup) {
-                int vk[kGroup];
-                float cg[kGroup];
-#pragma unroll
-                for (int g = 0; g < kGroup; ++g)
-                    vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
-                ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
-#pragma unroll
-                for (int g = 0; g < kGroup; ++g)
-                    if (k0 + g < c) scr[(size_t)(cd * K + k0 + g) * NT + t] = cg[g];
-            }
-        }
-
-        // view-selection priors from the 4 direct neighbours' previous selections
-        uint32_t dsel[4];
-        bool dval[4];
-        dval[0] = y > 0; dval[1] = y < W - 1; dval[2] = x > 0; dval[3] = x < W - 1;
-        dsel[0] = dval[0] ? a.sel_in[idx - W] : 0u;
-        dsel[1] = dval[1] ? a.sel_in[idx + W] : 0u;
-        dsel[2] = dval[2] ? a.sel_in[idx - 1] : 0u;
-        dsel[3] = dval[3] ? a.sel_in[idx + 1] : 0u;
-
-        // sampling probabilities
-        const float thr = 0.8f * expf((float)(a.iter * a.iter) / (-90.0f));
-        float psum = 0.0f;
-        for (int k = 0; k < c; ++k) {
-            float count = 0.0f, tmpw = 0.0f;
-            int cf = 0;
-            for (int cd = 0; cd < 8; ++cd) {
-                const int sc = cand_src(cd);
-                if (sc < 0) continue;
-                const float cv = scr[(size_t)(sc * K + k) * NT + t];
-                if (cv < thr) { tmpw += expf(cv * cv / (-0.18f)); count += 1.0f; }
-                if (cv > 1.2f) ++cf;
-            }
-            float pr = 0.0f;
-            if (count > 2.0f && cf < 3) pr = tmpw / count;
-            else if (cf < 3) pr = expf(thr * thr / (-0.32f));
-            float prior = 0.0f;
-            for (int ii = 0; ii < 4; ++ii)
-                if (dval[ii]) prior += ((dsel[ii] >> k) & 1u) ? 0.9f : 0.1f;
-            pr *= prior;
-            scr[(size_t)(8 * K + k) * NT + t] = pr;
-            psum += pr;
-        }
-
-        // 15 draws through the CDF (first view whose cumulative probability exceeds the draw)
-        DevStream st((uint32_t)idx, (uint32_t)(1 + 2 * a.iter + phase), P.seed, P.uid);
-        float draw[VIEW_DRAWS];
-#pragma unroll
-        for (int s = 0; s < (int)VIEW_DRAWS; ++s)
-            draw[s] = u01_dev(st.raw(SLOT_VIEW_DRAW + s)) - 1.1920928955078125e-07f;
-        uint32_t tsel = 0u;
-        int nsel = 0;
-        if (psum > 0.0f) {
-            const float inv = 1.0f / psum;
-            float cum = 0.0f;
+        float mean = 2.0f;
+        uint32_t mask = 0u;
+        if (valid > 0) {
+            // k smallest costs in ascending order (ComputeSelectedViewsTopK_Frame :227-282)
+            const int kk = min(valid, P.top_k);
             uint32_t taken = 0u;
-            for (int k = 0; k < c; ++k) {
-                cum += scr[(size_t)(8 * K + k) * NT + t] * inv;
-                int wk = 0;
-#pragma unroll
-                for (int s = 0; s < (int)VIEW_DRAWS; ++s)
-                    if (!((taken >> s) & 1u) && cum > draw[s]) { ++wk; taken |= 1u << s; }
-                L.wts[k * NT + t] = (uint8_t)wk;
-                if (wk > 0) { tsel |= 1u << k; L.slist[nsel * NT + t] = (uint8_t)k; ++nsel; }
-            }
-        } else {
-            for (int k = 0; k < c; ++k) L.wts[k * NT + t] = 0;
-        }
-        if (nsel == 0) {
-            for (int k = 0; k < c; ++k) {
-                L.wts[k * NT + t] = 1;
-                L.slist[k * NT + t] = (uint8_t)k;
-                tsel |= 1u << k;
-            }
-            nsel = c;
-        }
-
-        // weighted cost over the selected views (ComputeWeightedCost_Frame :284-317)
-        auto wcost = [&](const float4& h) -> float {
-            float wn = 0.0f, tot = 0.0f;
-            for (int j0 = 0; j0 < nsel; j0 += kGroup) {
-                int vk[kGroup];
-                float cg[kGroup];
-#pragma unroll
-                for (int g = 0; g < kGroup; ++g) {
-                    vk[g] = -1;
-                    if (j0 + g < nsel) {
-                        const int k = L.slist[(j0 + g) * NT + t];
-                        if (L.nid[k] >= 0) vk[g] = k;
-                    }
+            float sum = 0.0f, thr = 2.0f;
+            for (int m = 0; m < kk; ++m) {
+                int bi = -1;
+                float bv = 0.0f;
+                for (int k = 0; k < c; ++k) {
+                    if ((taken >> k) & 1u) continue;
+                    const float cv = scr[(size_t)k * NT + t];
+                    if (bi < 0 || cv < bv) { bi = k; bv = cv; }
                 }
-                ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
-#pragma unroll
-                for (int g = 0; g < kGroup; ++g) {
-                    if (j0 + g >= nsel) break;
-                    const int k = L.slist[(j0 + g) * NT + t];
-                    const float w = (float)L.wts[k * NT + t];
-                    wn += w;
-                    tot += w * cg[g];
-                }
+                taken |= 1u << bi;
+                sum += bv;
+                thr = bv;
             }
-            return wn <= 1e-6f ? 2.0f : tot / wn;
-        };
-
-        float4 best = a.plane_in[idx];
-        float bcost = wcost(best);
-        for (int cd = 0; cd < 8; ++cd) {
-            const int sc = cand_src(cd);
-            if (sc < 0) continue;
-            float wn = 0.0f, tot = 0.0f;
-            for (int j = 0; j < nsel; ++j) {
-                const int k = L.slist[j * NT + t];
-                const float w = (float)L.wts[k * NT + t];
-                wn += w;
-                tot += w * scr[(size_t)(sc * K + k) * NT + t];
-            }
-            const float cc = wn <= 1e-6f ? 2.0f : tot / wn;
-            if (cc < bcost) { bcost = cc; best = a.plane_in[base + cand_pos(cd)]; }
+            mean = sum / (float)kk;
+            for (int k = 0; k < c; ++k)
+                if (scr[(size_t)k * NT + t] <= thr) mask |= 1u << k;
         }
-
-        // cross-view proposals through each selected view (BuildCrossViewPlaneProposal :350-398)
-        for (int j = 0; j < nsel; ++j) {
-            const int k = L.slist[j * NT + t];
-            const int n = L.nid[k];
-            if (n < 0) continue;
-            const float2 sk = L.step[k];
-            const float qfx = (float)x + sk.x * best.w, qfy = (float)y + sk.y * best.w;
-            const float fqx = floorf(qfx + 0.5f), fqy = floorf(qfy + 0.5f);
-            if (!(fqx >= 0.0f && fqx < (float)W && fqy >= 0.0f && fqy < (float)W)) continue;
-            const int qx = (int)fqx, qy = (int)fqy;
-            const float4 hn = a.plane_in[(size_t)n * WW + qy * W + qx];
-            const float al = -hn.x / hn.z, be = -hn.y / hn.z;
-            const float ga = hn.w - al * (float)qx - be * (float)qy;
-            const float2 cdv = L.cdiff[k];
-            float den = 1.0f + al * cdv.x + be * cdv.y;
-            if (fabsf(den) < 1e-6f) den = den >= 0.0f ? 1e-6f : -1e-6f;
-            const float aR = al / den, bR = be / den, cR = ga / den;
-            float nx = -aR, ny = -bR, nz = 1.0f;
-            normalize3(nx, ny, nz);
-            const float4 prop = make_float4(nx, ny, nz, aR * (float)x + bR * (float)y + cR);
-            const float cc = wcost(prop);
-            if (cc < bcost) { bcost = cc; best = prop; }
-        }
-
-        // refinement: ACMP depth x normal candidates (AdaptMIPMUtil.cu:181-183)
-        {
-            const float pert = 0.02f;
-            const float dnow = best.w;
-            float drand;
-            if (bcost < 0.1f)
-                drand = (u01_dev(st.raw(SLOT_REF_DEPTH_RAND)) - 0.5f) * 2.0f + dnow;
-            else
-                drand = u01_dev(st.raw(SLOT_REF_DEPTH_RAND)) * (P.dmax - P.dmin) + P.dmin;
-            float rnx, rny, rnz;
-            random_normal(st, SLOT_REF_MARSAGLIA, rnx, rny, rnz);
-            // perturbed normal (GeneratePerturbedNormalLF, CudaUtil.cu:105-142)
-            float vx = (float)x, vy = (float)y, vz = 1.0f;
-            normalize3(vx, vy, vz);
-            const float pamp = pert * 3.14159265358979323846f;
-            const float a1 = (u01_dev(st.raw(SLOT_REF_ANGLE)) - 0.5f) * pamp;
-            const float a2 = (u01_dev(st.raw(SLOT_REF_ANGLE + 1)) - 0.5f) * pamp;
-            const float a3 = (u01_dev(st.raw(SLOT_REF_ANGLE + 2)) - 0.5f) * pamp;
-            const float s1 = sinf(a1), s2 = sinf(a2), s3 = sinf(a3);
-            const float c1 = cosf(a1), c2 = cosf(a2), c3 = cosf(a3);
-            float px_ = (c2 * c3) * best.x + (c3 * s1 * s2 - c1 * s3) * best.y +
-                        (s1 * s3 + c1 * c3 * s2) * best.z;
-            float py_ = (c2 * s3) * best.x + (c1 * c3 + s1 * s2 * s3) * best.y +
-                        (c1 * s2 * s3 - c3 * s1) * best.z;
-            float pz_ = (-s2) * best.x + (c2 * s1) * best.y + (c1 * c2) * best.z;
-            if (px_ * vx + py_ * vy + pz_ * vz >= 0.0f) { px_ = best.x; py_ = best.y; pz_ = best.z; }
-            normalize3(px_, py_, pz_);
-            const float lo = (1.0f - pert) * dnow, hi = (1.0f + pert) * dnow;
-            const float dpert = u01_dev(st.raw(SLOT_REF_DEPTH_PERT)) * (hi - lo) + lo;
-            const float4 cur = best;
-            for (int i = 0; i < 5; ++i) {
-                // (n_cur, d_rand), (n_rand, d_now), (n_rand, d_rand), (n_pert, d_now),
-                // (n_cur, d_pert), selected without a dynamically indexed array
-                float4 hc;
-                hc.x = (i == 0 || i == 4) ? cur.x : (i == 3 ? px_ : rnx);
-                hc.y = (i == 0 || i == 4) ? cur.y : (i == 3 ? py_ : rny);
-                hc.z = (i == 0 || i == 4) ? cur.z : (i == 3 ? pz_ : rnz);
-                hc.w = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dnow);
-                const float cc = wcost(hc);
-                if (cc < bcost) { bcost = cc; best = hc; }
-            }
-        }
-        a.plane_out[idx] = best;
-        a.cost_out[idx] = bcost;
-        a.sel_out[idx] = tsel;
+        F.plane_out[idx] = h;
+        F.cost_out[idx] = mean;
+        F.sel_out[idx] = mask;
     }
-    flush_counts<COUNT>(a.counters, wc);
 }
 
-// ------------------------------------------------------------------------------------------
-// Final disparity/baseline record: lowest-cost view of the final plane over every view
-// (the disp argument of ComputeWeightedCost_Frame, AdaptMIPMFrame.cu:303-306).
-// ------------------------------------------------------------------------------------------
-struct FinalArgs {
-    const float* tiles; const float2* centers; const int* nbr; const int* cnt;
-    const float4* plane; float4* disp;
-    unsigned long long* counters;
-    DevParams P;
-};
-
+// Lowest-cost view record of the final plane over every view (the disp argument of
+// ComputeWeightedCost_Frame, AdaptMIPMFrame.cu:303-306).
 template <bool COUNT>
-__global__ void __launch_bounds__(1024) k_finalize(FinalArgs a) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const DevParams& P = a.P;
+__device__ __forceinline__ void final_mi(const Lds& L, const DevParams& P, const FrameDesc& F,
+                                         int v, int c, WorkCount& wc) {
     const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW;
-    Lds L = carve(lds_raw, P, NT);
-    fill_spatial(L, P);
-    int first, end, stride;
-    mi_range(P, first, end, stride);
-    WorkCount wc{0u, 0u};
-    for (int mi = first; mi < end; mi += stride) {
-        __syncthreads();
-        const int c = stage_mi(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
-        __syncthreads();
-        if (t >= WW) continue;
-        const int x = t % W, y = t / W;
-        const size_t idx = (size_t)mi * WW + t;
-        const float4 h = a.plane[idx];
+    for (int pix = t; pix < WW; pix += NT) {
+        const int x = pix % W, y = pix / W;
+        const size_t idx = (size_t)v * WW + pix;
+        const float4 h = F.plane_in[idx];
         const float rc = L.tiles[y * P.W1 + x];
         float mc = 2.0f;
         int am = -1;
@@ -907,7 +631,277 @@ __global__ void __launch_bounds__(1024) k_finalize(FinalArgs a) {
             for (int g = 0; g < kGroup; ++g)
                 if (k0 + g < c && cg[g] < mc) { mc = cg[g]; am = k0 + g; }
         }
-        a.disp[idx] = am >= 0 ? disp_record(L, P, am, h.w) : make_float4(0.f, 0.f, 0.f, 0.f);
+        F.disp[idx] = am >= 0 ? disp_record(L, P, am, h.w) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// One red/black phase of MI v: the active lane t owns one pixel of colour F.phase.
+template <bool COUNT>
+__device__ __forceinline__ void phase_mi(const Lds& L, const DevParams& P, const FrameDesc& F,
+                                         int v, int c, float* __restrict__ scr, WorkCount& wc) {
+    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW, K = P.K;
+    const int phase = F.phase;
+    const int nact = (W & 1) ? (phase == 0 ? (WW + 1) / 2 : WW / 2) : WW / 2;
+    const size_t base = (size_t)v * WW;
+    // pass-through of the inactive colour (and of every pixel of an MI without neighbours)
+    for (int i = t; i < WW; i += NT) {
+        const int x = i % W, y = i / W;
+        if (((x + y) & 1) != phase || c <= 0) {
+            F.plane_out[base + i] = F.plane_in[base + i];
+            F.cost_out[base + i] = F.cost_in[base + i];
+            F.sel_out[base + i] = F.sel_in[base + i];
+        }
+    }
+    if (c <= 0 || t >= nact) return;
+    int x, y;
+    active_pixel(W, t, phase, x, y);
+    const int center = y * W + x;
+    const size_t idx = base + center;
+    const float rc = L.tiles[y * P.W1 + x];
+    // candidate table packed into registers (no dynamically indexed private arrays):
+    // pos of slot cd in bits 10*cd.. of a 128-bit pair, src+1 in bits 4*cd.. of srcp
+    uint64_t posp[2] = {0ull, 0ull};
+    uint32_t srcp = 0u;
+    {
+        int pos[8], src[8];
+        select_candidates(L.costp, W, x, y, pos, src);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            posp[q / 6] |= (uint64_t)pos[q] << (10 * (q % 6));
+            srcp |= (uint32_t)(src[q] + 1) << (4 * q);
+        }
+    }
+    auto cand_src = [&](int cd) -> int { return (int)((srcp >> (4 * cd)) & 15u) - 1; };
+    auto cand_pos = [&](int cd) -> int {
+        return (int)((cd < 6 ? posp[0] >> (10 * cd) : posp[1] >> (10 * (cd - 6))) & 1023ull);
+    };
+
+    // cost vectors of the propagation candidates over every view
+    for (int cd = 0; cd < 8; ++cd) {
+        if (cand_src(cd) != cd) continue;
+        const float4 h = F.plane_in[base + cand_pos(cd)];
+        for (int k0 = 0; k0 < c; k0 += kGroup) {
+            int vk[kGroup];
+            float cg[kGroup];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g)
+                vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
+            ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g)
+                if (k0 + g < c) scr[(size_t)(cd * K + k0 + g) * NT + t] = cg[g];
+        }
+    }
+
+    // view-selection priors from the 4 direct neighbours' previous selections
+    uint32_t dsel[4];
+    bool dval[4];
+    dval[0] = y > 0; dval[1] = y < W - 1; dval[2] = x > 0; dval[3] = x < W - 1;
+    dsel[0] = dval[0] ? F.sel_in[idx - W] : 0u;
+    dsel[1] = dval[1] ? F.sel_in[idx + W] : 0u;
+    dsel[2] = dval[2] ? F.sel_in[idx - 1] : 0u;
+    dsel[3] = dval[3] ? F.sel_in[idx + 1] : 0u;
+
+    // sampling probabilities
+    const float thr = 0.8f * expf((float)(F.iter * F.iter) / (-90.0f));
+    float psum = 0.0f;
+    for (int k = 0; k < c; ++k) {
+        float count = 0.0f, tmpw = 0.0f;
+        int cf = 0;
+        for (int cd = 0; cd < 8; ++cd) {
+            const int sc = cand_src(cd);
+            if (sc < 0) continue;
+            const float cv = scr[(size_t)(sc * K + k) * NT + t];
+            if (cv < thr) { tmpw += expf(cv * cv / (-0.18f)); count += 1.0f; }
+            if (cv > 1.2f) ++cf;
+        }
+        float pr = 0.0f;
+        if (count > 2.0f && cf < 3) pr = tmpw / count;
+        else if (cf < 3) pr = expf(thr * thr / (-0.32f));
+        float prior = 0.0f;
+        for (int ii = 0; ii < 4; ++ii)
+            if (dval[ii]) prior += ((dsel[ii] >> k) & 1u) ? 0.9f : 0.1f;
+        pr *= prior;
+        scr[(size_t)(8 * K + k) * NT + t] = pr;
+        psum += pr;
+    }
+
+    // 15 draws through the CDF (first view whose cumulative probability exceeds the draw)
+    DevStream st((uint32_t)idx, (uint32_t)(1 + 2 * F.iter + phase), F.seed, F.uid);
+    float draw[VIEW_DRAWS];
+#pragma unroll
+    for (int s = 0; s < (int)VIEW_DRAWS; ++s)
+        draw[s] = u01_dev(st.raw(SLOT_VIEW_DRAW + s)) - 1.1920928955078125e-07f;
+    uint32_t tsel = 0u;
+    int nsel = 0;
+    if (psum > 0.0f) {
+        const float inv = 1.0f / psum;
+        float cum = 0.0f;
+        uint32_t taken = 0u;
+        for (int k = 0; k < c; ++k) {
+            cum += scr[(size_t)(8 * K + k) * NT + t] * inv;
+            int wk = 0;
+#pragma unroll
+            for (int s = 0; s < (int)VIEW_DRAWS; ++s)
+                if (!((taken >> s) & 1u) && cum > draw[s]) { ++wk; taken |= 1u << s; }
+            L.wts[k * NT + t] = (uint8_t)wk;
+            if (wk > 0) { tsel |= 1u << k; L.slist[nsel * NT + t] = (uint8_t)k; ++nsel; }
+        }
+    } else {
+        for (int k = 0; k < c; ++k) L.wts[k * NT + t] = 0;
+    }
+    if (nsel == 0) {
+        for (int k = 0; k < c; ++k) {
+            L.wts[k * NT + t] = 1;
+            L.slist[k * NT + t] = (uint8_t)k;
+            tsel |= 1u << k;
+        }
+        nsel = c;
+    }
+
+    // weighted cost over the selected views (ComputeWeightedCost_Frame :284-317)
+    auto wcost = [&](const float4& h) -> float {
+        float wn = 0.0f, tot = 0.0f;
+        for (int j0 = 0; j0 < nsel; j0 += kGroup) {
+            int vk[kGroup];
+            float cg[kGroup];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                vk[g] = -1;
+                if (j0 + g < nsel) {
+                    const int k = L.slist[(j0 + g) * NT + t];
+                    if (L.nid[k] >= 0) vk[g] = k;
+                }
+            }
+            ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                if (j0 + g >= nsel) break;
+                const int k = L.slist[(j0 + g) * NT + t];
+                const float w = (float)L.wts[k * NT + t];
+                wn += w;
+                tot += w * cg[g];
+            }
+        }
+        return wn <= 1e-6f ? 2.0f : tot / wn;
+    };
+
+    float4 best = F.plane_in[idx];
+    float bcost = wcost(best);
+    for (int cd = 0; cd < 8; ++cd) {
+        const int sc = cand_src(cd);
+        if (sc < 0) continue;
+        float wn = 0.0f, tot = 0.0f;
+        for (int j = 0; j < nsel; ++j) {
+            const int k = L.slist[j * NT + t];
+            const float w = (float)L.wts[k * NT + t];
+            wn += w;
+            tot += w * scr[(size_t)(sc * K + k) * NT + t];
+        }
+        const float cc = wn <= 1e-6f ? 2.0f : tot / wn;
+        if (cc < bcost) { bcost = cc; best = F.plane_in[base + cand_pos(cd)]; }
+    }
+
+    // cross-view proposals through each selected view (BuildCrossViewPlaneProposal :350-398)
+    for (int j = 0; j < nsel; ++j) {
+        const int k = L.slist[j * NT + t];
+        const int n = L.nid[k];
+        if (n < 0) continue;
+        const float2 sk = L.step[k];
+        const float qfx = (float)x + sk.x * best.w, qfy = (float)y + sk.y * best.w;
+        const float fqx = floorf(qfx + 0.5f), fqy = floorf(qfy + 0.5f);
+        if (!(fqx >= 0.0f && fqx < (float)W && fqy >= 0.0f && fqy < (float)W)) continue;
+        const int qx = (int)fqx, qy = (int)fqy;
+        const float4 hn = F.plane_in[(size_t)n * WW + qy * W + qx];
+        const float al = -hn.x / hn.z, be = -hn.y / hn.z;
+        const float ga = hn.w - al * (float)qx - be * (float)qy;
+        const float2 cdv = L.cdiff[k];
+        float den = 1.0f + al * cdv.x + be * cdv.y;
+        if (fabsf(den) < 1e-6f) den = den >= 0.0f ? 1e-6f : -1e-6f;
+        const float aR = al / den, bR = be / den, cR = ga / den;
+        float nx = -aR, ny = -bR, nz = 1.0f;
+        normalize3(nx, ny, nz);
+        const float4 prop = make_float4(nx, ny, nz, aR * (float)x + bR * (float)y + cR);
+        const float cc = wcost(prop);
+        if (cc < bcost) { bcost = cc; best = prop; }
+    }
+
+    // refinement: ACMP depth x normal candidates (AdaptMIPMUtil.cu:181-183)
+    {
+        const float pert = 0.02f;
+        const float dnow = best.w;
+        float drand;
+        if (bcost < 0.1f)
+            drand = (u01_dev(st.raw(SLOT_REF_DEPTH_RAND)) - 0.5f) * 2.0f + dnow;
+        else
+            drand = u01_dev(st.raw(SLOT_REF_DEPTH_RAND)) * (P.dmax - P.dmin) + P.dmin;
+        float rnx, rny, rnz;
+        random_normal(st, SLOT_REF_MARSAGLIA, rnx, rny, rnz);
+        // perturbed normal (GeneratePerturbedNormalLF, CudaUtil.cu:105-142)
+        float vx = (float)x, vy = (float)y, vz = 1.0f;
+        normalize3(vx, vy, vz);
+        const float pamp = pert * 3.14159265358979323846f;
+        const float a1 = (u01_dev(st.raw(SLOT_REF_ANGLE)) - 0.5f) * pamp;
+        const float a2 = (u01_dev(st.raw(SLOT_REF_ANGLE + 1)) - 0.5f) * pamp;
+        const float a3 = (u01_dev(st.raw(SLOT_REF_ANGLE + 2)) - 0.5f) * pamp;
+        const float s1 = sinf(a1), s2 = sinf(a2), s3 = sinf(a3);
+        const float c1 = cosf(a1), c2 = cosf(a2), c3 = cosf(a3);
+        float px_ = (c2 * c3) * best.x + (c3 * s1 * s2 - c1 * s3) * best.y +
+                    (s1 * s3 + c1 * c3 * s2) * best.z;
+        float py_ = (c2 * s3) * best.x + (c1 * c3 + s1 * s2 * s3) * best.y +
+                    (c1 * s2 * s3 - c3 * s1) * best.z;
+        float pz_ = (-s2) * best.x + (c2 * s1) * best.y + (c1 * c2) * best.z;
+        if (px_ * vx + py_ * vy + pz_ * vz >= 0.0f) { px_ = best.x; py_ = best.y; pz_ = best.z; }
+        normalize3(px_, py_, pz_);
+        const float lo = (1.0f - pert) * dnow, hi = (1.0f + pert) * dnow;
+        const float dpert = u01_dev(st.raw(SLOT_REF_DEPTH_PERT)) * (hi - lo) + lo;
+        const float4 cur = best;
+        for (int i = 0; i < 5; ++i) {
+            // (n_cur, d_rand), (n_rand, d_now), (n_rand, d_rand), (n_pert, d_now),
+            // (n_cur, d_pert), selected without a dynamically indexed array
+            float4 hc;
+            hc.x = (i == 0 || i == 4) ? cur.x : (i == 3 ? px_ : rnx);
+            hc.y = (i == 0 || i == 4) ? cur.y : (i == 3 ? py_ : rny);
+            hc.z = (i == 0 || i == 4) ? cur.z : (i == 3 ? pz_ : rnz);
+            hc.w = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dnow);
+            const float cc = wcost(hc);
+            if (cc < bcost) { bcost = cc; best = hc; }
+        }
+    }
+    F.plane_out[idx] = best;
+    F.cost_out[idx] = bcost;
+    F.sel_out[idx] = tsel;
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_stage(StageArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const DevParams& P = a.P;
+    const int NT = blockDim.x, t = threadIdx.x, WW = P.WW, K = P.K;
+    Lds L = carve(lds_raw, P, NT);
+    fill_spatial(L, P);
+    float* scr = a.scratch + (size_t)blockIdx.x * 9 * K * NT;
+    // XCD-aware contiguous MI ranges over the concatenated index space of the table
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int lo = (int)(((long long)a.total * xcd) >> 3);
+    const int hi = (int)(((long long)a.total * (xcd + 1)) >> 3);
+    WorkCount wc{0u, 0u};
+    int f = 0;
+    for (int gm = lo + local; gm < hi; gm += P.slots_per_xcd) {
+        while (f + 1 < a.nf && gm >= a.f[f + 1].mi_begin) ++f;
+        const FrameDesc& F = a.f[f];
+        const int stage = F.stage;
+        if (stage == STAGE_IDLE) continue;  // uniform across the workgroup
+        const int v = gm - F.mi_begin;
+        __syncthreads();
+        const int c = stage_mi(L, P, v, F.tiles, F.centers, F.nbr, F.cnt);
+        if (stage == STAGE_PHASE)
+            for (int i = t; i < WW; i += NT) L.costp[i] = F.cost_in[(size_t)v * WW + i];
+        __syncthreads();
+        if (stage == STAGE_INIT) init_mi<COUNT>(L, P, F, v, c, scr, wc);
+        else if (stage == STAGE_PHASE) phase_mi<COUNT>(L, P, F, v, c, scr, wc);
+        else final_mi<COUNT>(L, P, F, v, c, wc);
     }
     flush_counts<COUNT>(a.counters, wc);
 }
@@ -1352,33 +1346,126 @@ static int phase_threads(const DevParams& P) {
     return std::min(256, 64 * ((nact + 63) / 64));
 }
 
-static int full_threads(const DevParams& P) { return 64 * ((P.WW + 63) / 64); }
-
-// Persistent grid: a multiple of 8 workgroups (one share per XCD), at most V rounded up.
-static int persistent_grid(const DevParams& P, int per_cu) {
+// Persistent grid over `mis` work items: a multiple of 8 workgroups (one share per XCD).
+static int persistent_grid_n(int mis, int per_cu) {
     int g = device_cus() * per_cu;
-    const int vv = ((P.V + 7) / 8) * 8;
+    const int vv = ((mis + 7) / 8) * 8;
     g = std::min(g, std::max(8, vv));
     g = (g / 8) * 8;
     return std::max(8, g);
 }
+static int persistent_grid(const DevParams& P, int per_cu) { return persistent_grid_n(P.V, per_cu); }
 
 static thread_local unsigned long long* g_counters = nullptr;
 
-static const int kPhasePerCu = 8;
-static const int kInitPerCu = 4;
+static const int kPhasePerCu = 8;    // k_stage workgroups per CU (persistent slots), default
+static const int kMaxSlotsPerCu = 8; // upper bound of the autotuner's search
 
-static size_t init_scratch_bytes(const DevParams& P) {
-    return (size_t)persistent_grid(P, kInitPerCu) * P.K * full_threads(P) * 4;
+static size_t stage_scratch_bytes(const DevParams& P, int grid) {
+    return (size_t)grid * 9 * P.K * phase_threads(P) * 4;
 }
-static size_t phase_scratch_bytes(const DevParams& P) {
-    return (size_t)persistent_grid(P, kPhasePerCu) * 9 * P.K * phase_threads(P) * 4;
+static size_t phase_scratch_bytes(const DevParams& P, int per_cu) {
+    return stage_scratch_bytes(P, persistent_grid(P, per_cu));
+}
+
+static FrameDesc frame_desc(const float* tiles, const float* centers, const int* nbr,
+                            const int* cnt, int V, uint32_t seed, uint32_t uid) {
+    FrameDesc f;
+    std::memset(&f, 0, sizeof(f));
+    f.tiles = tiles; f.centers = (const float2*)centers; f.nbr = nbr; f.cnt = cnt;
+    f.V = V; f.seed = seed; f.uid = uid;
+    return f;
+}
+
+// Launch k_stage over the frames of `a` (mi_begin/total filled here).
+static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_bytes,
+                        hipStream_t s) {
+    int total = 0;
+    for (int i = 0; i < a.nf; ++i) {
+        a.f[i].mi_begin = total;
+        total += a.f[i].V;
+    }
+    a.total = total;
+    if (total <= 0) return LFPM_OK;
+    const int grid = persistent_grid_n(total, per_cu);
+    a.P.slots_per_xcd = grid / 8;
+    const int NT = phase_threads(a.P);
+    if ((a.P.WW + 1) / 2 > NT) return set_error(LFPM_E_INVALID, "crop too large for k_stage");
+    bool needs_scratch = false;  // only init and phase stages keep per-lane cost vectors
+    for (int i = 0; i < a.nf; ++i)
+        needs_scratch |= a.f[i].stage == STAGE_INIT || a.f[i].stage == STAGE_PHASE;
+    const size_t need = stage_scratch_bytes(a.P, grid);
+    if (needs_scratch && (!scratch || scratch_bytes < need))
+        return set_error(LFPM_E_INVALID, "workspace too small (%zu < %zu)", scratch_bytes, need);
+    a.scratch = (float*)scratch;
+    a.counters = g_counters;
+    const size_t lds = lds_bytes(a.P, NT, true);
+    if (g_counters)
+        hipLaunchKernelGGL(k_stage<true>, dim3(grid), dim3(NT), lds, s, a);
+    else
+        hipLaunchKernelGGL(k_stage<false>, dim3(grid), dim3(NT), lds, s, a);
+    return launch_check("k_stage");
 }
 static size_t state_bytes(const DevParams& P) {
     return (size_t)P.V * P.WW * (16 + 4 + 4);
 }
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Autotuned k_stage occupancy per parameter class (crop, neighbours, radius).
+struct TuneKey {
+    int crop, K, r;
+    bool operator==(const TuneKey& o) const { return crop == o.crop && K == o.K && r == o.r; }
+};
+static std::vector<std::pair<TuneKey, int>> g_tuned;
+
+int stage_slots_per_cu(const lfpm_pm_params* p) {
+    const TuneKey k{p->crop, p->max_neighbors, p->radius};
+    for (auto& e : g_tuned)
+        if (e.first == k) return e.second;
+    return kPhasePerCu;
+}
+
+void set_stage_slots_per_cu(const lfpm_pm_params* p, int per_cu) {
+    const TuneKey k{p->crop, p->max_neighbors, p->radius};
+    for (auto& e : g_tuned)
+        if (e.first == k) { e.second = per_cu; return; }
+    g_tuned.push_back({k, per_cu});
+}
+
+size_t stage_scratch_size(const lfpm_pm_params* p, int total_mis) {
+    if (check_pm_params(p, std::max(total_mis, 1))) return 0;
+    const DevParams P = make_dev_params(p, total_mis);
+    return stage_scratch_bytes(P, persistent_grid_n(total_mis, stage_slots_per_cu(p)));
+}
+
+int stage_launch(const lfpm_pm_params* p, const StageFrame* frames, int n, void* scratch,
+                 size_t scratch_bytes, void* stream) {
+    int rc = check_pm_params(p, 1);
+    if (rc) return rc;
+    for (int i0 = 0; i0 < n; i0 += kMaxFrames) {
+        StageArgs a;
+        a.nf = std::min(kMaxFrames, n - i0);
+        int total = 0;
+        for (int i = 0; i < a.nf; ++i) {
+            const StageFrame& s = frames[i0 + i];
+            if (!s.tiles || !s.centers || !s.nbr || !s.cnt || s.V <= 0)
+                return set_error(LFPM_E_INVALID, "stage: frame %d has no inputs", i0 + i);
+            FrameDesc& f = a.f[i];
+            f = frame_desc(s.tiles, s.centers, s.nbr, s.cnt, s.V, s.seed, s.uid);
+            f.stage = s.stage; f.iter = s.iter; f.phase = s.phase;
+            f.plane_in = (const float4*)s.plane_in; f.cost_in = s.cost_in; f.sel_in = s.sel_in;
+            f.plane_out = (float4*)s.plane_out; f.cost_out = s.cost_out; f.sel_out = s.sel_out;
+            f.disp = (float4*)s.disp;
+            total += s.V;
+        }
+        a.P = make_dev_params(p, total);
+        if ((rc = launch_stage(a, stage_slots_per_cu(p), scratch, scratch_bytes,
+                               (hipStream_t)stream)))
+            return rc;
+    }
+    return LFPM_OK;
+}
 
 } // namespace lfpm
 
@@ -1415,8 +1502,8 @@ int lfpm_slice_gray(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pit
 size_t lfpm_patchmatch_workspace_size(int num_mis, const lfpm_pm_params* p) {
     if (check_pm_params(p, num_mis)) return 0;
     const DevParams P = make_dev_params(p, num_mis);
-    return align_up(state_bytes(P)) +
-           align_up(std::max(init_scratch_bytes(P), phase_scratch_bytes(P)));
+    // the largest occupancy the tuner may pick, so a workspace stays valid after tuning
+    return align_up(state_bytes(P)) + align_up(phase_scratch_bytes(P, kMaxSlotsPerCu));
 }
 
 static int check_solver_ptrs(const float* tiles, const float* centers, const int* nbr,
@@ -1435,25 +1522,15 @@ int lfpm_patchmatch_init(const float* tiles_dev, const float* centers_dev,
     if ((rc = check_solver_ptrs(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev)))
         return rc;
     if (!plane_dev || !cost_dev || !sel_dev) return set_error(LFPM_E_INVALID, "null output");
-    DevParams P = make_dev_params(p, num_mis);
-    const int grid = persistent_grid(P, kInitPerCu);
-    P.slots_per_xcd = grid / 8;
-    const int NT = full_threads(P);
-    if (NT > 1024) return set_error(LFPM_E_INVALID, "crop too large for init kernel");
-    if (!workspace_dev || workspace_bytes < init_scratch_bytes(P))
-        return set_error(LFPM_E_INVALID, "init: workspace too small (%zu < %zu)", workspace_bytes,
-                         init_scratch_bytes(P));
-    InitArgs a;
-    a.tiles = tiles_dev; a.centers = (const float2*)centers_dev; a.nbr = neighbor_ids_dev;
-    a.cnt = neighbor_counts_dev; a.plane = (float4*)plane_dev; a.cost = cost_dev; a.sel = sel_dev;
-    a.scratch = (float*)workspace_dev; a.P = P; a.counters = g_counters;
-    if (g_counters)
-        hipLaunchKernelGGL(k_init<true>, dim3(grid), dim3(NT), lds_bytes(P, NT, false),
-                           (hipStream_t)stream, a);
-    else
-        hipLaunchKernelGGL(k_init<false>, dim3(grid), dim3(NT), lds_bytes(P, NT, false),
-                           (hipStream_t)stream, a);
-    return launch_check("init");
+    StageArgs a;
+    a.nf = 1;
+    a.P = make_dev_params(p, num_mis);
+    a.f[0] = frame_desc(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev, num_mis,
+                        p->seed, p->frame_uid);
+    a.f[0].stage = STAGE_INIT;
+    a.f[0].plane_out = (float4*)plane_dev; a.f[0].cost_out = cost_dev; a.f[0].sel_out = sel_dev;
+    return launch_stage(a, stage_slots_per_cu(p), workspace_dev, workspace_bytes,
+                        (hipStream_t)stream);
 }
 
 int lfpm_patchmatch_phase(const float* tiles_dev, const float* centers_dev,
@@ -1472,28 +1549,17 @@ int lfpm_patchmatch_phase(const float* tiles_dev, const float* centers_dev,
         return set_error(LFPM_E_INVALID, "null state buffer");
     if (phase != 0 && phase != 1) return set_error(LFPM_E_INVALID, "phase must be 0 or 1");
     if (iter < 0) return set_error(LFPM_E_INVALID, "iter must be >= 0");
-    DevParams P = make_dev_params(p, num_mis);
-    const int grid = persistent_grid(P, kPhasePerCu);
-    P.slots_per_xcd = grid / 8;
-    const int NT = phase_threads(P);
-    if ((P.WW + 1) / 2 > NT) return set_error(LFPM_E_INVALID, "crop too large for phase kernel");
-    if (!workspace_dev || workspace_bytes < phase_scratch_bytes(P))
-        return set_error(LFPM_E_INVALID, "phase: workspace too small (%zu < %zu)",
-                         workspace_bytes, phase_scratch_bytes(P));
-    PhaseArgs a;
-    a.tiles = tiles_dev; a.centers = (const float2*)centers_dev; a.nbr = neighbor_ids_dev;
-    a.cnt = neighbor_counts_dev;
-    a.plane_in = (const float4*)plane_in_dev; a.cost_in = cost_in_dev; a.sel_in = sel_in_dev;
-    a.plane_out = (float4*)plane_out_dev; a.cost_out = cost_out_dev; a.sel_out = sel_out_dev;
-    a.scratch = (float*)workspace_dev; a.P = P; a.iter = iter; a.phase = phase;
-    a.counters = g_counters;
-    if (g_counters)
-        hipLaunchKernelGGL(k_phase<true>, dim3(grid), dim3(NT), lds_bytes(P, NT, true),
-                           (hipStream_t)stream, a);
-    else
-        hipLaunchKernelGGL(k_phase<false>, dim3(grid), dim3(NT), lds_bytes(P, NT, true),
-                           (hipStream_t)stream, a);
-    return launch_check("phase");
+    StageArgs a;
+    a.nf = 1;
+    a.P = make_dev_params(p, num_mis);
+    a.f[0] = frame_desc(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev, num_mis,
+                        p->seed, p->frame_uid);
+    FrameDesc& f = a.f[0];
+    f.stage = STAGE_PHASE; f.iter = iter; f.phase = phase;
+    f.plane_in = (const float4*)plane_in_dev; f.cost_in = cost_in_dev; f.sel_in = sel_in_dev;
+    f.plane_out = (float4*)plane_out_dev; f.cost_out = cost_out_dev; f.sel_out = sel_out_dev;
+    return launch_stage(a, stage_slots_per_cu(p), workspace_dev, workspace_bytes,
+                        (hipStream_t)stream);
 }
 
 int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
@@ -1505,21 +1571,14 @@ int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
     if ((rc = check_solver_ptrs(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev)))
         return rc;
     if (!plane_dev || !disp_dev) return set_error(LFPM_E_INVALID, "null buffer");
-    DevParams P = make_dev_params(p, num_mis);
-    const int grid = persistent_grid(P, kInitPerCu);
-    P.slots_per_xcd = grid / 8;
-    const int NT = full_threads(P);
-    FinalArgs a;
-    a.tiles = tiles_dev; a.centers = (const float2*)centers_dev; a.nbr = neighbor_ids_dev;
-    a.cnt = neighbor_counts_dev; a.plane = (const float4*)plane_dev; a.disp = (float4*)disp_dev;
-    a.P = P; a.counters = g_counters;
-    if (g_counters)
-        hipLaunchKernelGGL(k_finalize<true>, dim3(grid), dim3(NT), lds_bytes(P, NT, false),
-                           (hipStream_t)stream, a);
-    else
-        hipLaunchKernelGGL(k_finalize<false>, dim3(grid), dim3(NT), lds_bytes(P, NT, false),
-                           (hipStream_t)stream, a);
-    return launch_check("finalize");
+    StageArgs a;
+    a.nf = 1;
+    a.P = make_dev_params(p, num_mis);
+    a.f[0] = frame_desc(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev, num_mis,
+                        p->seed, p->frame_uid);
+    a.f[0].stage = STAGE_FINAL;
+    a.f[0].plane_in = (const float4*)plane_dev; a.f[0].disp = (float4*)disp_dev;
+    return launch_stage(a, stage_slots_per_cu(p), nullptr, 0, (hipStream_t)stream);
 }
 
 int lfpm_patchmatch_frame(const float* tiles_dev, const float* centers_dev,

@@ -355,3 +355,146 @@ def version():
 
 
 _ = math
+
+
+# --------------------------------------------------------------------------------------
+# Batched / served forms
+# --------------------------------------------------------------------------------------
+STAGE_IDLE, STAGE_INIT, STAGE_PHASE, STAGE_FINAL = 0, 1, 2, 3
+
+
+class BatchFrame(C.Structure):
+    _fields_ = [("tiles_dev", C.c_void_p), ("gray_dev", C.c_void_p), ("centers_dev", C.c_void_p),
+                ("neighbor_ids_dev", C.c_void_p), ("neighbor_counts_dev", C.c_void_p),
+                ("lattice_to_mi_dev", C.c_void_p), ("num_mis", C.c_int),
+                ("lattice_cols", C.c_int), ("lattice_rows", C.c_int), ("raw_w", C.c_int),
+                ("raw_h", C.c_int), ("params", C.POINTER(PmParams)),
+                ("filter", C.POINTER(FilterParams)),
+                ("plane_dev", C.c_void_p), ("cost_dev", C.c_void_p), ("sel_dev", C.c_void_p),
+                ("disp_dev", C.c_void_p), ("plane_b_dev", C.c_void_p), ("cost_b_dev", C.c_void_p),
+                ("sel_b_dev", C.c_void_p), ("f_plane_dev", C.c_void_p), ("f_cost_dev", C.c_void_p),
+                ("f_disp_dev", C.c_void_p), ("f_sel_dev", C.c_void_p), ("reason_dev", C.c_void_p),
+                ("refocus_slices", C.c_int), ("refocus_vmin", C.c_float),
+                ("refocus_vmax", C.c_float), ("refocus_dev", C.c_void_p)]
+
+
+class FrameSlot:
+    """Device-resident inputs + state + products of one frame in a batch (one serving slot)."""
+
+    def __init__(self, wl, device="cuda", filt=True, refocus=True):
+        import torch
+        self.wl = wl
+        self.df = DeviceFrame(wl, device=device)
+        V, W = wl.V, wl.crop
+        n = V * W * W
+        f32 = dict(dtype=torch.float32, device=device)
+        self.plane = torch.empty((n, 4), **f32)
+        self.cost = torch.empty(n, **f32)
+        self.sel = torch.empty(n, dtype=torch.int32, device=device)
+        self.disp = torch.empty((n, 4), **f32)
+        self.plane_b = torch.empty((n, 4), **f32)
+        self.cost_b = torch.empty(n, **f32)
+        self.sel_b = torch.empty(n, dtype=torch.int32, device=device)
+        self.filt = filt
+        if filt:
+            self.f_plane = torch.empty((n, 4), **f32)
+            self.f_cost = torch.empty(n, **f32)
+            self.f_disp = torch.empty((n, 4), **f32)
+            self.f_sel = torch.empty(n, dtype=torch.int32, device=device)
+            self.reason = torch.empty(n, dtype=torch.uint8, device=device)
+        self.slices = wl.slices if refocus else 0
+        h, w = self.df.raw.shape
+        self.refocus = torch.empty((self.slices, h, w), **f32) if refocus else None
+        self.pm = wl.pm
+        self.fp = wl.fp
+
+    def nbytes(self):
+        ts = [self.df.raw, self.df.tiles, self.df.gray, self.plane, self.cost, self.sel, self.disp,
+              self.plane_b, self.cost_b, self.sel_b]
+        if self.filt:
+            ts += [self.f_plane, self.f_cost, self.f_disp, self.f_sel, self.reason]
+        if self.refocus is not None:
+            ts.append(self.refocus)
+        return sum(t.numel() * t.element_size() for t in ts)
+
+    def desc(self):
+        d = BatchFrame()
+        df, L = self.df, self.wl.L
+        P = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
+        d.tiles_dev, d.gray_dev, d.centers_dev = P(df.tiles), P(df.gray), P(df.centers)
+        d.neighbor_ids_dev, d.neighbor_counts_dev = P(df.nbr), P(df.cnt)
+        d.lattice_to_mi_dev = P(df.l2m)
+        d.num_mis, d.lattice_cols, d.lattice_rows = self.wl.V, L["cols"], L["rows"]
+        d.raw_h, d.raw_w = df.raw.shape
+        d.params = C.pointer(self.pm)
+        d.filter = C.pointer(self.fp) if self.filt else None
+        d.plane_dev, d.cost_dev, d.sel_dev, d.disp_dev = (P(self.plane), P(self.cost),
+                                                          P(self.sel), P(self.disp))
+        d.plane_b_dev, d.cost_b_dev, d.sel_b_dev = P(self.plane_b), P(self.cost_b), P(self.sel_b)
+        if self.filt:
+            d.f_plane_dev, d.f_cost_dev, d.f_disp_dev = (P(self.f_plane), P(self.f_cost),
+                                                         P(self.f_disp))
+            d.f_sel_dev, d.reason_dev = P(self.f_sel), P(self.reason)
+        d.refocus_slices = self.slices
+        d.refocus_vmin, d.refocus_vmax = self.wl.vmin, self.wl.vmax
+        d.refocus_dev = P(self.refocus)
+        return d
+
+
+def _frames_array(slots):
+    arr = (BatchFrame * len(slots))()
+    for i, s in enumerate(slots):
+        arr[i] = s.desc()
+    return arr
+
+
+class Batch:
+    """A fixed set of frames solved together (eagerly or through one captured hipGraph)."""
+
+    def __init__(self, slots, device="cuda"):
+        import torch
+        self.slots = list(slots)
+        self.arr = _frames_array(self.slots)
+        n = lib().lfpm_batch_workspace_size(self.arr, len(self.slots))
+        if n == 0:
+            _check(-1)
+        self.ws_bytes = n
+        self.ws = torch.empty(n, dtype=torch.uint8, device=device)
+        self.graph = None
+
+    def run(self, stream=None):
+        _check(lib().lfpm_batch_run(self.arr, len(self.slots), _dp(self.ws),
+                                    C.c_size_t(self.ws_bytes), _stream(stream)))
+
+    def capture(self):
+        g = C.c_void_p()
+        _check(lib().lfpm_batch_graph_create(self.arr, len(self.slots), _dp(self.ws),
+                                             C.c_size_t(self.ws_bytes), C.byref(g)))
+        self.graph = g
+        return g
+
+    def replay(self, stream=None):
+        if self.graph is None:
+            self.capture()
+        _check(lib().lfpm_graph_launch(self.graph, _stream(stream)))
+
+    def close(self):
+        if self.graph is not None:
+            lib().lfpm_graph_destroy(self.graph)
+            self.graph = None
+
+    def step(self, stages, iters, phases, stream=None):
+        n = len(self.slots)
+        st = (C.c_int * n)(*stages)
+        it = (C.c_int * n)(*iters)
+        ph = (C.c_int * n)(*phases)
+        _check(lib().lfpm_batch_step(self.arr, n, st, it, ph, _dp(self.ws),
+                                     C.c_size_t(self.ws_bytes), _stream(stream)))
+
+    def autotune(self, i=0):
+        best = C.c_int()
+        ms = C.c_float()
+        a1 = (BatchFrame * 1)(self.arr[i])
+        _check(lib().lfpm_autotune_class(a1, _dp(self.ws), C.c_size_t(self.ws_bytes),
+                                         C.byref(best), C.byref(ms)))
+        return best.value, ms.value

@@ -1,0 +1,125 @@
+"""Batched and served forms on the GPU: a frame's results are the same alone and in a mixed batch
+(bit for bit — the random stream is keyed by (seed, frame_uid), never by batch position), through
+the eager schedule, the captured hipGraph (replayed several times with eager work in between) and
+the continuous-batching step API with staggered frames; autotuning changes no result."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def _workloads(lfpm):
+    return [
+        lfpm.workload_c1(raw_w=320, raw_h=256, slices=4, frame_uid=1),
+        lfpm.workload_c1(raw_w=256, raw_h=200, slices=4, frame_uid=2, seed=77),
+        lfpm.workload_c2(raw_w=300, raw_h=260, pitch=15.0, lens_types=3, per_type=(6, 12, 18),
+                         iterations=2, slices=3, frame_uid=3),
+        lfpm.workload_c2(raw_w=330, raw_h=280, pitch=31.0, iterations=2, slices=3, frame_uid=4),
+    ]
+
+
+def _plain(lfpm, torch, slot):
+    bufs = lfpm.patchmatch_frame(slot.df)
+    fo = lfpm.filter_frame(slot.df, bufs)
+    rf = lfpm.refocus(slot.df, slices=slot.slices)
+    torch.cuda.synchronize()
+    out = dict(plane=bufs.plane, cost=bufs.cost, sel=bufs.sel, disp=bufs.disp,
+               reason=fo["reason"], fplane=fo["plane"], refocus=rf)
+    return {k: v.cpu().numpy().copy() for k, v in out.items()}
+
+
+def _batched(slot):
+    out = dict(plane=slot.plane, cost=slot.cost, sel=slot.sel, disp=slot.disp,
+               reason=slot.reason, fplane=slot.f_plane, refocus=slot.refocus)
+    return {k: v.cpu().numpy().copy() for k, v in out.items()}
+
+
+def _same(a, b, tag):
+    for k in a:
+        assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), (tag, k)
+
+
+@pytest.fixture(scope="module")
+def setup(lfpm, torch_cuda):
+    slots = [lfpm.FrameSlot(wl) for wl in _workloads(lfpm)]
+    ref = [_plain(lfpm, torch_cuda, s) for s in slots]
+    return slots, ref
+
+
+def test_batch_eager_matches_plain(lfpm, torch_cuda, setup):
+    slots, ref = setup
+    _say("case batch eager: %d frames, MIs %s" % (len(slots), [s.wl.V for s in slots]))
+    b = lfpm.Batch(slots)
+    b.run()
+    torch_cuda.cuda.synchronize()
+    for i, s in enumerate(slots):
+        _same(ref[i], _batched(s), "frame %d" % i)
+
+
+def test_batch_graph_replays_match(lfpm, torch_cuda, setup):
+    slots, ref = setup
+    _say("case batch graph")
+    b = lfpm.Batch(slots)
+    b.capture()
+    try:
+        for rep in range(3):
+            for s in slots:  # eager work between replays: scribble over every output
+                for t in (s.plane, s.cost, s.disp, s.f_plane, s.refocus):
+                    t.fill_(-7.0)
+                s.reason.fill_(9)
+            torch_cuda.cuda.synchronize()
+            b.replay()
+            torch_cuda.cuda.synchronize()
+            for i, s in enumerate(slots):
+                _same(ref[i], _batched(s), "replay %d frame %d" % (rep, i))
+    finally:
+        b.close()
+
+
+def test_serving_steps_staggered(lfpm, torch_cuda, setup):
+    """Frames join at different steps (continuous batching); each still matches its solo run."""
+    slots, ref = setup
+    _say("case serving steps")
+    b = lfpm.Batch(slots)
+    n = len(slots)
+    start = [0, 2, 5, 1]
+    # per-frame schedule: init, iterations x (phase 0, phase 1), final
+    sched = []
+    for s in slots:
+        q = [(lfpm.STAGE_INIT, 0, 0)]
+        for it in range(s.pm.iterations):
+            q += [(lfpm.STAGE_PHASE, it, 0), (lfpm.STAGE_PHASE, it, 1)]
+        q.append((lfpm.STAGE_FINAL, 0, 0))
+        sched.append(q)
+    total = max(start[i] + len(sched[i]) for i in range(n))
+    for t in range(total):
+        st, it, ph = [], [], []
+        for i in range(n):
+            k = t - start[i]
+            e = sched[i][k] if 0 <= k < len(sched[i]) else (lfpm.STAGE_IDLE, 0, 0)
+            st.append(e[0]); it.append(e[1]); ph.append(e[2])
+        b.step(st, it, ph)
+    torch_cuda.cuda.synchronize()
+    for i, s in enumerate(slots):
+        _same(ref[i], _batched(s), "served frame %d" % i)
+
+
+def test_autotune_keeps_results(lfpm, torch_cuda, setup):
+    slots, ref = setup
+    b = lfpm.Batch(slots[:1])
+    best, ms = b.autotune(0)
+    _say("autotune", best, ms)
+    assert best in (2, 4, 6, 8) and ms > 0
+    again = _plain(lfpm, torch_cuda, slots[0])
+    _same(ref[0], again, "after tune")
