@@ -1,10 +1,18 @@
 """Benchmark of the micro-image PatchMatch depth path on one MI355X.
 
-A step = one full frame of the hot path on inputs already resident in HBM: random init,
-6 red-black propagation+refinement iterations, final disparity record, fused CheckModule pass
-and a 32-slice refocus, on the 4080x3068 synthetic frame (pitch 23, 27,258 MIs, 18 neighbours,
-15x15 window).  `--mode plain` mirrors the reference's launch pattern (one launch per phase,
-stream sync after each).  Prints one JSON line.
+The hot path per frame, on inputs already resident in HBM: random init, 6 red-black
+propagation+refinement iterations, final disparity record, fused CheckModule pass and a 32-slice
+refocus, on the 4080x3068 synthetic frame (pitch 23, 27,258 MIs, 18 neighbours, 15x15 window).
+Modes:
+  plain    one frame per step, one launch per phase with a stream sync after each (the
+           reference's launch pattern); value = frames/s
+  batched  (default) the plain measurement, then --batch frames of the same size per step in one
+           captured hipGraph; value = batched frames/s, with the plain figures beside it and a
+           bit-for-bit check that the batched frame 0 equals its plain solve
+  mixed    --batch uneven frames (sizes, pitches and neighbour counts mixed) in one graph
+  serve    Poisson request stream through the continuous-batching server; value = completed
+           frames/s, p50/p99 = request latency
+Prints one JSON line.
 """
 import argparse
 import ctypes as C
@@ -34,7 +42,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="plain", choices=["plain"])
+    ap.add_argument("--mode", default="batched",
+                    choices=["plain", "batched", "mixed", "serve", "precision"])
+    ap.add_argument("--hypotheses", type=int, default=1000000, help="precision: hypotheses")
+    ap.add_argument("--batch", type=int, default=4, help="frames per batch (batched / mixed)")
+    ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrival rate (1/s)")
+    ap.add_argument("--hbm-budget-gb", type=float, default=200.0, help="serve: admission budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the work-counting pass")
     ap.add_argument("--raw", default="4080x3068")
@@ -209,17 +223,9 @@ def cpu_baseline(lfpm):
                       % (wl.V, dt, full.V)}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported "
-              "(multi-GPU sharding is a design note only)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
-    import torch
-    import lfpm
-    rw, rh = (int(x) for x in args.raw.split("x"))
-    log("bench: synthesising %dx%d frame" % (rw, rh))
-    wl = lfpm.workload_c2(raw_w=rw, raw_h=rh)
+def plain_section(args, lfpm, torch, wl):
+    """One frame at a time, one launch per phase with a stream sync after each."""
+    rw, rh = wl.sp.raw_w, wl.sp.raw_h
     df = lfpm.DeviceFrame(wl)
     bufs = lfpm.SolveBuffers(wl.V, wl.crop, wl.pm)
     n = wl.V * wl.crop ** 2
@@ -229,11 +235,9 @@ def main():
               reason=torch.empty(n, dtype=torch.uint8, device="cuda"))
     rf = torch.empty((wl.slices, rh, rw), device="cuda")
     torch.cuda.synchronize()
-    log("bench: V=%d crop=%d K=%d iters=%d" % (wl.V, wl.crop, wl.pm.max_neighbors,
-                                                wl.pm.iterations))
     for i in range(args.warmup):
         run_step(lfpm, df, bufs, fo, rf, Timer(torch), torch)
-        log("bench: warmup %d done" % i)
+        log("bench: plain warmup %d done" % i)
     timer = Timer(torch)
     lat = []
     torch.cuda.synchronize()
@@ -243,21 +247,13 @@ def main():
         run_step(lfpm, df, bufs, fo, rf, timer, torch)
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - t0) * 1e3)
-        log("bench: step %d %.1f ms" % (i, lat[-1]))
-    torch.cuda.synchronize()
+        log("bench: plain step %d %.1f ms" % (i, lat[-1]))
     t_all = time.perf_counter() - t_all0
-    if args.calib_copy:
-        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-    fps = args.steps / t_all
     kt = {k: float(np.mean(timer.ms(k))) for k in ("init", "phase", "finalize", "filter",
                                                     "refocus")}
-    roof = None
-    work = None
+    out = dict(fps=args.steps / t_all, ms_per_step=t_all * 1e3 / args.steps, lat=lat, kt=kt,
+               result=dict(plane=bufs.plane.clone(), cost=bufs.cost.clone(),
+                           reason=fo["reason"].clone(), refocus=rf[-1].clone()))
     if not args.no_count:
         log("bench: counting pass")
         work = count_work(lfpm, df, bufs, fo, rf, torch)
@@ -265,28 +261,236 @@ def main():
         flop_per_launch = (ph["samples"] * FLOP_PER_SAMPLE + ph["triples"] * FLOP_PER_TRIPLE) / \
             ph["launches"]
         achieved = flop_per_launch / (kt["phase"] * 1e-3) / 1e12
-        roof = {"bound": "valu", "kernel": "k_phase", "achieved": achieved,
-                "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / (PEAK_FP32 / 1e12),
-                "traffic": measured_traffic("k_phase"), "avg_launch_ms": kt["phase"],
-                "flop_per_launch": flop_per_launch}
-    cpu = None
+        out["work"] = work
+        out["roofline"] = {
+            "bound": "valu", "kernel": "k_stage (red/black phase)", "achieved": achieved,
+            "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / (PEAK_FP32 / 1e12),
+            "traffic": measured_traffic("k_stage"), "avg_launch_ms": kt["phase"],
+            "flop_per_launch": flop_per_launch,
+            "per_unit": "22 FLOP per evaluated window sample + 16 per evaluated "
+                        "(pixel, view, hypothesis) triple"}
+    del df, bufs, fo, rf
+    return out
+
+
+def batched_section(args, lfpm, torch, wls):
+    """All frames in one captured hipGraph (class branches, persistent k_stage launches)."""
+    slots = [lfpm.FrameSlot(wl) for wl in wls]
+    batch = lfpm.Batch(slots)
+    batch.capture()
+    torch.cuda.synchronize()
+    for i in range(args.warmup):
+        batch.replay()
+        torch.cuda.synchronize()
+        log("bench: batched warmup %d done" % i)
+    lat = []
+    t_all0 = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        batch.replay()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+        log("bench: batched step %d (%d frames) %.1f ms" % (i, len(slots), lat[-1]))
+    t_all = time.perf_counter() - t_all0
+    s0 = slots[0]
+    res = dict(plane=s0.plane, cost=s0.cost, reason=s0.reason, refocus=s0.refocus[-1])
+    batch.close()
+    return dict(fps=len(slots) * args.steps / t_all, ms_per_step=t_all * 1e3 / args.steps,
+                lat=lat, frames=len(slots), result=res, mis=sum(w.V for w in wls))
+
+
+def same_results(a, b, torch):
+    return all(torch.equal(a[k], b[k]) for k in a)
+
+
+def mixed_workloads(lfpm, n, seed=11):
+    """The uneven batch: raw size from {1024x800, 2048x1536, 4080x3068} and lens pitch from
+    {15, 23, 31} per frame, multi-focus 3-type hex grid with 6/12/18 neighbours by lens type,
+    6-12 seeded planes with virtual depth in [2, 8], 6 iterations."""
+    rng = np.random.default_rng(seed)
+    sizes = [(1024, 800), (2048, 1536), (4080, 3068)]
+    radius = {15.0: 5, 23.0: 7, 31.0: 9}
+    wls = []
+    for i in range(n):
+        w, h = sizes[int(rng.integers(3))]
+        p = [15.0, 23.0, 31.0][int(rng.integers(3))]
+        wls.append(lfpm.workload_c2(raw_w=w, raw_h=h, pitch=p, radius=radius[p],
+                                    planes=int(rng.integers(6, 13)), lens_types=3,
+                                    per_type=(6, 12, 18), scene_seed=1000 + i,
+                                    noise_seed=2000 + i, seed=1000 + i, frame_uid=i))
+    return wls
+
+
+def serve_section(args, lfpm, torch):
+    import serving
+    sizes = [(1024, 800), (2048, 1536), (4080, 3068)]
+    radius = {15.0: 5, 23.0: 7, 31.0: 9}
+
+    def make(w, h, p, planes, s):
+        wl = lfpm.workload_c2(raw_w=w, raw_h=h, pitch=p, radius=radius[p], planes=planes,
+                              scene_seed=s, noise_seed=s + 1, seed=s, frame_uid=s)
+        wl.raw()
+        return wl
+    log("bench: synthesising %d requests" % args.requests)
+    reqs = serving.poisson_requests(args.requests, args.rate, 7, sizes, [15.0, 23.0, 31.0], make)
+    srv = serving.Server(hbm_budget_bytes=args.hbm_budget_gb * 2 ** 30, max_slots=16)
+    t0 = time.perf_counter()
+    done = srv.run(reqs, log=log)
+    span = time.perf_counter() - t0
+    st = serving.latency_stats(done)
+    return dict(fps=len(done) / span, span_s=span, requests=len(done), steps=srv.steps,
+                mean_batch=float(np.mean(srv.batch_sizes)), **st)
+
+
+def precision_section(args, lfpm, torch):
+    """FP32 vs FP16 LDS storage against the FP64 oracle: 10^6 seeded hypotheses (normals within
+    60 degrees of the optical axis, virtual depth in [2, 8]) on the 4080x3068 frame, and the
+    full-solve disparity agreement on a 1024x800 frame with the same solve parameters."""
+    import copy
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    wl = lfpm.workload_c2()
+    hyp = lfpm.cone_hypotheses(wl, args.hypotheses, seed=2024)
+    tiles = oracle.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    opm = oracle.PmParams(*[getattr(wl.pm, f[0]) for f in oracle.PmParams._fields_])
+    log("bench: oracle FP64 costs for %d hypotheses" % len(hyp))
+    t0 = time.perf_counter()
+    ref = oracle.eval_costs(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], opm, hyp,
+                            threads=threads)
+    t_orc = time.perf_counter() - t0
+    df = lfpm.DeviceFrame(wl)
+    hyp_d = torch.from_numpy(hyp).cuda()
+    out = {"hypotheses": len(hyp), "oracle_s": t_orc, "oracle_threads": threads}
+    for name, storage in (("fp32", 0), ("fp16", 1)):
+        pm = copy.copy(wl.pm)
+        pm.storage = storage
+        lfpm.eval_costs(df, hyp_d, pm=pm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g = lfpm.eval_costs(df, hyp_d, pm=pm)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        g = g.cpu().numpy().astype(np.float64)
+        both = (g < 2) & (ref < 2)
+        e = np.abs(g[both] - ref[both]) / np.maximum(np.abs(ref[both]), 1e-3)
+        out[name] = {"views_scored": int(both.sum()),
+                     "flip_frac": float(((g < 2) != (ref < 2)).mean()),
+                     "rel_err_median": float(np.median(e)), "rel_err_p99": float(np.quantile(e, 0.99)),
+                     "rel_err_p999": float(np.quantile(e, 0.999)), "rel_err_max": float(e.max()),
+                     "frac_le_1e5": float((e <= 1e-5).mean()), "frac_le_2e3": float((e <= 2e-3).mean()),
+                     "gpu_ms": dt * 1e3, "hyp_views_per_s": both.size / dt}
+        log("bench: %s %s" % (name, out[name]))
+    # full-solve disparity agreement on a 1024x800 frame with the 4080x3068 solve parameters
+    sub = lfpm.workload_c2(raw_w=1024, raw_h=800)
+    stiles = oracle.slice_u8(sub.raw(), sub.L["origins"], sub.crop)
+    log("bench: oracle FP64 solve of the 1024x800 frame (%d MIs)" % sub.V)
+    t0 = time.perf_counter()
+    o = oracle.solve(stiles, sub.L["centers"], sub.L["nbr"], sub.L["cnt"],
+                     oracle.PmParams(*[getattr(sub.pm, f[0]) for f in oracle.PmParams._fields_]),
+                     threads=threads)
+    out["oracle_solve_s"] = time.perf_counter() - t0
+    sdf = lfpm.DeviceFrame(sub)
+    valid = o["cost"] <= 1.8
+    for name, storage in (("fp32", 0), ("fp16", 1)):
+        pm = copy.copy(sub.pm)
+        pm.storage = storage
+        bufs = lfpm.patchmatch_frame(sdf, pm=pm)
+        torch.cuda.synchronize()
+        d = bufs.plane[:, 3].cpu().numpy().astype(np.float64)
+        err = np.abs(d - o["plane"][:, 3])
+        out[name]["solve_frac_le_005px"] = float((err[valid] <= 0.05).mean())
+        out[name]["solve_median_err_px"] = float(np.median(err[valid]))
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is a design note only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import lfpm
+    rw, rh = (int(x) for x in args.raw.split("x"))
+    line = {"metric": METRIC, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic"}
+    if args.mode == "precision":
+        r = precision_section(args, lfpm, torch)
+        line.update(value=r["fp16"]["hyp_views_per_s"], unit="hypothesis-views/s", steps=1,
+                    ms_per_step=r["fp16"]["gpu_ms"], precision=r,
+                    config={"workload": "10^6 seeded hypotheses on the 4080x3068 frame (18 views, "
+                                        "15x15 window), FP32 and FP16 LDS tiles vs the FP64 "
+                                        "oracle; full-solve agreement on a 1024x800 frame"})
+        line["dtype"] = "fp16 storage / fp32 arithmetic"
+        print(json.dumps(line), flush=True)
+        return
+    if args.mode == "serve":
+        r = serve_section(args, lfpm, torch)
+        line.update(value=r["fps"], ms_per_step=1e3 * r["span_s"] / max(1, r["steps"]),
+                    p50_ms=r["p50_ms"], p99_ms=r["p99_ms"], serve=r,
+                    config={"workload": "continuous-batching server: Poisson %.0f req/s (seed 7), "
+                                        "raw from {1024x800, 2048x1536, 4080x3068}, pitch from "
+                                        "{15, 23, 31}, 6 iterations + cycle check + 32-slice "
+                                        "refocus per request" % args.rate,
+                            "requests": args.requests, "hbm_budget_gb": args.hbm_budget_gb})
+        print(json.dumps(line), flush=True)
+        return
+    log("bench: synthesising %dx%d frame" % (rw, rh))
+    wl = lfpm.workload_c2(raw_w=rw, raw_h=rh)
+    log("bench: V=%d crop=%d K=%d iters=%d" % (wl.V, wl.crop, wl.pm.max_neighbors,
+                                                wl.pm.iterations))
+    plain = plain_section(args, lfpm, torch, wl)
+    if args.calib_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    cfg = {"raw": "%dx%d" % (rw, rh), "pitch": 23.0, "mis": wl.V,
+           "neighbors": wl.pm.max_neighbors, "window": 2 * wl.pm.radius + 1,
+           "iterations": wl.pm.iterations, "slices": wl.slices, "mode": args.mode}
+    line.update(plain_fps=plain["fps"], plain_p50_ms=float(np.percentile(plain["lat"], 50)),
+                plain_p99_ms=float(np.percentile(plain["lat"], 99)), kernel_ms=plain["kt"],
+                work=plain.get("work"), roofline=plain.get("roofline"))
+    if args.mode == "plain":
+        line.update(value=plain["fps"], ms_per_step=plain["ms_per_step"],
+                    p50_ms=line["plain_p50_ms"], p99_ms=line["plain_p99_ms"])
+        cfg["workload"] = ("plain GPU single-frame solve (init + %d red-black iterations + disp "
+                           "record) + CheckModule filter + %d-slice refocus, one launch per "
+                           "phase with a sync after each" % (wl.pm.iterations, wl.slices))
+    else:
+        if args.mode == "mixed":
+            log("bench: synthesising %d mixed frames" % args.batch)
+            wls = mixed_workloads(lfpm, args.batch)
+            cfg.update(raw="mixed {1024x800, 2048x1536, 4080x3068}", pitch="mixed {15, 23, 31}",
+                       neighbors="6/12/18 by lens type")
+            cfg["workload"] = ("%d uneven frames in one captured hipGraph: solve + filter + "
+                               "refocus per frame" % args.batch)
+        else:
+            log("bench: synthesising %d frames" % args.batch)
+            wls = [wl] + [lfpm.workload_c2(raw_w=rw, raw_h=rh, scene_seed=4321 + 17 * i,
+                                           noise_seed=4322 + 17 * i, frame_uid=i)
+                          for i in range(1, args.batch)]
+            cfg["workload"] = ("%d frames of %dx%d in one captured hipGraph (persistent k_stage "
+                               "launches over all frames' MIs) + per-frame CheckModule filter + "
+                               "%d-slice refocus" % (args.batch, rw, rh, wl.slices))
+        bat = batched_section(args, lfpm, torch, wls)
+        line.update(value=bat["fps"], ms_per_step=bat["ms_per_step"],
+                    p50_ms=float(np.percentile(bat["lat"], 50)),
+                    p99_ms=float(np.percentile(bat["lat"], 99)),
+                    batched_frames=bat["frames"], speedup_vs_plain=bat["fps"] / plain["fps"])
+        if args.mode == "batched":
+            line["batched_equals_plain"] = same_results(plain["result"], bat["result"], torch)
+        cfg["mode"] = args.mode
+    line["config"] = {"workload": cfg.pop("workload"), **cfg}
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
-        cpu = cpu_baseline(lfpm)
-    line = {
-        "metric": METRIC, "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": t_all * 1e3 / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
-        "data": "synthetic",
-        "config": {"workload": "plain GPU single-frame solve (init + %d red-black iterations + "
-                               "disp record) + CheckModule filter + %d-slice refocus"
-                               % (wl.pm.iterations, wl.slices),
-                   "raw": "%dx%d" % (rw, rh), "pitch": 23.0, "mis": wl.V,
-                   "neighbors": wl.pm.max_neighbors, "window": 2 * wl.pm.radius + 1,
-                   "iterations": wl.pm.iterations, "slices": wl.slices, "mode": args.mode},
-        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-        "kernel_ms": kt, "work": work, "roofline": roof, "cpu_baseline": cpu,
-    }
+        line["cpu_baseline"] = cpu_baseline(lfpm)
+    else:
+        line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
 
 

@@ -339,6 +339,15 @@ int lfpm_ctx_solve_frame(lfpm_ctx* ctx, const lfpm_layout_params* lp, const uint
                          const lfpm_pm_params* p, const lfpm_filter_params* f,
                          float* plane, float* cost, float* disp, uint32_t* sel, uint8_t* reason);
 
+/* Solve caller-supplied MI crops (the reference's MLA_Problem gray images, in view order) with a
+ * caller-supplied neighbour table — the data model of AdaptMIPMFrame::BuildViewIndexMap
+ * (MIStereo/AdaptMIPMFrame.cpp:145-222).  tiles_u8: V*crop*crop host bytes; centers 2V floats;
+ * neighbor_ids V*K; neighbor_counts V (all host).  Outputs as lfpm_ctx_solve_frame. */
+int lfpm_ctx_solve_tiles(lfpm_ctx* ctx, const uint8_t* tiles_u8, const float* centers,
+                         const int* neighbor_ids, const int* neighbor_counts, int num_mis,
+                         const lfpm_pm_params* p, const lfpm_filter_params* f, float* plane,
+                         float* cost, float* disp, uint32_t* sel, uint8_t* reason);
+
 #ifdef __cplusplus
 }
 #endif

@@ -186,4 +186,86 @@ int lfpm_ctx_solve_frame(lfpm_ctx* c, const lfpm_layout_params* lp, const uint8_
     return LFPM_OK;
 }
 
+int lfpm_ctx_solve_tiles(lfpm_ctx* c, const uint8_t* tiles_u8, const float* centers,
+                         const int* neighbor_ids, const int* neighbor_counts, int num_mis,
+                         const lfpm_pm_params* p, const lfpm_filter_params* f, float* plane,
+                         float* cost, float* disp, uint32_t* sel, uint8_t* reason) {
+    if (!c || !tiles_u8 || !centers || !neighbor_ids || !neighbor_counts || !p || num_mis <= 0)
+        return set_error(LFPM_E_INVALID, "solve_tiles: null argument");
+    CTX_TRY(hipSetDevice(c->device));
+    const int V = num_mis, W = p->crop, K = p->max_neighbors;
+    if (W < 3 || W > LFPM_MAX_CROP || K < 1 || K > LFPM_MAX_NEIGHBORS)
+        return set_error(LFPM_E_INVALID, "solve_tiles: bad crop or K");
+    // the crops stacked vertically form a crop-wide "raw" image; the tile gather then reads
+    // crop v at origin (0, v*crop)
+    lfpm_layout_params lp{};
+    lp.pitch = p->base; lp.crop = W; lp.raw_w = W; lp.raw_h = V * W; lp.max_neighbors = K;
+    lp.lens_types = -V;  // cache key distinct from any frame layout
+    free_layout(c);
+    const size_t npx = (size_t)V * W * W;
+    c->raw_bytes = npx;
+    std::vector<int> origins(2 * (size_t)V);
+    for (int v = 0; v < V; ++v) { origins[2 * v] = 0; origins[2 * v + 1] = v * W; }
+    CTX_TRY(hipMalloc((void**)&c->raw, npx));
+    CTX_TRY(hipMalloc((void**)&c->centers, 8 * (size_t)V));
+    CTX_TRY(hipMalloc((void**)&c->origins, 8 * (size_t)V));
+    CTX_TRY(hipMalloc((void**)&c->nbr, 4 * (size_t)V * K));
+    CTX_TRY(hipMalloc((void**)&c->cnt, 4 * (size_t)V));
+    CTX_TRY(hipMalloc((void**)&c->tiles, npx * 4));
+    CTX_TRY(hipMalloc((void**)&c->gray, npx * 4));
+    CTX_TRY(hipMalloc((void**)&c->plane, npx * 16));
+    CTX_TRY(hipMalloc((void**)&c->cost, npx * 4));
+    CTX_TRY(hipMalloc((void**)&c->disp, npx * 16));
+    CTX_TRY(hipMalloc((void**)&c->sel, npx * 4));
+    CTX_TRY(hipMalloc((void**)&c->fplane, npx * 16));
+    CTX_TRY(hipMalloc((void**)&c->fcost, npx * 4));
+    CTX_TRY(hipMalloc((void**)&c->fdisp, npx * 16));
+    CTX_TRY(hipMalloc((void**)&c->fsel, npx * 4));
+    CTX_TRY(hipMalloc((void**)&c->reason, npx));
+    c->ws_bytes = lfpm_patchmatch_workspace_size(V, p);
+    if (c->ws_bytes == 0) return set_error(LFPM_E_INVALID, "solve_tiles: bad solve parameters");
+    CTX_TRY(hipMalloc(&c->ws, c->ws_bytes));
+    c->V = V; c->crop = W; c->K = K;
+    CTX_TRY(hipMemcpyAsync(c->raw, tiles_u8, npx, hipMemcpyHostToDevice, c->stream));
+    CTX_TRY(hipMemcpyAsync(c->centers, centers, 8 * (size_t)V, hipMemcpyHostToDevice, c->stream));
+    CTX_TRY(hipMemcpyAsync(c->origins, origins.data(), 8 * (size_t)V, hipMemcpyHostToDevice,
+                           c->stream));
+    CTX_TRY(hipMemcpyAsync(c->nbr, neighbor_ids, 4 * (size_t)V * K, hipMemcpyHostToDevice,
+                           c->stream));
+    CTX_TRY(hipMemcpyAsync(c->cnt, neighbor_counts, 4 * (size_t)V, hipMemcpyHostToDevice,
+                           c->stream));
+    int rc;
+    if ((rc = lfpm_slice_tiles(c->raw, W, V * W, (size_t)W, c->origins, V, W, c->tiles,
+                               c->stream)))
+        return rc;
+    if ((rc = lfpm_patchmatch_frame(c->tiles, c->centers, c->nbr, c->cnt, V, p, c->plane,
+                                    c->cost, c->disp, c->sel, c->ws, c->ws_bytes, 0, c->stream)))
+        return rc;
+    float* po = c->plane;
+    float* co = c->cost;
+    float* dout = c->disp;
+    uint32_t* so = c->sel;
+    if (reason) {
+        if (!f) return set_error(LFPM_E_INVALID, "solve_tiles: reason requested without filter");
+        if ((rc = lfpm_slice_gray(c->raw, W, V * W, (size_t)W, c->origins, V, W, c->gray,
+                                  c->stream)))
+            return rc;
+        if ((rc = lfpm_filter_frame(c->gray, c->centers, c->nbr, c->cnt, V, W, K, p->base, f,
+                                    c->plane, c->cost, c->disp, c->sel, c->fplane, c->fcost,
+                                    c->fdisp, c->fsel, c->reason, c->stream)))
+            return rc;
+        po = c->fplane; co = c->fcost; dout = c->fdisp; so = c->fsel;
+    }
+    if (plane) CTX_TRY(hipMemcpyAsync(plane, po, npx * 16, hipMemcpyDeviceToHost, c->stream));
+    if (cost) CTX_TRY(hipMemcpyAsync(cost, co, npx * 4, hipMemcpyDeviceToHost, c->stream));
+    if (disp) CTX_TRY(hipMemcpyAsync(disp, dout, npx * 16, hipMemcpyDeviceToHost, c->stream));
+    if (sel) CTX_TRY(hipMemcpyAsync(sel, so, npx * 4, hipMemcpyDeviceToHost, c->stream));
+    if (reason)
+        CTX_TRY(hipMemcpyAsync(reason, c->reason, npx, hipMemcpyDeviceToHost, c->stream));
+    CTX_TRY(hipStreamSynchronize(c->stream));
+    c->lp = lp;
+    c->have_layout = false;  // buffers belong to this tile set; the next frame call re-plans
+    return LFPM_OK;
+}
+
 }  // extern "C"

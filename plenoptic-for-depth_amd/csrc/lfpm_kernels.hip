@@ -6,6 +6,7 @@
 // tiles stay in that XCD's L2).  The MI's reference tile and its K neighbour tiles are staged
 // in LDS; one lane owns one pixel of the active red/black colour and evaluates all of that
 // pixel's hypotheses; every bilinear gather of the bilateral NCC is an LDS read.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +26,7 @@ namespace lfpm {
 struct DevParams {
     int W, WW, K, r, inc, top_k, V;
     int W1, TS;          // padded tile row stride (W+1) and slot stride (W+1)^2 in LDS
+    int tile_bytes;      // 4 (FP32 tiles in LDS) or 2 (FP16 tiles in LDS)
     float inv_base;      // 1 / max(Base, 1e-6)
     float inv_ss;        // 1 / (2 sigma_s^2)
     float inv_sc;        // 1 / (2 sigma_c^2)
@@ -41,6 +43,7 @@ static DevParams make_dev_params(const lfpm_pm_params* p, int V) {
     d.WW = p->crop * p->crop;
     d.W1 = p->crop + 1;
     d.TS = (p->crop + 1) * (p->crop + 1);
+    d.tile_bytes = p->storage == 1 ? 2 : 4;
     d.K = p->max_neighbors;
     d.r = p->radius;
     d.inc = std::max(1, p->radius_increment);
@@ -72,7 +75,8 @@ static int check_pm_params(const lfpm_pm_params* p, int V) {
     if (p->top_k < 1) return set_error(LFPM_E_INVALID, "top_k must be >= 1");
     if (!(p->sigma_spatial > 0.0f) || !(p->sigma_color > 0.0f))
         return set_error(LFPM_E_INVALID, "sigmas must be > 0");
-    if (p->storage != 0) return set_error(LFPM_E_UNSUPPORTED, "storage mode %d", p->storage);
+    if (p->storage != 0 && p->storage != 1)
+        return set_error(LFPM_E_UNSUPPORTED, "storage mode %d", p->storage);
     return LFPM_OK;
 }
 
@@ -209,6 +213,7 @@ __device__ __forceinline__ float ncc_eval(const float* __restrict__ ref,
 // LDS carve-up shared by the solver kernels.
 struct Lds {
     float* tiles;     // (K+1) slots of (W+1)^2: slot 0 = reference, slot 1+k = neighbour k;
+    __half* htiles;   // the same region holding FP16 intensities (storage mode 1)
                       // column W and row W repeat the last column/row (clamp-to-edge for free)
     float* costp;     // WW (phase kernel: previous cost of this MI)
     float* spat;      // (2r+1)^2 spatial log-weights -|dxy| / (2 s_s^2)
@@ -222,7 +227,7 @@ struct Lds {
 static size_t lds_bytes(const DevParams& P, int NT, bool phase) {
     const int n = 2 * P.r + 1;
     size_t b = 0;
-    b += (size_t)(P.K + 1) * P.TS * 4;
+    b += (((size_t)(P.K + 1) * P.TS * P.tile_bytes) + 15) & ~(size_t)15;
     b += (size_t)P.WW * 4;
     b += (size_t)n * n * 4;
     b += (size_t)P.K * 8 * 2;
@@ -235,7 +240,9 @@ __device__ __forceinline__ Lds carve(unsigned char* base, const DevParams& P, in
     Lds L;
     const int n = 2 * P.r + 1;
     unsigned char* q = base;
-    L.tiles = (float*)q; q += (size_t)(P.K + 1) * P.TS * 4;
+    L.tiles = (float*)q;
+    L.htiles = (__half*)q;
+    q += (((size_t)(P.K + 1) * P.TS * P.tile_bytes) + 15) & ~(size_t)15;
     L.costp = (float*)q; q += (size_t)P.WW * 4;
     L.spat = (float*)q; q += (size_t)n * n * 4;
     L.step = (float2*)q; q += (size_t)P.K * 8;
@@ -246,6 +253,17 @@ __device__ __forceinline__ Lds carve(unsigned char* base, const DevParams& P, in
     return L;
 }
 
+// Tile element access for the two LDS storage modes; arithmetic is FP32 in both.
+template <typename TT> __device__ __forceinline__ const TT* tile_base(const Lds& L);
+template <> __device__ __forceinline__ const float* tile_base<float>(const Lds& L) { return L.tiles; }
+template <> __device__ __forceinline__ const __half* tile_base<__half>(const Lds& L) {
+    return L.htiles;
+}
+__device__ __forceinline__ float ldt(const float* p) { return *p; }
+__device__ __forceinline__ float ldt(const __half* p) { return __half2float(*p); }
+__device__ __forceinline__ void stt(float* p, float v) { *p = v; }
+__device__ __forceinline__ void stt(__half* p, float v) { *p = __float2half_rn(v); }
+
 __device__ __forceinline__ void fill_spatial(Lds& L, const DevParams& P) {
     const int n = 2 * P.r + 1;
     for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
@@ -255,6 +273,7 @@ __device__ __forceinline__ void fill_spatial(Lds& L, const DevParams& P) {
 }
 
 // Stage the MI's tables and tiles (reference + its neighbours) into LDS.  Caller syncs.
+template <typename TT>
 __device__ __forceinline__ int stage_mi(Lds& L, const DevParams& P, int mi,
                                         const float* __restrict__ tiles,
                                         const float2* __restrict__ centers,
@@ -285,7 +304,7 @@ __device__ __forceinline__ int stage_mi(Lds& L, const DevParams& P, int mi,
             src = nbr[(size_t)mi * K + s - 1];
             if (src < 0) src = mi;  // never read: views with nid < 0 are not scored
         }
-        L.tiles[idx] = tiles[(size_t)src * WW + yy * W + xx];
+        stt((TT*)tile_base<TT>(L) + idx, tiles[(size_t)src * WW + yy * W + xx]);
     }
     return c;
 }
@@ -296,7 +315,7 @@ __device__ __forceinline__ int stage_mi(Lds& L, const DevParams& P, int mi,
 // Same definition as ncc_eval (ComputeBilateralNCC_Frame, AdaptMIPMFrame.cu:73-179).
 constexpr int kGroup = 6;
 
-template <bool COUNT>
+template <bool COUNT, typename TT>
 __device__ __forceinline__ void ncc_group(const Lds& L, const DevParams& P, int px, int py,
                                           float rc, float4 h, const int (&vk)[kGroup],
                                           float (&out)[kGroup], WorkCount& wc) {
@@ -343,7 +362,7 @@ __device__ __forceinline__ void ncc_group(const Lds& L, const DevParams& P, int 
             const int ry = py + j;
             if ((unsigned)ry >= (unsigned)W) continue;
             const float d = di + B * (float)j;
-            const float x = L.tiles[ry * W1 + rx] - rc;
+            const float x = ldt(tile_base<TT>(L) + ry * W1 + rx) - rc;
             const float wgt = __expf(spat_i[(j + r) * n] - fabsf(x) * P.inv_sc);
             const float wx = wgt * x;
             const float wxx = wx * x;
@@ -357,8 +376,8 @@ __device__ __forceinline__ void ncc_group(const Lds& L, const DevParams& P, int 
                 const float fx = qx - fx0, fy = qy - fy0;
                 const float xa = __builtin_amdgcn_fmed3f(fx0, 0.0f, Wm1);
                 const float ya = __builtin_amdgcn_fmed3f(fy0, 0.0f, Wm1);
-                const float* t = L.tiles + base[g] + (int)(ya * W1f + xa);
-                const float a = t[0], b = t[1], c = t[W1], e = t[W1 + 1];
+                const TT* t = tile_base<TT>(L) + base[g] + (int)(ya * W1f + xa);
+                const float a = ldt(t), b = ldt(t + 1), c = ldt(t + W1), e = ldt(t + W1 + 1);
                 const float top = a + fx * (b - a);
                 const float bot = c + fx * (e - c);
                 const float y = (top + fy * (bot - top)) - rc;
@@ -545,7 +564,7 @@ __device__ __forceinline__ void active_pixel(int W, int a, int phase, int& x, in
 
 
 // Random initialisation of every pixel of MI v (RandomInitializationForFrame :474-518).
-template <bool COUNT>
+template <bool COUNT, typename TT>
 __device__ __forceinline__ void init_mi(const Lds& L, const DevParams& P, const FrameDesc& F,
                                         int v, int c, float* __restrict__ scr, WorkCount& wc) {
     const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW;
@@ -556,7 +575,7 @@ __device__ __forceinline__ void init_mi(const Lds& L, const DevParams& P, const 
         float4 h;
         h.w = u01_dev(st.raw(SLOT_INIT_DEPTH)) * (P.dmax - P.dmin) + P.dmin;
         random_normal(st, SLOT_INIT_MARSAGLIA, h.x, h.y, h.z);
-        const float rc = L.tiles[y * P.W1 + x];
+        const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
         int valid = 0, best = -1;
         float bestc = 2.0f;
         for (int k0 = 0; k0 < c; k0 += kGroup) {
@@ -565,7 +584,7 @@ __device__ __forceinline__ void init_mi(const Lds& L, const DevParams& P, const 
 #pragma unroll
             for (int g = 0; g < kGroup; ++g)
                 vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
-            ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+            ncc_group<COUNT, TT>(L, P, x, y, rc, h, vk, cg, wc);
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
                 const int k = k0 + g;
@@ -609,7 +628,7 @@ __device__ __forceinline__ void init_mi(const Lds& L, const DevParams& P, const 
 
 // Lowest-cost view record of the final plane over every view (the disp argument of
 // ComputeWeightedCost_Frame, AdaptMIPMFrame.cu:303-306).
-template <bool COUNT>
+template <bool COUNT, typename TT>
 __device__ __forceinline__ void final_mi(const Lds& L, const DevParams& P, const FrameDesc& F,
                                          int v, int c, WorkCount& wc) {
     const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW;
@@ -617,7 +636,7 @@ __device__ __forceinline__ void final_mi(const Lds& L, const DevParams& P, const
         const int x = pix % W, y = pix / W;
         const size_t idx = (size_t)v * WW + pix;
         const float4 h = F.plane_in[idx];
-        const float rc = L.tiles[y * P.W1 + x];
+        const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
         float mc = 2.0f;
         int am = -1;
         for (int k0 = 0; k0 < c; k0 += kGroup) {
@@ -626,7 +645,7 @@ __device__ __forceinline__ void final_mi(const Lds& L, const DevParams& P, const
 #pragma unroll
             for (int g = 0; g < kGroup; ++g)
                 vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
-            ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+            ncc_group<COUNT, TT>(L, P, x, y, rc, h, vk, cg, wc);
 #pragma unroll
             for (int g = 0; g < kGroup; ++g)
                 if (k0 + g < c && cg[g] < mc) { mc = cg[g]; am = k0 + g; }
@@ -636,7 +655,7 @@ __device__ __forceinline__ void final_mi(const Lds& L, const DevParams& P, const
 }
 
 // One red/black phase of MI v: the active lane t owns one pixel of colour F.phase.
-template <bool COUNT>
+template <bool COUNT, typename TT>
 __device__ __forceinline__ void phase_mi(const Lds& L, const DevParams& P, const FrameDesc& F,
                                          int v, int c, float* __restrict__ scr, WorkCount& wc) {
     const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW, K = P.K;
@@ -657,7 +676,7 @@ __device__ __forceinline__ void phase_mi(const Lds& L, const DevParams& P, const
     active_pixel(W, t, phase, x, y);
     const int center = y * W + x;
     const size_t idx = base + center;
-    const float rc = L.tiles[y * P.W1 + x];
+    const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
     // candidate table packed into registers (no dynamically indexed private arrays):
     // pos of slot cd in bits 10*cd.. of a 128-bit pair, src+1 in bits 4*cd.. of srcp
     uint64_t posp[2] = {0ull, 0ull};
@@ -686,7 +705,7 @@ __device__ __forceinline__ void phase_mi(const Lds& L, const DevParams& P, const
 #pragma unroll
             for (int g = 0; g < kGroup; ++g)
                 vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
-            ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+            ncc_group<COUNT, TT>(L, P, x, y, rc, h, vk, cg, wc);
 #pragma unroll
             for (int g = 0; g < kGroup; ++g)
                 if (k0 + g < c) scr[(size_t)(cd * K + k0 + g) * NT + t] = cg[g];
@@ -773,7 +792,7 @@ __device__ __forceinline__ void phase_mi(const Lds& L, const DevParams& P, const
                     if (L.nid[k] >= 0) vk[g] = k;
                 }
             }
-            ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+            ncc_group<COUNT, TT>(L, P, x, y, rc, h, vk, cg, wc);
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
                 if (j0 + g >= nsel) break;
@@ -873,7 +892,7 @@ __device__ __forceinline__ void phase_mi(const Lds& L, const DevParams& P, const
     F.sel_out[idx] = tsel;
 }
 
-template <bool COUNT>
+template <bool COUNT, typename TT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_stage(StageArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -895,13 +914,13 @@ k_stage(StageArgs a) {
         if (stage == STAGE_IDLE) continue;  // uniform across the workgroup
         const int v = gm - F.mi_begin;
         __syncthreads();
-        const int c = stage_mi(L, P, v, F.tiles, F.centers, F.nbr, F.cnt);
+        const int c = stage_mi<TT>(L, P, v, F.tiles, F.centers, F.nbr, F.cnt);
         if (stage == STAGE_PHASE)
             for (int i = t; i < WW; i += NT) L.costp[i] = F.cost_in[(size_t)v * WW + i];
         __syncthreads();
-        if (stage == STAGE_INIT) init_mi<COUNT>(L, P, F, v, c, scr, wc);
-        else if (stage == STAGE_PHASE) phase_mi<COUNT>(L, P, F, v, c, scr, wc);
-        else final_mi<COUNT>(L, P, F, v, c, wc);
+        if (stage == STAGE_INIT) init_mi<COUNT, TT>(L, P, F, v, c, scr, wc);
+        else if (stage == STAGE_PHASE) phase_mi<COUNT, TT>(L, P, F, v, c, scr, wc);
+        else final_mi<COUNT, TT>(L, P, F, v, c, wc);
     }
     flush_counts<COUNT>(a.counters, wc);
 }
@@ -930,7 +949,7 @@ __global__ void k_segments(const int* __restrict__ hyp, int n, int V, int* __res
     seg[v] = lo;
 }
 
-template <bool COUNT>
+template <bool COUNT, typename TT>
 __global__ void __launch_bounds__(256) k_eval(EvalArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const DevParams& P = a.P;
@@ -944,7 +963,7 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs a) {
         const int h0 = a.seg[mi], h1 = a.seg[mi + 1];
         if (h0 >= h1) continue;  // uniform across the workgroup
         __syncthreads();
-        const int c = stage_mi(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
+        const int c = stage_mi<TT>(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
         __syncthreads();
         for (int hi = h0 + t; hi < h1; hi += NT) {
             const int* rec = a.hyp + (size_t)hi * 6;
@@ -956,14 +975,14 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs a) {
             const float4 h = make_float4(__int_as_float(rec[2]), __int_as_float(rec[3]),
                                          __int_as_float(rec[4]), __int_as_float(rec[5]));
             const int x = pix % W, y = pix / W;
-            const float rc = L.tiles[y * P.W1 + x];
+            const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
             for (int k0 = 0; k0 < K; k0 += kGroup) {
                 int vk[kGroup];
                 float cg[kGroup];
 #pragma unroll
                 for (int g = 0; g < kGroup; ++g)
                     vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
-                ncc_group<COUNT>(L, P, x, y, rc, h, vk, cg, wc);
+                ncc_group<COUNT, TT>(L, P, x, y, rc, h, vk, cg, wc);
 #pragma unroll
                 for (int g = 0; g < kGroup; ++g)
                     if (k0 + g < K) a.out[(size_t)hi * K + k0 + g] = cg[g];
@@ -1400,10 +1419,14 @@ static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_
     a.scratch = (float*)scratch;
     a.counters = g_counters;
     const size_t lds = lds_bytes(a.P, NT, true);
-    if (g_counters)
-        hipLaunchKernelGGL(k_stage<true>, dim3(grid), dim3(NT), lds, s, a);
-    else
-        hipLaunchKernelGGL(k_stage<false>, dim3(grid), dim3(NT), lds, s, a);
+    const bool fp16 = a.P.tile_bytes == 2;
+    if (g_counters) {
+        if (fp16) hipLaunchKernelGGL((k_stage<true, __half>), dim3(grid), dim3(NT), lds, s, a);
+        else hipLaunchKernelGGL((k_stage<true, float>), dim3(grid), dim3(NT), lds, s, a);
+    } else {
+        if (fp16) hipLaunchKernelGGL((k_stage<false, __half>), dim3(grid), dim3(NT), lds, s, a);
+        else hipLaunchKernelGGL((k_stage<false, float>), dim3(grid), dim3(NT), lds, s, a);
+    }
     return launch_check("k_stage");
 }
 static size_t state_bytes(const DevParams& P) {
@@ -1659,10 +1682,15 @@ int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,
     a.cnt = neighbor_counts_dev; a.hyp = (const int*)hyp_dev; a.seg = seg; a.n = n;
     a.out = cost_out_dev; a.P = P; a.counters = g_counters;
     const int NT = 256;
-    if (g_counters)
-        hipLaunchKernelGGL(k_eval<true>, dim3(grid), dim3(NT), lds_bytes(P, NT, false), s, a);
-    else
-        hipLaunchKernelGGL(k_eval<false>, dim3(grid), dim3(NT), lds_bytes(P, NT, false), s, a);
+    const size_t lds = lds_bytes(P, NT, false);
+    const bool fp16 = P.tile_bytes == 2;
+    if (g_counters) {
+        if (fp16) hipLaunchKernelGGL((k_eval<true, __half>), dim3(grid), dim3(NT), lds, s, a);
+        else hipLaunchKernelGGL((k_eval<true, float>), dim3(grid), dim3(NT), lds, s, a);
+    } else {
+        if (fp16) hipLaunchKernelGGL((k_eval<false, __half>), dim3(grid), dim3(NT), lds, s, a);
+        else hipLaunchKernelGGL((k_eval<false, float>), dim3(grid), dim3(NT), lds, s, a);
+    }
     rc = launch_check("eval");
     HIP_TRY(hipFreeAsync(seg, s));
     return rc;

@@ -350,6 +350,21 @@ def hypotheses_from_arrays(mi, pix, planes):
     return rec
 
 
+def cone_hypotheses(wl, n, seed, vmin=2.0, vmax=8.0, cone_deg=60.0):
+    """n seeded (MI, pixel, plane) records sorted by MI: normals uniform on the spherical cap of
+    `cone_deg` about +z, virtual depth uniform in [vmin, vmax] -> disparity pitch / v."""
+    rng = np.random.default_rng(seed)
+    V, W = wl.V, wl.crop
+    mi = np.sort(rng.integers(0, V, n)).astype(np.int32)
+    pix = rng.integers(0, W * W, n).astype(np.int32)
+    cz = 1.0 - rng.random(n) * (1.0 - np.cos(np.radians(cone_deg)))
+    phi = 2 * np.pi * rng.random(n)
+    sz = np.sqrt(np.maximum(0.0, 1.0 - cz * cz))
+    v = vmin + (vmax - vmin) * rng.random(n)
+    planes = np.stack([sz * np.cos(phi), sz * np.sin(phi), cz, wl.L["pitch"] / v], 1)
+    return hypotheses_from_arrays(mi, pix, planes.astype(np.float32))
+
+
 def version():
     return "lfpm-0.1"
 

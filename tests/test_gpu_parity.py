@@ -42,15 +42,20 @@ def test_slice_exact(lfpm, orc, c1, torch_cuda):
     assert np.array_equal(df.gray.cpu().numpy(), tiles_u8.astype(np.float32))
 
 
-@pytest.mark.parametrize("cfg", ["c1", "c2params"])
+@pytest.mark.parametrize("cfg", ["c1", "c2params", "c2params_fp16"])
 def test_eval_costs_vs_fp64(lfpm, orc, torch_cuda, cfg):
     """Fixed-plane bilateral NCC (ComputeBilateralNCC_Frame) vs the FP64 oracle.  Relative error
-    |c - c*| / max(|c*|, 1e-3); both sides return exactly 2 for unscored views."""
+    |c - c*| / max(|c*|, 1e-3); both sides return exactly 2 for unscored views.  FP32 tiles:
+    >= 99.5 % within 1e-5 and >= 99.9 % within 1e-4.  FP16 tiles in LDS (FP32 arithmetic):
+    >= 99 % within 2e-3."""
     _say("case eval_costs", cfg)
     if cfg == "c1":
         wl = lfpm.workload_c1()
     else:  # C2 window/neighbour parameters on a 1024x800 frame
         wl = lfpm.workload_c2(raw_w=1024, raw_h=800)
+    fp16 = cfg.endswith("fp16")
+    if fp16:
+        wl.pm.storage = 1
     df = lfpm.DeviceFrame(wl)
     hyp = cone_hypotheses(lfpm, wl, 40000, seed=7)
     out = lfpm.eval_costs(df, torch_cuda.from_numpy(hyp).cuda()).cpu().numpy()
@@ -63,14 +68,16 @@ def test_eval_costs_vs_fp64(lfpm, orc, torch_cuda, cfg):
     st = dict(cfg=cfg, n=int(both.sum()), flip=float(flip), max=float(e.max()),
               p999=float(np.quantile(e, 0.999)), p99=float(np.quantile(e, 0.99)),
               median=float(np.median(e)), frac_le_1e5=float((e <= 1e-5).mean()),
-              frac_le_1e4=float((e <= 1e-4).mean()))
+              frac_le_1e4=float((e <= 1e-4).mean()), frac_le_2e3=float((e <= 2e-3).mean()))
     _say("eval_costs", st)
     stats_dump("eval_costs_" + cfg, st)
-    # FP32 vs FP64: >= 99.5 % of scored views within 1e-5 relative, >= 99.9 % within 1e-4; the
-    # tail are windows whose edge samples fall on opposite sides of the crop boundary
+    # the tail are windows whose edge samples fall on opposite sides of the crop boundary
     assert flip < 1e-3
-    assert st["frac_le_1e5"] >= 0.995
-    assert st["frac_le_1e4"] >= 0.999
+    if fp16:
+        assert st["frac_le_2e3"] >= 0.99
+    else:
+        assert st["frac_le_1e5"] >= 0.995
+        assert st["frac_le_1e4"] >= 0.999
 
 
 def _solve_gpu(lfpm, df, torch_cuda, pm=None):
@@ -109,6 +116,25 @@ def test_solve_c1_vs_fp64(lfpm, orc, c1, torch_cuda):
     _say("solve_c1", st)
     stats_dump("solve_c1", st)
     assert frac >= 0.995
+
+
+def test_solve_c1_fp16_storage_vs_fp64(lfpm, orc, torch_cuda):
+    """FP16 tiles in LDS: full-solve disparity agreement with the FP64 oracle, stated."""
+    import copy
+    wl = lfpm.workload_c1()
+    pm = copy.copy(wl.pm)
+    pm.storage = 1
+    df = lfpm.DeviceFrame(wl)
+    g, _ = _solve_gpu(lfpm, df, torch_cuda, pm=pm)
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    o = orc.solve(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm_to_oracle(orc, wl.pm))
+    valid = o["cost"] <= 1.8
+    err = np.abs(g["plane"][:, 3].astype(np.float64) - o["plane"][:, 3])
+    st = dict(frac_le_005=float((err[valid] <= 0.05).mean()),
+              frac_le_01=float((err[valid] <= 0.1).mean()), median=float(np.median(err)))
+    _say("solve_c1_fp16", st)
+    stats_dump("solve_c1_fp16", st)
+    assert st["frac_le_01"] >= 0.95
 
 
 def test_solve_deterministic(lfpm, c1, torch_cuda):
