@@ -186,15 +186,16 @@ def count_work(lfpm, df, bufs, fo, rf, torch):
     return out
 
 
-def measured_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary (scaled by the copy-kernel calibration), or None if absent."""
+def measured_traffic(row):
+    """HBM bytes per dispatch of the summary row `row` (kernel name and stage) from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by the copy-kernel calibration), or None if
+    absent."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_plain_4080x3068.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0].startswith("lfpm::" + kernel):
+        if len(cells) == 5 and cells[0] == row:
             try:
                 return float(cells[4])
             except ValueError:
@@ -204,10 +205,11 @@ def measured_traffic(kernel):
 
 def cpu_baseline(lfpm):
     """FP32 CPU restatement (oracle, OpenMP) of the same solve on a bounded sample: the same
-    parameters on a 400x320 sub-frame, scaled to a 4080x3068 frame by MI count."""
+    parameters on an 800x640 sub-frame (1088 MIs, about 10 s on 16 cores), scaled to a 4080x3068
+    frame by MI count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    wl = lfpm.workload_c2(raw_w=400, raw_h=320)
+    wl = lfpm.workload_c2(raw_w=800, raw_h=640)
     tiles = oracle.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
     pm = oracle.PmParams(*[getattr(wl.pm, f[0]) for f in oracle.PmParams._fields_])
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -218,7 +220,7 @@ def cpu_baseline(lfpm):
     full = lfpm.workload_c2()
     s_per_frame = dt * full.V / wl.V
     return {"value": 1.0 / s_per_frame, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": "FP32 oracle solve (init + 6 iterations, 18 neighbours, 15x15) of a 400x320 "
+            "sample": "FP32 oracle solve (init + 6 iterations, 18 neighbours, 15x15) of an 800x640 "
                       "sub-frame (%d MIs) in %.1f s, scaled by MI count to the %d-MI frame"
                       % (wl.V, dt, full.V)}
 
@@ -263,9 +265,11 @@ def plain_section(args, lfpm, torch, wl):
         achieved = flop_per_launch / (kt["phase"] * 1e-3) / 1e12
         out["work"] = work
         out["roofline"] = {
-            "bound": "valu", "kernel": "k_stage (red/black phase)", "achieved": achieved,
+            "bound": "mfma", "pipe": "FP32 VALU (compute-bound class; the NCC is not a "
+                                     "contraction, so no MFMA instructions are issued)",
+            "kernel": "k_stage (red/black phase)", "achieved": achieved,
             "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / (PEAK_FP32 / 1e12),
-            "traffic": measured_traffic("k_stage"), "avg_launch_ms": kt["phase"],
+            "traffic": measured_traffic("k_stage phase"), "avg_launch_ms": kt["phase"],
             "flop_per_launch": flop_per_launch,
             "per_unit": "22 FLOP per evaluated window sample + 16 per evaluated "
                         "(pixel, view, hypothesis) triple"}

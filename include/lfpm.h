@@ -121,7 +121,10 @@ typedef struct lfpm_pm_params {
     int max_neighbors;         /* K: neighbour table row stride */
     uint32_t seed;             /* Philox key word 0 */
     uint32_t frame_uid;        /* Philox key word 1: results depend on (seed, frame_uid) only */
-    int storage;               /* 0 = FP32 tiles in LDS, 1 = FP16 tiles in LDS (FP32 accumulate) */
+    int storage;               /* 0 = FP32 tiles in LDS; 1 = FP16 tiles in LDS holding the 8-bit
+                                  intensity codes 0..255 exactly (tiles must be k/255, as
+                                  lfpm_slice_tiles writes them; other values round to the nearest
+                                  1/255), FP32 arithmetic on the 0..255 scale */
     float cos_max_slant;       /* planes with |nz| < cos_max_slant score 2 (0 = no gate, the
                                   reference behaviour; default 0.5 = 60 degrees) */
 } lfpm_pm_params;
@@ -240,6 +243,42 @@ int lfpm_refocus(const float* tiles_dev, const float* centers_dev, const int* la
                  int lattice_cols, int lattice_rows, float pitch, int crop,
                  int out_w, int out_h, int num_slices, float vmin, float vmax,
                  float* out_dev, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Per-frame products.  Device workspace for either call: lfpm_products_workspace_size().
+ *
+ * lfpm_disparity_mosaic replaces DepthSolver::WriteDisMap_TileKey_new_AccuSequence
+ * (DepthSolver.cpp:6716-6897): pixel (x, y) of MI v lands at raw position
+ * trunc(c_v - (round(crop/2) - 1)) + (x, y) when its cost <= max_cost and its disparity
+ * d = plane.w > 0; mosaic_dev (raw_h*raw_w floats) holds that d (0 elsewhere; where crops
+ * overlap the highest MI index wins, the reference's last-writer order).  range_dev (2 floats,
+ * may be NULL) = {min, max} of every d > 0 (no cost gate), the reference's colour-scale range
+ * (+inf/-inf when empty).  The reference's uint8/JET renderings and its tolower() on the
+ * integer corner are not reproduced (the corner is plainly truncated).
+ *
+ * lfpm_virtual_depth_map replaces DepthSolver::Virtual_depth_map_TileKeySequence
+ * (DepthSolver.cpp:7832-8000): pixel (x, y) of MI v with d = plane.w (d <= 0 -> 1) splats
+ * f_v = mla_base / d at round(c_v + trunc((x, y) - ((crop-1)/2 + 1)) * f_v) when that position
+ * is strictly inside the frame.  vd_dev (raw_h*raw_w floats) = the median of each pixel's
+ * samples by the reference's index rule (s[n/2+1] + s[(n+1)/2]) / 2 on the sorted samples,
+ * indices clamped to n-1 (the rule reads past the end for n <= 2), 0 where none.
+ * color_dev (optional, raw_h*raw_w bytes): median gray (gray_dev, 0..255 floats, the colour
+ * image of the gray frames) with the reference's fill of empty pixels (probe order p+s, p-s,
+ * p+s*(raw_w-1), p-s*raw_w) and ten 5x5 median passes with replicated borders.
+ * range_dev (2 floats, may be NULL) = {min, max} of the medians.  mla_base = baseline *
+ * bl0 / (bl0 + B) of the reference; the synthetic workloads use the lens pitch.
+ * ------------------------------------------------------------------------------------- */
+size_t lfpm_products_workspace_size(int num_mis, int crop, int raw_w, int raw_h);
+
+int lfpm_disparity_mosaic(const float* plane_dev, const float* cost_dev, const float* centers_dev,
+                          int num_mis, int crop, float max_cost, int raw_w, int raw_h,
+                          float* mosaic_dev, float* range_dev, void* workspace_dev,
+                          size_t workspace_bytes, void* stream);
+
+int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, const float* gray_dev,
+                           int num_mis, int crop, float mla_base, int raw_w, int raw_h,
+                           float* vd_dev, uint8_t* color_dev, float* range_dev,
+                           void* workspace_dev, size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * Virtual-to-real depth behavioural model, FP64 on the GPU.

@@ -38,6 +38,11 @@
 //   fit_v2r()/v2r_convert() : VirtualToRealDepthFunc::BehavioralModel / convertVirtualToRealDepth,
 //                             FocalStack/VirtualToRealDepthFunc.cpp:3390-3451 (Householder QR
 //                             least squares in FP64 instead of cv::solve(DECOMP_SVD)).
+//   disparity_mosaic()      : DepthSolver::WriteDisMap_TileKey_new_AccuSequence,
+//                             DepthSolver.cpp:6716-6897 (float map instead of the renderings).
+//   vd_map()                : DepthSolver::Virtual_depth_map_TileKeySequence,
+//                             DepthSolver.cpp:7832-8000 (median splat, colour fill, 10 x
+//                             medianBlur 5x5 with replicated borders per OpenCV's documentation).
 //
 // Semantics the reference leaves undefined and that this restatement fixes (the product does
 // the same): planes whose normal is more than acos(cos_max_slant) from the optical axis score the
@@ -1118,6 +1123,121 @@ static int fit_v2r(const float* aL, const float* v, int n, double c[3]) {
     return 0;
 }
 
+// ------------------------------------------------------------------------------------------
+// Per-frame products, written as the reference's loops (DepthSolver.cpp:6716-6897 and
+// :7832-8000) over MIs in map order with per-pixel sample vectors, with the fixes the product
+// documents: plain truncation of the mosaic corner (no tolower), VD median indices clamped to
+// n-1, float outputs instead of the uint8 renderings.
+// ------------------------------------------------------------------------------------------
+static void disparity_mosaic(const float* plane, const float* cost, const float* centers, int V,
+                             int crop, float max_cost, int raw_w, int raw_h, float* out,
+                             float* range) {
+    std::fill(out, out + (size_t)raw_w * raw_h, 0.0f);
+    float dmin = INFINITY, dmax = -INFINITY;
+    const int WW = crop * crop;
+    for (int v = 0; v < V; ++v)
+        for (int i = 0; i < WW; ++i) {
+            const float d = plane[4 * ((size_t)v * WW + i) + 3];
+            if (d <= 0.0f) continue;
+            dmin = std::min(dmin, d);
+            dmax = std::max(dmax, d);
+        }
+    for (int v = 0; v < V; ++v) {
+        const float ox = (float)(centers[2 * v] - (std::round(crop * 0.5) - 1));
+        const float oy = (float)(centers[2 * v + 1] - (std::round(crop * 0.5) - 1));
+        for (int y = 0; y < crop; ++y)
+            for (int x = 0; x < crop; ++x) {
+                const size_t i = (size_t)v * WW + y * crop + x;
+                if (cost[i] > max_cost) continue;
+                const float d = plane[4 * i + 3];
+                if (d <= 0.0f) continue;
+                const int X = (int)ox + x, Y = (int)oy + y;
+                if (X < 0 || X >= raw_w || Y < 0 || Y >= raw_h) continue;
+                out[(size_t)Y * raw_w + X] = d;
+            }
+    }
+    if (range) { range[0] = dmin; range[1] = dmax; }
+}
+
+static void vd_map(const float* plane, const float* centers, const float* gray, int V, int crop,
+                   float mla_base, int raw_w, int raw_h, float* vd_out, uint8_t* color_out,
+                   float* range) {
+    const size_t N = (size_t)raw_w * raw_h;
+    std::vector<std::vector<float>> vd(N);
+    std::vector<std::vector<int>> col(N);
+    const int WW = crop * crop;
+    for (int v = 0; v < V; ++v)
+        for (int y = 0; y < crop; ++y)
+            for (int x = 0; x < crop; ++x) {
+                const size_t i = (size_t)v * WW + y * crop + x;
+                float disp = plane[4 * i + 3];
+                if (disp <= 0.0f) disp = 1;
+                const float f_v = mla_base / disp;
+                const int xm = (int)(x - ((crop - 1) * 0.5 + 1));
+                const float Xv = xm * f_v + centers[2 * v];
+                const int ym = (int)(y - ((crop - 1) * 0.5 + 1));
+                const float Yv = ym * f_v + centers[2 * v + 1];
+                const int vy = (int)std::round(Yv), vx = (int)std::round(Xv);
+                if (vx > 0 && vx < raw_w && vy > 0 && vy < raw_h) {
+                    vd[(size_t)vy * raw_w + vx].push_back(f_v);
+                    col[(size_t)vy * raw_w + vx].push_back(gray ? (int)(uint8_t)gray[i] : 0);
+                }
+            }
+    std::vector<float> vdm(N, 0.0f);
+    std::vector<uint8_t> cm(N, 0);
+    float vmin = INFINITY, vmax = -INFINITY;
+    for (size_t p = 0; p < N; ++p) {
+        const int n = (int)vd[p].size();
+        if (n == 0) continue;
+        std::sort(vd[p].begin(), vd[p].end());
+        std::sort(col[p].begin(), col[p].end());
+        const int i1 = std::min(n / 2 + 1, n - 1), i2 = std::min((n + 1) / 2, n - 1);
+        vdm[p] = (vd[p][i1] + vd[p][i2]) / 2;
+        cm[p] = (uint8_t)((col[p][n / 2] + col[p][(n + 1) / 2 - 1]) / 2);
+        vmin = std::min(vmin, vdm[p]);
+        vmax = std::max(vmax, vdm[p]);
+    }
+    for (size_t p = 0; p < N; ++p) vd_out[p] = vdm[p];
+    if (range) { range[0] = vmin; range[1] = vmax; }
+    if (!color_out) return;
+    std::vector<uint8_t> img(N, 0);
+    const long long rows = raw_h, cols = raw_w;
+    for (long long y = 0; y < rows; ++y)
+        for (long long x = 0; x < cols; ++x) {
+            const long long p = y * cols + x;
+            if (!vd[p].empty()) { img[p] = cm[p]; continue; }
+            long long t = 4, k = p;
+            while (true) {
+                switch (t % 4) {
+                    case 0: k = y * cols + x + t / 4; break;
+                    case 1: k = y * cols + x - t / 4; break;
+                    case 2: k = (y + t / 4) * cols + x - t / 4; break;
+                    case 3: k = (y - t / 4) * cols + x; break;
+                }
+                ++t;
+                if (k >= rows * cols || k < 0) break;
+                if (!vd[k].empty()) { img[p] = cm[k]; break; }
+            }
+        }
+    std::vector<uint8_t> tmp(N);
+    for (int it = 0; it < 10; ++it) {
+        for (long long y = 0; y < rows; ++y)
+            for (long long x = 0; x < cols; ++x) {
+                int w[25], n = 0;
+                for (int j = -2; j <= 2; ++j)
+                    for (int i = -2; i <= 2; ++i) {
+                        const long long yy = std::min(std::max(y + j, 0LL), rows - 1);
+                        const long long xx = std::min(std::max(x + i, 0LL), cols - 1);
+                        w[n++] = img[yy * cols + xx];
+                    }
+                std::nth_element(w, w + 12, w + 25);
+                tmp[y * cols + x] = (uint8_t)w[12];
+            }
+        img.swap(tmp);
+    }
+    std::copy(img.begin(), img.end(), color_out);
+}
+
 } // namespace orc
 
 // ------------------------------------------------------------------------------------------
@@ -1205,6 +1325,17 @@ void orc_random_normals_f64(uint32_t seed, uint32_t uid, uint32_t step, int n, d
         orc::Stream st(seed, uid, (uint32_t)i, step);
         orc::random_normal<double>(st, orc::SLOT_INIT_MARSAGLIA, out + 3 * (size_t)i);
     }
+}
+
+void orc_disparity_mosaic(const float* plane, const float* cost, const float* centers, int V,
+                          int crop, float max_cost, int raw_w, int raw_h, float* out,
+                          float* range) {
+    orc::disparity_mosaic(plane, cost, centers, V, crop, max_cost, raw_w, raw_h, out, range);
+}
+
+void orc_vd_map(const float* plane, const float* centers, const float* gray, int V, int crop,
+                float mla_base, int raw_w, int raw_h, float* vd, uint8_t* color, float* range) {
+    orc::vd_map(plane, centers, gray, V, crop, mla_base, raw_w, raw_h, vd, color, range);
 }
 
 int orc_fit_v2r(const float* aL, const float* v, int n, double* c) {

@@ -204,3 +204,31 @@ def random_normals(seed, uid, step, n):
     out = np.zeros((n, 3))
     lib().orc_random_normals_f64(C.c_uint32(seed), C.c_uint32(uid), C.c_uint32(step), n, _p(out))
     return out
+
+
+def disparity_mosaic(plane, cost, centers, crop, max_cost, raw_w, raw_h):
+    """plane (V*crop^2, 4) float32, cost (V*crop^2,) float32 -> (mosaic (h, w) f32, range (2,))."""
+    plane = np.ascontiguousarray(plane, np.float32)
+    cost = np.ascontiguousarray(cost, np.float32)
+    centers = np.ascontiguousarray(centers, np.float32)
+    V = centers.shape[0]
+    out = np.zeros((raw_h, raw_w), np.float32)
+    rng = np.zeros(2, np.float32)
+    lib().orc_disparity_mosaic(_p(plane), _p(cost), _p(centers), V, crop, C.c_float(max_cost),
+                               raw_w, raw_h, _p(out), _p(rng))
+    return out, rng
+
+
+def vd_map(plane, centers, gray, crop, mla_base, raw_w, raw_h, color=True):
+    """-> (vd (h, w) f32, colour (h, w) u8 or None, range (2,) f32)."""
+    plane = np.ascontiguousarray(plane, np.float32)
+    centers = np.ascontiguousarray(centers, np.float32)
+    V = centers.shape[0]
+    g = None if gray is None else np.ascontiguousarray(gray, np.float32)
+    vd = np.zeros((raw_h, raw_w), np.float32)
+    col = np.zeros((raw_h, raw_w), np.uint8) if color else None
+    rng = np.zeros(2, np.float32)
+    lib().orc_vd_map(_p(plane), _p(centers), _p(g) if g is not None else None, V, crop,
+                     C.c_float(mla_base), raw_w, raw_h, _p(vd),
+                     _p(col) if col is not None else None, _p(rng))
+    return vd, col, rng

@@ -259,10 +259,23 @@ template <> __device__ __forceinline__ const float* tile_base<float>(const Lds& 
 template <> __device__ __forceinline__ const __half* tile_base<__half>(const Lds& L) {
     return L.htiles;
 }
+// FP16 mode holds the 8-bit intensity codes 0..255 (k/255 in the FP32 tiles, recovered exactly by
+// rint(255 t)); every integer up to 2048 is exact in FP16, so the storage loses nothing and the
+// NCC runs on the 0..255 scale.  NCC is scale invariant; the two scale-dependent constants (colour
+// weight 1/(2 s_c^2) and the variance gate) are rescaled by TileScale.
 __device__ __forceinline__ float ldt(const float* p) { return *p; }
 __device__ __forceinline__ float ldt(const __half* p) { return __half2float(*p); }
 __device__ __forceinline__ void stt(float* p, float v) { *p = v; }
-__device__ __forceinline__ void stt(__half* p, float v) { *p = __float2half_rn(v); }
+__device__ __forceinline__ void stt(__half* p, float v) { *p = __float2half_rn(rintf(v * 255.0f)); }
+template <typename TT> struct TileScale;
+template <> struct TileScale<float> {
+    __device__ static float inv_sc(const DevParams& P) { return P.inv_sc; }
+    __device__ static float min_var(const DevParams& P) { return P.min_var; }
+};
+template <> struct TileScale<__half> {
+    __device__ static float inv_sc(const DevParams& P) { return P.inv_sc * (1.0f / 255.0f); }
+    __device__ static float min_var(const DevParams& P) { return P.min_var * 65025.0f; }
+};
 
 __device__ __forceinline__ void fill_spatial(Lds& L, const DevParams& P) {
     const int n = 2 * P.r + 1;
@@ -353,6 +366,7 @@ __device__ __forceinline__ void ncc_group(const Lds& L, const DevParams& P, int 
         for (int q = 0; q < 6; ++q) acc[g][q] = 0.0f;
     const int r = P.r, inc = P.inc, n = 2 * r + 1;
     const float W1f = (float)W1, Wm1 = (float)(W - 1);
+    const float inv_sc = TileScale<TT>::inv_sc(P), min_var = TileScale<TT>::min_var(P);
     for (int i = -r; i <= r; i += inc) {
         const int rx = px + i;
         if ((unsigned)rx >= (unsigned)W) continue;
@@ -363,7 +377,7 @@ __device__ __forceinline__ void ncc_group(const Lds& L, const DevParams& P, int 
             if ((unsigned)ry >= (unsigned)W) continue;
             const float d = di + B * (float)j;
             const float x = ldt(tile_base<TT>(L) + ry * W1 + rx) - rc;
-            const float wgt = __expf(spat_i[(j + r) * n] - fabsf(x) * P.inv_sc);
+            const float wgt = __expf(spat_i[(j + r) * n] - fabsf(x) * inv_sc);
             const float wx = wgt * x;
             const float wxx = wx * x;
             // branchless view block: invalid samples get weight 0 and read a clamped in-crop
@@ -401,7 +415,7 @@ __device__ __forceinline__ void ncc_group(const Lds& L, const DevParams& P, int 
         const float mx = acc[g][1] * inv, my = acc[g][3] * inv;
         const float vx = acc[g][2] * inv - mx * mx;
         const float vy = acc[g][4] * inv - my * my;
-        if (vx < P.min_var || vy < P.min_var) continue;
+        if (vx < min_var || vy < min_var) continue;
         const float cov = acc[g][5] * inv - mx * my;
         out[g] = fminf(2.0f, fmaxf(0.0f, 1.0f - cov / sqrtf(vx * vy)));
     }

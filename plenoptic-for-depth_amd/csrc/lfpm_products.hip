@@ -1,0 +1,604 @@
+// lfpm_products.hip — per-frame products of a solved frame, on the GPU:
+//   * the raw-size disparity mosaic (every MI pixel whose cost passes the gate lands at its raw
+//     position) and the frame's disparity range, and
+//   * the virtual-depth (VD) map: every MI pixel splats f_v = mla_base / d at its projected raw
+//     position; each raw pixel keeps the median of its samples, plus the gray "real colour"
+//     image (median colour, empty pixels filled by the directional search, ten 5x5 median
+//     passes).
+// Both are HBM-bound scatter / gather passes over V*crop^2 MI pixels and raw_w*raw_h raw
+// pixels.  Every result is deterministic: scatters either resolve collisions by an order-free
+// rule (highest MI index wins) or sort what they collected before reducing it.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "../../include/lfpm.h"
+#include "lfpm_internal.h"
+
+namespace lfpm {
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+// Order-preserving float <-> int map, so float min/max can use integer atomics.
+__device__ __forceinline__ int f2o(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float o2f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+__global__ void k_fill_i32(int* __restrict__ p, long long n, int v) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// range[0] = ordered(min), range[1] = ordered(max) of the values a block saw; empty blocks add
+// nothing.  Caller initialises range to {ordered(+inf), ordered(-inf)}.
+__device__ __forceinline__ void block_minmax(int* range, bool has, float val) {
+    __shared__ int smin, smax;
+    if (threadIdx.x == 0) { smin = f2o(INFINITY); smax = f2o(-INFINITY); }
+    __syncthreads();
+    if (has) {
+        atomicMin(&smin, f2o(val));
+        atomicMax(&smax, f2o(val));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (smin != f2o(INFINITY)) atomicMin(&range[0], smin);
+        if (smax != f2o(-INFINITY)) atomicMax(&range[1], smax);
+    }
+}
+
+__global__ void k_range_init(int* range) {
+    range[0] = f2o(INFINITY);
+    range[1] = f2o(-INFINITY);
+}
+
+__global__ void k_range_out(const int* __restrict__ range, float* __restrict__ out) {
+    out[0] = o2f(range[0]);
+    out[1] = o2f(range[1]);
+}
+
+// Raw position of local pixel (x, y) of an MI centred at (cx, cy) in the mosaic: the crop's
+// top-left corner c - (round(crop/2) - 1), truncated toward zero
+// (DepthSolver::WriteDisMap_TileKey_new_AccuSequence, DepthSolver.cpp:6790-6812).
+__device__ __forceinline__ int mosaic_origin(float c, int crop) {
+#pragma clang fp contract(off)
+    return (int)(c - (roundf((float)crop * 0.5f) - 1.0f));
+}
+
+// ------------------------------------------------------------------------------------------
+// Disparity mosaic
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_mosaic_scatter(const float4* __restrict__ plane,
+                                                        const float* __restrict__ cost,
+                                                        const float2* __restrict__ centers,
+                                                        int V, int crop, float max_cost,
+                                                        int raw_w, int raw_h,
+                                                        int* __restrict__ owner,
+                                                        int* __restrict__ range) {
+    const int WW = crop * crop;
+    const long long M = (long long)V * WW;
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    float d = 0.0f;
+    bool has = false;
+    if (i < M) {
+        const int v = (int)(i / WW), pix = (int)(i - (long long)v * WW);
+        d = plane[i].w;
+        has = d > 0.0f;
+        if (has && cost[i] <= max_cost) {
+            const float2 c = centers[v];
+            const int X = mosaic_origin(c.x, crop) + pix % crop;
+            const int Y = mosaic_origin(c.y, crop) + pix / crop;
+            if (X >= 0 && X < raw_w && Y >= 0 && Y < raw_h)
+                atomicMax(&owner[(long long)Y * raw_w + X], v);
+        }
+    }
+    block_minmax(range, has, d);
+}
+
+__global__ void __launch_bounds__(256) k_mosaic_gather(const float4* __restrict__ plane,
+                                                       const float2* __restrict__ centers,
+                                                       const int* __restrict__ owner, int crop,
+                                                       int raw_w, int raw_h,
+                                                       float* __restrict__ out) {
+    const long long N = (long long)raw_w * raw_h;
+    const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const int o = owner[p];
+    float d = 0.0f;
+    if (o >= 0) {
+        const int Y = (int)(p / raw_w), X = (int)(p - (long long)Y * raw_w);
+        const float2 c = centers[o];
+        const int x = X - mosaic_origin(c.x, crop), y = Y - mosaic_origin(c.y, crop);
+        d = plane[(long long)o * crop * crop + y * crop + x].w;
+    }
+    out[p] = d;
+}
+
+// ------------------------------------------------------------------------------------------
+// Virtual-depth splat (DepthSolver::Virtual_depth_map_TileKeySequence, DepthSolver.cpp:7832-7905)
+// ------------------------------------------------------------------------------------------
+// Raw target of MI pixel i (or -1) and its virtual depth f_v.
+__device__ __forceinline__ long long vd_target(const float4* __restrict__ plane,
+                                               const float2* __restrict__ centers, long long i,
+                                               int crop, float mla_base, int raw_w, int raw_h,
+                                               float& fv) {
+#pragma clang fp contract(off)
+    const int WW = crop * crop;
+    const int v = (int)(i / WW), pix = (int)(i - (long long)v * WW);
+    const int x = pix % crop, y = pix / crop;
+    float d = plane[i].w;
+    if (d <= 0.0f) d = 1.0f;
+    fv = mla_base / d;
+    const float half1 = (float)(crop - 1) * 0.5f + 1.0f;
+    const int xm = (int)((float)x - half1), ym = (int)((float)y - half1);
+    const float2 c = centers[v];
+    const float Xv = (float)xm * fv + c.x, Yv = (float)ym * fv + c.y;
+    const int X = (int)roundf(Xv), Y = (int)roundf(Yv);
+    if (X > 0 && X < raw_w && Y > 0 && Y < raw_h) return (long long)Y * raw_w + X;
+    return -1;
+}
+
+__global__ void __launch_bounds__(256) k_vd_count(const float4* __restrict__ plane,
+                                                  const float2* __restrict__ centers, int V,
+                                                  int crop, float mla_base, int raw_w, int raw_h,
+                                                  int* __restrict__ count) {
+    const long long M = (long long)V * crop * crop;
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    float fv;
+    const long long t = vd_target(plane, centers, i, crop, mla_base, raw_w, raw_h, fv);
+    if (t >= 0) atomicAdd(&count[t], 1);
+}
+
+// Exclusive scan of n ints (three passes: chunk sums, scan of sums, chunk scans).
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(const int* __restrict__ in, long long n,
+                                                            int* __restrict__ sums) {
+    __shared__ int s[kScanThreads];
+    const long long base = (long long)blockIdx.x * kScanChunk;
+    int acc = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const long long i = base + (long long)k * kScanThreads + threadIdx.x;
+        if (i < n) acc += in[i];
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kScanThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = s[0];
+}
+
+// Inclusive Hillis-Steele scan of one value per thread over the block; returns the exclusive
+// prefix of this thread and the block total.
+__device__ __forceinline__ int block_exclusive(int v, int& total) {
+    __shared__ int s[kScanThreads];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int add = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    total = s[kScanThreads - 1];
+    const int incl = s[threadIdx.x];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums_scan(int* __restrict__ sums, int nb) {
+    // one block: each thread owns a contiguous run of ceil(nb / threads) sums
+    const int per = (nb + kScanThreads - 1) / kScanThreads;
+    const int b0 = threadIdx.x * per;
+    int acc = 0;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < nb) acc += sums[b0 + k];
+    int total;
+    int run = block_exclusive(acc, total);
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < nb) {
+            const int v = sums[b0 + k];
+            sums[b0 + k] = run;
+            run += v;
+        }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_chunks(const int* __restrict__ in,
+                                                              long long n,
+                                                              const int* __restrict__ sums,
+                                                              int* __restrict__ out) {
+    // thread t owns items [base + t*kScanItems, +kScanItems) (contiguous per thread)
+    const long long base = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanItems;
+    int vals[kScanItems];
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const long long i = base + k;
+        vals[k] = i < n ? in[i] : 0;
+        acc += vals[k];
+    }
+    int total;
+    int run = block_exclusive(acc, total) + sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const long long i = base + k;
+        if (i < n) out[i] = run;
+        run += vals[k];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_vd_scatter(const float4* __restrict__ plane,
+                                                    const float2* __restrict__ centers,
+                                                    const float* __restrict__ gray, int V,
+                                                    int crop, float mla_base, int raw_w,
+                                                    int raw_h, int* __restrict__ count,
+                                                    const int* __restrict__ offs,
+                                                    float* __restrict__ sv,
+                                                    uint8_t* __restrict__ sc) {
+    const long long M = (long long)V * crop * crop;
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    float fv;
+    const long long t = vd_target(plane, centers, i, crop, mla_base, raw_w, raw_h, fv);
+    if (t < 0) return;
+    const int slot = offs[t] + atomicSub(&count[t], 1) - 1;
+    sv[slot] = fv;
+    sc[slot] = gray ? (uint8_t)gray[i] : (uint8_t)0;
+}
+
+// Median of each raw pixel's samples (sorted in place).  VD value: the reference's index rule
+// (s[n/2+1] + s[(n+1)/2]) / 2, with indices clamped to n-1 (the rule reads past the end for
+// n <= 2); colour: (c[n/2] + c[(n+1)/2-1]) / 2 in integers.
+__global__ void __launch_bounds__(256) k_vd_median(const int* __restrict__ offs, long long N,
+                                                   float* __restrict__ sv,
+                                                   uint8_t* __restrict__ sc,
+                                                   float* __restrict__ vd,
+                                                   uint8_t* __restrict__ col,
+                                                   int* __restrict__ range) {
+    const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    bool has = false;
+    float m = 0.0f;
+    if (p < N) {
+        const int b = offs[p], n = offs[p + 1] - b;
+        uint8_t cm = 0;
+        if (n > 0) {
+            float* s = sv + b;
+            uint8_t* c = sc + b;
+            for (int a = 1; a < n; ++a) {
+                const float x = s[a];
+                int j = a - 1;
+                while (j >= 0 && s[j] > x) { s[j + 1] = s[j]; --j; }
+                s[j + 1] = x;
+                const uint8_t y = c[a];
+                j = a - 1;
+                while (j >= 0 && c[j] > y) { c[j + 1] = c[j]; --j; }
+                c[j + 1] = y;
+            }
+            const int i1 = min(n / 2 + 1, n - 1), i2 = min((n + 1) / 2, n - 1);
+            m = (s[i1] + s[i2]) / 2.0f;
+            cm = (uint8_t)(((int)c[n / 2] + (int)c[(n + 1) / 2 - 1]) / 2);
+            has = true;
+        }
+        vd[p] = m;
+        col[p] = cm;
+    }
+    block_minmax(range, has, m);
+}
+
+// Search tables of the colour fill (thread per raw row): in-row next / previous pixel with
+// samples, and each row's first / last such pixel.
+__global__ void k_fill_rows(const int* __restrict__ offs, int raw_w, int raw_h,
+                            int* __restrict__ nr, int* __restrict__ pr, int* __restrict__ rfirst,
+                            int* __restrict__ rlast) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= raw_h) return;
+    const long long row = (long long)y * raw_w;
+    int nxt = -1;
+    for (int x = raw_w - 1; x >= 0; --x) {
+        nr[row + x] = nxt;
+        if (offs[row + x + 1] > offs[row + x]) nxt = (int)(row + x);
+    }
+    rfirst[y] = nxt;
+    int prv = -1;
+    for (int x = 0; x < raw_w; ++x) {
+        pr[row + x] = prv;
+        if (offs[row + x + 1] > offs[row + x]) prv = (int)(row + x);
+    }
+    rlast[y] = prv;
+}
+
+// nextRowFirst[y] = first sampled pixel in a row below y (N if none); prevRowLast[y] = last
+// sampled pixel in a row above y (-1 if none).  One thread: raw_h steps.
+__global__ void k_fill_rowlinks(const int* __restrict__ rfirst, const int* __restrict__ rlast,
+                                int raw_h, long long N, int* __restrict__ nrf,
+                                int* __restrict__ prl) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int nxt = (int)N;
+    for (int y = raw_h - 1; y >= 0; --y) {
+        nrf[y] = nxt;
+        if (rfirst[y] >= 0) nxt = rfirst[y];
+    }
+    int prv = -1;
+    for (int y = 0; y < raw_h; ++y) {
+        prl[y] = prv;
+        if (rlast[y] >= 0) prv = rlast[y];
+    }
+}
+
+// Chains of the down-left step (linear stride raw_w - 1; thread per chain): next sampled index
+// strictly after p along the chain, N if none.
+__global__ void k_fill_diag(const int* __restrict__ offs, int raw_w, long long N,
+                            int* __restrict__ nd) {
+    const long long step = raw_w - 1;
+    const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (step <= 0 || c >= step) return;
+    const long long last = c + ((N - 1 - c) / step) * step;
+    int nxt = (int)N;
+    for (long long p = last; p >= c; p -= step) {
+        nd[p] = nxt;
+        if (offs[p + 1] > offs[p]) nxt = (int)p;
+    }
+}
+
+// Columns (thread per column): nearest sampled pixel above p in its column, -1 if none.
+__global__ void k_fill_cols(const int* __restrict__ offs, int raw_w, int raw_h,
+                            int* __restrict__ pu) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= raw_w) return;
+    int prv = -1;
+    for (int y = 0; y < raw_h; ++y) {
+        const long long p = (long long)y * raw_w + x;
+        pu[p] = prv;
+        if (offs[p + 1] > offs[p]) prv = (int)p;
+    }
+}
+
+// Colour of pixels without samples: the reference probes, for s = 1, 2, ..., the linear indices
+// p + s, p - s, p + s*(raw_w-1), p - s*raw_w in that order, stops at the first probe that leaves
+// [0, N) (colour stays 0) or hits a pixel with samples (takes its median colour)
+// (DepthSolver.cpp:7955-7993).  Here each probe sequence's first hit and first exit are read
+// from the tables, and the earliest event in probe order decides.
+__global__ void __launch_bounds__(256) k_fill_resolve(const int* __restrict__ offs, int raw_w,
+                                                      int raw_h, const int* __restrict__ nr,
+                                                      const int* __restrict__ pr,
+                                                      const int* __restrict__ nrf,
+                                                      const int* __restrict__ prl,
+                                                      const int* __restrict__ nd,
+                                                      const int* __restrict__ pu,
+                                                      const uint8_t* __restrict__ colm,
+                                                      uint8_t* __restrict__ out) {
+    const long long N = (long long)raw_w * raw_h;
+    const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    if (offs[p + 1] > offs[p]) { out[p] = colm[p]; return; }
+    const int y = (int)(p / raw_w);
+    const long long INF = 0x7fffffffffffffffLL;
+    long long best_t = INF;
+    long long best_k = -1;
+    auto consider = [&](int c, long long s_hit, long long k_hit, long long s_exit) {
+        const bool hit = s_hit < s_exit;
+        const long long s = hit ? s_hit : s_exit;
+        if (s == INF) return;
+        const long long t = 4 * s + c;
+        if (t < best_t) { best_t = t; best_k = hit ? k_hit : -1; }
+    };
+    {   // p + s
+        const long long k = nr[p] >= 0 ? nr[p] : nrf[y];
+        consider(0, k < N ? k - p : INF, k, N - p);
+    }
+    {   // p - s
+        const long long k = pr[p] >= 0 ? pr[p] : prl[y];
+        consider(1, k >= 0 ? p - k : INF, k, p + 1);
+    }
+    if (raw_w >= 2) {   // p + s*(raw_w - 1)
+        const long long st = raw_w - 1;
+        const long long k = nd[p];
+        consider(2, k < N ? (k - p) / st : INF, k, (N - p + st - 1) / st);
+    }
+    {   // p - s*raw_w
+        const long long k = pu[p];
+        consider(3, k >= 0 ? (p - k) / raw_w : INF, k, (long long)y + 1);
+    }
+    out[p] = best_k >= 0 ? colm[best_k] : (uint8_t)0;
+}
+
+// 5x5 median with replicated borders (cv::medianBlur's documented border rule).
+__global__ void __launch_bounds__(256) k_median5(const uint8_t* __restrict__ in, int raw_w,
+                                                 int raw_h, uint8_t* __restrict__ out) {
+    const long long N = (long long)raw_w * raw_h;
+    const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const int y = (int)(p / raw_w), x = (int)(p - (long long)y * raw_w);
+    int v[25];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const int yy = min(max(y + j - 2, 0), raw_h - 1);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const int xx = min(max(x + i - 2, 0), raw_w - 1);
+            v[j * 5 + i] = in[(long long)yy * raw_w + xx];
+        }
+    }
+    // rank selection: the median is the value with 12 smaller-or-equal predecessors in a
+    // stable order, i.e. the 13th smallest
+    int med = 0;
+#pragma unroll
+    for (int a = 0; a < 25; ++a) {
+        int lt = 0, le = 0;
+#pragma unroll
+        for (int b = 0; b < 25; ++b) {
+            lt += v[b] < v[a];
+            le += v[b] <= v[a];
+        }
+        if (lt <= 12 && le > 12) med = v[a];
+    }
+    out[p] = (uint8_t)med;
+}
+
+int grid_for(long long n, int threads) {
+    long long g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > 0x7fffffffLL) g = 0x7fffffffLL;
+    return (int)g;
+}
+
+size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct VdLayout {
+    size_t count, offs, sums, sv, sc, nr, pr, nd, pu, rfirst, rlast, nrf, prl, col, range, total;
+};
+
+VdLayout vd_layout(int V, int crop, int raw_w, int raw_h) {
+    const size_t N = (size_t)raw_w * raw_h, M = (size_t)V * crop * crop;
+    const size_t nb = (N + 1 + kScanChunk - 1) / kScanChunk;
+    VdLayout L;
+    size_t o = 0;
+    L.count = o; o += al((N + 1) * 4);
+    L.offs = o; o += al((N + 1) * 4);
+    L.sums = o; o += al(nb * 4);
+    L.sv = o; o += al(M * 4);
+    L.sc = o; o += al(M);
+    L.nr = o; o += al(N * 4);
+    L.pr = o; o += al(N * 4);
+    L.nd = o; o += al(N * 4);
+    L.pu = o; o += al(N * 4);
+    L.rfirst = o; o += al((size_t)raw_h * 4);
+    L.rlast = o; o += al((size_t)raw_h * 4);
+    L.nrf = o; o += al((size_t)raw_h * 4);
+    L.prl = o; o += al((size_t)raw_h * 4);
+    L.col = o; o += al(N);
+    L.range = o; o += al(8);
+    L.total = o;
+    return L;
+}
+
+size_t mosaic_bytes(int raw_w, int raw_h) { return al((size_t)raw_w * raw_h * 4) + al(8); }
+
+int check_launch(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(LFPM_E_HIP, "%s: %s", what, hipGetErrorString(e));
+    return LFPM_OK;
+}
+
+int check_frame(int V, int crop, int raw_w, int raw_h) {
+    if (V <= 0 || crop <= 0 || crop > LFPM_MAX_CROP || raw_w <= 0 || raw_h <= 0)
+        return set_error(LFPM_E_INVALID, "products: bad sizes (V=%d crop=%d raw %dx%d)", V, crop,
+                         raw_w, raw_h);
+    if ((long long)raw_w * raw_h + 1 >= 0x7fffffffLL || (long long)V * crop * crop >= 0x7fffffffLL)
+        return set_error(LFPM_E_INVALID, "products: frame too large for 32-bit indices");
+    return LFPM_OK;
+}
+
+}  // namespace
+}  // namespace lfpm
+
+using namespace lfpm;
+
+extern "C" {
+
+size_t lfpm_products_workspace_size(int num_mis, int crop, int raw_w, int raw_h) {
+    if (num_mis <= 0 || crop <= 0 || raw_w <= 0 || raw_h <= 0) return 0;
+    const size_t a = mosaic_bytes(raw_w, raw_h), b = vd_layout(num_mis, crop, raw_w, raw_h).total;
+    return a > b ? a : b;
+}
+
+int lfpm_disparity_mosaic(const float* plane_dev, const float* cost_dev, const float* centers_dev,
+                          int num_mis, int crop, float max_cost, int raw_w, int raw_h,
+                          float* mosaic_dev, float* range_dev, void* workspace_dev,
+                          size_t workspace_bytes, void* stream) {
+    if (int e = check_frame(num_mis, crop, raw_w, raw_h)) return e;
+    if (!plane_dev || !cost_dev || !centers_dev || !mosaic_dev || !workspace_dev)
+        return set_error(LFPM_E_INVALID, "lfpm_disparity_mosaic: null pointer");
+    if (workspace_bytes < mosaic_bytes(raw_w, raw_h))
+        return set_error(LFPM_E_INVALID, "lfpm_disparity_mosaic: workspace %zu < %zu bytes",
+                         workspace_bytes, mosaic_bytes(raw_w, raw_h));
+    hipStream_t s = (hipStream_t)stream;
+    const long long N = (long long)raw_w * raw_h, M = (long long)num_mis * crop * crop;
+    unsigned char* ws = (unsigned char*)workspace_dev;
+    int* owner = (int*)ws;
+    int* range = (int*)(ws + al((size_t)N * 4));
+    k_fill_i32<<<grid_for(N, 256 * 8), 256, 0, s>>>(owner, N, -1);
+    k_range_init<<<1, 1, 0, s>>>(range);
+    k_mosaic_scatter<<<grid_for(M, 256), 256, 0, s>>>(
+        (const float4*)plane_dev, cost_dev, (const float2*)centers_dev, num_mis, crop, max_cost,
+        raw_w, raw_h, owner, range);
+    k_mosaic_gather<<<grid_for(N, 256), 256, 0, s>>>((const float4*)plane_dev,
+                                                      (const float2*)centers_dev, owner, crop,
+                                                      raw_w, raw_h, mosaic_dev);
+    if (range_dev) k_range_out<<<1, 1, 0, s>>>(range, range_dev);
+    return check_launch("lfpm_disparity_mosaic");
+}
+
+int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, const float* gray_dev,
+                           int num_mis, int crop, float mla_base, int raw_w, int raw_h,
+                           float* vd_dev, uint8_t* color_dev, float* range_dev,
+                           void* workspace_dev, size_t workspace_bytes, void* stream) {
+    if (int e = check_frame(num_mis, crop, raw_w, raw_h)) return e;
+    if (!plane_dev || !centers_dev || !vd_dev || !workspace_dev)
+        return set_error(LFPM_E_INVALID, "lfpm_virtual_depth_map: null pointer");
+    if (!(mla_base > 0.0f))
+        return set_error(LFPM_E_INVALID, "lfpm_virtual_depth_map: mla_base must be > 0");
+    const VdLayout L = vd_layout(num_mis, crop, raw_w, raw_h);
+    if (workspace_bytes < L.total)
+        return set_error(LFPM_E_INVALID, "lfpm_virtual_depth_map: workspace %zu < %zu bytes",
+                         workspace_bytes, L.total);
+    hipStream_t s = (hipStream_t)stream;
+    const long long N = (long long)raw_w * raw_h, M = (long long)num_mis * crop * crop;
+    unsigned char* ws = (unsigned char*)workspace_dev;
+    int* count = (int*)(ws + L.count);
+    int* offs = (int*)(ws + L.offs);
+    int* sums = (int*)(ws + L.sums);
+    float* sv = (float*)(ws + L.sv);
+    uint8_t* sc = ws + L.sc;
+    int* range = (int*)(ws + L.range);
+    const float4* plane = (const float4*)plane_dev;
+    const float2* centers = (const float2*)centers_dev;
+    const int nb = (int)((N + 1 + kScanChunk - 1) / kScanChunk);
+
+    k_fill_i32<<<grid_for(N + 1, 256 * 8), 256, 0, s>>>(count, N + 1, 0);
+    k_range_init<<<1, 1, 0, s>>>(range);
+    k_vd_count<<<grid_for(M, 256), 256, 0, s>>>(plane, centers, num_mis, crop, mla_base, raw_w,
+                                                 raw_h, count);
+    k_scan_sums<<<nb, kScanThreads, 0, s>>>(count, N + 1, sums);
+    k_scan_sums_scan<<<1, kScanThreads, 0, s>>>(sums, nb);
+    k_scan_chunks<<<nb, kScanThreads, 0, s>>>(count, N + 1, sums, offs);
+    k_vd_scatter<<<grid_for(M, 256), 256, 0, s>>>(plane, centers, gray_dev, num_mis, crop,
+                                                   mla_base, raw_w, raw_h, count, offs, sv, sc);
+    uint8_t* colm = ws + L.col;
+    k_vd_median<<<grid_for(N, 256), 256, 0, s>>>(offs, N, sv, sc, vd_dev, colm, range);
+    if (range_dev) k_range_out<<<1, 1, 0, s>>>(range, range_dev);
+    if (color_dev) {
+        int* nr = (int*)(ws + L.nr);
+        int* pr = (int*)(ws + L.pr);
+        int* nd = (int*)(ws + L.nd);
+        int* pu = (int*)(ws + L.pu);
+        int* rfirst = (int*)(ws + L.rfirst);
+        int* rlast = (int*)(ws + L.rlast);
+        int* nrf = (int*)(ws + L.nrf);
+        int* prl = (int*)(ws + L.prl);
+        k_fill_rows<<<grid_for(raw_h, 64), 64, 0, s>>>(offs, raw_w, raw_h, nr, pr, rfirst, rlast);
+        k_fill_rowlinks<<<1, 1, 0, s>>>(rfirst, rlast, raw_h, N, nrf, prl);
+        if (raw_w >= 2) k_fill_diag<<<grid_for(raw_w - 1, 64), 64, 0, s>>>(offs, raw_w, N, nd);
+        k_fill_cols<<<grid_for(raw_w, 64), 64, 0, s>>>(offs, raw_w, raw_h, pu);
+        // filled colour -> color_dev, then ten 5x5 median passes ping-ponging with a scratch
+        // image in the fill tables' memory (no longer needed); an even pass count ends in
+        // color_dev
+        k_fill_resolve<<<grid_for(N, 256), 256, 0, s>>>(offs, raw_w, raw_h, nr, pr, nrf, prl, nd,
+                                                        pu, colm, color_dev);
+        uint8_t* tmp = (uint8_t*)nr;
+        for (int it = 0; it < 10; ++it) {
+            if (it & 1) k_median5<<<grid_for(N, 256), 256, 0, s>>>(tmp, raw_w, raw_h, color_dev);
+            else k_median5<<<grid_for(N, 256), 256, 0, s>>>(color_dev, raw_w, raw_h, tmp);
+        }
+    }
+    return check_launch("lfpm_virtual_depth_map");
+}
+
+}  // extern "C"

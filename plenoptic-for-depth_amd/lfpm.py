@@ -71,6 +71,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.lfpm_last_error.restype = C.c_char_p
         L.lfpm_patchmatch_workspace_size.restype = C.c_size_t
+        L.lfpm_products_workspace_size.restype = C.c_size_t
+        L.lfpm_batch_workspace_size.restype = C.c_size_t
         _lib = L
     return _lib
 
@@ -319,6 +321,47 @@ def refocus(df, slices=None, vmin=None, vmax=None, out=None, stream=None):
                               wl.L["rows"], C.c_float(wl.L["pitch"]), wl.crop, w, h, S,
                               C.c_float(vmin), C.c_float(vmax), _dp(out), _stream(stream)))
     return out
+
+
+def products_workspace(df):
+    import torch
+    wl = df.wl
+    h, w = df.raw.shape
+    n = lib().lfpm_products_workspace_size(wl.V, wl.crop, w, h)
+    return torch.empty(n, dtype=torch.uint8, device=df.tiles.device)
+
+
+def disparity_mosaic(df, plane, cost, max_cost=0.2, ws=None, stream=None):
+    """Raw-size disparity mosaic of a solved frame (WriteDisMap_TileKey_new_AccuSequence,
+    DepthSolver.cpp:6716-6897).  Returns (mosaic float32 (h, w), range float32 (2,))."""
+    import torch
+    wl = df.wl
+    h, w = df.raw.shape
+    ws = products_workspace(df) if ws is None else ws
+    out = torch.empty((h, w), dtype=torch.float32, device=plane.device)
+    rng = torch.empty(2, dtype=torch.float32, device=plane.device)
+    _check(lib().lfpm_disparity_mosaic(_dp(plane), _dp(cost), _dp(df.centers), wl.V, wl.crop,
+                                       C.c_float(max_cost), w, h, _dp(out), _dp(rng), _dp(ws),
+                                       C.c_size_t(ws.numel()), _stream(stream)))
+    return out, rng
+
+
+def virtual_depth_map(df, plane, mla_base=None, color=True, ws=None, stream=None):
+    """Virtual-depth median splat (Virtual_depth_map_TileKeySequence, DepthSolver.cpp:7832-8000).
+    Returns (vd float32 (h, w), colour uint8 (h, w) or None, range float32 (2,))."""
+    import torch
+    wl = df.wl
+    h, w = df.raw.shape
+    mla_base = wl.L["pitch"] if mla_base is None else mla_base
+    ws = products_workspace(df) if ws is None else ws
+    vd = torch.empty((h, w), dtype=torch.float32, device=plane.device)
+    col = torch.empty((h, w), dtype=torch.uint8, device=plane.device) if color else None
+    rng = torch.empty(2, dtype=torch.float32, device=plane.device)
+    _check(lib().lfpm_virtual_depth_map(_dp(plane), _dp(df.centers), _dp(df.gray), wl.V,
+                                        wl.crop, C.c_float(mla_base), w, h, _dp(vd),
+                                        _dp(col) if col is not None else None, _dp(rng), _dp(ws),
+                                        C.c_size_t(ws.numel()), _stream(stream)))
+    return vd, col, rng
 
 
 def fit_v2r(a_l, v, stream=None):
