@@ -387,6 +387,29 @@ int lfpm_ctx_solve_tiles(lfpm_ctx* ctx, const uint8_t* tiles_u8, const float* ce
                          const lfpm_pm_params* p, const lfpm_filter_params* f, float* plane,
                          float* cost, float* disp, uint32_t* sel, uint8_t* reason);
 
+/* Batched form of lfpm_ctx_solve_frame: n host frames of one layout solved together through
+ * lfpm_batch_run (persistent launches over all frames' MIs).  Frame i keys its random stream
+ * with frame_uid = p->frame_uid + i, so its results equal lfpm_ctx_solve_frame of that frame
+ * with that frame_uid, bit for bit.  Output arguments are arrays of n host pointers (an array,
+ * or any entry, may be NULL).  No reference counterpart: the reference's batch pipeline solves
+ * its frames one after another (DepthSolverManager::DepthSolverPipelineByBatch,
+ * DepthSolverManager.cpp:93-135). */
+int lfpm_ctx_solve_frames(lfpm_ctx* ctx, const lfpm_layout_params* lp, int n,
+                          const uint8_t* const* raws, const lfpm_pm_params* p,
+                          const lfpm_filter_params* f, float* const* plane, float* const* cost,
+                          float* const* disp, uint32_t* const* sel, uint8_t* const* reason);
+
+/* Per-frame products of frame `frame_index` of the context's last solve call (its filtered
+ * results when that call ran the filter): lfpm_disparity_mosaic and lfpm_virtual_depth_map on
+ * the resident device state, copied to host.  The solve's centres must be raw-frame
+ * coordinates of a raw_w x raw_h frame.  Host outputs (any may be NULL): mosaic and vd
+ * raw_h*raw_w floats, color raw_h*raw_w bytes, mosaic_range and vd_range 2 floats each.
+ * Replaces the writers that follow the solve in the reference's pipelines
+ * (DepthSolver.cpp:6716-6897, 7832-8000). */
+int lfpm_ctx_frame_products(lfpm_ctx* ctx, int frame_index, int raw_w, int raw_h,
+                            float max_cost, float mla_base, float* mosaic, float* mosaic_range,
+                            float* vd, uint8_t* color, float* vd_range);
+
 #ifdef __cplusplus
 }
 #endif

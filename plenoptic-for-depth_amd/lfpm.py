@@ -364,6 +364,80 @@ def virtual_depth_map(df, plane, mla_base=None, color=True, ws=None, stream=None
     return vd, col, rng
 
 
+class Context:
+    """Host-level context (lfpm_create / lfpm_ctx_*): host frames in, host results out; device
+    buffers stay resident across calls of the same layout."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib().lfpm_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().lfpm_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _outs(n):
+        return dict(plane=np.zeros((n, 4), np.float32), cost=np.zeros(n, np.float32),
+                    disp=np.zeros((n, 4), np.float32), sel=np.zeros(n, np.uint32),
+                    reason=np.zeros(n, np.uint8))
+
+    def solve_frame(self, wl, raw=None, filt=True, pm=None):
+        raw = np.ascontiguousarray(wl.raw() if raw is None else raw)
+        o = self._outs(wl.V * wl.crop ** 2)
+        _check(lib().lfpm_ctx_solve_frame(self.h, C.byref(wl.lp), _np(raw), C.byref(pm or wl.pm),
+                                          C.byref(wl.fp), _np(o["plane"]), _np(o["cost"]),
+                                          _np(o["disp"]), _np(o["sel"]),
+                                          _np(o["reason"]) if filt else None))
+        return o
+
+    def solve_frames(self, wl, raws, filt=True, pm=None):
+        """Frames of wl's layout in one batched solve; frame i uses frame_uid + i."""
+        n = len(raws)
+        raws = [np.ascontiguousarray(r) for r in raws]
+        outs = [self._outs(wl.V * wl.crop ** 2) for _ in range(n)]
+        VP = C.c_void_p * n
+
+        def arr(key):
+            return VP(*[o[key].ctypes.data for o in outs])
+        _check(lib().lfpm_ctx_solve_frames(
+            self.h, C.byref(wl.lp), n, VP(*[r.ctypes.data for r in raws]), C.byref(pm or wl.pm),
+            C.byref(wl.fp), arr("plane"), arr("cost"), arr("disp"), arr("sel"),
+            arr("reason") if filt else None))
+        return outs
+
+    def solve_tiles(self, tiles_u8, centers, nbr, cnt, pm, fp=None):
+        V = centers.shape[0]
+        o = self._outs(V * pm.crop ** 2)
+        tiles_u8 = np.ascontiguousarray(tiles_u8, np.uint8)
+        centers = np.ascontiguousarray(centers, np.float32)
+        nbr = np.ascontiguousarray(nbr, np.int32)
+        cnt = np.ascontiguousarray(cnt, np.int32)
+        _check(lib().lfpm_ctx_solve_tiles(self.h, _np(tiles_u8), _np(centers), _np(nbr), _np(cnt),
+                                          V, C.byref(pm), C.byref(fp) if fp else None,
+                                          _np(o["plane"]), _np(o["cost"]), _np(o["disp"]),
+                                          _np(o["sel"]), _np(o["reason"]) if fp else None))
+        return o
+
+    def frame_products(self, raw_w, raw_h, mla_base, max_cost=0.2, frame=0):
+        mosaic = np.zeros((raw_h, raw_w), np.float32)
+        vd = np.zeros((raw_h, raw_w), np.float32)
+        col = np.zeros((raw_h, raw_w), np.uint8)
+        mr = np.zeros(2, np.float32)
+        vr = np.zeros(2, np.float32)
+        _check(lib().lfpm_ctx_frame_products(self.h, frame, raw_w, raw_h, C.c_float(max_cost),
+                                             C.c_float(mla_base), _np(mosaic), _np(mr), _np(vd),
+                                             _np(col), _np(vr)))
+        return dict(mosaic=mosaic, mosaic_range=mr, vd=vd, color=col, vd_range=vr)
+
+
 def fit_v2r(a_l, v, stream=None):
     c = (C.c_double * 3)()
     _check(lib().lfpm_fit_v2r(_dp(a_l), _dp(v), a_l.numel(), c, _stream(stream)))

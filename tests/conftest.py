@@ -18,8 +18,11 @@ def pytest_configure(config):
     # build only what is missing (the build step proper is __graft_entry__.build())
     if not os.path.exists(os.path.join(ORACLE, "liblfpm_oracle.so")):
         subprocess.check_call(["make", "-s", "-C", ORACLE])
-    if not os.path.exists(os.path.join(PKG, "liblfpm.so")):
+    if not (os.path.exists(os.path.join(PKG, "liblfpm.so")) and
+            os.path.exists(os.path.join(PKG, "liblfmvs.so"))):
         subprocess.check_call(["make", "-s", "-C", PKG])
+    if not os.path.exists(os.path.join(ROOT, "tests", "cpp", "_build", "facade_driver")):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp")])
 
 
 def stats_dump(name, payload):

@@ -78,3 +78,28 @@ def test_device_calls_fail_loudly_without_device_pointers(lfpm):
                                           None, None, None, C.c_size_t(0), 0, None)
     assert rc != 0
     _ = np
+
+
+def test_facade_library_exports_reference_entry_points():
+    """liblfmvs.so (C++ facades over the C ABI) loads and defines the reference's pipeline
+    entry points with their names (DepthSolverManager.h:23-26, AdaptMIPMFrame.h:22-27)."""
+    import ctypes
+    import os
+    import subprocess
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "plenoptic-for-depth_amd")
+    so = os.path.join(pkg, "liblfmvs.so")
+    ctypes.CDLL(os.path.join(pkg, "liblfpm.so"), mode=ctypes.RTLD_GLOBAL)
+    ctypes.CDLL(so)
+    syms = subprocess.run(["nm", "-DC", "--defined-only", so], capture_output=True, text=True,
+                          check=True).stdout
+    for name in ("LFMVS::DepthSolverManager::DepthSolverPipelineBySequence(LFMVS::eStereoType)",
+                 "LFMVS::DepthSolverManager::DepthSolverPipelineByBatch()",
+                 "LFMVS::DepthSolver::ProcessProblemsImpSequence(",
+                 "LFMVS::AdaptMIPMFrame::Initialize(",
+                 "LFMVS::AdaptMIPMFrame::RunPatchMatchCUDAForFrame()",
+                 "LFMVS::AdaptMIPMFrame::WriteBackResults(",
+                 "LFMVS::AdaptMIPMFrame::ReleaseMemory()",
+                 "LFMVS::AdaptMIPM::RunPatchMatchCUDAForMI()",
+                 "LFMVS::AdaptMIPM::Initialize("):
+        assert name in syms, name
