@@ -83,7 +83,7 @@ def test_facade_pipelines(lfpm, expected, mode, tmp_path):
     wl = _frame_workload(lfpm, 0)
     out = _run(lfpm, wl, 2, mode, tmp_path)
     for i in range(2):
-        _compare(out, expected, i)
+        _compare(out, expected[i], i)
 
 
 def test_facade_adaptmipm_single_mi(lfpm, orc, tmp_path):
