@@ -187,7 +187,8 @@ def count_work(lfpm, df, bufs, fo, rf, torch):
 
 
 def measured_traffic(row):
-    """HBM bytes per dispatch of the summary row `row` (kernel name and stage) from the committed
+    """HBM bytes per dispatch of the k_stage summary row of stage `row` ("init", "phase", "final")
+    from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by the copy-kernel calibration), or None if
     absent."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_plain_4080x3068.md")
@@ -195,7 +196,7 @@ def measured_traffic(row):
         return None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == row:
+        if len(cells) == 5 and cells[0].startswith("lfpm::k_stage") and cells[0].endswith(row):
             try:
                 return float(cells[4])
             except ValueError:
@@ -269,7 +270,7 @@ def plain_section(args, lfpm, torch, wl):
                                      "contraction, so no MFMA instructions are issued)",
             "kernel": "k_stage (red/black phase)", "achieved": achieved,
             "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / (PEAK_FP32 / 1e12),
-            "traffic": measured_traffic("k_stage phase"), "avg_launch_ms": kt["phase"],
+            "traffic": measured_traffic(" phase"), "avg_launch_ms": kt["phase"],
             "flop_per_launch": flop_per_launch,
             "per_unit": "22 FLOP per evaluated window sample + 16 per evaluated "
                         "(pixel, view, hypothesis) triple"}
