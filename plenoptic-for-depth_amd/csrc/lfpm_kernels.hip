@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -940,6 +941,599 @@ k_stage(StageArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Wave-per-pixel solver (default).  One wavefront owns one pixel at a time and evaluates a
+// task list of (hypothesis, view) pairs — every view whose anchor stays inside the crop, for
+// every hypothesis of a batch — in passes of 8 tasks x 8 window-sample lanes.  The pixel's
+// valid window samples (reference intensity, bilateral weight) are compacted once into LDS,
+// so lanes never idle on clipped window positions or on views whose anchor left the crop, and
+// each task's six weighted moments are reduced across its 8 lanes with DPP.  Results follow
+// the same definitions as ncc_group (ComputeBilateralNCC_Frame, AdaptMIPMFrame.cu:73-179)
+// with the moments summed in a different (fixed) order.
+// ------------------------------------------------------------------------------------------
+namespace wv {
+
+constexpr int kSlots = 8;      // tasks per pass
+constexpr int kHMax = 16;      // hypotheses per batch
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ int prefix_below(uint64_t bal) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dppf(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the 8 lanes of a task slot (lanes 8q..8q+7); every lane gets the same bits.
+__device__ __forceinline__ float red8(float v) {
+    v += dppf<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dppf<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dppf<0x141>(v);  // row_half_mirror
+    return v;
+}
+
+// Window grid of ncc_group: offsets -r, -r+inc, ..., <= r on each axis.
+__device__ __forceinline__ int grid_side(const DevParams& P) { return 2 * P.r / P.inc + 1; }
+
+struct WaveLds {
+    float4* samp;      // compacted valid window samples {i, j, x = I - I_c, w}, zero-padded
+    float4* hyp;       // kHMax hypotheses of the current batch
+    float* cost;       // [kHMax][K] NCC cost of (hypothesis, view); 2 = not scored
+    uint16_t* task;    // (h << 8) | k
+    float* aux;        // K floats (per-view probabilities / cumulative sums)
+    int* iaux;         // 2*K ints (draw weights, selected-view list)
+};
+
+__host__ __device__ inline size_t a16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+__host__ __device__ inline size_t shared_bytes(const DevParams& P) {
+    const int n = 2 * P.r + 1;
+    return a16((size_t)(P.K + 1) * P.TS * P.tile_bytes) + a16((size_t)P.WW * 4) +
+           a16((size_t)n * n * 4) + a16((size_t)P.K * 16) + a16((size_t)P.K * 4);
+}
+
+__host__ __device__ inline size_t wave_bytes(const DevParams& P) {
+    const int m = 2 * P.r / P.inc + 1;
+    const size_t ns = (size_t)((m * m + 7) & ~7);
+    return a16(ns * 16) + a16((size_t)kHMax * 16) + a16((size_t)kHMax * P.K * 4) +
+           a16((size_t)kHMax * P.K * 2) + a16((size_t)P.K * 4) + a16((size_t)P.K * 8);
+}
+
+__host__ __device__ inline size_t lds_total(const DevParams& P, int nwaves) {
+    return shared_bytes(P) + (size_t)nwaves * wave_bytes(P);
+}
+
+// Shared part: the same regions and meaning as Lds (wts/slist unused).
+__device__ __forceinline__ Lds carve_shared(unsigned char* base, const DevParams& P) {
+    Lds L;
+    const int n = 2 * P.r + 1;
+    unsigned char* q = base;
+    L.tiles = (float*)q;
+    L.htiles = (__half*)q;
+    q += a16((size_t)(P.K + 1) * P.TS * P.tile_bytes);
+    L.costp = (float*)q; q += a16((size_t)P.WW * 4);
+    L.spat = (float*)q; q += a16((size_t)n * n * 4);
+    L.step = (float2*)q; q += (size_t)P.K * 8;
+    L.cdiff = (float2*)q; q += (size_t)P.K * 8;
+    q = base + (a16((size_t)(P.K + 1) * P.TS * P.tile_bytes) + a16((size_t)P.WW * 4) +
+                a16((size_t)n * n * 4) + a16((size_t)P.K * 16));
+    L.nid = (int*)q;
+    L.wts = nullptr;
+    L.slist = nullptr;
+    return L;
+}
+
+__device__ __forceinline__ WaveLds carve_wave(unsigned char* base, const DevParams& P, int w) {
+    const int m = 2 * P.r / P.inc + 1;
+    const size_t ns = (size_t)((m * m + 7) & ~7);
+    unsigned char* q = base + shared_bytes(P) + (size_t)w * wave_bytes(P);
+    WaveLds S;
+    S.samp = (float4*)q; q += a16(ns * 16);
+    S.hyp = (float4*)q; q += a16((size_t)kHMax * 16);
+    S.cost = (float*)q; q += a16((size_t)kHMax * P.K * 4);
+    S.task = (uint16_t*)q; q += a16((size_t)kHMax * P.K * 2);
+    S.aux = (float*)q; q += a16((size_t)P.K * 4);
+    S.iaux = (int*)q;
+    return S;
+}
+
+// Compact the pixel's valid window samples (reference point inside the crop) into S.samp, in
+// the window order of ncc_group (i outer, j inner), zero-padded to a multiple of 8.
+template <typename TT>
+__device__ int build_samples(const Lds& L, const DevParams& P, const WaveLds& S, int px, int py,
+                             float rc) {
+    const int lane = lane_id();
+    const int r = P.r, inc = P.inc, W = P.W, n = 2 * r + 1, m = grid_side(P);
+    const float inv_sc = TileScale<TT>::inv_sc(P);
+    int total = 0;
+    for (int b = 0; b < m * m; b += 64) {
+        const int w = b + lane;
+        bool valid = false;
+        float4 rec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (w < m * m) {
+            const int ii = w / m, jj = w - ii * m;
+            const int i = -r + ii * inc, j = -r + jj * inc;
+            const int rx = px + i, ry = py + j;
+            valid = (unsigned)rx < (unsigned)W && (unsigned)ry < (unsigned)W;
+            if (valid) {
+                const float x = ldt(tile_base<TT>(L) + ry * P.W1 + rx) - rc;
+                const float wgt = __expf(L.spat[(j + r) * n + (i + r)] - fabsf(x) * inv_sc);
+                rec = make_float4((float)i, (float)j, x, wgt);
+            }
+        }
+        const uint64_t bal = __ballot(valid);
+        if (valid) S.samp[total + prefix_below(bal)] = rec;
+        total += __popcll(bal);
+    }
+    const int pad = ((total + 7) & ~7) - total;
+    if (lane < pad) S.samp[total + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return total;
+}
+
+// Append the tasks of hypothesis slot h (plane hv) over the views in vmask (k < c) whose anchor
+// stays inside the crop (the act test of ncc_group); clears the hypothesis' cost row to 2.
+__device__ __forceinline__ int add_tasks(const Lds& L, const DevParams& P, const WaveLds& S,
+                                         int h, float4 hv, uint32_t vmask, int c, int px, int py,
+                                         int ntask) {
+    const int lane = lane_id();
+    const float Wf = (float)P.W;
+    for (int k = lane; k < P.K; k += 64) S.cost[h * P.K + k] = 2.0f;
+    const float A = -hv.x / hv.z, B = -hv.y / hv.z;
+    const bool hyp_ok = fabsf(hv.z) >= P.cos_max_slant && isfinite(A) && isfinite(B);
+    bool act = false;
+    if (lane < c && hyp_ok && ((vmask >> lane) & 1u) && L.nid[lane] >= 0) {
+        const float2 st = L.step[lane];
+        const float ax = (float)px + st.x * hv.w, ay = (float)py + st.y * hv.w;
+        act = ax >= 0.0f && ax < Wf && ay >= 0.0f && ay < Wf;
+    }
+    const uint64_t bal = __ballot(act);
+    if (act) S.task[ntask + prefix_below(bal)] = (uint16_t)((h << 8) | lane);
+    return ntask + __popcll(bal);
+}
+
+// Evaluate the task list: S.cost[h][k] for every task.  ns = valid window samples.
+template <bool COUNT, typename TT>
+__device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, int ntask, int ns,
+                          int px, int py, float rc, WorkCount& wc) {
+    const int lane = lane_id();
+    const int q = lane >> 3, u = lane & 7;
+    const int T = (ns + 7) >> 3;
+    const float Wm1 = (float)(P.W - 1), W1f = (float)P.W1;
+    const float pxf = (float)px, pyf = (float)py;
+    const float min_var = TileScale<TT>::min_var(P);
+    const TT* tiles = tile_base<TT>(L);
+    const int W1 = P.W1;
+    for (int t0 = 0; t0 < ntask; t0 += kSlots) {
+        const int t = t0 + q;
+        const bool live = t < ntask;
+        int h = 0, k = 0;
+        float4 hv = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+        float2 st = make_float2(0.0f, 0.0f);
+        int base = 0;
+        if (live) {
+            const int e = S.task[t];
+            h = e >> 8;
+            k = e & 255;
+            hv = S.hyp[h];
+            st = L.step[k];
+            base = (k + 1) * P.TS;
+        }
+        const float A = -hv.x / hv.z, B = -hv.y / hv.z;
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f, a5 = 0.0f;
+#pragma unroll 2
+        for (int it = 0; it < T; ++it) {
+            const int s = (it << 3) | u;
+            const float4 rec = S.samp[s];
+            const float d = fmaf(B, rec.y, fmaf(A, rec.x, hv.w));
+            const float qx = fmaf(st.x, d, pxf + rec.x), qy = fmaf(st.y, d, pyf + rec.y);
+            const float fx0 = floorf(qx), fy0 = floorf(qy);
+            const float xa = __builtin_amdgcn_fmed3f(fx0, 0.0f, Wm1);
+            const float ya = __builtin_amdgcn_fmed3f(fy0, 0.0f, Wm1);
+            // 0 <= q < W on both axes  <=>  the clamped cell is the cell itself
+            const bool ok = live && xa == fx0 && ya == fy0;
+            const float fx = qx - fx0, fy = qy - fy0;
+            const TT* tp = tiles + base + (int)(ya * W1f + xa);
+            const float ta = ldt(tp), tb = ldt(tp + 1), tc = ldt(tp + W1), te = ldt(tp + W1 + 1);
+            const float top = ta + fx * (tb - ta);
+            const float bot = tc + fx * (te - tc);
+            const float y = (top + fy * (bot - top)) - rc;
+            // masked by select, so a non-finite interpolant of a rejected sample adds nothing
+            const float wm = ok ? rec.w : 0.0f;
+            const float yv = ok ? y : 0.0f;
+            const float wmx = wm * rec.z;
+            const float wy = wm * yv;
+            if (COUNT && ok && s < ns) ++wc.samples;
+            a0 += wm;
+            a1 += wmx;
+            a2 = fmaf(wmx, rec.z, a2);
+            a3 += wy;
+            a4 = fmaf(wy, yv, a4);
+            a5 = fmaf(wy, rec.z, a5);
+        }
+        a0 = red8(a0); a1 = red8(a1); a2 = red8(a2);
+        a3 = red8(a3); a4 = red8(a4); a5 = red8(a5);
+        float cst = 2.0f;
+        if (live && a0 > 1e-6f) {
+            const float inv = 1.0f / a0;
+            const float mx = a1 * inv, my = a3 * inv;
+            const float vx = a2 * inv - mx * mx;
+            const float vy = a4 * inv - my * my;
+            if (!(vx < min_var || vy < min_var)) {
+                const float cov = a5 * inv - mx * my;
+                cst = fminf(2.0f, fmaxf(0.0f, 1.0f - cov / sqrtf(vx * vy)));
+            }
+        }
+        if (live && u == 0) {
+            S.cost[h * P.K + k] = cst;
+            if (COUNT) ++wc.triples;
+        }
+    }
+    wave_sync();
+}
+
+// Weighted cost of hypothesis row h over the selected views (ComputeWeightedCost_Frame
+// :284-317), summed in selection order.
+__device__ __forceinline__ float wsum(const DevParams& P, const WaveLds& S, int h, int nsel) {
+    float wn = 0.0f, tot = 0.0f;
+    for (int j = 0; j < nsel; ++j) {
+        const int k = S.iaux[P.K + j];
+        const float w = (float)S.iaux[k];
+        wn += w;
+        tot += w * S.cost[h * P.K + k];
+    }
+    return wn <= 1e-6f ? 2.0f : tot / wn;
+}
+
+// Random initialisation of one pixel (RandomInitializationForFrame :474-518).
+template <bool COUNT, typename TT>
+__device__ void init_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
+                           const WaveLds& S, int v, int c, int pix, WorkCount& wc) {
+    const int W = P.W, K = P.K;
+    const int x = pix % W, y = pix / W;
+    const size_t idx = (size_t)v * P.WW + pix;
+    const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
+    const int ns = build_samples<TT>(L, P, S, x, y, rc);
+    DevStream st((uint32_t)idx, 0u, F.seed, F.uid);
+    float4 h;
+    h.w = u01_dev(st.raw(SLOT_INIT_DEPTH)) * (P.dmax - P.dmin) + P.dmin;
+    random_normal(st, SLOT_INIT_MARSAGLIA, h.x, h.y, h.z);
+    if (lane_id() == 0) S.hyp[0] = h;
+    const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
+    const int nt = add_tasks(L, P, S, 0, h, allv, c, x, y, 0);
+    wave_sync();
+    run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
+    // top-k mean and selection mask (ComputeSelectedViewsTopK_Frame :227-282)
+    int valid = 0;
+    for (int k = 0; k < c; ++k) valid += S.cost[k] < 2.0f;
+    float mean = 2.0f;
+    uint32_t mask = 0u;
+    if (valid > 0) {
+        const int kk = min(valid, P.top_k);
+        uint32_t taken = 0u;
+        float sum = 0.0f, thr = 2.0f;
+        for (int mth = 0; mth < kk; ++mth) {
+            int bi = -1;
+            float bv = 0.0f;
+            for (int k = 0; k < c; ++k) {
+                if ((taken >> k) & 1u) continue;
+                const float cv = S.cost[k];
+                if (bi < 0 || cv < bv) { bi = k; bv = cv; }
+            }
+            taken |= 1u << bi;
+            sum += bv;
+            thr = bv;
+        }
+        mean = sum / (float)kk;
+        for (int k = 0; k < c; ++k)
+            if (S.cost[k] <= thr) mask |= 1u << k;
+    }
+    (void)K;
+    if (lane_id() == 0) {
+        F.plane_out[idx] = h;
+        F.cost_out[idx] = mean;
+        F.sel_out[idx] = mask;
+    }
+}
+
+// Lowest-cost view record of the final plane (ComputeWeightedCost_Frame's disp, :303-306).
+template <bool COUNT, typename TT>
+__device__ void final_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
+                            const WaveLds& S, int v, int c, int pix, WorkCount& wc) {
+    const int W = P.W;
+    const int x = pix % W, y = pix / W;
+    const size_t idx = (size_t)v * P.WW + pix;
+    const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
+    const int ns = build_samples<TT>(L, P, S, x, y, rc);
+    const float4 h = F.plane_in[idx];
+    if (lane_id() == 0) S.hyp[0] = h;
+    const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
+    const int nt = add_tasks(L, P, S, 0, h, allv, c, x, y, 0);
+    wave_sync();
+    run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
+    float mc = 2.0f;
+    int am = -1;
+    for (int k = 0; k < c; ++k) {
+        const float cv = S.cost[k];
+        if (cv < mc) { mc = cv; am = k; }
+    }
+    if (lane_id() == 0)
+        F.disp[idx] = am >= 0 ? disp_record(L, P, am, h.w) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// One red/black update of one pixel (CheckerboardUpdateFrame :520-947).
+template <bool COUNT, typename TT>
+__device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
+                            const WaveLds& S, int v, int c, int x, int y, WorkCount& wc) {
+    const int lane = lane_id();
+    const int W = P.W, WW = P.WW, K = P.K;
+    const int phase = F.phase;
+    const size_t base = (size_t)v * WW;
+    const int center = y * W + x;
+    const size_t idx = base + center;
+    const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
+    const int ns = build_samples<TT>(L, P, S, x, y, rc);
+
+    // batch 1: the current plane (row 0) and the propagation candidates (rows 1..)
+    int pos[8], src[8];
+    select_candidates(L.costp, W, x, y, pos, src);
+    int row[8];   // cost row of the plane candidate slot cd reads, -1 if absent
+    int nh = 1;
+    const float4 hcur = F.plane_in[idx];
+    if (lane == 0) S.hyp[0] = hcur;
+#pragma unroll
+    for (int cd = 0; cd < 8; ++cd) {
+        row[cd] = -1;
+        if (src[cd] == cd) {
+            row[cd] = nh;
+            if (lane == 0) S.hyp[nh] = F.plane_in[base + pos[cd]];
+            ++nh;
+        } else if (cd & 1) {
+            if (src[cd] == cd - 1) row[cd] = row[cd - 1];  // border duplicate of the near slot
+        }
+    }
+    wave_sync();
+    const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
+    int nt = 0;
+    for (int h = 0; h < nh; ++h) nt = add_tasks(L, P, S, h, S.hyp[h], allv, c, x, y, nt);
+    wave_sync();
+    run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
+
+    // view-selection priors from the 4 direct neighbours' previous selections
+    uint32_t dsel[4];
+    bool dval[4];
+    dval[0] = y > 0; dval[1] = y < W - 1; dval[2] = x > 0; dval[3] = x < W - 1;
+    dsel[0] = dval[0] ? F.sel_in[idx - W] : 0u;
+    dsel[1] = dval[1] ? F.sel_in[idx + W] : 0u;
+    dsel[2] = dval[2] ? F.sel_in[idx - 1] : 0u;
+    dsel[3] = dval[3] ? F.sel_in[idx + 1] : 0u;
+
+    // sampling probabilities, lane k = view k
+    const float thr = 0.8f * expf((float)(F.iter * F.iter) / (-90.0f));
+    if (lane < c) {
+        const int k = lane;
+        float count = 0.0f, tmpw = 0.0f;
+        int cf = 0;
+#pragma unroll
+        for (int cd = 0; cd < 8; ++cd) {
+            if (row[cd] < 0) continue;
+            const float cv = S.cost[row[cd] * K + k];
+            if (cv < thr) { tmpw += expf(cv * cv / (-0.18f)); count += 1.0f; }
+            if (cv > 1.2f) ++cf;
+        }
+        float pr = 0.0f;
+        if (count > 2.0f && cf < 3) pr = tmpw / count;
+        else if (cf < 3) pr = expf(thr * thr / (-0.32f));
+        float prior = 0.0f;
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii)
+            if (dval[ii]) prior += ((dsel[ii] >> k) & 1u) ? 0.9f : 0.1f;
+        S.aux[k] = pr * prior;
+    }
+    wave_sync();
+    // cumulative distribution in view order (sequential, as the reference sums)
+    float psum = 0.0f;
+    for (int k = 0; k < c; ++k) psum += S.aux[k];
+    DevStream rs((uint32_t)idx, (uint32_t)(1 + 2 * F.iter + phase), F.seed, F.uid);
+    uint32_t tsel = 0u;
+    int nsel = 0;
+    if (psum > 0.0f) {
+        const float inv = 1.0f / psum;
+        // lane s < 15 owns draw s: the first view whose cumulative probability exceeds it
+        float draw = 2.0f;
+        if (lane < (int)VIEW_DRAWS) {
+            DevStream ls((uint32_t)idx, (uint32_t)(1 + 2 * F.iter + phase), F.seed, F.uid);
+            draw = u01_dev(ls.raw(SLOT_VIEW_DRAW + lane)) - 1.1920928955078125e-07f;
+        }
+        int vs = -1;
+        float cum = 0.0f;
+        for (int k = 0; k < c; ++k) {
+            cum += S.aux[k] * inv;
+            if (vs < 0 && cum > draw) vs = k;
+        }
+        for (int k = 0; k < c; ++k) {
+            const int wk = __popcll(__ballot(lane < (int)VIEW_DRAWS && vs == k));
+            if (wk > 0) {
+                tsel |= 1u << k;
+                if (lane == 0) S.iaux[K + nsel] = k;
+                ++nsel;
+            }
+            if (lane == 0) S.iaux[k] = wk;
+        }
+    } else {
+        for (int k = lane; k < c; k += 64) S.iaux[k] = 0;
+    }
+    if (nsel == 0) {
+        for (int k = lane; k < c; k += 64) {
+            S.iaux[k] = 1;
+            S.iaux[K + k] = k;
+        }
+        tsel = allv;
+        nsel = c;
+    }
+    wave_sync();
+
+    // current plane and propagation candidates under the selected views
+    float4 best = hcur;
+    float bcost = wsum(P, S, 0, nsel);
+#pragma unroll
+    for (int cd = 0; cd < 8; ++cd) {
+        if (row[cd] < 0) continue;
+        const float cc = wsum(P, S, row[cd], nsel);
+        if (cc < bcost) { bcost = cc; best = S.hyp[row[cd]]; }
+    }
+
+    // cross-view proposals through each selected view (BuildCrossViewPlaneProposal :350-398)
+    for (int j = 0; j < nsel; ++j) {
+        const int k = S.iaux[K + j];
+        const int n = L.nid[k];
+        if (n < 0) continue;
+        const float2 sk = L.step[k];
+        const float qfx = (float)x + sk.x * best.w, qfy = (float)y + sk.y * best.w;
+        const float fqx = floorf(qfx + 0.5f), fqy = floorf(qfy + 0.5f);
+        if (!(fqx >= 0.0f && fqx < (float)W && fqy >= 0.0f && fqy < (float)W)) continue;
+        const int qx = (int)fqx, qy = (int)fqy;
+        const float4 hn = F.plane_in[(size_t)n * WW + qy * W + qx];
+        const float al = -hn.x / hn.z, be = -hn.y / hn.z;
+        const float ga = hn.w - al * (float)qx - be * (float)qy;
+        const float2 cdv = L.cdiff[k];
+        float den = 1.0f + al * cdv.x + be * cdv.y;
+        if (fabsf(den) < 1e-6f) den = den >= 0.0f ? 1e-6f : -1e-6f;
+        const float aR = al / den, bR = be / den, cR = ga / den;
+        float nx = -aR, ny = -bR, nz = 1.0f;
+        normalize3(nx, ny, nz);
+        const float4 prop = make_float4(nx, ny, nz, aR * (float)x + bR * (float)y + cR);
+        wave_sync();
+        if (lane == 0) S.hyp[0] = prop;
+        const int npt = add_tasks(L, P, S, 0, prop, tsel, c, x, y, 0);
+        wave_sync();
+        run_tasks<COUNT, TT>(L, P, S, npt, ns, x, y, rc, wc);
+        const float cc = wsum(P, S, 0, nsel);
+        if (cc < bcost) { bcost = cc; best = prop; }
+    }
+
+    // refinement: ACMP depth x normal candidates (AdaptMIPMUtil.cu:181-183)
+    {
+        const float pert = 0.02f;
+        const float dnow = best.w;
+        float drand;
+        if (bcost < 0.1f)
+            drand = (u01_dev(rs.raw(SLOT_REF_DEPTH_RAND)) - 0.5f) * 2.0f + dnow;
+        else
+            drand = u01_dev(rs.raw(SLOT_REF_DEPTH_RAND)) * (P.dmax - P.dmin) + P.dmin;
+        float rnx, rny, rnz;
+        random_normal(rs, SLOT_REF_MARSAGLIA, rnx, rny, rnz);
+        float vx = (float)x, vy = (float)y, vz = 1.0f;
+        normalize3(vx, vy, vz);
+        const float pamp = pert * 3.14159265358979323846f;
+        const float a1 = (u01_dev(rs.raw(SLOT_REF_ANGLE)) - 0.5f) * pamp;
+        const float a2 = (u01_dev(rs.raw(SLOT_REF_ANGLE + 1)) - 0.5f) * pamp;
+        const float a3 = (u01_dev(rs.raw(SLOT_REF_ANGLE + 2)) - 0.5f) * pamp;
+        const float s1 = sinf(a1), s2 = sinf(a2), s3 = sinf(a3);
+        const float c1 = cosf(a1), c2 = cosf(a2), c3 = cosf(a3);
+        float px_ = (c2 * c3) * best.x + (c3 * s1 * s2 - c1 * s3) * best.y +
+                    (s1 * s3 + c1 * c3 * s2) * best.z;
+        float py_ = (c2 * s3) * best.x + (c1 * c3 + s1 * s2 * s3) * best.y +
+                    (c1 * s2 * s3 - c3 * s1) * best.z;
+        float pz_ = (-s2) * best.x + (c2 * s1) * best.y + (c1 * c2) * best.z;
+        if (px_ * vx + py_ * vy + pz_ * vz >= 0.0f) { px_ = best.x; py_ = best.y; pz_ = best.z; }
+        normalize3(px_, py_, pz_);
+        const float lo = (1.0f - pert) * dnow, hi = (1.0f + pert) * dnow;
+        const float dpert = u01_dev(rs.raw(SLOT_REF_DEPTH_PERT)) * (hi - lo) + lo;
+        const float4 cur = best;
+        wave_sync();
+        int nrt = 0;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            // (n_cur, d_rand), (n_rand, d_now), (n_rand, d_rand), (n_pert, d_now), (n_cur, d_pert)
+            float4 hc;
+            hc.x = (i == 0 || i == 4) ? cur.x : (i == 3 ? px_ : rnx);
+            hc.y = (i == 0 || i == 4) ? cur.y : (i == 3 ? py_ : rny);
+            hc.z = (i == 0 || i == 4) ? cur.z : (i == 3 ? pz_ : rnz);
+            hc.w = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dnow);
+            if (lane == 0) S.hyp[i] = hc;
+            nrt = add_tasks(L, P, S, i, hc, tsel, c, x, y, nrt);
+        }
+        wave_sync();
+        run_tasks<COUNT, TT>(L, P, S, nrt, ns, x, y, rc, wc);
+        for (int i = 0; i < 5; ++i) {
+            const float cc = wsum(P, S, i, nsel);
+            if (cc < bcost) { bcost = cc; best = S.hyp[i]; }
+        }
+    }
+    if (lane == 0) {
+        F.plane_out[idx] = best;
+        F.cost_out[idx] = bcost;
+        F.sel_out[idx] = tsel;
+    }
+    wave_sync();
+}
+
+template <bool COUNT, typename TT>
+__global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const DevParams& P = a.P;
+    const int NT = blockDim.x, t = threadIdx.x, WW = P.WW, W = P.W;
+    const int wave = t >> 6, nwaves = NT >> 6;
+    Lds L = carve_shared(lds_raw, P);
+    const WaveLds S = carve_wave(lds_raw, P, wave);
+    fill_spatial(L, P);
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int lo = (int)(((long long)a.total * xcd) >> 3);
+    const int hi = (int)(((long long)a.total * (xcd + 1)) >> 3);
+    WorkCount wc{0u, 0u};
+    int f = 0;
+    for (int gm = lo + local; gm < hi; gm += P.slots_per_xcd) {
+        while (f + 1 < a.nf && gm >= a.f[f + 1].mi_begin) ++f;
+        const FrameDesc& F = a.f[f];
+        const int stage = F.stage;
+        if (stage == STAGE_IDLE) continue;  // uniform across the workgroup
+        const int v = gm - F.mi_begin;
+        __syncthreads();
+        const int c = stage_mi<TT>(L, P, v, F.tiles, F.centers, F.nbr, F.cnt);
+        if (stage == STAGE_PHASE)
+            for (int i = t; i < WW; i += NT) L.costp[i] = F.cost_in[(size_t)v * WW + i];
+        __syncthreads();
+        if (stage == STAGE_PHASE) {
+            const size_t base = (size_t)v * WW;
+            const int phase = F.phase;
+            // pass-through of the inactive colour (and of every pixel of an MI without views)
+            for (int i = t; i < WW; i += NT) {
+                const int x = i % W, y = i / W;
+                if (((x + y) & 1) != phase || c <= 0) {
+                    F.plane_out[base + i] = F.plane_in[base + i];
+                    F.cost_out[base + i] = F.cost_in[base + i];
+                    F.sel_out[base + i] = F.sel_in[base + i];
+                }
+            }
+            if (c > 0) {
+                const int nact = (W & 1) ? (phase == 0 ? (WW + 1) / 2 : WW / 2) : WW / 2;
+                for (int aidx = wave; aidx < nact; aidx += nwaves) {
+                    int x, y;
+                    active_pixel(W, aidx, phase, x, y);
+                    phase_pixel<COUNT, TT>(L, P, F, S, v, c, x, y, wc);
+                }
+            }
+        } else if (stage == STAGE_INIT) {
+            for (int pix = wave; pix < WW; pix += nwaves) init_pixel<COUNT, TT>(L, P, F, S, v, c, pix, wc);
+        } else {
+            for (int pix = wave; pix < WW; pix += nwaves) final_pixel<COUNT, TT>(L, P, F, S, v, c, pix, wc);
+        }
+    }
+    flush_counts<COUNT>(a.counters, wc);
+}
+
+}  // namespace wv
+
+// ------------------------------------------------------------------------------------------
 // Fixed-plane cost evaluation (hypotheses sorted by MI; one workgroup per MI)
 // ------------------------------------------------------------------------------------------
 struct EvalArgs {
@@ -1001,6 +1595,50 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs a) {
                 for (int g = 0; g < kGroup; ++g)
                     if (k0 + g < K) a.out[(size_t)hi * K + k0 + g] = cg[g];
             }
+        }
+    }
+    flush_counts<COUNT>(a.counters, wc);
+}
+
+// Wave-per-hypothesis form of k_eval (the default): one wavefront scores one record over all
+// of its MI's views with the task engine of wv::run_tasks.
+template <bool COUNT, typename TT>
+__global__ void __launch_bounds__(256) k_eval_w(EvalArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const DevParams& P = a.P;
+    const int NT = blockDim.x, W = P.W, WW = P.WW, K = P.K;
+    const int wave = threadIdx.x >> 6, nwaves = NT >> 6, lane = threadIdx.x & 63;
+    Lds L = wv::carve_shared(lds_raw, P);
+    const wv::WaveLds S = wv::carve_wave(lds_raw, P, wave);
+    fill_spatial(L, P);
+    int first, end, stride;
+    mi_range(P, first, end, stride);
+    WorkCount wc{0u, 0u};
+    for (int mi = first; mi < end; mi += stride) {
+        const int h0 = a.seg[mi], h1 = a.seg[mi + 1];
+        if (h0 >= h1) continue;  // uniform across the workgroup
+        __syncthreads();
+        const int c = stage_mi<TT>(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
+        __syncthreads();
+        for (int hi = h0 + wave; hi < h1; hi += nwaves) {
+            const int* rec = a.hyp + (size_t)hi * 6;
+            const int pix = rec[1];
+            if (pix < 0 || pix >= WW || rec[0] != mi) {  // malformed record: no score
+                for (int k = lane; k < K; k += 64) a.out[(size_t)hi * K + k] = 2.0f;
+                continue;
+            }
+            const float4 h = make_float4(__int_as_float(rec[2]), __int_as_float(rec[3]),
+                                         __int_as_float(rec[4]), __int_as_float(rec[5]));
+            const int x = pix % W, y = pix / W;
+            const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
+            const int ns = wv::build_samples<TT>(L, P, S, x, y, rc);
+            if (lane == 0) S.hyp[0] = h;
+            const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
+            const int nt = wv::add_tasks(L, P, S, 0, h, allv, c, x, y, 0);
+            wv::wave_sync();
+            wv::run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
+            for (int k = lane; k < K; k += 64) a.out[(size_t)hi * K + k] = S.cost[k];
+            wv::wave_sync();
         }
     }
     flush_counts<COUNT>(a.counters, wc);
@@ -1394,7 +2032,9 @@ static thread_local unsigned long long* g_counters = nullptr;
 static const int kPhasePerCu = 8;    // k_stage workgroups per CU (persistent slots), default
 static const int kMaxSlotsPerCu = 8; // upper bound of the autotuner's search
 
+static bool wave_solver();
 static size_t stage_scratch_bytes(const DevParams& P, int grid) {
+    if (wave_solver()) return 256;  // the wave engine keeps its per-pixel state in LDS
     return (size_t)grid * 9 * P.K * phase_threads(P) * 4;
 }
 static size_t phase_scratch_bytes(const DevParams& P, int per_cu) {
@@ -1410,6 +2050,28 @@ static FrameDesc frame_desc(const float* tiles, const float* centers, const int*
     return f;
 }
 
+// Solver engine: wave-per-pixel task engine (default) or the lane-per-pixel kernels;
+// LFPM_SOLVER=lane selects the latter (kept for A/B measurements).  Both give results within
+// the parity tolerances; each is deterministic.
+static int g_solver = -1;
+static bool wave_solver() {
+    if (g_solver < 0) {
+        const char* e = std::getenv("LFPM_SOLVER");
+        g_solver = (e && std::strcmp(e, "lane") == 0) ? 0 : 1;
+    }
+    return g_solver == 1;
+}
+static const int kWaveThreads = 256;
+
+// Workgroups per CU that fit (registers, LDS) for a kernel at this block size and LDS.
+template <typename KERN>
+static int fit_per_cu(KERN kern, int threads, size_t lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, threads, lds) != hipSuccess || n < 1)
+        return 1;
+    return n;
+}
+
 // Launch k_stage over the frames of `a` (mi_begin/total filled here).
 static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_bytes,
                         hipStream_t s) {
@@ -1420,6 +2082,31 @@ static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_
     }
     a.total = total;
     if (total <= 0) return LFPM_OK;
+    if (wave_solver()) {
+        const int NT = kWaveThreads;
+        const size_t lds = wv::lds_total(a.P, NT / 64);
+        if (lds > 65536)
+            return set_error(LFPM_E_UNSUPPORTED, "k_stage: %zu B of LDS per workgroup (crop %d, "
+                             "%d views, radius %d) exceeds 64 KiB", lds, a.P.W, a.P.K, a.P.r);
+        const bool fp16 = a.P.tile_bytes == 2;
+        int fit;
+        if (g_counters) fit = fp16 ? fit_per_cu(wv::k_stage_w<true, __half>, NT, lds)
+                                   : fit_per_cu(wv::k_stage_w<true, float>, NT, lds);
+        else fit = fp16 ? fit_per_cu(wv::k_stage_w<false, __half>, NT, lds)
+                        : fit_per_cu(wv::k_stage_w<false, float>, NT, lds);
+        const int grid = persistent_grid_n(total, std::max(1, std::min(per_cu, fit)));
+        a.P.slots_per_xcd = grid / 8;
+        a.scratch = nullptr;
+        a.counters = g_counters;
+        if (g_counters) {
+            if (fp16) hipLaunchKernelGGL((wv::k_stage_w<true, __half>), dim3(grid), dim3(NT), lds, s, a);
+            else hipLaunchKernelGGL((wv::k_stage_w<true, float>), dim3(grid), dim3(NT), lds, s, a);
+        } else {
+            if (fp16) hipLaunchKernelGGL((wv::k_stage_w<false, __half>), dim3(grid), dim3(NT), lds, s, a);
+            else hipLaunchKernelGGL((wv::k_stage_w<false, float>), dim3(grid), dim3(NT), lds, s, a);
+        }
+        return launch_check("k_stage");
+    }
     const int grid = persistent_grid_n(total, per_cu);
     a.P.slots_per_xcd = grid / 8;
     const int NT = phase_threads(a.P);
@@ -1696,14 +2383,30 @@ int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,
     a.cnt = neighbor_counts_dev; a.hyp = (const int*)hyp_dev; a.seg = seg; a.n = n;
     a.out = cost_out_dev; a.P = P; a.counters = g_counters;
     const int NT = 256;
-    const size_t lds = lds_bytes(P, NT, false);
     const bool fp16 = P.tile_bytes == 2;
-    if (g_counters) {
-        if (fp16) hipLaunchKernelGGL((k_eval<true, __half>), dim3(grid), dim3(NT), lds, s, a);
-        else hipLaunchKernelGGL((k_eval<true, float>), dim3(grid), dim3(NT), lds, s, a);
+    if (wave_solver()) {
+        const size_t lds = wv::lds_total(P, NT / 64);
+        if (lds > 65536) {
+            (void)hipFreeAsync(seg, s);
+            return set_error(LFPM_E_UNSUPPORTED, "eval: %zu B of LDS per workgroup exceeds 64 KiB",
+                             lds);
+        }
+        if (g_counters) {
+            if (fp16) hipLaunchKernelGGL((k_eval_w<true, __half>), dim3(grid), dim3(NT), lds, s, a);
+            else hipLaunchKernelGGL((k_eval_w<true, float>), dim3(grid), dim3(NT), lds, s, a);
+        } else {
+            if (fp16) hipLaunchKernelGGL((k_eval_w<false, __half>), dim3(grid), dim3(NT), lds, s, a);
+            else hipLaunchKernelGGL((k_eval_w<false, float>), dim3(grid), dim3(NT), lds, s, a);
+        }
     } else {
-        if (fp16) hipLaunchKernelGGL((k_eval<false, __half>), dim3(grid), dim3(NT), lds, s, a);
-        else hipLaunchKernelGGL((k_eval<false, float>), dim3(grid), dim3(NT), lds, s, a);
+        const size_t lds = lds_bytes(P, NT, false);
+        if (g_counters) {
+            if (fp16) hipLaunchKernelGGL((k_eval<true, __half>), dim3(grid), dim3(NT), lds, s, a);
+            else hipLaunchKernelGGL((k_eval<true, float>), dim3(grid), dim3(NT), lds, s, a);
+        } else {
+            if (fp16) hipLaunchKernelGGL((k_eval<false, __half>), dim3(grid), dim3(NT), lds, s, a);
+            else hipLaunchKernelGGL((k_eval<false, float>), dim3(grid), dim3(NT), lds, s, a);
+        }
     }
     rc = launch_check("eval");
     HIP_TRY(hipFreeAsync(seg, s));
