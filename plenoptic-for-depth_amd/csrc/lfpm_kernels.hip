@@ -987,6 +987,7 @@ __device__ __forceinline__ int grid_side(const DevParams& P) { return 2 * P.r / 
 struct WaveLds {
     float4* samp;      // compacted valid window samples {i, j, x = I - I_c, w}, zero-padded
     float4* hyp;       // kHMax hypotheses of the current batch
+    float2* hab;       // kHMax disparity slopes (A, B) = (-nx/nz, -ny/nz) of the hypotheses
     float* cost;       // [kHMax][K] NCC cost of (hypothesis, view); 2 = not scored
     uint16_t* task;    // (h << 8) | k
     float* aux;        // K floats (per-view probabilities / cumulative sums)
@@ -1004,8 +1005,9 @@ __host__ __device__ inline size_t shared_bytes(const DevParams& P) {
 __host__ __device__ inline size_t wave_bytes(const DevParams& P) {
     const int m = 2 * P.r / P.inc + 1;
     const size_t ns = (size_t)((m * m + 7) & ~7);
-    return a16(ns * 16) + a16((size_t)kHMax * 16) + a16((size_t)kHMax * P.K * 4) +
-           a16((size_t)kHMax * P.K * 2) + a16((size_t)P.K * 4) + a16((size_t)P.K * 8);
+    return a16(ns * 16) + a16((size_t)kHMax * 16) + a16((size_t)kHMax * 8) +
+           a16((size_t)kHMax * P.K * 4) + a16((size_t)kHMax * P.K * 2) + a16((size_t)P.K * 4) +
+           a16((size_t)P.K * 8);
 }
 
 __host__ __device__ inline size_t lds_total(const DevParams& P, int nwaves) {
@@ -1039,6 +1041,7 @@ __device__ __forceinline__ WaveLds carve_wave(unsigned char* base, const DevPara
     WaveLds S;
     S.samp = (float4*)q; q += a16(ns * 16);
     S.hyp = (float4*)q; q += a16((size_t)kHMax * 16);
+    S.hab = (float2*)q; q += a16((size_t)kHMax * 8);
     S.cost = (float*)q; q += a16((size_t)kHMax * P.K * 4);
     S.task = (uint16_t*)q; q += a16((size_t)kHMax * P.K * 2);
     S.aux = (float*)q; q += a16((size_t)P.K * 4);
@@ -1089,6 +1092,7 @@ __device__ __forceinline__ int add_tasks(const Lds& L, const DevParams& P, const
     for (int k = lane; k < P.K; k += 64) S.cost[h * P.K + k] = 2.0f;
     const float A = -hv.x / hv.z, B = -hv.y / hv.z;
     const bool hyp_ok = fabsf(hv.z) >= P.cos_max_slant && isfinite(A) && isfinite(B);
+    if (lane == 0) S.hab[h] = make_float2(A, B);
     bool act = false;
     if (lane < c && hyp_ok && ((vmask >> lane) & 1u) && L.nid[lane] >= 0) {
         const float2 st = L.step[lane];
@@ -1116,25 +1120,28 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
         const int t = t0 + q;
         const bool live = t < ntask;
         int h = 0, k = 0;
-        float4 hv = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
-        float2 st = make_float2(0.0f, 0.0f);
         int base = 0;
+        // source point of window offset (i, j): q = (px, py) + (i, j) + step * (w + A i + B j),
+        // i.e. q = Q0 + Qi * i + Qj * j per axis
+        float Qx0 = 0.0f, Qxi = 0.0f, Qxj = 0.0f, Qy0 = 0.0f, Qyi = 0.0f, Qyj = 0.0f;
         if (live) {
             const int e = S.task[t];
             h = e >> 8;
             k = e & 255;
-            hv = S.hyp[h];
-            st = L.step[k];
+            const float hw = S.hyp[h].w;
+            const float2 ab = S.hab[h];
+            const float2 st = L.step[k];
             base = (k + 1) * P.TS;
+            Qx0 = fmaf(st.x, hw, pxf); Qxi = fmaf(st.x, ab.x, 1.0f); Qxj = st.x * ab.y;
+            Qy0 = fmaf(st.y, hw, pyf); Qyi = st.y * ab.x; Qyj = fmaf(st.y, ab.y, 1.0f);
         }
-        const float A = -hv.x / hv.z, B = -hv.y / hv.z;
         float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f, a5 = 0.0f;
 #pragma unroll 2
         for (int it = 0; it < T; ++it) {
             const int s = (it << 3) | u;
             const float4 rec = S.samp[s];
-            const float d = fmaf(B, rec.y, fmaf(A, rec.x, hv.w));
-            const float qx = fmaf(st.x, d, pxf + rec.x), qy = fmaf(st.y, d, pyf + rec.y);
+            const float qx = fmaf(Qxj, rec.y, fmaf(Qxi, rec.x, Qx0));
+            const float qy = fmaf(Qyj, rec.y, fmaf(Qyi, rec.x, Qy0));
             const float fx0 = floorf(qx), fy0 = floorf(qy);
             const float xa = __builtin_amdgcn_fmed3f(fx0, 0.0f, Wm1);
             const float ya = __builtin_amdgcn_fmed3f(fy0, 0.0f, Wm1);
@@ -1161,15 +1168,17 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
         }
         a0 = red8(a0); a1 = red8(a1); a2 = red8(a2);
         a3 = red8(a3); a4 = red8(a4); a5 = red8(a5);
+        // epilogue once per pass: hardware reciprocal / reciprocal square root (1 ulp), the
+        // precision class of the reference's fast-math build
         float cst = 2.0f;
         if (live && a0 > 1e-6f) {
-            const float inv = 1.0f / a0;
+            const float inv = __builtin_amdgcn_rcpf(a0);
             const float mx = a1 * inv, my = a3 * inv;
             const float vx = a2 * inv - mx * mx;
             const float vy = a4 * inv - my * my;
             if (!(vx < min_var || vy < min_var)) {
                 const float cov = a5 * inv - mx * my;
-                cst = fminf(2.0f, fmaxf(0.0f, 1.0f - cov / sqrtf(vx * vy)));
+                cst = fminf(2.0f, fmaxf(0.0f, 1.0f - cov * __builtin_amdgcn_rsqf(vx * vy)));
             }
         }
         if (live && u == 0) {
