@@ -38,6 +38,10 @@
 //   fit_v2r()/v2r_convert() : VirtualToRealDepthFunc::BehavioralModel / convertVirtualToRealDepth,
 //                             FocalStack/VirtualToRealDepthFunc.cpp:3390-3451 (Householder QR
 //                             least squares in FP64 instead of cv::solve(DECOMP_SVD)).
+//   solve_acmm()            : AdaptMIPMFrameACMM (MIStereo/AdaptMIPMFrameACMM.cpp:155-665, .cu:
+//                             373-991): level schedule, INTER_AREA level images (area average),
+//                             warm start with the 3x3 seed gate, warm-seeded init, detail
+//                             restorer, top-k metadata after every update, post-refine pass.
 //   disparity_mosaic()      : DepthSolver::WriteDisMap_TileKey_new_AccuSequence,
 //                             DepthSolver.cpp:6716-6897 (float map instead of the renderings).
 //   vd_map()                : DepthSolver::Virtual_depth_map_TileKeySequence,
@@ -119,6 +123,8 @@ enum : uint32_t {
     SLOT_REF_DEPTH_PERT = 17,
     SLOT_REF_ANGLE = 20,         // 20, 21, 22
     SLOT_REF_MARSAGLIA = 24,
+    SLOT_WARM_JITTER = 40,       // ACMM warm-seed disparity jitter (init step)
+    SLOT_WARM_ANGLE = 44,        // ACMM warm-seed normal perturbation, 44..46 (init step)
     MARSAGLIA_ATTEMPTS = 16,
     VIEW_DRAWS = 15,
 };
@@ -332,13 +338,14 @@ template <typename T> static void random_normal(Stream& st, uint32_t base, T n[3
 }
 
 template <typename T>
-static P4<T> perturbed_normal(Stream& st, int px, int py, const P4<T>& h, T perturbation) {
+static P4<T> perturbed_normal(Stream& st, int px, int py, const P4<T>& h, T perturbation,
+                              uint32_t angle_slot = SLOT_REF_ANGLE) {
     T vx = (T)px, vy = (T)py, vz = (T)1;
     const T vn = std::sqrt(vx * vx + vy * vy + vz * vz);
     vx = vx / vn; vy = vy / vn; vz = vz / vn;
-    const T a1 = (uniform<T>(st.raw(SLOT_REF_ANGLE)) - (T)0.5) * perturbation;
-    const T a2 = (uniform<T>(st.raw(SLOT_REF_ANGLE + 1)) - (T)0.5) * perturbation;
-    const T a3 = (uniform<T>(st.raw(SLOT_REF_ANGLE + 2)) - (T)0.5) * perturbation;
+    const T a1 = (uniform<T>(st.raw(angle_slot)) - (T)0.5) * perturbation;
+    const T a2 = (uniform<T>(st.raw(angle_slot + 1)) - (T)0.5) * perturbation;
+    const T a3 = (uniform<T>(st.raw(angle_slot + 2)) - (T)0.5) * perturbation;
     const T s1 = std::sin(a1), s2 = std::sin(a2), s3 = std::sin(a3);
     const T c1 = std::cos(a1), c2 = std::cos(a2), c3 = std::cos(a3);
     T R[9];
@@ -419,9 +426,40 @@ static void init_pixel(const Frame<T>& f, State<T>& s, int v, int x, int y) {
     s.sel[idx] = m;
 }
 
+// ACMM extras of one level (AdaptMIPMFrameACMM.cu): warm-start seeds of the level (null when
+// the level starts from random planes).
+template <typename T> struct Acmm {
+    bool warm;
+    const P4<T>* plane_init;
+    const T* cost_init;
+    const uint32_t* sel_init;
+};
+
+static int popcount32(uint32_t v) { return __builtin_popcount(v); }
+
+// RecomputeSupportCostDisp_Frame (AdaptMIPMFrameACMM.cu:259-279): cost vector of h, top-k mean
+// and mask, disparity record of the lowest-cost view.
+template <typename T>
+static void support(const Frame<T>& f, const Win<T>& ww, int v, int x, int y, const P4<T>& h,
+                    T& cost, uint32_t& sel, P4<T>& disp) {
+    T cv[KMAX];
+    cost_vector(f, ww, v, x, y, h, cv);
+    T mean; int best;
+    sel = topk(cv, f.count(v), f.p.top_k, mean, best);
+    cost = mean;
+    disp = best >= 0 ? disp_record(f, v, best, h) : P4<T>{0, 0, 0, 0};
+}
+
+// WarmSeedIsValid_Frame (AdaptMIPMFrameACMM.cu:186-194).
+template <typename T>
+static bool warm_seed_valid(const Frame<T>& f, const P4<T>& h, T cost, uint32_t sel) {
+    return h.w >= (T)f.p.depth_min && h.w <= (T)f.p.depth_max && cost < (T)1 &&
+           popcount32(sel) >= 3;
+}
+
 template <typename T>
 static void update_pixel(const Frame<T>& f, const State<T>& prev, State<T>& next, int v, int x,
-                         int y, int iter, int phase) {
+                         int y, int iter, int phase, const Acmm<T>* ac = nullptr) {
     const int W = f.W, H = f.W;
     const size_t ppv = (size_t)W * W;
     const int center = y * W + x;
@@ -640,10 +678,86 @@ static void update_pixel(const Frame<T>& f, const State<T>& prev, State<T>& next
             }
         }
     }
+    if (ac) {
+        // DetailRestorer_FrameACMM (AdaptMIPMFrameACMM.cu:470-503) against the level's warm seed,
+        // then the metadata of the final plane from its own top-k support (:881-898).  The
+        // reference's best_sel is the sampled selection throughout the update.
+        uint32_t bsel = tsel;
+        if (ac->warm) {
+            const P4<T> seed = ac->plane_init[idx];
+            if (warm_seed_valid(f, seed, ac->cost_init[idx], ac->sel_init[idx])) {
+                T scost; uint32_t ssel; P4<T> sdisp;
+                support(f, ww, v, x, y, seed, scost, ssel, sdisp);
+                const int seed_count = popcount32(ssel), best_count = popcount32(bsel);
+                const T disp_gap = std::fabs(best.w - seed.w);
+                const T large_gap = (T)0.08 * std::max((T)f.p.base, (T)1e-3);
+                const bool clearly_better =
+                    (bcost + (T)0.10 < scost) || (best_count >= seed_count + 2);
+                const bool ambiguous = bcost > scost - (T)0.03;
+                const bool maybe_carrier = disp_gap > large_gap && ambiguous &&
+                                           best_count <= seed_count;
+                if ((!clearly_better && maybe_carrier) || (scost + (T)0.02 < bcost)) {
+                    best = seed; bcost = scost; bdisp = sdisp; bsel = ssel;
+                }
+            }
+        }
+        support(f, ww, v, x, y, best, bcost, bsel, bdisp);
+        next.plane[idx] = best;
+        next.cost[idx] = bcost;
+        next.disp[idx] = bdisp;
+        next.sel[idx] = bsel;
+        return;
+    }
     next.plane[idx] = best;
     next.cost[idx] = bcost;
     next.disp[idx] = bdisp;
     next.sel[idx] = tsel;
+}
+
+// InitializeFrameFromRandomOrWarm (AdaptMIPMFrameACMM.cu:373-472): the random plane of
+// init_pixel, then (warm levels) the seed, a jittered seed (disparity x (1 +- 0.15)) and a seed
+// with its normal perturbed by 0.015 pi, each kept if its top-k cost is strictly lower.  The
+// reference re-encodes the perturbed seed through its own plane (an identity); skipped.
+template <typename T>
+static void init_pixel_acmm(const Frame<T>& f, State<T>& s, int v, int x, int y,
+                            const Acmm<T>& ac) {
+    const int W = f.W;
+    const size_t idx = (size_t)v * W * W + (size_t)y * W + x;
+    Stream st(f.p.seed, f.p.frame_uid, (uint32_t)idx, 0u);
+    P4<T> h;
+    const T depth = uniform<T>(st.raw(SLOT_INIT_DEPTH)) * ((T)f.p.depth_max - (T)f.p.depth_min) +
+                    (T)f.p.depth_min;
+    T n[3];
+    random_normal<T>(st, SLOT_INIT_MARSAGLIA, n);
+    h.x = n[0]; h.y = n[1]; h.z = n[2]; h.w = depth;
+    Win<T> ww;
+    window_weights(f, v, x, y, ww);
+    P4<T> best = h;
+    T bcost; uint32_t bsel; P4<T> bdisp;
+    support(f, ww, v, x, y, h, bcost, bsel, bdisp);
+    if (ac.warm) {
+        const P4<T> warm = ac.plane_init[idx];
+        if (warm_seed_valid(f, warm, ac.cost_init[idx], ac.sel_init[idx])) {
+            T c; uint32_t m; P4<T> d;
+            support(f, ww, v, x, y, warm, c, m, d);
+            if (c < bcost) { best = warm; bcost = c; bsel = m; bdisp = d; }
+            P4<T> jit = warm;
+            const T u = uniform<T>(st.raw(SLOT_WARM_JITTER));
+            jit.w = std::min((T)f.p.depth_max, std::max((T)f.p.depth_min,
+                             warm.w * ((T)1 + (u - (T)0.5) * (T)2 * (T)0.15)));
+            support(f, ww, v, x, y, jit, c, m, d);
+            if (c < bcost) { best = jit; bcost = c; bsel = m; bdisp = d; }
+            const P4<T> np = perturbed_normal<T>(st, x, y, warm, (T)0.015 * (T)M_PI,
+                                                 SLOT_WARM_ANGLE);
+            const P4<T> ref{np.x, np.y, np.z, warm.w};
+            support(f, ww, v, x, y, ref, c, m, d);
+            if (c < bcost) { best = ref; bcost = c; bsel = m; bdisp = d; }
+        }
+    }
+    s.plane[idx] = best;
+    s.cost[idx] = bcost;
+    s.disp[idx] = bdisp;
+    s.sel[idx] = bsel;
 }
 
 template <typename T>
@@ -685,6 +799,202 @@ static void solve(const uint8_t* tiles_u8, const float* centers, const int* nbr,
                     for (int x = 0; x < W; ++x) update_pixel(f, src, dst, v, x, y, it, ph);
         }
     }
+    for (size_t i = 0; i < n; ++i) {
+        plane_out[4 * i] = pa[i].x; plane_out[4 * i + 1] = pa[i].y;
+        plane_out[4 * i + 2] = pa[i].z; plane_out[4 * i + 3] = pa[i].w;
+        cost_out[i] = ca[i];
+        disp_out[4 * i] = da[i].x; disp_out[4 * i + 1] = da[i].y;
+        disp_out[4 * i + 2] = da[i].z; disp_out[4 * i + 3] = da[i].w;
+        sel_out[i] = sa[i];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// ACMM multi-scale variant (AdaptMIPMFrameACMM.cpp:155-665, AdaptMIPMFrameACMM.cu:373-991)
+// ------------------------------------------------------------------------------------------
+// Area-average resize of one W x W tile to W0 x W0 (cv::resize INTER_AREA as OpenCV documents
+// it: every output pixel is the mean of the source area it covers, partial pixels weighted by
+// the covered fraction).
+template <typename T>
+static void area_resize(const T* src, int W, int W0, T* dst) {
+    const T S = (T)W / (T)W0;
+    std::vector<T> wt((size_t)W0 * W, (T)0);
+    for (int d = 0; d < W0; ++d) {
+        const T a = (T)d * S, b = std::min((T)(d + 1) * S, (T)W);
+        const T cell = b - a;
+        for (int sx = 0; sx < W; ++sx) {
+            const T lo = std::max(a, (T)sx), hi = std::min(b, (T)(sx + 1));
+            if (hi > lo) wt[(size_t)d * W + sx] = (hi - lo) / cell;
+        }
+    }
+    for (int dy = 0; dy < W0; ++dy)
+        for (int dx = 0; dx < W0; ++dx) {
+            T acc = 0;
+            for (int sy = 0; sy < W; ++sy) {
+                const T wy = wt[(size_t)dy * W + sy];
+                if (wy == (T)0) continue;
+                for (int sx = 0; sx < W; ++sx) {
+                    const T wx = wt[(size_t)dx * W + sx];
+                    if (wx != (T)0) acc += wy * wx * src[(size_t)sy * W + sx];
+                }
+            }
+            dst[(size_t)dy * W0 + dx] = acc;
+        }
+}
+
+// Level schedule of AdaptMIPMFrameACMM::BuildBlurAlignedPyramids / InitGPUParamsFromCPUParams
+// (AdaptMIPMFrameACMM.cpp:155-200, 300-345): two levels when the crop is at least 24 px,
+// otherwise one; per level the patch, iteration count and top-k of the reference, Base and the
+// disparity range scaled with the level.
+struct AcmmLevel { int width; float scale; int radius, iterations, top_k; };
+
+static int acmm_levels(int W, AcmmLevel out[2]) {
+    int max_levels = 1, cur = W;
+    while (max_levels < 4 && cur >= 24) {
+        cur /= 2;
+        if (cur < 12) break;
+        ++max_levels;
+    }
+    const int L = std::max(1, std::min(2, max_levels));
+    for (int lid = 0; lid < L; ++lid) {
+        const int coarse = L - 1 - lid;
+        const float scale = 1.0f / std::pow(2.0f, (float)coarse);
+        AcmmLevel& l = out[lid];
+        l.scale = scale;
+        l.width = std::max(8, (int)std::round((float)W * scale));
+        if (L <= 1) { l.iterations = 5; l.radius = 2; }
+        else if (lid == 0) { l.iterations = 3; l.radius = 3; }
+        else { l.iterations = 6; l.radius = 2; }
+        l.top_k = lid == 0 ? 6 : 5;
+    }
+    return L;
+}
+
+template <typename T>
+static void solve_level(Frame<T>& f, const Acmm<T>& ac, std::vector<P4<T>>& pa,
+                        std::vector<T>& ca, std::vector<P4<T>>& da, std::vector<uint32_t>& sa,
+                        int nthreads) {
+    const int V = f.V, W = f.W;
+    const size_t n = (size_t)V * W * W;
+    pa.assign(n, P4<T>{0, 0, 1, 0}); ca.assign(n, (T)2); da.assign(n, P4<T>{0, 0, 0, 0});
+    sa.assign(n, 0u);
+    std::vector<P4<T>> pb(n), db(n);
+    std::vector<T> cb(n);
+    std::vector<uint32_t> sb(n);
+    State<T> A{pa.data(), ca.data(), da.data(), sa.data()};
+    State<T> B{pb.data(), cb.data(), db.data(), sb.data()};
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+    for (int v = 0; v < V; ++v)
+        for (int y = 0; y < W; ++y)
+            for (int x = 0; x < W; ++x) init_pixel_acmm(f, A, v, x, y, ac);
+    for (int it = 0; it < f.p.iterations; ++it)
+        for (int ph = 0; ph < 2; ++ph) {
+            const State<T>& src = ph == 0 ? A : B;
+            State<T>& dst = ph == 0 ? B : A;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+            for (int v = 0; v < V; ++v)
+                for (int y = 0; y < W; ++y)
+                    for (int x = 0; x < W; ++x) update_pixel(f, src, dst, v, x, y, it, ph, &ac);
+        }
+    // PostRefineMetadataFrame (AdaptMIPMFrameACMM.cu:903-929)
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+    for (int v = 0; v < V; ++v)
+        for (int y = 0; y < W; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t i = (size_t)v * W * W + (size_t)y * W + x;
+                Win<T> ww;
+                window_weights(f, v, x, y, ww);
+                support(f, ww, v, x, y, pa[i], ca[i], sa[i], da[i]);
+            }
+}
+
+template <typename T>
+static void solve_acmm(const uint8_t* tiles_u8, const float* centers, const int* nbr,
+                       const int* cnt, int V, const PmParams& p, T* plane_out, T* cost_out,
+                       T* disp_out, uint32_t* sel_out, int nthreads) {
+    if (nthreads <= 0) nthreads = 1;
+    AcmmLevel lv[2];
+    const int L = acmm_levels(p.crop, lv);
+    const int W = p.crop;
+    std::vector<T> full((size_t)V * W * W);
+    for (size_t i = 0; i < full.size(); ++i) full[i] = (T)tiles_u8[i] / (T)255;
+    std::vector<P4<T>> pa, da, pinit;
+    std::vector<T> ca, cinit;
+    std::vector<uint32_t> sa, sinit;
+    int prev_w = 0;
+    float prev_scale = 1.0f;
+    for (int lid = 0; lid < L; ++lid) {
+        const AcmmLevel& l = lv[lid];
+        PmParams q = p;
+        q.crop = l.width;
+        q.base = p.base * l.scale;
+        q.radius = l.radius;
+        q.iterations = l.iterations;
+        q.top_k = l.top_k;
+        q.depth_min = p.depth_min * l.scale;
+        q.depth_max = p.depth_max * l.scale;
+        std::vector<float> lc(2 * (size_t)V);
+        for (size_t i = 0; i < lc.size(); ++i) lc[i] = centers[i] * l.scale;
+        Frame<T> f;
+        f.V = V; f.W = l.width; f.K = p.max_neighbors;
+        f.centers = lc.data(); f.nbr = nbr; f.cnt = cnt; f.p = q;
+        f.inv_base = (T)1 / std::max((T)q.base, (T)1e-6);
+        const size_t n = (size_t)V * l.width * l.width;
+        f.img.resize(n);
+        if (l.width == W) {
+            f.img = full;
+        } else {
+            for (int v = 0; v < V; ++v)
+                area_resize(full.data() + (size_t)v * W * W, W, l.width,
+                            f.img.data() + (size_t)v * l.width * l.width);
+        }
+        Acmm<T> ac{false, nullptr, nullptr, nullptr};
+        if (lid > 0) {
+            // PrepareWarmStartFromPreviousLevel (AdaptMIPMFrameACMM.cpp:496-570)
+            const int W0 = prev_w, W1 = l.width;
+            pinit.assign(n, P4<T>{0, 0, 1, (T)-1});
+            cinit.assign(n, (T)2);
+            sinit.assign(n, 0u);
+            const float sx = (float)W0 / (float)std::max(1, W1);
+            const float dscale = l.scale / std::max(prev_scale, 1e-6f);
+            const float prev_base = p.base * prev_scale;
+            const T thresh = (T)std::max(0.03f, 0.12f * prev_base);
+            for (int v = 0; v < V; ++v)
+                for (int y = 0; y < W1; ++y) {
+                    const int py = std::min(W0 - 1, (int)std::floor((float)y * sx));
+                    for (int x = 0; x < W1; ++x) {
+                        const int px = std::min(W0 - 1, (int)std::floor((float)x * sx));
+                        const size_t pi = (size_t)v * W0 * W0 + (size_t)py * W0 + px;
+                        T mean = 0, sq = 0;
+                        for (int oy = -1; oy <= 1; ++oy) {
+                            const int yy = std::min(W0 - 1, std::max(0, py + oy));
+                            for (int ox = -1; ox <= 1; ++ox) {
+                                const int xx = std::min(W0 - 1, std::max(0, px + ox));
+                                const T d = pa[(size_t)v * W0 * W0 + (size_t)yy * W0 + xx].w;
+                                mean += d;
+                                sq += d * d;
+                            }
+                        }
+                        mean /= (T)9;
+                        const T var = std::max((T)0, sq / (T)9 - mean * mean);
+                        const bool reliable = ca[pi] < (T)1 && popcount32(sa[pi]) >= 3 &&
+                                              std::sqrt(var) <= thresh;
+                        if (!reliable) continue;
+                        const size_t ci = (size_t)v * W1 * W1 + (size_t)y * W1 + x;
+                        P4<T> h = pa[pi];
+                        h.w *= (T)dscale;
+                        pinit[ci] = h;
+                        cinit[ci] = ca[pi];
+                        sinit[ci] = sa[pi];
+                    }
+                }
+            ac = Acmm<T>{true, pinit.data(), cinit.data(), sinit.data()};
+        }
+        solve_level(f, ac, pa, ca, da, sa, nthreads);
+        prev_w = l.width;
+        prev_scale = l.scale;
+    }
+    const size_t n = pa.size();
     for (size_t i = 0; i < n; ++i) {
         plane_out[4 * i] = pa[i].x; plane_out[4 * i + 1] = pa[i].y;
         plane_out[4 * i + 2] = pa[i].z; plane_out[4 * i + 3] = pa[i].w;
@@ -1274,6 +1584,26 @@ void orc_solve_f32(const uint8_t* tiles, const float* centers, const int* nbr, c
                    int V, const orc::PmParams* p, float* plane, float* cost, float* disp,
                    uint32_t* sel, int nthreads) {
     orc::solve<float>(tiles, centers, nbr, cnt, V, *p, plane, cost, disp, sel, nthreads);
+}
+
+void orc_solve_acmm_f64(const uint8_t* tiles, const float* centers, const int* nbr,
+                        const int* cnt, int V, const orc::PmParams* p, double* plane,
+                        double* cost, double* disp, uint32_t* sel, int nthreads) {
+    orc::solve_acmm<double>(tiles, centers, nbr, cnt, V, *p, plane, cost, disp, sel, nthreads);
+}
+
+void orc_area_resize_f64(const double* src, int W, int W0, double* dst) {
+    orc::area_resize<double>(src, W, W0, dst);
+}
+
+int orc_acmm_levels(int W, int* width, float* scale, int* radius, int* iterations, int* top_k) {
+    orc::AcmmLevel lv[2];
+    const int L = orc::acmm_levels(W, lv);
+    for (int i = 0; i < L; ++i) {
+        width[i] = lv[i].width; scale[i] = lv[i].scale; radius[i] = lv[i].radius;
+        iterations[i] = lv[i].iterations; top_k[i] = lv[i].top_k;
+    }
+    return L;
 }
 
 void orc_eval_costs_f64(const uint8_t* tiles, const float* centers, const int* nbr,

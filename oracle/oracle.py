@@ -232,3 +232,38 @@ def vd_map(plane, centers, gray, crop, mla_base, raw_w, raw_h, color=True):
                      C.c_float(mla_base), raw_w, raw_h, _p(vd),
                      _p(col) if col is not None else None, _p(rng))
     return vd, col, rng
+
+
+def solve_acmm(tiles_u8, centers, nbr, cnt, pm, threads=None):
+    """ACMM multi-scale solve (FP64) of the final level: dict plane/cost/disp/sel."""
+    V, W, _ = tiles_u8.shape
+    n = V * W * W
+    plane = np.zeros((n, 4), np.float64)
+    cost = np.zeros(n, np.float64)
+    disp = np.zeros((n, 4), np.float64)
+    sel = np.zeros(n, np.uint32)
+    lib().orc_solve_acmm_f64(_p(np.ascontiguousarray(tiles_u8)),
+                             _p(np.ascontiguousarray(centers, np.float32)),
+                             _p(np.ascontiguousarray(nbr, np.int32)),
+                             _p(np.ascontiguousarray(cnt, np.int32)), V, C.byref(pm), _p(plane),
+                             _p(cost), _p(disp), _p(sel), threads or nthreads())
+    return dict(plane=plane, cost=cost, disp=disp, sel=sel)
+
+
+def area_resize(tile, W0):
+    tile = np.ascontiguousarray(tile, np.float64)
+    W = tile.shape[0]
+    out = np.zeros((W0, W0), np.float64)
+    lib().orc_area_resize_f64(_p(tile), W, W0, _p(out))
+    return out
+
+
+def acmm_levels(W):
+    width = np.zeros(2, np.int32)
+    scale = np.zeros(2, np.float32)
+    radius = np.zeros(2, np.int32)
+    iters = np.zeros(2, np.int32)
+    topk = np.zeros(2, np.int32)
+    L = lib().orc_acmm_levels(W, _p(width), _p(scale), _p(radius), _p(iters), _p(topk))
+    return [dict(width=int(width[i]), scale=float(scale[i]), radius=int(radius[i]),
+                 iterations=int(iters[i]), top_k=int(topk[i])) for i in range(L)]
