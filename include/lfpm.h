@@ -186,6 +186,25 @@ int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
                              int num_mis, const lfpm_pm_params* p, const float* plane_dev,
                              float* disp_dev, void* stream);
 
+/* ACMM multi-scale variant.  Replaces AdaptMIPMFrameACMM::RunPatchMatchCUDAForFrameACMM
+ * (MIStereo/AdaptMIPMFrameACMM.cpp:572-665) and RunPatchMatchCUDAForFrameACMM_Impl
+ * (MIStereo/AdaptMIPMFrameACMM.cu:931-991).  Level schedule of the reference
+ * (AdaptMIPMFrameACMM.cpp:155-200, 300-345): crops of at least 24 px run a half-scale level
+ * (INTER_AREA area-average tiles, centres and Base scaled, patch 7, 3 iterations, top-k 6)
+ * whose reliable planes (cost < 1, >= 3 selected views, 3x3 disparity spread <=
+ * max(0.03, 0.12 Base0)) warm-start the full level (patch 5, 6 iterations, top-k 5); smaller
+ * crops run one level (patch 5, 5 iterations, top-k 6).  Every update ends with the warm-seed
+ * detail restorer and the final plane's own top-k cost and selection; a post pass writes the
+ * disp record.  p supplies crop, base, sigmas, depth range (scaled per level), neighbours, seed
+ * and frame_uid; its radius/iterations/top_k are replaced by the schedule.  storage must be 0.
+ * Outputs as lfpm_patchmatch_frame (cost = top-k mean, sel = top-k mask). */
+size_t lfpm_patchmatch_acmm_workspace_size(int num_mis, const lfpm_pm_params* p);
+int lfpm_patchmatch_frame_acmm(const float* tiles_dev, const float* centers_dev,
+                               const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                               int num_mis, const lfpm_pm_params* p, float* plane_dev,
+                               float* cost_dev, float* disp_dev, uint32_t* sel_dev,
+                               void* workspace_dev, size_t workspace_bytes, void* stream);
+
 /* Cost of fixed plane hypotheses.  hyp_dev: n records of {int mi, int pixel, float4 plane}
  * packed as 6 x 32-bit words (mi, pixel as int32; then nx, ny, nz, w as float).
  * cost_out_dev: n*K floats (per-view bilateral NCC cost, 2 for views beyond the MI's count).

@@ -44,6 +44,8 @@ enum : uint32_t {
     SLOT_REF_DEPTH_PERT = 17,
     SLOT_REF_ANGLE = 20,       // 20, 21, 22
     SLOT_REF_MARSAGLIA = 24,   // attempt a: slots 24+2a, 25+2a
+    SLOT_WARM_JITTER = 40,     // ACMM warm-seed disparity jitter (init step)
+    SLOT_WARM_ANGLE = 44,      // ACMM warm-seed normal perturbation, 44..46 (init step)
     MARSAGLIA_ATTEMPTS = 16,
     VIEW_DRAWS = 15,
 };

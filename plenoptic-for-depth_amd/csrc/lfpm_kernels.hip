@@ -28,6 +28,7 @@ struct DevParams {
     int W, WW, K, r, inc, top_k, V;
     int W1, TS;          // padded tile row stride (W+1) and slot stride (W+1)^2 in LDS
     int tile_bytes;      // 4 (FP32 tiles in LDS) or 2 (FP16 tiles in LDS)
+    float base;          // Base (disparity unit, lens pitch in px)
     float inv_base;      // 1 / max(Base, 1e-6)
     float inv_ss;        // 1 / (2 sigma_s^2)
     float inv_sc;        // 1 / (2 sigma_c^2)
@@ -50,6 +51,7 @@ static DevParams make_dev_params(const lfpm_pm_params* p, int V) {
     d.inc = std::max(1, p->radius_increment);
     d.top_k = p->top_k;
     d.V = V;
+    d.base = p->base;
     d.inv_base = 1.0f / std::max(p->base, 1e-6f);
     d.inv_ss = 1.0f / (2.0f * p->sigma_spatial * p->sigma_spatial);
     d.inv_sc = 1.0f / (2.0f * p->sigma_color * p->sigma_color);
@@ -483,6 +485,10 @@ struct FrameDesc {
     const float4* plane_in; const float* cost_in; const uint32_t* sel_in;   // phase / final input
     float4* plane_out; float* cost_out; uint32_t* sel_out;                  // init / phase output
     float4* disp;                                                           // final output
+    // ACMM variant: top-k metadata after every update (and, in the final stage, cost/sel of the
+    // final planes through cost_out/sel_out); warm levels carry the seeds of the coarser level
+    const float4* plane_init; const float* cost_init; const uint32_t* sel_init;
+    int acmm, warm;
     int V, mi_begin, stage, iter, phase;
     uint32_t seed, uid;
 };
@@ -1202,7 +1208,72 @@ __device__ __forceinline__ float wsum(const DevParams& P, const WaveLds& S, int 
     return wn <= 1e-6f ? 2.0f : tot / wn;
 }
 
-// Random initialisation of one pixel (RandomInitializationForFrame :474-518).
+// Top-k support of cost row `row` (ComputeSelectedViewsTopK_Frame :227-282): mean of the k
+// smallest costs below 2 (summed in ascending order), mask of the views at or under the k-th,
+// and the first lowest-cost view.
+__device__ __forceinline__ void topk_row(const DevParams& P, const WaveLds& S, int row, int c,
+                                         float& mean, uint32_t& mask, int& bestk) {
+    const float* cv = S.cost + row * P.K;
+    int valid = 0;
+    bestk = -1;
+    float bv = 2.0f;
+    for (int k = 0; k < c; ++k) {
+        const float x = cv[k];
+        if (x < 2.0f) {
+            ++valid;
+            if (bestk < 0 || x < bv) { bestk = k; bv = x; }
+        }
+    }
+    mean = 2.0f;
+    mask = 0u;
+    if (valid <= 0) return;
+    const int kk = min(valid, P.top_k);
+    uint32_t taken = 0u;
+    float sum = 0.0f, thr = 2.0f;
+    for (int m = 0; m < kk; ++m) {
+        int bi = -1;
+        float b = 0.0f;
+        for (int k = 0; k < c; ++k) {
+            if ((taken >> k) & 1u) continue;
+            if (bi < 0 || cv[k] < b) { bi = k; b = cv[k]; }
+        }
+        taken |= 1u << bi;
+        sum += b;
+        thr = b;
+    }
+    mean = sum / (float)kk;
+    for (int k = 0; k < c; ++k)
+        if (cv[k] <= thr) mask |= 1u << k;
+}
+
+// Normal of plane b rotated by three random angles in (-amp/2, amp/2), kept facing the camera
+// (GeneratePerturbedNormalLF, CudaUtil.cu:105-142); draws angle_slot .. angle_slot + 2.
+__device__ __forceinline__ float3 perturb_normal(DevStream& rs, int x, int y, float4 b,
+                                                 float amp, uint32_t angle_slot) {
+    float vx = (float)x, vy = (float)y, vz = 1.0f;
+    normalize3(vx, vy, vz);
+    const float a1 = (u01_dev(rs.raw(angle_slot)) - 0.5f) * amp;
+    const float a2 = (u01_dev(rs.raw(angle_slot + 1)) - 0.5f) * amp;
+    const float a3 = (u01_dev(rs.raw(angle_slot + 2)) - 0.5f) * amp;
+    const float s1 = sinf(a1), s2 = sinf(a2), s3 = sinf(a3);
+    const float c1 = cosf(a1), c2 = cosf(a2), c3 = cosf(a3);
+    float px_ = (c2 * c3) * b.x + (c3 * s1 * s2 - c1 * s3) * b.y + (s1 * s3 + c1 * c3 * s2) * b.z;
+    float py_ = (c2 * s3) * b.x + (c1 * c3 + s1 * s2 * s3) * b.y + (c1 * s2 * s3 - c3 * s1) * b.z;
+    float pz_ = (-s2) * b.x + (c2 * s1) * b.y + (c1 * c2) * b.z;
+    if (px_ * vx + py_ * vy + pz_ * vz >= 0.0f) { px_ = b.x; py_ = b.y; pz_ = b.z; }
+    normalize3(px_, py_, pz_);
+    return make_float3(px_, py_, pz_);
+}
+
+// Warm seed admissible for an ACMM level (WarmSeedIsValid_Frame, AdaptMIPMFrameACMM.cu:186-194).
+__device__ __forceinline__ bool warm_valid(const DevParams& P, float4 h, float cost,
+                                           uint32_t sel) {
+    return h.w >= P.dmin && h.w <= P.dmax && cost < 1.0f && __popc(sel) >= 3;
+}
+
+// Random initialisation of one pixel (RandomInitializationForFrame :474-518); ACMM warm levels
+// add the seed, a jittered seed and a seed with a perturbed normal
+// (InitializeFrameFromRandomOrWarm, AdaptMIPMFrameACMM.cu:373-472).
 template <bool COUNT, typename TT>
 __device__ void init_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
                            const WaveLds& S, int v, int c, int pix, WorkCount& wc) {
@@ -1215,39 +1286,47 @@ __device__ void init_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
     float4 h;
     h.w = u01_dev(st.raw(SLOT_INIT_DEPTH)) * (P.dmax - P.dmin) + P.dmin;
     random_normal(st, SLOT_INIT_MARSAGLIA, h.x, h.y, h.z);
-    if (lane_id() == 0) S.hyp[0] = h;
     const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
-    const int nt = add_tasks(L, P, S, 0, h, allv, c, x, y, 0);
+    int nh = 1;
+    float4 hs[4];
+    hs[0] = h;
+    if (F.acmm && F.warm) {
+        const float4 warm = F.plane_init[idx];
+        if (warm_valid(P, warm, F.cost_init[idx], F.sel_init[idx])) {
+            float4 jit = warm;
+            const float u = u01_dev(st.raw(SLOT_WARM_JITTER));
+            jit.w = fminf(P.dmax, fmaxf(P.dmin, warm.w * (1.0f + (u - 0.5f) * 2.0f * 0.15f)));
+            const float3 nr = perturb_normal(st, x, y, warm, 0.015f * 3.14159265358979323846f,
+                                             SLOT_WARM_ANGLE);
+            hs[1] = warm;
+            hs[2] = jit;
+            hs[3] = make_float4(nr.x, nr.y, nr.z, warm.w);
+            nh = 4;
+        }
+    }
+    int nt = 0;
+    for (int i = 0; i < nh; ++i) {
+        if (lane_id() == 0) S.hyp[i] = hs[i];
+        nt = add_tasks(L, P, S, i, hs[i], allv, c, x, y, nt);
+    }
     wave_sync();
     run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
-    // top-k mean and selection mask (ComputeSelectedViewsTopK_Frame :227-282)
-    int valid = 0;
-    for (int k = 0; k < c; ++k) valid += S.cost[k] < 2.0f;
-    float mean = 2.0f;
-    uint32_t mask = 0u;
-    if (valid > 0) {
-        const int kk = min(valid, P.top_k);
-        uint32_t taken = 0u;
-        float sum = 0.0f, thr = 2.0f;
-        for (int mth = 0; mth < kk; ++mth) {
-            int bi = -1;
-            float bv = 0.0f;
-            for (int k = 0; k < c; ++k) {
-                if ((taken >> k) & 1u) continue;
-                const float cv = S.cost[k];
-                if (bi < 0 || cv < bv) { bi = k; bv = cv; }
-            }
-            taken |= 1u << bi;
-            sum += bv;
-            thr = bv;
-        }
-        mean = sum / (float)kk;
-        for (int k = 0; k < c; ++k)
-            if (S.cost[k] <= thr) mask |= 1u << k;
+    // top-k mean and selection mask of each candidate; the lowest mean wins (first on ties)
+    float mean;
+    uint32_t mask;
+    int bk;
+    topk_row(P, S, 0, c, mean, mask, bk);
+    int best = 0;
+    for (int i = 1; i < nh; ++i) {
+        float m2;
+        uint32_t k2;
+        int b2;
+        topk_row(P, S, i, c, m2, k2, b2);
+        if (m2 < mean) { mean = m2; mask = k2; best = i; }
     }
     (void)K;
     if (lane_id() == 0) {
-        F.plane_out[idx] = h;
+        F.plane_out[idx] = best == 0 ? hs[0] : (best == 1 ? hs[1] : (best == 2 ? hs[2] : hs[3]));
         F.cost_out[idx] = mean;
         F.sel_out[idx] = mask;
     }
@@ -1276,6 +1355,16 @@ __device__ void final_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     }
     if (lane_id() == 0)
         F.disp[idx] = am >= 0 ? disp_record(L, P, am, h.w) : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (F.acmm) {  // PostRefineMetadataFrame (AdaptMIPMFrameACMM.cu:903-929)
+        float mean;
+        uint32_t mask;
+        int bk;
+        topk_row(P, S, 0, c, mean, mask, bk);
+        if (lane_id() == 0) {
+            F.cost_out[idx] = mean;
+            F.sel_out[idx] = mask;
+        }
+    }
 }
 
 // One red/black update of one pixel (CheckerboardUpdateFrame :520-947).
@@ -1440,21 +1529,9 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
             drand = u01_dev(rs.raw(SLOT_REF_DEPTH_RAND)) * (P.dmax - P.dmin) + P.dmin;
         float rnx, rny, rnz;
         random_normal(rs, SLOT_REF_MARSAGLIA, rnx, rny, rnz);
-        float vx = (float)x, vy = (float)y, vz = 1.0f;
-        normalize3(vx, vy, vz);
-        const float pamp = pert * 3.14159265358979323846f;
-        const float a1 = (u01_dev(rs.raw(SLOT_REF_ANGLE)) - 0.5f) * pamp;
-        const float a2 = (u01_dev(rs.raw(SLOT_REF_ANGLE + 1)) - 0.5f) * pamp;
-        const float a3 = (u01_dev(rs.raw(SLOT_REF_ANGLE + 2)) - 0.5f) * pamp;
-        const float s1 = sinf(a1), s2 = sinf(a2), s3 = sinf(a3);
-        const float c1 = cosf(a1), c2 = cosf(a2), c3 = cosf(a3);
-        float px_ = (c2 * c3) * best.x + (c3 * s1 * s2 - c1 * s3) * best.y +
-                    (s1 * s3 + c1 * c3 * s2) * best.z;
-        float py_ = (c2 * s3) * best.x + (c1 * c3 + s1 * s2 * s3) * best.y +
-                    (c1 * s2 * s3 - c3 * s1) * best.z;
-        float pz_ = (-s2) * best.x + (c2 * s1) * best.y + (c1 * c2) * best.z;
-        if (px_ * vx + py_ * vy + pz_ * vz >= 0.0f) { px_ = best.x; py_ = best.y; pz_ = best.z; }
-        normalize3(px_, py_, pz_);
+        const float3 pn = perturb_normal(rs, x, y, best, pert * 3.14159265358979323846f,
+                                         SLOT_REF_ANGLE);
+        const float px_ = pn.x, py_ = pn.y, pz_ = pn.z;
         const float lo = (1.0f - pert) * dnow, hi = (1.0f + pert) * dnow;
         const float dpert = u01_dev(rs.raw(SLOT_REF_DEPTH_PERT)) * (hi - lo) + lo;
         const float4 cur = best;
@@ -1477,6 +1554,49 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
             const float cc = wsum(P, S, i, nsel);
             if (cc < bcost) { bcost = cc; best = S.hyp[i]; }
         }
+    }
+    if (F.acmm) {
+        // DetailRestorer_FrameACMM (AdaptMIPMFrameACMM.cu:470-503) against the warm seed, then
+        // the final plane's own top-k cost and selection (:881-898).  The reference's best_sel
+        // is the sampled selection throughout the update.
+        float4 seed = make_float4(0.f, 0.f, 1.f, 0.f);
+        bool seed_ok = false;
+        if (F.warm) {
+            seed = F.plane_init[idx];
+            seed_ok = warm_valid(P, seed, F.cost_init[idx], F.sel_init[idx]);
+        }
+        wave_sync();
+        if (lane == 0) S.hyp[0] = best;
+        int nst = add_tasks(L, P, S, 0, best, allv, c, x, y, 0);
+        if (seed_ok) {
+            if (lane == 0) S.hyp[1] = seed;
+            nst = add_tasks(L, P, S, 1, seed, allv, c, x, y, nst);
+        }
+        wave_sync();
+        run_tasks<COUNT, TT>(L, P, S, nst, ns, x, y, rc, wc);
+        float fmean;
+        uint32_t fmask;
+        int fb;
+        topk_row(P, S, 0, c, fmean, fmask, fb);
+        if (seed_ok) {
+            float scost;
+            uint32_t ssel;
+            int sb;
+            topk_row(P, S, 1, c, scost, ssel, sb);
+            const int seed_count = __popc(ssel), best_count = __popc(tsel);
+            const float disp_gap = fabsf(best.w - seed.w);
+            const float large_gap = 0.08f * fmaxf(P.base, 1e-3f);
+            const bool clearly_better = (bcost + 0.10f < scost) || (best_count >= seed_count + 2);
+            const bool ambiguous = bcost > scost - 0.03f;
+            const bool maybe_carrier = disp_gap > large_gap && ambiguous && best_count <= seed_count;
+            if ((!clearly_better && maybe_carrier) || (scost + 0.02f < bcost)) {
+                best = seed;
+                fmean = scost;
+                fmask = ssel;
+            }
+        }
+        bcost = fmean;
+        tsel = fmask;
     }
     if (lane == 0) {
         F.plane_out[idx] = best;
@@ -2366,6 +2486,266 @@ int lfpm_patchmatch_frame(const float* tiles_dev, const float* centers_dev,
         if ((rc = sync())) return rc;
     }
     return LFPM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// ACMM multi-scale variant (AdaptMIPMFrameACMM.cpp:155-665, AdaptMIPMFrameACMM.cu:373-991)
+// ------------------------------------------------------------------------------------------
+struct AcmmLevel { int width; float scale; int radius, iterations, top_k; };
+
+// Level schedule of BuildBlurAlignedPyramids / InitGPUParamsFromCPUParams
+// (AdaptMIPMFrameACMM.cpp:155-200, 300-345): two levels only when the crop is at least 24 px.
+static int acmm_levels(int W, AcmmLevel out[2]) {
+    int max_levels = 1, cur = W;
+    while (max_levels < 4 && cur >= 24) {
+        cur /= 2;
+        if (cur < 12) break;
+        ++max_levels;
+    }
+    const int L = std::max(1, std::min(2, max_levels));
+    for (int lid = 0; lid < L; ++lid) {
+        const int coarse = L - 1 - lid;
+        const float scale = 1.0f / std::pow(2.0f, (float)coarse);
+        AcmmLevel& l = out[lid];
+        l.scale = scale;
+        l.width = std::max(8, (int)std::round((float)W * scale));
+        if (L <= 1) { l.iterations = 5; l.radius = 2; }
+        else if (lid == 0) { l.iterations = 3; l.radius = 3; }
+        else { l.iterations = 6; l.radius = 2; }
+        l.top_k = lid == 0 ? 6 : 5;
+    }
+    return L;
+}
+
+static lfpm_pm_params acmm_level_params(const lfpm_pm_params* p, const AcmmLevel& l) {
+    lfpm_pm_params q = *p;
+    q.crop = l.width;
+    q.base = p->base * l.scale;
+    q.radius = l.radius;
+    q.iterations = l.iterations;
+    q.top_k = l.top_k;
+    q.depth_min = p->depth_min * l.scale;
+    q.depth_max = p->depth_max * l.scale;
+    return q;
+}
+
+// Area-average resize of every tile W -> W0 (cv::resize INTER_AREA as OpenCV documents it:
+// each output pixel is the mean of the source area it covers, partial pixels weighted by the
+// covered fraction).  One thread per output pixel.
+__global__ void k_area_resize(const float* __restrict__ src, int V, int W, int W0,
+                              float* __restrict__ dst) {
+    const long long n = (long long)V * W0 * W0;
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = (int)(i / (W0 * W0)), r = (int)(i - (long long)v * W0 * W0);
+    const int dy = r / W0, dx = r - dy * W0;
+    const float S = (float)W / (float)W0;
+    const float ax = (float)dx * S, bx = fminf((float)(dx + 1) * S, (float)W);
+    const float ay = (float)dy * S, by = fminf((float)(dy + 1) * S, (float)W);
+    const int sx0 = (int)floorf(ax), sx1 = min(W - 1, (int)ceilf(bx) - 1);
+    const int sy0 = (int)floorf(ay), sy1 = min(W - 1, (int)ceilf(by) - 1);
+    const float* t = src + (size_t)v * W * W;
+    float acc = 0.0f;
+    for (int sy = sy0; sy <= sy1; ++sy) {
+        const float wy = (fminf(by, (float)(sy + 1)) - fmaxf(ay, (float)sy)) / (by - ay);
+        if (!(wy > 0.0f)) continue;
+        float row = 0.0f;
+        for (int sx = sx0; sx <= sx1; ++sx) {
+            const float wx = (fminf(bx, (float)(sx + 1)) - fmaxf(ax, (float)sx)) / (bx - ax);
+            if (wx > 0.0f) row += wx * t[sy * W + sx];
+        }
+        acc += wy * row;
+    }
+    dst[i] = acc;
+}
+
+__global__ void k_scale_centers(const float* __restrict__ c, int n, float s, float* __restrict__ o) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = c[i] * s;
+}
+
+// PrepareWarmStartFromPreviousLevel (AdaptMIPMFrameACMM.cpp:496-570): a fine pixel inherits the
+// coarse plane under it (disparity scaled by the level ratio) when the coarse cost is below 1,
+// at least 3 views were selected and the 3x3 disparity spread there is small; otherwise the
+// marker plane (0, 0, 1, -1) with cost 2.
+__global__ void k_warm_start(const float4* __restrict__ plane0, const float* __restrict__ cost0,
+                             const uint32_t* __restrict__ sel0, int V, int W0, int W1,
+                             float dscale, float thresh, float4* __restrict__ pinit,
+                             float* __restrict__ cinit, uint32_t* __restrict__ sinit) {
+    const long long n = (long long)V * W1 * W1;
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = (int)(i / (W1 * W1)), r = (int)(i - (long long)v * W1 * W1);
+    const int y = r / W1, x = r - y * W1;
+    const float sx = (float)W0 / (float)max(1, W1);
+    const int py = min(W0 - 1, (int)floorf((float)y * sx));
+    const int px = min(W0 - 1, (int)floorf((float)x * sx));
+    const size_t pi = (size_t)v * W0 * W0 + (size_t)py * W0 + px;
+    float mean = 0.0f, sq = 0.0f;
+    for (int oy = -1; oy <= 1; ++oy) {
+        const int yy = min(W0 - 1, max(0, py + oy));
+        for (int ox = -1; ox <= 1; ++ox) {
+            const int xx = min(W0 - 1, max(0, px + ox));
+            const float d = plane0[(size_t)v * W0 * W0 + (size_t)yy * W0 + xx].w;
+            mean += d;
+            sq += d * d;
+        }
+    }
+    mean /= 9.0f;
+    const float var = fmaxf(0.0f, sq / 9.0f - mean * mean);
+    const bool reliable = cost0[pi] < 1.0f && __popc(sel0[pi]) >= 3 && sqrtf(var) <= thresh;
+    float4 h = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
+    float c = 2.0f;
+    uint32_t m = 0u;
+    if (reliable) {
+        h = plane0[pi];
+        h.w *= dscale;
+        c = cost0[pi];
+        m = sel0[pi];
+    }
+    pinit[i] = h;
+    cinit[i] = c;
+    sinit[i] = m;
+}
+
+// One ACMM level: init (random, or warm-seeded), iterations x 2 ACMM phases, then the
+// post-refine metadata pass (disp record, top-k cost and selection) into plane/cost/sel/disp.
+static int acmm_level_solve(const float* tiles, const float* centers, const int* nbr,
+                            const int* cnt, int V, const lfpm_pm_params* q, float* plane,
+                            float* cost, uint32_t* sel, float* disp, float* plane_b, float* cost_b,
+                            uint32_t* sel_b, const float4* pinit, const float* cinit,
+                            const uint32_t* sinit, hipStream_t s) {
+    const int warm = pinit != nullptr;
+    auto base_desc = [&](int stage) {
+        StageArgs a;
+        a.nf = 1;
+        a.P = make_dev_params(q, V);
+        a.f[0] = frame_desc(tiles, centers, nbr, cnt, V, q->seed, q->frame_uid);
+        FrameDesc& f = a.f[0];
+        f.stage = stage;
+        f.acmm = 1;
+        f.warm = warm;
+        f.plane_init = pinit; f.cost_init = cinit; f.sel_init = sinit;
+        return a;
+    };
+    int rc;
+    {
+        StageArgs a = base_desc(STAGE_INIT);
+        a.f[0].plane_out = (float4*)plane; a.f[0].cost_out = cost; a.f[0].sel_out = sel;
+        if ((rc = launch_stage(a, stage_slots_per_cu(q), nullptr, 0, s))) return rc;
+    }
+    for (int it = 0; it < q->iterations; ++it)
+        for (int ph = 0; ph < 2; ++ph) {
+            StageArgs a = base_desc(STAGE_PHASE);
+            FrameDesc& f = a.f[0];
+            f.iter = it; f.phase = ph;
+            f.plane_in = (const float4*)(ph == 0 ? plane : plane_b);
+            f.cost_in = ph == 0 ? cost : cost_b;
+            f.sel_in = ph == 0 ? sel : sel_b;
+            f.plane_out = (float4*)(ph == 0 ? plane_b : plane);
+            f.cost_out = ph == 0 ? cost_b : cost;
+            f.sel_out = ph == 0 ? sel_b : sel;
+            if ((rc = launch_stage(a, stage_slots_per_cu(q), nullptr, 0, s))) return rc;
+        }
+    StageArgs a = base_desc(STAGE_FINAL);
+    a.f[0].plane_in = (const float4*)plane;
+    a.f[0].disp = (float4*)disp;
+    a.f[0].cost_out = cost;
+    a.f[0].sel_out = sel;
+    return launch_stage(a, stage_slots_per_cu(q), nullptr, 0, s);
+}
+
+struct AcmmWs {
+    size_t tiles0, centers0, plane0, cost0, sel0, disp0, plane0b, cost0b, sel0b;
+    size_t pinit, cinit, sinit, plane_b, cost_b, sel_b, total;
+};
+
+static AcmmWs acmm_ws(int V, int W, int W0, bool two) {
+    AcmmWs w{};
+    const size_t n1 = (size_t)V * W * W, n0 = two ? (size_t)V * W0 * W0 : 0;
+    size_t o = 0;
+    auto take = [&](size_t b) { const size_t at = o; o += align_up(b); return at; };
+    w.plane_b = take(n1 * 16); w.cost_b = take(n1 * 4); w.sel_b = take(n1 * 4);
+    if (two) {
+        w.tiles0 = take(n0 * 4); w.centers0 = take((size_t)V * 8);
+        w.plane0 = take(n0 * 16); w.cost0 = take(n0 * 4); w.sel0 = take(n0 * 4);
+        w.disp0 = take(n0 * 16);
+        w.plane0b = take(n0 * 16); w.cost0b = take(n0 * 4); w.sel0b = take(n0 * 4);
+        w.pinit = take(n1 * 16); w.cinit = take(n1 * 4); w.sinit = take(n1 * 4);
+    }
+    w.total = o;
+    return w;
+}
+
+size_t lfpm_patchmatch_acmm_workspace_size(int num_mis, const lfpm_pm_params* p) {
+    if (check_pm_params(p, num_mis)) return 0;
+    AcmmLevel lv[2];
+    const int L = acmm_levels(p->crop, lv);
+    return acmm_ws(num_mis, p->crop, lv[0].width, L == 2).total;
+}
+
+int lfpm_patchmatch_frame_acmm(const float* tiles_dev, const float* centers_dev,
+                               const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                               int num_mis, const lfpm_pm_params* p, float* plane_dev,
+                               float* cost_dev, float* disp_dev, uint32_t* sel_dev,
+                               void* workspace_dev, size_t workspace_bytes, void* stream) {
+    int rc = check_pm_params(p, num_mis);
+    if (rc) return rc;
+    if ((rc = check_solver_ptrs(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev)))
+        return rc;
+    if (!plane_dev || !cost_dev || !disp_dev || !sel_dev)
+        return set_error(LFPM_E_INVALID, "acmm: null output");
+    if (p->storage != 0)
+        return set_error(LFPM_E_UNSUPPORTED, "acmm: level images are not 8-bit; storage must be 0");
+    if (!wave_solver())
+        return set_error(LFPM_E_UNSUPPORTED, "acmm: needs the wave solver engine");
+    AcmmLevel lv[2];
+    const int L = acmm_levels(p->crop, lv);
+    const AcmmWs w = acmm_ws(num_mis, p->crop, lv[0].width, L == 2);
+    if (!workspace_dev || workspace_bytes < w.total)
+        return set_error(LFPM_E_INVALID, "acmm: workspace too small (%zu < %zu)", workspace_bytes,
+                         w.total);
+    unsigned char* ws = (unsigned char*)workspace_dev;
+    hipStream_t s = (hipStream_t)stream;
+    const int V = num_mis;
+    float* plane_b = (float*)(ws + w.plane_b);
+    float* cost_b = (float*)(ws + w.cost_b);
+    uint32_t* sel_b = (uint32_t*)(ws + w.sel_b);
+    const lfpm_pm_params q1 = acmm_level_params(p, lv[L - 1]);
+    if (L == 1)
+        return acmm_level_solve(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev, V,
+                                &q1, plane_dev, cost_dev, sel_dev, disp_dev, plane_b, cost_b,
+                                sel_b, nullptr, nullptr, nullptr, s);
+    const int W = p->crop, W0 = lv[0].width;
+    const lfpm_pm_params q0 = acmm_level_params(p, lv[0]);
+    float* tiles0 = (float*)(ws + w.tiles0);
+    float* centers0 = (float*)(ws + w.centers0);
+    const long long n0 = (long long)V * W0 * W0, n1 = (long long)V * W * W;
+    hipLaunchKernelGGL(k_area_resize, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s,
+                       tiles_dev, V, W, W0, tiles0);
+    hipLaunchKernelGGL(k_scale_centers, dim3((2 * V + 255) / 256), dim3(256), 0, s, centers_dev,
+                       2 * V, lv[0].scale, centers0);
+    if ((rc = launch_check("acmm level images"))) return rc;
+    float* plane0 = (float*)(ws + w.plane0);
+    float* cost0 = (float*)(ws + w.cost0);
+    uint32_t* sel0 = (uint32_t*)(ws + w.sel0);
+    if ((rc = acmm_level_solve(tiles0, centers0, neighbor_ids_dev, neighbor_counts_dev, V, &q0,
+                               plane0, cost0, sel0, (float*)(ws + w.disp0),
+                               (float*)(ws + w.plane0b), (float*)(ws + w.cost0b),
+                               (uint32_t*)(ws + w.sel0b), nullptr, nullptr, nullptr, s)))
+        return rc;
+    float4* pinit = (float4*)(ws + w.pinit);
+    float* cinit = (float*)(ws + w.cinit);
+    uint32_t* sinit = (uint32_t*)(ws + w.sinit);
+    const float dscale = lv[1].scale / std::max(lv[0].scale, 1e-6f);
+    const float thresh = std::max(0.03f, 0.12f * (p->base * lv[0].scale));
+    hipLaunchKernelGGL(k_warm_start, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, s,
+                       (const float4*)plane0, cost0, sel0, V, W0, W, dscale, thresh, pinit, cinit,
+                       sinit);
+    if ((rc = launch_check("acmm warm start"))) return rc;
+    return acmm_level_solve(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev, V, &q1,
+                            plane_dev, cost_dev, sel_dev, disp_dev, plane_b, cost_b, sel_b, pinit,
+                            cinit, sinit, s);
 }
 
 int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,

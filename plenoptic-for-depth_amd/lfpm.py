@@ -72,6 +72,7 @@ def lib():
         L.lfpm_last_error.restype = C.c_char_p
         L.lfpm_patchmatch_workspace_size.restype = C.c_size_t
         L.lfpm_products_workspace_size.restype = C.c_size_t
+        L.lfpm_patchmatch_acmm_workspace_size.restype = C.c_size_t
         L.lfpm_batch_workspace_size.restype = C.c_size_t
         _lib = L
     return _lib
@@ -275,6 +276,23 @@ def patchmatch_frame(df, pm=None, bufs=None, sync_each_phase=False, stream=None)
         _dp(df.tiles), _dp(df.centers), _dp(df.nbr), _dp(df.cnt), V, C.byref(pm),
         _dp(bufs.plane), _dp(bufs.cost), _dp(bufs.disp), _dp(bufs.sel), _dp(bufs.ws),
         C.c_size_t(bufs.ws_bytes), 1 if sync_each_phase else 0, _stream(stream)))
+    return bufs
+
+
+def patchmatch_frame_acmm(df, pm=None, stream=None):
+    """ACMM multi-scale solve (AdaptMIPMFrameACMM): SolveBuffers with plane/cost/disp/sel."""
+    import torch
+    pm = pm or df.wl.pm
+    V, W = df.wl.V, df.wl.crop
+    bufs = SolveBuffers(V, W, pm, df.tiles.device)
+    n = lib().lfpm_patchmatch_acmm_workspace_size(V, C.byref(pm))
+    if n == 0:
+        _check(-1)
+    ws = torch.empty(n, dtype=torch.uint8, device=df.tiles.device)
+    _check(lib().lfpm_patchmatch_frame_acmm(
+        _dp(df.tiles), _dp(df.centers), _dp(df.nbr), _dp(df.cnt), V, C.byref(pm),
+        _dp(bufs.plane), _dp(bufs.cost), _dp(bufs.disp), _dp(bufs.sel), _dp(ws), C.c_size_t(n),
+        _stream(stream)))
     return bufs
 
 

@@ -236,3 +236,27 @@ def test_ctx_api_matches_device_path(lfpm, c1, torch_cuda):
         L.lfpm_destroy(ctx)
     assert np.array_equal(reason, fo["reason"].cpu().numpy())
     assert np.array_equal(plane.view(np.uint32), fo["plane"].cpu().numpy().view(np.uint32))
+
+
+@pytest.mark.parametrize("pitch", [23.0, 35.0], ids=["one_level", "two_levels"])
+def test_solve_acmm_vs_fp64(lfpm, orc, torch_cuda, pitch):
+    """ACMM variant (AdaptMIPMFrameACMM) vs the FP64 oracle: final-level disparities within
+    0.05 px on >= 99.5 % of pixels; cost/sel are the top-k support of the final planes."""
+    w, h = (400, 320) if pitch < 30 else (560, 448)
+    wl = lfpm.workload_c1(raw_w=w, raw_h=h, pitch=pitch)
+    _say("case acmm pitch %g V=%d crop=%d" % (pitch, wl.V, wl.crop))
+    df = lfpm.DeviceFrame(wl)
+    bufs = lfpm.patchmatch_frame_acmm(df)
+    torch_cuda.cuda.synchronize()
+    d = bufs.plane[:, 3].cpu().numpy().astype(np.float64)
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    o = orc.solve_acmm(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"],
+                       pm_to_oracle(orc, wl.pm))
+    err = np.abs(d - o["plane"][:, 3])
+    frac = float((err <= 0.05).mean())
+    sel_eq = float((bufs.sel.cpu().numpy().view(np.uint32) == o["sel"]).mean())
+    st = dict(pitch=pitch, V=wl.V, frac_le_005=frac, sel_equal=sel_eq,
+              median=float(np.median(err)))
+    _say("acmm", st)
+    stats_dump("acmm_%g" % pitch, st)
+    assert frac >= 0.995
