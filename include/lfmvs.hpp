@@ -133,6 +133,10 @@ struct LightFieldParams {
     // iteration count (3) (MIStereo/AdaptMIPMFrame.cpp:17, :128).
     lfpm_pm_params solve_params{};
     bool solve_params_set = false;
+    // frame matcher of ProcessProblemsImpSequence: false = AdaptMIPMFrame
+    // (MIStereoMatch::StereoMatchingForMIA_FrameCrossViews, MIStereo/MIStereoMatch.cpp:626-696),
+    // true = AdaptMIPMFrameACMM (StereoMatchingForMIA_FrameCrossViews_ACMM, :698-740)
+    bool frame_matcher_acmm = false;
     // CheckModule filter after the solve (MIStereoMatch.cpp:759-778); on by default
     lfpm_filter_params filter_params{};
     bool run_filter = true;
@@ -183,6 +187,7 @@ public:
 protected:
     void InitGPUParamsFromCPUParams();
     bool BuildViewIndexMap(QuadTreeTileInfoMap& MLA_info_map, QuadTreeProblemMap& problem_map);
+    void Solve(int variant);
 
     LightFieldParams m_Params;
     lfpm_pm_params params{};
@@ -201,6 +206,15 @@ protected:
     std::vector<uint32_t> selected_final_host;
     std::vector<uint8_t> reason_final_host;
     lfpm_ctx* m_ctx = nullptr;
+};
+
+// Multi-scale frame matcher (AdaptMIPMFrameACMM, MIStereo/AdaptMIPMFrameACMM.h:25-42): the ACMM
+// variant (level schedule, warm start, detail restorer, top-k metadata) on the same views.
+class AdaptMIPMFrameACMM : public AdaptMIPMFrame {
+public:
+    explicit AdaptMIPMFrameACMM(LightFieldParams& params) : AdaptMIPMFrame(params) {}
+    void RunPatchMatchCUDAForFrameACMM();
+    int GetNumLevels() const;
 };
 
 // Per-micro-image matcher (AdaptMIPM, MIStereo/AdaptMIPM.h:47-107): the same solver on a work
@@ -285,7 +299,8 @@ public:
     // Matching neighbours of every problem (SelectNeighborsForProblemsBySequence).
     void SelectNeighborsForProblemsBySequence(std::string& strName,
                                               QuadTreeProblemMap& problem_map);
-    // Solve (+ CheckModule filter) one frame into its DisparityAndNormal map.
+    // Solve (+ CheckModule filter) one frame into its DisparityAndNormal map with the matcher
+    // LightFieldParams::frame_matcher_acmm names.
     void ProcessProblemsImpSequence(QuadTreeProblemMapMap::iterator itr, bool geom_consistency);
     // Per-frame products of the last solved frame.
     void WriteDisMap_TileKey_new_AccuSequence(std::string& strName,

@@ -390,6 +390,13 @@ int lfpm_create(int device, lfpm_ctx** out);
 void lfpm_destroy(lfpm_ctx* ctx);
 const char* lfpm_last_error(void);
 
+/* Solver variant of the context's single-frame solves (lfpm_ctx_solve_frame, _solve_tiles):
+ * LFPM_VARIANT_PLAIN = AdaptMIPMFrame (lfpm_patchmatch_frame, the default),
+ * LFPM_VARIANT_ACMM = AdaptMIPMFrameACMM (lfpm_patchmatch_frame_acmm). */
+#define LFPM_VARIANT_PLAIN 0
+#define LFPM_VARIANT_ACMM 1
+int lfpm_ctx_set_variant(lfpm_ctx* ctx, int variant);
+
 /* Upload a layout and a raw host frame, solve it, filter it, and copy results to host.
  * Host outputs (any may be NULL): plane 4*V*crop^2, cost, disp 4*V*crop^2, sel, reason
  * (filter reason mask; NULL skips the filter pass). */

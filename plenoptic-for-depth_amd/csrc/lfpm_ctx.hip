@@ -64,6 +64,7 @@ struct lfpm_ctx {
     void* ws = nullptr;
     size_t ws_bytes = 0;
     int solved = 0;               // frames of the last solve call
+    int variant = 0;              // LFPM_VARIANT_* of single-frame solves
     // per-frame products (sized for the last raw frame)
     void* pws = nullptr;
     size_t pws_bytes = 0;
@@ -143,7 +144,9 @@ static int ensure_frames(lfpm_ctx* c, int n, bool batched, const lfpm_pm_params*
         c->frames_batched = batched;
     }
     if (!batched) {
-        const size_t need = lfpm_patchmatch_workspace_size(c->V, p);
+        const size_t need = c->variant == LFPM_VARIANT_ACMM
+                                ? lfpm_patchmatch_acmm_workspace_size(c->V, p)
+                                : lfpm_patchmatch_workspace_size(c->V, p);
         if (need == 0) return set_error(LFPM_E_INVALID, "ctx: bad solve parameters");
         if (need > c->ws_bytes) {
             if (c->ws) (void)hipFree(c->ws);
@@ -206,9 +209,13 @@ static int solve_one(lfpm_ctx* c, FrameBufs& f, const lfpm_pm_params* p,
     if ((rc = lfpm_slice_tiles(f.raw, c->raw_w, c->raw_h, (size_t)c->raw_w, c->origins, c->V,
                                c->crop, f.tiles, c->stream)))
         return rc;
-    if ((rc = lfpm_patchmatch_frame(f.tiles, c->centers, c->nbr, c->cnt, c->V, p, f.plane,
-                                    f.cost, f.disp, f.sel, c->ws, c->ws_bytes, 0, c->stream)))
-        return rc;
+    if (c->variant == LFPM_VARIANT_ACMM)
+        rc = lfpm_patchmatch_frame_acmm(f.tiles, c->centers, c->nbr, c->cnt, c->V, p, f.plane,
+                                        f.cost, f.disp, f.sel, c->ws, c->ws_bytes, c->stream);
+    else
+        rc = lfpm_patchmatch_frame(f.tiles, c->centers, c->nbr, c->cnt, c->V, p, f.plane, f.cost,
+                                   f.disp, f.sel, c->ws, c->ws_bytes, 0, c->stream);
+    if (rc) return rc;
     f.out_plane = f.plane;
     f.out_cost = f.cost;
     if (flt) {
@@ -273,6 +280,14 @@ void lfpm_destroy(lfpm_ctx* c) {
     delete c;
 }
 
+int lfpm_ctx_set_variant(lfpm_ctx* c, int variant) {
+    if (!c) return set_error(LFPM_E_INVALID, "set_variant: null ctx");
+    if (variant != LFPM_VARIANT_PLAIN && variant != LFPM_VARIANT_ACMM)
+        return set_error(LFPM_E_INVALID, "set_variant: unknown variant %d", variant);
+    c->variant = variant;  // the next solve re-sizes the workspace for it
+    return LFPM_OK;
+}
+
 int lfpm_ctx_solve_frame(lfpm_ctx* c, const lfpm_layout_params* lp, const uint8_t* raw_host,
                          const lfpm_pm_params* p, const lfpm_filter_params* f, float* plane,
                          float* cost, float* disp, uint32_t* sel, uint8_t* reason) {
@@ -303,6 +318,9 @@ int lfpm_ctx_solve_frames(lfpm_ctx* c, const lfpm_layout_params* lp, int n,
     if (n > LFPM_MAX_BATCH)
         return set_error(LFPM_E_INVALID, "solve_frames: %d frames > %d", n, LFPM_MAX_BATCH);
     if (reason && !f) return set_error(LFPM_E_INVALID, "solve_frames: reason without filter");
+    if (c->variant != LFPM_VARIANT_PLAIN)
+        return set_error(LFPM_E_UNSUPPORTED, "solve_frames: the batched form runs the plain "
+                         "variant");
     for (int i = 0; i < n; ++i)
         if (!raws[i]) return set_error(LFPM_E_INVALID, "solve_frames: raw %d is null", i);
     CTX_TRY(hipSetDevice(c->device));

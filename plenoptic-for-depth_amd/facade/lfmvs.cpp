@@ -171,9 +171,20 @@ bool AdaptMIPMFrame::Initialize(QuadTreeTileInfoMap& MLA_info_map,
     return true;
 }
 
-void AdaptMIPMFrame::RunPatchMatchCUDAForFrame() {
+void AdaptMIPMFrame::RunPatchMatchCUDAForFrame() { Solve(LFPM_VARIANT_PLAIN); }
+
+void AdaptMIPMFrameACMM::RunPatchMatchCUDAForFrameACMM() { Solve(LFPM_VARIANT_ACMM); }
+
+int AdaptMIPMFrameACMM::GetNumLevels() const {
+    // level rule of AdaptMIPMFrameACMM::BuildBlurAlignedPyramids (AdaptMIPMFrameACMM.cpp:
+    // 300-345): a half-scale level exists when the crop is at least 24 px
+    return m_crop >= 24 ? 2 : 1;
+}
+
+void AdaptMIPMFrame::Solve(int variant) {
     if (m_bReleased || m_num_views <= 0)
         throw std::runtime_error("AdaptMIPMFrame: RunPatchMatchCUDAForFrame before Initialize");
+    check(lfpm_ctx_set_variant(Context(), variant), "AdaptMIPMFrame: solver variant");
     const size_t n = (size_t)m_num_views * m_crop * m_crop;
     plane_final_host.assign(4 * n, 0.0f);
     cost_final_host.assign(n, 0.0f);
@@ -406,7 +417,8 @@ void DepthSolver::ProcessProblemsImpSequence(QuadTreeProblemMapMap::iterator itr
     (void)geom_consistency;
     const std::string& name = itr->first;
     QuadTreeProblemMap& problem_map = itr->second;
-    m_frame.reset(new AdaptMIPMFrame(m_Params));
+    const bool acmm = m_Params.frame_matcher_acmm;
+    m_frame.reset(acmm ? new AdaptMIPMFrameACMM(m_Params) : new AdaptMIPMFrame(m_Params));
     // frame index = position in the raw-frame map (the order the batched form uses)
     auto pos = m_RawImageMap.find(name);
     m_frame->SetFrameIndex(pos == m_RawImageMap.end()
@@ -414,7 +426,8 @@ void DepthSolver::ProcessProblemsImpSequence(QuadTreeProblemMapMap::iterator itr
                                : (int)std::distance(m_RawImageMap.begin(), pos));
     if (!m_frame->Initialize(m_MLA_info_map, problem_map))
         throw std::runtime_error("DepthSolver: frame " + name + ": " + m_frame->LastError());
-    m_frame->RunPatchMatchCUDAForFrame();
+    if (acmm) static_cast<AdaptMIPMFrameACMM*>(m_frame.get())->RunPatchMatchCUDAForFrameACMM();
+    else m_frame->RunPatchMatchCUDAForFrame();
     QuadTreeDisNormalMap& dn = m_MIA_dispNormal_map_map[name];
     for (auto& kv : problem_map) {
         if (dn.find(kv.first) == dn.end()) {
@@ -495,6 +508,8 @@ int DepthSolver::ProcessAllFramesBatched() {
         slp[i] = sl[i].data(); rsp[i] = rs[i].data();
     }
     if (!m_frame) m_frame.reset(new AdaptMIPMFrame(m_Params));
+    check(lfpm_ctx_set_variant(m_frame->Context(), LFPM_VARIANT_PLAIN),
+          "DepthSolver::ProcessAllFramesBatched");
     const bool filt = m_Params.run_filter;
     check(lfpm_ctx_solve_frames(m_frame->Context(), &m_lp, n, raws.data(), &p,
                                 filt ? &m_Params.filter_params : nullptr, plp.data(), csp.data(),

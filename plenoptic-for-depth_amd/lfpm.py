@@ -401,6 +401,12 @@ class Context:
         except Exception:
             pass
 
+    PLAIN, ACMM = 0, 1
+
+    def set_variant(self, variant):
+        """Solver used by solve_frame / solve_tiles: Context.PLAIN or Context.ACMM."""
+        _check(lib().lfpm_ctx_set_variant(self.h, int(variant)))
+
     @staticmethod
     def _outs(n):
         return dict(plane=np.zeros((n, 4), np.float32), cost=np.zeros(n, np.float32),

@@ -5,7 +5,8 @@
 // usage: facade_driver <cfg.bin> <out_dir>
 // cfg.bin: lfpm_layout_params, lfpm_synth_params, lfpm_pm_params, lfpm_filter_params (raw
 // struct bytes), then int32 nframes, int32 mode (0 = DepthSolverPipelineBySequence,
-// 1 = DepthSolverPipelineByBatch, 2 = AdaptMIPM on the middle micro-image of frame 0).
+// 1 = DepthSolverPipelineByBatch, 2 = AdaptMIPM on the middle micro-image of frame 0,
+// 3 = DepthSolverPipelineBySequence with the AdaptMIPMFrameACMM matcher).
 // Frame i uses scene/noise seeds + 17*i.
 #include <cstdio>
 #include <cstdlib>
@@ -52,6 +53,7 @@ int main(int argc, char** argv) {
         lf.solve_params_set = true;
         lf.filter_params = fp;
         lf.run_filter = true;
+        lf.frame_matcher_acmm = mode == 3;
         if (!s.SetMLALayout(lp)) {
             std::fprintf(stderr, "layout: %s\n", lfpm_last_error());
             return 1;
@@ -92,7 +94,7 @@ int main(int argc, char** argv) {
             std::printf("facade_driver: AdaptMIPM views=%d\n", V);
             return 0;
         }
-        const int rc = mode == 0 ? mgr.DepthSolverPipelineBySequence(eST_ACMH)
+        const int rc = mode != 1 ? mgr.DepthSolverPipelineBySequence(eST_ACMH)
                                  : mgr.DepthSolverPipelineByBatch();
         if (rc != 0) {
             std::fprintf(stderr, "pipeline rc=%d: %s\n", rc, lfpm_last_error());

@@ -65,3 +65,35 @@ def test_ctx_solve_tiles_matches_frame(lfpm):
         ctx.close()
     for k in whole:
         assert np.array_equal(whole[k].view(np.uint8), part[k].view(np.uint8)), k
+
+
+def test_ctx_acmm_variant_matches_device_entry(lfpm):
+    """lfpm_ctx_set_variant(ACMM) + lfpm_ctx_solve_tiles (the AdaptMIPMFrameACMM seam) equals
+    lfpm_patchmatch_frame_acmm on the same crops bit for bit; switching back to PLAIN gives the
+    plain solve again."""
+    import torch
+    assert torch.cuda.is_available()
+    import oracle as orc
+    wl = lfpm.workload_c1(raw_w=560, raw_h=448, pitch=35.0)
+    _say("case ctx acmm variant V=%d crop=%d" % (wl.V, wl.crop))
+    df = lfpm.DeviceFrame(wl)
+    bufs = lfpm.patchmatch_frame_acmm(df)
+    torch.cuda.synchronize()
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    ctx = lfpm.Context()
+    try:
+        plain0 = ctx.solve_tiles(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], wl.pm)
+        ctx.set_variant(lfpm.Context.ACMM)
+        ac = ctx.solve_tiles(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], wl.pm)
+        ctx.set_variant(lfpm.Context.PLAIN)
+        plain1 = ctx.solve_tiles(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], wl.pm)
+        with pytest.raises(lfpm.LfpmError):
+            ctx.set_variant(7)
+    finally:
+        ctx.close()
+    for k, dev in (("plane", bufs.plane), ("cost", bufs.cost), ("disp", bufs.disp),
+                   ("sel", bufs.sel)):
+        assert np.array_equal(ac[k].view(np.uint8), dev.cpu().numpy().view(np.uint8)), k
+    for k in plain0:
+        assert np.array_equal(plain0[k].view(np.uint8), plain1[k].view(np.uint8)), k
+    assert not np.array_equal(plain0["plane"], ac["plane"])

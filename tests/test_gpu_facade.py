@@ -108,3 +108,25 @@ def test_facade_adaptmipm_single_mi(lfpm, orc, tmp_path):
     err = np.abs(plane[:, 3].astype(np.float64) - o["plane"][:px, 3])
     _say("AdaptMIPM views", V, "frac<=0.05px", float((err <= 0.05).mean()))
     assert (err <= 0.05).mean() >= 0.99
+
+
+def test_facade_acmm_matcher(lfpm, tmp_path):
+    """DepthSolverPipelineBySequence with LightFieldParams::frame_matcher_acmm (the
+    AdaptMIPMFrameACMM matcher) gives lfpm_ctx_solve_frame's ACMM results, filter included,
+    bit for bit, on a two-level layout."""
+    wl = lfpm.workload_c1(raw_w=560, raw_h=448, pitch=35.0, scene_seed=1234, noise_seed=1235)
+    wl.pm.frame_uid = 5
+    out = _run(lfpm, wl, 1, 3, tmp_path)
+    ctx = lfpm.Context()
+    try:
+        ctx.set_variant(lfpm.Context.ACMM)
+        exp = ctx.solve_frame(wl)
+    finally:
+        ctx.close()
+    b = str(out / "frame000")
+    got = dict(plane=np.fromfile(b + "_plane.bin", np.float32).reshape(-1, 4),
+               cost=np.fromfile(b + "_cost.bin", np.float32),
+               sel=np.fromfile(b + "_sel.bin", np.uint32),
+               reason=np.fromfile(b + "_reason.bin", np.uint8))
+    for k, v in got.items():
+        assert np.array_equal(v.view(np.uint8), exp[k].view(np.uint8)), k
