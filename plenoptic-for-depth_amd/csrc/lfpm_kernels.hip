@@ -1208,6 +1208,55 @@ __device__ __forceinline__ float wsum(const DevParams& P, const WaveLds& S, int 
     return wn <= 1e-6f ? 2.0f : tot / wn;
 }
 
+// wsum of row `lane` for every lane below nrows (lanes at or above it score row 0): the rows
+// of a batch summed side by side, each in selection order as wsum sums it.
+__device__ __forceinline__ float wsum_lanes(const DevParams& P, const WaveLds& S, int nrows,
+                                            int nsel) {
+    const int lane = lane_id();
+    const int h = lane < nrows ? lane : 0;
+    float wn = 0.0f, tot = 0.0f;
+    for (int j = 0; j < nsel; ++j) {
+        const int k = S.iaux[P.K + j];
+        const float w = (float)S.iaux[k];
+        wn += w;
+        tot += w * S.cost[h * P.K + k];
+    }
+    return wn <= 1e-6f ? 2.0f : tot / wn;
+}
+
+__device__ __forceinline__ float lane_value(float v, int l) { return __shfl(v, l); }
+
+// Rounded point of the cross-view proposal through view k at disparity w
+// (BuildCrossViewPlaneProposal :362-376): its pixel index in the neighbour crop, -1 when the
+// view has no neighbour or the point leaves the crop.
+__device__ __forceinline__ int xview_point(const Lds& L, const DevParams& P, int k, int x, int y,
+                                           float w) {
+    if (L.nid[k] < 0) return -1;
+    const float2 sk = L.step[k];
+    const float qfx = (float)x + sk.x * w, qfy = (float)y + sk.y * w;
+    const float fqx = floorf(qfx + 0.5f), fqy = floorf(qfy + 0.5f);
+    const float Wf = (float)P.W;
+    if (!(fqx >= 0.0f && fqx < Wf && fqy >= 0.0f && fqy < Wf)) return -1;
+    return (int)fqy * P.W + (int)fqx;
+}
+
+// The proposal itself (:377-397): the neighbour's plane at q re-expressed in the reference MI.
+__device__ __forceinline__ float4 xview_plane(const Lds& L, const DevParams& P,
+                                              const float4* __restrict__ plane_in, int k, int x,
+                                              int y, int q) {
+    const int qx = q % P.W, qy = q / P.W;
+    const float4 hn = plane_in[(size_t)L.nid[k] * P.WW + q];
+    const float al = -hn.x / hn.z, be = -hn.y / hn.z;
+    const float ga = hn.w - al * (float)qx - be * (float)qy;
+    const float2 cdv = L.cdiff[k];
+    float den = 1.0f + al * cdv.x + be * cdv.y;
+    if (fabsf(den) < 1e-6f) den = den >= 0.0f ? 1e-6f : -1e-6f;
+    const float aR = al / den, bR = be / den, cR = ga / den;
+    float nx = -aR, ny = -bR, nz = 1.0f;
+    normalize3(nx, ny, nz);
+    return make_float4(nx, ny, nz, aR * (float)x + bR * (float)y + cR);
+}
+
 // Top-k support of cost row `row` (ComputeSelectedViewsTopK_Frame :227-282): mean of the k
 // smallest costs below 2 (summed in ascending order), mask of the views at or under the k-th,
 // and the first lowest-cost view.
@@ -1481,41 +1530,63 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
 
     // current plane and propagation candidates under the selected views
     float4 best = hcur;
-    float bcost = wsum(P, S, 0, nsel);
+    float bcost;
+    {
+        const float wrow = wsum_lanes(P, S, nh, nsel);
+        bcost = lane_value(wrow, 0);
 #pragma unroll
-    for (int cd = 0; cd < 8; ++cd) {
-        if (row[cd] < 0) continue;
-        const float cc = wsum(P, S, row[cd], nsel);
-        if (cc < bcost) { bcost = cc; best = S.hyp[row[cd]]; }
+        for (int cd = 0; cd < 8; ++cd) {
+            if (row[cd] < 0) continue;
+            const float cc = lane_value(wrow, row[cd]);
+            if (cc < bcost) { bcost = cc; best = S.hyp[row[cd]]; }
+        }
     }
 
-    // cross-view proposals through each selected view (BuildCrossViewPlaneProposal :350-398)
-    for (int j = 0; j < nsel; ++j) {
-        const int k = S.iaux[K + j];
-        const int n = L.nid[k];
-        if (n < 0) continue;
-        const float2 sk = L.step[k];
-        const float qfx = (float)x + sk.x * best.w, qfy = (float)y + sk.y * best.w;
-        const float fqx = floorf(qfx + 0.5f), fqy = floorf(qfy + 0.5f);
-        if (!(fqx >= 0.0f && fqx < (float)W && fqy >= 0.0f && fqy < (float)W)) continue;
-        const int qx = (int)fqx, qy = (int)fqy;
-        const float4 hn = F.plane_in[(size_t)n * WW + qy * W + qx];
-        const float al = -hn.x / hn.z, be = -hn.y / hn.z;
-        const float ga = hn.w - al * (float)qx - be * (float)qy;
-        const float2 cdv = L.cdiff[k];
-        float den = 1.0f + al * cdv.x + be * cdv.y;
-        if (fabsf(den) < 1e-6f) den = den >= 0.0f ? 1e-6f : -1e-6f;
-        const float aR = al / den, bR = be / den, cR = ga / den;
-        float nx = -aR, ny = -bR, nz = 1.0f;
-        normalize3(nx, ny, nz);
-        const float4 prop = make_float4(nx, ny, nz, aR * (float)x + bR * (float)y + cR);
+    // cross-view proposals through each selected view (BuildCrossViewPlaneProposal :350-398,
+    // loop :907-936).  The reference builds proposal j at the rounded point of best.w with best
+    // as updated by proposals < j.  Here the proposals of a batch are built at the batch's
+    // starting best and scored together; the in-order replay keeps a proposal's score when its
+    // rounded point is the same under the replayed best, and rebuilds and scores it alone (row
+    // kHMax - 1) otherwise, so the outcome is the sequential loop's.
+    constexpr int kSpec = kHMax - 1;
+    for (int j0 = 0; j0 < nsel; j0 += kSpec) {
+        const int jn = min(nsel - j0, kSpec);
+        int qspec = -1;
+        float4 pspec = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+        if (lane < jn) {
+            const int k = S.iaux[K + j0 + lane];
+            qspec = xview_point(L, P, k, x, y, best.w);
+            if (qspec >= 0) pspec = xview_plane(L, P, F.plane_in, k, x, y, qspec);
+        }
         wave_sync();
-        if (lane == 0) S.hyp[0] = prop;
-        const int npt = add_tasks(L, P, S, 0, prop, tsel, c, x, y, 0);
+        if (lane < jn && qspec >= 0) S.hyp[lane] = pspec;
+        wave_sync();
+        int npt = 0;
+        for (int j = 0; j < jn; ++j)
+            if (__shfl(qspec, j) >= 0) npt = add_tasks(L, P, S, j, S.hyp[j], tsel, c, x, y, npt);
         wave_sync();
         run_tasks<COUNT, TT>(L, P, S, npt, ns, x, y, rc, wc);
-        const float cc = wsum(P, S, 0, nsel);
-        if (cc < bcost) { bcost = cc; best = prop; }
+        const float wrow = wsum_lanes(P, S, jn, nsel);
+        for (int j = 0; j < jn; ++j) {
+            const int k = S.iaux[K + j0 + j];
+            const int q = xview_point(L, P, k, x, y, best.w);
+            if (q < 0) continue;
+            float4 prop;
+            float cc;
+            if (q == __shfl(qspec, j)) {
+                prop = S.hyp[j];
+                cc = lane_value(wrow, j);
+            } else {
+                prop = xview_plane(L, P, F.plane_in, k, x, y, q);
+                wave_sync();
+                if (lane == 0) S.hyp[kSpec] = prop;
+                const int n1 = add_tasks(L, P, S, kSpec, prop, tsel, c, x, y, 0);
+                wave_sync();
+                run_tasks<COUNT, TT>(L, P, S, n1, ns, x, y, rc, wc);
+                cc = wsum(P, S, kSpec, nsel);
+            }
+            if (cc < bcost) { bcost = cc; best = prop; }
+        }
     }
 
     // refinement: ACMP depth x normal candidates (AdaptMIPMUtil.cu:181-183)
@@ -1550,8 +1621,9 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         }
         wave_sync();
         run_tasks<COUNT, TT>(L, P, S, nrt, ns, x, y, rc, wc);
+        const float wrow = wsum_lanes(P, S, 5, nsel);
         for (int i = 0; i < 5; ++i) {
-            const float cc = wsum(P, S, i, nsel);
+            const float cc = lane_value(wrow, i);
             if (cc < bcost) { bcost = cc; best = S.hyp[i]; }
         }
     }
