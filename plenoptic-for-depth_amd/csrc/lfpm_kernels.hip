@@ -996,7 +996,6 @@ struct WaveLds {
     float2* hab;       // kHMax disparity slopes (A, B) = (-nx/nz, -ny/nz) of the hypotheses
     float* cost;       // [kHMax][K] NCC cost of (hypothesis, view); 2 = not scored
     uint16_t* task;    // (h << 8) | k
-    float* aux;        // K floats (per-view probabilities / cumulative sums)
     int* iaux;         // 2*K ints (draw weights, selected-view list)
 };
 
@@ -1012,8 +1011,7 @@ __host__ __device__ inline size_t wave_bytes(const DevParams& P) {
     const int m = 2 * P.r / P.inc + 1;
     const size_t ns = (size_t)((m * m + 7) & ~7);
     return a16(ns * 16) + a16((size_t)kHMax * 16) + a16((size_t)kHMax * 8) +
-           a16((size_t)kHMax * P.K * 4) + a16((size_t)kHMax * P.K * 2) + a16((size_t)P.K * 4) +
-           a16((size_t)P.K * 8);
+           a16((size_t)kHMax * P.K * 4) + a16((size_t)kHMax * P.K * 2) + a16((size_t)P.K * 8);
 }
 
 __host__ __device__ inline size_t lds_total(const DevParams& P, int nwaves) {
@@ -1050,7 +1048,6 @@ __device__ __forceinline__ WaveLds carve_wave(unsigned char* base, const DevPara
     S.hab = (float2*)q; q += a16((size_t)kHMax * 8);
     S.cost = (float*)q; q += a16((size_t)kHMax * P.K * 4);
     S.task = (uint16_t*)q; q += a16((size_t)kHMax * P.K * 2);
-    S.aux = (float*)q; q += a16((size_t)P.K * 4);
     S.iaux = (int*)q;
     return S;
 }
@@ -1108,6 +1105,42 @@ __device__ __forceinline__ int add_tasks(const Lds& L, const DevParams& P, const
     const uint64_t bal = __ballot(act);
     if (act) S.task[ntask + prefix_below(bal)] = (uint16_t)((h << 8) | lane);
     return ntask + __popcll(bal);
+}
+
+// add_tasks for the rows h0 .. h0 + nh - 1 in `rows` (bit r = row h0 + r; planes already in
+// S.hyp), built side by side: lane (g, k) of a round takes row h0 + r0 + g and view k.  Tasks
+// are appended in (row, view) order, as consecutive add_tasks calls append them; every cost row
+// of the range is cleared to 2.
+__device__ __forceinline__ int add_tasks_rows(const Lds& L, const DevParams& P,
+                                              const WaveLds& S, int h0, int nh, uint32_t rows,
+                                              uint32_t vmask, int c, int px, int py, int ntask) {
+    const int lane = lane_id();
+    const float Wf = (float)P.W;
+    for (int i = lane; i < nh * P.K; i += 64) S.cost[h0 * P.K + i] = 2.0f;
+    if (c <= 0) return ntask;
+    const int per = 64 / c;
+    const int g = lane / c, k = lane - g * c;
+    const float2 st = L.step[k];
+    const bool view_ok = g < per && ((vmask >> k) & 1u) && L.nid[k] >= 0;
+    for (int r0 = 0; r0 < nh; r0 += per) {
+        const int r = r0 + g;
+        const int h = h0 + r;
+        bool act = false;
+        if (g < per && r < nh) {
+            const float4 hv = S.hyp[h];
+            const float A = -hv.x / hv.z, B = -hv.y / hv.z;
+            if (k == 0) S.hab[h] = make_float2(A, B);
+            const bool hyp_ok = fabsf(hv.z) >= P.cos_max_slant && isfinite(A) && isfinite(B);
+            if (view_ok && hyp_ok && ((rows >> r) & 1u)) {
+                const float ax = (float)px + st.x * hv.w, ay = (float)py + st.y * hv.w;
+                act = ax >= 0.0f && ax < Wf && ay >= 0.0f && ay < Wf;
+            }
+        }
+        const uint64_t bal = __ballot(act);
+        if (act) S.task[ntask + prefix_below(bal)] = (uint16_t)((h << 8) | k);
+        ntask += __popcll(bal);
+    }
+    return ntask;
 }
 
 // Evaluate the task list: S.cost[h][k] for every task.  ns = valid window samples.
@@ -1353,11 +1386,10 @@ __device__ void init_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
             nh = 4;
         }
     }
-    int nt = 0;
-    for (int i = 0; i < nh; ++i) {
-        if (lane_id() == 0) S.hyp[i] = hs[i];
-        nt = add_tasks(L, P, S, i, hs[i], allv, c, x, y, nt);
-    }
+    if (lane_id() == 0)
+        for (int i = 0; i < nh; ++i) S.hyp[i] = hs[i];
+    wave_sync();
+    const int nt = add_tasks_rows(L, P, S, 0, nh, 0xFFFFFFFFu, allv, c, x, y, 0);
     wave_sync();
     run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
     // top-k mean and selection mask of each candidate; the lowest mean wins (first on ties)
@@ -1435,22 +1467,22 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     int row[8];   // cost row of the plane candidate slot cd reads, -1 if absent
     int nh = 1;
     const float4 hcur = F.plane_in[idx];
-    if (lane == 0) S.hyp[0] = hcur;
+    int my_src = lane == 0 ? center : -1, my_row = 0;  // lane 1 + cd loads candidate cd
 #pragma unroll
     for (int cd = 0; cd < 8; ++cd) {
         row[cd] = -1;
         if (src[cd] == cd) {
             row[cd] = nh;
-            if (lane == 0) S.hyp[nh] = F.plane_in[base + pos[cd]];
+            if (lane == cd + 1) { my_src = pos[cd]; my_row = nh; }
             ++nh;
         } else if (cd & 1) {
             if (src[cd] == cd - 1) row[cd] = row[cd - 1];  // border duplicate of the near slot
         }
     }
+    if (my_src >= 0) S.hyp[my_row] = F.plane_in[base + my_src];
     wave_sync();
     const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
-    int nt = 0;
-    for (int h = 0; h < nh; ++h) nt = add_tasks(L, P, S, h, S.hyp[h], allv, c, x, y, nt);
+    const int nt = add_tasks_rows(L, P, S, 0, nh, 0xFFFFFFFFu, allv, c, x, y, 0);
     wave_sync();
     run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
 
@@ -1465,6 +1497,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
 
     // sampling probabilities, lane k = view k
     const float thr = 0.8f * expf((float)(F.iter * F.iter) / (-90.0f));
+    float pk = 0.0f;
     if (lane < c) {
         const int k = lane;
         float count = 0.0f, tmpw = 0.0f;
@@ -1483,12 +1516,11 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
 #pragma unroll
         for (int ii = 0; ii < 4; ++ii)
             if (dval[ii]) prior += ((dsel[ii] >> k) & 1u) ? 0.9f : 0.1f;
-        S.aux[k] = pr * prior;
+        pk = pr * prior;
     }
-    wave_sync();
     // cumulative distribution in view order (sequential, as the reference sums)
     float psum = 0.0f;
-    for (int k = 0; k < c; ++k) psum += S.aux[k];
+    for (int k = 0; k < c; ++k) psum += lane_value(pk, k);
     DevStream rs((uint32_t)idx, (uint32_t)(1 + 2 * F.iter + phase), F.seed, F.uid);
     uint32_t tsel = 0u;
     int nsel = 0;
@@ -1503,18 +1535,18 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         int vs = -1;
         float cum = 0.0f;
         for (int k = 0; k < c; ++k) {
-            cum += S.aux[k] * inv;
+            cum += lane_value(pk, k) * inv;
             if (vs < 0 && cum > draw) vs = k;
         }
-        for (int k = 0; k < c; ++k) {
-            const int wk = __popcll(__ballot(lane < (int)VIEW_DRAWS && vs == k));
-            if (wk > 0) {
-                tsel |= 1u << k;
-                if (lane == 0) S.iaux[K + nsel] = k;
-                ++nsel;
-            }
-            if (lane == 0) S.iaux[k] = wk;
-        }
+        // lane k (< c) counts the draws that chose view k
+        int wk = 0;
+        for (int d = 0; d < (int)VIEW_DRAWS; ++d) wk += __shfl(vs, d) == lane ? 1 : 0;
+        const bool chosen = lane < c && wk > 0;
+        const uint64_t bal = __ballot(chosen);
+        tsel = (uint32_t)bal;
+        nsel = __popcll(bal);
+        if (chosen) S.iaux[K + prefix_below(bal)] = lane;
+        if (lane < c) S.iaux[lane] = wk;
     } else {
         for (int k = lane; k < c; k += 64) S.iaux[k] = 0;
     }
@@ -1551,25 +1583,25 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     constexpr int kSpec = kHMax - 1;
     for (int j0 = 0; j0 < nsel; j0 += kSpec) {
         const int jn = min(nsel - j0, kSpec);
-        int qspec = -1;
+        int qspec = -1, kj = 0;
         float4 pspec = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
         if (lane < jn) {
-            const int k = S.iaux[K + j0 + lane];
-            qspec = xview_point(L, P, k, x, y, best.w);
-            if (qspec >= 0) pspec = xview_plane(L, P, F.plane_in, k, x, y, qspec);
+            kj = S.iaux[K + j0 + lane];
+            qspec = xview_point(L, P, kj, x, y, best.w);
+            if (qspec >= 0) pspec = xview_plane(L, P, F.plane_in, kj, x, y, qspec);
         }
         wave_sync();
         if (lane < jn && qspec >= 0) S.hyp[lane] = pspec;
         wave_sync();
-        int npt = 0;
-        for (int j = 0; j < jn; ++j)
-            if (__shfl(qspec, j) >= 0) npt = add_tasks(L, P, S, j, S.hyp[j], tsel, c, x, y, npt);
+        const uint32_t live = (uint32_t)__ballot(lane < jn && qspec >= 0);
+        const int npt = add_tasks_rows(L, P, S, 0, jn, live, tsel, c, x, y, 0);
         wave_sync();
         run_tasks<COUNT, TT>(L, P, S, npt, ns, x, y, rc, wc);
         const float wrow = wsum_lanes(P, S, jn, nsel);
+        // lane j's rounded point under the replayed best (recomputed after each acceptance)
+        int qcur = qspec;
         for (int j = 0; j < jn; ++j) {
-            const int k = S.iaux[K + j0 + j];
-            const int q = xview_point(L, P, k, x, y, best.w);
+            const int q = __shfl(qcur, j);
             if (q < 0) continue;
             float4 prop;
             float cc;
@@ -1577,6 +1609,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
                 prop = S.hyp[j];
                 cc = lane_value(wrow, j);
             } else {
+                const int k = __shfl(kj, j);
                 prop = xview_plane(L, P, F.plane_in, k, x, y, q);
                 wave_sync();
                 if (lane == 0) S.hyp[kSpec] = prop;
@@ -1585,7 +1618,11 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
                 run_tasks<COUNT, TT>(L, P, S, n1, ns, x, y, rc, wc);
                 cc = wsum(P, S, kSpec, nsel);
             }
-            if (cc < bcost) { bcost = cc; best = prop; }
+            if (cc < bcost) {
+                bcost = cc;
+                best = prop;
+                if (lane > j && lane < jn) qcur = xview_point(L, P, kj, x, y, best.w);
+            }
         }
     }
 
@@ -1607,18 +1644,18 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         const float dpert = u01_dev(rs.raw(SLOT_REF_DEPTH_PERT)) * (hi - lo) + lo;
         const float4 cur = best;
         wave_sync();
-        int nrt = 0;
-#pragma unroll
-        for (int i = 0; i < 5; ++i) {
+        if (lane < 5) {
             // (n_cur, d_rand), (n_rand, d_now), (n_rand, d_rand), (n_pert, d_now), (n_cur, d_pert)
+            const int i = lane;
             float4 hc;
             hc.x = (i == 0 || i == 4) ? cur.x : (i == 3 ? px_ : rnx);
             hc.y = (i == 0 || i == 4) ? cur.y : (i == 3 ? py_ : rny);
             hc.z = (i == 0 || i == 4) ? cur.z : (i == 3 ? pz_ : rnz);
             hc.w = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dnow);
-            if (lane == 0) S.hyp[i] = hc;
-            nrt = add_tasks(L, P, S, i, hc, tsel, c, x, y, nrt);
+            S.hyp[i] = hc;
         }
+        wave_sync();
+        const int nrt = add_tasks_rows(L, P, S, 0, 5, 0x1Fu, tsel, c, x, y, 0);
         wave_sync();
         run_tasks<COUNT, TT>(L, P, S, nrt, ns, x, y, rc, wc);
         const float wrow = wsum_lanes(P, S, 5, nsel);
