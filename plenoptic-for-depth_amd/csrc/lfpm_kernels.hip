@@ -1257,7 +1257,11 @@ __device__ __forceinline__ float wsum_lanes(const DevParams& P, const WaveLds& S
     return wn <= 1e-6f ? 2.0f : tot / wn;
 }
 
-__device__ __forceinline__ float lane_value(float v, int l) { return __shfl(v, l); }
+// Value of v in lane l; l is the same in every lane.
+__device__ __forceinline__ int lane_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float lane_value(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
 
 // Rounded point of the cross-view proposal through view k at disparity w
 // (BuildCrossViewPlaneProposal :362-376): its pixel index in the neighbour crop, -1 when the
@@ -1334,11 +1338,12 @@ __device__ __forceinline__ float3 perturb_normal(DevStream& rs, int x, int y, fl
                                                  float amp, uint32_t angle_slot) {
     float vx = (float)x, vy = (float)y, vz = 1.0f;
     normalize3(vx, vy, vz);
-    const float a1 = (u01_dev(rs.raw(angle_slot)) - 0.5f) * amp;
-    const float a2 = (u01_dev(rs.raw(angle_slot + 1)) - 0.5f) * amp;
-    const float a3 = (u01_dev(rs.raw(angle_slot + 2)) - 0.5f) * amp;
-    const float s1 = sinf(a1), s2 = sinf(a2), s3 = sinf(a3);
-    const float c1 = cosf(a1), c2 = cosf(a2), c3 = cosf(a3);
+    // lane l < 3 takes angle l + 1: the three sines and cosines side by side (every lane active)
+    const int l = lane_id() < 3 ? lane_id() : 0;
+    const float a = (u01_dev(rs.raw(angle_slot + l)) - 0.5f) * amp;
+    const float sa = sinf(a), ca = cosf(a);
+    const float s1 = lane_value(sa, 0), s2 = lane_value(sa, 1), s3 = lane_value(sa, 2);
+    const float c1 = lane_value(ca, 0), c2 = lane_value(ca, 1), c3 = lane_value(ca, 2);
     float px_ = (c2 * c3) * b.x + (c3 * s1 * s2 - c1 * s3) * b.y + (s1 * s3 + c1 * c3 * s2) * b.z;
     float py_ = (c2 * s3) * b.x + (c1 * c3 + s1 * s2 * s3) * b.y + (c1 * s2 * s3 - c3 * s1) * b.z;
     float pz_ = (-s2) * b.x + (c2 * s1) * b.y + (c1 * c2) * b.z;
@@ -1458,6 +1463,15 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     const size_t base = (size_t)v * WW;
     const int center = y * W + x;
     const size_t idx = base + center;
+    // view-selection priors from the 4 direct neighbours' previous selections (loaded first so
+    // the loads overlap the window setup and the first batch)
+    uint32_t dsel[4];
+    bool dval[4];
+    dval[0] = y > 0; dval[1] = y < W - 1; dval[2] = x > 0; dval[3] = x < W - 1;
+    dsel[0] = dval[0] ? F.sel_in[idx - W] : 0u;
+    dsel[1] = dval[1] ? F.sel_in[idx + W] : 0u;
+    dsel[2] = dval[2] ? F.sel_in[idx - 1] : 0u;
+    dsel[3] = dval[3] ? F.sel_in[idx + 1] : 0u;
     const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
     const int ns = build_samples<TT>(L, P, S, x, y, rc);
 
@@ -1485,15 +1499,6 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     const int nt = add_tasks_rows(L, P, S, 0, nh, 0xFFFFFFFFu, allv, c, x, y, 0);
     wave_sync();
     run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
-
-    // view-selection priors from the 4 direct neighbours' previous selections
-    uint32_t dsel[4];
-    bool dval[4];
-    dval[0] = y > 0; dval[1] = y < W - 1; dval[2] = x > 0; dval[3] = x < W - 1;
-    dsel[0] = dval[0] ? F.sel_in[idx - W] : 0u;
-    dsel[1] = dval[1] ? F.sel_in[idx + W] : 0u;
-    dsel[2] = dval[2] ? F.sel_in[idx - 1] : 0u;
-    dsel[3] = dval[3] ? F.sel_in[idx + 1] : 0u;
 
     // sampling probabilities, lane k = view k
     const float thr = 0.8f * expf((float)(F.iter * F.iter) / (-90.0f));
@@ -1540,7 +1545,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         }
         // lane k (< c) counts the draws that chose view k
         int wk = 0;
-        for (int d = 0; d < (int)VIEW_DRAWS; ++d) wk += __shfl(vs, d) == lane ? 1 : 0;
+        for (int d = 0; d < (int)VIEW_DRAWS; ++d) wk += lane_int(vs, d) == lane ? 1 : 0;
         const bool chosen = lane < c && wk > 0;
         const uint64_t bal = __ballot(chosen);
         tsel = (uint32_t)bal;
@@ -1601,15 +1606,15 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         // lane j's rounded point under the replayed best (recomputed after each acceptance)
         int qcur = qspec;
         for (int j = 0; j < jn; ++j) {
-            const int q = __shfl(qcur, j);
+            const int q = lane_int(qcur, j);
             if (q < 0) continue;
             float4 prop;
             float cc;
-            if (q == __shfl(qspec, j)) {
+            if (q == lane_int(qspec, j)) {
                 prop = S.hyp[j];
                 cc = lane_value(wrow, j);
             } else {
-                const int k = __shfl(kj, j);
+                const int k = lane_int(kj, j);
                 prop = xview_plane(L, P, F.plane_in, k, x, y, q);
                 wave_sync();
                 if (lane == 0) S.hyp[kSpec] = prop;
