@@ -10,6 +10,7 @@
 //   AdaptMIPM                   MIStereo/AdaptMIPM.h:47-107
 //   DisparityAndNormal          MVStereo/LFDepthInfo.h:273-320
 //   MLA_Problem                 MVStereo/LFDepthInfo.h:225-268 (fields used by the matcher)
+//   SelectNeighbors             MINeighbor/SelectNeighbors.h (FixedPosition selection)
 //   QuadTreeTileKey             Common/QuadTree.h:65-148
 //   LightFieldParams            Common/Common.h:154-187
 //   eStereoType / eStereoStage  Common/Common.h:133-151
@@ -105,13 +106,22 @@ private:
 typedef std::shared_ptr<MLA_Info> MLA_InfoPtr;
 typedef std::map<QuadTreeTileKeyPtr, MLA_InfoPtr, QuadTreeTileKeyMapCmpLess> QuadTreeTileInfoMap;
 
+// Neighbour with its centre distance (Res_image_Key, MVStereo/LFDepthInfo.h:160-224).
+struct Res_image_Key {
+    QuadTreeTileKeyPtr m_ptrKey;
+    float Base_line = 0.0f;
+};
+
 struct MLA_Problem {
     QuadTreeTileKeyPtr m_ptrKey;
     GrayImage m_Image_gray;                       // crop x crop micro-image
     QuadTreeTileKeyPtrVec m_NeighsSortVecForMatch;  // matching neighbours, best first
+    std::vector<Res_image_Key> m_Res_Image_KeyVec;  // FixedPosition neighbours, nearest first
     bool m_bGarbage = false;
     bool m_bNeedMatch = true;
 };
+
+enum eSelectNeighborsType { eSNT_FixedPosition = 0, eSNT_Features, eSNT_Gradients };
 typedef std::map<QuadTreeTileKeyPtr, MLA_Problem, QuadTreeTileKeyMapCmpLess> QuadTreeProblemMap;
 typedef std::map<std::string, QuadTreeProblemMap> QuadTreeProblemMapMap;
 
@@ -322,6 +332,7 @@ public:
 
 private:
     friend class DepthSolverManager;
+    friend class SelectNeighbors;
     std::string m_strRootPath, m_strSavePath;
     LightFieldParams m_Params;
     eStereoType m_eStereoType = eST_ACMH;
@@ -339,6 +350,21 @@ private:
     std::map<std::string, std::vector<float>> m_DisRange, m_VdRange;
     std::unique_ptr<AdaptMIPMFrame> m_frame;      // matcher of the last solved frame
     std::string m_lastSolved;
+};
+
+// Neighbour selection (SelectNeighbors, MINeighbor/SelectNeighbors.cpp:981-1030).  The
+// FixedPosition type fills every problem's m_Res_Image_KeyVec with the mode-1 key offsets of
+// its place in the lens lattice, nearest first (capped at the layout's max_neighbors), and sets
+// m_bGarbage as DepthSolver::ConfirmProblemForEstimation does; the feature and gradient types
+// are outside this library and throw.
+class SelectNeighbors {
+public:
+    explicit SelectNeighbors(DepthSolver* solver) : m_ptrDepthSolver(solver) {}
+    void SelectNeighborsForMIA(QuadTreeProblemMapMap::iterator& itrPM,
+                               eSelectNeighborsType eSelectType);
+
+private:
+    DepthSolver* m_ptrDepthSolver;
 };
 
 // Pipeline entry points (DepthSolverManager, DepthSolverManager.h:14-29).

@@ -81,6 +81,18 @@ int lfpm_layout_build(const lfpm_layout_params* p, float* centers, int* origins,
                       int* lens_type, int* neighbor_ids, int* neighbor_counts,
                       int* lattice_to_mi);
 
+/* FixedPosition neighbour table (SelectNeighbors::FixedPositionAlgorithm with
+ * SelectFixedPositionMIByMode1, MINeighbor/SelectNeighbors.cpp:1035-1286): for each kept MI of
+ * the layout (lfpm_layout_build order), the mode-1 key offsets of its place in the lens
+ * lattice (interior: 12; edges and corners: 6) that name kept MIs, sorted by centre distance
+ * (MLA_Problem::CreateResTileKeysFromNeiKeyVec, MVStereo/LFDepthInfo.cpp:431-453; equal
+ * distances keep the collection order), capped at max_neighbors.  garbage[v] = 1 where
+ * DepthSolver::ConfirmProblemForEstimation (DepthSolver.cpp:2933-2946) excludes the MI from
+ * estimation (key column or row 0, or no neighbour).  Host only; any output may be NULL.
+ * Sizes: neighbor_ids V*K, neighbor_counts V, garbage V. */
+int lfpm_layout_fixed_neighbors(const lfpm_layout_params* p, int* neighbor_ids,
+                                int* neighbor_counts, int* garbage);
+
 /* ---------------------------------------------------------------------------------------
  * Synthetic focused-plenoptic frame (host only; test and benchmark workload generator — the
  * reference ships no data).  Seeded slanted planes in virtual space on a Voronoi partition,

@@ -38,6 +38,10 @@
 //   fit_v2r()/v2r_convert() : VirtualToRealDepthFunc::BehavioralModel / convertVirtualToRealDepth,
 //                             FocalStack/VirtualToRealDepthFunc.cpp:3390-3451 (Householder QR
 //                             least squares in FP64 instead of cv::solve(DECOMP_SVD)).
+//   fixed_neighbors()       : SelectNeighbors::FixedPositionAlgorithm / SelectFixedPosition-
+//                             MIByMode1, MINeighbor/SelectNeighbors.cpp:1035-1352, sort of
+//                             MVStereo/LFDepthInfo.cpp:431-453, garbage rule DepthSolver.cpp:
+//                             2933-2946 (the reference's if/else chain, branch for branch).
 //   solve_acmm()            : AdaptMIPMFrameACMM (MIStereo/AdaptMIPMFrameACMM.cpp:155-665, .cu:
 //                             373-991): level schedule, INTER_AREA level images (area average),
 //                             warm start with the 3x3 seed gate, warm-seeded init, detail
@@ -1104,6 +1108,83 @@ static int layout(const LayoutParams& lp, int cap, float* centers, int* origins,
     return V;
 }
 
+// FixedPosition mode-1 key offsets of key (X, Y) in a u x v lens lattice, in collection
+// order: the reference's if/else chain (SelectNeighbors.cpp:1062-1286) branch for branch,
+// including its top-row test, which holds for every X on row 0.
+static int fixed_offsets(int X, int Y, int u, int v, int out[12][2]) {
+    int n = 0;
+    auto add = [&](int dx, int dy) { out[n][0] = dx; out[n][1] = dy; ++n; };
+    if (X > 1 && X < u - 2 && Y > 0 && Y < v - 1) {
+        add(1, 0); add(-1, 0); add(2, 0); add(0, -1); add(-2, -1); add(0, 1);
+        add(-2, 0); add(-1, -1); add(1, -1); add(-1, 1); add(1, 1); add(2, 1);
+    } else if (X == 0 && Y == 0) {
+        add(1, 0); add(2, 0); add(0, 1); add(1, 1); add(2, 2); add(0, 2);
+    } else if ((X == 1 && Y == 0) || ((X != u - 2 || X != u - 1) && Y == 0)) {
+        add(1, 0); add(-1, 0); add(2, 0); add(0, 1); add(-1, 1); add(1, 1);
+    } else if (X == 0 && Y == v - 1) {
+        add(1, 0); add(2, 0); add(0, -1); add(1, -1); add(2, -2); add(0, -2);
+    } else if (X == 1 && Y == v - 1) {
+        add(1, 0); add(-1, 0); add(2, 0); add(0, -1); add(-1, -1); add(1, -1);
+    } else if (X == u - 1 && Y == v - 1) {
+        add(-1, 0); add(0, -1); add(-2, -1); add(-2, 0); add(0, -2); add(-1, -1);
+    } else if (X == u - 2 && Y == v - 1) {
+        add(1, 0); add(-1, 0); add(0, -1); add(-2, -1); add(-1, -1); add(1, -1);
+    } else if (X == u - 1 && Y == 0) {
+        add(-1, 0); add(0, 1); add(-2, 2); add(-2, 0); add(0, 2); add(-1, 1);
+    } else if (X == u - 2 && Y == 0) {
+        add(1, 0); add(-1, 0); add(0, 1); add(-2, 0); add(-1, 1); add(1, 1);
+    } else if (X == 0) {
+        add(1, 0); add(2, 0); add(0, -1); add(0, 1); add(1, -1); add(1, 1);
+    } else if (X == 1) {
+        add(1, 0); add(-1, 0); add(2, 0); add(0, -1); add(0, 1); add(-1, -1);
+    } else if (Y == v - 1) {
+        add(1, 0); add(-1, 0); add(2, 0); add(0, -1); add(-2, -1); add(1, -1);
+    } else if (X == u - 1) {
+        add(-1, 0); add(0, -1); add(-2, -1); add(0, 1); add(-1, -1); add(-1, 1);
+    } else if (X == u - 2) {
+        add(1, 0); add(-1, 0); add(0, -1); add(-2, -1); add(0, 1); add(1, 1);
+    }
+    return n;
+}
+
+// FixedPosition table of a layout: kept neighbours sorted by centre distance (insertion sort,
+// so equal distances keep the collection order), cap K; garbage = key column/row 0 or none.
+static int fixed_neighbors(const LayoutParams& lp, int* nbr, int* cnt, int* garbage) {
+    const int V = layout(lp, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    std::vector<float> cen(2 * (size_t)V);
+    std::vector<int> keys(2 * (size_t)V);
+    layout(lp, V, cen.data(), nullptr, keys.data(), nullptr, nullptr, nullptr);
+    const double rs = (double)lp.pitch * 0.8660254037844386;
+    const int u = (int)std::ceil(lp.raw_w / (double)lp.pitch) + 1;
+    const int v = (int)std::ceil(lp.raw_h / rs) + 1;
+    std::vector<int> cell((size_t)u * v, -1);
+    for (int m = 0; m < V; ++m) cell[(size_t)keys[2 * m + 1] * u + keys[2 * m]] = m;
+    const int K = lp.max_neighbors;
+    for (int m = 0; m < V; ++m) {
+        const int X = keys[2 * m], Y = keys[2 * m + 1];
+        int off[12][2];
+        const int n = fixed_offsets(X, Y, u, v, off);
+        std::vector<std::pair<float, int>> list;  // (distance, MI), insertion-sorted
+        for (int e = 0; e < n; ++e) {
+            const int nx = X + off[e][0], ny = Y + off[e][1];
+            if (nx < 0 || ny < 0 || nx >= u || ny >= v) continue;
+            const int id = cell[(size_t)ny * u + nx];
+            if (id < 0) continue;
+            const float dx = cen[2 * id] - cen[2 * m], dy = cen[2 * id + 1] - cen[2 * m + 1];
+            const float len = (float)std::sqrt(std::pow(dy, 2) + std::pow(dx, 2));
+            size_t at = list.size();
+            while (at > 0 && list[at - 1].first > len) --at;
+            list.insert(list.begin() + (long)at, std::make_pair(len, id));
+        }
+        const int c = std::min((int)list.size(), K);
+        if (cnt) cnt[m] = c;
+        if (nbr)
+            for (int k = 0; k < K; ++k) nbr[(size_t)m * K + k] = k < c ? list[k].second : -1;
+        if (garbage) garbage[m] = (list.empty() || X < 1 || Y < 1) ? 1 : 0;
+    }
+    return V;
+}
+
 // ------------------------------------------------------------------------------------------
 // Fused CheckModule filter, FP32 twin (bit-exact contract with the GPU pass)
 // ------------------------------------------------------------------------------------------
@@ -1562,6 +1643,17 @@ void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
 int orc_layout(const orc::LayoutParams* lp, int cap, float* centers, int* origins, int* tilekeys,
                int* lens_type, int* nbr, int* cnt) {
     return orc::layout(*lp, cap, centers, origins, tilekeys, lens_type, nbr, cnt);
+}
+
+int orc_fixed_offsets(int X, int Y, int u, int v, int* out) {
+    int off[12][2];
+    const int n = orc::fixed_offsets(X, Y, u, v, off);
+    for (int e = 0; e < n; ++e) { out[2 * e] = off[e][0]; out[2 * e + 1] = off[e][1]; }
+    return n;
+}
+
+int orc_fixed_neighbors(const orc::LayoutParams* lp, int* nbr, int* cnt, int* garbage) {
+    return orc::fixed_neighbors(*lp, nbr, cnt, garbage);
 }
 
 void orc_slice_u8(const uint8_t* raw, int raw_w, int raw_h, const int* origins, int V, int crop,

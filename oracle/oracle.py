@@ -89,6 +89,25 @@ def layout(lp):
                 nbr=nbr, cnt=cnt)
 
 
+def fixed_offsets(X, Y, u, v):
+    """FixedPosition mode-1 key offsets of key (X, Y) in a u x v lattice, collection order."""
+    out = np.zeros(24, np.int32)
+    n = lib().orc_fixed_offsets(int(X), int(Y), int(u), int(v), _p(out))
+    return [tuple(int(t) for t in out[2 * e:2 * e + 2]) for e in range(n)]
+
+
+def fixed_neighbors(lp):
+    """FixedPosition neighbour table of a layout: (nbr (V, K), cnt (V,), garbage (V,))."""
+    L = lib()
+    V = L.orc_layout(C.byref(lp), 0, None, None, None, None, None, None)
+    K = lp.max_neighbors
+    nbr = np.zeros((V, K), np.int32)
+    cnt = np.zeros(V, np.int32)
+    garbage = np.zeros(V, np.int32)
+    L.orc_fixed_neighbors(C.byref(lp), _p(nbr), _p(cnt), _p(garbage))
+    return nbr, cnt, garbage
+
+
 def slice_u8(raw, origins, crop):
     raw = np.ascontiguousarray(raw, np.uint8)
     V = origins.shape[0]

@@ -112,6 +112,54 @@ static int build_lattice(const lfpm_layout_params* p, Lattice& L) {
     return LFPM_OK;
 }
 
+// FixedPosition neighbours, mode 1 (SelectNeighbors::SelectFixedPositionMIByMode1,
+// MINeighbor/SelectNeighbors.cpp:1055-1286): key offsets (dX, dY) in the order the reference
+// collects them, one list per place of the key in the u x v lens lattice.
+enum FixedPlace {
+    FP_INTERIOR, FP_TOP_LEFT, FP_TOP, FP_BOTTOM_LEFT, FP_BOTTOM_1, FP_BOTTOM_RIGHT,
+    FP_BOTTOM_RIGHT_2, FP_TOP_RIGHT, FP_TOP_RIGHT_2, FP_LEFT, FP_LEFT_1, FP_BOTTOM, FP_RIGHT,
+    FP_RIGHT_2, FP_NONE
+};
+
+struct FixedList { int n; int8_t d[12][2]; };
+
+static const FixedList kFixedLists[FP_NONE] = {
+    {12, {{1, 0}, {-1, 0}, {2, 0}, {0, -1}, {-2, -1}, {0, 1}, {-2, 0}, {-1, -1}, {1, -1},
+          {-1, 1}, {1, 1}, {2, 1}}},                                              // :1065-1087
+    {6, {{1, 0}, {2, 0}, {0, 1}, {1, 1}, {2, 2}, {0, 2}}},                        // :1094-1104
+    {6, {{1, 0}, {-1, 0}, {2, 0}, {0, 1}, {-1, 1}, {1, 1}}},                      // :1110-1120
+    {6, {{1, 0}, {2, 0}, {0, -1}, {1, -1}, {2, -2}, {0, -2}}},                    // :1125-1135
+    {6, {{1, 0}, {-1, 0}, {2, 0}, {0, -1}, {-1, -1}, {1, -1}}},                   // :1140-1150
+    {6, {{-1, 0}, {0, -1}, {-2, -1}, {-2, 0}, {0, -2}, {-1, -1}}},                // :1155-1165
+    {6, {{1, 0}, {-1, 0}, {0, -1}, {-2, -1}, {-1, -1}, {1, -1}}},                 // :1170-1180
+    {6, {{-1, 0}, {0, 1}, {-2, 2}, {-2, 0}, {0, 2}, {-1, 1}}},                    // :1185-1195
+    {6, {{1, 0}, {-1, 0}, {0, 1}, {-2, 0}, {-1, 1}, {1, 1}}},                     // :1200-1210
+    {6, {{1, 0}, {2, 0}, {0, -1}, {0, 1}, {1, -1}, {1, 1}}},                      // :1215-1225
+    {6, {{1, 0}, {-1, 0}, {2, 0}, {0, -1}, {0, 1}, {-1, -1}}},                    // :1230-1240
+    {6, {{1, 0}, {-1, 0}, {2, 0}, {0, -1}, {-2, -1}, {1, -1}}},                   // :1245-1255
+    {6, {{-1, 0}, {0, -1}, {-2, -1}, {0, 1}, {-1, -1}, {-1, 1}}},                 // :1260-1270
+    {6, {{1, 0}, {-1, 0}, {0, -1}, {-2, -1}, {0, 1}, {1, 1}}},                    // :1275-1285
+};
+
+// Place of key (X, Y), tested in the reference's order.  Its top-row test
+// `(X == 1 && Y == 0) || ((X != u-2 || X != u-1) && Y == 0)` holds for every X on row 0, so
+// FP_TOP_RIGHT and FP_TOP_RIGHT_2 are never chosen, as in the reference.
+static FixedPlace fixed_place(int X, int Y, int u, int v) {
+    if (X > 1 && X < u - 2 && Y > 0 && Y < v - 1) return FP_INTERIOR;
+    if (X == 0 && Y == 0) return FP_TOP_LEFT;
+    if (Y == 0) return FP_TOP;
+    if (X == 0 && Y == v - 1) return FP_BOTTOM_LEFT;
+    if (X == 1 && Y == v - 1) return FP_BOTTOM_1;
+    if (X == u - 1 && Y == v - 1) return FP_BOTTOM_RIGHT;
+    if (X == u - 2 && Y == v - 1) return FP_BOTTOM_RIGHT_2;
+    if (X == 0) return FP_LEFT;
+    if (X == 1) return FP_LEFT_1;
+    if (Y == v - 1) return FP_BOTTOM;
+    if (X == u - 1) return FP_RIGHT;
+    if (X == u - 2) return FP_RIGHT_2;
+    return FP_NONE;
+}
+
 } // namespace lfpm
 
 using namespace lfpm;
@@ -179,6 +227,49 @@ int lfpm_layout_build(const lfpm_layout_params* p, float* centers, int* origins,
         if (neighbor_counts) neighbor_counts[v] = c;
     }
     if (lattice_to_mi) std::memcpy(lattice_to_mi, L.cell_to_mi.data(), L.cell_to_mi.size() * 4);
+    return LFPM_OK;
+}
+
+int lfpm_layout_fixed_neighbors(const lfpm_layout_params* p, int* neighbor_ids,
+                                int* neighbor_counts, int* garbage) {
+    Lattice L;
+    const int rc = build_lattice(p, L);
+    if (rc) return rc;
+    const int K = p->max_neighbors;
+    if (K <= 0 || K > LFPM_MAX_NEIGHBORS)
+        return set_error(LFPM_E_INVALID, "fixed neighbours: max_neighbors %d outside [1, %d]", K,
+                         LFPM_MAX_NEIGHBORS);
+    const int V = (int)L.mi_i.size();
+    struct Nb { int id; float len; };
+    std::vector<Nb> nb;
+    for (int v = 0; v < V; ++v) {
+        const int X = L.mi_i[v], Y = L.mi_j[v];
+        nb.clear();
+        const FixedPlace place = fixed_place(X, Y, L.cols, L.rows);
+        if (place != FP_NONE) {
+            const FixedList& fl = kFixedLists[place];
+            for (int e = 0; e < fl.n; ++e) {
+                // CollectValidNeighborKey (:1330-1352): the key must name a kept lens
+                const int nx = X + fl.d[e][0], ny = Y + fl.d[e][1];
+                if (nx < 0 || ny < 0 || nx >= L.cols || ny >= L.rows) continue;
+                const int u = L.cell_to_mi[(size_t)ny * L.cols + nx];
+                if (u < 0) continue;
+                // CreateResTileKeysFromNeiKeyVec (MVStereo/LFDepthInfo.cpp:431-453): centre
+                // distance from float differences, squared and summed in double
+                const float dx = L.cx[u] - L.cx[v], dy = L.cy[u] - L.cy[v];
+                nb.push_back(Nb{u, (float)std::sqrt((double)dy * dy + (double)dx * dx)});
+            }
+        }
+        // ascending distance; ties keep the collection order (std::sort leaves them unspecified)
+        std::stable_sort(nb.begin(), nb.end(), [](const Nb& a, const Nb& b) { return a.len < b.len; });
+        const int c = std::min((int)nb.size(), K);
+        if (neighbor_ids)
+            for (int k = 0; k < K; ++k) neighbor_ids[(size_t)v * K + k] = k < c ? nb[k].id : -1;
+        if (neighbor_counts) neighbor_counts[v] = c;
+        // DepthSolver::ConfirmProblemForEstimation (DepthSolver.cpp:2933-2946) with
+        // m_iGarbageRows = 1 (:117): first key row/column, or no neighbour at all
+        if (garbage) garbage[v] = (nb.empty() || X < 1 || Y < 1) ? 1 : 0;
+    }
     return LFPM_OK;
 }
 

@@ -576,6 +576,37 @@ DepthSolverManager::DepthSolverManager(std::string& strRootPath, std::string& st
 
 DepthSolverManager::~DepthSolverManager() = default;
 
+void SelectNeighbors::SelectNeighborsForMIA(QuadTreeProblemMapMap::iterator& itrPM,
+                                            eSelectNeighborsType eSelectType) {
+    if (!m_ptrDepthSolver) return;
+    if (eSelectType != eSNT_FixedPosition)
+        throw std::runtime_error("SelectNeighbors: only eSNT_FixedPosition is provided");
+    DepthSolver& s = *m_ptrDepthSolver;
+    if (!s.m_have_layout) throw std::runtime_error("SelectNeighbors: no MLA layout set");
+    const int V = (int)s.m_keys.size(), K = s.m_lp.max_neighbors;
+    std::vector<int> nbr((size_t)V * K), cnt(V), garbage(V);
+    check(lfpm_layout_fixed_neighbors(&s.m_lp, nbr.data(), cnt.data(), garbage.data()),
+          "SelectNeighbors::SelectNeighborsForMIA");
+    std::vector<float> cen(2 * (size_t)V);
+    check(lfpm_layout_build(&s.m_lp, cen.data(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                            nullptr),
+          "SelectNeighbors::SelectNeighborsForMIA");
+    QuadTreeProblemMap& problem_map = itrPM->second;
+    for (int v = 0; v < V; ++v) {
+        auto it = problem_map.find(s.m_keys[v]);
+        if (it == problem_map.end()) continue;
+        MLA_Problem& problem = it->second;
+        problem.m_Res_Image_KeyVec.clear();
+        for (int k = 0; k < cnt[v]; ++k) {
+            const int u = nbr[(size_t)v * K + k];
+            const float dx = cen[2 * u] - cen[2 * v], dy = cen[2 * u + 1] - cen[2 * v + 1];
+            problem.m_Res_Image_KeyVec.push_back(
+                Res_image_Key{s.m_keys[u], (float)std::sqrt((double)dy * dy + (double)dx * dx)});
+        }
+        problem.m_bGarbage = garbage[v] != 0;
+    }
+}
+
 int DepthSolverManager::DepthSolverPipelineBySequence(eStereoType eStereoType) {
     if (!m_ptrDepthSolver) return -1;
     DepthSolver& s = *m_ptrDepthSolver;

@@ -132,6 +132,18 @@ def layout(lp):
     return out
 
 
+def fixed_neighbors(lp):
+    """FixedPosition neighbour table (lfpm_layout_fixed_neighbors): nbr (V, K), cnt, garbage."""
+    info = LayoutInfo()
+    _check(lib().lfpm_layout_query(C.byref(lp), C.byref(info)))
+    V, K = info.num_mis, lp.max_neighbors
+    nbr = np.zeros((V, K), np.int32)
+    cnt = np.zeros(V, np.int32)
+    garbage = np.zeros(V, np.int32)
+    _check(lib().lfpm_layout_fixed_neighbors(C.byref(lp), _np(nbr), _np(cnt), _np(garbage)))
+    return nbr, cnt, garbage
+
+
 def synth(sp, want_gt=True):
     raw = np.zeros((sp.raw_h, sp.raw_w), np.uint8)
     gt = np.zeros((sp.raw_h, sp.raw_w), np.float32) if want_gt else None

@@ -21,7 +21,8 @@ def pytest_configure(config):
     if not (os.path.exists(os.path.join(PKG, "liblfpm.so")) and
             os.path.exists(os.path.join(PKG, "liblfmvs.so"))):
         subprocess.check_call(["make", "-s", "-C", PKG])
-    if not os.path.exists(os.path.join(ROOT, "tests", "cpp", "_build", "facade_driver")):
+    if not all(os.path.exists(os.path.join(ROOT, "tests", "cpp", "_build", n))
+               for n in ("facade_driver", "select_neighbors_driver")):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp")])
 
 
