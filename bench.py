@@ -196,7 +196,8 @@ def measured_traffic(row):
         return None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0].startswith("lfpm::k_stage") and cells[0].endswith(row):
+        if len(cells) == 5 and cells[0].startswith("lfpm::") and "k_stage" in cells[0] and \
+                cells[0].endswith(" " + row):
             try:
                 return float(cells[4])
             except ValueError:
