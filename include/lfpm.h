@@ -139,6 +139,17 @@ typedef struct lfpm_pm_params {
                                   1/255), FP32 arithmetic on the 0..255 scale */
     float cos_max_slant;       /* planes with |nz| < cos_max_slant score 2 (0 = no gate, the
                                   reference behaviour; default 0.5 = 60 degrees) */
+    float blur_lambda;         /* blur term of the cost vector: cost = min(2, ncc + blur_lambda *
+                                  (1 - exp(-0.007368 * (sum b_ref - sum b_src)^2))) over the
+                                  window samples that enter the NCC (ComputeMultiViewCostVector_
+                                  Frame, MIStereo/AdaptMIPMFrame.cu:211-218: 2.0; the ACMM twin
+                                  AdaptMIPMFrameACMM.cu:164: 0.8).  0 (default) = off.  When > 0
+                                  the tiles buffer holds 2*V*crop^2 floats: the intensities, then
+                                  the MIs' blur maps (blureness / 255) in the same layout.
+                                  Provided by lfpm_patchmatch_frame/_init/_phase/_finalize and
+                                  lfpm_eval_costs with FP32 tiles (storage 0) and the default
+                                  solver; the ACMM, batched and host-context entry points
+                                  return LFPM_E_UNSUPPORTED for it. */
 } lfpm_pm_params;
 
 /* Fill defaults for a given crop/base (reference defaults + config conventions). */

@@ -147,7 +147,7 @@ struct PmParams {
     float sigma_spatial; float sigma_color; float min_var; int top_k;
     float depth_min, depth_max; int iterations; int max_neighbors;
     uint32_t seed; uint32_t frame_uid; int storage;
-    float cos_max_slant;
+    float cos_max_slant; float blur_lambda;
 };
 
 struct FilterParams {
@@ -171,6 +171,7 @@ template <typename T> struct P4 { T x, y, z, w; };
 template <typename T> struct Frame {
     int V, W, K;
     std::vector<T> img;           // V*W*W intensities in [0,1]
+    std::vector<T> blur;          // V*W*W blur values in [0,1] (blur term on), else empty
     const float* centers;         // 2V
     const int* nbr;               // V*K
     const int* cnt;               // V
@@ -178,18 +179,22 @@ template <typename T> struct Frame {
     T inv_base;
 
     T I(int v, int x, int y) const { return img[(size_t)v * W * W + (size_t)y * W + x]; }
+    T Bv(int v, int x, int y) const { return blur[(size_t)v * W * W + (size_t)y * W + x]; }
 
-    // Bilinear with clamp-to-edge; (x, y) in [0, W) x [0, W).
-    T bilinear(int v, T x, T y) const {
+    // Bilinear with clamp-to-edge of plane `src` (img or blur); (x, y) in [0, W) x [0, W).
+    T bilinear_of(const std::vector<T>& src, int v, T x, T y) const {
         const T fx0 = std::floor(x), fy0 = std::floor(y);
         const int x0 = (int)fx0, y0 = (int)fy0;
         const int x1 = std::min(x0 + 1, W - 1), y1 = std::min(y0 + 1, W - 1);
         const T fx = x - fx0, fy = y - fy0;
-        const T a = I(v, x0, y0), b = I(v, x1, y0), c = I(v, x0, y1), d = I(v, x1, y1);
+        const size_t o = (size_t)v * W * W;
+        const T a = src[o + (size_t)y0 * W + x0], b = src[o + (size_t)y0 * W + x1];
+        const T c = src[o + (size_t)y1 * W + x0], d = src[o + (size_t)y1 * W + x1];
         const T top = a + fx * (b - a);
         const T bot = c + fx * (d - c);
         return top + fy * (bot - top);
     }
+    T bilinear(int v, T x, T y) const { return bilinear_of(img, v, x, y); }
     int count(int v) const { return std::min(cnt[v], K); }
 };
 
@@ -233,6 +238,8 @@ static T ncc(const Frame<T>& f, const Win<T>& ww, int v, int k, int px, int py, 
     const T A = -h.x / h.z, B = -h.y / h.z;
     const int r = f.p.radius, inc = f.p.radius_increment, nn = 2 * r + 1;
     T s_w = 0, s_x = 0, s_xx = 0, s_y = 0, s_yy = 0, s_xy = 0;
+    T b_ref = 0, b_src = 0;  // blur sums (ComputeBilateralNCC_Frame :140-144)
+    const bool blur = !f.blur.empty();
     for (int i = -r; i <= r; i += inc) {
         for (int j = -r; j <= r; j += inc) {
             const int rx = px + i, ry = py + j;
@@ -240,6 +247,10 @@ static T ncc(const Frame<T>& f, const Win<T>& ww, int v, int k, int px, int py, 
             const T d = h.w + A * (T)i + B * (T)j;
             const T qx = (T)rx + sx * d, qy = (T)ry + sy * d;
             if (!(qx >= (T)0 && qx < W && qy >= (T)0 && qy < W)) continue;
+            if (blur) {
+                b_ref += f.Bv(v, rx, ry);
+                b_src += f.bilinear_of(f.blur, n, qx, qy);
+            }
             const T wgt = ww.w[(j + r) * nn + (i + r)];
             const T x = f.I(v, rx, ry) - ww.rc;
             const T y = f.bilinear(n, qx, qy) - ww.rc;
@@ -258,8 +269,15 @@ static T ncc(const Frame<T>& f, const Win<T>& ww, int v, int k, int px, int py, 
     const T vy = s_yy * inv - my * my;
     if (vx < (T)f.p.min_var || vy < (T)f.p.min_var) return (T)2;
     const T cov = s_xy * inv - mx * my;
-    const T c = (T)1 - cov / std::sqrt(vx * vy);
-    return std::min((T)2, std::max((T)0, c));
+    T c = std::min((T)2, std::max((T)0, (T)1 - cov / std::sqrt(vx * vy)));
+    if (blur) {
+        // blur term of ComputeMultiViewCostVector_Frame (AdaptMIPMFrame.cu:211-218); the
+        // early returns above are cost 2, which the clamp keeps at 2 with or without it
+        const T db = b_ref - b_src;
+        const T bw = std::exp((T)-0.007368f * db * db);
+        c = std::min((T)2, std::max((T)0, c + (T)f.p.blur_lambda * ((T)1 - bw)));
+    }
+    return c;
 }
 
 template <typename T>
@@ -773,6 +791,10 @@ static void build_frame(Frame<T>& f, const uint8_t* tiles_u8, const float* cente
     const size_t n = (size_t)V * p.crop * p.crop;
     f.img.resize(n);
     for (size_t i = 0; i < n; ++i) f.img[i] = (T)tiles_u8[i] / (T)255;
+    if (p.blur_lambda > 0.0f) {  // blur maps follow the intensity tiles
+        f.blur.resize(n);
+        for (size_t i = 0; i < n; ++i) f.blur[i] = (T)tiles_u8[n + i] / (T)255;
+    }
 }
 
 template <typename T>

@@ -19,7 +19,8 @@ class PmParams(C.Structure):
                 ("sigma_color", C.c_float), ("min_var", C.c_float), ("top_k", C.c_int),
                 ("depth_min", C.c_float), ("depth_max", C.c_float), ("iterations", C.c_int),
                 ("max_neighbors", C.c_int), ("seed", C.c_uint32), ("frame_uid", C.c_uint32),
-                ("storage", C.c_int), ("cos_max_slant", C.c_float)]
+                ("storage", C.c_int), ("cos_max_slant", C.c_float),
+                ("blur_lambda", C.c_float)]
 
 
 class FilterParams(C.Structure):
@@ -117,8 +118,19 @@ def slice_u8(raw, origins, crop):
     return tiles
 
 
-def solve(tiles_u8, centers, nbr, cnt, pm, dtype=np.float64, threads=None):
+def _with_blur(tiles_u8, blur_u8, pm):
+    """Tiles as the C functions read them: the blur maps follow the intensities when the blur
+    term is on (pm.blur_lambda > 0)."""
+    tiles_u8 = np.ascontiguousarray(tiles_u8, np.uint8)
+    if pm.blur_lambda > 0:
+        assert blur_u8 is not None and blur_u8.shape == tiles_u8.shape
+        return np.ascontiguousarray(np.concatenate([tiles_u8, blur_u8.astype(np.uint8)]))
+    return tiles_u8
+
+
+def solve(tiles_u8, centers, nbr, cnt, pm, dtype=np.float64, threads=None, blur_u8=None):
     V, W, _ = tiles_u8.shape
+    tiles_u8 = _with_blur(tiles_u8, blur_u8, pm)
     n = V * W * W
     plane = np.zeros((n, 4), dtype)
     cost = np.zeros(n, dtype)
@@ -131,9 +143,11 @@ def solve(tiles_u8, centers, nbr, cnt, pm, dtype=np.float64, threads=None):
     return dict(plane=plane, cost=cost, disp=disp, sel=sel)
 
 
-def eval_costs(tiles_u8, centers, nbr, cnt, pm, hyp, dtype=np.float64, threads=None):
+def eval_costs(tiles_u8, centers, nbr, cnt, pm, hyp, dtype=np.float64, threads=None,
+               blur_u8=None):
     """hyp: structured (n, 6) int32 view: mi, pixel, then 4 float32 bit patterns."""
     V = tiles_u8.shape[0]
+    tiles_u8 = _with_blur(tiles_u8, blur_u8, pm)
     n = hyp.shape[0]
     out = np.zeros((n, pm.max_neighbors), dtype)
     f = lib().orc_eval_costs_f64 if dtype == np.float64 else lib().orc_eval_costs_f32
@@ -255,6 +269,7 @@ def vd_map(plane, centers, gray, crop, mla_base, raw_w, raw_h, color=True):
 
 def solve_acmm(tiles_u8, centers, nbr, cnt, pm, threads=None):
     """ACMM multi-scale solve (FP64) of the final level: dict plane/cost/disp/sel."""
+    assert pm.blur_lambda == 0, "the ACMM restatement has no blur term"
     V, W, _ = tiles_u8.shape
     n = V * W * W
     plane = np.zeros((n, 4), np.float64)

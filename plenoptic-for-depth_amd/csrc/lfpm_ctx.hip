@@ -117,6 +117,9 @@ static int alloc_tables(lfpm_ctx* c, int V, int W, int K) {
 
 // Device buffers for n frames (kept when the count and batching match) and the workspace.
 static int ensure_frames(lfpm_ctx* c, int n, bool batched, const lfpm_pm_params* p) {
+    if (p && p->blur_lambda != 0.0f)
+        return set_error(LFPM_E_UNSUPPORTED, "ctx: the blur term needs blur tiles; it is "
+                         "provided by lfpm_patchmatch_frame / lfpm_eval_costs only");
     const size_t npx = (size_t)c->V * c->crop * c->crop;
     const size_t rawb = (size_t)c->raw_w * c->raw_h;
     if ((int)c->frames.size() != n || c->frames_batched != batched) {

@@ -291,6 +291,7 @@ void lfpm_pm_params_default(lfpm_pm_params* p, int crop, float base) {
     p->frame_uid = 0u;
     p->storage = 0;
     p->cos_max_slant = 0.5f;
+    p->blur_lambda = 0.0f;          // blur term off: it needs per-MI blur maps
 }
 
 void lfpm_filter_params_default(lfpm_filter_params* f) {

@@ -37,6 +37,8 @@ struct DevParams {
     float cos_max_slant;
     uint32_t seed, uid;
     int slots_per_xcd;   // persistent grid: workgroups per XCD
+    int blur;            // blur term on (blur_lambda > 0; wave engine, FP32 tiles)
+    float blur_lambda;
 };
 
 static DevParams make_dev_params(const lfpm_pm_params* p, int V) {
@@ -62,6 +64,8 @@ static DevParams make_dev_params(const lfpm_pm_params* p, int V) {
     d.seed = p->seed;
     d.uid = p->frame_uid;
     d.slots_per_xcd = 1;
+    d.blur = p->blur_lambda > 0.0f ? 1 : 0;
+    d.blur_lambda = p->blur_lambda;
     return d;
 }
 
@@ -80,6 +84,10 @@ static int check_pm_params(const lfpm_pm_params* p, int V) {
         return set_error(LFPM_E_INVALID, "sigmas must be > 0");
     if (p->storage != 0 && p->storage != 1)
         return set_error(LFPM_E_UNSUPPORTED, "storage mode %d", p->storage);
+    if (!(p->blur_lambda >= 0.0f) || !std::isfinite(p->blur_lambda))
+        return set_error(LFPM_E_INVALID, "blur_lambda must be finite and >= 0");
+    if (p->blur_lambda > 0.0f && p->storage != 0)
+        return set_error(LFPM_E_UNSUPPORTED, "blur term: storage must be 0 (FP32 tiles)");
     return LFPM_OK;
 }
 
@@ -225,6 +233,7 @@ struct Lds {
     int* nid;         // K
     uint8_t* wts;     // K*NT view weights (phase kernel)
     uint8_t* slist;   // K*NT selected-view lists (phase kernel)
+    float* btiles;    // blur term: blur values of the same (K+1) slots, FP32 (else null)
 };
 
 static size_t lds_bytes(const DevParams& P, int NT, bool phase) {
@@ -253,6 +262,7 @@ __device__ __forceinline__ Lds carve(unsigned char* base, const DevParams& P, in
     L.nid = (int*)q; q += (size_t)P.K * 4;
     L.wts = (uint8_t*)q; q += (size_t)P.K * NT;
     L.slist = (uint8_t*)q;
+    L.btiles = nullptr;
     return L;
 }
 
@@ -293,7 +303,8 @@ template <typename TT>
 __device__ __forceinline__ int stage_mi(Lds& L, const DevParams& P, int mi,
                                         const float* __restrict__ tiles,
                                         const float2* __restrict__ centers,
-                                        const int* __restrict__ nbr, const int* __restrict__ cnt) {
+                                        const int* __restrict__ nbr, const int* __restrict__ cnt,
+                                        const float* __restrict__ btiles = nullptr) {
     const int K = P.K, WW = P.WW;
     const int c = min(cnt[mi], K);
     const float2 cr = centers[mi];
@@ -321,6 +332,7 @@ __device__ __forceinline__ int stage_mi(Lds& L, const DevParams& P, int mi,
             if (src < 0) src = mi;  // never read: views with nid < 0 are not scored
         }
         stt((TT*)tile_base<TT>(L) + idx, tiles[(size_t)src * WW + yy * W + xx]);
+        if (btiles) L.btiles[idx] = btiles[(size_t)src * WW + yy * W + xx];
     }
     return c;
 }
@@ -997,21 +1009,28 @@ struct WaveLds {
     float* cost;       // [kHMax][K] NCC cost of (hypothesis, view); 2 = not scored
     uint16_t* task;    // (h << 8) | k
     int* iaux;         // 2*K ints (draw weights, selected-view list)
+    float* sblur;      // blur term: reference blur minus the centre blur per compacted sample
 };
 
 __host__ __device__ inline size_t a16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-__host__ __device__ inline size_t shared_bytes(const DevParams& P) {
+__host__ __device__ inline size_t shared_core_bytes(const DevParams& P) {
     const int n = 2 * P.r + 1;
     return a16((size_t)(P.K + 1) * P.TS * P.tile_bytes) + a16((size_t)P.WW * 4) +
            a16((size_t)n * n * 4) + a16((size_t)P.K * 16) + a16((size_t)P.K * 4);
+}
+
+// Shared part: tiles, tables, and (blur term) the FP32 blur slots after them.
+__host__ __device__ inline size_t shared_bytes(const DevParams& P) {
+    return shared_core_bytes(P) + (P.blur ? a16((size_t)(P.K + 1) * P.TS * 4) : 0);
 }
 
 __host__ __device__ inline size_t wave_bytes(const DevParams& P) {
     const int m = 2 * P.r / P.inc + 1;
     const size_t ns = (size_t)((m * m + 7) & ~7);
     return a16(ns * 16) + a16((size_t)kHMax * 16) + a16((size_t)kHMax * 8) +
-           a16((size_t)kHMax * P.K * 4) + a16((size_t)kHMax * P.K * 2) + a16((size_t)P.K * 8);
+           a16((size_t)kHMax * P.K * 4) + a16((size_t)kHMax * P.K * 2) + a16((size_t)P.K * 8) +
+           (P.blur ? a16(ns * 4) : 0);
 }
 
 __host__ __device__ inline size_t lds_total(const DevParams& P, int nwaves) {
@@ -1035,6 +1054,7 @@ __device__ __forceinline__ Lds carve_shared(unsigned char* base, const DevParams
     L.nid = (int*)q;
     L.wts = nullptr;
     L.slist = nullptr;
+    L.btiles = P.blur ? (float*)(base + shared_core_bytes(P)) : nullptr;
     return L;
 }
 
@@ -1048,7 +1068,8 @@ __device__ __forceinline__ WaveLds carve_wave(unsigned char* base, const DevPara
     S.hab = (float2*)q; q += a16((size_t)kHMax * 8);
     S.cost = (float*)q; q += a16((size_t)kHMax * P.K * 4);
     S.task = (uint16_t*)q; q += a16((size_t)kHMax * P.K * 2);
-    S.iaux = (int*)q;
+    S.iaux = (int*)q; q += a16((size_t)P.K * 8);
+    S.sblur = P.blur ? (float*)q : nullptr;
     return S;
 }
 
@@ -1056,7 +1077,7 @@ __device__ __forceinline__ WaveLds carve_wave(unsigned char* base, const DevPara
 // the window order of ncc_group (i outer, j inner), zero-padded to a multiple of 8.
 template <typename TT>
 __device__ int build_samples(const Lds& L, const DevParams& P, const WaveLds& S, int px, int py,
-                             float rc) {
+                             float rc, float rbc) {
     const int lane = lane_id();
     const int r = P.r, inc = P.inc, W = P.W, n = 2 * r + 1, m = grid_side(P);
     const float inv_sc = TileScale<TT>::inv_sc(P);
@@ -1065,6 +1086,7 @@ __device__ int build_samples(const Lds& L, const DevParams& P, const WaveLds& S,
         const int w = b + lane;
         bool valid = false;
         float4 rec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float bv = 0.0f;
         if (w < m * m) {
             const int ii = w / m, jj = w - ii * m;
             const int i = -r + ii * inc, j = -r + jj * inc;
@@ -1074,14 +1096,21 @@ __device__ int build_samples(const Lds& L, const DevParams& P, const WaveLds& S,
                 const float x = ldt(tile_base<TT>(L) + ry * P.W1 + rx) - rc;
                 const float wgt = __expf(L.spat[(j + r) * n + (i + r)] - fabsf(x) * inv_sc);
                 rec = make_float4((float)i, (float)j, x, wgt);
+                if (P.blur) bv = L.btiles[ry * P.W1 + rx] - rbc;
             }
         }
         const uint64_t bal = __ballot(valid);
-        if (valid) S.samp[total + prefix_below(bal)] = rec;
+        if (valid) {
+            S.samp[total + prefix_below(bal)] = rec;
+            if (P.blur) S.sblur[total + prefix_below(bal)] = bv;
+        }
         total += __popcll(bal);
     }
     const int pad = ((total + 7) & ~7) - total;
-    if (lane < pad) S.samp[total + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (lane < pad) {
+        S.samp[total + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (P.blur) S.sblur[total + lane] = 0.0f;
+    }
     return total;
 }
 
@@ -1143,10 +1172,14 @@ __device__ __forceinline__ int add_tasks_rows(const Lds& L, const DevParams& P,
     return ntask;
 }
 
-// Evaluate the task list: S.cost[h][k] for every task.  ns = valid window samples.
-template <bool COUNT, typename TT>
+// Evaluate the task list: S.cost[h][k] for every task.  ns = valid window samples.  BLUR adds
+// the blur term of ComputeMultiViewCostVector_Frame (MIStereo/AdaptMIPMFrame.cu:211-218; blur
+// sums of ComputeBilateralNCC_Frame :140-144 over the samples that enter the moments):
+// cost = min(2, ncc + lambda * (1 - exp(-0.007368 * (sum_ref_blur - sum_src_blur)^2))), with
+// both sums taken about the centre pixel's blur rbc (the offsets cancel in the difference).
+template <bool COUNT, typename TT, bool BLUR>
 __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, int ntask, int ns,
-                          int px, int py, float rc, WorkCount& wc) {
+                          int px, int py, float rc, float rbc, WorkCount& wc) {
     const int lane = lane_id();
     const int q = lane >> 3, u = lane & 7;
     const int T = (ns + 7) >> 3;
@@ -1175,6 +1208,7 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
             Qy0 = fmaf(st.y, hw, pyf); Qyi = st.y * ab.x; Qyj = fmaf(st.y, ab.y, 1.0f);
         }
         float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f, a5 = 0.0f;
+        float b0 = 0.0f, b1 = 0.0f;
 #pragma unroll 2
         for (int it = 0; it < T; ++it) {
             const int s = (it << 3) | u;
@@ -1192,6 +1226,16 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
             const float top = ta + fx * (tb - ta);
             const float bot = tc + fx * (te - tc);
             const float y = (top + fy * (bot - top)) - rc;
+            if (BLUR) {
+                const float* bp = L.btiles + base + (int)(ya * W1f + xa);
+                const float ba = bp[0], bb = bp[1], bc = bp[W1], be = bp[W1 + 1];
+                const float btop = ba + fx * (bb - ba);
+                const float bbot = bc + fx * (be - bc);
+                const float yb = (btop + fy * (bbot - btop)) - rbc;
+                const bool okb = ok && s < ns;  // the zero padding has weight 0 but not blur 0
+                b0 += okb ? S.sblur[s] : 0.0f;
+                b1 += okb ? yb : 0.0f;
+            }
             // masked by select, so a non-finite interpolant of a rejected sample adds nothing
             const float wm = ok ? rec.w : 0.0f;
             const float yv = ok ? y : 0.0f;
@@ -1207,6 +1251,7 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
         }
         a0 = red8(a0); a1 = red8(a1); a2 = red8(a2);
         a3 = red8(a3); a4 = red8(a4); a5 = red8(a5);
+        if (BLUR) { b0 = red8(b0); b1 = red8(b1); }
         // epilogue once per pass: hardware reciprocal / reciprocal square root (1 ulp), the
         // precision class of the reference's fast-math build
         float cst = 2.0f;
@@ -1220,12 +1265,27 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
                 cst = fminf(2.0f, fmaxf(0.0f, 1.0f - cov * __builtin_amdgcn_rsqf(vx * vy)));
             }
         }
+        if (BLUR && live) {
+            const float db = b0 - b1;
+            const float bw = expf(-0.007368f * db * db);
+            cst = fminf(2.0f, fmaxf(0.0f, cst + P.blur_lambda * (1.0f - bw)));
+        }
         if (live && u == 0) {
             S.cost[h * P.K + k] = cst;
             if (COUNT) ++wc.triples;
         }
     }
     wave_sync();
+}
+
+// run_tasks of the kernel variant (BLUR: the frame carries blur tiles).  The blur variant is
+// its own kernel instantiation so that its extra registers do not lower the occupancy of the
+// kernel without it.
+template <bool COUNT, typename TT, bool BLUR>
+__device__ __forceinline__ void run_batch(const Lds& L, const DevParams& P, const WaveLds& S,
+                                          int ntask, int ns, int px, int py, float rc, float rbc,
+                                          WorkCount& wc) {
+    run_tasks<COUNT, TT, BLUR>(L, P, S, ntask, ns, px, py, rc, rbc, wc);
 }
 
 // Weighted cost of hypothesis row h over the selected views (ComputeWeightedCost_Frame
@@ -1361,14 +1421,15 @@ __device__ __forceinline__ bool warm_valid(const DevParams& P, float4 h, float c
 // Random initialisation of one pixel (RandomInitializationForFrame :474-518); ACMM warm levels
 // add the seed, a jittered seed and a seed with a perturbed normal
 // (InitializeFrameFromRandomOrWarm, AdaptMIPMFrameACMM.cu:373-472).
-template <bool COUNT, typename TT>
+template <bool COUNT, typename TT, bool BLUR>
 __device__ void init_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
                            const WaveLds& S, int v, int c, int pix, WorkCount& wc) {
     const int W = P.W, K = P.K;
     const int x = pix % W, y = pix / W;
     const size_t idx = (size_t)v * P.WW + pix;
     const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
-    const int ns = build_samples<TT>(L, P, S, x, y, rc);
+    const float rbc = P.blur ? L.btiles[y * P.W1 + x] : 0.0f;
+    const int ns = build_samples<TT>(L, P, S, x, y, rc, rbc);
     DevStream st((uint32_t)idx, 0u, F.seed, F.uid);
     float4 h;
     h.w = u01_dev(st.raw(SLOT_INIT_DEPTH)) * (P.dmax - P.dmin) + P.dmin;
@@ -1396,7 +1457,7 @@ __device__ void init_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
     wave_sync();
     const int nt = add_tasks_rows(L, P, S, 0, nh, 0xFFFFFFFFu, allv, c, x, y, 0);
     wave_sync();
-    run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
+    run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
     // top-k mean and selection mask of each candidate; the lowest mean wins (first on ties)
     float mean;
     uint32_t mask;
@@ -1419,20 +1480,21 @@ __device__ void init_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
 }
 
 // Lowest-cost view record of the final plane (ComputeWeightedCost_Frame's disp, :303-306).
-template <bool COUNT, typename TT>
+template <bool COUNT, typename TT, bool BLUR>
 __device__ void final_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
                             const WaveLds& S, int v, int c, int pix, WorkCount& wc) {
     const int W = P.W;
     const int x = pix % W, y = pix / W;
     const size_t idx = (size_t)v * P.WW + pix;
     const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
-    const int ns = build_samples<TT>(L, P, S, x, y, rc);
+    const float rbc = P.blur ? L.btiles[y * P.W1 + x] : 0.0f;
+    const int ns = build_samples<TT>(L, P, S, x, y, rc, rbc);
     const float4 h = F.plane_in[idx];
     if (lane_id() == 0) S.hyp[0] = h;
     const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
     const int nt = add_tasks(L, P, S, 0, h, allv, c, x, y, 0);
     wave_sync();
-    run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
+    run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
     float mc = 2.0f;
     int am = -1;
     for (int k = 0; k < c; ++k) {
@@ -1454,7 +1516,7 @@ __device__ void final_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
 }
 
 // One red/black update of one pixel (CheckerboardUpdateFrame :520-947).
-template <bool COUNT, typename TT>
+template <bool COUNT, typename TT, bool BLUR>
 __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
                             const WaveLds& S, int v, int c, int x, int y, WorkCount& wc) {
     const int lane = lane_id();
@@ -1473,7 +1535,8 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     dsel[2] = dval[2] ? F.sel_in[idx - 1] : 0u;
     dsel[3] = dval[3] ? F.sel_in[idx + 1] : 0u;
     const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
-    const int ns = build_samples<TT>(L, P, S, x, y, rc);
+    const float rbc = P.blur ? L.btiles[y * P.W1 + x] : 0.0f;
+    const int ns = build_samples<TT>(L, P, S, x, y, rc, rbc);
 
     // batch 1: the current plane (row 0) and the propagation candidates (rows 1..)
     int pos[8], src[8];
@@ -1498,7 +1561,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
     const int nt = add_tasks_rows(L, P, S, 0, nh, 0xFFFFFFFFu, allv, c, x, y, 0);
     wave_sync();
-    run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
+    run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
 
     // sampling probabilities, lane k = view k
     const float thr = 0.8f * expf((float)(F.iter * F.iter) / (-90.0f));
@@ -1601,7 +1664,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         const uint32_t live = (uint32_t)__ballot(lane < jn && qspec >= 0);
         const int npt = add_tasks_rows(L, P, S, 0, jn, live, tsel, c, x, y, 0);
         wave_sync();
-        run_tasks<COUNT, TT>(L, P, S, npt, ns, x, y, rc, wc);
+        run_batch<COUNT, TT, BLUR>(L, P, S, npt, ns, x, y, rc, rbc, wc);
         const float wrow = wsum_lanes(P, S, jn, nsel);
         // lane j's rounded point under the replayed best (recomputed after each acceptance)
         int qcur = qspec;
@@ -1620,7 +1683,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
                 if (lane == 0) S.hyp[kSpec] = prop;
                 const int n1 = add_tasks(L, P, S, kSpec, prop, tsel, c, x, y, 0);
                 wave_sync();
-                run_tasks<COUNT, TT>(L, P, S, n1, ns, x, y, rc, wc);
+                run_batch<COUNT, TT, BLUR>(L, P, S, n1, ns, x, y, rc, rbc, wc);
                 cc = wsum(P, S, kSpec, nsel);
             }
             if (cc < bcost) {
@@ -1662,7 +1725,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         wave_sync();
         const int nrt = add_tasks_rows(L, P, S, 0, 5, 0x1Fu, tsel, c, x, y, 0);
         wave_sync();
-        run_tasks<COUNT, TT>(L, P, S, nrt, ns, x, y, rc, wc);
+        run_batch<COUNT, TT, BLUR>(L, P, S, nrt, ns, x, y, rc, rbc, wc);
         const float wrow = wsum_lanes(P, S, 5, nsel);
         for (int i = 0; i < 5; ++i) {
             const float cc = lane_value(wrow, i);
@@ -1687,7 +1750,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
             nst = add_tasks(L, P, S, 1, seed, allv, c, x, y, nst);
         }
         wave_sync();
-        run_tasks<COUNT, TT>(L, P, S, nst, ns, x, y, rc, wc);
+        run_batch<COUNT, TT, BLUR>(L, P, S, nst, ns, x, y, rc, rbc, wc);
         float fmean;
         uint32_t fmask;
         int fb;
@@ -1720,7 +1783,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     wave_sync();
 }
 
-template <bool COUNT, typename TT>
+template <bool COUNT, typename TT, bool BLUR>
 __global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const DevParams& P = a.P;
@@ -1741,7 +1804,8 @@ __global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
         if (stage == STAGE_IDLE) continue;  // uniform across the workgroup
         const int v = gm - F.mi_begin;
         __syncthreads();
-        const int c = stage_mi<TT>(L, P, v, F.tiles, F.centers, F.nbr, F.cnt);
+        const int c = stage_mi<TT>(L, P, v, F.tiles, F.centers, F.nbr, F.cnt,
+                                    P.blur ? F.tiles + (size_t)F.V * WW : nullptr);
         if (stage == STAGE_PHASE)
             for (int i = t; i < WW; i += NT) L.costp[i] = F.cost_in[(size_t)v * WW + i];
         __syncthreads();
@@ -1762,13 +1826,13 @@ __global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
                 for (int aidx = wave; aidx < nact; aidx += nwaves) {
                     int x, y;
                     active_pixel(W, aidx, phase, x, y);
-                    phase_pixel<COUNT, TT>(L, P, F, S, v, c, x, y, wc);
+                    phase_pixel<COUNT, TT, BLUR>(L, P, F, S, v, c, x, y, wc);
                 }
             }
         } else if (stage == STAGE_INIT) {
-            for (int pix = wave; pix < WW; pix += nwaves) init_pixel<COUNT, TT>(L, P, F, S, v, c, pix, wc);
+            for (int pix = wave; pix < WW; pix += nwaves) init_pixel<COUNT, TT, BLUR>(L, P, F, S, v, c, pix, wc);
         } else {
-            for (int pix = wave; pix < WW; pix += nwaves) final_pixel<COUNT, TT>(L, P, F, S, v, c, pix, wc);
+            for (int pix = wave; pix < WW; pix += nwaves) final_pixel<COUNT, TT, BLUR>(L, P, F, S, v, c, pix, wc);
         }
     }
     flush_counts<COUNT>(a.counters, wc);
@@ -1845,7 +1909,7 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs a) {
 
 // Wave-per-hypothesis form of k_eval (the default): one wavefront scores one record over all
 // of its MI's views with the task engine of wv::run_tasks.
-template <bool COUNT, typename TT>
+template <bool COUNT, typename TT, bool BLUR>
 __global__ void __launch_bounds__(256) k_eval_w(EvalArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const DevParams& P = a.P;
@@ -1861,7 +1925,8 @@ __global__ void __launch_bounds__(256) k_eval_w(EvalArgs a) {
         const int h0 = a.seg[mi], h1 = a.seg[mi + 1];
         if (h0 >= h1) continue;  // uniform across the workgroup
         __syncthreads();
-        const int c = stage_mi<TT>(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
+        const int c = stage_mi<TT>(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt,
+                                    P.blur ? a.tiles + (size_t)P.V * WW : nullptr);
         __syncthreads();
         for (int hi = h0 + wave; hi < h1; hi += nwaves) {
             const int* rec = a.hyp + (size_t)hi * 6;
@@ -1874,12 +1939,13 @@ __global__ void __launch_bounds__(256) k_eval_w(EvalArgs a) {
                                          __int_as_float(rec[4]), __int_as_float(rec[5]));
             const int x = pix % W, y = pix / W;
             const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
-            const int ns = wv::build_samples<TT>(L, P, S, x, y, rc);
+            const float rbc = P.blur ? L.btiles[y * P.W1 + x] : 0.0f;
+            const int ns = wv::build_samples<TT>(L, P, S, x, y, rc, rbc);
             if (lane == 0) S.hyp[0] = h;
             const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
             const int nt = wv::add_tasks(L, P, S, 0, h, allv, c, x, y, 0);
             wv::wave_sync();
-            wv::run_tasks<COUNT, TT>(L, P, S, nt, ns, x, y, rc, wc);
+            wv::run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
             for (int k = lane; k < K; k += 64) a.out[(size_t)hi * K + k] = S.cost[k];
             wv::wave_sync();
         }
@@ -2304,7 +2370,13 @@ static bool wave_solver() {
     }
     return g_solver == 1;
 }
-static const int kWaveThreads = 256;
+// Threads per workgroup of the wave engine: 4 wavefronts, or fewer when the LDS of 4 would pass
+// 64 KiB (the blur term's second set of tiles with many views and a wide window).
+static int wave_threads(const DevParams& P) {
+    for (int nw = 4; nw > 1; nw /= 2)
+        if (wv::lds_total(P, nw) <= 65536) return 64 * nw;
+    return 64;
+}
 
 // Workgroups per CU that fit (registers, LDS) for a kernel at this block size and LDS.
 template <typename KERN>
@@ -2313,6 +2385,17 @@ static int fit_per_cu(KERN kern, int threads, size_t lds) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, threads, lds) != hipSuccess || n < 1)
         return 1;
     return n;
+}
+
+// Wave-engine kernels of the counting / FP16-storage / blur variants (FP16 storage has no blur
+// variant: check_pm_params refuses the pair).
+using StageKernel = void (*)(StageArgs);
+static StageKernel stage_w_kernel(bool count, bool fp16, bool blur) {
+    if (count)
+        return fp16 ? wv::k_stage_w<true, __half, false>
+                    : (blur ? wv::k_stage_w<true, float, true> : wv::k_stage_w<true, float, false>);
+    return fp16 ? wv::k_stage_w<false, __half, false>
+                : (blur ? wv::k_stage_w<false, float, true> : wv::k_stage_w<false, float, false>);
 }
 
 // Launch k_stage over the frames of `a` (mi_begin/total filled here).
@@ -2326,30 +2409,22 @@ static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_
     a.total = total;
     if (total <= 0) return LFPM_OK;
     if (wave_solver()) {
-        const int NT = kWaveThreads;
+        const int NT = wave_threads(a.P);
         const size_t lds = wv::lds_total(a.P, NT / 64);
         if (lds > 65536)
             return set_error(LFPM_E_UNSUPPORTED, "k_stage: %zu B of LDS per workgroup (crop %d, "
                              "%d views, radius %d) exceeds 64 KiB", lds, a.P.W, a.P.K, a.P.r);
-        const bool fp16 = a.P.tile_bytes == 2;
-        int fit;
-        if (g_counters) fit = fp16 ? fit_per_cu(wv::k_stage_w<true, __half>, NT, lds)
-                                   : fit_per_cu(wv::k_stage_w<true, float>, NT, lds);
-        else fit = fp16 ? fit_per_cu(wv::k_stage_w<false, __half>, NT, lds)
-                        : fit_per_cu(wv::k_stage_w<false, float>, NT, lds);
+        const StageKernel kern = stage_w_kernel(g_counters != nullptr, a.P.tile_bytes == 2,
+                                                a.P.blur != 0);
+        const int fit = fit_per_cu(kern, NT, lds);
         const int grid = persistent_grid_n(total, std::max(1, std::min(per_cu, fit)));
         a.P.slots_per_xcd = grid / 8;
         a.scratch = nullptr;
         a.counters = g_counters;
-        if (g_counters) {
-            if (fp16) hipLaunchKernelGGL((wv::k_stage_w<true, __half>), dim3(grid), dim3(NT), lds, s, a);
-            else hipLaunchKernelGGL((wv::k_stage_w<true, float>), dim3(grid), dim3(NT), lds, s, a);
-        } else {
-            if (fp16) hipLaunchKernelGGL((wv::k_stage_w<false, __half>), dim3(grid), dim3(NT), lds, s, a);
-            else hipLaunchKernelGGL((wv::k_stage_w<false, float>), dim3(grid), dim3(NT), lds, s, a);
-        }
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
         return launch_check("k_stage");
     }
+    if (a.P.blur) return set_error(LFPM_E_UNSUPPORTED, "blur term: needs the wave solver");
     const int grid = persistent_grid_n(total, per_cu);
     a.P.slots_per_xcd = grid / 8;
     const int NT = phase_threads(a.P);
@@ -2813,6 +2888,8 @@ int lfpm_patchmatch_frame_acmm(const float* tiles_dev, const float* centers_dev,
         return set_error(LFPM_E_UNSUPPORTED, "acmm: level images are not 8-bit; storage must be 0");
     if (!wave_solver())
         return set_error(LFPM_E_UNSUPPORTED, "acmm: needs the wave solver engine");
+    if (p->blur_lambda > 0.0f)
+        return set_error(LFPM_E_UNSUPPORTED, "acmm: the blur term is not provided for level images");
     AcmmLevel lv[2];
     const int L = acmm_levels(p->crop, lv);
     const AcmmWs w = acmm_ws(num_mis, p->crop, lv[0].width, L == 2);
@@ -2885,7 +2962,7 @@ int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,
     a.tiles = tiles_dev; a.centers = (const float2*)centers_dev; a.nbr = neighbor_ids_dev;
     a.cnt = neighbor_counts_dev; a.hyp = (const int*)hyp_dev; a.seg = seg; a.n = n;
     a.out = cost_out_dev; a.P = P; a.counters = g_counters;
-    const int NT = 256;
+    const int NT = wave_solver() ? wave_threads(P) : 256;
     const bool fp16 = P.tile_bytes == 2;
     if (wave_solver()) {
         const size_t lds = wv::lds_total(P, NT / 64);
@@ -2894,14 +2971,20 @@ int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,
             return set_error(LFPM_E_UNSUPPORTED, "eval: %zu B of LDS per workgroup exceeds 64 KiB",
                              lds);
         }
-        if (g_counters) {
-            if (fp16) hipLaunchKernelGGL((k_eval_w<true, __half>), dim3(grid), dim3(NT), lds, s, a);
-            else hipLaunchKernelGGL((k_eval_w<true, float>), dim3(grid), dim3(NT), lds, s, a);
-        } else {
-            if (fp16) hipLaunchKernelGGL((k_eval_w<false, __half>), dim3(grid), dim3(NT), lds, s, a);
-            else hipLaunchKernelGGL((k_eval_w<false, float>), dim3(grid), dim3(NT), lds, s, a);
-        }
+        using EvalKernel = void (*)(EvalArgs);
+        const bool blur = P.blur != 0;
+        const EvalKernel kern =
+            g_counters ? (fp16 ? k_eval_w<true, __half, false>
+                               : (blur ? k_eval_w<true, float, true> : k_eval_w<true, float, false>))
+                       : (fp16 ? k_eval_w<false, __half, false>
+                               : (blur ? k_eval_w<false, float, true>
+                                       : k_eval_w<false, float, false>));
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
     } else {
+        if (P.blur) {
+            (void)hipFreeAsync(seg, s);
+            return set_error(LFPM_E_UNSUPPORTED, "blur term: needs the wave solver");
+        }
         const size_t lds = lds_bytes(P, NT, false);
         if (g_counters) {
             if (fp16) hipLaunchKernelGGL((k_eval<true, __half>), dim3(grid), dim3(NT), lds, s, a);

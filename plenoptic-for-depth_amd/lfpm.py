@@ -45,7 +45,8 @@ class PmParams(C.Structure):
                 ("sigma_color", C.c_float), ("min_var", C.c_float), ("top_k", C.c_int),
                 ("depth_min", C.c_float), ("depth_max", C.c_float), ("iterations", C.c_int),
                 ("max_neighbors", C.c_int), ("seed", C.c_uint32), ("frame_uid", C.c_uint32),
-                ("storage", C.c_int), ("cos_max_slant", C.c_float)]
+                ("storage", C.c_int), ("cos_max_slant", C.c_float),
+                ("blur_lambda", C.c_float)]
 
 
 class FilterParams(C.Structure):
@@ -215,6 +216,14 @@ class Workload:
         self.raw()
         return self._gt
 
+    def blur_map(self):
+        """Synthetic raw-size uint8 blur map: defocus grows with the distance of the scene's
+        virtual depth from v = 4.5 (0 outside the MI discs), so MIs that see the same point
+        carry similar blur."""
+        v = self.gt_virtual_depth()
+        b = np.where(v > 0, np.clip(np.abs(v - 4.5) / 6.0, 0.0, 1.0) * 255.0, 0.0)
+        return np.floor(b + 0.5).astype(np.uint8)
+
 
 def workload_c1(**kw):
     """1024x800 raw, pitch 23, 6 ring-1 neighbours, 11x11 window (s_s=3, s_c=0.1), 3 iterations,
@@ -241,9 +250,11 @@ def workload_c2(**kw):
 # Device entry points (torch tensors on the current device)
 # --------------------------------------------------------------------------------------
 class DeviceFrame:
-    """Device-resident inputs of one frame: raw, layout tables and tiles."""
+    """Device-resident inputs of one frame: raw, layout tables and tiles.  With blur_raw (a
+    raw-size uint8 blur map) the tiles carry the MIs' blur maps after the intensities, the
+    layout the blur term of the cost vector reads (lfpm_pm_params.blur_lambda > 0)."""
 
-    def __init__(self, wl, raw=None, device="cuda"):
+    def __init__(self, wl, raw=None, device="cuda", blur_raw=None):
         import torch
         self.wl = wl
         L = wl.L
@@ -255,7 +266,10 @@ class DeviceFrame:
         self.cnt = torch.from_numpy(L["cnt"]).to(device)
         self.l2m = torch.from_numpy(L["lattice_to_mi"]).to(device)
         V, W = L["V"], L["crop"]
-        self.tiles = torch.empty((V, W, W), dtype=torch.float32, device=device)
+        self.blur_raw = None if blur_raw is None else \
+            torch.from_numpy(np.ascontiguousarray(blur_raw, np.uint8)).to(device)
+        nt = V if blur_raw is None else 2 * V
+        self.tiles = torch.empty((nt, W, W), dtype=torch.float32, device=device)
         self.gray = torch.empty((V, W, W), dtype=torch.float32, device=device)
         self.slice()
 
@@ -266,6 +280,10 @@ class DeviceFrame:
                                       L["V"], L["crop"], _dp(self.tiles), _stream(stream)))
         _check(lib().lfpm_slice_gray(_dp(self.raw), w, h, C.c_size_t(w), _dp(self.origins),
                                      L["V"], L["crop"], _dp(self.gray), _stream(stream)))
+        if self.blur_raw is not None:
+            _check(lib().lfpm_slice_tiles(_dp(self.blur_raw), w, h, C.c_size_t(w),
+                                          _dp(self.origins), L["V"], L["crop"],
+                                          _dp(self.tiles[L["V"]:]), _stream(stream)))
 
 
 class SolveBuffers:

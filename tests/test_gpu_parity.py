@@ -260,3 +260,81 @@ def test_solve_acmm_vs_fp64(lfpm, orc, torch_cuda, pitch):
     _say("acmm", st)
     stats_dump("acmm_%g" % pitch, st)
     assert frac >= 0.995
+
+
+@pytest.mark.parametrize("cfg", ["c1", "c2params"])
+def test_eval_costs_blur_vs_fp64(lfpm, orc, torch_cuda, cfg):
+    """Blur term of the cost vector (lambda 2, AdaptMIPMFrame.cu:211-218) with per-MI blur maps
+    from the synthetic defocus map: fixed-plane costs vs the FP64 oracle, same bars as the FP32
+    NCC (>= 99.5 % within 1e-5, >= 99.9 % within 1e-4 relative)."""
+    _say("case eval_costs_blur", cfg)
+    wl = lfpm.workload_c1() if cfg == "c1" else lfpm.workload_c2(raw_w=1024, raw_h=800)
+    wl.pm.blur_lambda = 2.0
+    blur = wl.blur_map()
+    df = lfpm.DeviceFrame(wl, blur_raw=blur)
+    hyp = cone_hypotheses(lfpm, wl, 40000, seed=11)
+    out = lfpm.eval_costs(df, torch_cuda.from_numpy(hyp).cuda()).cpu().numpy()
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    btiles = orc.slice_u8(blur, wl.L["origins"], wl.crop)
+    ref = orc.eval_costs(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"],
+                         pm_to_oracle(orc, wl.pm), hyp, blur_u8=btiles)
+    pm0 = pm_to_oracle(orc, wl.pm)
+    pm0.blur_lambda = 0.0
+    noblur = orc.eval_costs(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm0, hyp)
+    both = (out < 2) & (ref < 2)
+    flip = ((out < 2) != (ref < 2)).mean()
+    e = rel_err(out[both].astype(np.float64), ref[both])
+    st = dict(cfg=cfg, n=int(both.sum()), flip=float(flip), max=float(e.max()),
+              p999=float(np.quantile(e, 0.999)), median=float(np.median(e)),
+              frac_le_1e5=float((e <= 1e-5).mean()), frac_le_1e4=float((e <= 1e-4).mean()),
+              term_active=float((ref[both] > noblur[both] + 1e-6).mean()))
+    _say("eval_costs_blur", st)
+    stats_dump("eval_costs_blur_" + cfg, st)
+    assert st["term_active"] > 0.5
+    assert flip < 1e-3
+    assert st["frac_le_1e5"] >= 0.995
+    assert st["frac_le_1e4"] >= 0.999
+
+
+def test_solve_blur_vs_fp64(lfpm, orc, torch_cuda):
+    """Full solve with the blur term (lambda 2) on a 400x320 C1-parameter frame: disparities
+    within 0.05 px of the FP64 oracle on >= 99.5 % of valid pixels."""
+    wl = lfpm.workload_c1(raw_w=400, raw_h=320)
+    wl.pm.blur_lambda = 2.0
+    blur = wl.blur_map()
+    _say("case solve_blur V=%d" % wl.V)
+    df = lfpm.DeviceFrame(wl, blur_raw=blur)
+    g, _ = _solve_gpu(lfpm, df, torch_cuda)
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    btiles = orc.slice_u8(blur, wl.L["origins"], wl.crop)
+    o = orc.solve(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm_to_oracle(orc, wl.pm),
+                  blur_u8=btiles)
+    valid = o["cost"] <= 1.8
+    err = np.abs(g["plane"][:, 3].astype(np.float64) - o["plane"][:, 3])
+    st = dict(V=wl.V, frac_le_005=float((err[valid] <= 0.05).mean()),
+              valid=float(valid.mean()), sel_equal=float((g["sel"] == o["sel"]).mean()))
+    _say("solve_blur", st)
+    stats_dump("solve_blur", st)
+    assert st["frac_le_005"] >= 0.995
+
+
+def test_blur_rejected_where_not_provided(lfpm, torch_cuda):
+    """The blur term needs blur tiles: the host context, the ACMM variant and FP16 storage
+    refuse it with LFPM_E_UNSUPPORTED instead of solving without it."""
+    import copy
+    wl = lfpm.workload_c1(raw_w=320, raw_h=256)
+    pm = copy.copy(wl.pm)
+    pm.blur_lambda = 2.0
+    df = lfpm.DeviceFrame(wl, blur_raw=wl.blur_map())
+    with pytest.raises(lfpm.LfpmError):
+        lfpm.patchmatch_frame_acmm(df, pm=pm)
+    pm16 = copy.copy(pm)
+    pm16.storage = 1
+    with pytest.raises(lfpm.LfpmError):
+        lfpm.patchmatch_frame(df, pm=pm16)
+    ctx = lfpm.Context()
+    try:
+        with pytest.raises(lfpm.LfpmError):
+            ctx.solve_frame(wl, pm=pm)
+    finally:
+        ctx.close()
