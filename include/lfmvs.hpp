@@ -8,6 +8,7 @@
 //   DepthSolver                 DepthSolver.h:62-200 (the members on the matching path)
 //   AdaptMIPMFrame              MIStereo/AdaptMIPMFrame.h:16-29
 //   AdaptMIPM                   MIStereo/AdaptMIPM.h:47-107
+//   AdaptMIPMPFPGR              MIStereo/AdaptMIPMPFPGR.h:14-46
 //   DisparityAndNormal          MVStereo/LFDepthInfo.h:273-320
 //   MLA_Problem                 MVStereo/LFDepthInfo.h:225-268 (fields used by the matcher)
 //   SelectNeighbors             MINeighbor/SelectNeighbors.h (FixedPosition selection)
@@ -34,6 +35,7 @@ namespace LFMVS {
 
 struct Float4 { float x, y, z, w; };
 struct Point2f { float x, y; };
+struct Int3 { int x, y, z; };
 
 // 8-bit single-channel image (stands in for cv::Mat CV_8UC1).
 struct GrayImage {
@@ -233,9 +235,13 @@ public:
 class AdaptMIPM {
 public:
     explicit AdaptMIPM(LightFieldParams& params);
-    ~AdaptMIPM();
+    virtual ~AdaptMIPM();
 
     void RunPatchMatchCUDAForMI();
+    // The reference's planar-prior and soft-proxy patch-repair variants (AdaptMIPM.h:54-57) run
+    // other per-MI solvers; here both run the solve of RunPatchMatchCUDAForMI.
+    void RunPatchMatchCUDAForMI_plane() { RunPatchMatchCUDAForMI(); }
+    void RunPatchMatchCUDAForMI_SoftProxy_PatchRepair() { RunPatchMatchCUDAForMI(); }
     void SetTileKey(int tile_x, int tile_y);
     void Initialize(QuadTreeTileInfoMap& MLA_info_map, MLA_Problem& problem,
                     QuadTreeProblemMap& problem_map, std::vector<Float4>& planeVec,
@@ -273,6 +279,56 @@ private:
     std::vector<float> cost;
     std::vector<unsigned int> sel;
     lfpm_ctx* m_ctx = nullptr;
+};
+
+// Proxy planes of one pixel for the PFPGR repair stage (Proxy_DisPlane, Common/CommonUtil.h:15-25):
+// 8 slots, unset slots hold -9999.
+struct Proxy_DisPlane {
+    Float4 plane[8];
+    Proxy_DisPlane() {
+        for (Float4& p : plane) p = Float4{-9999.0f, -9999.0f, -9999.0f, -9999.0f};
+    }
+};
+
+// First-stage results of one MI handed to the second stage (StereoResultInfo,
+// MIStereo/AdaptMIPM.h:18-45; host arrays only).
+struct StereoResultInfo {
+    StereoResultInfo(const int mi_width, const int mi_height)
+        : mi_image_width(mi_width), mi_image_height(mi_height),
+          plane_hypotheses_host((size_t)mi_width * mi_height),
+          costs_host((size_t)mi_width * mi_height),
+          neighbor_patchFill_host((size_t)mi_width * mi_height, Int3{-1, -1, -1}),
+          disp_baseline_host((size_t)mi_width * mi_height),
+          selected_views_host((size_t)mi_width * mi_height) {}
+    int mi_image_width, mi_image_height;
+    std::vector<Float4> plane_hypotheses_host;
+    std::vector<float> costs_host;
+    std::vector<Int3> neighbor_patchFill_host;
+    std::vector<Float4> disp_baseline_host;
+    std::vector<unsigned int> selected_views_host;
+};
+
+// Two-stage per-MI matcher (AdaptMIPMPFPGR, MIStereo/AdaptMIPMPFPGR.h:14-46) with the reference's
+// method names.  The edge-aware propagation and the boundary/broken-pixel proxy search of the
+// reference's collect and repair kernels are not part of this library: the collect stage runs
+// the per-MI solve and reports no proxy for any pixel (GetNeighborPGR = (-1, -1, -1)), and the
+// repair stage, having no proxies to apply, runs the same solve again (same seed, same result).
+class AdaptMIPMPFPGR : public AdaptMIPM {
+public:
+    explicit AdaptMIPMPFPGR(LightFieldParams& params) : AdaptMIPM(params) {}
+
+    void RunPatchMatchCUDAForMI_PFPGR_Collect();
+    void RunPatchMatchCUDAForMI_PFPGR_Repair();
+    void Initialize_SecondStage(QuadTreeTileInfoMap& MLA_info_map, MLA_Problem& problem,
+                                QuadTreeProblemMap& problem_map, std::vector<Float4>& planeVec,
+                                std::vector<float>& costVec, Proxy_DisPlane* proxy_Plane,
+                                StereoResultInfo* pStereoResult);
+    Int3 GetNeighborPGR(const int index) const { return neighbor_PGR.at(index); }
+
+private:
+    std::vector<Int3> neighbor_PGR;
+    Proxy_DisPlane* proxy_plane = nullptr;
+    StereoResultInfo* stereo_result = nullptr;
 };
 
 // Frame-level driver (DepthSolver, DepthSolver.h:62-200): frames are supplied in memory (the

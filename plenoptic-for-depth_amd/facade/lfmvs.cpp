@@ -313,6 +313,29 @@ void AdaptMIPM::RunPatchMatchCUDAForMI() {
     }
 }
 
+void AdaptMIPMPFPGR::RunPatchMatchCUDAForMI_PFPGR_Collect() {
+    RunPatchMatchCUDAForMI();
+    const int px = (int)(GetParamsGPU().crop * GetParamsGPU().crop);
+    neighbor_PGR.assign(px, Int3{-1, -1, -1});
+}
+
+void AdaptMIPMPFPGR::Initialize_SecondStage(QuadTreeTileInfoMap& MLA_info_map,
+                                            MLA_Problem& problem, QuadTreeProblemMap& problem_map,
+                                            std::vector<Float4>& planeVec,
+                                            std::vector<float>& costVec,
+                                            Proxy_DisPlane* proxy_Plane,
+                                            StereoResultInfo* pStereoResult) {
+    Initialize(MLA_info_map, problem, problem_map, planeVec, costVec);
+    proxy_plane = proxy_Plane;
+    stereo_result = pStereoResult;
+}
+
+void AdaptMIPMPFPGR::RunPatchMatchCUDAForMI_PFPGR_Repair() {
+    // no proxies were collected, so the repair is the solve itself; the second-stage inputs
+    // (proxy planes, first-stage results) are kept for the caller but not read
+    RunPatchMatchCUDAForMI();
+}
+
 Float4 AdaptMIPM::GetPlaneHypothesis(const int index) { return plane.at(index); }
 Float4* AdaptMIPM::GetPlaneHypothesisVector() { return plane.data(); }
 float AdaptMIPM::GetCost(const int index) { return cost.at(index); }

@@ -91,6 +91,23 @@ int main(int argc, char** argv) {
             put(out + "/mi_cnt.bin", mi.NeighborCounts().data(), mi.NeighborCounts().size() * 4);
             const int meta[3] = {V, mi.GetParamsGPU().max_neighbors, px};
             put(out + "/mi_meta.bin", meta, sizeof meta);
+            // AdaptMIPMPFPGR on the same MI: collect, then the second stage (repair)
+            AdaptMIPMPFPGR pf(lf);
+            pf.Initialize(s.GetMLAInfoMap(), it->second, prob, planeVec, costVec);
+            pf.RunPatchMatchCUDAForMI_PFPGR_Collect();
+            put(out + "/pf_collect_plane.bin", pf.GetPlaneHypothesisVector(), px * 16);
+            std::vector<int> pgr;
+            for (int i = 0; i < px; ++i) {
+                const Int3 g = pf.GetNeighborPGR(i);
+                pgr.insert(pgr.end(), {g.x, g.y, g.z});
+            }
+            put(out + "/pf_pgr.bin", pgr.data(), pgr.size() * 4);
+            std::vector<Proxy_DisPlane> proxies(px);
+            StereoResultInfo first(lf.mi_width_for_match, lf.mi_width_for_match);
+            pf.Initialize_SecondStage(s.GetMLAInfoMap(), it->second, prob, planeVec, costVec,
+                                      proxies.data(), &first);
+            pf.RunPatchMatchCUDAForMI_PFPGR_Repair();
+            put(out + "/pf_repair_plane.bin", pf.GetPlaneHypothesisVector(), px * 16);
             std::printf("facade_driver: AdaptMIPM views=%d\n", V);
             return 0;
         }

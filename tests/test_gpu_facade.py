@@ -103,6 +103,11 @@ def test_facade_adaptmipm_single_mi(lfpm, orc, tmp_path):
     finally:
         ctx.close()
     assert np.array_equal(plane.view(np.uint32), ref["plane"][:px].view(np.uint32))
+    # AdaptMIPMPFPGR: both stages give the per-MI solve; no proxy is collected
+    for f in ("pf_collect_plane.bin", "pf_repair_plane.bin"):
+        pf = np.fromfile(str(out / f), np.float32).reshape(-1, 4)
+        assert np.array_equal(pf.view(np.uint32), plane.view(np.uint32)), f
+    assert np.all(np.fromfile(str(out / "pf_pgr.bin"), np.int32) == -1)
     o = orc.solve(tiles.reshape(V, wl.crop, wl.crop), centers, nbr.reshape(V, K), cnt,
                   orc.PmParams(*[getattr(pm, f[0]) for f in orc.PmParams._fields_]))
     err = np.abs(plane[:, 3].astype(np.float64) - o["plane"][:px, 3])
