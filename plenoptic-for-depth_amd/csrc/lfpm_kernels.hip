@@ -1558,8 +1558,29 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     }
     if (my_src >= 0) S.hyp[my_row] = F.plane_in[base + my_src];
     wave_sync();
+    // A row whose plane is bit-equal to an earlier row's (propagation copies planes unchanged,
+    // so once the field settles neighbours often carry the same plane) has that row's costs at
+    // this pixel: it is scored once and the candidate slots read the earlier row.
+    int canon = lane;
+    if (lane < nh) {
+        const float4 a = S.hyp[lane];
+        for (int g = 0; g < lane; ++g) {
+            const float4 b = S.hyp[g];
+            if (__float_as_uint(a.x) == __float_as_uint(b.x) &&
+                __float_as_uint(a.y) == __float_as_uint(b.y) &&
+                __float_as_uint(a.z) == __float_as_uint(b.z) &&
+                __float_as_uint(a.w) == __float_as_uint(b.w)) {
+                canon = g;
+                break;
+            }
+        }
+    }
+    const uint32_t own = (uint32_t)__ballot(lane < nh && canon == lane);
+#pragma unroll
+    for (int cd = 0; cd < 8; ++cd)
+        if (row[cd] >= 0) row[cd] = lane_int(canon, row[cd]);
     const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
-    const int nt = add_tasks_rows(L, P, S, 0, nh, 0xFFFFFFFFu, allv, c, x, y, 0);
+    const int nt = add_tasks_rows(L, P, S, 0, nh, own, allv, c, x, y, 0);
     wave_sync();
     run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
 
