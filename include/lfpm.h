@@ -168,6 +168,16 @@ int lfpm_slice_tiles(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pi
 int lfpm_slice_gray(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pitch,
                     const int* origins_dev, int num_mis, int crop, float* gray_dev, void* stream);
 
+/* Blur tiles for the blur term (lfpm_pm_params.blur_lambda): the Laplacian blur score of the
+ * gray frame (ImageQualityEstimate::QuantizeBlurByLaplacian, Util/ImageQualityEstimate.cpp:
+ * 211-230 — cv::Laplacian with ksize 1, i.e. the 4-neighbour stencil, FP32, border reflected
+ * without repeating the edge pixel), cropped per MI (DepthSolver::SliceAndIndicatlizeForMI,
+ * DepthSolver.cpp:2709-2745) and divided by 255 as ComputeBilateralNCC_Frame reads it.
+ * Write it to tiles_dev + V*crop*crop to pass both halves to the solver. */
+int lfpm_slice_blur_tiles(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pitch,
+                          const int* origins_dev, int num_mis, int crop, float* blur_tiles_dev,
+                          void* stream);
+
 /* Device workspace (bytes) needed by lfpm_patchmatch_frame / _init / _phase. */
 size_t lfpm_patchmatch_workspace_size(int num_mis, const lfpm_pm_params* p);
 

@@ -783,26 +783,27 @@ static void init_pixel_acmm(const Frame<T>& f, State<T>& s, int v, int x, int y,
 }
 
 template <typename T>
-static void build_frame(Frame<T>& f, const uint8_t* tiles_u8, const float* centers, const int* nbr,
-                        const int* cnt, int V, const PmParams& p) {
+static void build_frame(Frame<T>& f, const uint8_t* tiles_u8, const double* blur,
+                        const float* centers, const int* nbr, const int* cnt, int V,
+                        const PmParams& p) {
     f.V = V; f.W = p.crop; f.K = p.max_neighbors;
     f.centers = centers; f.nbr = nbr; f.cnt = cnt; f.p = p;
     f.inv_base = (T)1 / std::max((T)p.base, (T)1e-6);
     const size_t n = (size_t)V * p.crop * p.crop;
     f.img.resize(n);
     for (size_t i = 0; i < n; ++i) f.img[i] = (T)tiles_u8[i] / (T)255;
-    if (p.blur_lambda > 0.0f) {  // blur maps follow the intensity tiles
+    if (p.blur_lambda > 0.0f && blur) {  // blur values as the NCC reads them (score / 255)
         f.blur.resize(n);
-        for (size_t i = 0; i < n; ++i) f.blur[i] = (T)tiles_u8[n + i] / (T)255;
+        for (size_t i = 0; i < n; ++i) f.blur[i] = (T)blur[i];
     }
 }
 
 template <typename T>
-static void solve(const uint8_t* tiles_u8, const float* centers, const int* nbr, const int* cnt,
-                  int V, const PmParams& p, T* plane_out, T* cost_out, T* disp_out,
-                  uint32_t* sel_out, int nthreads) {
+static void solve(const uint8_t* tiles_u8, const double* blur, const float* centers,
+                  const int* nbr, const int* cnt, int V, const PmParams& p, T* plane_out,
+                  T* cost_out, T* disp_out, uint32_t* sel_out, int nthreads) {
     Frame<T> f;
-    build_frame(f, tiles_u8, centers, nbr, cnt, V, p);
+    build_frame(f, tiles_u8, blur, centers, nbr, cnt, V, p);
     const size_t n = (size_t)V * p.crop * p.crop;
     std::vector<P4<T>> pa(n), pb(n), da(n), db(n);
     std::vector<T> ca(n), cb(n);
@@ -1032,11 +1033,11 @@ static void solve_acmm(const uint8_t* tiles_u8, const float* centers, const int*
 }
 
 template <typename T>
-static void eval_costs(const uint8_t* tiles_u8, const float* centers, const int* nbr,
-                       const int* cnt, int V, const PmParams& p, const int32_t* hyp, int n,
-                       T* out, int nthreads) {
+static void eval_costs(const uint8_t* tiles_u8, const double* blur, const float* centers,
+                       const int* nbr, const int* cnt, int V, const PmParams& p,
+                       const int32_t* hyp, int n, T* out, int nthreads) {
     Frame<T> f;
-    build_frame(f, tiles_u8, centers, nbr, cnt, V, p);
+    build_frame(f, tiles_u8, blur, centers, nbr, cnt, V, p);
     const int K = p.max_neighbors, W = p.crop;
     if (nthreads <= 0) nthreads = 1;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
@@ -1688,16 +1689,16 @@ void orc_slice_u8(const uint8_t* raw, int raw_w, int raw_h, const int* origins, 
                     raw[(size_t)(origins[2 * v + 1] + y) * raw_w + origins[2 * v] + x];
 }
 
-void orc_solve_f64(const uint8_t* tiles, const float* centers, const int* nbr, const int* cnt,
-                   int V, const orc::PmParams* p, double* plane, double* cost, double* disp,
-                   uint32_t* sel, int nthreads) {
-    orc::solve<double>(tiles, centers, nbr, cnt, V, *p, plane, cost, disp, sel, nthreads);
+void orc_solve_f64(const uint8_t* tiles, const double* blur, const float* centers,
+                   const int* nbr, const int* cnt, int V, const orc::PmParams* p, double* plane,
+                   double* cost, double* disp, uint32_t* sel, int nthreads) {
+    orc::solve<double>(tiles, blur, centers, nbr, cnt, V, *p, plane, cost, disp, sel, nthreads);
 }
 
-void orc_solve_f32(const uint8_t* tiles, const float* centers, const int* nbr, const int* cnt,
-                   int V, const orc::PmParams* p, float* plane, float* cost, float* disp,
-                   uint32_t* sel, int nthreads) {
-    orc::solve<float>(tiles, centers, nbr, cnt, V, *p, plane, cost, disp, sel, nthreads);
+void orc_solve_f32(const uint8_t* tiles, const double* blur, const float* centers,
+                   const int* nbr, const int* cnt, int V, const orc::PmParams* p, float* plane,
+                   float* cost, float* disp, uint32_t* sel, int nthreads) {
+    orc::solve<float>(tiles, blur, centers, nbr, cnt, V, *p, plane, cost, disp, sel, nthreads);
 }
 
 void orc_solve_acmm_f64(const uint8_t* tiles, const float* centers, const int* nbr,
@@ -1720,16 +1721,16 @@ int orc_acmm_levels(int W, int* width, float* scale, int* radius, int* iteration
     return L;
 }
 
-void orc_eval_costs_f64(const uint8_t* tiles, const float* centers, const int* nbr,
-                        const int* cnt, int V, const orc::PmParams* p, const int32_t* hyp, int n,
-                        double* out, int nthreads) {
-    orc::eval_costs<double>(tiles, centers, nbr, cnt, V, *p, hyp, n, out, nthreads);
+void orc_eval_costs_f64(const uint8_t* tiles, const double* blur, const float* centers,
+                        const int* nbr, const int* cnt, int V, const orc::PmParams* p,
+                        const int32_t* hyp, int n, double* out, int nthreads) {
+    orc::eval_costs<double>(tiles, blur, centers, nbr, cnt, V, *p, hyp, n, out, nthreads);
 }
 
-void orc_eval_costs_f32(const uint8_t* tiles, const float* centers, const int* nbr,
-                        const int* cnt, int V, const orc::PmParams* p, const int32_t* hyp, int n,
-                        float* out, int nthreads) {
-    orc::eval_costs<float>(tiles, centers, nbr, cnt, V, *p, hyp, n, out, nthreads);
+void orc_eval_costs_f32(const uint8_t* tiles, const double* blur, const float* centers,
+                        const int* nbr, const int* cnt, int V, const orc::PmParams* p,
+                        const int32_t* hyp, int n, float* out, int nthreads) {
+    orc::eval_costs<float>(tiles, blur, centers, nbr, cnt, V, *p, hyp, n, out, nthreads);
 }
 
 void orc_filter_f32(const float* gray, const float* centers, const int* nbr, const int* cnt,

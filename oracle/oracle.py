@@ -118,40 +118,58 @@ def slice_u8(raw, origins, crop):
     return tiles
 
 
-def _with_blur(tiles_u8, blur_u8, pm):
-    """Tiles as the C functions read them: the blur maps follow the intensities when the blur
-    term is on (pm.blur_lambda > 0)."""
-    tiles_u8 = np.ascontiguousarray(tiles_u8, np.uint8)
-    if pm.blur_lambda > 0:
+def _blur_values(tiles_u8, pm, blur_u8=None, blur=None):
+    """Blur values as the NCC reads them (score / 255, FP64) when the blur term is on: from
+    8-bit blur maps (blur_u8) or from blur scores already divided by 255 (blur)."""
+    if pm.blur_lambda <= 0:
+        return None
+    if blur is None:
         assert blur_u8 is not None and blur_u8.shape == tiles_u8.shape
-        return np.ascontiguousarray(np.concatenate([tiles_u8, blur_u8.astype(np.uint8)]))
-    return tiles_u8
+        blur = blur_u8.astype(np.float64) / 255.0
+    assert blur.shape == tiles_u8.shape
+    return np.ascontiguousarray(blur, np.float64)
 
 
-def solve(tiles_u8, centers, nbr, cnt, pm, dtype=np.float64, threads=None, blur_u8=None):
+def laplacian_blur_tiles(raw, origins, crop):
+    """Blur tiles of ImageQualityEstimate::QuantizeBlurByLaplacian (Util/ImageQualityEstimate.cpp:
+    211-230): cv::Laplacian with ksize 1 (the 4-neighbour stencil) of the gray frame with
+    BORDER_REFLECT_101, cropped per MI and divided by 255 in FP32 as the kernel reads it."""
+    g = np.asarray(raw, np.int64)
+    p = np.pad(g, 1, mode="reflect")  # numpy 'reflect' = edge pixel not repeated
+    lap = p[1:-1, :-2] + p[1:-1, 2:] + p[:-2, 1:-1] + p[2:, 1:-1] - 4 * g
+    t = np.stack([lap[y:y + crop, x:x + crop] for x, y in np.asarray(origins)])
+    return t.astype(np.float32) / np.float32(255.0)
+
+
+def solve(tiles_u8, centers, nbr, cnt, pm, dtype=np.float64, threads=None, blur_u8=None,
+          blur=None):
     V, W, _ = tiles_u8.shape
-    tiles_u8 = _with_blur(tiles_u8, blur_u8, pm)
+    tiles_u8 = np.ascontiguousarray(tiles_u8, np.uint8)
+    bv = _blur_values(tiles_u8, pm, blur_u8, blur)
     n = V * W * W
     plane = np.zeros((n, 4), dtype)
     cost = np.zeros(n, dtype)
     disp = np.zeros((n, 4), dtype)
     sel = np.zeros(n, np.uint32)
     f = lib().orc_solve_f64 if dtype == np.float64 else lib().orc_solve_f32
-    f(_p(np.ascontiguousarray(tiles_u8)), _p(np.ascontiguousarray(centers, np.float32)),
+    f(_p(tiles_u8), _p(bv) if bv is not None else None,
+      _p(np.ascontiguousarray(centers, np.float32)),
       _p(np.ascontiguousarray(nbr, np.int32)), _p(np.ascontiguousarray(cnt, np.int32)), V,
       C.byref(pm), _p(plane), _p(cost), _p(disp), _p(sel), threads or nthreads())
     return dict(plane=plane, cost=cost, disp=disp, sel=sel)
 
 
 def eval_costs(tiles_u8, centers, nbr, cnt, pm, hyp, dtype=np.float64, threads=None,
-               blur_u8=None):
+               blur_u8=None, blur=None):
     """hyp: structured (n, 6) int32 view: mi, pixel, then 4 float32 bit patterns."""
     V = tiles_u8.shape[0]
-    tiles_u8 = _with_blur(tiles_u8, blur_u8, pm)
+    tiles_u8 = np.ascontiguousarray(tiles_u8, np.uint8)
+    bv = _blur_values(tiles_u8, pm, blur_u8, blur)
     n = hyp.shape[0]
     out = np.zeros((n, pm.max_neighbors), dtype)
     f = lib().orc_eval_costs_f64 if dtype == np.float64 else lib().orc_eval_costs_f32
-    f(_p(np.ascontiguousarray(tiles_u8)), _p(np.ascontiguousarray(centers, np.float32)),
+    f(_p(tiles_u8), _p(bv) if bv is not None else None,
+      _p(np.ascontiguousarray(centers, np.float32)),
       _p(np.ascontiguousarray(nbr, np.int32)), _p(np.ascontiguousarray(cnt, np.int32)), V,
       C.byref(pm), _p(np.ascontiguousarray(hyp, np.int32)), n, _p(out),
       threads or nthreads())

@@ -480,6 +480,38 @@ __global__ void k_slice(const uint8_t* __restrict__ raw, size_t pitch, const int
     }
 }
 
+// Laplacian blur score of the gray frame (ImageQualityEstimate::QuantizeBlurByLaplacian,
+// Util/ImageQualityEstimate.cpp:211-230: cv::Laplacian, ksize 1 = the 4-neighbour stencil, FP32,
+// BORDER_REFLECT_101), gathered straight into the MIs' blur tiles (the crops of
+// DepthSolver::SliceAndIndicatlizeForMI, DepthSolver.cpp:2709-2745) and divided by 255 as
+// ComputeBilateralNCC_Frame reads it (MIStereo/AdaptMIPMFrame.cu:140-141).
+__device__ __forceinline__ int reflect101(int i, int n) {
+    if (n == 1) return 0;
+    if (i < 0) return -i;
+    if (i >= n) return 2 * n - 2 - i;
+    return i;
+}
+
+__global__ void k_slice_blur(const uint8_t* __restrict__ raw, size_t pitch, int w, int h,
+                             const int2* __restrict__ org, int V, int W, float* __restrict__ out) {
+    const size_t WW = (size_t)W * W;
+    const size_t total = (size_t)V * WW;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int v = (int)(i / WW);
+        const int o = (int)(i - (size_t)v * WW);
+        const int y = o / W, x = o - y * W;
+        const int2 g = org[v];
+        const int gx = g.x + x, gy = g.y + y;
+        auto at = [&](int xx, int yy) {
+            return (int)raw[(size_t)reflect101(yy, h) * pitch + (size_t)reflect101(xx, w)];
+        };
+        const int lap = at(gx - 1, gy) + at(gx + 1, gy) + at(gx, gy - 1) + at(gx, gy + 1) -
+                        4 * at(gx, gy);
+        out[i] = (float)lap / 255.0f;
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Frame table.  One persistent launch serves up to kMaxFrames frames of one parameter class
 // (crop, neighbours, window), each at its own stage: random init (RandomInitializationForFrame,
@@ -2560,6 +2592,21 @@ int lfpm_slice_gray(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pit
                     const int* origins_dev, int num_mis, int crop, float* gray_dev, void* stream) {
     return slice_impl(raw_dev, raw_w, raw_h, raw_pitch, origins_dev, num_mis, crop, gray_dev,
                       0.0f, stream);
+}
+
+int lfpm_slice_blur_tiles(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pitch,
+                          const int* origins_dev, int num_mis, int crop, float* blur_tiles_dev,
+                          void* stream) {
+    if (!raw_dev || !origins_dev || !blur_tiles_dev || num_mis <= 0 || crop < 1 ||
+        raw_w < crop || raw_h < crop || raw_pitch < (size_t)raw_w)
+        return set_error(LFPM_E_INVALID, "slice_blur: bad arguments");
+    const size_t total = (size_t)num_mis * crop * crop;
+    const int threads = 256;
+    const int blocks = (int)std::min<size_t>((total + threads - 1) / threads, 65536);
+    hipLaunchKernelGGL(k_slice_blur, dim3(blocks), dim3(threads), 0, (hipStream_t)stream,
+                       raw_dev, raw_pitch, raw_w, raw_h, (const int2*)origins_dev, num_mis, crop,
+                       blur_tiles_dev);
+    return launch_check("slice_blur");
 }
 
 size_t lfpm_patchmatch_workspace_size(int num_mis, const lfpm_pm_params* p) {

@@ -250,9 +250,11 @@ def workload_c2(**kw):
 # Device entry points (torch tensors on the current device)
 # --------------------------------------------------------------------------------------
 class DeviceFrame:
-    """Device-resident inputs of one frame: raw, layout tables and tiles.  With blur_raw (a
-    raw-size uint8 blur map) the tiles carry the MIs' blur maps after the intensities, the
-    layout the blur term of the cost vector reads (lfpm_pm_params.blur_lambda > 0)."""
+    """Device-resident inputs of one frame: raw, layout tables and tiles.  With blur_raw the
+    tiles carry the MIs' blur values after the intensities, the layout the blur term of the
+    cost vector reads (lfpm_pm_params.blur_lambda > 0): a raw-size uint8 blur map (value / 255),
+    or "laplacian" for the reference's Laplacian blur score of the frame itself
+    (lfpm_slice_blur_tiles)."""
 
     def __init__(self, wl, raw=None, device="cuda", blur_raw=None):
         import torch
@@ -266,7 +268,8 @@ class DeviceFrame:
         self.cnt = torch.from_numpy(L["cnt"]).to(device)
         self.l2m = torch.from_numpy(L["lattice_to_mi"]).to(device)
         V, W = L["V"], L["crop"]
-        self.blur_raw = None if blur_raw is None else \
+        self.blur_lap = isinstance(blur_raw, str) and blur_raw == "laplacian"
+        self.blur_raw = None if blur_raw is None or self.blur_lap else \
             torch.from_numpy(np.ascontiguousarray(blur_raw, np.uint8)).to(device)
         nt = V if blur_raw is None else 2 * V
         self.tiles = torch.empty((nt, W, W), dtype=torch.float32, device=device)
@@ -284,6 +287,10 @@ class DeviceFrame:
             _check(lib().lfpm_slice_tiles(_dp(self.blur_raw), w, h, C.c_size_t(w),
                                           _dp(self.origins), L["V"], L["crop"],
                                           _dp(self.tiles[L["V"]:]), _stream(stream)))
+        if self.blur_lap:
+            _check(lib().lfpm_slice_blur_tiles(_dp(self.raw), w, h, C.c_size_t(w),
+                                               _dp(self.origins), L["V"], L["crop"],
+                                               _dp(self.tiles[L["V"]:]), _stream(stream)))
 
 
 class SolveBuffers:

@@ -262,59 +262,79 @@ def test_solve_acmm_vs_fp64(lfpm, orc, torch_cuda, pitch):
     assert frac >= 0.995
 
 
-@pytest.mark.parametrize("cfg", ["c1", "c2params"])
-def test_eval_costs_blur_vs_fp64(lfpm, orc, torch_cuda, cfg):
-    """Blur term of the cost vector (lambda 2, AdaptMIPMFrame.cu:211-218) with per-MI blur maps
-    from the synthetic defocus map: fixed-plane costs vs the FP64 oracle, same bars as the FP32
-    NCC (>= 99.5 % within 1e-5, >= 99.9 % within 1e-4 relative)."""
-    _say("case eval_costs_blur", cfg)
+def _blur_inputs(orc, wl, src):
+    """(DeviceFrame blur argument, oracle blur values) for a blur source: the synthetic defocus
+    map, or the reference's Laplacian blur score of the frame."""
+    if src == "laplacian":
+        return "laplacian", orc.laplacian_blur_tiles(wl.raw(), wl.L["origins"],
+                                                     wl.crop).astype(np.float64)
+    blur = wl.blur_map()
+    return blur, orc.slice_u8(blur, wl.L["origins"], wl.crop).astype(np.float64) / 255.0
+
+
+def test_slice_blur_tiles_exact(lfpm, orc, torch_cuda):
+    """lfpm_slice_blur_tiles == the oracle's Laplacian blur tiles, bit for bit."""
+    wl = lfpm.workload_c1()
+    df = lfpm.DeviceFrame(wl, blur_raw="laplacian")
+    got = df.tiles[wl.V:].cpu().numpy()
+    want = orc.laplacian_blur_tiles(wl.raw(), wl.L["origins"], wl.crop)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.abs(want).max() > 0.5
+
+
+@pytest.mark.parametrize("cfg,src", [("c1", "defocus"), ("c2params", "defocus"),
+                                     ("c2params", "laplacian")])
+def test_eval_costs_blur_vs_fp64(lfpm, orc, torch_cuda, cfg, src):
+    """Blur term of the cost vector (lambda 2, AdaptMIPMFrame.cu:211-218) with per-MI blur
+    values from the synthetic defocus map or the Laplacian blur score: fixed-plane costs vs the
+    FP64 oracle, same bars as the FP32 NCC (>= 99.5 % within 1e-5, >= 99.9 % within 1e-4)."""
+    _say("case eval_costs_blur", cfg, src)
     wl = lfpm.workload_c1() if cfg == "c1" else lfpm.workload_c2(raw_w=1024, raw_h=800)
     wl.pm.blur_lambda = 2.0
-    blur = wl.blur_map()
-    df = lfpm.DeviceFrame(wl, blur_raw=blur)
+    barg, bval = _blur_inputs(orc, wl, src)
+    df = lfpm.DeviceFrame(wl, blur_raw=barg)
     hyp = cone_hypotheses(lfpm, wl, 40000, seed=11)
     out = lfpm.eval_costs(df, torch_cuda.from_numpy(hyp).cuda()).cpu().numpy()
     tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
-    btiles = orc.slice_u8(blur, wl.L["origins"], wl.crop)
     ref = orc.eval_costs(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"],
-                         pm_to_oracle(orc, wl.pm), hyp, blur_u8=btiles)
+                         pm_to_oracle(orc, wl.pm), hyp, blur=bval)
     pm0 = pm_to_oracle(orc, wl.pm)
     pm0.blur_lambda = 0.0
     noblur = orc.eval_costs(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm0, hyp)
     both = (out < 2) & (ref < 2)
     flip = ((out < 2) != (ref < 2)).mean()
     e = rel_err(out[both].astype(np.float64), ref[both])
-    st = dict(cfg=cfg, n=int(both.sum()), flip=float(flip), max=float(e.max()),
+    st = dict(cfg=cfg, src=src, n=int(both.sum()), flip=float(flip), max=float(e.max()),
               p999=float(np.quantile(e, 0.999)), median=float(np.median(e)),
               frac_le_1e5=float((e <= 1e-5).mean()), frac_le_1e4=float((e <= 1e-4).mean()),
               term_active=float((ref[both] > noblur[both] + 1e-6).mean()))
     _say("eval_costs_blur", st)
-    stats_dump("eval_costs_blur_" + cfg, st)
+    stats_dump("eval_costs_blur_%s_%s" % (cfg, src), st)
     assert st["term_active"] > 0.5
     assert flip < 1e-3
     assert st["frac_le_1e5"] >= 0.995
     assert st["frac_le_1e4"] >= 0.999
 
 
-def test_solve_blur_vs_fp64(lfpm, orc, torch_cuda):
+@pytest.mark.parametrize("src", ["defocus", "laplacian"])
+def test_solve_blur_vs_fp64(lfpm, orc, torch_cuda, src):
     """Full solve with the blur term (lambda 2) on a 400x320 C1-parameter frame: disparities
     within 0.05 px of the FP64 oracle on >= 99.5 % of valid pixels."""
     wl = lfpm.workload_c1(raw_w=400, raw_h=320)
     wl.pm.blur_lambda = 2.0
-    blur = wl.blur_map()
-    _say("case solve_blur V=%d" % wl.V)
-    df = lfpm.DeviceFrame(wl, blur_raw=blur)
+    barg, bval = _blur_inputs(orc, wl, src)
+    _say("case solve_blur %s V=%d" % (src, wl.V))
+    df = lfpm.DeviceFrame(wl, blur_raw=barg)
     g, _ = _solve_gpu(lfpm, df, torch_cuda)
     tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
-    btiles = orc.slice_u8(blur, wl.L["origins"], wl.crop)
     o = orc.solve(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm_to_oracle(orc, wl.pm),
-                  blur_u8=btiles)
+                  blur=bval)
     valid = o["cost"] <= 1.8
     err = np.abs(g["plane"][:, 3].astype(np.float64) - o["plane"][:, 3])
-    st = dict(V=wl.V, frac_le_005=float((err[valid] <= 0.05).mean()),
+    st = dict(V=wl.V, src=src, frac_le_005=float((err[valid] <= 0.05).mean()),
               valid=float(valid.mean()), sel_equal=float((g["sel"] == o["sel"]).mean()))
     _say("solve_blur", st)
-    stats_dump("solve_blur", st)
+    stats_dump("solve_blur_" + src, st)
     assert st["frac_le_005"] >= 0.995
 
 

@@ -105,3 +105,22 @@ def test_blur_matches_independent_implementation(orc, lfpm, lam):
     no_blur = orc.eval_costs(*_two_mi_frame(orc, ref, nei), _pm(orc, W, r=r, sc=0.15, ss=2.5),
                              hyp)[:, 0]
     assert np.sum(got > no_blur + 1e-6) > n // 4  # the term is active on many planes
+
+
+def test_laplacian_blur_tiles_restatement(orc):
+    """The oracle's Laplacian blur tiles equal scipy's Laplacian with mirrored borders (OpenCV's
+    BORDER_REFLECT_101) and hit the closed forms of a linear ramp: 0 inside, +-2 at the edges
+    (the reflected neighbour repeats the inner one)."""
+    from scipy.ndimage import laplace
+    rng = np.random.default_rng(8)
+    raw = rng.integers(0, 256, (40, 56)).astype(np.uint8)
+    origins = np.array([[0, 0], [56 - 16, 40 - 16], [20, 11]], np.int32)
+    got = orc.laplacian_blur_tiles(raw, origins, 16)
+    full = laplace(raw.astype(np.float64), mode="mirror")
+    want = np.stack([full[y:y + 16, x:x + 16] for x, y in origins]).astype(np.float32) / \
+        np.float32(255.0)
+    assert np.array_equal(got, want)
+    ramp = np.tile(np.arange(56, dtype=np.uint8), (40, 1))
+    t = orc.laplacian_blur_tiles(ramp, np.array([[0, 0], [40, 0]], np.int32), 16) * 255.0
+    assert np.all(t[0][:, 1:] == 0) and np.all(t[0][:, 0] == 2)
+    assert np.all(t[1][:, :15] == 0) and np.all(t[1][:, 15] == -2)
