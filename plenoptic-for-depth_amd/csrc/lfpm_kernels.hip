@@ -1841,7 +1841,9 @@ __global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const DevParams& P = a.P;
     const int NT = blockDim.x, t = threadIdx.x, WW = P.WW, W = P.W;
-    const int wave = t >> 6, nwaves = NT >> 6;
+    // the wavefront index is uniform; saying so keeps the per-pixel index arithmetic and the
+    // counter-based RNG on the scalar unit
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6), nwaves = NT >> 6;
     Lds L = carve_shared(lds_raw, P);
     const WaveLds S = carve_wave(lds_raw, P, wave);
     fill_spatial(L, P);
@@ -1967,7 +1969,8 @@ __global__ void __launch_bounds__(256) k_eval_w(EvalArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const DevParams& P = a.P;
     const int NT = blockDim.x, W = P.W, WW = P.WW, K = P.K;
-    const int wave = threadIdx.x >> 6, nwaves = NT >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwaves = NT >> 6;
+    const int lane = threadIdx.x & 63;
     Lds L = wv::carve_shared(lds_raw, P);
     const wv::WaveLds S = wv::carve_wave(lds_raw, P, wave);
     fill_spatial(L, P);
