@@ -2259,15 +2259,18 @@ __global__ void __launch_bounds__(256) k_refocus(const float* __restrict__ tiles
     const float halff = (float)(W - 1) * 0.5f;
     const float reach = vk * halff + 1.0f;
     const float rs = pitch * 0.8660254037844386f;
-    const int j0 = max(0, (int)floorf((y - reach - 0.5f * pitch) / rs) - 1);
-    const int j1 = min(rows - 1, (int)ceilf((y + reach - 0.5f * pitch) / rs) + 1);
+    // lattice rows/columns whose centre can lie within `reach` (the snapped centre is within
+    // 0.5 px of the lattice point, and reach carries 1 px of slack); the inclusion test below
+    // decides, so any superset of these cells gives the same sums in the same order
+    const int j0 = max(0, (int)floorf((y - reach - 0.5f * pitch) / rs));
+    const int j1 = min(rows - 1, (int)ceilf((y + reach - 0.5f * pitch) / rs));
     const int WW = W * W;
     float acc = 0.0f;
     int nacc = 0;
     for (int j = j0; j <= j1; ++j) {
         const float off = 0.5f * pitch * (float)(1 + (j & 1));
-        const int i0 = max(0, (int)floorf((x - reach - off) / pitch) - 1);
-        const int i1 = min(cols - 1, (int)ceilf((x + reach - off) / pitch) + 1);
+        const int i0 = max(0, (int)floorf((x - reach - off) / pitch));
+        const int i1 = min(cols - 1, (int)ceilf((x + reach - off) / pitch));
         for (int i = i0; i <= i1; ++i) {
             const int v = l2m[j * cols + i];
             if (v < 0) continue;
