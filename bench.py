@@ -271,7 +271,7 @@ def plain_section(args, lfpm, torch, wl):
                                      "contraction, so no MFMA instructions are issued)",
             "kernel": "k_stage (red/black phase)", "achieved": achieved,
             "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / (PEAK_FP32 / 1e12),
-            "traffic": measured_traffic(" phase"), "avg_launch_ms": kt["phase"],
+            "traffic": measured_traffic("phase"), "avg_launch_ms": kt["phase"],
             "flop_per_launch": flop_per_launch,
             "per_unit": "22 FLOP per evaluated window sample + 16 per evaluated "
                         "(pixel, view, hypothesis) triple"}

@@ -201,7 +201,13 @@ int lfpm_patchmatch_frame(const float* tiles_dev, const float* centers_dev,
 
 /* Building blocks of lfpm_patchmatch_frame for callers that drive the loop themselves.
  * _init writes the random-initialised state; _phase reads (plane,cost,sel)_in and writes
- * (plane,cost,sel)_out for one red/black phase; _finalize writes disp for the final planes. */
+ * (plane,cost,sel)_out for one red/black phase; _finalize writes disp for the final planes.
+ * A workspace of lfpm_patchmatch_workspace_size bytes also carries per-pixel cost records
+ * (the costs of each pixel's plane over all its views) from _init through the phases of one
+ * solve, so a phase does not score a pixel's unchanged plane again: the phases of a solve
+ * take the workspace its _init was given, with the same inputs and parameters, and two solves
+ * in flight take two workspaces.  Results are the same with or without the records; a smaller
+ * workspace (the solve's state excluded) runs without them. */
 int lfpm_patchmatch_init(const float* tiles_dev, const float* centers_dev,
                          const int* neighbor_ids_dev, const int* neighbor_counts_dev,
                          int num_mis, const lfpm_pm_params* p,

@@ -128,10 +128,20 @@ int frame_products(const lfpm_batch_frame& f, void* stream) {
 int run_class(const lfpm_batch_frame* fr, const ClassPlan& c, unsigned char* ws, void* stream) {
     std::vector<StageFrame> sf(c.frames.size());
     void* scratch = ws + c.scratch_off;
+    // each frame's cost records sit at its MI offset in the class scratch; this schedule runs
+    // init, the phases and the final stage of every frame in order on that scratch
+    std::vector<float*> rec(c.frames.size());
+    int mi_off = 0;
+    for (size_t i = 0; i < c.frames.size(); ++i) {
+        rec[i] = lfpm::stage_records(c.p, scratch, mi_off);
+        mi_off += fr[c.frames[i]].num_mis;
+    }
     int rc;
     auto step = [&](int stage, int iter, int phase) -> int {
-        for (size_t i = 0; i < c.frames.size(); ++i)
+        for (size_t i = 0; i < c.frames.size(); ++i) {
             sf[i] = stage_frame(fr[c.frames[i]], stage, iter, phase);
+            sf[i].rec = rec[i];
+        }
         return lfpm::stage_launch(c.p, sf.data(), (int)sf.size(), scratch, c.scratch_bytes, stream);
     };
     if ((rc = step(lfpm::kStageInit, 0, 0))) return rc;
@@ -141,9 +151,11 @@ int run_class(const lfpm_batch_frame* fr, const ClassPlan& c, unsigned char* ws,
     bool need_final = false;
     for (int i : c.frames) need_final |= fr[i].disp_dev != nullptr;
     if (need_final) {
-        for (size_t i = 0; i < c.frames.size(); ++i)
+        for (size_t i = 0; i < c.frames.size(); ++i) {
             sf[i] = stage_frame(fr[c.frames[i]], fr[c.frames[i]].disp_dev ? lfpm::kStageFinal
                                                                           : lfpm::kStageIdle, 0, 0);
+            sf[i].rec = rec[i];
+        }
         if ((rc = lfpm::stage_launch(c.p, sf.data(), (int)sf.size(), scratch, c.scratch_bytes,
                                      stream)))
             return rc;

@@ -16,11 +16,17 @@ struct StageFrame {
     const float* plane_in; const float* cost_in; const uint32_t* sel_in;
     float* plane_out; float* cost_out; uint32_t* sel_out;
     float* disp;
+    float* rec;  // cost records of the frame's pixels (wave engine; null: none)
 };
 enum : int { kStageIdle = 0, kStageInit = 1, kStagePhase = 2, kStageFinal = 3 };
 
 // Scratch bytes one k_stage launch over `total_mis` MIs of parameter class `p` needs.
 size_t stage_scratch_size(const lfpm_pm_params* p, int total_mis);
+// Cost records of the frame whose MIs start at MI `mi_offset` of a class scratch of
+// stage_scratch_size bytes (null when the engine keeps none or scratch is null).  A frame's
+// records are written by its init stage and read by its later phase and final stages, so those
+// stages must run on the same scratch region.
+float* stage_records(const lfpm_pm_params* p, void* scratch, int mi_offset);
 // Launch k_stage for `n` frames of one parameter class (chunks of 16 frames per launch).
 int stage_launch(const lfpm_pm_params* p, const StageFrame* frames, int n, void* scratch,
                  size_t scratch_bytes, void* stream);

@@ -535,6 +535,9 @@ struct FrameDesc {
     int acmm, warm;
     int V, mi_begin, stage, iter, phase;
     uint32_t seed, uid;
+    // cost-vector records of the solve (wave engine; null = none), 5 + K floats per pixel:
+    // [valid, plane.x, plane.y, plane.z, plane.w, cost_0 .. cost_{K-1}]
+    float* cache;
 };
 
 struct StageArgs {
@@ -1450,6 +1453,47 @@ __device__ __forceinline__ bool warm_valid(const DevParams& P, float4 h, float c
     return h.w >= P.dmin && h.w <= P.dmax && cost < 1.0f && __popc(sel) >= 3;
 }
 
+// Cost-vector record of a pixel: the costs of its current plane over all views at this pixel,
+// written by the update that chose the plane from fully scored rows and read by the pixel's next
+// update (its current plane, row 0 of the first batch) and by the final record, which then skip
+// scoring it again.  A record is used only when it is marked valid and holds the pixel's current
+// plane bit for bit; the init stage writes every pixel's record, so within one solve a record
+// stands for exactly the costs a fresh scoring of that plane gives.
+constexpr uint32_t kRecValid = 1u;
+
+// lane l < 5 loads header word l, lane k < K cost k (hdr 0 / cost 2 without records)
+__device__ __forceinline__ void rec_load(const FrameDesc& F, const DevParams& P, size_t idx,
+                                         float& hdr, float& cost) {
+    hdr = 0.0f;
+    cost = 2.0f;
+    if (!F.cache) return;
+    const float* r = F.cache + idx * (size_t)(5 + P.K);
+    const int lane = lane_id();
+    if (lane < 5) hdr = r[lane];
+    if (lane < P.K) cost = r[5 + lane];
+}
+
+__device__ __forceinline__ bool rec_match(float hdr, float4 h) {
+    return (uint32_t)lane_int(__float_as_int(hdr), 0) == kRecValid &&
+           lane_int(__float_as_int(hdr), 1) == __float_as_int(h.x) &&
+           lane_int(__float_as_int(hdr), 2) == __float_as_int(h.y) &&
+           lane_int(__float_as_int(hdr), 3) == __float_as_int(h.z) &&
+           lane_int(__float_as_int(hdr), 4) == __float_as_int(h.w);
+}
+
+// lane k < K stores cost k of plane h; an invalid record only clears the valid word
+__device__ __forceinline__ void rec_store(const FrameDesc& F, const DevParams& P, size_t idx,
+                                          float4 h, float ck, bool valid) {
+    if (!F.cache) return;
+    float* r = F.cache + idx * (size_t)(5 + P.K);
+    const int lane = lane_id();
+    if (lane == 0) r[0] = __uint_as_float(valid ? kRecValid : 0u);
+    if (valid) {
+        if (lane >= 1 && lane < 5) r[lane] = lane == 1 ? h.x : (lane == 2 ? h.y : (lane == 3 ? h.z : h.w));
+        if (lane < P.K) r[5 + lane] = ck;
+    }
+}
+
 // Random initialisation of one pixel (RandomInitializationForFrame :474-518); ACMM warm levels
 // add the seed, a jittered seed and a seed with a perturbed normal
 // (InitializeFrameFromRandomOrWarm, AdaptMIPMFrameACMM.cu:373-472).
@@ -1503,9 +1547,10 @@ __device__ void init_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
         topk_row(P, S, i, c, m2, k2, b2);
         if (m2 < mean) { mean = m2; mask = k2; best = i; }
     }
-    (void)K;
+    const float4 hb = best == 0 ? hs[0] : (best == 1 ? hs[1] : (best == 2 ? hs[2] : hs[3]));
+    rec_store(F, P, idx, hb, lane_id() < K ? S.cost[best * K + lane_id()] : 2.0f, true);
     if (lane_id() == 0) {
-        F.plane_out[idx] = best == 0 ? hs[0] : (best == 1 ? hs[1] : (best == 2 ? hs[2] : hs[3]));
+        F.plane_out[idx] = hb;
         F.cost_out[idx] = mean;
         F.sel_out[idx] = mask;
     }
@@ -1518,15 +1563,22 @@ __device__ void final_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     const int W = P.W;
     const int x = pix % W, y = pix / W;
     const size_t idx = (size_t)v * P.WW + pix;
-    const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
-    const float rbc = P.blur ? L.btiles[y * P.W1 + x] : 0.0f;
-    const int ns = build_samples<TT>(L, P, S, x, y, rc, rbc);
     const float4 h = F.plane_in[idx];
-    if (lane_id() == 0) S.hyp[0] = h;
-    const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
-    const int nt = add_tasks(L, P, S, 0, h, allv, c, x, y, 0);
-    wave_sync();
-    run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
+    float rhdr, rcost;
+    rec_load(F, P, idx, rhdr, rcost);
+    if (F.cache && rec_match(rhdr, h)) {
+        if (lane_id() < P.K) S.cost[lane_id()] = rcost;
+        wave_sync();
+    } else {
+        const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
+        const float rbc = P.blur ? L.btiles[y * P.W1 + x] : 0.0f;
+        const int ns = build_samples<TT>(L, P, S, x, y, rc, rbc);
+        if (lane_id() == 0) S.hyp[0] = h;
+        const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
+        const int nt = add_tasks(L, P, S, 0, h, allv, c, x, y, 0);
+        wave_sync();
+        run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
+    }
     float mc = 2.0f;
     int am = -1;
     for (int k = 0; k < c; ++k) {
@@ -1566,6 +1618,8 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     dsel[1] = dval[1] ? F.sel_in[idx + W] : 0u;
     dsel[2] = dval[2] ? F.sel_in[idx - 1] : 0u;
     dsel[3] = dval[3] ? F.sel_in[idx + 1] : 0u;
+    float rhdr, rcost;
+    rec_load(F, P, idx, rhdr, rcost);
     const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
     const float rbc = P.blur ? L.btiles[y * P.W1 + x] : 0.0f;
     const int ns = build_samples<TT>(L, P, S, x, y, rc, rbc);
@@ -1612,7 +1666,10 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     for (int cd = 0; cd < 8; ++cd)
         if (row[cd] >= 0) row[cd] = lane_int(canon, row[cd]);
     const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
-    const int nt = add_tasks_rows(L, P, S, 0, nh, own, allv, c, x, y, 0);
+    // row 0 comes from the pixel's record when it holds the current plane
+    const bool rec_hit = F.cache && rec_match(rhdr, hcur);
+    const int nt = add_tasks_rows(L, P, S, 0, nh, rec_hit ? own & ~1u : own, allv, c, x, y, 0);
+    if (rec_hit && lane < K) S.cost[lane] = rcost;
     wave_sync();
     run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
 
@@ -1684,15 +1741,22 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     // current plane and propagation candidates under the selected views
     float4 best = hcur;
     float bcost;
+    // all-view costs of the first batch's winner, stored as the pixel's record when neither the
+    // cross-view nor the refinement candidates replace it (those are scored on the selected
+    // views only)
+    float keep = 2.0f;
+    bool replaced = false;
     {
         const float wrow = wsum_lanes(P, S, nh, nsel);
         bcost = lane_value(wrow, 0);
+        int brow = 0;
 #pragma unroll
         for (int cd = 0; cd < 8; ++cd) {
             if (row[cd] < 0) continue;
             const float cc = lane_value(wrow, row[cd]);
-            if (cc < bcost) { bcost = cc; best = S.hyp[row[cd]]; }
+            if (cc < bcost) { bcost = cc; best = S.hyp[row[cd]]; brow = row[cd]; }
         }
+        if (F.cache && lane < K) keep = S.cost[brow * K + lane];
     }
 
     // cross-view proposals through each selected view (BuildCrossViewPlaneProposal :350-398,
@@ -1742,6 +1806,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
             if (cc < bcost) {
                 bcost = cc;
                 best = prop;
+                replaced = true;
                 if (lane > j && lane < jn) qcur = xview_point(L, P, kj, x, y, best.w);
             }
         }
@@ -1782,7 +1847,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         const float wrow = wsum_lanes(P, S, 5, nsel);
         for (int i = 0; i < 5; ++i) {
             const float cc = lane_value(wrow, i);
-            if (cc < bcost) { bcost = cc; best = S.hyp[i]; }
+            if (cc < bcost) { bcost = cc; best = S.hyp[i]; replaced = true; }
         }
     }
     if (F.acmm) {
@@ -1828,6 +1893,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         bcost = fmean;
         tsel = fmask;
     }
+    rec_store(F, P, idx, best, keep, !replaced && !F.acmm);
     if (lane == 0) {
         F.plane_out[idx] = best;
         F.cost_out[idx] = bcost;
@@ -2401,9 +2467,20 @@ static const int kPhasePerCu = 8;    // k_stage workgroups per CU (persistent sl
 static const int kMaxSlotsPerCu = 8; // upper bound of the autotuner's search
 
 static bool wave_solver();
+// Cost records of V MIs (5 + K floats per pixel; the lane engine keeps none).
+static size_t rec_bytes(const DevParams& P, int V) {
+    return wave_solver() ? (size_t)V * P.WW * (5 + P.K) * 4 : 0;
+}
+// The wave engine keeps its working state in LDS; its scratch is a 256-byte header followed by
+// the cost records of the P.V MIs.  The lane engine keeps per-lane cost vectors.
 static size_t stage_scratch_bytes(const DevParams& P, int grid) {
-    if (wave_solver()) return 256;  // the wave engine keeps its per-pixel state in LDS
+    if (wave_solver()) return 256 + rec_bytes(P, P.V);
     return (size_t)grid * 9 * P.K * phase_threads(P) * 4;
+}
+// Records of the MIs from `mi_offset` on in a scratch laid out as above (null: none).
+static float* rec_region(const DevParams& P, void* scratch, int mi_offset) {
+    if (!wave_solver() || !scratch) return nullptr;
+    return (float*)((unsigned char*)scratch + 256) + (size_t)mi_offset * P.WW * (5 + P.K);
 }
 static size_t phase_scratch_bytes(const DevParams& P, int per_cu) {
     return stage_scratch_bytes(P, persistent_grid(P, per_cu));
@@ -2534,6 +2611,10 @@ void set_stage_slots_per_cu(const lfpm_pm_params* p, int per_cu) {
     g_tuned.push_back({k, per_cu});
 }
 
+float* stage_records(const lfpm_pm_params* p, void* scratch, int mi_offset) {
+    return rec_region(make_dev_params(p, std::max(mi_offset, 1)), scratch, mi_offset);
+}
+
 size_t stage_scratch_size(const lfpm_pm_params* p, int total_mis) {
     if (check_pm_params(p, std::max(total_mis, 1))) return 0;
     const DevParams P = make_dev_params(p, total_mis);
@@ -2558,6 +2639,7 @@ int stage_launch(const lfpm_pm_params* p, const StageFrame* frames, int n, void*
             f.plane_in = (const float4*)s.plane_in; f.cost_in = s.cost_in; f.sel_in = s.sel_in;
             f.plane_out = (float4*)s.plane_out; f.cost_out = s.cost_out; f.sel_out = s.sel_out;
             f.disp = (float4*)s.disp;
+            f.cache = s.rec;
             total += s.V;
         }
         a.P = make_dev_params(p, total);
@@ -2622,6 +2704,12 @@ size_t lfpm_patchmatch_workspace_size(int num_mis, const lfpm_pm_params* p) {
     return align_up(state_bytes(P)) + align_up(phase_scratch_bytes(P, kMaxSlotsPerCu));
 }
 
+// Cost records of the step calls: kept in the workspace when it has room for them.
+static float* step_records(const DevParams& P, void* ws, size_t ws_bytes) {
+    if (!ws || ws_bytes < stage_scratch_bytes(P, 8)) return nullptr;
+    return rec_region(P, ws, 0);
+}
+
 static int check_solver_ptrs(const float* tiles, const float* centers, const int* nbr,
                              const int* cnt) {
     if (!tiles || !centers || !nbr || !cnt) return set_error(LFPM_E_INVALID, "null input");
@@ -2645,6 +2733,7 @@ int lfpm_patchmatch_init(const float* tiles_dev, const float* centers_dev,
                         p->seed, p->frame_uid);
     a.f[0].stage = STAGE_INIT;
     a.f[0].plane_out = (float4*)plane_dev; a.f[0].cost_out = cost_dev; a.f[0].sel_out = sel_dev;
+    a.f[0].cache = step_records(a.P, workspace_dev, workspace_bytes);
     return launch_stage(a, stage_slots_per_cu(p), workspace_dev, workspace_bytes,
                         (hipStream_t)stream);
 }
@@ -2674,14 +2763,16 @@ int lfpm_patchmatch_phase(const float* tiles_dev, const float* centers_dev,
     f.stage = STAGE_PHASE; f.iter = iter; f.phase = phase;
     f.plane_in = (const float4*)plane_in_dev; f.cost_in = cost_in_dev; f.sel_in = sel_in_dev;
     f.plane_out = (float4*)plane_out_dev; f.cost_out = cost_out_dev; f.sel_out = sel_out_dev;
+    f.cache = step_records(a.P, workspace_dev, workspace_bytes);
     return launch_stage(a, stage_slots_per_cu(p), workspace_dev, workspace_bytes,
                         (hipStream_t)stream);
 }
 
-int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
-                             const int* neighbor_ids_dev, const int* neighbor_counts_dev,
-                             int num_mis, const lfpm_pm_params* p, const float* plane_dev,
-                             float* disp_dev, void* stream) {
+// The final stage of a solve; `rec` = the solve's cost records (null: score every pixel).
+static int finalize_impl(const float* tiles_dev, const float* centers_dev,
+                         const int* neighbor_ids_dev, const int* neighbor_counts_dev, int num_mis,
+                         const lfpm_pm_params* p, const float* plane_dev, float* disp_dev,
+                         float* rec, void* stream) {
     int rc = check_pm_params(p, num_mis);
     if (rc) return rc;
     if ((rc = check_solver_ptrs(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev)))
@@ -2694,7 +2785,16 @@ int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
                         p->seed, p->frame_uid);
     a.f[0].stage = STAGE_FINAL;
     a.f[0].plane_in = (const float4*)plane_dev; a.f[0].disp = (float4*)disp_dev;
+    a.f[0].cache = rec;
     return launch_stage(a, stage_slots_per_cu(p), nullptr, 0, (hipStream_t)stream);
+}
+
+int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
+                             const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                             int num_mis, const lfpm_pm_params* p, const float* plane_dev,
+                             float* disp_dev, void* stream) {
+    return finalize_impl(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev, num_mis,
+                         p, plane_dev, disp_dev, nullptr, stream);
 }
 
 int lfpm_patchmatch_frame(const float* tiles_dev, const float* centers_dev,
@@ -2742,9 +2842,9 @@ int lfpm_patchmatch_frame(const float* tiles_dev, const float* centers_dev,
         if ((rc = sync())) return rc;
     }
     if (disp_dev) {
-        if ((rc = lfpm_patchmatch_finalize(tiles_dev, centers_dev, neighbor_ids_dev,
-                                           neighbor_counts_dev, num_mis, p, plane_dev, disp_dev,
-                                           stream)))
+        if ((rc = finalize_impl(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev,
+                                num_mis, p, plane_dev, disp_dev,
+                                step_records(P, scratch, scratch_bytes), stream)))
             return rc;
         if ((rc = sync())) return rc;
     }

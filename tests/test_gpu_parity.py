@@ -155,6 +155,51 @@ def test_solve_seed_changes_result(lfpm, c1, torch_cuda):
     assert not np.array_equal(a["plane"], b["plane"])
 
 
+def _solve_steps_without_records(lfpm, df, torch, pm):
+    """The same solve through the step calls with no workspace, so that no per-pixel cost
+    records are kept and every update scores the pixel's current plane afresh."""
+    V, W = df.wl.V, df.wl.crop
+    n = V * W * W
+    L, st, dp = lfpm.lib(), lfpm._stream(), lfpm._dp
+    a = (torch.empty((n, 4), device="cuda"), torch.empty(n, device="cuda"),
+         torch.empty(n, dtype=torch.int32, device="cuda"))
+    b = (torch.empty((n, 4), device="cuda"), torch.empty(n, device="cuda"),
+         torch.empty(n, dtype=torch.int32, device="cuda"))
+    disp = torch.empty((n, 4), device="cuda")
+    args = (dp(df.tiles), dp(df.centers), dp(df.nbr), dp(df.cnt), V, C.byref(pm))
+    lfpm._check(L.lfpm_patchmatch_init(*args, dp(a[0]), dp(a[1]), dp(a[2]), None,
+                                       C.c_size_t(0), st))
+    for it in range(pm.iterations):
+        for ph, (src, dst) in enumerate(((a, b), (b, a))):
+            lfpm._check(L.lfpm_patchmatch_phase(*args, it, ph, dp(src[0]), dp(src[1]),
+                                                dp(src[2]), dp(dst[0]), dp(dst[1]), dp(dst[2]),
+                                                None, C.c_size_t(0), st))
+    lfpm._check(L.lfpm_patchmatch_finalize(*args, dp(a[0]), dp(disp), st))
+    torch.cuda.synchronize()
+    return dict(plane=a[0].cpu().numpy(), cost=a[1].cpu().numpy(), disp=disp.cpu().numpy(),
+                sel=a[2].cpu().numpy().view(np.uint32))
+
+
+@pytest.mark.parametrize("blur", [False, True])
+def test_cost_records_keep_results(lfpm, torch_cuda, blur):
+    """The frame solve reuses each pixel's recorded all-view costs while its plane is unchanged;
+    its outputs equal, bit for bit, those of the step calls without a workspace (no records)."""
+    import copy
+    import os
+    if os.environ.get("LFPM_SOLVER") == "lane":
+        pytest.skip("the lane engine needs a workspace for its step calls")
+    wl = lfpm.workload_c2(raw_w=800, raw_h=640)
+    pm = copy.copy(wl.pm)
+    if blur:
+        pm.blur_lambda = 0.25
+    df = lfpm.DeviceFrame(wl, blur_raw="laplacian" if blur else None)
+    _say("case cost records blur=%s V=%d" % (blur, wl.V))
+    g, _ = _solve_gpu(lfpm, df, torch_cuda, pm=pm)
+    s = _solve_steps_without_records(lfpm, df, torch_cuda, pm)
+    for k in g:
+        assert np.array_equal(g[k].view(np.uint32), s[k].view(np.uint32)), k
+
+
 def test_filter_bitexact(lfpm, orc, c1, torch_cuda):
     """Fused CheckModule pass vs the FP32 CPU twin on the GPU solve's outputs: identical reason
     masks and outputs, bit for bit."""
