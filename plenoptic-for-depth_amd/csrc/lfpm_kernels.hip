@@ -234,6 +234,8 @@ struct Lds {
     uint8_t* wts;     // K*NT view weights (phase kernel)
     uint8_t* slist;   // K*NT selected-view lists (phase kernel)
     float* btiles;    // blur term: blur values of the same (K+1) slots, FP32 (else null)
+    unsigned* next;   // wave engine: next pixel of the current MI to hand out
+    int* cur;         // wave engine: the workgroup's current MI (-1: none left)
 };
 
 static size_t lds_bytes(const DevParams& P, int NT, bool phase) {
@@ -545,6 +547,7 @@ struct StageArgs {
     int nf, total;
     float* scratch;   // [grid][9*K][NT]: 8 candidate cost vectors + sampling probabilities
     unsigned long long* counters;  // counting variant: {samples, triples}
+    unsigned* tickets;  // wave engine: 8 zeroed per-XCD MI counters (null: fixed MI stride)
     DevParams P;
 };
 
@@ -1027,6 +1030,13 @@ __device__ __forceinline__ float dppf(float v) {
 }
 
 // Sum over the 8 lanes of a task slot (lanes 8q..8q+7); every lane gets the same bits.
+// floor(v) as an integer in one instruction (v_cvt_flr_i32_f32; saturates, 0 for NaN)
+__device__ __forceinline__ int floor_int(float v) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
 __device__ __forceinline__ float red8(float v) {
     v += dppf<0xB1>(v);   // quad_perm [1,0,3,2]
     v += dppf<0x4E>(v);   // quad_perm [2,3,0,1]
@@ -1052,7 +1062,7 @@ __host__ __device__ inline size_t a16(size_t x) { return (x + 15) & ~(size_t)15;
 __host__ __device__ inline size_t shared_core_bytes(const DevParams& P) {
     const int n = 2 * P.r + 1;
     return a16((size_t)(P.K + 1) * P.TS * P.tile_bytes) + a16((size_t)P.WW * 4) +
-           a16((size_t)n * n * 4) + a16((size_t)P.K * 16) + a16((size_t)P.K * 4);
+           a16((size_t)n * n * 4) + a16((size_t)P.K * 16) + a16((size_t)P.K * 4) + 16;
 }
 
 // Shared part: tiles, tables, and (blur term) the FP32 blur slots after them.
@@ -1087,6 +1097,8 @@ __device__ __forceinline__ Lds carve_shared(unsigned char* base, const DevParams
     q = base + (a16((size_t)(P.K + 1) * P.TS * P.tile_bytes) + a16((size_t)P.WW * 4) +
                 a16((size_t)n * n * 4) + a16((size_t)P.K * 16));
     L.nid = (int*)q;
+    L.next = (unsigned*)(base + shared_core_bytes(P) - 16);
+    L.cur = (int*)(base + shared_core_bytes(P) - 12);
     L.wts = nullptr;
     L.slist = nullptr;
     L.btiles = P.blur ? (float*)(base + shared_core_bytes(P)) : nullptr;
@@ -1218,14 +1230,17 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
     const int lane = lane_id();
     const int q = lane >> 3, u = lane & 7;
     const int T = (ns + 7) >> 3;
-    const float Wm1 = (float)(P.W - 1), W1f = (float)P.W1;
     const float pxf = (float)px, pyf = (float)py;
     const float min_var = TileScale<TT>::min_var(P);
     const TT* tiles = tile_base<TT>(L);
     const int W1 = P.W1;
+    const unsigned Wu = (unsigned)P.W;
+    // the last cell whose 2x2 footprint lies in the padded tile: a rejected sample reads at its
+    // cell clamped to this one, so every read stays inside the tile
+    const unsigned maxcell = (unsigned)((P.W - 1) * W1 + P.W - 1);
     for (int t0 = 0; t0 < ntask; t0 += kSlots) {
         const int t = t0 + q;
-        const bool live = t < ntask;
+        bool live = t < ntask;
         int h = 0, k = 0;
         int base = 0;
         // source point of window offset (i, j): q = (px, py) + (i, j) + step * (w + A i + B j),
@@ -1238,34 +1253,60 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
             const float hw = S.hyp[h].w;
             const float2 ab = S.hab[h];
             const float2 st = L.step[k];
-            base = (k + 1) * P.TS;
+            base = (int)__umul24((unsigned)(k + 1), (unsigned)P.TS);
             Qx0 = fmaf(st.x, hw, pxf); Qxi = fmaf(st.x, ab.x, 1.0f); Qxj = st.x * ab.y;
             Qy0 = fmaf(st.y, hw, pyf); Qyi = st.y * ab.x; Qyj = fmaf(st.y, ab.y, 1.0f);
+            // a map with a non-finite coefficient puts every sample outside the crop (cost 2,
+            // the value its row was cleared to); excluding it keeps every q below finite
+            live = __builtin_isfinite(Qx0) && __builtin_isfinite(Qxi) &&
+                   __builtin_isfinite(Qxj) && __builtin_isfinite(Qy0) &&
+                   __builtin_isfinite(Qyi) && __builtin_isfinite(Qyj);
         }
+        const TT* tsrc = tiles + base;
         float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f, a5 = 0.0f;
         float b0 = 0.0f, b1 = 0.0f;
-#pragma unroll 2
-        for (int it = 0; it < T; ++it) {
-            const int s = (it << 3) | u;
-            const float4 rec = S.samp[s];
+        // window samples u, u + 8, ... in order, one per lane and step, two steps at a time:
+        // both records are read, both cells located and every tap read before either sample
+        // is accumulated (the floor conversion is inline asm, which neither the unroller nor
+        // the scheduler moves loads across, so the overlap is written out here)
+        struct Cell {
+            float4 rec;
+            float fx, fy;
+            unsigned cell;
+            bool ok;
+        };
+        auto locate = [&](const float4& rec) {
+            Cell m;
+            m.rec = rec;
             const float qx = fmaf(Qxj, rec.y, fmaf(Qxi, rec.x, Qx0));
             const float qy = fmaf(Qyj, rec.y, fmaf(Qyi, rec.x, Qy0));
-            const float fx0 = floorf(qx), fy0 = floorf(qy);
-            const float xa = __builtin_amdgcn_fmed3f(fx0, 0.0f, Wm1);
-            const float ya = __builtin_amdgcn_fmed3f(fy0, 0.0f, Wm1);
-            // 0 <= q < W on both axes  <=>  the clamped cell is the cell itself
-            const bool ok = live && xa == fx0 && ya == fy0;
-            const float fx = qx - fx0, fy = qy - fy0;
-            const TT* tp = tiles + base + (int)(ya * W1f + xa);
-            const float ta = ldt(tp), tb = ldt(tp + 1), tc = ldt(tp + W1), te = ldt(tp + W1 + 1);
-            const float top = ta + fx * (tb - ta);
-            const float bot = tc + fx * (te - tc);
+            // cell and in-cell offset: floor(q) and q - floor(q) (exact for q >= 0)
+            const int ix = floor_int(qx), iy = floor_int(qy);
+            m.fx = __builtin_amdgcn_fractf(qx);
+            m.fy = __builtin_amdgcn_fractf(qy);
+            // 0 <= q < W on both axes
+            m.ok = live && (unsigned)ix < Wu && (unsigned)iy < Wu;
+            m.cell = min(__umul24((unsigned)iy, (unsigned)W1) + (unsigned)ix, maxcell);
+            return m;
+        };
+        auto taps = [&](const Cell& m, float4& t4, float4& b4) {
+            const TT* tp = tsrc + m.cell;
+            t4 = make_float4(ldt(tp), ldt(tp + 1), ldt(tp + W1), ldt(tp + W1 + 1));
+            if (BLUR) {
+                const float* bp = L.btiles + base + m.cell;
+                b4 = make_float4(bp[0], bp[1], bp[W1], bp[W1 + 1]);
+            }
+        };
+        auto accumulate = [&](const Cell& m, int s, const float4& t4, const float4& b4) {
+            const float4 rec = m.rec;
+            const float fx = m.fx, fy = m.fy;
+            const bool ok = m.ok;
+            const float top = t4.x + fx * (t4.y - t4.x);
+            const float bot = t4.z + fx * (t4.w - t4.z);
             const float y = (top + fy * (bot - top)) - rc;
             if (BLUR) {
-                const float* bp = L.btiles + base + (int)(ya * W1f + xa);
-                const float ba = bp[0], bb = bp[1], bc = bp[W1], be = bp[W1 + 1];
-                const float btop = ba + fx * (bb - ba);
-                const float bbot = bc + fx * (be - bc);
+                const float btop = b4.x + fx * (b4.y - b4.x);
+                const float bbot = b4.z + fx * (b4.w - b4.z);
                 const float yb = (btop + fy * (bbot - btop)) - rbc;
                 const bool okb = ok && s < ns;  // the zero padding has weight 0 but not blur 0
                 b0 += okb ? S.sblur[s] : 0.0f;
@@ -1283,6 +1324,24 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
             a3 += wy;
             a4 = fmaf(wy, yv, a4);
             a5 = fmaf(wy, rec.z, a5);
+        };
+        int it = 0;
+        for (; it + 2 <= T; it += 2) {
+            const int s0 = (it << 3) | u, s1 = s0 + 8;
+            const float4 r0 = S.samp[s0], r1 = S.samp[s1];
+            const Cell m0 = locate(r0), m1 = locate(r1);
+            float4 t0, t1, c0, c1;
+            taps(m0, t0, c0);
+            taps(m1, t1, c1);
+            accumulate(m0, s0, t0, c0);
+            accumulate(m1, s1, t1, c1);
+        }
+        if (it < T) {
+            const int s0 = (it << 3) | u;
+            const Cell m0 = locate(S.samp[s0]);
+            float4 t0, c0;
+            taps(m0, t0, c0);
+            accumulate(m0, s0, t0, c0);
         }
         a0 = red8(a0); a1 = red8(a1); a2 = red8(a2);
         a3 = red8(a3); a4 = red8(a4); a5 = red8(a5);
@@ -1902,6 +1961,45 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     wave_sync();
 }
 
+// MI hand-out.  XCD x owns the MI range [x*total/8, (x+1)*total/8) (neighbouring MIs share
+// tiles in the XCD's L2).  With tickets each workgroup takes the next MI of its XCD's range and,
+// once that is exhausted, of the other XCDs' ranges in turn, so the workgroups finish together
+// however the cost of their MIs varies; without them it walks its XCD's range with a fixed
+// stride.  Which workgroup updates an MI does not change the result.  Called by wavefront 0
+// (uniform control flow; lane 0 takes the tickets) and stored to L.cur; -1: none left.
+__device__ __forceinline__ void next_mi(const Lds& L, unsigned* tickets, int total, int stride,
+                                        int& turn, int& prev) {
+    const int xcd = blockIdx.x & 7;
+    int m = -1;
+    if (!tickets) {
+        const int hi = (int)(((long long)total * (xcd + 1)) >> 3);
+        prev = prev < 0 ? (int)(((long long)total * xcd) >> 3) + (int)(blockIdx.x >> 3)
+                        : prev + stride;
+        m = prev < hi ? prev : -1;
+    } else {
+        for (; turn < 8; ++turn) {
+            const int x = (xcd + turn) & 7;
+            const int lo = (int)(((long long)total * x) >> 3);
+            const int hi = (int)(((long long)total * (x + 1)) >> 3);
+            unsigned r = 0u;
+            if (lane_id() == 0) r = atomicAdd(&tickets[x], 1u);
+            const int c = lo + (int)__builtin_amdgcn_readfirstlane(r);
+            if (c < hi) {
+                m = c;
+                break;
+            }
+        }
+    }
+    if (lane_id() == 0) *L.cur = m;
+}
+
+// The next pixel index of the workgroup's MI for this wavefront (uniform).
+__device__ __forceinline__ int next_pixel(const Lds& L) {
+    unsigned r = 0u;
+    if (lane_id() == 0) r = atomicAdd(L.next, 1u);
+    return (int)__builtin_amdgcn_readfirstlane(r);
+}
+
 template <bool COUNT, typename TT, bool BLUR>
 __global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -1913,23 +2011,39 @@ __global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
     Lds L = carve_shared(lds_raw, P);
     const WaveLds S = carve_wave(lds_raw, P, wave);
     fill_spatial(L, P);
-    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
-    const int lo = (int)(((long long)a.total * xcd) >> 3);
-    const int hi = (int)(((long long)a.total * (xcd + 1)) >> 3);
+    // MIs come from next_mi; wavefront 0 fetches and the workgroup reads the result from L.cur
+    unsigned* const tickets = a.tickets;
+    const int total = a.total, stride = P.slots_per_xcd;
+    int turn = 0, prev = -1;  // wavefront 0's hand-out state
     WorkCount wc{0u, 0u};
-    int f = 0;
-    for (int gm = lo + local; gm < hi; gm += P.slots_per_xcd) {
+    int f = 0;  // frame of the current MI
+    if (wave == 0) next_mi(L, tickets, total, stride, turn, prev);
+    for (;;) {
+        // every wavefront is done with the previous MI, and its successor is in L.cur
+        __syncthreads();
+        const int gm = __builtin_amdgcn_readfirstlane(*L.cur);
+        if (gm < 0) break;
+        while (f > 0 && gm < a.f[f].mi_begin) --f;  // MIs come in order within an XCD range
         while (f + 1 < a.nf && gm >= a.f[f + 1].mi_begin) ++f;
         const FrameDesc& F = a.f[f];
         const int stage = F.stage;
-        if (stage == STAGE_IDLE) continue;  // uniform across the workgroup
+        if (stage == STAGE_IDLE) {  // uniform across the workgroup
+            __syncthreads();  // every wavefront has read L.cur
+            if (wave == 0) next_mi(L, tickets, total, stride, turn, prev);
+            continue;
+        }
         const int v = gm - F.mi_begin;
-        __syncthreads();
         const int c = stage_mi<TT>(L, P, v, F.tiles, F.centers, F.nbr, F.cnt,
                                     P.blur ? F.tiles + (size_t)F.V * WW : nullptr);
         if (stage == STAGE_PHASE)
             for (int i = t; i < WW; i += NT) L.costp[i] = F.cost_in[(size_t)v * WW + i];
+        // pixels are handed out one at a time (wave w starts at pixel w), so the wavefronts
+        // of the workgroup finish the MI together however their pixels' work varies; which
+        // wavefront updates a pixel does not change its result
+        if (t == 0) *L.next = (unsigned)nwaves;
         __syncthreads();
+        if (wave == 0) next_mi(L, tickets, total, stride, turn, prev);  // L.cur was read
+                                                                    // before the barrier
         if (stage == STAGE_PHASE) {
             const size_t base = (size_t)v * WW;
             const int phase = F.phase;
@@ -1944,16 +2058,18 @@ __global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
             }
             if (c > 0) {
                 const int nact = (W & 1) ? (phase == 0 ? (WW + 1) / 2 : WW / 2) : WW / 2;
-                for (int aidx = wave; aidx < nact; aidx += nwaves) {
+                for (int aidx = wave; aidx < nact; aidx = next_pixel(L)) {
                     int x, y;
                     active_pixel(W, aidx, phase, x, y);
                     phase_pixel<COUNT, TT, BLUR>(L, P, F, S, v, c, x, y, wc);
                 }
             }
         } else if (stage == STAGE_INIT) {
-            for (int pix = wave; pix < WW; pix += nwaves) init_pixel<COUNT, TT, BLUR>(L, P, F, S, v, c, pix, wc);
+            for (int pix = wave; pix < WW; pix = next_pixel(L))
+                init_pixel<COUNT, TT, BLUR>(L, P, F, S, v, c, pix, wc);
         } else {
-            for (int pix = wave; pix < WW; pix += nwaves) final_pixel<COUNT, TT, BLUR>(L, P, F, S, v, c, pix, wc);
+            for (int pix = wave; pix < WW; pix = next_pixel(L))
+                final_pixel<COUNT, TT, BLUR>(L, P, F, S, v, c, pix, wc);
         }
     }
     flush_counts<COUNT>(a.counters, wc);
@@ -2534,6 +2650,10 @@ static StageKernel stage_w_kernel(bool count, bool fp16, bool blur) {
                 : (blur ? wv::k_stage_w<false, float, true> : wv::k_stage_w<false, float, false>);
 }
 
+__global__ void k_zero_tickets(unsigned* __restrict__ t) {
+    if (threadIdx.x < 8) t[threadIdx.x] = 0u;
+}
+
 // Launch k_stage over the frames of `a` (mi_begin/total filled here).
 static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_bytes,
                         hipStream_t s) {
@@ -2557,6 +2677,13 @@ static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_
         a.P.slots_per_xcd = grid / 8;
         a.scratch = nullptr;
         a.counters = g_counters;
+        // MI tickets in the scratch header, cleared by a kernel (not a memset node, so the
+        // launch pair can sit in a captured graph); without a scratch the fixed MI stride
+        a.tickets = nullptr;
+        if (scratch && scratch_bytes >= 256) {
+            a.tickets = (unsigned*)scratch;
+            hipLaunchKernelGGL(k_zero_tickets, dim3(1), dim3(64), 0, s, a.tickets);
+        }
         hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
         return launch_check("k_stage");
     }
@@ -2768,11 +2895,12 @@ int lfpm_patchmatch_phase(const float* tiles_dev, const float* centers_dev,
                         (hipStream_t)stream);
 }
 
-// The final stage of a solve; `rec` = the solve's cost records (null: score every pixel).
+// The final stage of a solve on the solve's scratch (its cost records and MI tickets; null:
+// score every pixel, fixed MI stride).
 static int finalize_impl(const float* tiles_dev, const float* centers_dev,
                          const int* neighbor_ids_dev, const int* neighbor_counts_dev, int num_mis,
                          const lfpm_pm_params* p, const float* plane_dev, float* disp_dev,
-                         float* rec, void* stream) {
+                         void* scratch, size_t scratch_bytes, void* stream) {
     int rc = check_pm_params(p, num_mis);
     if (rc) return rc;
     if ((rc = check_solver_ptrs(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev)))
@@ -2785,8 +2913,8 @@ static int finalize_impl(const float* tiles_dev, const float* centers_dev,
                         p->seed, p->frame_uid);
     a.f[0].stage = STAGE_FINAL;
     a.f[0].plane_in = (const float4*)plane_dev; a.f[0].disp = (float4*)disp_dev;
-    a.f[0].cache = rec;
-    return launch_stage(a, stage_slots_per_cu(p), nullptr, 0, (hipStream_t)stream);
+    a.f[0].cache = step_records(a.P, scratch, scratch_bytes);
+    return launch_stage(a, stage_slots_per_cu(p), scratch, scratch_bytes, (hipStream_t)stream);
 }
 
 int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
@@ -2794,7 +2922,7 @@ int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
                              int num_mis, const lfpm_pm_params* p, const float* plane_dev,
                              float* disp_dev, void* stream) {
     return finalize_impl(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev, num_mis,
-                         p, plane_dev, disp_dev, nullptr, stream);
+                         p, plane_dev, disp_dev, nullptr, 0, stream);
 }
 
 int lfpm_patchmatch_frame(const float* tiles_dev, const float* centers_dev,
@@ -2843,8 +2971,8 @@ int lfpm_patchmatch_frame(const float* tiles_dev, const float* centers_dev,
     }
     if (disp_dev) {
         if ((rc = finalize_impl(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev,
-                                num_mis, p, plane_dev, disp_dev,
-                                step_records(P, scratch, scratch_bytes), stream)))
+                                num_mis, p, plane_dev, disp_dev, scratch, scratch_bytes,
+                                stream)))
             return rc;
         if ((rc = sync())) return rc;
     }
