@@ -126,6 +126,13 @@ struct DevStream {
         const uint32_t q = slot & 3u;
         return q == 0 ? w[0] : (q == 1 ? w[1] : (q == 2 ? w[2] : w[3]));
     }
+    // Slot slot0 + l, where every lane's slot lies in slot0's block: the block is computed once
+    // for the wavefront (slot0 is uniform) and each lane picks its word.
+    __device__ uint32_t raw_lane(uint32_t slot0, uint32_t l) {
+        (void)raw(slot0);
+        const uint32_t q = (slot0 + l) & 3u;
+        return q == 0 ? w[0] : (q == 1 ? w[1] : (q == 2 ? w[2] : w[3]));
+    }
 };
 
 __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
@@ -1128,6 +1135,9 @@ __device__ int build_samples(const Lds& L, const DevParams& P, const WaveLds& S,
     const int lane = lane_id();
     const int r = P.r, inc = P.inc, W = P.W, n = 2 * r + 1, m = grid_side(P);
     const float inv_sc = TileScale<TT>::inv_sc(P);
+    // w / m as (w + 1/2) * (1/m) in FP32: exact, since (w + 1/2) / m lies at least 1/(2m) from
+    // an integer and the two roundings move it by less than m * 2^-23 for w < m^2
+    const float inv_m = 1.0f / (float)m;
     int total = 0;
     for (int b = 0; b < m * m; b += 64) {
         const int w = b + lane;
@@ -1135,7 +1145,7 @@ __device__ int build_samples(const Lds& L, const DevParams& P, const WaveLds& S,
         float4 rec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float bv = 0.0f;
         if (w < m * m) {
-            const int ii = w / m, jj = w - ii * m;
+            const int ii = (int)(((float)w + 0.5f) * inv_m), jj = w - ii * m;
             const int i = -r + ii * inc, j = -r + jj * inc;
             const int rx = px + i, ry = py + j;
             valid = (unsigned)rx < (unsigned)W && (unsigned)ry < (unsigned)W;
@@ -1312,17 +1322,17 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
                 b0 += okb ? S.sblur[s] : 0.0f;
                 b1 += okb ? yb : 0.0f;
             }
-            // masked by select, so a non-finite interpolant of a rejected sample adds nothing
+            // a rejected sample has weight 0; its interpolant is finite (its reads are clamped
+            // into the tile), so every product it enters is a zero that changes no sum
             const float wm = ok ? rec.w : 0.0f;
-            const float yv = ok ? y : 0.0f;
             const float wmx = wm * rec.z;
-            const float wy = wm * yv;
+            const float wy = wm * y;
             if (COUNT && ok && s < ns) ++wc.samples;
             a0 += wm;
             a1 += wmx;
             a2 = fmaf(wmx, rec.z, a2);
             a3 += wy;
-            a4 = fmaf(wy, yv, a4);
+            a4 = fmaf(wy, y, a4);
             a5 = fmaf(wy, rec.z, a5);
         };
         int it = 0;
@@ -1493,8 +1503,10 @@ __device__ __forceinline__ float3 perturb_normal(DevStream& rs, int x, int y, fl
     float vx = (float)x, vy = (float)y, vz = 1.0f;
     normalize3(vx, vy, vz);
     // lane l < 3 takes angle l + 1: the three sines and cosines side by side (every lane active)
+    // (both angle slot groups start a Philox block, so the three slots share it)
+    static_assert(SLOT_REF_ANGLE % 4 == 0 && SLOT_WARM_ANGLE % 4 == 0, "angle slots in one block");
     const int l = lane_id() < 3 ? lane_id() : 0;
-    const float a = (u01_dev(rs.raw(angle_slot + l)) - 0.5f) * amp;
+    const float a = (u01_dev(rs.raw_lane(angle_slot, (uint32_t)l)) - 0.5f) * amp;
     const float sa = sinf(a), ca = cosf(a);
     const float s1 = lane_value(sa, 0), s2 = lane_value(sa, 1), s3 = lane_value(sa, 2);
     const float c1 = lane_value(ca, 0), c2 = lane_value(ca, 1), c3 = lane_value(ca, 2);
