@@ -1234,7 +1234,7 @@ __device__ __forceinline__ int add_tasks_rows(const Lds& L, const DevParams& P,
 // sums of ComputeBilateralNCC_Frame :140-144 over the samples that enter the moments):
 // cost = min(2, ncc + lambda * (1 - exp(-0.007368 * (sum_ref_blur - sum_src_blur)^2))), with
 // both sums taken about the centre pixel's blur rbc (the offsets cancel in the difference).
-template <bool COUNT, typename TT, bool BLUR>
+template <bool COUNT, typename TT, bool BLUR, int W1C>
 __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, int ntask, int ns,
                           int px, int py, float rc, float rbc, WorkCount& wc) {
     const int lane = lane_id();
@@ -1243,11 +1243,13 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
     const float pxf = (float)px, pyf = (float)py;
     const float min_var = TileScale<TT>::min_var(P);
     const TT* tiles = tile_base<TT>(L);
-    const int W1 = P.W1;
-    const unsigned Wu = (unsigned)P.W;
+    // tile row pitch: a compile-time constant in the kernel specialised for one crop (the
+    // second row of a 2x2 footprint is then an immediate offset of the first), else P.W1
+    const int W1 = W1C ? W1C : P.W1;
+    const unsigned Wu = (unsigned)(W1 - 1);
     // the last cell whose 2x2 footprint lies in the padded tile: a rejected sample reads at its
     // cell clamped to this one, so every read stays inside the tile
-    const unsigned maxcell = (unsigned)((P.W - 1) * W1 + P.W - 1);
+    const unsigned maxcell = (unsigned)((W1 - 2) * W1 + W1 - 2);
     for (int t0 = 0; t0 < ntask; t0 += kSlots) {
         const int t = t0 + q;
         bool live = t < ntask;
@@ -1385,11 +1387,11 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
 // run_tasks of the kernel variant (BLUR: the frame carries blur tiles).  The blur variant is
 // its own kernel instantiation so that its extra registers do not lower the occupancy of the
 // kernel without it.
-template <bool COUNT, typename TT, bool BLUR>
+template <bool COUNT, typename TT, bool BLUR, int W1C>
 __device__ __forceinline__ void run_batch(const Lds& L, const DevParams& P, const WaveLds& S,
                                           int ntask, int ns, int px, int py, float rc, float rbc,
                                           WorkCount& wc) {
-    run_tasks<COUNT, TT, BLUR>(L, P, S, ntask, ns, px, py, rc, rbc, wc);
+    run_tasks<COUNT, TT, BLUR, W1C>(L, P, S, ntask, ns, px, py, rc, rbc, wc);
 }
 
 // Weighted cost of hypothesis row h over the selected views (ComputeWeightedCost_Frame
@@ -1568,7 +1570,7 @@ __device__ __forceinline__ void rec_store(const FrameDesc& F, const DevParams& P
 // Random initialisation of one pixel (RandomInitializationForFrame :474-518); ACMM warm levels
 // add the seed, a jittered seed and a seed with a perturbed normal
 // (InitializeFrameFromRandomOrWarm, AdaptMIPMFrameACMM.cu:373-472).
-template <bool COUNT, typename TT, bool BLUR>
+template <bool COUNT, typename TT, bool BLUR, int W1C>
 __device__ void init_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
                            const WaveLds& S, int v, int c, int pix, WorkCount& wc) {
     const int W = P.W, K = P.K;
@@ -1604,7 +1606,7 @@ __device__ void init_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
     wave_sync();
     const int nt = add_tasks_rows(L, P, S, 0, nh, 0xFFFFFFFFu, allv, c, x, y, 0);
     wave_sync();
-    run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
+    run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nt, ns, x, y, rc, rbc, wc);
     // top-k mean and selection mask of each candidate; the lowest mean wins (first on ties)
     float mean;
     uint32_t mask;
@@ -1628,7 +1630,7 @@ __device__ void init_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
 }
 
 // Lowest-cost view record of the final plane (ComputeWeightedCost_Frame's disp, :303-306).
-template <bool COUNT, typename TT, bool BLUR>
+template <bool COUNT, typename TT, bool BLUR, int W1C>
 __device__ void final_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
                             const WaveLds& S, int v, int c, int pix, WorkCount& wc) {
     const int W = P.W;
@@ -1648,7 +1650,7 @@ __device__ void final_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
         const int nt = add_tasks(L, P, S, 0, h, allv, c, x, y, 0);
         wave_sync();
-        run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
+        run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nt, ns, x, y, rc, rbc, wc);
     }
     float mc = 2.0f;
     int am = -1;
@@ -1671,7 +1673,7 @@ __device__ void final_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
 }
 
 // One red/black update of one pixel (CheckerboardUpdateFrame :520-947).
-template <bool COUNT, typename TT, bool BLUR>
+template <bool COUNT, typename TT, bool BLUR, int W1C>
 __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
                             const WaveLds& S, int v, int c, int x, int y, WorkCount& wc) {
     const int lane = lane_id();
@@ -1742,7 +1744,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     const int nt = add_tasks_rows(L, P, S, 0, nh, rec_hit ? own & ~1u : own, allv, c, x, y, 0);
     if (rec_hit && lane < K) S.cost[lane] = rcost;
     wave_sync();
-    run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
+    run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nt, ns, x, y, rc, rbc, wc);
 
     // sampling probabilities, lane k = view k
     const float thr = 0.8f * expf((float)(F.iter * F.iter) / (-90.0f));
@@ -1852,7 +1854,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         const uint32_t live = (uint32_t)__ballot(lane < jn && qspec >= 0);
         const int npt = add_tasks_rows(L, P, S, 0, jn, live, tsel, c, x, y, 0);
         wave_sync();
-        run_batch<COUNT, TT, BLUR>(L, P, S, npt, ns, x, y, rc, rbc, wc);
+        run_batch<COUNT, TT, BLUR, W1C>(L, P, S, npt, ns, x, y, rc, rbc, wc);
         const float wrow = wsum_lanes(P, S, jn, nsel);
         // lane j's rounded point under the replayed best (recomputed after each acceptance)
         int qcur = qspec;
@@ -1871,7 +1873,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
                 if (lane == 0) S.hyp[kSpec] = prop;
                 const int n1 = add_tasks(L, P, S, kSpec, prop, tsel, c, x, y, 0);
                 wave_sync();
-                run_batch<COUNT, TT, BLUR>(L, P, S, n1, ns, x, y, rc, rbc, wc);
+                run_batch<COUNT, TT, BLUR, W1C>(L, P, S, n1, ns, x, y, rc, rbc, wc);
                 cc = wsum(P, S, kSpec, nsel);
             }
             if (cc < bcost) {
@@ -1914,7 +1916,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         wave_sync();
         const int nrt = add_tasks_rows(L, P, S, 0, 5, 0x1Fu, tsel, c, x, y, 0);
         wave_sync();
-        run_batch<COUNT, TT, BLUR>(L, P, S, nrt, ns, x, y, rc, rbc, wc);
+        run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nrt, ns, x, y, rc, rbc, wc);
         const float wrow = wsum_lanes(P, S, 5, nsel);
         for (int i = 0; i < 5; ++i) {
             const float cc = lane_value(wrow, i);
@@ -1939,7 +1941,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
             nst = add_tasks(L, P, S, 1, seed, allv, c, x, y, nst);
         }
         wave_sync();
-        run_batch<COUNT, TT, BLUR>(L, P, S, nst, ns, x, y, rc, rbc, wc);
+        run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nst, ns, x, y, rc, rbc, wc);
         float fmean;
         uint32_t fmask;
         int fb;
@@ -2012,7 +2014,7 @@ __device__ __forceinline__ int next_pixel(const Lds& L) {
     return (int)__builtin_amdgcn_readfirstlane(r);
 }
 
-template <bool COUNT, typename TT, bool BLUR>
+template <bool COUNT, typename TT, bool BLUR, int W1C>
 __global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const DevParams& P = a.P;
@@ -2073,15 +2075,15 @@ __global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
                 for (int aidx = wave; aidx < nact; aidx = next_pixel(L)) {
                     int x, y;
                     active_pixel(W, aidx, phase, x, y);
-                    phase_pixel<COUNT, TT, BLUR>(L, P, F, S, v, c, x, y, wc);
+                    phase_pixel<COUNT, TT, BLUR, W1C>(L, P, F, S, v, c, x, y, wc);
                 }
             }
         } else if (stage == STAGE_INIT) {
             for (int pix = wave; pix < WW; pix = next_pixel(L))
-                init_pixel<COUNT, TT, BLUR>(L, P, F, S, v, c, pix, wc);
+                init_pixel<COUNT, TT, BLUR, W1C>(L, P, F, S, v, c, pix, wc);
         } else {
             for (int pix = wave; pix < WW; pix = next_pixel(L))
-                final_pixel<COUNT, TT, BLUR>(L, P, F, S, v, c, pix, wc);
+                final_pixel<COUNT, TT, BLUR, W1C>(L, P, F, S, v, c, pix, wc);
         }
     }
     flush_counts<COUNT>(a.counters, wc);
@@ -2158,7 +2160,7 @@ __global__ void __launch_bounds__(256) k_eval(EvalArgs a) {
 
 // Wave-per-hypothesis form of k_eval (the default): one wavefront scores one record over all
 // of its MI's views with the task engine of wv::run_tasks.
-template <bool COUNT, typename TT, bool BLUR>
+template <bool COUNT, typename TT, bool BLUR, int W1C>
 __global__ void __launch_bounds__(256) k_eval_w(EvalArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const DevParams& P = a.P;
@@ -2195,7 +2197,7 @@ __global__ void __launch_bounds__(256) k_eval_w(EvalArgs a) {
             const uint32_t allv = c >= 32 ? 0xFFFFFFFFu : ((1u << c) - 1u);
             const int nt = wv::add_tasks(L, P, S, 0, h, allv, c, x, y, 0);
             wv::wave_sync();
-            wv::run_batch<COUNT, TT, BLUR>(L, P, S, nt, ns, x, y, rc, rbc, wc);
+            wv::run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nt, ns, x, y, rc, rbc, wc);
             for (int k = lane; k < K; k += 64) a.out[(size_t)hi * K + k] = S.cost[k];
             wv::wave_sync();
         }
@@ -2654,12 +2656,16 @@ static int fit_per_cu(KERN kern, int threads, size_t lds) {
 // Wave-engine kernels of the counting / FP16-storage / blur variants (FP16 storage has no blur
 // variant: check_pm_params refuses the pair).
 using StageKernel = void (*)(StageArgs);
-static StageKernel stage_w_kernel(bool count, bool fp16, bool blur) {
+// kSpecialCrop: the crop of the kernel specialised for one tile pitch (FP32, no blur term)
+static const int kSpecialCrop = 16;
+static StageKernel stage_w_kernel(bool count, bool fp16, bool blur, int crop) {
     if (count)
-        return fp16 ? wv::k_stage_w<true, __half, false>
-                    : (blur ? wv::k_stage_w<true, float, true> : wv::k_stage_w<true, float, false>);
-    return fp16 ? wv::k_stage_w<false, __half, false>
-                : (blur ? wv::k_stage_w<false, float, true> : wv::k_stage_w<false, float, false>);
+        return fp16 ? wv::k_stage_w<true, __half, false, 0>
+                    : (blur ? wv::k_stage_w<true, float, true, 0>
+                            : wv::k_stage_w<true, float, false, 0>);
+    if (!fp16 && !blur && crop == kSpecialCrop) return wv::k_stage_w<false, float, false, kSpecialCrop + 1>;
+    return fp16 ? wv::k_stage_w<false, __half, false, 0>
+                : (blur ? wv::k_stage_w<false, float, true, 0> : wv::k_stage_w<false, float, false, 0>);
 }
 
 __global__ void k_zero_tickets(unsigned* __restrict__ t) {
@@ -2683,7 +2689,7 @@ static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_
             return set_error(LFPM_E_UNSUPPORTED, "k_stage: %zu B of LDS per workgroup (crop %d, "
                              "%d views, radius %d) exceeds 64 KiB", lds, a.P.W, a.P.K, a.P.r);
         const StageKernel kern = stage_w_kernel(g_counters != nullptr, a.P.tile_bytes == 2,
-                                                a.P.blur != 0);
+                                                a.P.blur != 0, a.P.W);
         const int fit = fit_per_cu(kern, NT, lds);
         const int grid = persistent_grid_n(total, std::max(1, std::min(per_cu, fit)));
         a.P.slots_per_xcd = grid / 8;
@@ -3288,11 +3294,12 @@ int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,
         using EvalKernel = void (*)(EvalArgs);
         const bool blur = P.blur != 0;
         const EvalKernel kern =
-            g_counters ? (fp16 ? k_eval_w<true, __half, false>
-                               : (blur ? k_eval_w<true, float, true> : k_eval_w<true, float, false>))
-                       : (fp16 ? k_eval_w<false, __half, false>
-                               : (blur ? k_eval_w<false, float, true>
-                                       : k_eval_w<false, float, false>));
+            g_counters ? (fp16 ? k_eval_w<true, __half, false, 0>
+                               : (blur ? k_eval_w<true, float, true, 0>
+                                       : k_eval_w<true, float, false, 0>))
+                       : (fp16 ? k_eval_w<false, __half, false, 0>
+                               : (blur ? k_eval_w<false, float, true, 0>
+                                       : k_eval_w<false, float, false, 0>));
         hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
     } else {
         if (P.blur) {
