@@ -1031,9 +1031,11 @@ __device__ __forceinline__ int prefix_below(uint64_t bal) {
                                           __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
 }
 
+// (the permutations used read a lane of every lane's own row, so no lane falls back to the
+// bound value; bound_ctrl lets the compiler fold the move into the add)
 template <int CTRL>
 __device__ __forceinline__ float dppf(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
 // Sum over the 8 lanes of a task slot (lanes 8q..8q+7); every lane gets the same bits.
@@ -1062,6 +1064,7 @@ struct WaveLds {
     uint16_t* task;    // (h << 8) | k
     int* iaux;         // 2*K ints (draw weights, selected-view list)
     float* sblur;      // blur term: reference blur minus the centre blur per compacted sample
+    float* spare;      // a word that stores of masked task slots go to
 };
 
 __host__ __device__ inline size_t a16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -1082,7 +1085,7 @@ __host__ __device__ inline size_t wave_bytes(const DevParams& P) {
     const size_t ns = (size_t)((m * m + 7) & ~7);
     return a16(ns * 16) + a16((size_t)kHMax * 16) + a16((size_t)kHMax * 8) +
            a16((size_t)kHMax * P.K * 4) + a16((size_t)kHMax * P.K * 2) + a16((size_t)P.K * 8) +
-           (P.blur ? a16(ns * 4) : 0);
+           (P.blur ? a16(ns * 4) : 0) + 16;
 }
 
 __host__ __device__ inline size_t lds_total(const DevParams& P, int nwaves) {
@@ -1124,6 +1127,8 @@ __device__ __forceinline__ WaveLds carve_wave(unsigned char* base, const DevPara
     S.task = (uint16_t*)q; q += a16((size_t)kHMax * P.K * 2);
     S.iaux = (int*)q; q += a16((size_t)P.K * 8);
     S.sblur = P.blur ? (float*)q : nullptr;
+    if (P.blur) q += a16(ns * 4);
+    S.spare = (float*)q;
     return S;
 }
 
@@ -1252,28 +1257,22 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
     const unsigned maxcell = (unsigned)((W1 - 2) * W1 + W1 - 2);
     for (int t0 = 0; t0 < ntask; t0 += kSlots) {
         const int t = t0 + q;
-        bool live = t < ntask;
-        int h = 0, k = 0;
-        int base = 0;
+        // slots past the last task set up task 0's map (no branch) and stay masked
+        const int e = S.task[t < ntask ? t : 0];
+        const int h = e >> 8, k = e & 255;
+        const float hw = S.hyp[h].w;
+        const float2 ab = S.hab[h];
+        const float2 st = L.step[k];
+        const int base = (int)__umul24((unsigned)(k + 1), (unsigned)P.TS);
         // source point of window offset (i, j): q = (px, py) + (i, j) + step * (w + A i + B j),
         // i.e. q = Q0 + Qi * i + Qj * j per axis
-        float Qx0 = 0.0f, Qxi = 0.0f, Qxj = 0.0f, Qy0 = 0.0f, Qyi = 0.0f, Qyj = 0.0f;
-        if (live) {
-            const int e = S.task[t];
-            h = e >> 8;
-            k = e & 255;
-            const float hw = S.hyp[h].w;
-            const float2 ab = S.hab[h];
-            const float2 st = L.step[k];
-            base = (int)__umul24((unsigned)(k + 1), (unsigned)P.TS);
-            Qx0 = fmaf(st.x, hw, pxf); Qxi = fmaf(st.x, ab.x, 1.0f); Qxj = st.x * ab.y;
-            Qy0 = fmaf(st.y, hw, pyf); Qyi = st.y * ab.x; Qyj = fmaf(st.y, ab.y, 1.0f);
-            // a map with a non-finite coefficient puts every sample outside the crop (cost 2,
-            // the value its row was cleared to); excluding it keeps every q below finite
-            live = __builtin_isfinite(Qx0) && __builtin_isfinite(Qxi) &&
-                   __builtin_isfinite(Qxj) && __builtin_isfinite(Qy0) &&
-                   __builtin_isfinite(Qyi) && __builtin_isfinite(Qyj);
-        }
+        const float Qx0 = fmaf(st.x, hw, pxf), Qxi = fmaf(st.x, ab.x, 1.0f), Qxj = st.x * ab.y;
+        const float Qy0 = fmaf(st.y, hw, pyf), Qyi = st.y * ab.x, Qyj = fmaf(st.y, ab.y, 1.0f);
+        // a map with a non-finite coefficient puts every sample outside the crop (cost 2, the
+        // value its row was cleared to); excluding it keeps every q below finite
+        const bool live = t < ntask && __builtin_isfinite(Qx0) && __builtin_isfinite(Qxi) &&
+                          __builtin_isfinite(Qxj) && __builtin_isfinite(Qy0) &&
+                          __builtin_isfinite(Qyi) && __builtin_isfinite(Qyj);
         const TT* tsrc = tiles + base;
         float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f, a5 = 0.0f;
         float b0 = 0.0f, b1 = 0.0f;
@@ -1360,26 +1359,24 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
         if (BLUR) { b0 = red8(b0); b1 = red8(b1); }
         // epilogue once per pass: hardware reciprocal / reciprocal square root (1 ulp), the
         // precision class of the reference's fast-math build
-        float cst = 2.0f;
-        if (live && a0 > 1e-6f) {
-            const float inv = __builtin_amdgcn_rcpf(a0);
-            const float mx = a1 * inv, my = a3 * inv;
-            const float vx = a2 * inv - mx * mx;
-            const float vy = a4 * inv - my * my;
-            if (!(vx < min_var || vy < min_var)) {
-                const float cov = a5 * inv - mx * my;
-                cst = fminf(2.0f, fmaxf(0.0f, 1.0f - cov * __builtin_amdgcn_rsqf(vx * vy)));
-            }
-        }
-        if (BLUR && live) {
+        // (computed in every lane without a branch; lanes failing the tests keep cost 2)
+        const float inv = __builtin_amdgcn_rcpf(a0);
+        const float mx = a1 * inv, my = a3 * inv;
+        const float vx = a2 * inv - mx * mx;
+        const float vy = a4 * inv - my * my;
+        const float cov = a5 * inv - mx * my;
+        const float ncc = fminf(2.0f, fmaxf(0.0f, 1.0f - cov * __builtin_amdgcn_rsqf(vx * vy)));
+        float cst = live && a0 > 1e-6f && !(vx < min_var || vy < min_var) ? ncc : 2.0f;
+        if (BLUR) {
             const float db = b0 - b1;
             const float bw = expf(-0.007368f * db * db);
-            cst = fminf(2.0f, fmaxf(0.0f, cst + P.blur_lambda * (1.0f - bw)));
+            cst = live ? fminf(2.0f, fmaxf(0.0f, cst + P.blur_lambda * (1.0f - bw))) : cst;
         }
-        if (live && u == 0) {
-            S.cost[h * P.K + k] = cst;
-            if (COUNT) ++wc.triples;
-        }
+        // every lane of a live slot stores the same cost, the other slots store to a spare word
+        // (a store without a branch keeps the reductions' last steps folded into their adds)
+        float* dst = live ? S.cost + __umul24((unsigned)h, (unsigned)P.K) + k : S.spare;
+        *dst = cst;
+        if (COUNT && live && u == 0) ++wc.triples;
     }
     wave_sync();
 }
@@ -1394,40 +1391,64 @@ __device__ __forceinline__ void run_batch(const Lds& L, const DevParams& P, cons
     run_tasks<COUNT, TT, BLUR, W1C>(L, P, S, ntask, ns, px, py, rc, rbc, wc);
 }
 
-// Weighted cost of hypothesis row h over the selected views (ComputeWeightedCost_Frame
-// :284-317), summed in selection order.
-__device__ __forceinline__ float wsum(const DevParams& P, const WaveLds& S, int h, int nsel) {
-    float wn = 0.0f, tot = 0.0f;
-    for (int j = 0; j < nsel; ++j) {
-        const int k = S.iaux[P.K + j];
-        const float w = (float)S.iaux[k];
-        wn += w;
-        tot += w * S.cost[h * P.K + k];
-    }
-    return wn <= 1e-6f ? 2.0f : tot / wn;
-}
-
-// wsum of row `lane` for every lane below nrows (lanes at or above it score row 0): the rows
-// of a batch summed side by side, each in selection order as wsum sums it.
-__device__ __forceinline__ float wsum_lanes(const DevParams& P, const WaveLds& S, int nrows,
-                                            int nsel) {
-    const int lane = lane_id();
-    const int h = lane < nrows ? lane : 0;
-    float wn = 0.0f, tot = 0.0f;
-    for (int j = 0; j < nsel; ++j) {
-        const int k = S.iaux[P.K + j];
-        const float w = (float)S.iaux[k];
-        wn += w;
-        tot += w * S.cost[h * P.K + k];
-    }
-    return wn <= 1e-6f ? 2.0f : tot / wn;
-}
-
 // Value of v in lane l; l is the same in every lane.
 __device__ __forceinline__ int lane_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __device__ __forceinline__ float lane_value(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
+
+// The selected views of an update, held in registers: lane j < n holds the view and draw weight
+// of selection slot j; the total weight wn (summed in selection order) and RN(1/wn) are the same
+// for every row.
+struct Sel {
+    int k;
+    float w;
+    float wn, winv;
+    int n;
+};
+
+__device__ __forceinline__ Sel load_sel(const DevParams& P, const WaveLds& S, int nsel) {
+    Sel sl;
+    const int lane = lane_id();
+    sl.n = nsel;
+    sl.k = lane < nsel ? S.iaux[P.K + lane] : 0;
+    sl.w = lane < nsel ? (float)S.iaux[sl.k] : 0.0f;
+    float wn = 0.0f;
+    for (int j = 0; j < nsel; ++j) wn += lane_value(sl.w, j);
+    sl.wn = wn;
+    sl.winv = wn > 0.0f ? 1.0f / wn : 0.0f;
+    return sl;
+}
+
+// tot / wn correctly rounded: Markstein's correction of tot * RN(1/wn).  Here wn is a whole
+// number of draws (1..32) and tot is 0 or in [2^-60, 64] (weights times costs in [0, 2]); for
+// that domain the result equals the IEEE quotient for every float tot (checked exhaustively).
+__device__ __forceinline__ float div_wn(float tot, float wn, float winv) {
+    const float q0 = tot * winv;
+    const float r = fmaf(-q0, wn, tot);
+    return fmaf(r, winv, q0);
+}
+
+// Weighted cost of hypothesis row h over the selected views (ComputeWeightedCost_Frame
+// :284-317), summed in selection order.
+__device__ __forceinline__ float wsum(const DevParams& P, const WaveLds& S, const Sel& sl, int h) {
+    float tot = 0.0f;
+    for (int j = 0; j < sl.n; ++j) {
+        const int k = lane_int(sl.k, j);
+        const float w = lane_value(sl.w, j);
+        tot += w * S.cost[h * P.K + k];
+    }
+    return sl.wn <= 1e-6f ? 2.0f : div_wn(tot, sl.wn, sl.winv);
+}
+
+// wsum of row `lane` for every lane below nrows (lanes at or above it score row 0): the rows
+// of a batch summed side by side, each in selection order as wsum sums it.
+__device__ __forceinline__ float wsum_lanes(const DevParams& P, const WaveLds& S, const Sel& sl,
+                                            int nrows) {
+    const int lane = lane_id();
+    return wsum(P, S, sl, lane < nrows ? lane : 0);
+}
+
 
 // Rounded point of the cross-view proposal through view k at disparity w
 // (BuildCrossViewPlaneProposal :362-376): its pixel index in the neighbour crop, -1 when the
@@ -1721,17 +1742,16 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     // so once the field settles neighbours often carry the same plane) has that row's costs at
     // this pixel: it is scored once and the candidate slots read the earlier row.
     int canon = lane;
-    if (lane < nh) {
-        const float4 a = S.hyp[lane];
-        for (int g = 0; g < lane; ++g) {
-            const float4 b = S.hyp[g];
-            if (__float_as_uint(a.x) == __float_as_uint(b.x) &&
-                __float_as_uint(a.y) == __float_as_uint(b.y) &&
-                __float_as_uint(a.z) == __float_as_uint(b.z) &&
-                __float_as_uint(a.w) == __float_as_uint(b.w)) {
+    {
+        // row g's plane is broadcast from lane g; every later row still its own canon that
+        // matches it takes g (so the first equal row wins)
+        const float4 a = S.hyp[lane < nh ? lane : 0];
+        for (int g = 0; g + 1 < nh; ++g) {
+            const int bx = lane_int(__float_as_int(a.x), g), by = lane_int(__float_as_int(a.y), g);
+            const int bz = lane_int(__float_as_int(a.z), g), bw = lane_int(__float_as_int(a.w), g);
+            if (lane > g && lane < nh && canon == lane && __float_as_int(a.x) == bx &&
+                __float_as_int(a.y) == by && __float_as_int(a.z) == bz && __float_as_int(a.w) == bw)
                 canon = g;
-                break;
-            }
         }
     }
     const uint32_t own = (uint32_t)__ballot(lane < nh && canon == lane);
@@ -1810,6 +1830,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         nsel = c;
     }
     wave_sync();
+    const Sel sl = load_sel(P, S, nsel);
 
     // current plane and propagation candidates under the selected views
     float4 best = hcur;
@@ -1820,7 +1841,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     float keep = 2.0f;
     bool replaced = false;
     {
-        const float wrow = wsum_lanes(P, S, nh, nsel);
+        const float wrow = wsum_lanes(P, S, sl, nh);
         bcost = lane_value(wrow, 0);
         int brow = 0;
 #pragma unroll
@@ -1855,7 +1876,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         const int npt = add_tasks_rows(L, P, S, 0, jn, live, tsel, c, x, y, 0);
         wave_sync();
         run_batch<COUNT, TT, BLUR, W1C>(L, P, S, npt, ns, x, y, rc, rbc, wc);
-        const float wrow = wsum_lanes(P, S, jn, nsel);
+        const float wrow = wsum_lanes(P, S, sl, jn);
         // lane j's rounded point under the replayed best (recomputed after each acceptance)
         int qcur = qspec;
         for (int j = 0; j < jn; ++j) {
@@ -1874,7 +1895,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
                 const int n1 = add_tasks(L, P, S, kSpec, prop, tsel, c, x, y, 0);
                 wave_sync();
                 run_batch<COUNT, TT, BLUR, W1C>(L, P, S, n1, ns, x, y, rc, rbc, wc);
-                cc = wsum(P, S, kSpec, nsel);
+                cc = wsum(P, S, sl, kSpec);
             }
             if (cc < bcost) {
                 bcost = cc;
@@ -1917,7 +1938,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         const int nrt = add_tasks_rows(L, P, S, 0, 5, 0x1Fu, tsel, c, x, y, 0);
         wave_sync();
         run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nrt, ns, x, y, rc, rbc, wc);
-        const float wrow = wsum_lanes(P, S, 5, nsel);
+        const float wrow = wsum_lanes(P, S, sl, 5);
         for (int i = 0; i < 5; ++i) {
             const float cc = lane_value(wrow, i);
             if (cc < bcost) { bcost = cc; best = S.hyp[i]; replaced = true; }
