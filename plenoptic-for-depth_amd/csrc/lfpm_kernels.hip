@@ -112,6 +112,15 @@ __device__ __forceinline__ float u01_dev(uint32_t x) {
 }
 
 // Per-(pixel, step) Philox draw stream with a one-block cache.
+// x / -0.18f correctly rounded: Markstein's correction of x * RN(1 / -0.18f).  The arguments here
+// are squared costs below 0.64 (0, or at least 2^-48); for every float in {0} u [2^-60, 1] the
+// result equals the IEEE quotient (checked exhaustively).
+__device__ __forceinline__ float div_m018(float x) {
+    const float c = -0.18f, y = -0x1.638e38p+2f;  // y = RN(1 / c)
+    const float q0 = x * y;
+    return fmaf(fmaf(-q0, c, x), y, q0);
+}
+
 struct DevStream {
     uint32_t pix, step, k0, k1, blk;
     uint32_t w[4];
@@ -809,7 +818,7 @@ __device__ __forceinline__ void phase_mi(const Lds& L, const DevParams& P, const
             const int sc = cand_src(cd);
             if (sc < 0) continue;
             const float cv = scr[(size_t)(sc * K + k) * NT + t];
-            if (cv < thr) { tmpw += expf(cv * cv / (-0.18f)); count += 1.0f; }
+            if (cv < thr) { tmpw += expf(div_m018(cv * cv)); count += 1.0f; }
             if (cv > 1.2f) ++cf;
         }
         float pr = 0.0f;
@@ -1208,6 +1217,16 @@ __device__ __forceinline__ int add_tasks_rows(const Lds& L, const DevParams& P,
     const int lane = lane_id();
     const float Wf = (float)P.W;
     for (int i = lane; i < nh * P.K; i += 64) S.cost[h0 * P.K + i] = 2.0f;
+    // slopes and admissibility of the rows, one row per lane (nh <= kHMax)
+    bool rok = false;
+    if (lane < nh) {
+        const float4 hv = S.hyp[h0 + lane];
+        const float A = -hv.x / hv.z, B = -hv.y / hv.z;
+        S.hab[h0 + lane] = make_float2(A, B);
+        rok = ((rows >> lane) & 1u) && fabsf(hv.z) >= P.cos_max_slant && isfinite(A) &&
+              isfinite(B);
+    }
+    const uint64_t okrows = __ballot(rok);
     if (c <= 0) return ntask;
     const int per = 64 / c;
     const int g = lane / c, k = lane - g * c;
@@ -1217,15 +1236,10 @@ __device__ __forceinline__ int add_tasks_rows(const Lds& L, const DevParams& P,
         const int r = r0 + g;
         const int h = h0 + r;
         bool act = false;
-        if (g < per && r < nh) {
-            const float4 hv = S.hyp[h];
-            const float A = -hv.x / hv.z, B = -hv.y / hv.z;
-            if (k == 0) S.hab[h] = make_float2(A, B);
-            const bool hyp_ok = fabsf(hv.z) >= P.cos_max_slant && isfinite(A) && isfinite(B);
-            if (view_ok && hyp_ok && ((rows >> r) & 1u)) {
-                const float ax = (float)px + st.x * hv.w, ay = (float)py + st.y * hv.w;
-                act = ax >= 0.0f && ax < Wf && ay >= 0.0f && ay < Wf;
-            }
+        if (view_ok && r < nh && ((okrows >> r) & 1u)) {
+            const float hw = S.hyp[h].w;
+            const float ax = (float)px + st.x * hw, ay = (float)py + st.y * hw;
+            act = ax >= 0.0f && ax < Wf && ay >= 0.0f && ay < Wf;
         }
         const uint64_t bal = __ballot(act);
         if (act) S.task[ntask + prefix_below(bal)] = (uint16_t)((h << 8) | k);
@@ -1777,7 +1791,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         for (int cd = 0; cd < 8; ++cd) {
             if (row[cd] < 0) continue;
             const float cv = S.cost[row[cd] * K + k];
-            if (cv < thr) { tmpw += expf(cv * cv / (-0.18f)); count += 1.0f; }
+            if (cv < thr) { tmpw += expf(div_m018(cv * cv)); count += 1.0f; }
             if (cv > 1.2f) ++cf;
         }
         float pr = 0.0f;
