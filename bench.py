@@ -117,8 +117,9 @@ def run_step(lfpm, df, bufs, fo, rf, timer, torch, plain=True):
             sync()
     s, e = timer.span("finalize")
     s.record()
-    lfpm._check(L.lfpm_patchmatch_finalize(dp(df.tiles), dp(df.centers), dp(df.nbr), dp(df.cnt),
-                                           V, C.byref(pm), dp(bufs.plane), dp(bufs.disp), st))
+    lfpm._check(L.lfpm_patchmatch_finalize_ws(dp(df.tiles), dp(df.centers), dp(df.nbr),
+                                              dp(df.cnt), V, C.byref(pm), dp(bufs.plane),
+                                              dp(bufs.disp), dp(scr), scr_bytes, st))
     e.record()
     sync()
     s, e = timer.span("filter")

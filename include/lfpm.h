@@ -224,6 +224,14 @@ int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
                              const int* neighbor_ids_dev, const int* neighbor_counts_dev,
                              int num_mis, const lfpm_pm_params* p, const float* plane_dev,
                              float* disp_dev, void* stream);
+/* _finalize on the solve's workspace (the one its _init and phases were given): pixels whose
+ * cost record still holds their final plane are not scored again.  Same outputs as
+ * _finalize. */
+int lfpm_patchmatch_finalize_ws(const float* tiles_dev, const float* centers_dev,
+                                const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                                int num_mis, const lfpm_pm_params* p, const float* plane_dev,
+                                float* disp_dev, void* workspace_dev, size_t workspace_bytes,
+                                void* stream);
 
 /* ACMM multi-scale variant.  Replaces AdaptMIPMFrameACMM::RunPatchMatchCUDAForFrameACMM
  * (MIStereo/AdaptMIPMFrameACMM.cpp:572-665) and RunPatchMatchCUDAForFrameACMM_Impl

@@ -1707,10 +1707,60 @@ __device__ void final_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
     }
 }
 
+// Propagation candidates over lanes: lane 8s + j holds candidate j of slot s with the positions,
+// conditions and order select_candidates visits them in (bits 0-3 dx + 8, 4-7 dy + 8, then the
+// margins x >= xmin, x <= W-1-xoff, y >= ymin, y <= W-1-yoff in 4 bits each, bit 24 present).
+// Each slot keeps its first lowest-cost candidate, as the sequential picks do; the border
+// duplicates are applied after.
+__constant__ uint32_t kCandLanes[64] = {
+    0x1010078u, 0x1020168u, 0x1021068u, 0x1030257u, 0x1032059u, 0x0000000u, 0x0000000u, 0x0000000u,
+    0x1030058u, 0x1040048u, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u,
+    0x1100098u, 0x12001a8u, 0x12010a8u, 0x13002b7u, 0x13020b9u, 0x0000000u, 0x0000000u, 0x0000000u,
+    0x13000b8u, 0x14000c8u, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u,
+    0x1000187u, 0x1010286u, 0x1100286u, 0x1020375u, 0x1200395u, 0x0000000u, 0x0000000u, 0x0000000u,
+    0x1000385u, 0x1000484u, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u,
+    0x1001089u, 0x101208au, 0x110208au, 0x102307bu, 0x120309bu, 0x0000000u, 0x0000000u, 0x0000000u,
+    0x100308bu, 0x100408cu, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u, 0x0000000u,
+};
+
+__device__ __forceinline__ void select_candidates_lanes(const float* __restrict__ costp, int W,
+                                                        int x, int y, uint32_t ct, int pos[8],
+                                                        int src[8]) {
+    const int lane = lane_id();
+    const int dx = (int)(ct & 15u) - 8, dy = (int)((ct >> 4) & 15u) - 8;
+    const int m = (x - (int)((ct >> 8) & 15u)) | (W - 1 - x - (int)((ct >> 12) & 15u)) |
+                  (y - (int)((ct >> 16) & 15u)) | (W - 1 - y - (int)((ct >> 20) & 15u));
+    const bool ok = ((ct >> 24) & 1u) && m >= 0;
+    const int p = (y + dy) * W + (x + dx);
+    const float cv = ok ? costp[p] : __builtin_inff();
+    float mn = fminf(cv, dppf<0xB1>(cv));
+    mn = fminf(mn, dppf<0x4E>(mn));
+    mn = fminf(mn, dppf<0x141>(mn));
+    const uint64_t first = __ballot(ok && cv == mn);
+    const uint64_t base = __ballot(ok && (lane & 7) == 0);
+    const int center = y * W + x;
+#pragma unroll
+    for (int sl = 0; sl < 8; ++sl) {
+        pos[sl] = center;
+        src[sl] = -1;
+        if ((base >> (8 * sl)) & 1u) {
+            const int j = __builtin_ctzll((first >> (8 * sl)) & 0xFFull);
+            pos[sl] = lane_int(p, 8 * sl + j);
+            src[sl] = sl;
+        }
+    }
+    const int H = W;
+    if (y <= 2 && y > 0) { pos[1] = pos[0]; src[1] = 0; }
+    if (y >= H - 3 && y < H - 1) { pos[3] = pos[2]; src[3] = 2; }
+    if (x <= 2 && x > 0) { pos[5] = pos[4]; src[5] = 4; }
+    if (x >= W - 3 && x < W - 1) { pos[7] = pos[6]; src[7] = 6; }
+}
+
 // One red/black update of one pixel (CheckerboardUpdateFrame :520-947).
 template <bool COUNT, typename TT, bool BLUR, int W1C>
 __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F,
-                            const WaveLds& S, int v, int c, int x, int y, WorkCount& wc) {
+                            const WaveLds& S, int v, int c, int x, int y, uint32_t ctab,
+                            WorkCount& wc) {
     const int lane = lane_id();
     const int W = P.W, WW = P.WW, K = P.K;
     const int phase = F.phase;
@@ -1734,7 +1784,7 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
 
     // batch 1: the current plane (row 0) and the propagation candidates (rows 1..)
     int pos[8], src[8];
-    select_candidates(L.costp, W, x, y, pos, src);
+    select_candidates_lanes(L.costp, W, x, y, ctab, pos, src);
     int row[8];   // cost row of the plane candidate slot cd reads, -1 if absent
     int nh = 1;
     const float4 hcur = F.plane_in[idx];
@@ -2050,7 +2100,7 @@ __device__ __forceinline__ int next_pixel(const Lds& L) {
 }
 
 template <bool COUNT, typename TT, bool BLUR, int W1C>
-__global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_stage_w(StageArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const DevParams& P = a.P;
     const int NT = blockDim.x, t = threadIdx.x, WW = P.WW, W = P.W;
@@ -2066,6 +2116,7 @@ __global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
     int turn = 0, prev = -1;  // wavefront 0's hand-out state
     WorkCount wc{0u, 0u};
     int f = 0;  // frame of the current MI
+    const uint32_t ctab = kCandLanes[t & 63];
     if (wave == 0) next_mi(L, tickets, total, stride, turn, prev);
     for (;;) {
         // every wavefront is done with the previous MI, and its successor is in L.cur
@@ -2110,7 +2161,7 @@ __global__ void __launch_bounds__(256) k_stage_w(StageArgs a) {
                 for (int aidx = wave; aidx < nact; aidx = next_pixel(L)) {
                     int x, y;
                     active_pixel(W, aidx, phase, x, y);
-                    phase_pixel<COUNT, TT, BLUR, W1C>(L, P, F, S, v, c, x, y, wc);
+                    phase_pixel<COUNT, TT, BLUR, W1C>(L, P, F, S, v, c, x, y, ctab, wc);
                 }
             }
         } else if (stage == STAGE_INIT) {
@@ -2976,6 +3027,15 @@ int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
                              float* disp_dev, void* stream) {
     return finalize_impl(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev, num_mis,
                          p, plane_dev, disp_dev, nullptr, 0, stream);
+}
+
+int lfpm_patchmatch_finalize_ws(const float* tiles_dev, const float* centers_dev,
+                                const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                                int num_mis, const lfpm_pm_params* p, const float* plane_dev,
+                                float* disp_dev, void* workspace_dev, size_t workspace_bytes,
+                                void* stream) {
+    return finalize_impl(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev, num_mis,
+                         p, plane_dev, disp_dev, workspace_dev, workspace_bytes, stream);
 }
 
 int lfpm_patchmatch_frame(const float* tiles_dev, const float* centers_dev,

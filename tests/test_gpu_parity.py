@@ -194,10 +194,21 @@ def test_cost_records_keep_results(lfpm, torch_cuda, blur):
         pm.blur_lambda = 0.25
     df = lfpm.DeviceFrame(wl, blur_raw="laplacian" if blur else None)
     _say("case cost records blur=%s V=%d" % (blur, wl.V))
-    g, _ = _solve_gpu(lfpm, df, torch_cuda, pm=pm)
+    g, bufs = _solve_gpu(lfpm, df, torch_cuda, pm=pm)
     s = _solve_steps_without_records(lfpm, df, torch_cuda, pm)
     for k in g:
         assert np.array_equal(g[k].view(np.uint32), s[k].view(np.uint32)), k
+    # the step-API final record on the solve's workspace (records of the final planes)
+    V, W = df.wl.V, df.wl.crop
+    n = V * W * W
+    off = (n * 24 + 255) // 256 * 256
+    disp = torch_cuda.empty((n, 4), device="cuda")
+    L, dp = lfpm.lib(), lfpm._dp
+    lfpm._check(L.lfpm_patchmatch_finalize_ws(
+        dp(df.tiles), dp(df.centers), dp(df.nbr), dp(df.cnt), V, C.byref(pm), dp(bufs.plane),
+        dp(disp), dp(bufs.ws[off:]), C.c_size_t(bufs.ws_bytes - off), lfpm._stream()))
+    torch_cuda.cuda.synchronize()
+    assert np.array_equal(disp.cpu().numpy().view(np.uint32), g["disp"].view(np.uint32))
 
 
 def test_filter_bitexact(lfpm, orc, c1, torch_cuda):
