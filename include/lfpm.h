@@ -146,10 +146,10 @@ typedef struct lfpm_pm_params {
                                   AdaptMIPMFrameACMM.cu:164: 0.8).  0 (default) = off.  When > 0
                                   the tiles buffer holds 2*V*crop^2 floats: the intensities, then
                                   the MIs' blur maps (blureness / 255) in the same layout.
-                                  Provided by lfpm_patchmatch_frame/_init/_phase/_finalize and
-                                  lfpm_eval_costs with FP32 tiles (storage 0) and the default
-                                  solver; the ACMM, batched and host-context entry points
-                                  return LFPM_E_UNSUPPORTED for it. */
+                                  Provided by lfpm_patchmatch_frame/_init/_phase/_finalize,
+                                  lfpm_eval_costs and the batched forms with FP32 tiles
+                                  (storage 0) and the default solver; the ACMM and host-context
+                                  entry points return LFPM_E_UNSUPPORTED for it. */
 } lfpm_pm_params;
 
 /* Fill defaults for a given crop/base (reference defaults + config conventions). */
@@ -373,7 +373,8 @@ int lfpm_v2r_convert(const float* v_dev, int n, const double c[3], double* z_dev
 
 typedef struct lfpm_batch_frame {
     /* inputs (device) */
-    const float* tiles_dev;           /* V*crop^2 in [0,1] */
+    const float* tiles_dev;           /* V*crop^2 in [0,1]; with params->blur_lambda != 0 the
+                                         V*crop^2 blur tiles follow (lfpm_slice_blur_tiles) */
     const float* gray_dev;            /* V*crop^2 in 0..255 (filter input), may be NULL */
     const float* centers_dev;         /* 2V */
     const int* neighbor_ids_dev;      /* V*K */

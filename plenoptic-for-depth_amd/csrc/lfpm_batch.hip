@@ -51,9 +51,6 @@ int plan(const lfpm_batch_frame* fr, int n, std::vector<ClassPlan>& classes, siz
             return set_error(LFPM_E_INVALID, "batch: frame %d is missing inputs or state", i);
         if (f.params->max_neighbors < 1 || f.params->max_neighbors > LFPM_MAX_NEIGHBORS)
             return set_error(LFPM_E_INVALID, "batch: frame %d bad max_neighbors", i);
-        if (f.params->blur_lambda != 0.0f)
-            return set_error(LFPM_E_UNSUPPORTED, "batch: frame %d: the blur term is provided by "
-                             "lfpm_patchmatch_frame / lfpm_eval_costs only", i);
         int c = -1;
         for (size_t k = 0; k < classes.size(); ++k)
             if (same_class(*classes[k].p, *f.params)) { c = (int)k; break; }

@@ -61,6 +61,7 @@ struct lfpm_ctx {
     int* cnt = nullptr;
     std::vector<FrameBufs> frames;
     bool frames_batched = false;  // frames carry B state
+    bool frames_blur = false;     // tiles hold the blur tiles after the intensities
     void* ws = nullptr;
     size_t ws_bytes = 0;
     int solved = 0;               // frames of the last solve call
@@ -116,18 +117,23 @@ static int alloc_tables(lfpm_ctx* c, int V, int W, int K) {
 }
 
 // Device buffers for n frames (kept when the count and batching match) and the workspace.
-static int ensure_frames(lfpm_ctx* c, int n, bool batched, const lfpm_pm_params* p) {
-    if (p && p->blur_lambda != 0.0f)
-        return set_error(LFPM_E_UNSUPPORTED, "ctx: the blur term needs blur tiles; it is "
-                         "provided by lfpm_patchmatch_frame / lfpm_eval_costs only");
+// With the blur term (p->blur_lambda != 0) the tiles carry the frame's Laplacian blur tiles after
+// the intensities; that needs the raw frame (not caller-supplied tiles) and the plain variant.
+static int ensure_frames(lfpm_ctx* c, int n, bool batched, const lfpm_pm_params* p,
+                         bool raw_frames) {
+    const bool blur = p && p->blur_lambda != 0.0f;
+    if (blur && (!raw_frames || c->variant == LFPM_VARIANT_ACMM))
+        return set_error(LFPM_E_UNSUPPORTED, "ctx: the blur term needs the raw frame's blur "
+                         "tiles and the plain variant");
     const size_t npx = (size_t)c->V * c->crop * c->crop;
     const size_t rawb = (size_t)c->raw_w * c->raw_h;
-    if ((int)c->frames.size() != n || c->frames_batched != batched) {
+    if ((int)c->frames.size() != n || c->frames_batched != batched || c->frames_blur != blur) {
         free_frames(c);
         c->frames.resize(n);
+        c->frames_blur = blur;
         for (FrameBufs& f : c->frames) {
             CTX_TRY(hipMalloc((void**)&f.raw, rawb));
-            CTX_TRY(hipMalloc((void**)&f.tiles, npx * 4));
+            CTX_TRY(hipMalloc((void**)&f.tiles, npx * 4 * (blur ? 2 : 1)));
             CTX_TRY(hipMalloc((void**)&f.gray, npx * 4));
             CTX_TRY(hipMalloc((void**)&f.plane, npx * 16));
             CTX_TRY(hipMalloc((void**)&f.cost, npx * 4));
@@ -211,6 +217,11 @@ static int solve_one(lfpm_ctx* c, FrameBufs& f, const lfpm_pm_params* p,
     int rc;
     if ((rc = lfpm_slice_tiles(f.raw, c->raw_w, c->raw_h, (size_t)c->raw_w, c->origins, c->V,
                                c->crop, f.tiles, c->stream)))
+        return rc;
+    if (p->blur_lambda != 0.0f &&
+        (rc = lfpm_slice_blur_tiles(f.raw, c->raw_w, c->raw_h, (size_t)c->raw_w, c->origins,
+                                    c->V, c->crop, f.tiles + (size_t)c->V * c->crop * c->crop,
+                                    c->stream)))
         return rc;
     if (c->variant == LFPM_VARIANT_ACMM)
         rc = lfpm_patchmatch_frame_acmm(f.tiles, c->centers, c->nbr, c->cnt, c->V, p, f.plane,
@@ -300,7 +311,7 @@ int lfpm_ctx_solve_frame(lfpm_ctx* c, const lfpm_layout_params* lp, const uint8_
     CTX_TRY(hipSetDevice(c->device));
     int rc = ensure_layout(c, lp, p);
     if (rc) return rc;
-    if ((rc = ensure_frames(c, 1, false, p))) return rc;
+    if ((rc = ensure_frames(c, 1, false, p, true))) return rc;
     FrameBufs& fb = c->frames[0];
     CTX_TRY(hipMemcpyAsync(fb.raw, raw_host, (size_t)c->raw_w * c->raw_h, hipMemcpyHostToDevice,
                            c->stream));
@@ -329,7 +340,7 @@ int lfpm_ctx_solve_frames(lfpm_ctx* c, const lfpm_layout_params* lp, int n,
     CTX_TRY(hipSetDevice(c->device));
     int rc = ensure_layout(c, lp, p);
     if (rc) return rc;
-    if ((rc = ensure_frames(c, n, true, p))) return rc;
+    if ((rc = ensure_frames(c, n, true, p, true))) return rc;
     const lfpm_filter_params* flt = reason ? f : nullptr;
     std::vector<lfpm_pm_params> pms(n, *p);
     std::vector<lfpm_batch_frame> bf(n);
@@ -340,6 +351,11 @@ int lfpm_ctx_solve_frames(lfpm_ctx* c, const lfpm_layout_params* lp, int n,
                                hipMemcpyHostToDevice, c->stream));
         if ((rc = lfpm_slice_tiles(fb.raw, c->raw_w, c->raw_h, (size_t)c->raw_w, c->origins, c->V,
                                    c->crop, fb.tiles, c->stream)))
+            return rc;
+        if (p->blur_lambda != 0.0f &&
+            (rc = lfpm_slice_blur_tiles(fb.raw, c->raw_w, c->raw_h, (size_t)c->raw_w, c->origins,
+                                        c->V, c->crop,
+                                        fb.tiles + (size_t)c->V * c->crop * c->crop, c->stream)))
             return rc;
         if ((rc = lfpm_slice_gray(fb.raw, c->raw_w, c->raw_h, (size_t)c->raw_w, c->origins, c->V,
                                   c->crop, fb.gray, c->stream)))
@@ -418,7 +434,7 @@ int lfpm_ctx_solve_tiles(lfpm_ctx* c, const uint8_t* tiles_u8, const float* cent
                            c->stream));
     CTX_TRY(hipMemcpyAsync(c->cnt, neighbor_counts, 4 * (size_t)V, hipMemcpyHostToDevice,
                            c->stream));
-    if ((rc = ensure_frames(c, 1, false, p))) return rc;
+    if ((rc = ensure_frames(c, 1, false, p, false))) return rc;
     FrameBufs& fb = c->frames[0];
     CTX_TRY(hipMemcpyAsync(fb.raw, tiles_u8, (size_t)V * W * W, hipMemcpyHostToDevice,
                            c->stream));

@@ -574,10 +574,10 @@ class BatchFrame(C.Structure):
 class FrameSlot:
     """Device-resident inputs + state + products of one frame in a batch (one serving slot)."""
 
-    def __init__(self, wl, device="cuda", filt=True, refocus=True):
+    def __init__(self, wl, device="cuda", filt=True, refocus=True, blur_raw=None, pm=None):
         import torch
         self.wl = wl
-        self.df = DeviceFrame(wl, device=device)
+        self.df = DeviceFrame(wl, device=device, blur_raw=blur_raw)
         V, W = wl.V, wl.crop
         n = V * W * W
         f32 = dict(dtype=torch.float32, device=device)
@@ -598,7 +598,7 @@ class FrameSlot:
         self.slices = wl.slices if refocus else 0
         h, w = self.df.raw.shape
         self.refocus = torch.empty((self.slices, h, w), **f32) if refocus else None
-        self.pm = wl.pm
+        self.pm = pm if pm is not None else wl.pm
         self.fp = wl.fp
 
     def nbytes(self):

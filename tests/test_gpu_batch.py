@@ -123,3 +123,25 @@ def test_autotune_keeps_results(lfpm, torch_cuda, setup):
     assert best in (2, 4, 6, 8) and ms > 0
     again = _plain(lfpm, torch_cuda, slots[0])
     _same(ref[0], again, "after tune")
+
+
+def test_batch_blur_term_matches_plain(lfpm, torch_cuda):
+    """The blur term in a batch (Laplacian blur tiles after the intensity tiles): the batched
+    solve equals the single-frame solve bit for bit, and both differ from the solve without it."""
+    import copy
+    wl = lfpm.workload_c2(raw_w=400, raw_h=320, iterations=2, slices=2, frame_uid=5)
+    pm = copy.copy(wl.pm)
+    pm.blur_lambda = 0.5
+    _say("case batch blur: V=%d" % wl.V)
+    slot = lfpm.FrameSlot(wl, filt=False, refocus=False, blur_raw="laplacian", pm=pm)
+    b = lfpm.Batch([slot])
+    b.run()
+    torch_cuda.cuda.synchronize()
+    plain = lfpm.patchmatch_frame(slot.df, pm=pm)
+    off = lfpm.patchmatch_frame(slot.df)
+    torch_cuda.cuda.synchronize()
+    for k in ("plane", "cost", "sel", "disp"):
+        a = getattr(slot, k).cpu().numpy()
+        assert np.array_equal(a.view(np.uint8), getattr(plain, k).cpu().numpy().view(np.uint8)), k
+    assert not np.array_equal(slot.cost.cpu().numpy(), off.cost.cpu().numpy())
+

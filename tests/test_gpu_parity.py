@@ -395,8 +395,9 @@ def test_solve_blur_vs_fp64(lfpm, orc, torch_cuda, src):
 
 
 def test_blur_rejected_where_not_provided(lfpm, torch_cuda):
-    """The blur term needs blur tiles: the host context, the ACMM variant and FP16 storage
-    refuse it with LFPM_E_UNSUPPORTED instead of solving without it."""
+    """The blur term needs blur tiles: the ACMM variant, FP16 storage and caller-supplied tiles
+    in the host context (no raw frame to take the Laplacian of) refuse it with
+    LFPM_E_UNSUPPORTED instead of solving without it."""
     import copy
     wl = lfpm.workload_c1(raw_w=320, raw_h=256)
     pm = copy.copy(wl.pm)
@@ -410,7 +411,32 @@ def test_blur_rejected_where_not_provided(lfpm, torch_cuda):
         lfpm.patchmatch_frame(df, pm=pm16)
     ctx = lfpm.Context()
     try:
+        tiles = np.ascontiguousarray(
+            np.stack([wl.raw()[y:y + wl.crop, x:x + wl.crop] for x, y in wl.L["origins"]]))
+        with pytest.raises(lfpm.LfpmError):
+            ctx.solve_tiles(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm)
+        ctx.set_variant(lfpm.Context.ACMM)
         with pytest.raises(lfpm.LfpmError):
             ctx.solve_frame(wl, pm=pm)
     finally:
         ctx.close()
+
+
+def test_ctx_blur_term_matches_device_path(lfpm, torch_cuda):
+    """The host context takes the raw frame's Laplacian blur tiles for the blur term: its
+    single-frame and batched solves equal the device path with those tiles, bit for bit."""
+    import copy
+    wl = lfpm.workload_c1(raw_w=320, raw_h=256)
+    pm = copy.copy(wl.pm)
+    pm.blur_lambda = 1.0
+    df = lfpm.DeviceFrame(wl, blur_raw="laplacian")
+    g, _ = _solve_gpu(lfpm, df, torch_cuda, pm=pm)
+    ctx = lfpm.Context()
+    try:
+        o = ctx.solve_frame(wl, filt=False, pm=pm)
+        outs = ctx.solve_frames(wl, [wl.raw()], filt=False, pm=pm)
+    finally:
+        ctx.close()
+    for out in (o, outs[0]):
+        for k in ("plane", "cost", "sel", "disp"):
+            assert np.array_equal(np.asarray(out[k]).view(np.uint32), g[k].view(np.uint32)), k
