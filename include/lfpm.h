@@ -146,10 +146,11 @@ typedef struct lfpm_pm_params {
                                   AdaptMIPMFrameACMM.cu:164: 0.8).  0 (default) = off.  When > 0
                                   the tiles buffer holds 2*V*crop^2 floats: the intensities, then
                                   the MIs' blur maps (blureness / 255) in the same layout.
-                                  Provided by lfpm_patchmatch_frame/_init/_phase/_finalize,
-                                  lfpm_eval_costs and the batched forms with FP32 tiles
-                                  (storage 0) and the default solver; the ACMM and host-context
-                                  entry points return LFPM_E_UNSUPPORTED for it. */
+                                  Provided with FP32 tiles (storage 0) and the default solver by
+                                  lfpm_patchmatch_frame/_init/_phase/_finalize, lfpm_eval_costs,
+                                  the batched forms, the ACMM variant (its coarse level takes
+                                  the blur maps area-resized like the intensities) and the host
+                                  context on raw frames (Laplacian blur tiles). */
 } lfpm_pm_params;
 
 /* Fill defaults for a given crop/base (reference defaults + config conventions). */

@@ -935,16 +935,22 @@ static void solve_level(Frame<T>& f, const Acmm<T>& ac, std::vector<P4<T>>& pa,
             }
 }
 
+// blur: V*crop*crop blur values as the NCC reads them (score / 255) when p.blur_lambda > 0, else
+// null; a coarse level reads them area-resized like the intensities (BuildBlurAlignedPyramids,
+// AdaptMIPMFrameACMM.cpp:338-350; its Gaussian alignment, :295-302, is switched off).
 template <typename T>
-static void solve_acmm(const uint8_t* tiles_u8, const float* centers, const int* nbr,
-                       const int* cnt, int V, const PmParams& p, T* plane_out, T* cost_out,
-                       T* disp_out, uint32_t* sel_out, int nthreads) {
+static void solve_acmm(const uint8_t* tiles_u8, const double* blur, const float* centers,
+                       const int* nbr, const int* cnt, int V, const PmParams& p, T* plane_out,
+                       T* cost_out, T* disp_out, uint32_t* sel_out, int nthreads) {
     if (nthreads <= 0) nthreads = 1;
     AcmmLevel lv[2];
     const int L = acmm_levels(p.crop, lv);
     const int W = p.crop;
     std::vector<T> full((size_t)V * W * W);
     for (size_t i = 0; i < full.size(); ++i) full[i] = (T)tiles_u8[i] / (T)255;
+    const bool with_blur = p.blur_lambda > 0.0f && blur;
+    std::vector<T> full_blur(with_blur ? full.size() : 0);
+    for (size_t i = 0; i < full_blur.size(); ++i) full_blur[i] = (T)blur[i];
     std::vector<P4<T>> pa, da, pinit;
     std::vector<T> ca, cinit;
     std::vector<uint32_t> sa, sinit;
@@ -970,10 +976,17 @@ static void solve_acmm(const uint8_t* tiles_u8, const float* centers, const int*
         f.img.resize(n);
         if (l.width == W) {
             f.img = full;
+            f.blur = full_blur;
         } else {
             for (int v = 0; v < V; ++v)
                 area_resize(full.data() + (size_t)v * W * W, W, l.width,
                             f.img.data() + (size_t)v * l.width * l.width);
+            if (with_blur) {
+                f.blur.resize(n);
+                for (int v = 0; v < V; ++v)
+                    area_resize(full_blur.data() + (size_t)v * W * W, W, l.width,
+                                f.blur.data() + (size_t)v * l.width * l.width);
+            }
         }
         Acmm<T> ac{false, nullptr, nullptr, nullptr};
         if (lid > 0) {
@@ -1701,10 +1714,11 @@ void orc_solve_f32(const uint8_t* tiles, const double* blur, const float* center
     orc::solve<float>(tiles, blur, centers, nbr, cnt, V, *p, plane, cost, disp, sel, nthreads);
 }
 
-void orc_solve_acmm_f64(const uint8_t* tiles, const float* centers, const int* nbr,
-                        const int* cnt, int V, const orc::PmParams* p, double* plane,
-                        double* cost, double* disp, uint32_t* sel, int nthreads) {
-    orc::solve_acmm<double>(tiles, centers, nbr, cnt, V, *p, plane, cost, disp, sel, nthreads);
+void orc_solve_acmm_f64(const uint8_t* tiles, const double* blur, const float* centers,
+                        const int* nbr, const int* cnt, int V, const orc::PmParams* p,
+                        double* plane, double* cost, double* disp, uint32_t* sel, int nthreads) {
+    orc::solve_acmm<double>(tiles, blur, centers, nbr, cnt, V, *p, plane, cost, disp, sel,
+                            nthreads);
 }
 
 void orc_area_resize_f64(const double* src, int W, int W0, double* dst) {

@@ -285,9 +285,10 @@ def vd_map(plane, centers, gray, crop, mla_base, raw_w, raw_h, color=True):
     return vd, col, rng
 
 
-def solve_acmm(tiles_u8, centers, nbr, cnt, pm, threads=None):
-    """ACMM multi-scale solve (FP64) of the final level: dict plane/cost/disp/sel."""
-    assert pm.blur_lambda == 0, "the ACMM restatement has no blur term"
+def solve_acmm(tiles_u8, centers, nbr, cnt, pm, threads=None, blur_u8=None, blur=None):
+    """ACMM multi-scale solve (FP64) of the final level: dict plane/cost/disp/sel.  With
+    pm.blur_lambda > 0, blur maps as in solve (a coarse level reads them area-resized)."""
+    bv = _blur_values(tiles_u8, pm, blur_u8, blur)
     V, W, _ = tiles_u8.shape
     n = V * W * W
     plane = np.zeros((n, 4), np.float64)
@@ -295,6 +296,7 @@ def solve_acmm(tiles_u8, centers, nbr, cnt, pm, threads=None):
     disp = np.zeros((n, 4), np.float64)
     sel = np.zeros(n, np.uint32)
     lib().orc_solve_acmm_f64(_p(np.ascontiguousarray(tiles_u8)),
+                             _p(bv) if bv is not None else None,
                              _p(np.ascontiguousarray(centers, np.float32)),
                              _p(np.ascontiguousarray(nbr, np.int32)),
                              _p(np.ascontiguousarray(cnt, np.int32)), V, C.byref(pm), _p(plane),

@@ -118,13 +118,13 @@ static int alloc_tables(lfpm_ctx* c, int V, int W, int K) {
 
 // Device buffers for n frames (kept when the count and batching match) and the workspace.
 // With the blur term (p->blur_lambda != 0) the tiles carry the frame's Laplacian blur tiles after
-// the intensities; that needs the raw frame (not caller-supplied tiles) and the plain variant.
+// the intensities; that needs the raw frame (not caller-supplied tiles).
 static int ensure_frames(lfpm_ctx* c, int n, bool batched, const lfpm_pm_params* p,
                          bool raw_frames) {
     const bool blur = p && p->blur_lambda != 0.0f;
-    if (blur && (!raw_frames || c->variant == LFPM_VARIANT_ACMM))
+    if (blur && !raw_frames)
         return set_error(LFPM_E_UNSUPPORTED, "ctx: the blur term needs the raw frame's blur "
-                         "tiles and the plain variant");
+                         "tiles");
     const size_t npx = (size_t)c->V * c->crop * c->crop;
     const size_t rawb = (size_t)c->raw_w * c->raw_h;
     if ((int)c->frames.size() != n || c->frames_batched != batched || c->frames_blur != blur) {

@@ -3264,14 +3264,15 @@ struct AcmmWs {
     size_t pinit, cinit, sinit, plane_b, cost_b, sel_b, total;
 };
 
-static AcmmWs acmm_ws(int V, int W, int W0, bool two) {
+// (blur: the coarse level's tiles carry its blur tiles after the intensities)
+static AcmmWs acmm_ws(int V, int W, int W0, bool two, bool blur) {
     AcmmWs w{};
     const size_t n1 = (size_t)V * W * W, n0 = two ? (size_t)V * W0 * W0 : 0;
     size_t o = 0;
     auto take = [&](size_t b) { const size_t at = o; o += align_up(b); return at; };
     w.plane_b = take(n1 * 16); w.cost_b = take(n1 * 4); w.sel_b = take(n1 * 4);
     if (two) {
-        w.tiles0 = take(n0 * 4); w.centers0 = take((size_t)V * 8);
+        w.tiles0 = take(n0 * 4 * (blur ? 2 : 1)); w.centers0 = take((size_t)V * 8);
         w.plane0 = take(n0 * 16); w.cost0 = take(n0 * 4); w.sel0 = take(n0 * 4);
         w.disp0 = take(n0 * 16);
         w.plane0b = take(n0 * 16); w.cost0b = take(n0 * 4); w.sel0b = take(n0 * 4);
@@ -3285,7 +3286,7 @@ size_t lfpm_patchmatch_acmm_workspace_size(int num_mis, const lfpm_pm_params* p)
     if (check_pm_params(p, num_mis)) return 0;
     AcmmLevel lv[2];
     const int L = acmm_levels(p->crop, lv);
-    return acmm_ws(num_mis, p->crop, lv[0].width, L == 2).total;
+    return acmm_ws(num_mis, p->crop, lv[0].width, L == 2, p->blur_lambda > 0.0f).total;
 }
 
 int lfpm_patchmatch_frame_acmm(const float* tiles_dev, const float* centers_dev,
@@ -3303,11 +3304,10 @@ int lfpm_patchmatch_frame_acmm(const float* tiles_dev, const float* centers_dev,
         return set_error(LFPM_E_UNSUPPORTED, "acmm: level images are not 8-bit; storage must be 0");
     if (!wave_solver())
         return set_error(LFPM_E_UNSUPPORTED, "acmm: needs the wave solver engine");
-    if (p->blur_lambda > 0.0f)
-        return set_error(LFPM_E_UNSUPPORTED, "acmm: the blur term is not provided for level images");
+    const bool blur = p->blur_lambda > 0.0f;
     AcmmLevel lv[2];
     const int L = acmm_levels(p->crop, lv);
-    const AcmmWs w = acmm_ws(num_mis, p->crop, lv[0].width, L == 2);
+    const AcmmWs w = acmm_ws(num_mis, p->crop, lv[0].width, L == 2, blur);
     if (!workspace_dev || workspace_bytes < w.total)
         return set_error(LFPM_E_INVALID, "acmm: workspace too small (%zu < %zu)", workspace_bytes,
                          w.total);
@@ -3329,6 +3329,12 @@ int lfpm_patchmatch_frame_acmm(const float* tiles_dev, const float* centers_dev,
     const long long n0 = (long long)V * W0 * W0, n1 = (long long)V * W * W;
     hipLaunchKernelGGL(k_area_resize, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s,
                        tiles_dev, V, W, W0, tiles0);
+    // the level's blur images: the MIs' blur maps resized like the intensities
+    // (BuildBlurAlignedPyramids, AdaptMIPMFrameACMM.cpp:338-350; the Gaussian alignment of
+    // :295-302 is switched off in the reference)
+    if (blur)
+        hipLaunchKernelGGL(k_area_resize, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s,
+                           tiles_dev + n1, V, W, W0, tiles0 + n0);
     hipLaunchKernelGGL(k_scale_centers, dim3((2 * V + 255) / 256), dim3(256), 0, s, centers_dev,
                        2 * V, lv[0].scale, centers0);
     if ((rc = launch_check("acmm level images"))) return rc;

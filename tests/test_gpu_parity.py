@@ -318,6 +318,34 @@ def test_solve_acmm_vs_fp64(lfpm, orc, torch_cuda, pitch):
     assert frac >= 0.995
 
 
+@pytest.mark.parametrize("pitch", [23.0, 35.0], ids=["one_level", "two_levels"])
+def test_solve_acmm_blur_vs_fp64(lfpm, orc, torch_cuda, pitch):
+    """ACMM with the blur term (lambda 0.8, the frame's defocus maps; a coarse level reads them
+    area-resized) vs the FP64 oracle: final-level disparities within 0.05 px on >= 99.5 % of
+    pixels."""
+    import copy
+    w, h = (400, 320) if pitch < 30 else (560, 448)
+    wl = lfpm.workload_c1(raw_w=w, raw_h=h, pitch=pitch)
+    pm = copy.copy(wl.pm)
+    pm.blur_lambda = 0.8
+    _say("case acmm blur pitch %g V=%d crop=%d" % (pitch, wl.V, wl.crop))
+    blur = wl.blur_map()
+    df = lfpm.DeviceFrame(wl, blur_raw=blur)
+    bufs = lfpm.patchmatch_frame_acmm(df, pm=pm)
+    torch_cuda.cuda.synchronize()
+    d = bufs.plane[:, 3].cpu().numpy().astype(np.float64)
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    bv = orc.slice_u8(blur, wl.L["origins"], wl.crop).astype(np.float64) / 255.0
+    o = orc.solve_acmm(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm_to_oracle(orc, pm),
+                       blur=bv)
+    err = np.abs(d - o["plane"][:, 3])
+    frac = float((err <= 0.05).mean())
+    st = dict(pitch=pitch, V=wl.V, frac_le_005=frac, median=float(np.median(err)))
+    _say("acmm blur", st)
+    stats_dump("acmm_blur_%g" % pitch, st)
+    assert frac >= 0.995
+
+
 def _blur_inputs(orc, wl, src):
     """(DeviceFrame blur argument, oracle blur values) for a blur source: the synthetic defocus
     map, or the reference's Laplacian blur score of the frame."""
@@ -395,16 +423,14 @@ def test_solve_blur_vs_fp64(lfpm, orc, torch_cuda, src):
 
 
 def test_blur_rejected_where_not_provided(lfpm, torch_cuda):
-    """The blur term needs blur tiles: the ACMM variant, FP16 storage and caller-supplied tiles
-    in the host context (no raw frame to take the Laplacian of) refuse it with
-    LFPM_E_UNSUPPORTED instead of solving without it."""
+    """The blur term needs FP32 blur tiles: FP16 storage and caller-supplied tiles in the host
+    context (no raw frame to take the Laplacian of) refuse it with LFPM_E_UNSUPPORTED instead
+    of solving without it."""
     import copy
     wl = lfpm.workload_c1(raw_w=320, raw_h=256)
     pm = copy.copy(wl.pm)
     pm.blur_lambda = 2.0
     df = lfpm.DeviceFrame(wl, blur_raw=wl.blur_map())
-    with pytest.raises(lfpm.LfpmError):
-        lfpm.patchmatch_frame_acmm(df, pm=pm)
     pm16 = copy.copy(pm)
     pm16.storage = 1
     with pytest.raises(lfpm.LfpmError):
@@ -415,9 +441,6 @@ def test_blur_rejected_where_not_provided(lfpm, torch_cuda):
             np.stack([wl.raw()[y:y + wl.crop, x:x + wl.crop] for x, y in wl.L["origins"]]))
         with pytest.raises(lfpm.LfpmError):
             ctx.solve_tiles(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm)
-        ctx.set_variant(lfpm.Context.ACMM)
-        with pytest.raises(lfpm.LfpmError):
-            ctx.solve_frame(wl, pm=pm)
     finally:
         ctx.close()
 

@@ -99,3 +99,23 @@ def test_acmm_solve(orc, lfpm, pitch):
     assert _support_check(orc, wl, tiles, out, pm) == 0
     err = _gt_err(wl, out)
     assert np.median(err) < 0.25, np.median(err)
+
+
+def test_acmm_blur_term(orc, lfpm):
+    """ACMM with the blur term (lambda 0.8 as the ACMM twin, AdaptMIPMFrameACMM.cu:165) on the
+    two-level schedule: equal blur maps everywhere leave every blur sum difference at zero, so
+    the solve is the one without the term; the frame's defocus maps change it."""
+    import copy
+    wl = lfpm.workload_c1(raw_w=560, raw_h=448, pitch=35.0)
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    pm = pm_to_oracle(orc, wl.pm)
+    base = orc.solve_acmm(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm)
+    pmb = copy.copy(pm)
+    pmb.blur_lambda = 0.8
+    flat = np.full(tiles.shape, 0.3, np.float64)
+    same = orc.solve_acmm(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pmb, blur=flat)
+    for k in ("plane", "cost", "sel"):
+        assert np.allclose(same[k], base[k], rtol=0, atol=1e-12), k
+    blur = orc.slice_u8(wl.blur_map(), wl.L["origins"], wl.crop).astype(np.float64) / 255.0
+    out = orc.solve_acmm(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pmb, blur=blur)
+    assert not np.array_equal(out["plane"], base["plane"])
