@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="batched",
-                    choices=["plain", "batched", "mixed", "serve", "precision"])
+                    choices=["plain", "batched", "mixed", "serve", "precision", "ingest"])
     ap.add_argument("--hypotheses", type=int, default=1000000, help="precision: hypotheses")
     ap.add_argument("--batch", type=int, default=4, help="frames per batch (batched / mixed)")
     ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
@@ -411,6 +411,87 @@ def precision_section(args, lfpm, torch):
     return out
 
 
+def calib_copy(torch):
+    """One 1 GiB device-to-device copy: the known-size kernel that scales the rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def ingest_section(args, lfpm, torch, rw, rh):
+    """Colour-frame ingest: lfpm_bgr_to_gray over rw x rh BGR frames.  Eight input/output pairs
+    (8 x 4 B/px, more than the 256 MB last-level cache for 4080x3068) are converted in turn, so
+    every launch reads its frame from HBM; 64 launches are captured in one graph so that the
+    host's launch rate does not limit the 50 MB launches.  Timed with HIP events around the
+    graph replays on the capture stream."""
+    n_buf, per_graph = 8, 64
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    bgr = [torch.randint(0, 256, (rh, rw, 3), dtype=torch.uint8, device="cuda", generator=gen)
+           for _ in range(n_buf)]
+    gray = [torch.empty((rh, rw), dtype=torch.uint8, device="cuda") for _ in range(n_buf)]
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        for i in range(n_buf):  # warm-up outside the capture (module load, first-touch)
+            lfpm.bgr_to_gray(bgr[i], out=gray[i])
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        for i in range(per_graph):
+            lfpm.bgr_to_gray(bgr[i % n_buf], out=gray[i % n_buf])
+    with torch.cuda.stream(s):
+        for _ in range(max(1, args.warmup)):
+            g.replay()
+    s.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(s):
+        e0.record(s)
+        for _ in range(args.steps):
+            g.replay()
+        e1.record(s)
+    s.synchronize()
+    launches = args.steps * per_graph
+    ms = e0.elapsed_time(e1) / launches
+    # every replayed frame equals a fresh eager conversion
+    check = torch.empty_like(gray[0])
+    lfpm.bgr_to_gray(bgr[0], out=check)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(check, gray[0]))
+    alg_bytes = 4.0 * rw * rh  # 3 B read + 1 B written per pixel
+    gbs = alg_bytes / (ms * 1e-3) / 1e9
+    # CPU baseline: the oracle's numpy rule on one frame (whole frame, one core)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    host = bgr[0].cpu().numpy()
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < 10.0:
+        oracle.bgr_to_gray(host)
+        reps += 1
+    cpu_fps = reps / (time.perf_counter() - t0)
+    traffic = None
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_ingest_4080x3068.md")
+    if os.path.exists(path):
+        for row in open(path):
+            cells = [c.strip() for c in row.strip().strip("|").split("|")]
+            if len(cells) == 5 and "k_bgr_gray" in cells[0]:
+                try:
+                    traffic = float(cells[4])
+                except ValueError:
+                    pass
+    return {"fps": 1e3 / ms, "ms": ms, "launches": launches, "replay_equals_eager": same,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
+                         "frac": gbs / 8000.0, "traffic": traffic, "kernel": "k_bgr_gray",
+                         "avg_launch_ms": ms,
+                         "per_unit": "4 B per pixel (3 B of B, G, R read, 1 B gray written)"},
+            "cpu_baseline": {"value": cpu_fps, "unit": "frames/s", "cores": 1, "kind": "port",
+                             "sample": "oracle rule (numpy, int64) on one whole %dx%d frame, "
+                                       "%d frames in %.0f s" % (rw, rh, reps, 10.0)}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -444,18 +525,25 @@ def main():
                             "requests": args.requests, "hbm_budget_gb": args.hbm_budget_gb})
         print(json.dumps(line), flush=True)
         return
+    if args.mode == "ingest":
+        r = ingest_section(args, lfpm, torch, rw, rh)
+        if args.calib_copy:
+            calib_copy(torch)
+        line.update(metric="colour raw frames converted to gray per s (%dx%d BGR)" % (rw, rh),
+                    value=r["fps"], ms_per_step=r["ms"], dtype="u8", roofline=r["roofline"],
+                    cpu_baseline=r["cpu_baseline"], ingest=r,
+                    config={"workload": "cv::cvtColor(COLOR_BGR2GRAY) of %dx%d colour raw "
+                                        "frames on the device, 8 frames in rotation, 64 launches "
+                                        "per captured graph" % (rw, rh)})
+        print(json.dumps(line), flush=True)
+        return
     log("bench: synthesising %dx%d frame" % (rw, rh))
     wl = lfpm.workload_c2(raw_w=rw, raw_h=rh)
     log("bench: V=%d crop=%d K=%d iters=%d" % (wl.V, wl.crop, wl.pm.max_neighbors,
                                                 wl.pm.iterations))
     plain = plain_section(args, lfpm, torch, wl)
     if args.calib_copy:
-        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calib_copy(torch)
     cfg = {"raw": "%dx%d" % (rw, rh), "pitch": 23.0, "mis": wl.V,
            "neighbors": wl.pm.max_neighbors, "window": 2 * wl.pm.radius + 1,
            "iterations": wl.pm.iterations, "slices": wl.slices, "mode": args.mode}

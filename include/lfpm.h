@@ -156,6 +156,16 @@ typedef struct lfpm_pm_params {
 /* Fill defaults for a given crop/base (reference defaults + config conventions). */
 void lfpm_pm_params_default(lfpm_pm_params* p, int crop, float base);
 
+/* Colour raw frame -> gray raw frame, the input of every slice entry point below.  Replaces the
+ * per-crop cv::cvtColor(problem.m_Image_rgb, problem.m_Image_gray, cv::COLOR_BGR2GRAY) of
+ * DepthSolver::Slice_RawMLAImage (DepthSolver.cpp:1985-1990) on the frame read with
+ * cv::imread(path, 1) (:2503); the conversion is per pixel, so converting the frame and then
+ * slicing gives the same crops.  OpenCV's 8-bit rule: Y = (1868 B + 9617 G + 4899 R + 8192) >> 14.
+ * bgr_dev: height rows of bgr_pitch bytes, pixels B, G, R.  gray_dev: height rows of gray_pitch
+ * bytes.  16-byte aligned rows take the vector path; any pitch >= 3 * width (>= width) works. */
+int lfpm_bgr_to_gray(const uint8_t* bgr_dev, int width, int height, size_t bgr_pitch,
+                     uint8_t* gray_dev, size_t gray_pitch, void* stream);
+
 /* Gather MI crops from a raw uint8 frame into FP32 tiles in [0,1].
  * Replaces DepthSolver::Slice_RawMLAImage (DepthSolver.cpp:1951-1997) and the uint8->float
  * convertTo of AdaptMIPMFrame::BuildViewIndexMap (MIStereo/AdaptMIPMFrame.cpp:173-179).

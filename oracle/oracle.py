@@ -130,6 +130,15 @@ def _blur_values(tiles_u8, pm, blur_u8=None, blur=None):
     return np.ascontiguousarray(blur, np.float64)
 
 
+def bgr_to_gray(bgr):
+    """cv::cvtColor(..., cv::COLOR_BGR2GRAY) on an 8-bit (H, W, 3) B, G, R image, the conversion
+    of the colour MI crops in DepthSolver::Slice_RawMLAImage (DepthSolver.cpp:1985-1990).
+    OpenCV's published 8-bit form of Y = 0.299 R + 0.587 G + 0.114 B: coefficients rounded to
+    14-bit fixed point (4899, 9617, 1868, summing to 2^14), result rounded half up."""
+    c = np.asarray(bgr, np.int64)
+    return ((c[..., 0] * 1868 + c[..., 1] * 9617 + c[..., 2] * 4899 + 8192) >> 14).astype(np.uint8)
+
+
 def laplacian_blur_tiles(raw, origins, crop):
     """Blur tiles of ImageQualityEstimate::QuantizeBlurByLaplacian (Util/ImageQualityEstimate.cpp:
     211-230): cv::Laplacian with ksize 1 (the 4-neighbour stencil) of the gray frame with

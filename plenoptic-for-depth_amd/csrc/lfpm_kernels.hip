@@ -530,6 +530,52 @@ __global__ void k_slice_blur(const uint8_t* __restrict__ raw, size_t pitch, int 
     }
 }
 
+// BGR -> gray of cv::cvtColor(..., cv::COLOR_BGR2GRAY) on 8-bit images, the conversion of each
+// MI crop cut from the colour raw frame (DepthSolver::Slice_RawMLAImage, DepthSolver.cpp:
+// 1985-1990; the frame is read with cv::imread(path, 1), :2503).  The map is per pixel, so the
+// frame is converted once and then sliced.  OpenCV's 8-bit form of 0.114 B + 0.587 G + 0.299 R
+// with 14-bit fixed-point coefficients, rounded half up: Y = (1868 B + 9617 G + 4899 R + 2^13)
+// >> 14 (the coefficients sum to 2^14, so a gray pixel keeps its value).
+__device__ __forceinline__ uint32_t bgr_luma(uint32_t b, uint32_t g, uint32_t r) {
+    return (b * 1868u + g * 9617u + r * 4899u + 8192u) >> 14;
+}
+
+// One thread per 16 pixels of a row: three 16-byte loads and one 16-byte store when both row
+// slices are 16-byte aligned and the 16 pixels lie inside the row, byte accesses otherwise (row
+// tails, pitches that are not multiples of 16).
+__global__ void k_bgr_gray(const uint8_t* __restrict__ bgr, size_t bpitch, int w, int h,
+                           uint8_t* __restrict__ gray, size_t gpitch) {
+    const size_t chunks = (size_t)((w + 15) >> 4);
+    const size_t total = chunks * (size_t)h;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < total;
+         c += (size_t)gridDim.x * blockDim.x) {
+        const size_t y = c / chunks;
+        const int x0 = (int)(c - y * chunks) << 4;
+        const uint8_t* src = bgr + y * bpitch + (size_t)x0 * 3;
+        uint8_t* dst = gray + y * gpitch + (size_t)x0;
+        if (x0 + 16 <= w && (((uintptr_t)src | (uintptr_t)dst) & 15) == 0) {
+            const uint4* s4 = (const uint4*)src;
+            const uint4 q0 = s4[0], q1 = s4[1], q2 = s4[2];
+            const uint32_t wd[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y,
+                                     q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+            uint32_t o[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int p = 0; p < 16; ++p) {
+                const int k = 3 * p;
+                const uint32_t b = (wd[k >> 2] >> ((k & 3) * 8)) & 255u;
+                const uint32_t g = (wd[(k + 1) >> 2] >> (((k + 1) & 3) * 8)) & 255u;
+                const uint32_t r = (wd[(k + 2) >> 2] >> (((k + 2) & 3) * 8)) & 255u;
+                o[p >> 2] |= bgr_luma(b, g, r) << ((p & 3) * 8);
+            }
+            *(uint4*)dst = make_uint4(o[0], o[1], o[2], o[3]);
+        } else {
+            const int n = min(16, w - x0);
+            for (int p = 0; p < n; ++p)
+                dst[p] = (uint8_t)bgr_luma(src[3 * p], src[3 * p + 1], src[3 * p + 2]);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Frame table.  One persistent launch serves up to kMaxFrames frames of one parameter class
 // (crop, neighbours, window), each at its own stage: random init (RandomInitializationForFrame,
@@ -2926,6 +2972,19 @@ int lfpm_slice_blur_tiles(const uint8_t* raw_dev, int raw_w, int raw_h, size_t r
                        raw_dev, raw_pitch, raw_w, raw_h, (const int2*)origins_dev, num_mis, crop,
                        blur_tiles_dev);
     return launch_check("slice_blur");
+}
+
+int lfpm_bgr_to_gray(const uint8_t* bgr_dev, int width, int height, size_t bgr_pitch,
+                     uint8_t* gray_dev, size_t gray_pitch, void* stream) {
+    if (!bgr_dev || !gray_dev || width < 1 || height < 1 || bgr_pitch < (size_t)width * 3 ||
+        gray_pitch < (size_t)width)
+        return set_error(LFPM_E_INVALID, "bgr_to_gray: bad arguments");
+    const size_t total = (size_t)((width + 15) >> 4) * (size_t)height;
+    const int threads = 256;
+    const int blocks = (int)std::min<size_t>((total + threads - 1) / threads, 65536);
+    hipLaunchKernelGGL(k_bgr_gray, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, bgr_dev,
+                       bgr_pitch, width, height, gray_dev, gray_pitch);
+    return launch_check("bgr_to_gray");
 }
 
 size_t lfpm_patchmatch_workspace_size(int num_mis, const lfpm_pm_params* p) {

@@ -293,6 +293,26 @@ class DeviceFrame:
                                                _dp(self.tiles[L["V"]:]), _stream(stream)))
 
 
+def bgr_to_gray(bgr, out=None, stream=None):
+    """Colour raw frame (H, W, 3) uint8 B, G, R on the device -> gray frame (H, W) uint8
+    (lfpm_bgr_to_gray: cv::cvtColor(..., COLOR_BGR2GRAY) of DepthSolver.cpp:1985-1990).  Row
+    pitches come from the tensors' strides."""
+    import torch
+    if bgr.dim() != 3 or bgr.shape[2] != 3 or bgr.dtype != torch.uint8 or \
+            bgr.stride(2) != 1 or bgr.stride(1) != 3:
+        raise ValueError("bgr_to_gray: expects (H, W, 3) uint8 with packed pixels")
+    h, w = bgr.shape[0], bgr.shape[1]
+    if out is None:
+        out = torch.empty((h, w), dtype=torch.uint8, device=bgr.device)
+    if out.shape != (h, w) or out.dtype != torch.uint8 or out.stride(1) != 1:
+        raise ValueError("bgr_to_gray: out must be (H, W) uint8 with unit column stride")
+    # row-pitched views are allowed: only the pixels of a row must be packed
+    _check(lib().lfpm_bgr_to_gray(C.c_void_p(bgr.data_ptr()), w, h, C.c_size_t(bgr.stride(0)),
+                                  C.c_void_p(out.data_ptr()), C.c_size_t(out.stride(0)),
+                                  _stream(stream)))
+    return out
+
+
 class SolveBuffers:
     def __init__(self, V, W, pm, device="cuda"):
         import torch
