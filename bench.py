@@ -43,7 +43,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="batched",
-                    choices=["plain", "batched", "mixed", "serve", "precision", "ingest"])
+                    choices=["plain", "batched", "mixed", "serve", "precision", "ingest",
+                             "products"])
     ap.add_argument("--hypotheses", type=int, default=1000000, help="precision: hypotheses")
     ap.add_argument("--batch", type=int, default=4, help="frames per batch (batched / mixed)")
     ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
@@ -411,6 +412,90 @@ def precision_section(args, lfpm, torch):
     return out
 
 
+def products_section(args, lfpm, torch, wl):
+    """Per-frame products of one solved frame: disparity mosaic, VD map with its gray colour
+    image, and the VD point cloud (PLY records).  The frame is solved once; each step runs the
+    three products on the launch stream, timed with HIP events per product (no host sync
+    inside a step)."""
+    import ctypes as C
+    df = lfpm.DeviceFrame(wl)
+    bufs = lfpm.patchmatch_frame(df)
+    torch.cuda.synchronize()
+    h, w = df.raw.shape
+    N = h * w
+    ws = lfpm.products_workspace(df)
+    vd = torch.empty((h, w), dtype=torch.float32, device="cuda")
+    col = torch.empty((h, w), dtype=torch.uint8, device="cuda")
+    mos = torch.empty((h, w), dtype=torch.float32, device="cuda")
+    rng = torch.empty(2, dtype=torch.float32, device="cuda")
+    recs = torch.empty(N * 27, dtype=torch.uint8, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    L, st, dp = lfpm.lib(), lfpm._stream(), lfpm._dp
+    V, W = wl.V, wl.crop
+    prm = (C.c_float(W * 0.5), C.c_float(W * 0.5), C.c_float(0.0055), C.c_float(35.0),
+           C.c_float(0.8), C.c_float(2.5))
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    def step():
+        e0 = ev()
+        lfpm._check(L.lfpm_disparity_mosaic(dp(bufs.plane), dp(bufs.cost), dp(df.centers), V, W,
+                                            C.c_float(0.2), w, h, dp(mos), dp(rng), dp(ws),
+                                            C.c_size_t(ws.numel()), st))
+        e1 = ev()
+        lfpm._check(L.lfpm_virtual_depth_map(dp(bufs.plane), dp(df.centers), dp(df.gray), V, W,
+                                             C.c_float(wl.L["pitch"]), w, h, dp(vd), dp(col),
+                                             dp(rng), dp(ws), C.c_size_t(ws.numel()), st))
+        e2 = ev()
+        lfpm._check(L.lfpm_vd_point_cloud(dp(vd), dp(col), w, h, *prm, dp(recs),
+                                          C.c_size_t(recs.numel()), dp(cnt), dp(ws),
+                                          C.c_size_t(ws.numel()), st))
+        e3 = ev()
+        return e0, e1, e2, e3
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    marks = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        marks.append(step())
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    ms = {k: float(np.mean([m[i].elapsed_time(m[i + 1]) for m in marks]))
+          for i, k in enumerate(("mosaic", "vd_map", "point_cloud"))}
+    n = int(cnt.item())
+    pc_bytes = 5.0 * N + 27.0 * n  # vd + colour read, records written
+    gbs = pc_bytes / (ms["point_cloud"] * 1e-3) / 1e9
+    # CPU baseline: the oracle's three products of the same solved frame
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    p = bufs.plane.cpu().numpy()
+    c = bufs.cost.cpu().numpy()
+    g = df.gray.cpu().numpy().reshape(-1)
+    t1 = time.perf_counter()
+    oracle.disparity_mosaic(p, c, wl.L["centers"], W, 0.2, w, h)
+    ovd, ocol, _ = oracle.vd_map(p, wl.L["centers"], g, W, wl.L["pitch"], w, h)
+    orec = oracle.vd_point_cloud(ovd, ocol, (W * 0.5, W * 0.5), 0.0055, 35.0, 0.8, 2.5)
+    cpu_s = time.perf_counter() - t1
+    same = bool(np.array_equal(recs[:n * 27].cpu().numpy().reshape(-1, 27), orec))
+    return {"fps": 1e3 / sum(ms.values()), "ms": ms, "wall_ms": 1e3 * wall, "points": n,
+            "records_equal_oracle": same,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
+                         "frac": gbs / 8000.0, "traffic": None,
+                         "kernel": "lfpm_vd_point_cloud (flag, scan, emit)",
+                         "avg_launch_ms": ms["point_cloud"],
+                         "per_unit": "5 B per raw pixel read (vd, colour) + 27 B per point "
+                                     "written"},
+            "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "frames/s", "cores": 1,
+                             "kind": "port",
+                             "sample": "oracle mosaic + VD map + point cloud of the same solved "
+                                       "%dx%d frame, one frame in %.1f s" % (w, h, cpu_s)}}
+
+
 def calib_copy(torch):
     """One 1 GiB device-to-device copy: the known-size kernel that scales the rocprofv3
     FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -535,6 +620,18 @@ def main():
                     config={"workload": "cv::cvtColor(COLOR_BGR2GRAY) of %dx%d colour raw "
                                         "frames on the device, 8 frames in rotation, 64 launches "
                                         "per captured graph" % (rw, rh)})
+        print(json.dumps(line), flush=True)
+        return
+    if args.mode == "products":
+        wl = lfpm.workload_c2(raw_w=rw, raw_h=rh)
+        r = products_section(args, lfpm, torch, wl)
+        line.update(metric="per-frame products per s (%dx%d: mosaic, VD map, point cloud)"
+                    % (rw, rh), value=r["fps"], ms_per_step=sum(r["ms"].values()),
+                    dtype="fp32", roofline=r["roofline"], cpu_baseline=r["cpu_baseline"],
+                    products=r,
+                    config={"workload": "disparity mosaic + VD median splat with gray colour "
+                                        "image + VD point cloud of one solved %dx%d frame "
+                                        "(%d MIs)" % (rw, rh, wl.V)})
         print(json.dumps(line), flush=True)
         return
     log("bench: synthesising %dx%d frame" % (rw, rh))

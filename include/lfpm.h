@@ -357,6 +357,36 @@ int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, con
                            float* vd_dev, uint8_t* color_dev, float* range_dev,
                            void* workspace_dev, size_t workspace_bytes, void* stream);
 
+/* VD point cloud of DepthSolver::Virtual_depth_map_TileKeySequence (DepthSolver.cpp:8019-8053),
+ * built on the device as the binary payload of StoreColorPlyFileBinaryPointCloud
+ * (MVStereo/ACMP.cpp:451-500).  One 27-byte record {x, y, z, nx, ny, nz: float32 LE; red, green,
+ * blue: uint8} per raw pixel with vd_dev > 0 (a pixel that received a VD sample), in row-major
+ * pixel order (the order the reference collects them in; its OpenMP writer interleaves them in an
+ * unspecified order).  With vdm = vd_dev[y][x]:
+ *   tx = (x - origin_x) * pixel_size, ty = (y - origin_y) * pixel_size     (FP64)
+ *   tz = vdm * B + bl0                       (FP32, two roundings: the float line of the reference)
+ *   z = f / (tz - f) * tz, x = -f / (tz - f) * tx, y = -f / (tz - f) * ty  (FP64, then FP32)
+ * The reference passes origin = (crop / 2, crop / 2) (mi_width/height_for_match * 0.5), its
+ * sensor_pixel_size, mainlen_flocal_length and the calibration's B = d, bl0 = D (g_B, g_bl0).
+ * Points with a non-finite x, y or z are written as (0, 0, 0), the writer's test; the normal is
+ * (0, 0, 0) (the reference leaves it unset); the colour is color_dev's gray (NULL: 0) on all
+ * three channels.  records_dev: records_bytes >= 27 * raw_w * raw_h.  count_dev: one uint32, the
+ * number of records.  workspace: lfpm_products_workspace_size() bytes. */
+int lfpm_vd_point_cloud(const float* vd_dev, const uint8_t* color_dev, int raw_w, int raw_h,
+                        float origin_x, float origin_y, float pixel_size, float main_lens_f,
+                        float B, float bl0, uint8_t* records_dev, size_t records_bytes,
+                        unsigned* count_dev, void* workspace_dev, size_t workspace_bytes,
+                        void* stream);
+
+/* Host file writers.  lfpm_write_ply_points writes the header of StoreColorPlyFileBinaryPointCloud
+ * (MVStereo/ACMP.cpp:458-470) and count records of lfpm_vd_point_cloud (copied to the host).
+ * lfpm_write_dmb / lfpm_read_dmb are writeDepthDmb / readDepthDmb (MVStereo/ACMP.cpp:338-390):
+ * int32 {1, height, width, 1} then height*width float32, native (little-endian) order; reading
+ * rejects a type other than 1.  lfpm_read_dmb with data == NULL returns the size only. */
+int lfpm_write_ply_points(const char* path, const uint8_t* records, uint64_t count);
+int lfpm_write_dmb(const char* path, const float* data, int height, int width);
+int lfpm_read_dmb(const char* path, float* data, size_t capacity, int* height, int* width);
+
 /* ---------------------------------------------------------------------------------------
  * Virtual-to-real depth behavioural model, FP64 on the GPU.
  * lfpm_fit_v2r replaces VirtualToRealDepthFunc::BehavioralModel

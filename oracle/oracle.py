@@ -139,6 +139,41 @@ def bgr_to_gray(bgr):
     return ((c[..., 0] * 1868 + c[..., 1] * 9617 + c[..., 2] * 4899 + 8192) >> 14).astype(np.uint8)
 
 
+def vd_point_cloud(vd, color, origin, pixel_size, f, B, bl0):
+    """Records of the VD point cloud (DepthSolver::Virtual_depth_map_TileKeySequence,
+    DepthSolver.cpp:8019-8053) in the binary layout of StoreColorPlyFileBinaryPointCloud
+    (MVStereo/ACMP.cpp:451-500): bytes (n, 27) = x, y, z, nx, ny, nz float32 LE + red, green,
+    blue, for the pixels with vd > 0 in row-major order.  The image-plane point is FP64, the
+    virtual-depth line vdm * B + bl0 FP32 with two roundings, the thin-lens map FP64; non-finite
+    points are written as 0 as the writer does; normals 0; colour = the gray value."""
+    vd = np.asarray(vd, np.float32)
+    ys, xs = np.nonzero(vd > 0)
+    vdm = vd[ys, xs]
+    f32 = np.float32
+    tz = ((vdm * f32(B)).astype(f32) + f32(bl0)).astype(f32).astype(np.float64)
+    tx = (xs.astype(np.float64) - float(np.float32(origin[0]))) * float(np.float32(pixel_size))
+    ty = (ys.astype(np.float64) - float(np.float32(origin[1]))) * float(np.float32(pixel_size))
+    fd = float(np.float32(f))
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        z = (fd / (tz - fd) * tz).astype(f32)
+        x = (-fd / (tz - fd) * tx).astype(f32)
+        y = (-fd / (tz - fd) * ty).astype(f32)
+    fmax = np.finfo(np.float32).max
+    bad = ~((x < fmax) & (x > -fmax)) | ~((y < fmax) & (y > -fmax)) | ~((z < fmax) & (z >= -fmax))
+    x[bad] = 0
+    y[bad] = 0
+    z[bad] = 0
+    n = len(vdm)
+    rec = np.zeros((n, 27), np.uint8)
+    xyz = np.stack([x, y, z, np.zeros(n, f32), np.zeros(n, f32), np.zeros(n, f32)], axis=1)
+    rec[:, :24] = np.ascontiguousarray(xyz.astype("<f4")).view(np.uint8).reshape(n, 24)
+    g = np.zeros(n, np.uint8) if color is None else np.asarray(color, np.uint8)[ys, xs]
+    rec[:, 24] = g
+    rec[:, 25] = g
+    rec[:, 26] = g
+    return rec
+
+
 def laplacian_blur_tiles(raw, origins, crop):
     """Blur tiles of ImageQualityEstimate::QuantizeBlurByLaplacian (Util/ImageQualityEstimate.cpp:
     211-230): cv::Laplacian with ksize 1 (the 4-neighbour stencil) of the gray frame with

@@ -460,4 +460,69 @@ int lfpm_synth_frame(const lfpm_synth_params* sp, uint8_t* raw, float* gt_v) {
     return LFPM_OK;
 }
 
+// ------------------------------------------------------------------------------------------
+// File writers of the per-frame products (host side; the payloads are built on the device)
+// ------------------------------------------------------------------------------------------
+int lfpm_write_ply_points(const char* path, const uint8_t* records, uint64_t count) {
+    if (!path || (count && !records)) return lfpm::set_error(LFPM_E_INVALID, "write_ply: null");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return lfpm::set_error(LFPM_E_INVALID, "write_ply: cannot open %s", path);
+    // the header of StoreColorPlyFileBinaryPointCloud (MVStereo/ACMP.cpp:458-470)
+    std::fprintf(f, "ply\nformat binary_little_endian 1.0\nelement vertex %llu\n",
+                 (unsigned long long)count);
+    static const char* const props[] = {"float x", "float y", "float z", "float nx",
+                                        "float ny", "float nz", "uchar red", "uchar green",
+                                        "uchar blue"};
+    for (const char* q : props) std::fprintf(f, "property %s\n", q);
+    std::fprintf(f, "end_header\n");
+    const size_t n = (size_t)count * 27;
+    const bool ok = n == 0 || std::fwrite(records, 1, n, f) == n;
+    if (std::fclose(f) != 0 || !ok)
+        return lfpm::set_error(LFPM_E_INVALID, "write_ply: short write to %s", path);
+    return LFPM_OK;
+}
+
+int lfpm_write_dmb(const char* path, const float* data, int height, int width) {
+    if (!path || !data || height < 0 || width < 0)
+        return lfpm::set_error(LFPM_E_INVALID, "write_dmb: bad arguments");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return lfpm::set_error(LFPM_E_INVALID, "write_dmb: cannot open %s", path);
+    // writeDepthDmb (MVStereo/ACMP.cpp:367-390): int32 type = 1, rows, cols, bands = 1, floats
+    const int32_t hdr[4] = {1, height, width, 1};
+    const size_t n = (size_t)height * width;
+    bool ok = std::fwrite(hdr, sizeof(int32_t), 4, f) == 4;
+    ok = ok && (n == 0 || std::fwrite(data, sizeof(float), n, f) == n);
+    if (std::fclose(f) != 0 || !ok)
+        return lfpm::set_error(LFPM_E_INVALID, "write_dmb: short write to %s", path);
+    return LFPM_OK;
+}
+
+int lfpm_read_dmb(const char* path, float* data, size_t capacity, int* height, int* width) {
+    if (!path || !height || !width) return lfpm::set_error(LFPM_E_INVALID, "read_dmb: null");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return lfpm::set_error(LFPM_E_INVALID, "read_dmb: cannot open %s", path);
+    // readDepthDmb (MVStereo/ACMP.cpp:338-365): type must be 1; h * w * bands floats follow
+    int32_t hdr[4] = {-1, 0, 0, 0};
+    if (std::fread(hdr, sizeof(int32_t), 4, f) != 4 || hdr[0] != 1 || hdr[1] < 0 || hdr[2] < 0 ||
+        hdr[3] < 1) {
+        std::fclose(f);
+        return lfpm::set_error(LFPM_E_INVALID, "read_dmb: %s is not a type-1 dmb file", path);
+    }
+    *height = hdr[1];
+    *width = hdr[2];
+    const size_t n = (size_t)hdr[1] * hdr[2] * hdr[3];
+    if (!data) {  // size query
+        std::fclose(f);
+        return LFPM_OK;
+    }
+    if (capacity < n) {
+        std::fclose(f);
+        return lfpm::set_error(LFPM_E_INVALID, "read_dmb: capacity %zu < %zu floats", capacity, n);
+    }
+    const bool ok = n == 0 || std::fread(data, sizeof(float), n, f) == n;
+    std::fclose(f);
+    if (!ok) return lfpm::set_error(LFPM_E_INVALID, "read_dmb: %s is truncated", path);
+    return LFPM_OK;
+}
+
 } // extern "C"

@@ -451,6 +451,60 @@ int grid_for(long long n, int threads) {
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// VD point cloud (DepthSolver::Virtual_depth_map_TileKeySequence, DepthSolver.cpp:8019-8053) as
+// the binary payload of StoreColorPlyFileBinaryPointCloud (MVStereo/ACMP.cpp:451-500): one
+// 27-byte record {x, y, z, nx, ny, nz (float32), red, green, blue (uint8)} per raw pixel with a
+// VD sample, in row-major pixel order.  Flag pass, exclusive scan, emit pass.
+__global__ void __launch_bounds__(256) k_pt_flag(const float* __restrict__ vd, long long N,
+                                                 int* __restrict__ flag) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i > N) return;
+    flag[i] = (i < N && vd[i] > 0.0f) ? 1 : 0;
+}
+
+__device__ __forceinline__ bool ply_open(float v) { return v < FLT_MAX && v > -FLT_MAX; }
+
+__global__ void __launch_bounds__(256) k_pt_emit(const float* __restrict__ vd,
+                                                 const uint8_t* __restrict__ color,
+                                                 const int* __restrict__ offs, int raw_w,
+                                                 long long N, float ox, float oy, float pix,
+                                                 float f, float B, float bl0,
+                                                 uint8_t* __restrict__ rec,
+                                                 unsigned* __restrict__ count) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i == 0) *count = (unsigned)offs[N];
+    if (i >= N) return;
+    const float vdm = vd[i];
+    if (!(vdm > 0.0f)) return;
+    const int y = (int)(i / raw_w), x = (int)(i - (long long)y * raw_w);
+    // the reference's expression order: image-plane point in FP64, the virtual depth mapped by
+    // the FP32 line vdm * B + bl0 (two roundings, no contraction), then the thin-lens map
+    const double tx = ((double)x - (double)ox) * (double)pix;
+    const double ty = ((double)y - (double)oy) * (double)pix;
+    float tzf;
+    {
+#pragma clang fp contract(off)
+        tzf = vdm * B + bl0;
+    }
+    const double tz = (double)tzf;
+    const double fd = (double)f;
+    float X = (float)(fd / (tz - fd) * tz);
+    float Xx = (float)(-fd / (tz - fd) * tx);
+    float Xy = (float)(-fd / (tz - fd) * ty);
+    // the writer's finiteness test (z admits -FLT_MAX, x and y do not)
+    if (!ply_open(Xx) || !ply_open(Xy) || !(X < FLT_MAX && X >= -FLT_MAX)) Xx = Xy = X = 0.0f;
+    const uint8_t g = color ? color[i] : (uint8_t)0;
+    uint8_t r[27];
+    const float v[6] = {Xx, Xy, X, 0.0f, 0.0f, 0.0f};
+    __builtin_memcpy(r, v, 24);
+    r[24] = g;  // red, green, blue of the gray colour image
+    r[25] = g;
+    r[26] = g;
+    uint8_t* dst = rec + (size_t)offs[i] * 27;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) dst[k] = r[k];
+}
+
 struct VdLayout {
     size_t count, offs, sums, sv, sc, nr, pr, nd, pu, rfirst, rlast, nrf, prl, col, range, total;
 };
@@ -599,6 +653,39 @@ int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, con
         }
     }
     return check_launch("lfpm_virtual_depth_map");
+}
+
+int lfpm_vd_point_cloud(const float* vd_dev, const uint8_t* color_dev, int raw_w, int raw_h,
+                        float origin_x, float origin_y, float pixel_size, float main_lens_f,
+                        float B, float bl0, uint8_t* records_dev, size_t records_bytes,
+                        unsigned* count_dev, void* workspace_dev, size_t workspace_bytes,
+                        void* stream) {
+    if (raw_w <= 0 || raw_h <= 0 || (long long)raw_w * raw_h + 1 >= 0x7fffffffLL)
+        return set_error(LFPM_E_INVALID, "lfpm_vd_point_cloud: bad frame %dx%d", raw_w, raw_h);
+    if (!vd_dev || !records_dev || !count_dev || !workspace_dev)
+        return set_error(LFPM_E_INVALID, "lfpm_vd_point_cloud: null pointer");
+    const long long N = (long long)raw_w * raw_h;
+    if (records_bytes < (size_t)N * 27)
+        return set_error(LFPM_E_INVALID, "lfpm_vd_point_cloud: records %zu < %zu bytes",
+                         records_bytes, (size_t)N * 27);
+    const size_t nb = (size_t)((N + 1 + kScanChunk - 1) / kScanChunk);
+    const size_t need = 2 * al((size_t)(N + 1) * 4) + al(nb * 4);
+    if (workspace_bytes < need)
+        return set_error(LFPM_E_INVALID, "lfpm_vd_point_cloud: workspace %zu < %zu bytes",
+                         workspace_bytes, need);
+    hipStream_t s = (hipStream_t)stream;
+    unsigned char* ws = (unsigned char*)workspace_dev;
+    int* flag = (int*)ws;
+    int* offs = (int*)(ws + al((size_t)(N + 1) * 4));
+    int* sums = (int*)(ws + 2 * al((size_t)(N + 1) * 4));
+    k_pt_flag<<<grid_for(N + 1, 256), 256, 0, s>>>(vd_dev, N, flag);
+    k_scan_sums<<<(int)nb, kScanThreads, 0, s>>>(flag, N + 1, sums);
+    k_scan_sums_scan<<<1, kScanThreads, 0, s>>>(sums, (int)nb);
+    k_scan_chunks<<<(int)nb, kScanThreads, 0, s>>>(flag, N + 1, sums, offs);
+    k_pt_emit<<<grid_for(N, 256), 256, 0, s>>>(vd_dev, color_dev, offs, raw_w, N, origin_x,
+                                                origin_y, pixel_size, main_lens_f, B, bl0,
+                                                records_dev, count_dev);
+    return check_launch("lfpm_vd_point_cloud");
 }
 
 }  // extern "C"

@@ -313,6 +313,52 @@ def bgr_to_gray(bgr, out=None, stream=None):
     return out
 
 
+def vd_point_cloud(vd, color=None, origin=(8.0, 8.0), pixel_size=1.0, main_lens_f=1.0, B=1.0,
+                   bl0=0.0, ws=None, stream=None):
+    """VD point cloud on the device (lfpm_vd_point_cloud): (records uint8 (n, 27), n).  vd and
+    color are the (raw_h, raw_w) outputs of virtual_depth_map; origin is the reference's
+    (crop / 2, crop / 2)."""
+    import torch
+    h, w = vd.shape
+    recs = torch.empty((h * w, 27), dtype=torch.uint8, device=vd.device)
+    cnt = torch.zeros(1, dtype=torch.int32, device=vd.device)
+    if ws is None:
+        n = h * w + 1
+        ws = torch.empty(2 * ((n * 4 + 255) // 256 * 256) + (n // 4096 + 2) * 4 + 512,
+                         dtype=torch.uint8, device=vd.device)
+    _check(lib().lfpm_vd_point_cloud(
+        _dp(vd), _dp(color) if color is not None else None, w, h, C.c_float(origin[0]),
+        C.c_float(origin[1]), C.c_float(pixel_size), C.c_float(main_lens_f), C.c_float(B),
+        C.c_float(bl0), _dp(recs), C.c_size_t(recs.numel()), _dp(cnt), _dp(ws),
+        C.c_size_t(ws.numel()), _stream(stream)))
+    n = int(cnt.item())
+    return recs[:n], n
+
+
+def write_ply_points(path, records):
+    """lfpm_write_ply_points: the reference's PLY header + the (n, 27) records (host bytes)."""
+    r = np.ascontiguousarray(records, np.uint8)
+    _check(lib().lfpm_write_ply_points(str(path).encode(), r.ctypes.data_as(C.c_void_p),
+                                       C.c_uint64(r.shape[0] if r.ndim == 2 else 0)))
+
+
+def write_dmb(path, data):
+    """lfpm_write_dmb: int32 {1, h, w, 1} + h*w float32 (writeDepthDmb)."""
+    a = np.ascontiguousarray(data, np.float32)
+    _check(lib().lfpm_write_dmb(str(path).encode(), a.ctypes.data_as(C.c_void_p),
+                                a.shape[0], a.shape[1]))
+
+
+def read_dmb(path):
+    """lfpm_read_dmb: the (h, w) float32 map of a type-1 dmb file (readDepthDmb)."""
+    h, w = C.c_int(), C.c_int()
+    _check(lib().lfpm_read_dmb(str(path).encode(), None, C.c_size_t(0), C.byref(h), C.byref(w)))
+    out = np.empty((h.value, w.value), np.float32)
+    _check(lib().lfpm_read_dmb(str(path).encode(), out.ctypes.data_as(C.c_void_p),
+                               C.c_size_t(out.size), C.byref(h), C.byref(w)))
+    return out
+
+
 class SolveBuffers:
     def __init__(self, V, W, pm, device="cuda"):
         import torch
