@@ -15,6 +15,7 @@
 
 #include "../../include/lfpm.h"
 #include "lfpm_internal.h"
+#include "lfpm_median25.h"
 
 namespace lfpm {
 namespace {
@@ -409,37 +410,37 @@ __global__ void __launch_bounds__(256) k_fill_resolve(const int* __restrict__ of
     out[p] = best_k >= 0 ? colm[best_k] : (uint8_t)0;
 }
 
-// 5x5 median with replicated borders (cv::medianBlur's documented border rule).
+// 5x5 median with replicated borders (cv::medianBlur's documented border rule).  One lane per
+// two horizontally adjacent pixels: their 25 window values are packed into the 16-bit halves of
+// one word, and the median-of-25 selection network runs on packed 16-bit min / max.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
 __global__ void __launch_bounds__(256) k_median5(const uint8_t* __restrict__ in, int raw_w,
                                                  int raw_h, uint8_t* __restrict__ out) {
-    const long long N = (long long)raw_w * raw_h;
-    const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (p >= N) return;
-    const int y = (int)(p / raw_w), x = (int)(p - (long long)y * raw_w);
-    int v[25];
+    const int pairs = (raw_w + 1) >> 1;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)pairs * raw_h) return;
+    const int y = (int)(t / pairs);
+    const int x0 = (int)(t - (long long)y * pairs) * 2;
+    int cx[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) cx[i] = min(max(x0 + i - 2, 0), raw_w - 1);
+    u16x2 v[25];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
-        const int yy = min(max(y + j - 2, 0), raw_h - 1);
+        const uint8_t* row = in + (long long)min(max(y + j - 2, 0), raw_h - 1) * raw_w;
+        unsigned short c[6];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            const int xx = min(max(x + i - 2, 0), raw_w - 1);
-            v[j * 5 + i] = in[(long long)yy * raw_w + xx];
-        }
+        for (int i = 0; i < 6; ++i) c[i] = row[cx[i]];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) v[j * 5 + i] = u16x2{c[i], c[i + 1]};
     }
-    // rank selection: the median is the value with 12 smaller-or-equal predecessors in a
-    // stable order, i.e. the 13th smallest
-    int med = 0;
-#pragma unroll
-    for (int a = 0; a < 25; ++a) {
-        int lt = 0, le = 0;
-#pragma unroll
-        for (int b = 0; b < 25; ++b) {
-            lt += v[b] < v[a];
-            le += v[b] <= v[a];
-        }
-        if (lt <= 12 && le > 12) med = v[a];
-    }
-    out[p] = (uint8_t)med;
+    const u16x2 m = median25(
+        v, [](u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); },
+        [](u16x2 a, u16x2 b) { return __builtin_elementwise_max(a, b); });
+    uint8_t* o = out + (long long)y * raw_w + x0;
+    o[0] = (uint8_t)m.x;
+    if (x0 + 1 < raw_w) o[1] = (uint8_t)m.y;
 }
 
 int grid_for(long long n, int threads) {
@@ -647,9 +648,12 @@ int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, con
         k_fill_resolve<<<grid_for(N, 256), 256, 0, s>>>(offs, raw_w, raw_h, nr, pr, nrf, prl, nd,
                                                         pu, colm, color_dev);
         uint8_t* tmp = (uint8_t*)nr;
+        const long long npairs = (long long)((raw_w + 1) >> 1) * raw_h;
         for (int it = 0; it < 10; ++it) {
-            if (it & 1) k_median5<<<grid_for(N, 256), 256, 0, s>>>(tmp, raw_w, raw_h, color_dev);
-            else k_median5<<<grid_for(N, 256), 256, 0, s>>>(color_dev, raw_w, raw_h, tmp);
+            if (it & 1)
+                k_median5<<<grid_for(npairs, 256), 256, 0, s>>>(tmp, raw_w, raw_h, color_dev);
+            else
+                k_median5<<<grid_for(npairs, 256), 256, 0, s>>>(color_dev, raw_w, raw_h, tmp);
         }
     }
     return check_launch("lfpm_virtual_depth_map");
