@@ -2586,25 +2586,36 @@ __global__ void __launch_bounds__(256) k_refocus(const float* __restrict__ tiles
     const float vk = S > 1 ? vmin + (vmax - vmin) * (float)k / (float)(S - 1) : vmin;
     const float halff = (float)(W - 1) * 0.5f;
     const float reach = vk * halff + 1.0f;
+    // a / vk for the slice's fixed vk as a product with RN(1 / vk) and one fused correction:
+    // the correctly rounded quotient (Markstein's theorem: y = RN(1/b), q0 within an ulp of
+    // a/b, exact residual a - b q0 by FMA, then RN(q0 + r y)), the value of the division the
+    // inclusion test below compares
+    auto quot = [](float a, float b, float rb) {
+        const float q0 = a * rb;
+        return fmaf(fmaf(-q0, b, a), rb, q0);
+    };
+    const float rvk = 1.0f / vk;
+    auto over_vk = [&](float a) { return quot(a, vk, rvk); };
     const float rs = pitch * 0.8660254037844386f;
     // lattice rows/columns whose centre can lie within `reach` (the snapped centre is within
     // 0.5 px of the lattice point, and reach carries 1 px of slack); the inclusion test below
     // decides, so any superset of these cells gives the same sums in the same order
-    const int j0 = max(0, (int)floorf((y - reach - 0.5f * pitch) / rs));
-    const int j1 = min(rows - 1, (int)ceilf((y + reach - 0.5f * pitch) / rs));
+    const float rrs = 1.0f / rs, rpitch = 1.0f / pitch;
+    const int j0 = max(0, (int)floorf(quot(y - reach - 0.5f * pitch, rs, rrs)));
+    const int j1 = min(rows - 1, (int)ceilf(quot(y + reach - 0.5f * pitch, rs, rrs)));
     const int WW = W * W;
     float acc = 0.0f;
     int nacc = 0;
     for (int j = j0; j <= j1; ++j) {
         const float off = 0.5f * pitch * (float)(1 + (j & 1));
-        const int i0 = max(0, (int)floorf((x - reach - off) / pitch));
-        const int i1 = min(cols - 1, (int)ceilf((x + reach - off) / pitch));
+        const int i0 = max(0, (int)floorf(quot(x - reach - off, pitch, rpitch)));
+        const int i1 = min(cols - 1, (int)ceilf(quot(x + reach - off, pitch, rpitch)));
         for (int i = i0; i <= i1; ++i) {
             const int v = l2m[j * cols + i];
             if (v < 0) continue;
             const float2 c = centers[v];
-            const float ux = ((float)x - c.x) / vk + halff;
-            const float uy = ((float)y - c.y) / vk + halff;
+            const float ux = over_vk((float)x - c.x) + halff;
+            const float uy = over_vk((float)y - c.y) + halff;
             if (!(ux >= 0.0f && ux <= (float)(W - 1) && uy >= 0.0f && uy <= (float)(W - 1)))
                 continue;
             const float fx0 = floorf(ux), fy0 = floorf(uy);
