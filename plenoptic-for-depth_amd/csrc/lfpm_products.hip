@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <climits>
 #include <cmath>
 
 #include "../../include/lfpm.h"
@@ -295,25 +296,56 @@ __global__ void __launch_bounds__(256) k_vd_median(const int* __restrict__ offs,
 
 // Search tables of the colour fill (thread per raw row): in-row next / previous pixel with
 // samples, and each row's first / last such pixel.
-__global__ void k_fill_rows(const int* __restrict__ offs, int raw_w, int raw_h,
-                            int* __restrict__ nr, int* __restrict__ pr, int* __restrict__ rfirst,
-                            int* __restrict__ rlast) {
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y >= raw_h) return;
+// Row links: nr[p] / pr[p] = the next / previous raw pixel with samples in p's row (-1 if
+// none), rfirst / rlast = the row's first / last such pixel.  One workgroup per row: each lane
+// scans a contiguous segment; the carries across segments come from a workgroup min / max scan.
+constexpr int kFillThreads = 256;
+constexpr int kFillSeg = 64;  // rows (columns) or steps (diagonals) per segment
+
+__global__ void __launch_bounds__(kFillThreads) k_fill_rows(const int* __restrict__ offs,
+                                                            int raw_w, int* __restrict__ nr,
+                                                            int* __restrict__ pr,
+                                                            int* __restrict__ rfirst,
+                                                            int* __restrict__ rlast) {
+    __shared__ int sf[kFillThreads], sl[kFillThreads];
+    const int y = blockIdx.x, t = threadIdx.x;
     const long long row = (long long)y * raw_w;
-    int nxt = -1;
-    for (int x = raw_w - 1; x >= 0; --x) {
-        nr[row + x] = nxt;
-        if (offs[row + x + 1] > offs[row + x]) nxt = (int)(row + x);
+    const int per = (raw_w + kFillThreads - 1) / kFillThreads;
+    const int x0 = min(t * per, raw_w), x1 = min(x0 + per, raw_w);
+    int lf = INT_MAX, ll = -1;
+    for (int x = x0; x < x1; ++x)
+        if (offs[row + x + 1] > offs[row + x]) {
+            if (lf == INT_MAX) lf = (int)(row + x);
+            ll = (int)(row + x);
+        }
+    sf[t] = lf;
+    sl[t] = ll;
+    __syncthreads();
+    // inclusive scans: sl = max over lanes <= t, sf = min over lanes >= t
+    for (int o = 1; o < kFillThreads; o <<= 1) {
+        const int a = t >= o ? sl[t - o] : -1;
+        const int b = t + o < kFillThreads ? sf[t + o] : INT_MAX;
+        __syncthreads();
+        sl[t] = max(sl[t], a);
+        sf[t] = min(sf[t], b);
+        __syncthreads();
     }
-    rfirst[y] = nxt;
-    int prv = -1;
-    for (int x = 0; x < raw_w; ++x) {
+    int prv = t > 0 ? sl[t - 1] : -1;
+    for (int x = x0; x < x1; ++x) {
         pr[row + x] = prv;
         if (offs[row + x + 1] > offs[row + x]) prv = (int)(row + x);
     }
-    rlast[y] = prv;
+    int nxt = t + 1 < kFillThreads ? sf[t + 1] : INT_MAX;
+    for (int x = x1 - 1; x >= x0; --x) {
+        nr[row + x] = nxt == INT_MAX ? -1 : nxt;
+        if (offs[row + x + 1] > offs[row + x]) nxt = (int)(row + x);
+    }
+    if (t == 0) {
+        rfirst[y] = sf[0] == INT_MAX ? -1 : sf[0];
+        rlast[y] = sl[kFillThreads - 1];
+    }
 }
+
 
 // nextRowFirst[y] = first sampled pixel in a row below y (N if none); prevRowLast[y] = last
 // sampled pixel in a row above y (-1 if none).  One thread: raw_h steps.
@@ -335,31 +367,98 @@ __global__ void k_fill_rowlinks(const int* __restrict__ rfirst, const int* __res
 
 // Chains of the down-left step (linear stride raw_w - 1; thread per chain): next sampled index
 // strictly after p along the chain, N if none.
-__global__ void k_fill_diag(const int* __restrict__ offs, int raw_w, long long N,
-                            int* __restrict__ nd) {
-    const long long step = raw_w - 1;
+// Anti-diagonal links: nd[p] = the next pixel with samples on p's chain p, p + (raw_w - 1), ...
+// (N if none).  Chains are cut into kFillSeg-step segments (lane = chain, so a step reads
+// consecutive pixels across lanes): the first pass records each segment's first pixel with
+// samples, the second turns those into carries from the right, the third writes the links.
+__device__ __forceinline__ long long diag_len(long long c, long long step, long long N) {
+    return (N - 1 - c) / step + 1;
+}
+
+__global__ void k_diag_seg(const int* __restrict__ offs, long long step, long long N,
+                           int nseg, int* __restrict__ segv) {
     const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (step <= 0 || c >= step) return;
-    const long long last = c + ((N - 1 - c) / step) * step;
-    int nxt = (int)N;
-    for (long long p = last; p >= c; p -= step) {
+    const int sg = blockIdx.y;
+    if (c >= step) return;
+    const long long len = diag_len(c, step, N);
+    const long long k0 = (long long)sg * kFillSeg, k1 = min(k0 + kFillSeg, len);
+    int first = (int)N;
+    for (long long k = k1 - 1; k >= k0; --k) {
+        const long long p = c + k * step;
+        if (offs[p + 1] > offs[p]) first = (int)p;
+    }
+    segv[(long long)sg * step + c] = first;
+}
+
+__global__ void k_diag_carry(long long step, int nseg, int N, int* __restrict__ segv) {
+    const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (c >= step) return;
+    int carry = N;  // segment sg gets the first pixel with samples in the segments after it
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        int* q = segv + (long long)sg * step + c;
+        const int v = *q;
+        *q = carry;
+        if (v < N) carry = v;
+    }
+}
+
+__global__ void k_diag_links(const int* __restrict__ offs, long long step, long long N,
+                             const int* __restrict__ segv, int* __restrict__ nd) {
+    const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const int sg = blockIdx.y;
+    if (c >= step) return;
+    const long long len = diag_len(c, step, N);
+    const long long k0 = (long long)sg * kFillSeg, k1 = min(k0 + kFillSeg, len);
+    int nxt = segv[(long long)sg * step + c];
+    for (long long k = k1 - 1; k >= k0; --k) {
+        const long long p = c + k * step;
         nd[p] = nxt;
         if (offs[p + 1] > offs[p]) nxt = (int)p;
     }
 }
 
+
 // Columns (thread per column): nearest sampled pixel above p in its column, -1 if none.
-__global__ void k_fill_cols(const int* __restrict__ offs, int raw_w, int raw_h,
-                            int* __restrict__ pu) {
+// Column links: pu[p] = the previous pixel with samples in p's column (-1 if none); the same
+// three passes over kFillSeg-row segments (lane = column), carries from the top.
+__global__ void k_cols_seg(const int* __restrict__ offs, int raw_w, int raw_h,
+                           int* __restrict__ segv) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, sg = blockIdx.y;
+    if (x >= raw_w) return;
+    const int y0 = sg * kFillSeg, y1 = min(y0 + kFillSeg, raw_h);
+    int last = -1;
+    for (int y = y0; y < y1; ++y) {
+        const long long p = (long long)y * raw_w + x;
+        if (offs[p + 1] > offs[p]) last = (int)p;
+    }
+    segv[(long long)sg * raw_w + x] = last;
+}
+
+__global__ void k_cols_carry(int raw_w, int nseg, int* __restrict__ segv) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= raw_w) return;
-    int prv = -1;
-    for (int y = 0; y < raw_h; ++y) {
+    int carry = -1;  // segment sg gets the last pixel with samples in the segments above it
+    for (int sg = 0; sg < nseg; ++sg) {
+        int* q = segv + (long long)sg * raw_w + x;
+        const int v = *q;
+        *q = carry;
+        if (v >= 0) carry = v;
+    }
+}
+
+__global__ void k_cols_links(const int* __restrict__ offs, int raw_w, int raw_h,
+                             const int* __restrict__ segv, int* __restrict__ pu) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, sg = blockIdx.y;
+    if (x >= raw_w) return;
+    const int y0 = sg * kFillSeg, y1 = min(y0 + kFillSeg, raw_h);
+    int prv = segv[(long long)sg * raw_w + x];
+    for (int y = y0; y < y1; ++y) {
         const long long p = (long long)y * raw_w + x;
         pu[p] = prv;
         if (offs[p + 1] > offs[p]) prv = (int)p;
     }
 }
+
 
 // Colour of pixels without samples: the reference probes, for s = 1, 2, ..., the linear indices
 // p + s, p - s, p + s*(raw_w-1), p - s*raw_w in that order, stops at the first probe that leaves
@@ -507,7 +606,8 @@ __global__ void __launch_bounds__(256) k_pt_emit(const float* __restrict__ vd,
 }
 
 struct VdLayout {
-    size_t count, offs, sums, sv, sc, nr, pr, nd, pu, rfirst, rlast, nrf, prl, col, range, total;
+    size_t count, offs, sums, sv, sc, nr, pr, nd, pu, rfirst, rlast, nrf, prl, col, range, segc,
+        segd, total;
 };
 
 VdLayout vd_layout(int V, int crop, int raw_w, int raw_h) {
@@ -530,6 +630,12 @@ VdLayout vd_layout(int V, int crop, int raw_w, int raw_h) {
     L.prl = o; o += al((size_t)raw_h * 4);
     L.col = o; o += al(N);
     L.range = o; o += al(8);
+    // column segments (raw_w per segment) and diagonal segments (raw_w - 1 chains per segment)
+    const size_t ncs = ((size_t)raw_h + kFillSeg - 1) / kFillSeg;
+    const size_t step = raw_w > 1 ? (size_t)raw_w - 1 : 1;
+    const size_t nds = ((N - 1) / step + 1 + kFillSeg - 1) / kFillSeg;
+    L.segc = o; o += al(ncs * (size_t)raw_w * 4);
+    L.segd = o; o += al(nds * step * 4);
     L.total = o;
     return L;
 }
@@ -638,10 +744,25 @@ int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, con
         int* rlast = (int*)(ws + L.rlast);
         int* nrf = (int*)(ws + L.nrf);
         int* prl = (int*)(ws + L.prl);
-        k_fill_rows<<<grid_for(raw_h, 64), 64, 0, s>>>(offs, raw_w, raw_h, nr, pr, rfirst, rlast);
+        k_fill_rows<<<raw_h, kFillThreads, 0, s>>>(offs, raw_w, nr, pr, rfirst, rlast);
         k_fill_rowlinks<<<1, 1, 0, s>>>(rfirst, rlast, raw_h, N, nrf, prl);
-        if (raw_w >= 2) k_fill_diag<<<grid_for(raw_w - 1, 64), 64, 0, s>>>(offs, raw_w, N, nd);
-        k_fill_cols<<<grid_for(raw_w, 64), 64, 0, s>>>(offs, raw_w, raw_h, pu);
+        if (raw_w >= 2) {
+            const long long step = raw_w - 1;
+            const int nds = (int)(((N - 1) / step + 1 + kFillSeg - 1) / kFillSeg);
+            int* segd = (int*)(ws + L.segd);
+            const dim3 g((unsigned)grid_for(step, 256), (unsigned)nds);
+            k_diag_seg<<<g, 256, 0, s>>>(offs, step, N, nds, segd);
+            k_diag_carry<<<grid_for(step, 256), 256, 0, s>>>(step, nds, (int)N, segd);
+            k_diag_links<<<g, 256, 0, s>>>(offs, step, N, segd, nd);
+        }
+        {
+            const int ncs = (raw_h + kFillSeg - 1) / kFillSeg;
+            int* segc = (int*)(ws + L.segc);
+            const dim3 g((unsigned)grid_for(raw_w, 256), (unsigned)ncs);
+            k_cols_seg<<<g, 256, 0, s>>>(offs, raw_w, raw_h, segc);
+            k_cols_carry<<<grid_for(raw_w, 256), 256, 0, s>>>(raw_w, ncs, segc);
+            k_cols_links<<<g, 256, 0, s>>>(offs, raw_w, raw_h, segc, pu);
+        }
         // filled colour -> color_dev, then ten 5x5 median passes ping-ponging with a scratch
         // image in the fill tables' memory (no longer needed); an even pass count ends in
         // color_dev
