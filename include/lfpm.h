@@ -94,6 +94,47 @@ int lfpm_layout_fixed_neighbors(const lfpm_layout_params* p, int* neighbor_ids,
                                 int* neighbor_counts, int* garbage);
 
 /* ---------------------------------------------------------------------------------------
+ * Calibration ingest (host only).
+ *
+ * lfpm_read_intrinsics_xml replaces DepthSolver::ReadIntrinsicsFromXML (DepthSolver.cpp:
+ * 872-1110): root element "_"; D -> bl0 (g_bl0), d -> B (g_B), sensor/pose/rotation (three "_"
+ * columns of three values, stored here row-major), sensor/pose/translation/_ (three values),
+ * sensor/scale (mm per pixel), main_lens/f, mia/mesh/{width, height, pitch/_/...} (the last
+ * value under pitch wins, as the reference's loop assigns each in turn), distortions/{depth,
+ * radial, tangential}/_ values.  Numbers are read with the reference's sscanf formats; missing
+ * tags leave 0.
+ *
+ * lfpm_read_calib_xml replaces DepthSolver::ReadCalibrationParamsFromXML (:1112-1251): root
+ * "Calib"; BlurInfo/MLAType id="t" -> blur_radius[t] (t in 0..2, 0 when absent); MLAInfo/MLA
+ * {key "row,col" (%i: a leading 0 reads octal, 0x hex), center "x,y", MLAType, aera} per lens,
+ * returned in (row, col) order (the MLA map's QuadTreeTileKey order; a repeated key keeps its
+ * first entry).  tilekeys: 2*count ints (col, row) as lfpm_layout_build writes them.  With every
+ * array NULL only count (and blur_radius) is returned; otherwise capacity must hold count.
+ *
+ * lfpm_table_fixed_neighbors is lfpm_layout_fixed_neighbors over such a table: the lens
+ * lattice is lattice_cols x lattice_rows keys (the intrinsics' mla_u_size x mla_v_size); keys
+ * must lie inside it and be distinct.
+ * ------------------------------------------------------------------------------------- */
+typedef struct lfpm_intrinsics {
+    float D;                         /* main lens to MLA distance (g_bl0), mm */
+    float d;                         /* MLA to sensor distance (g_B), mm */
+    float sensor_pixel_size;         /* sensor/scale, mm per pixel */
+    float main_lens_f;               /* main_lens/f, mm */
+    int mla_u_size, mla_v_size;      /* mia/mesh width, height: lens lattice columns, rows */
+    float baseline;                  /* mia/mesh/pitch: lens pitch, pixels */
+    double sensor_rotation[9];       /* row-major 3x3 */
+    double sensor_translation[3];
+    double distortion_depth[3], distortion_radial[3], distortion_tangential[2];
+} lfpm_intrinsics;
+
+int lfpm_read_intrinsics_xml(const char* path, lfpm_intrinsics* out);
+int lfpm_read_calib_xml(const char* path, int capacity, int* tilekeys, float* centers,
+                        int* lens_type, float* areas, float* blur_radius, int* count);
+int lfpm_table_fixed_neighbors(const int* tilekeys, const float* centers, int num_mis,
+                               int lattice_cols, int lattice_rows, int max_neighbors,
+                               int* neighbor_ids, int* neighbor_counts, int* garbage);
+
+/* ---------------------------------------------------------------------------------------
  * Synthetic focused-plenoptic frame (host only; test and benchmark workload generator — the
  * reference ships no data).  Seeded slanted planes in virtual space on a Voronoi partition,
  * band-limited value-noise texture, Gaussian sensor noise.

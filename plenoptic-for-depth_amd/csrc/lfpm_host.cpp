@@ -164,6 +164,49 @@ static FixedPlace fixed_place(int X, int Y, int u, int v) {
 
 using namespace lfpm;
 
+namespace lfpm {
+// FixedPosition neighbour table over a lens lattice of cols x rows keys: MI v sits at key
+// (mi_i[v], mi_j[v]) with centre (cx[v], cy[v]); cell_to_mi maps a key to its MI (-1: none).
+int fixed_neighbors_core(int V, const int* mi_i, const int* mi_j, const float* cx,
+                         const float* cy, const int* cell_to_mi, int cols, int rows, int K,
+                         int* neighbor_ids, int* neighbor_counts, int* garbage) {
+    if (K <= 0 || K > LFPM_MAX_NEIGHBORS)
+        return set_error(LFPM_E_INVALID, "fixed neighbours: max_neighbors %d outside [1, %d]", K,
+                         LFPM_MAX_NEIGHBORS);
+    struct Nb { int id; float len; };
+    std::vector<Nb> nb;
+    for (int v = 0; v < V; ++v) {
+        const int X = mi_i[v], Y = mi_j[v];
+        nb.clear();
+        const FixedPlace place = fixed_place(X, Y, cols, rows);
+        if (place != FP_NONE) {
+            const FixedList& fl = kFixedLists[place];
+            for (int e = 0; e < fl.n; ++e) {
+                // CollectValidNeighborKey (:1330-1352): the key must name a kept lens
+                const int nx = X + fl.d[e][0], ny = Y + fl.d[e][1];
+                if (nx < 0 || ny < 0 || nx >= cols || ny >= rows) continue;
+                const int u = cell_to_mi[(size_t)ny * cols + nx];
+                if (u < 0) continue;
+                // CreateResTileKeysFromNeiKeyVec (MVStereo/LFDepthInfo.cpp:431-453): centre
+                // distance from float differences, squared and summed in double
+                const float dx = cx[u] - cx[v], dy = cy[u] - cy[v];
+                nb.push_back(Nb{u, (float)std::sqrt((double)dy * dy + (double)dx * dx)});
+            }
+        }
+        // ascending distance; ties keep the collection order (std::sort leaves them unspecified)
+        std::stable_sort(nb.begin(), nb.end(), [](const Nb& a, const Nb& b) { return a.len < b.len; });
+        const int c = std::min((int)nb.size(), K);
+        if (neighbor_ids)
+            for (int k = 0; k < K; ++k) neighbor_ids[(size_t)v * K + k] = k < c ? nb[k].id : -1;
+        if (neighbor_counts) neighbor_counts[v] = c;
+        // DepthSolver::ConfirmProblemForEstimation (DepthSolver.cpp:2933-2946) with
+        // m_iGarbageRows = 1 (:117): first key row/column, or no neighbour at all
+        if (garbage) garbage[v] = (nb.empty() || X < 1 || Y < 1) ? 1 : 0;
+    }
+    return LFPM_OK;
+}
+}  // namespace lfpm
+
 extern "C" {
 
 const char* lfpm_last_error(void) { return lfpm::last_error(); }
@@ -235,42 +278,10 @@ int lfpm_layout_fixed_neighbors(const lfpm_layout_params* p, int* neighbor_ids,
     Lattice L;
     const int rc = build_lattice(p, L);
     if (rc) return rc;
-    const int K = p->max_neighbors;
-    if (K <= 0 || K > LFPM_MAX_NEIGHBORS)
-        return set_error(LFPM_E_INVALID, "fixed neighbours: max_neighbors %d outside [1, %d]", K,
-                         LFPM_MAX_NEIGHBORS);
-    const int V = (int)L.mi_i.size();
-    struct Nb { int id; float len; };
-    std::vector<Nb> nb;
-    for (int v = 0; v < V; ++v) {
-        const int X = L.mi_i[v], Y = L.mi_j[v];
-        nb.clear();
-        const FixedPlace place = fixed_place(X, Y, L.cols, L.rows);
-        if (place != FP_NONE) {
-            const FixedList& fl = kFixedLists[place];
-            for (int e = 0; e < fl.n; ++e) {
-                // CollectValidNeighborKey (:1330-1352): the key must name a kept lens
-                const int nx = X + fl.d[e][0], ny = Y + fl.d[e][1];
-                if (nx < 0 || ny < 0 || nx >= L.cols || ny >= L.rows) continue;
-                const int u = L.cell_to_mi[(size_t)ny * L.cols + nx];
-                if (u < 0) continue;
-                // CreateResTileKeysFromNeiKeyVec (MVStereo/LFDepthInfo.cpp:431-453): centre
-                // distance from float differences, squared and summed in double
-                const float dx = L.cx[u] - L.cx[v], dy = L.cy[u] - L.cy[v];
-                nb.push_back(Nb{u, (float)std::sqrt((double)dy * dy + (double)dx * dx)});
-            }
-        }
-        // ascending distance; ties keep the collection order (std::sort leaves them unspecified)
-        std::stable_sort(nb.begin(), nb.end(), [](const Nb& a, const Nb& b) { return a.len < b.len; });
-        const int c = std::min((int)nb.size(), K);
-        if (neighbor_ids)
-            for (int k = 0; k < K; ++k) neighbor_ids[(size_t)v * K + k] = k < c ? nb[k].id : -1;
-        if (neighbor_counts) neighbor_counts[v] = c;
-        // DepthSolver::ConfirmProblemForEstimation (DepthSolver.cpp:2933-2946) with
-        // m_iGarbageRows = 1 (:117): first key row/column, or no neighbour at all
-        if (garbage) garbage[v] = (nb.empty() || X < 1 || Y < 1) ? 1 : 0;
-    }
-    return LFPM_OK;
+    return lfpm::fixed_neighbors_core((int)L.mi_i.size(), L.mi_i.data(), L.mi_j.data(),
+                                      L.cx.data(), L.cy.data(), L.cell_to_mi.data(), L.cols,
+                                      L.rows, p->max_neighbors, neighbor_ids, neighbor_counts,
+                                      garbage);
 }
 
 void lfpm_pm_params_default(lfpm_pm_params* p, int crop, float base) {

@@ -4,6 +4,9 @@
 
 namespace lfpm {
 int set_error(int code, const char* fmt, ...);
+int fixed_neighbors_core(int V, const int* mi_i, const int* mi_j, const float* cx,
+                         const float* cy, const int* cell_to_mi, int cols, int rows, int K,
+                         int* neighbor_ids, int* neighbor_counts, int* garbage);
 const char* last_error();
 int resolve_crop(const lfpm_layout_params* p);
 

@@ -359,6 +359,53 @@ def read_dmb(path):
     return out
 
 
+class Intrinsics(C.Structure):
+    _fields_ = [("D", C.c_float), ("d", C.c_float), ("sensor_pixel_size", C.c_float),
+                ("main_lens_f", C.c_float), ("mla_u_size", C.c_int), ("mla_v_size", C.c_int),
+                ("baseline", C.c_float), ("sensor_rotation", C.c_double * 9),
+                ("sensor_translation", C.c_double * 3), ("distortion_depth", C.c_double * 3),
+                ("distortion_radial", C.c_double * 3), ("distortion_tangential", C.c_double * 2)]
+
+
+def read_intrinsics_xml(path):
+    """lfpm_read_intrinsics_xml (DepthSolver::ReadIntrinsicsFromXML): an Intrinsics struct."""
+    out = Intrinsics()
+    _check(lib().lfpm_read_intrinsics_xml(str(path).encode(), C.byref(out)))
+    return out
+
+
+def read_calib_xml(path):
+    """lfpm_read_calib_xml (DepthSolver::ReadCalibrationParamsFromXML): dict of the lens table in
+    (row, col) order — tilekeys (n, 2) (col, row), centers (n, 2), lens_type, area — and
+    blur_radius (3,) per lens type."""
+    n = C.c_int()
+    rad = np.zeros(3, np.float32)
+    _check(lib().lfpm_read_calib_xml(str(path).encode(), 0, None, None, None, None,
+                                     rad.ctypes.data_as(C.c_void_p), C.byref(n)))
+    k = np.zeros((n.value, 2), np.int32)
+    c = np.zeros((n.value, 2), np.float32)
+    t = np.zeros(n.value, np.int32)
+    a = np.zeros(n.value, np.float32)
+    p = lambda x: x.ctypes.data_as(C.c_void_p)
+    _check(lib().lfpm_read_calib_xml(str(path).encode(), n.value, p(k), p(c), p(t), p(a),
+                                     p(rad), C.byref(n)))
+    return dict(tilekeys=k, centers=c, lens_type=t, area=a, blur_radius=rad)
+
+
+def table_fixed_neighbors(tilekeys, centers, cols, rows, max_neighbors):
+    """lfpm_table_fixed_neighbors: FixedPosition tables of a lens table -> (nbr, cnt, garbage)."""
+    k = np.ascontiguousarray(tilekeys, np.int32)
+    c = np.ascontiguousarray(centers, np.float32)
+    V = k.shape[0]
+    nbr = np.zeros((V, max_neighbors), np.int32)
+    cnt = np.zeros(V, np.int32)
+    g = np.zeros(V, np.int32)
+    p = lambda x: x.ctypes.data_as(C.c_void_p)
+    _check(lib().lfpm_table_fixed_neighbors(p(k), p(c), V, cols, rows, max_neighbors, p(nbr),
+                                            p(cnt), p(g)))
+    return nbr, cnt, g
+
+
 class SolveBuffers:
     def __init__(self, V, W, pm, device="cuda"):
         import torch

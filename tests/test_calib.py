@@ -1,0 +1,129 @@
+"""Calibration ingest: intrinsics.xml and CalibInfo.xml in the structure that
+DepthSolver::ReadIntrinsicsFromXML / ReadCalibrationParamsFromXML read (DepthSolver.cpp:
+872-1110, 1112-1251), and the FixedPosition neighbour table of a lens table read that way.
+
+The reference ships no calibration files, so the documents below are written from the readers'
+element walk (tag names, "_" wrappers, the order of values, sscanf formats); the checks pin the
+walk's rules: the rotation's "_" groups are columns, the last value under mia/mesh/pitch wins,
+keys are "row,col" read with %i (a leading 0 is octal), a blank before a comma ends a %f,%f
+read, lenses come back in (row, col) order
+with the first of a repeated key kept.  The table-driven neighbour builder must reproduce the
+layout-driven one (lfpm_layout_fixed_neighbors) on the same lenses."""
+import numpy as np
+import pytest
+
+INTRINSICS = """<?xml version="1.0" encoding="UTF-8"?>
+<!-- written from the reader's element walk -->
+<_>
+  <D>12.5</D>
+  <d>0.75</d>
+  <sensor>
+    <pose>
+      <rotation>
+        <_><_>1</_><_>2</_><_>3</_></_>
+        <_><_>4</_><_>5</_><_>6</_></_>
+        <_><_>7</_><_>8</_><_>9</_></_>
+      </rotation>
+      <translation><_><_>0.1</_><_>-0.2</_><_>0.3</_></_></translation>
+    </pose>
+    <scale>0.0055</scale>
+  </sensor>
+  <main_lens><f>35.0</f></main_lens>
+  <mia>
+    <mesh>
+      <width>177</width>
+      <height>154</height>
+      <pitch><_><_>22.9</_><_>23.1</_></_></pitch>
+    </mesh>
+  </mia>
+  <distortions>
+    <depth><_><_>1e-3</_><_>2e-3</_><_>3e-3</_></_></depth>
+    <radial><_><_>0.1</_><_>0.2</_><_>0.3</_></_></radial>
+    <tangential><_><_>0.01</_><_>0.02</_></_></tangential>
+  </distortions>
+</_>
+"""
+
+CALIB = """<?xml version="1.0"?>
+<Calib>
+  <BlurInfo>
+    <pose><r>1,0,0,1</r><t>0,0</t></pose>
+    <MLAType id="0">2.5</MLAType>
+    <MLAType id="2">4.25</MLAType>
+  </BlurInfo>
+  <MLAInfo>
+    <MLA><key>1,2</key><center>100.5,200.25</center><MLAType>1</MLAType><aera>415.5</aera></MLA>
+    <MLA><key>0,7</key><center>30,40</center><MLAType>2</MLAType><aera>400</aera></MLA>
+    <MLA><key>010,3</key><center>5.5,6.5</center><MLAType>0</MLAType><aera>1e2</aera></MLA>
+    <MLA><key>1,2</key><center>-1,-1</center><MLAType>0</MLAType><aera>0</aera></MLA>
+    <MLA><key>0,1</key><center>7 , 8</center><MLAType>0</MLAType><aera>3</aera></MLA>
+  </MLAInfo>
+</Calib>
+"""
+
+
+def test_intrinsics_walk(lfpm, tmp_path):
+    p = tmp_path / "intrinsics.xml"
+    p.write_text(INTRINSICS)
+    i = lfpm.read_intrinsics_xml(p)
+    assert (i.D, i.d, i.main_lens_f) == (np.float32(12.5), np.float32(0.75), np.float32(35.0))
+    assert i.sensor_pixel_size == np.float32(0.0055)
+    assert (i.mla_u_size, i.mla_v_size) == (177, 154)
+    assert i.baseline == np.float32(23.1)  # the last value under pitch
+    # "_" groups are columns: column j holds R[0][j], R[1][j], R[2][j]
+    assert list(i.sensor_rotation) == [1, 4, 7, 2, 5, 8, 3, 6, 9]
+    assert list(i.sensor_translation) == [0.1, -0.2, 0.3]
+    assert list(i.distortion_depth) == [1e-3, 2e-3, 3e-3]
+    assert list(i.distortion_radial) == [0.1, 0.2, 0.3]
+    assert list(i.distortion_tangential) == [0.01, 0.02]
+
+
+def test_intrinsics_errors(lfpm, tmp_path):
+    p = tmp_path / "bad.xml"
+    p.write_text("<root><D>1</D></root>")
+    with pytest.raises(Exception):
+        lfpm.read_intrinsics_xml(p)
+    p.write_text("<_><D>1</D>")  # unterminated
+    with pytest.raises(Exception):
+        lfpm.read_intrinsics_xml(p)
+    with pytest.raises(Exception):
+        lfpm.read_intrinsics_xml(tmp_path / "missing.xml")
+
+
+def test_calib_table(lfpm, tmp_path):
+    p = tmp_path / "CalibInfo.xml"
+    p.write_text(CALIB)
+    c = lfpm.read_calib_xml(p)
+    # (row, col) order: (0,1), (0,7), (1,2) [first of the repeated key], (8,3) [octal 010]
+    assert c["tilekeys"].tolist() == [[1, 0], [7, 0], [2, 1], [3, 8]]
+    # "7 , 8": %f,%f stops at the blank before the comma, y keeps the initial 0
+    assert c["centers"].tolist() == [[7, 0], [30, 40], [100.5, 200.25], [5.5, 6.5]]
+    assert c["lens_type"].tolist() == [0, 2, 1, 0]
+    assert c["area"].tolist() == [3, 400, 415.5, 100]
+    assert c["blur_radius"].tolist() == [2.5, 0.0, 4.25]
+
+
+@pytest.mark.parametrize("pitch,w,h,K", [(23.0, 1024, 800, 18), (15.0, 640, 480, 12),
+                                         (31.0, 900, 700, 8)])
+def test_table_neighbours_equal_layout_neighbours(lfpm, pitch, w, h, K):
+    lp = lfpm.layout_params(pitch, w, h, K)
+    L = lfpm.layout(lp)
+    want = lfpm.fixed_neighbors(lp)
+    # the table in a shuffled order: ids are table indices, so map them back
+    rng = np.random.default_rng(int(pitch))
+    perm = rng.permutation(L["V"])
+    inv = np.argsort(perm)
+    nbr, cnt, gar = lfpm.table_fixed_neighbors(L["tilekeys"][perm], L["centers"][perm],
+                                                L["cols"], L["rows"], K)
+    back = np.where(nbr >= 0, perm[np.maximum(nbr, 0)], -1)
+    assert np.array_equal(back[inv], want[0])
+    assert np.array_equal(cnt[inv], want[1]) and np.array_equal(gar[inv], want[2])
+
+
+def test_table_neighbours_reject_bad_keys(lfpm):
+    k = np.array([[0, 0], [0, 0]], np.int32)
+    c = np.zeros((2, 2), np.float32)
+    with pytest.raises(Exception):
+        lfpm.table_fixed_neighbors(k, c, 4, 4, 8)  # repeated key
+    with pytest.raises(Exception):
+        lfpm.table_fixed_neighbors(np.array([[4, 0]], np.int32), c[:1], 4, 4, 8)
