@@ -134,6 +134,14 @@ int lfpm_table_fixed_neighbors(const int* tilekeys, const float* centers, int nu
                                int lattice_cols, int lattice_rows, int max_neighbors,
                                int* neighbor_ids, int* neighbor_counts, int* garbage);
 
+/* Crop origins of a read lens table (DepthSolver::CreateProblemsAndDisNormals_Frame,
+ * DepthSolver.cpp:1894-1905, and the slicing rectangle of :1977): corner = centre - (crop-1)/2
+ * in FP32; kept[v] = 0 (origin -1, -1) when the crop leaves mla_u_size*baseline x
+ * mla_v_size*baseline, or the raw frame; else origins = the corner truncated to int. */
+int lfpm_table_origins(const float* centers, int num_mis, int crop, int mla_u_size,
+                       int mla_v_size, float baseline, int raw_w, int raw_h, int* origins,
+                       int* kept);
+
 /* ---------------------------------------------------------------------------------------
  * Synthetic focused-plenoptic frame (host only; test and benchmark workload generator — the
  * reference ships no data).  Seeded slanted planes in virtual space on a Voronoi partition,

@@ -335,4 +335,31 @@ int lfpm_table_fixed_neighbors(const int* tilekeys, const float* centers, int nu
                                       neighbor_ids, neighbor_counts, garbage);
 }
 
+int lfpm_table_origins(const float* centers, int num_mis, int crop, int mla_u_size,
+                       int mla_v_size, float baseline, int raw_w, int raw_h, int* origins,
+                       int* kept) {
+    if (!centers || num_mis < 0 || crop < 1 || !origins || !kept)
+        return lfpm::set_error(LFPM_E_INVALID, "table origins: bad arguments");
+    // DepthSolver::CreateProblemsAndDisNormals_Frame (DepthSolver.cpp:1894-1905): corners in
+    // FP32 from the double offset (crop - 1) / 2; a lens is abandoned when its crop leaves the
+    // lens area mla_u_size * baseline x mla_v_size * baseline; the slicing rectangle takes the
+    // corner truncated to int (:1977)
+    const double half = (crop - 1) * 0.5;
+    const float lim_x = (float)mla_u_size * baseline, lim_y = (float)mla_v_size * baseline;
+    for (int v = 0; v < num_mis; ++v) {
+        const float xl = (float)((double)centers[2 * v] - half);
+        const float yl = (float)((double)centers[2 * v + 1] - half);
+        const float xr = (float)((double)centers[2 * v] + half);
+        const float yr = (float)((double)centers[2 * v + 1] + half);
+        bool ok = !(xl < 0 || yl < 0 || xr > lim_x || yr > lim_y);
+        const int ox = ok ? (int)xl : 0, oy = ok ? (int)yl : 0;
+        // and the crop must lie in the raw frame (the reference would cut past its edge)
+        ok = ok && ox + crop <= raw_w && oy + crop <= raw_h;
+        origins[2 * v] = ok ? ox : -1;
+        origins[2 * v + 1] = ok ? oy : -1;
+        kept[v] = ok ? 1 : 0;
+    }
+    return LFPM_OK;
+}
+
 }  // extern "C"

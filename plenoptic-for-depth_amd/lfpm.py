@@ -392,6 +392,18 @@ def read_calib_xml(path):
     return dict(tilekeys=k, centers=c, lens_type=t, area=a, blur_radius=rad)
 
 
+def table_origins(centers, crop, mla_u_size, mla_v_size, baseline, raw_w, raw_h):
+    """lfpm_table_origins: (origins (n, 2) int32, kept (n,) int32) of a read lens table."""
+    c = np.ascontiguousarray(centers, np.float32)
+    n = c.shape[0]
+    o = np.zeros((n, 2), np.int32)
+    k = np.zeros(n, np.int32)
+    p = lambda x: x.ctypes.data_as(C.c_void_p)
+    _check(lib().lfpm_table_origins(p(c), n, crop, mla_u_size, mla_v_size, C.c_float(baseline),
+                                    raw_w, raw_h, p(o), p(k)))
+    return o, k
+
+
 def table_fixed_neighbors(tilekeys, centers, cols, rows, max_neighbors):
     """lfpm_table_fixed_neighbors: FixedPosition tables of a lens table -> (nbr, cnt, garbage)."""
     k = np.ascontiguousarray(tilekeys, np.int32)

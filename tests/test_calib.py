@@ -127,3 +127,26 @@ def test_table_neighbours_reject_bad_keys(lfpm):
         lfpm.table_fixed_neighbors(k, c, 4, 4, 8)  # repeated key
     with pytest.raises(Exception):
         lfpm.table_fixed_neighbors(np.array([[4, 0]], np.int32), c[:1], 4, 4, 8)
+
+
+def test_table_origins_corner_rule(lfpm):
+    # crop 16: half = 7.5.  Lens area 10 x 4 lenses of pitch 23 -> 230 x 92 px; raw 240 x 100.
+    cen = np.array([[7.5, 7.5],      # corner (0, 0): kept
+                    [7.4, 50.0],     # left corner -0.1 < 0: abandoned
+                    [222.5, 84.5],   # right / top corner exactly 230, 92: kept (not > limit)
+                    [222.6, 50.0],   # right corner 230.1 > 230: abandoned
+                    [100.9, 20.2]],  # corner (93.4, 12.7) truncated to (93, 12)
+                   np.float32)
+    o, k = lfpm.table_origins(cen, 16, 10, 4, 23.0, 240, 100)
+    assert k.tolist() == [1, 0, 1, 0, 1]
+    assert o.tolist() == [[0, 0], [-1, -1], [215, 77], [-1, -1], [93, 12]]
+    # the same lens inside the lens area but past a smaller raw frame is dropped
+    o, k = lfpm.table_origins(cen[2:3], 16, 10, 4, 23.0, 228, 100)
+    assert k.tolist() == [0]
+
+
+def test_table_origins_equal_layout_origins(lfpm):
+    lp = lfpm.layout_params(23.0, 1024, 800, 18)
+    L = lfpm.layout(lp)
+    o, k = lfpm.table_origins(L["centers"], L["crop"], 10 ** 6, 10 ** 6, 1.0, 1024, 800)
+    assert k.all() and np.array_equal(o, L["origins"])
