@@ -150,3 +150,40 @@ def test_table_origins_equal_layout_origins(lfpm):
     L = lfpm.layout(lp)
     o, k = lfpm.table_origins(L["centers"], L["crop"], 10 ** 6, 10 ** 6, 1.0, 1024, 800)
     assert k.all() and np.array_equal(o, L["origins"])
+
+
+@pytest.mark.gpu
+def test_calib_file_drives_the_solver(lfpm, tmp_path):
+    """A lens table written as CalibInfo.xml and read back (keys, centres, crop origins and the
+    FixedPosition table from the reader's path) solves a frame exactly as the layout's own
+    FixedPosition tables do."""
+    import torch
+    wl = lfpm.workload_c1()
+    L = wl.L
+    print("calib chain V", wl.V, flush=True)
+    lines = ["<?xml version=\"1.0\"?>", "<Calib>", "<MLAInfo>"]
+    for (X, Y), (cx, cy) in zip(L["tilekeys"], L["centers"]):
+        lines.append("<MLA><key>%d,%d</key><center>%.9g,%.9g</center><MLAType>0</MLAType>"
+                     "<aera>1</aera></MLA>" % (Y, X, cx, cy))
+    lines += ["</MLAInfo>", "</Calib>"]
+    path = tmp_path / "CalibInfo.xml"
+    path.write_text("\n".join(lines))
+    c = lfpm.read_calib_xml(path)
+    assert np.array_equal(c["tilekeys"], L["tilekeys"]) and np.array_equal(c["centers"], L["centers"])
+    h, w = wl.raw().shape
+    o, k = lfpm.table_origins(c["centers"], L["crop"], L["cols"], L["rows"], 1e6, w, h)
+    assert k.all() and np.array_equal(o, L["origins"])
+    K = wl.pm.max_neighbors
+    nbr_t, cnt_t, _ = lfpm.table_fixed_neighbors(c["tilekeys"], c["centers"], L["cols"],
+                                                  L["rows"], K)
+    nbr_l, cnt_l, _ = lfpm.fixed_neighbors(wl.lp)
+    outs = []
+    for nbr, cnt in ((nbr_t, cnt_t), (nbr_l, cnt_l)):
+        df = lfpm.DeviceFrame(wl)
+        df.nbr = torch.from_numpy(np.ascontiguousarray(nbr, np.int32)).cuda()
+        df.cnt = torch.from_numpy(np.ascontiguousarray(cnt, np.int32)).cuda()
+        b = lfpm.patchmatch_frame(df)
+        torch.cuda.synchronize()
+        outs.append((b.plane.cpu().numpy(), b.cost.cpu().numpy()))
+    assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
+    assert np.array_equal(outs[0][1].view(np.uint32), outs[1][1].view(np.uint32))
