@@ -38,9 +38,17 @@ __global__ void k_fill_i32(int* __restrict__ p, long long n, int v) {
         p[i] = v;
 }
 
-// range[0] = ordered(min), range[1] = ordered(max) of the values a block saw; empty blocks add
-// nothing.  Caller initialises range to {ordered(+inf), ordered(-inf)}.
+// Range accumulators: kRangeSlots {ordered(min), ordered(max)} pairs, one per 128-byte line,
+// so that the workgroups of a large grid spread their atomics over several addresses instead of
+// serialising on one; k_range_out folds the slots.
+constexpr int kRangeSlots = 64;
+constexpr int kRangeStride = 32;  // ints: 128 bytes
+constexpr size_t kRangeBytes = (size_t)kRangeSlots * kRangeStride * 4;
+
+// slot (blockIdx.x % kRangeSlots) takes the min / max of the values a block saw; empty blocks
+// add nothing.  k_range_init sets every slot to {ordered(+inf), ordered(-inf)}.
 __device__ __forceinline__ void block_minmax(int* range, bool has, float val) {
+    range += (blockIdx.x % kRangeSlots) * kRangeStride;
     __shared__ int smin, smax;
     if (threadIdx.x == 0) { smin = f2o(INFINITY); smax = f2o(-INFINITY); }
     __syncthreads();
@@ -56,13 +64,21 @@ __device__ __forceinline__ void block_minmax(int* range, bool has, float val) {
 }
 
 __global__ void k_range_init(int* range) {
-    range[0] = f2o(INFINITY);
-    range[1] = f2o(-INFINITY);
+    const int t = threadIdx.x;
+    if (t < kRangeSlots) {
+        range[t * kRangeStride] = f2o(INFINITY);
+        range[t * kRangeStride + 1] = f2o(-INFINITY);
+    }
 }
 
 __global__ void k_range_out(const int* __restrict__ range, float* __restrict__ out) {
-    out[0] = o2f(range[0]);
-    out[1] = o2f(range[1]);
+    int lo = f2o(INFINITY), hi = f2o(-INFINITY);
+    for (int t = 0; t < kRangeSlots; ++t) {
+        lo = min(lo, range[t * kRangeStride]);
+        hi = max(hi, range[t * kRangeStride + 1]);
+    }
+    out[0] = o2f(lo);
+    out[1] = o2f(hi);
 }
 
 // Raw position of local pixel (x, y) of an MI centred at (cx, cy) in the mosaic: the crop's
@@ -270,7 +286,53 @@ __global__ void __launch_bounds__(256) k_vd_median(const int* __restrict__ offs,
     if (p < N) {
         const int b = offs[p], n = offs[p + 1] - b;
         uint8_t cm = 0;
-        if (n > 0) {
+        if (n == 1 || n == 2) {
+            // the index rule reads s[n-1] twice for n <= 2 (the largest sample; halving the
+            // doubled value is exact), the colour rule averages the two samples
+            const float s0 = sv[b], s1 = n == 2 ? sv[b + 1] : s0;
+            const int c0 = sc[b], c1 = n == 2 ? sc[b + 1] : c0;
+            m = ((s0 > s1 ? s0 : s1) + (s0 > s1 ? s0 : s1)) / 2.0f;
+            cm = (uint8_t)((c0 + c1) / 2);
+            has = true;
+        } else if (n <= 8 && n > 0) {
+            // 8-wire sorting network in registers, padding above every sample (+inf, 256)
+            float f[8];
+            int q[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                f[k] = k < n ? sv[b + k] : INFINITY;
+                q[k] = k < n ? (int)sc[b + k] : 256;
+            }
+            auto cx = [&](int i, int j) {
+                const float lo = fminf(f[i], f[j]), hi = fmaxf(f[i], f[j]);
+                f[i] = lo;
+                f[j] = hi;
+                const int a = min(q[i], q[j]), c = max(q[i], q[j]);
+                q[i] = a;
+                q[j] = c;
+            };
+            // Batcher's odd-even merge sort, 8 inputs (19 comparators)
+            cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
+            cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
+            cx(1, 2); cx(5, 6);
+            cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
+            cx(2, 4); cx(3, 5);
+            cx(1, 2); cx(3, 4); cx(5, 6);
+            const int i1 = min(n / 2 + 1, n - 1), i2 = min((n + 1) / 2, n - 1);
+            const int j1 = n / 2, j2 = (n + 1) / 2 - 1;
+            float a1 = f[0], a2 = f[0];
+            int d1 = q[0], d2 = q[0];
+#pragma unroll
+            for (int k = 1; k < 8; ++k) {
+                a1 = k == i1 ? f[k] : a1;
+                a2 = k == i2 ? f[k] : a2;
+                d1 = k == j1 ? q[k] : d1;
+                d2 = k == j2 ? q[k] : d2;
+            }
+            m = (a1 + a2) / 2.0f;
+            cm = (uint8_t)((d1 + d2) / 2);
+            has = true;
+        } else if (n > 0) {
             float* s = sv + b;
             uint8_t* c = sc + b;
             for (int a = 1; a < n; ++a) {
@@ -349,19 +411,43 @@ __global__ void __launch_bounds__(kFillThreads) k_fill_rows(const int* __restric
 
 // nextRowFirst[y] = first sampled pixel in a row below y (N if none); prevRowLast[y] = last
 // sampled pixel in a row above y (-1 if none).  One thread: raw_h steps.
-__global__ void k_fill_rowlinks(const int* __restrict__ rfirst, const int* __restrict__ rlast,
-                                int raw_h, long long N, int* __restrict__ nrf,
-                                int* __restrict__ prl) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    int nxt = (int)N;
-    for (int y = raw_h - 1; y >= 0; --y) {
-        nrf[y] = nxt;
-        if (rfirst[y] >= 0) nxt = rfirst[y];
+__global__ void __launch_bounds__(kFillThreads) k_fill_rowlinks(const int* __restrict__ rfirst,
+                                                                const int* __restrict__ rlast,
+                                                                int raw_h, long long N,
+                                                                int* __restrict__ nrf,
+                                                                int* __restrict__ prl) {
+    // nrf[y] = the first pixel with samples in the rows after y (N if none), prl[y] = the last
+    // one in the rows before y (-1 if none): exclusive suffix-min / prefix-max over rows, one
+    // workgroup, each lane a contiguous run of rows
+    __shared__ int sf[kFillThreads], sl[kFillThreads];
+    const int t = threadIdx.x;
+    const int per = (raw_h + kFillThreads - 1) / kFillThreads;
+    const int y0 = min(t * per, raw_h), y1 = min(y0 + per, raw_h);
+    int lf = INT_MAX, ll = -1;
+    for (int y = y0; y < y1; ++y) {
+        if (rfirst[y] >= 0 && lf == INT_MAX) lf = rfirst[y];
+        if (rlast[y] >= 0) ll = rlast[y];
     }
-    int prv = -1;
-    for (int y = 0; y < raw_h; ++y) {
+    sf[t] = lf;
+    sl[t] = ll;
+    __syncthreads();
+    for (int o = 1; o < kFillThreads; o <<= 1) {
+        const int a = t >= o ? sl[t - o] : -1;
+        const int b = t + o < kFillThreads ? sf[t + o] : INT_MAX;
+        __syncthreads();
+        sl[t] = max(sl[t], a);
+        sf[t] = min(sf[t], b);
+        __syncthreads();
+    }
+    int prv = t > 0 ? sl[t - 1] : -1;
+    for (int y = y0; y < y1; ++y) {
         prl[y] = prv;
         if (rlast[y] >= 0) prv = rlast[y];
+    }
+    int nxt = t + 1 < kFillThreads ? sf[t + 1] : INT_MAX;
+    for (int y = y1 - 1; y >= y0; --y) {
+        nrf[y] = nxt == INT_MAX ? (int)N : nxt;
+        if (rfirst[y] >= 0) nxt = rfirst[y];
     }
 }
 
@@ -629,7 +715,7 @@ VdLayout vd_layout(int V, int crop, int raw_w, int raw_h) {
     L.nrf = o; o += al((size_t)raw_h * 4);
     L.prl = o; o += al((size_t)raw_h * 4);
     L.col = o; o += al(N);
-    L.range = o; o += al(8);
+    L.range = o; o += al(kRangeBytes);
     // column segments (raw_w per segment) and diagonal segments (raw_w - 1 chains per segment)
     const size_t ncs = ((size_t)raw_h + kFillSeg - 1) / kFillSeg;
     const size_t step = raw_w > 1 ? (size_t)raw_w - 1 : 1;
@@ -640,7 +726,7 @@ VdLayout vd_layout(int V, int crop, int raw_w, int raw_h) {
     return L;
 }
 
-size_t mosaic_bytes(int raw_w, int raw_h) { return al((size_t)raw_w * raw_h * 4) + al(8); }
+size_t mosaic_bytes(int raw_w, int raw_h) { return al((size_t)raw_w * raw_h * 4) + al(kRangeBytes); }
 
 int check_launch(const char* what) {
     const hipError_t e = hipGetLastError();
@@ -686,7 +772,7 @@ int lfpm_disparity_mosaic(const float* plane_dev, const float* cost_dev, const f
     int* owner = (int*)ws;
     int* range = (int*)(ws + al((size_t)N * 4));
     k_fill_i32<<<grid_for(N, 256 * 8), 256, 0, s>>>(owner, N, -1);
-    k_range_init<<<1, 1, 0, s>>>(range);
+    k_range_init<<<1, kRangeSlots, 0, s>>>(range);
     k_mosaic_scatter<<<grid_for(M, 256), 256, 0, s>>>(
         (const float4*)plane_dev, cost_dev, (const float2*)centers_dev, num_mis, crop, max_cost,
         raw_w, raw_h, owner, range);
@@ -724,7 +810,7 @@ int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, con
     const int nb = (int)((N + 1 + kScanChunk - 1) / kScanChunk);
 
     k_fill_i32<<<grid_for(N + 1, 256 * 8), 256, 0, s>>>(count, N + 1, 0);
-    k_range_init<<<1, 1, 0, s>>>(range);
+    k_range_init<<<1, kRangeSlots, 0, s>>>(range);
     k_vd_count<<<grid_for(M, 256), 256, 0, s>>>(plane, centers, num_mis, crop, mla_base, raw_w,
                                                  raw_h, count);
     k_scan_sums<<<nb, kScanThreads, 0, s>>>(count, N + 1, sums);
@@ -745,7 +831,7 @@ int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, con
         int* nrf = (int*)(ws + L.nrf);
         int* prl = (int*)(ws + L.prl);
         k_fill_rows<<<raw_h, kFillThreads, 0, s>>>(offs, raw_w, nr, pr, rfirst, rlast);
-        k_fill_rowlinks<<<1, 1, 0, s>>>(rfirst, rlast, raw_h, N, nrf, prl);
+        k_fill_rowlinks<<<1, kFillThreads, 0, s>>>(rfirst, rlast, raw_h, N, nrf, prl);
         if (raw_w >= 2) {
             const long long step = raw_w - 1;
             const int nds = (int)(((N - 1) / step + 1 + kFillSeg - 1) / kFillSeg);
