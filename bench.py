@@ -412,6 +412,23 @@ def precision_section(args, lfpm, torch):
     return out
 
 
+def point_cloud_traffic():
+    """HBM bytes per lfpm_vd_point_cloud call from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (flag, scan and emit kernels, calibrated), or None if absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_products_4080x3068.md")
+    if not os.path.exists(path):
+        return None
+    tot = 0.0
+    for row in open(path):
+        cells = [c.strip() for c in row.strip().strip("|").split("|")]
+        if len(cells) == 5 and ("k_pt_" in cells[0] or "k_scan_" in cells[0]):
+            try:
+                tot += float(cells[4])
+            except ValueError:
+                return None
+    return tot or None
+
+
 def products_section(args, lfpm, torch, wl):
     """Per-frame products of one solved frame: disparity mosaic, VD map with its gray colour
     image, and the VD point cloud (PLY records).  The frame is solved once; each step runs the
@@ -485,7 +502,7 @@ def products_section(args, lfpm, torch, wl):
     return {"fps": 1e3 / sum(ms.values()), "ms": ms, "wall_ms": 1e3 * wall, "points": n,
             "records_equal_oracle": same,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
-                         "frac": gbs / 8000.0, "traffic": None,
+                         "frac": gbs / 8000.0, "traffic": point_cloud_traffic(),
                          "kernel": "lfpm_vd_point_cloud (flag, scan, emit)",
                          "avg_launch_ms": ms["point_cloud"],
                          "per_unit": "5 B per raw pixel read (vd, colour) + 27 B per point "
@@ -625,6 +642,8 @@ def main():
     if args.mode == "products":
         wl = lfpm.workload_c2(raw_w=rw, raw_h=rh)
         r = products_section(args, lfpm, torch, wl)
+        if args.calib_copy:
+            calib_copy(torch)
         line.update(metric="per-frame products per s (%dx%d: mosaic, VD map, point cloud)"
                     % (rw, rh), value=r["fps"], ms_per_step=sum(r["ms"].values()),
                     dtype="fp32", roofline=r["roofline"], cpu_baseline=r["cpu_baseline"],
