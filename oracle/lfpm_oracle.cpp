@@ -98,17 +98,23 @@ static inline void philox(const uint32_t cin[4], const uint32_t kin[2], uint32_t
 }
 
 // Per-(pixel, step) draw stream.  Slot s -> Philox block s/4, word s%4 of
-// ctr = (pixel_uid, step, s/4, 0), key = (seed, frame_uid).
+// ctr = (pixel_uid, step, s/4, 0), key = (seed, frame_uid).  A stream built over a word array
+// returns fixed[s] instead (the pin entry points feed hand-chosen draws through it).
 struct Stream {
     uint32_t key[2];
     uint32_t pix, step;
     uint32_t cached_block;
     uint32_t words[4];
+    const uint32_t* fixed = nullptr;
     Stream(uint32_t seed, uint32_t uid, uint32_t pix_, uint32_t step_)
         : pix(pix_), step(step_), cached_block(0xFFFFFFFFu) {
         key[0] = seed; key[1] = uid;
     }
+    explicit Stream(const uint32_t* w) : pix(0), step(0), cached_block(0xFFFFFFFFu), fixed(w) {
+        key[0] = key[1] = 0;
+    }
     uint32_t raw(uint32_t slot) {
+        if (fixed) return fixed[slot];
         const uint32_t b = slot >> 2;
         if (b != cached_block) {
             const uint32_t c[4] = {pix, step, b, 0u};
@@ -195,6 +201,18 @@ template <typename T> struct Frame {
         return top + fy * (bot - top);
     }
     T bilinear(int v, T x, T y) const { return bilinear_of(img, v, x, y); }
+    // Magnitudes of the bilinear interpolant's partial derivatives at (x, y) (as bilinear_of).
+    void bilinear_grad(const std::vector<T>& img, int v, T x, T y, double& gx, double& gy) const {
+        const T fx0 = std::floor(x), fy0 = std::floor(y);
+        const int x0 = (int)fx0, y0 = (int)fy0;
+        const int x1 = std::min(x0 + 1, W - 1), y1 = std::min(y0 + 1, W - 1);
+        const double fx = (double)(x - fx0), fy = (double)(y - fy0);
+        const size_t o = (size_t)v * W * W;
+        const double a = img[o + (size_t)y0 * W + x0], b = img[o + (size_t)y0 * W + x1];
+        const double c = img[o + (size_t)y1 * W + x0], d = img[o + (size_t)y1 * W + x1];
+        gx = std::fabs((1 - fy) * (b - a) + fy * (d - c));
+        gy = std::fabs((1 - fx) * (c - a) + fx * (d - b));
+    }
     int count(int v) const { return std::min(cnt[v], K); }
 };
 
@@ -224,14 +242,49 @@ static void window_weights(const Frame<T>& f, int v, int px, int py, Win<T>& ww)
         }
 }
 
+// How far one evaluation is from the places where FP32 and FP64 may legitimately disagree:
+//  geo   smallest distance (px) by which the anchor or a window sample's source point would
+//        have to move to change its inside-the-crop test (AdaptMIPMFrame.cu:104-108,120-131);
+//        a coordinate whose step component is exactly 0 is an exact integer and never moves;
+//  gate  smallest distance of the two window variances from the variance gate (:170-174), in
+//        units of their estimated FP32 error (inf where the evaluation ends before the gate);
+//  ferr  first-order estimate of the FP32 error of the cost: the moments summed in FP32 with
+//        the window-centre shift (relative error n_eff * 2^-24 of each weighted absolute sum,
+//        n_eff = ceil(n / 8) + 3 terms per partial sum, the reduction of 8 lanes included),
+//        the source coordinates' FP32 rounding times the interpolant's slope, and the 1-ulp
+//        reciprocal and reciprocal square root; propagated through the variance and covariance
+//        into 1 - cov / sqrt(var_x var_y).  0 where the cost is exactly 2 by an early return.
+struct NccMargins { double geo, gate, ferr; };
+
+// Distance of (qx, qy) from a change of the inside test 0 <= q < W: for an inside point the
+// nearest edge; for an outside point the largest violated edge distance (all of them must be
+// crossed to get inside).  An exact axis (step component 0: q is an integer) is not counted.
+template <typename T> static double inside_margin(T qx, T qy, T W, bool exact_x, bool exact_y) {
+    const double inf = INFINITY;
+    const double dx0 = exact_x ? inf : (double)qx, dx1 = exact_x ? inf : (double)W - (double)qx;
+    const double dy0 = exact_y ? inf : (double)qy, dy1 = exact_y ? inf : (double)W - (double)qy;
+    const bool in = qx >= (T)0 && qx < W && qy >= (T)0 && qy < W;
+    if (in) return std::min(std::min(dx0, dx1), std::min(dy0, dy1));
+    double m = 0;
+    if (!exact_x && qx < (T)0) m = std::max(m, -dx0);
+    if (!exact_x && qx >= W) m = std::max(m, -dx1);
+    if (!exact_y && qy < (T)0) m = std::max(m, -dy0);
+    if (!exact_y && qy >= W) m = std::max(m, -dy1);
+    return m;
+}
+
 template <typename T>
-static T ncc(const Frame<T>& f, const Win<T>& ww, int v, int k, int px, int py, const P4<T>& h) {
+static T ncc(const Frame<T>& f, const Win<T>& ww, int v, int k, int px, int py, const P4<T>& h,
+             NccMargins* mg = nullptr) {
+    if (mg) { mg->geo = INFINITY; mg->gate = INFINITY; mg->ferr = 0; }
     const int n = f.nbr[(size_t)v * f.K + k];
     if (n < 0) return (T)2;
     const T sx = ((T)f.centers[2 * v] - (T)f.centers[2 * n]) * f.inv_base;
     const T sy = ((T)f.centers[2 * v + 1] - (T)f.centers[2 * n + 1]) * f.inv_base;
     const T W = (T)f.W;
     const T ax = (T)px + sx * h.w, ay = (T)py + sy * h.w;
+    const bool ex = sx == (T)0, ey = sy == (T)0;
+    if (mg) mg->geo = inside_margin(ax, ay, W, ex, ey);
     if (!(ax >= (T)0 && ax < W && ay >= (T)0 && ay < W)) return (T)2;
     // slant gate: planes steeper than acos(cos_max_slant) are not scored
     if (!(std::fabs(h.z) >= (T)f.p.cos_max_slant)) return (T)2;
@@ -239,6 +292,11 @@ static T ncc(const Frame<T>& f, const Win<T>& ww, int v, int k, int px, int py, 
     const int r = f.p.radius, inc = f.p.radius_increment, nn = 2 * r + 1;
     T s_w = 0, s_x = 0, s_xx = 0, s_y = 0, s_yy = 0, s_xy = 0;
     T b_ref = 0, b_src = 0;  // blur sums (ComputeBilateralNCC_Frame :140-144)
+    // margin bookkeeping (FP64): weighted absolute sums and the coordinate-rounding terms
+    double e_ax = 0, e_ay = 0, e_axy = 0, e_axx = 0, e_ayy = 0, e_dy = 0, e_ydy = 0, e_xdy = 0;
+    double e_b = 0, e_bd = 0;  // blur sums: absolute values about the centre blur, rounding
+    int e_n = 0;
+    const double u32 = 5.9604644775390625e-08;
     const bool blur = !f.blur.empty();
     for (int i = -r; i <= r; i += inc) {
         for (int j = -r; j <= r; j += inc) {
@@ -246,6 +304,7 @@ static T ncc(const Frame<T>& f, const Win<T>& ww, int v, int k, int px, int py, 
             if (rx < 0 || rx >= f.W || ry < 0 || ry >= f.W) continue;
             const T d = h.w + A * (T)i + B * (T)j;
             const T qx = (T)rx + sx * d, qy = (T)ry + sy * d;
+            if (mg) mg->geo = std::min(mg->geo, inside_margin(qx, qy, W, ex, ey));
             if (!(qx >= (T)0 && qx < W && qy >= (T)0 && qy < W)) continue;
             if (blur) {
                 b_ref += f.Bv(v, rx, ry);
@@ -260,15 +319,67 @@ static T ncc(const Frame<T>& f, const Win<T>& ww, int v, int k, int px, int py, 
             s_y += wgt * y;
             s_yy += wgt * y * y;
             s_xy += wgt * x * y;
+            if (mg) {
+                // FP32 rounding of the source point: the step, slope and map products and sums
+                // (a few roundings of terms up to |p| + |i| + |s| (|w| + |A i| + |B j|))
+                const double dq = 8 * u32 * ((double)std::fabs((T)rx) + (double)std::fabs((T)ry) +
+                                             (std::fabs((double)sx) + std::fabs((double)sy)) *
+                                                 ((double)std::fabs(h.w) + std::fabs((double)A * i) +
+                                                  std::fabs((double)B * j)));
+                double gx, gy;
+                f.bilinear_grad(f.img, n, qx, qy, gx, gy);
+                if (blur) {
+                    const double rbc = (double)f.Bv(v, px, py);
+                    double bgx, bgy;
+                    f.bilinear_grad(f.blur, n, qx, qy, bgx, bgy);
+                    e_b += std::fabs((double)f.Bv(v, rx, ry) - rbc) +
+                           std::fabs((double)f.bilinear_of(f.blur, n, qx, qy) - rbc);
+                    e_bd += (ex ? 0 : bgx * dq) + (ey ? 0 : bgy * dq);
+                }
+                const double dy = (ex ? 0 : gx * dq) + (ey ? 0 : gy * dq) +
+                                  4 * u32 * (std::fabs((double)y) + std::fabs((double)ww.rc));
+                const double wd = (double)wgt, xd = (double)x, yd = (double)y;
+                e_ax += wd * std::fabs(xd); e_ay += wd * std::fabs(yd);
+                e_axx += wd * xd * xd; e_ayy += wd * yd * yd;
+                e_axy += wd * std::fabs(xd * yd);
+                e_dy += wd * dy; e_ydy += wd * std::fabs(yd) * dy; e_xdy += wd * std::fabs(xd) * dy;
+                ++e_n;
+            }
         }
+    }
+    // FP32 error estimates of the normalised moments (see NccMargins)
+    double d_vx = 0, d_vy = 0, d_cov = 0;
+    if (mg && s_w > (T)0) {
+        const double sw = (double)s_w, g = (double)((e_n + 7) / 8 + 3) * u32;
+        const double mxd = (double)s_x / sw, myd = (double)s_y / sw;
+        const double ax = e_ax / sw, ay = e_ay / sw, axx = e_axx / sw, ayy = e_ayy / sw,
+                     axy = e_axy / sw;
+        // each normalised sum m = S / sw: |dm| <= g * (|S|-sum / sw + |m|) (sum and division)
+        const double d_mx = g * (ax + std::fabs(mxd));
+        const double d_my = g * (ay + std::fabs(myd)) + e_dy / sw;
+        d_vx = g * (2 * axx) + 2 * std::fabs(mxd) * d_mx + u32 * (axx + mxd * mxd);
+        d_vy = g * (2 * ayy) + 2 * e_ydy / sw + 2 * std::fabs(myd) * d_my + u32 * (ayy + myd * myd);
+        d_cov = g * (2 * axy) + e_xdy / sw + std::fabs(mxd) * d_my + std::fabs(myd) * d_mx +
+                u32 * (axy + std::fabs(mxd * myd));
     }
     if (s_w <= (T)1e-6) return (T)2;
     const T inv = (T)1 / s_w;
     const T mx = s_x * inv, my = s_y * inv;
     const T vx = s_xx * inv - mx * mx;
     const T vy = s_yy * inv - my * my;
+    if (mg) {
+        const double mv = (double)f.p.min_var;
+        mg->gate = std::min(std::fabs((double)vx - mv) / std::max(d_vx, 1e-300),
+                            std::fabs((double)vy - mv) / std::max(d_vy, 1e-300));
+    }
     if (vx < (T)f.p.min_var || vy < (T)f.p.min_var) return (T)2;
     const T cov = s_xy * inv - mx * my;
+    if (mg) {
+        const double sq = std::sqrt((double)vx * (double)vy);
+        const double rho = std::fabs((double)cov) / sq;
+        mg->ferr = d_cov / sq + 0.5 * rho * (d_vx / (double)vx + d_vy / (double)vy) +
+                   3 * u32 * rho;
+    }
     T c = std::min((T)2, std::max((T)0, (T)1 - cov / std::sqrt(vx * vy)));
     if (blur) {
         // blur term of ComputeMultiViewCostVector_Frame (AdaptMIPMFrame.cu:211-218); the
@@ -276,6 +387,13 @@ static T ncc(const Frame<T>& f, const Win<T>& ww, int v, int k, int px, int py, 
         const T db = b_ref - b_src;
         const T bw = std::exp((T)-0.007368f * db * db);
         c = std::min((T)2, std::max((T)0, c + (T)f.p.blur_lambda * ((T)1 - bw)));
+        if (mg) {
+            const double g = (double)((e_n + 7) / 8 + 3) * u32, kb = 0.007368;
+            const double ddb = g * e_b + e_bd;
+            mg->ferr += (double)f.p.blur_lambda * 2 * kb * std::fabs((double)db) *
+                            std::exp(-kb * (double)db * (double)db) * ddb +
+                        (double)f.p.blur_lambda * 4 * u32;
+        }
     }
     return c;
 }
@@ -421,6 +539,176 @@ static bool cross_view_proposal(const Frame<T>& f, const P4<T>* plane_prev, int 
     return true;
 }
 
+// The 8 propagation slots of CheckerboardUpdateFrame (AdaptMIPMFrame.cu:584-795) for pixel
+// (x, y) of a W x W MI whose previous costs are `cost`: far slots 1/3/5/7 at +-3 along an axis
+// with one extra step to +-4 (farDis = 2), near slots 0/2/4/6 at +-1 with the two diagonal
+// alternatives of nerDis = 2, each slot taking its lowest previous cost (first of equals).
+// Pixels within 3 of a border copy the near slot into the missing far one (:710-795): dup[a] is
+// then the slot copied, else -1.  pos[] are pixel indices y*W + x inside the MI; slots with
+// flag[a] false hold the centre.
+template <typename T>
+static void propagation_candidates(int W, int x, int y, const T* cost, bool flag[8], int pos[8],
+                                   int dup[8]) {
+    const int H = W, center = y * W + x;
+    for (int a = 0; a < 8; ++a) { flag[a] = false; pos[a] = center; dup[a] = -1; }
+    auto pick = [&](int& best, T& cm, int cand) {
+        if (cost[cand] < cm) { cm = cost[cand]; best = cand; }
+    };
+    if (y > 2) {
+        flag[1] = true; int b = center - 3 * W; T cm = cost[b];
+        if (y > 3) pick(b, cm, center - 4 * W);
+        pos[1] = b;
+    }
+    if (y < H - 3) {
+        flag[3] = true; int b = center + 3 * W; T cm = cost[b];
+        if (y < H - 4) pick(b, cm, center + 4 * W);
+        pos[3] = b;
+    }
+    if (x > 2) {
+        flag[5] = true; int b = center - 3; T cm = cost[b];
+        if (x > 3) pick(b, cm, center - 4);
+        pos[5] = b;
+    }
+    if (x < W - 3) {
+        flag[7] = true; int b = center + 3; T cm = cost[b];
+        if (x < W - 4) pick(b, cm, center + 4);
+        pos[7] = b;
+    }
+    if (y > 0) {
+        flag[0] = true; const int un = center - W; int b = un; T cm = cost[b];
+        for (int i = 0; i < 2; ++i) {
+            if (y > 1 + i && x > i) pick(b, cm, un - (1 + i) * W - i);
+            if (y > 1 + i && x < W - 1 - i) pick(b, cm, un - (1 + i) * W + i);
+        }
+        pos[0] = b;
+    }
+    if (y <= 2 && y > 0) { flag[1] = true; pos[1] = pos[0]; dup[1] = 0; }
+    if (y < H - 1) {
+        flag[2] = true; const int dn = center + W; int b = dn; T cm = cost[b];
+        for (int i = 0; i < 2; ++i) {
+            if (y < H - 2 - i && x > i) pick(b, cm, dn + (1 + i) * W - i);
+            if (y < H - 2 - i && x < W - 1 - i) pick(b, cm, dn + (1 + i) * W + i);
+        }
+        pos[2] = b;
+    }
+    if (y >= H - 3 && y < H - 1) { flag[3] = true; pos[3] = pos[2]; dup[3] = 2; }
+    if (x > 0) {
+        flag[4] = true; const int ln = center - 1; int b = ln; T cm = cost[b];
+        for (int i = 0; i < 2; ++i) {
+            if (x > 1 + i && y > i) pick(b, cm, ln - (1 + i) - i * W);
+            if (x > 1 + i && y < H - 1 - i) pick(b, cm, ln - (1 + i) + i * W);
+        }
+        pos[4] = b;
+    }
+    if (x <= 2 && x > 0) { flag[5] = true; pos[5] = pos[4]; dup[5] = 4; }
+    if (x < W - 1) {
+        flag[6] = true; const int rn = center + 1; int b = rn; T cm = cost[b];
+        for (int i = 0; i < 2; ++i) {
+            if (x < W - 2 - i && y > i) pick(b, cm, rn + (1 + i) - i * W);
+            if (x < W - 2 - i && y < H - 1 - i) pick(b, cm, rn + (1 + i) + i * W);
+        }
+        pos[6] = b;
+    }
+    if (x >= W - 3 && x < W - 1) { flag[7] = true; pos[7] = pos[6]; dup[7] = 6; }
+}
+
+// View-selection priors (AdaptMIPMFrame.cu:797-815): over the 4 direct neighbours that exist
+// (up, down, left, right; the same tests as near slots 0/2/4/6), +0.9 for each view whose bit is
+// set in the neighbour's previous selection, +0.1 otherwise.  sel = the MI's W*W selections.
+template <typename T>
+static void view_priors(int W, int x, int y, const uint32_t* sel, int cnt, T prior[KMAX]) {
+    const int H = W, center = y * W + x;
+    for (int j = 0; j < KMAX; ++j) prior[j] = 0;
+    const int direct[4] = {y > 0 ? center - W : -1, y < H - 1 ? center + W : -1,
+                           x > 0 ? center - 1 : -1, x < W - 1 ? center + 1 : -1};
+    for (int ii = 0; ii < 4; ++ii) {
+        if (direct[ii] < 0) continue;
+        const uint32_t sp = sel[direct[ii]];
+        for (int j = 0; j < cnt; ++j) prior[j] += ((sp >> j) & 1u) ? (T)0.9 : (T)0.1;
+    }
+}
+
+// Sampling probabilities, CDF and the 15 view draws of CheckerboardUpdateFrame
+// (AdaptMIPMFrame.cu:817-871, TransformPDFToCDF CudaUtil.cu:491-507): for view i over the slots
+// present, count = #(cost < thr) with thr = 0.8 exp(-iter^2/90), tmpw = sum exp(-c^2/0.18) of
+// those, count_false = #(cost > 1.2); p_i = tmpw/count if count > 2 and count_false < 3, else
+// exp(-thr^2/0.32) if count_false < 3, else 0; times the prior.  Each draw r = u - FLT_EPSILON
+// picks the first view whose CDF value exceeds r.  No view drawn: all views, weight 1.  prob /
+// cdf (if not null) receive the probabilities and CDF.
+template <typename T>
+static void view_select(T ca[8][KMAX], const bool flag[8], const T prior[KMAX], int cnt,
+                        int iter, Stream& st, T vw[KMAX], uint32_t& tsel, T* prob_out,
+                        T* cdf_out) {
+    T prob[KMAX];
+    const T thr = (T)0.8 * std::exp((T)(iter * iter) / (T)-90.0);
+    for (int i = 0; i < cnt; ++i) {
+        T count = 0, tmpw = 0;
+        int cf = 0;
+        for (int j = 0; j < 8; ++j) {
+            if (!flag[j]) continue;
+            const T c = ca[j][i];
+            if (c < thr) { tmpw += std::exp(c * c / (T)-0.18); count += (T)1; }
+            if (c > (T)1.2) ++cf;
+        }
+        T pr = 0;
+        if (count > (T)2 && cf < 3) pr = tmpw / count;
+        else if (cf < 3) pr = std::exp(thr * thr / (T)-0.32);
+        prob[i] = pr * prior[i];
+        if (prob_out) prob_out[i] = prob[i];
+    }
+    T psum = 0;
+    for (int i = 0; i < cnt; ++i) psum += prob[i];
+    for (int i = 0; i < KMAX; ++i) vw[i] = 0;
+    if (psum > (T)0) {
+        const T inv = (T)1 / psum;
+        T cum = 0;
+        for (int i = 0; i < cnt; ++i) { cum += prob[i] * inv; prob[i] = cum; }
+        if (cdf_out) for (int i = 0; i < cnt; ++i) cdf_out[i] = prob[i];
+        for (uint32_t sdraw = 0; sdraw < VIEW_DRAWS; ++sdraw) {
+            const T r = uniform<T>(st.raw(SLOT_VIEW_DRAW + sdraw)) - (T)1.1920928955078125e-07;
+            for (int id = 0; id < cnt; ++id)
+                if (prob[id] > r) { vw[id] += (T)1; break; }
+        }
+    } else if (cdf_out) {
+        for (int i = 0; i < cnt; ++i) cdf_out[i] = 0;
+    }
+    tsel = 0;
+    T wn = 0;
+    for (int i = 0; i < cnt; ++i)
+        if (vw[i] > (T)0) { tsel |= 1u << i; wn += vw[i]; }
+    if (wn <= (T)0) {
+        for (int i = 0; i < cnt; ++i) { vw[i] = (T)1; tsel |= 1u << i; }
+    }
+}
+
+// The 5 refinement hypotheses of PlaneHypothesisRefinement_Frame (AdaptMIPMFrame.cu:400-471)
+// with the candidate set of the ACMP intent (AdaptMIPMUtil.cu:181-183): d_rand = best.w +
+// 2(u - 0.5) if best cost < 0.1, else uniform in [depth_min, depth_max]; n_rand a Marsaglia
+// normal; n_pert the perturbed normal at 0.02 pi; d_pert uniform in [0.98 d, 1.02 d].
+// Candidates: (n_best, d_rand), (n_rand, d), (n_rand, d_rand), (n_pert, d), (n_best, d_pert).
+template <typename T>
+static void refine_candidates(const PmParams& p, int x, int y, const P4<T>& best, T bcost,
+                              Stream& st, P4<T> cand[5]) {
+    const T pert = (T)0.02;
+    const T dnow = best.w;
+    T drand;
+    if (bcost < (T)0.1)
+        drand = (uniform<T>(st.raw(SLOT_REF_DEPTH_RAND)) - (T)0.5) * (T)2 + dnow;
+    else
+        drand = uniform<T>(st.raw(SLOT_REF_DEPTH_RAND)) * ((T)p.depth_max - (T)p.depth_min) +
+                (T)p.depth_min;
+    T nr[3];
+    random_normal<T>(st, SLOT_REF_MARSAGLIA, nr);
+    const P4<T> np = perturbed_normal<T>(st, x, y, best, pert * (T)M_PI);
+    const T lo = ((T)1 - pert) * dnow, hi = ((T)1 + pert) * dnow;
+    const T dpert = uniform<T>(st.raw(SLOT_REF_DEPTH_PERT)) * (hi - lo) + lo;
+    cand[0] = P4<T>{best.x, best.y, best.z, drand};
+    cand[1] = P4<T>{nr[0], nr[1], nr[2], dnow};
+    cand[2] = P4<T>{nr[0], nr[1], nr[2], drand};
+    cand[3] = P4<T>{np.x, np.y, np.z, dnow};
+    cand[4] = P4<T>{best.x, best.y, best.z, dpert};
+}
+
 template <typename T> struct State {
     P4<T>* plane; T* cost; P4<T>* disp; uint32_t* sel;
 };
@@ -482,7 +770,7 @@ static bool warm_seed_valid(const Frame<T>& f, const P4<T>& h, T cost, uint32_t 
 template <typename T>
 static void update_pixel(const Frame<T>& f, const State<T>& prev, State<T>& next, int v, int x,
                          int y, int iter, int phase, const Acmm<T>* ac = nullptr) {
-    const int W = f.W, H = f.W;
+    const int W = f.W;
     const size_t ppv = (size_t)W * W;
     const int center = y * W + x;
     const size_t idx = (size_t)v * ppv + center;
@@ -494,144 +782,30 @@ static void update_pixel(const Frame<T>& f, const State<T>& prev, State<T>& next
         next.sel[idx] = prev.sel[idx];
         return;
     }
-    auto cprev = [&](int c) { return prev.cost[(size_t)v * ppv + c]; };
     auto pprev = [&](int c) { return prev.plane[(size_t)v * ppv + c]; };
 
     Win<T> ww;
     window_weights(f, v, x, y, ww);
 
     T ca[8][KMAX];
-    bool flag[8] = {false, false, false, false, false, false, false, false};
-    int pos[8];
+    bool flag[8];
+    int pos[8], dup[8];
+    propagation_candidates(W, x, y, prev.cost + (size_t)v * ppv, flag, pos, dup);
     for (int a = 0; a < 8; ++a) {
-        pos[a] = center;
-        for (int b = 0; b < KMAX; ++b) ca[a][b] = (T)2;
+        if (!flag[a] || dup[a] >= 0) continue;
+        cost_vector(f, ww, v, x, y, pprev(pos[a]), ca[a]);
     }
-    auto pick = [&](int& best, T& cm, int cand) {
-        if (cprev(cand) < cm) { cm = cprev(cand); best = cand; }
-    };
-    // far candidates: ±3 (±4) along the axis
-    if (y > 2) {
-        flag[1] = true; int b = center - 3 * W; T cm = cprev(b);
-        if (y > 3) pick(b, cm, center - 4 * W);
-        pos[1] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[1]);
-    }
-    if (y < H - 3) {
-        flag[3] = true; int b = center + 3 * W; T cm = cprev(b);
-        if (y < H - 4) pick(b, cm, center + 4 * W);
-        pos[3] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[3]);
-    }
-    if (x > 2) {
-        flag[5] = true; int b = center - 3; T cm = cprev(b);
-        if (x > 3) pick(b, cm, center - 4);
-        pos[5] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[5]);
-    }
-    if (x < W - 3) {
-        flag[7] = true; int b = center + 3; T cm = cprev(b);
-        if (x < W - 4) pick(b, cm, center + 4);
-        pos[7] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[7]);
-    }
-    // near candidates: ±1 plus two alternatives per step i (nerDis = 2)
-    if (y > 0) {
-        flag[0] = true; const int un = center - W; int b = un; T cm = cprev(b);
-        for (int i = 0; i < 2; ++i) {
-            if (y > 1 + i && x > i) pick(b, cm, un - (1 + i) * W - i);
-            if (y > 1 + i && x < W - 1 - i) pick(b, cm, un - (1 + i) * W + i);
-        }
-        pos[0] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[0]);
-    }
-    if (y <= 2 && y > 0) {
-        flag[1] = true; pos[1] = pos[0];
-        for (int j = 0; j < KMAX; ++j) ca[1][j] = ca[0][j];
-    }
-    if (y < H - 1) {
-        flag[2] = true; const int dn = center + W; int b = dn; T cm = cprev(b);
-        for (int i = 0; i < 2; ++i) {
-            if (y < H - 2 - i && x > i) pick(b, cm, dn + (1 + i) * W - i);
-            if (y < H - 2 - i && x < W - 1 - i) pick(b, cm, dn + (1 + i) * W + i);
-        }
-        pos[2] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[2]);
-    }
-    if (y >= H - 3 && y < H - 1) {
-        flag[3] = true; pos[3] = pos[2];
-        for (int j = 0; j < KMAX; ++j) ca[3][j] = ca[2][j];
-    }
-    if (x > 0) {
-        flag[4] = true; const int ln = center - 1; int b = ln; T cm = cprev(b);
-        for (int i = 0; i < 2; ++i) {
-            if (x > 1 + i && y > i) pick(b, cm, ln - (1 + i) - i * W);
-            if (x > 1 + i && y < H - 1 - i) pick(b, cm, ln - (1 + i) + i * W);
-        }
-        pos[4] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[4]);
-    }
-    if (x <= 2 && x > 0) {
-        flag[5] = true; pos[5] = pos[4];
-        for (int j = 0; j < KMAX; ++j) ca[5][j] = ca[4][j];
-    }
-    if (x < W - 1) {
-        flag[6] = true; const int rn = center + 1; int b = rn; T cm = cprev(b);
-        for (int i = 0; i < 2; ++i) {
-            if (x < W - 2 - i && y > i) pick(b, cm, rn + (1 + i) - i * W);
-            if (x < W - 2 - i && y < H - 1 - i) pick(b, cm, rn + (1 + i) + i * W);
-        }
-        pos[6] = b; cost_vector(f, ww, v, x, y, pprev(b), ca[6]);
-    }
-    if (x >= W - 3 && x < W - 1) {
-        flag[7] = true; pos[7] = pos[6];
-        for (int j = 0; j < KMAX; ++j) ca[7][j] = ca[6][j];
+    for (int a = 0; a < 8; ++a) {
+        if (!flag[a]) { for (int b = 0; b < KMAX; ++b) ca[a][b] = (T)2; continue; }
+        if (dup[a] >= 0) for (int b = 0; b < KMAX; ++b) ca[a][b] = ca[dup[a]][b];
     }
 
-    // view-selection priors from the 4 direct neighbours' previous selections
     T prior[KMAX];
-    for (int j = 0; j < KMAX; ++j) prior[j] = 0;
-    const int direct[4] = {y > 0 ? center - W : -1, y < H - 1 ? center + W : -1,
-                           x > 0 ? center - 1 : -1, x < W - 1 ? center + 1 : -1};
-    const int same_flag[4] = {0, 2, 4, 6};
-    for (int ii = 0; ii < 4; ++ii) {
-        if (direct[ii] < 0 || !flag[same_flag[ii]]) continue;
-        const uint32_t sp = prev.sel[(size_t)v * ppv + direct[ii]];
-        for (int j = 0; j < cnt; ++j) prior[j] += ((sp >> j) & 1u) ? (T)0.9 : (T)0.1;
-    }
-
-    // sampling probabilities -> CDF -> 15 draws
-    T prob[KMAX];
-    const T thr = (T)0.8 * std::exp((T)(iter * iter) / (T)-90.0);
-    for (int i = 0; i < cnt; ++i) {
-        T count = 0, tmpw = 0;
-        int cf = 0;
-        for (int j = 0; j < 8; ++j) {
-            if (!flag[j]) continue;
-            const T c = ca[j][i];
-            if (c < thr) { tmpw += std::exp(c * c / (T)-0.18); count += (T)1; }
-            if (c > (T)1.2) ++cf;
-        }
-        T pr = 0;
-        if (count > (T)2 && cf < 3) pr = tmpw / count;
-        else if (cf < 3) pr = std::exp(thr * thr / (T)-0.32);
-        prob[i] = pr * prior[i];
-    }
-    T psum = 0;
-    for (int i = 0; i < cnt; ++i) psum += prob[i];
-    T vw[KMAX];
-    for (int i = 0; i < KMAX; ++i) vw[i] = 0;
+    view_priors(W, x, y, prev.sel + (size_t)v * ppv, cnt, prior);
     Stream st(f.p.seed, f.p.frame_uid, (uint32_t)idx, (uint32_t)(1 + 2 * iter + phase));
-    if (psum > (T)0) {
-        const T inv = (T)1 / psum;
-        T cum = 0;
-        for (int i = 0; i < cnt; ++i) { cum += prob[i] * inv; prob[i] = cum; }
-        for (uint32_t sdraw = 0; sdraw < VIEW_DRAWS; ++sdraw) {
-            const T r = uniform<T>(st.raw(SLOT_VIEW_DRAW + sdraw)) - (T)1.1920928955078125e-07;
-            for (int id = 0; id < cnt; ++id)
-                if (prob[id] > r) { vw[id] += (T)1; break; }
-        }
-    }
+    T vw[KMAX];
     uint32_t tsel = 0;
-    T wn = 0;
-    for (int i = 0; i < cnt; ++i)
-        if (vw[i] > (T)0) { tsel |= 1u << i; wn += vw[i]; }
-    if (wn <= (T)0) {
-        for (int i = 0; i < cnt; ++i) { vw[i] = (T)1; tsel |= 1u << i; }
-    }
+    view_select<T>(ca, flag, prior, cnt, iter, st, vw, tsel, nullptr, nullptr);
 
     P4<T> best = prev.plane[idx];
     T bcost;
@@ -670,25 +844,8 @@ static void update_pixel(const Frame<T>& f, const State<T>& prev, State<T>& next
     }
     // refinement (ACMP depth x normal candidates)
     {
-        const T pert = (T)0.02;
-        const T dnow = best.w;
-        T drand;
-        if (bcost < (T)0.1)
-            drand = (uniform<T>(st.raw(SLOT_REF_DEPTH_RAND)) - (T)0.5) * (T)2 + dnow;
-        else
-            drand = uniform<T>(st.raw(SLOT_REF_DEPTH_RAND)) *
-                        ((T)f.p.depth_max - (T)f.p.depth_min) + (T)f.p.depth_min;
-        T nr[3];
-        random_normal<T>(st, SLOT_REF_MARSAGLIA, nr);
-        const P4<T> np = perturbed_normal<T>(st, x, y, best, pert * (T)M_PI);
-        const T lo = ((T)1 - pert) * dnow, hi = ((T)1 + pert) * dnow;
-        const T dpert = uniform<T>(st.raw(SLOT_REF_DEPTH_PERT)) * (hi - lo) + lo;
         P4<T> cand[5];
-        cand[0] = P4<T>{best.x, best.y, best.z, drand};
-        cand[1] = P4<T>{nr[0], nr[1], nr[2], dnow};
-        cand[2] = P4<T>{nr[0], nr[1], nr[2], drand};
-        cand[3] = P4<T>{np.x, np.y, np.z, dnow};
-        cand[4] = P4<T>{best.x, best.y, best.z, dpert};
+        refine_candidates(f.p, x, y, best, bcost, st, cand);
         for (int i = 0; i < 5; ++i) {
             T cv[KMAX];
             cost_vector(f, ww, v, x, y, cand[i], cv);
@@ -1048,7 +1205,8 @@ static void solve_acmm(const uint8_t* tiles_u8, const double* blur, const float*
 template <typename T>
 static void eval_costs(const uint8_t* tiles_u8, const double* blur, const float* centers,
                        const int* nbr, const int* cnt, int V, const PmParams& p,
-                       const int32_t* hyp, int n, T* out, int nthreads) {
+                       const int32_t* hyp, int n, T* out, int nthreads,
+                       double* margins = nullptr) {
     Frame<T> f;
     build_frame(f, tiles_u8, blur, centers, nbr, cnt, V, p);
     const int K = p.max_neighbors, W = p.crop;
@@ -1063,6 +1221,18 @@ static void eval_costs(const uint8_t* tiles_u8, const double* blur, const float*
         Win<T> ww;
         const int x = pix % W, y = pix / W;
         window_weights(f, v, x, y, ww);
+        if (margins) {
+            // per view: cost, then the margins of NccMargins (views past the count: 2)
+            for (int k = 0; k < K; ++k) {
+                NccMargins mg{INFINITY, INFINITY, 0};
+                const T c = k < f.count(v) ? ncc(f, ww, v, k, x, y, h, &mg) : (T)2;
+                out[(size_t)i * K + k] = c;
+                margins[3 * ((size_t)i * K + k)] = mg.geo;
+                margins[3 * ((size_t)i * K + k) + 1] = mg.gate;
+                margins[3 * ((size_t)i * K + k) + 2] = mg.ferr;
+            }
+            continue;
+        }
         T cv[KMAX];
         cost_vector(f, ww, v, x, y, h, cv);
         for (int k = 0; k < K; ++k) out[(size_t)i * K + k] = cv[k];
@@ -1741,6 +1911,16 @@ void orc_eval_costs_f64(const uint8_t* tiles, const double* blur, const float* c
     orc::eval_costs<double>(tiles, blur, centers, nbr, cnt, V, *p, hyp, n, out, nthreads);
 }
 
+// FP64 costs with, per (hypothesis, view), the margins of NccMargins: n x K x (geo, gate,
+// ferr).
+void orc_eval_margins_f64(const uint8_t* tiles, const double* blur, const float* centers,
+                          const int* nbr, const int* cnt, int V, const orc::PmParams* p,
+                          const int32_t* hyp, int n, double* out, double* margins,
+                          int nthreads) {
+    orc::eval_costs<double>(tiles, blur, centers, nbr, cnt, V, *p, hyp, n, out, nthreads,
+                            margins);
+}
+
 void orc_eval_costs_f32(const uint8_t* tiles, const double* blur, const float* centers,
                         const int* nbr, const int* cnt, int V, const orc::PmParams* p,
                         const int32_t* hyp, int n, float* out, int nthreads) {
@@ -1806,6 +1986,56 @@ void orc_v2r_convert(const float* v, int n, const double* c, double* z) {
         const double vv = v[i];
         const double zz = (vv * c[1] + c[2]) / (1.0 - vv * c[0]);
         z[i] = (std::isnan(zz) || std::isinf(zz) || zz < 0) ? 0.0 : zz;
+    }
+}
+
+// Pieces of the checkerboard update, FP64, for the pins.  Draws come from `words`, the raw
+// 32-bit word of every stream slot (64 slots: views 0..14, refinement 16/17, angles 20..22,
+// Marsaglia 24..55).
+void orc_propagation_candidates_f64(int W, int x, int y, const double* cost, int* flag, int* pos,
+                                    int* dup) {
+    bool fl[8];
+    orc::propagation_candidates<double>(W, x, y, cost, fl, pos, dup);
+    for (int a = 0; a < 8; ++a) flag[a] = fl[a] ? 1 : 0;
+}
+
+void orc_view_priors_f64(int W, int x, int y, const uint32_t* sel, int cnt, double* prior) {
+    orc::view_priors<double>(W, x, y, sel, cnt, prior);
+}
+
+// ca: 8 x 32 candidate costs (slot-major), flag: 8 ints, prior: cnt values.
+void orc_view_select_f64(const double* ca, const int* flag, const double* prior, int cnt,
+                         int iter, const uint32_t* words, double* prob, double* cdf,
+                         double* vw, uint32_t* tsel) {
+    double c[8][orc::KMAX];
+    bool fl[8];
+    double pr[orc::KMAX] = {0};
+    for (int a = 0; a < 8; ++a) {
+        fl[a] = flag[a] != 0;
+        for (int b = 0; b < orc::KMAX; ++b) c[a][b] = ca[a * orc::KMAX + b];
+    }
+    for (int i = 0; i < cnt; ++i) pr[i] = prior[i];
+    orc::Stream st(words);
+    orc::view_select<double>(c, fl, pr, cnt, iter, st, vw, *tsel, prob, cdf);
+}
+
+void orc_perturbed_normal_f64(const double* h, int px, int py, double perturbation,
+                              const uint32_t* words, double* out) {
+    orc::Stream st(words);
+    const orc::P4<double> hh{h[0], h[1], h[2], h[3]};
+    const orc::P4<double> o = orc::perturbed_normal<double>(st, px, py, hh, perturbation);
+    out[0] = o.x; out[1] = o.y; out[2] = o.z; out[3] = o.w;
+}
+
+void orc_refine_candidates_f64(const orc::PmParams* p, int x, int y, const double* best,
+                               double bcost, const uint32_t* words, double* cand) {
+    orc::Stream st(words);
+    const orc::P4<double> b{best[0], best[1], best[2], best[3]};
+    orc::P4<double> c[5];
+    orc::refine_candidates<double>(*p, x, y, b, bcost, st, c);
+    for (int i = 0; i < 5; ++i) {
+        cand[4 * i] = c[i].x; cand[4 * i + 1] = c[i].y;
+        cand[4 * i + 2] = c[i].z; cand[4 * i + 3] = c[i].w;
     }
 }
 

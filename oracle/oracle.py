@@ -220,6 +220,28 @@ def eval_costs(tiles_u8, centers, nbr, cnt, pm, hyp, dtype=np.float64, threads=N
     return out
 
 
+def eval_margins(tiles_u8, centers, nbr, cnt, pm, hyp, threads=None, blur_u8=None, blur=None):
+    """FP64 costs (n, K) and, per (hypothesis, view), where FP32 may legitimately disagree:
+    geo (n, K), the smallest move in px of the anchor or a window sample's source point that
+    changes its inside-the-crop test; gate (n, K), the distance of the window variances from
+    the variance gate in units of their FP32 error estimate (inf where the evaluation ends
+    before the gate); ferr (n, K), the first-order FP32 error estimate of the cost (0 where it
+    is 2 by an early return)."""
+    V = tiles_u8.shape[0]
+    tiles_u8 = np.ascontiguousarray(tiles_u8, np.uint8)
+    bv = _blur_values(tiles_u8, pm, blur_u8, blur)
+    n = hyp.shape[0]
+    K = pm.max_neighbors
+    out = np.zeros((n, K), np.float64)
+    mg = np.zeros((n, K, 3), np.float64)
+    lib().orc_eval_margins_f64(
+        _p(tiles_u8), _p(bv) if bv is not None else None,
+        _p(np.ascontiguousarray(centers, np.float32)), _p(np.ascontiguousarray(nbr, np.int32)),
+        _p(np.ascontiguousarray(cnt, np.int32)), V, C.byref(pm),
+        _p(np.ascontiguousarray(hyp, np.int32)), n, _p(out), _p(mg), threads or nthreads())
+    return out, mg[..., 0], mg[..., 1], mg[..., 2]
+
+
 def filter_f32(gray, centers, nbr, cnt, crop, K, base, fp, plane, cost, disp, sel, threads=None):
     V = centers.shape[0]
     n = V * crop * crop
@@ -365,3 +387,82 @@ def acmm_levels(W):
     L = lib().orc_acmm_levels(W, _p(width), _p(scale), _p(radius), _p(iters), _p(topk))
     return [dict(width=int(width[i]), scale=float(scale[i]), radius=int(radius[i]),
                  iterations=int(iters[i]), top_k=int(topk[i])) for i in range(L)]
+
+
+# --- pieces of the checkerboard update (FP64), fed with explicit stream words ---------------
+KMAX = 32
+SLOTS = 64
+
+
+def words_from_uniforms(slot_uniforms):
+    """Raw stream words whose uniform ((x >> 8) + 0.5) * 2^-24 is the given value, rounded down
+    to the 2^-24 grid: dict slot -> u in (0, 1).  Unset slots hold 0x80000000 (u = 0.5 + 2^-25)."""
+    w = np.full(SLOTS, 0x80000000, np.uint32)
+    for s, u in slot_uniforms.items():
+        k = int(np.floor(float(u) * 2 ** 24 - 0.5))
+        k = min(max(k, 0), 2 ** 24 - 1)
+        w[s] = np.uint32(k << 8)
+    return w
+
+
+def uniform_of_word(x):
+    return ((int(x) >> 8) + 0.5) * 2.0 ** -24
+
+
+def propagation_candidates(W, x, y, cost, L=None):
+    """(flag[8], pos[8], dup[8]) of pixel (x, y); cost = the MI's W*W previous costs."""
+    L = L or lib()
+    cost = np.ascontiguousarray(cost, np.float64).reshape(-1)
+    assert cost.size == W * W
+    flag = np.zeros(8, np.int32)
+    pos = np.zeros(8, np.int32)
+    dup = np.zeros(8, np.int32)
+    L.orc_propagation_candidates_f64(W, x, y, _p(cost), _p(flag), _p(pos), _p(dup))
+    return flag.astype(bool), pos, dup
+
+
+def view_priors(W, x, y, sel, cnt, L=None):
+    L = L or lib()
+    sel = np.ascontiguousarray(sel, np.uint32).reshape(-1)
+    assert sel.size == W * W
+    prior = np.zeros(KMAX)
+    L.orc_view_priors_f64(W, x, y, _p(sel), cnt, _p(prior))
+    return prior[:cnt]
+
+
+def view_select(ca, flag, prior, iter_, words, L=None):
+    """ca (8, cnt) candidate costs, flag (8,), prior (cnt,): returns prob, cdf, vw, tsel."""
+    L = L or lib()
+    ca = np.asarray(ca, np.float64)
+    cnt = ca.shape[1]
+    c = np.full((8, KMAX), 2.0)
+    c[:, :cnt] = ca
+    fl = np.ascontiguousarray(np.asarray(flag, bool).astype(np.int32))
+    pr = np.zeros(KMAX)
+    pr[:cnt] = prior
+    prob = np.zeros(KMAX)
+    cdf = np.zeros(KMAX)
+    vw = np.zeros(KMAX)
+    tsel = C.c_uint32()
+    L.orc_view_select_f64(_p(c), _p(fl), _p(pr), cnt, int(iter_),
+                          _p(np.ascontiguousarray(words, np.uint32)), _p(prob), _p(cdf), _p(vw),
+                          C.byref(tsel))
+    return prob[:cnt], cdf[:cnt], vw[:cnt], tsel.value
+
+
+def perturbed_normal(h, px, py, perturbation, words, L=None):
+    L = L or lib()
+    out = np.zeros(4)
+    L.orc_perturbed_normal_f64(_p(np.ascontiguousarray(h, np.float64)), int(px), int(py),
+                               C.c_double(perturbation),
+                               _p(np.ascontiguousarray(words, np.uint32)), _p(out))
+    return out
+
+
+def refine_candidates(pm, x, y, best, bcost, words, L=None):
+    L = L or lib()
+    out = np.zeros((5, 4))
+    L.orc_refine_candidates_f64(C.byref(pm), int(x), int(y),
+                                _p(np.ascontiguousarray(best, np.float64)), C.c_double(bcost),
+                                _p(np.ascontiguousarray(words, np.uint32)), _p(out))
+    return out

@@ -9,7 +9,7 @@ import time
 import numpy as np
 import pytest
 
-from _helpers import cone_hypotheses, fp_to_oracle, pm_to_oracle, rel_err
+from _helpers import cone_hypotheses, cost_bar, fp_to_oracle, pm_to_oracle
 from conftest import stats_dump
 
 pytestmark = pytest.mark.gpu
@@ -44,10 +44,11 @@ def test_slice_exact(lfpm, orc, c1, torch_cuda):
 
 @pytest.mark.parametrize("cfg", ["c1", "c2params", "c2params_fp16"])
 def test_eval_costs_vs_fp64(lfpm, orc, torch_cuda, cfg):
-    """Fixed-plane bilateral NCC (ComputeBilateralNCC_Frame) vs the FP64 oracle.  Relative error
-    |c - c*| / max(|c*|, 1e-3); both sides return exactly 2 for unscored views.  FP32 tiles:
-    >= 99.5 % within 1e-5 and >= 99.9 % within 1e-4.  FP16 tiles in LDS (FP32 arithmetic):
-    >= 99 % within 2e-3."""
+    """Fixed-plane bilateral NCC (ComputeBilateralNCC_Frame) vs the FP64 oracle on 40,000 cone
+    hypotheses x all views.  Relative error |c - c*| / max(|c*|, 1e-3) at most 1e-5 (FP32 tiles)
+    or 2e-3 (FP16 tiles in LDS, FP32 arithmetic) on every evaluation, except where FP32 cannot
+    decide it (cost_bar: straddles of the crop edge or the variance gate, ill-conditioned
+    windows); those excused evaluations are a small stated fraction."""
     _say("case eval_costs", cfg)
     if cfg == "c1":
         wl = lfpm.workload_c1()
@@ -60,24 +61,15 @@ def test_eval_costs_vs_fp64(lfpm, orc, torch_cuda, cfg):
     hyp = cone_hypotheses(lfpm, wl, 40000, seed=7)
     out = lfpm.eval_costs(df, torch_cuda.from_numpy(hyp).cuda()).cpu().numpy()
     tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
-    ref = orc.eval_costs(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"],
-                         pm_to_oracle(orc, wl.pm), hyp)
-    both = (out < 2) & (ref < 2)
-    flip = ((out < 2) != (ref < 2)).mean()
-    e = rel_err(out[both].astype(np.float64), ref[both])
-    st = dict(cfg=cfg, n=int(both.sum()), flip=float(flip), max=float(e.max()),
-              p999=float(np.quantile(e, 0.999)), p99=float(np.quantile(e, 0.99)),
-              median=float(np.median(e)), frac_le_1e5=float((e <= 1e-5).mean()),
-              frac_le_1e4=float((e <= 1e-4).mean()), frac_le_2e3=float((e <= 2e-3).mean()))
+    ref, geo, gate, ferr = orc.eval_margins(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"],
+                                            pm_to_oracle(orc, wl.pm), hyp)
+    tol = 2e-3 if fp16 else 1e-5
+    st = dict(cfg=cfg, tol=tol, **cost_bar(out, ref, geo, gate, ferr, tol))
     _say("eval_costs", st)
     stats_dump("eval_costs_" + cfg, st)
-    # the tail are windows whose edge samples fall on opposite sides of the crop boundary
-    assert flip < 1e-3
-    if fp16:
-        assert st["frac_le_2e3"] >= 0.99
-    else:
-        assert st["frac_le_1e5"] >= 0.995
-        assert st["frac_le_1e4"] >= 0.999
+    assert st["unexplained"] == 0, st
+    assert st["excused_frac"] <= 0.15, st
+    assert st["frac_le_tol"] >= 0.995, st
 
 
 def _solve_gpu(lfpm, df, torch_cuda, pm=None):
@@ -119,7 +111,8 @@ def test_solve_c1_vs_fp64(lfpm, orc, c1, torch_cuda):
 
 
 def test_solve_c1_fp16_storage_vs_fp64(lfpm, orc, torch_cuda):
-    """FP16 tiles in LDS: full-solve disparity agreement with the FP64 oracle, stated."""
+    """FP16 tiles in LDS: full-solve disparities within 0.05 px of the FP64 oracle on >= 99.5 %
+    of valid pixels (the FP32 bar)."""
     import copy
     wl = lfpm.workload_c1()
     pm = copy.copy(wl.pm)
@@ -134,7 +127,7 @@ def test_solve_c1_fp16_storage_vs_fp64(lfpm, orc, torch_cuda):
               frac_le_01=float((err[valid] <= 0.1).mean()), median=float(np.median(err)))
     _say("solve_c1_fp16", st)
     stats_dump("solve_c1_fp16", st)
-    assert st["frac_le_01"] >= 0.95
+    assert st["frac_le_005"] >= 0.995
 
 
 def test_solve_deterministic(lfpm, c1, torch_cuda):
@@ -371,7 +364,8 @@ def test_slice_blur_tiles_exact(lfpm, orc, torch_cuda):
 def test_eval_costs_blur_vs_fp64(lfpm, orc, torch_cuda, cfg, src):
     """Blur term of the cost vector (lambda 2, AdaptMIPMFrame.cu:211-218) with per-MI blur
     values from the synthetic defocus map or the Laplacian blur score: fixed-plane costs vs the
-    FP64 oracle, same bars as the FP32 NCC (>= 99.5 % within 1e-5, >= 99.9 % within 1e-4)."""
+    FP64 oracle at the FP32 bar of test_eval_costs_vs_fp64 (1e-5 on every evaluation that FP32
+    can decide; the error estimate includes the blur sums)."""
     _say("case eval_costs_blur", cfg, src)
     wl = lfpm.workload_c1() if cfg == "c1" else lfpm.workload_c2(raw_w=1024, raw_h=800)
     wl.pm.blur_lambda = 2.0
@@ -380,24 +374,20 @@ def test_eval_costs_blur_vs_fp64(lfpm, orc, torch_cuda, cfg, src):
     hyp = cone_hypotheses(lfpm, wl, 40000, seed=11)
     out = lfpm.eval_costs(df, torch_cuda.from_numpy(hyp).cuda()).cpu().numpy()
     tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
-    ref = orc.eval_costs(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"],
-                         pm_to_oracle(orc, wl.pm), hyp, blur=bval)
+    ref, geo, gate, ferr = orc.eval_margins(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"],
+                                            pm_to_oracle(orc, wl.pm), hyp, blur=bval)
     pm0 = pm_to_oracle(orc, wl.pm)
     pm0.blur_lambda = 0.0
     noblur = orc.eval_costs(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm0, hyp)
     both = (out < 2) & (ref < 2)
-    flip = ((out < 2) != (ref < 2)).mean()
-    e = rel_err(out[both].astype(np.float64), ref[both])
-    st = dict(cfg=cfg, src=src, n=int(both.sum()), flip=float(flip), max=float(e.max()),
-              p999=float(np.quantile(e, 0.999)), median=float(np.median(e)),
-              frac_le_1e5=float((e <= 1e-5).mean()), frac_le_1e4=float((e <= 1e-4).mean()),
+    st = dict(cfg=cfg, src=src, **cost_bar(out, ref, geo, gate, ferr, 1e-5),
               term_active=float((ref[both] > noblur[both] + 1e-6).mean()))
     _say("eval_costs_blur", st)
     stats_dump("eval_costs_blur_%s_%s" % (cfg, src), st)
     assert st["term_active"] > 0.5
-    assert flip < 1e-3
-    assert st["frac_le_1e5"] >= 0.995
-    assert st["frac_le_1e4"] >= 0.999
+    assert st["unexplained"] == 0, st
+    assert st["excused_frac"] <= 0.15, st
+    assert st["frac_le_tol"] >= 0.995, st
 
 
 @pytest.mark.parametrize("src", ["defocus", "laplacian"])
