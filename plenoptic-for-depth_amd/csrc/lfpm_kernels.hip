@@ -178,65 +178,6 @@ struct WorkCount {
     unsigned int samples, triples;
 };
 
-template <bool COUNT>
-__device__ __forceinline__ float ncc_eval(const float* __restrict__ ref,
-                                          const float* __restrict__ src,
-                                          const float* __restrict__ spat, const DevParams& P,
-                                          int px, int py, float rc, float4 h, float sx, float sy,
-                                          WorkCount& wc) {
-    const int W = P.W;
-    const float Wf = (float)W;
-    const float ax = (float)px + sx * h.w, ay = (float)py + sy * h.w;
-    if (!(ax >= 0.0f && ax < Wf && ay >= 0.0f && ay < Wf)) return 2.0f;
-    if (!(fabsf(h.z) >= P.cos_max_slant)) return 2.0f;  // slant gate
-    if (COUNT) ++wc.triples;
-    const float A = -h.x / h.z, B = -h.y / h.z;
-    const int r = P.r, inc = P.inc, n = 2 * r + 1;
-    float s_w = 0.0f, s_x = 0.0f, s_xx = 0.0f, s_y = 0.0f, s_yy = 0.0f, s_xy = 0.0f;
-    for (int i = -r; i <= r; i += inc) {
-        const int rx = px + i;
-        if (rx < 0 || rx >= W) continue;
-        const float di = h.w + A * (float)i;
-        const float* spat_i = spat + (i + r);
-        for (int j = -r; j <= r; j += inc) {
-            const int ry = py + j;
-            if (ry < 0 || ry >= W) continue;
-            const float d = di + B * (float)j;
-            const float qx = (float)rx + sx * d, qy = (float)ry + sy * d;
-            if (!(qx >= 0.0f && qx < Wf && qy >= 0.0f && qy < Wf)) continue;
-            const float rv = ref[ry * W + rx];
-            const float x = rv - rc;
-            const float wgt = __expf(spat_i[(j + r) * n] - fabsf(x) * P.inv_sc);
-            const float fx0 = floorf(qx), fy0 = floorf(qy);
-            const int x0 = (int)fx0, y0 = (int)fy0;
-            const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, W - 1);
-            const float fx = qx - fx0, fy = qy - fy0;
-            const float a = src[y0 * W + x0], b = src[y0 * W + x1];
-            const float c = src[y1 * W + x0], e = src[y1 * W + x1];
-            const float top = a + fx * (b - a);
-            const float bot = c + fx * (e - c);
-            const float y = (top + fy * (bot - top)) - rc;
-            const float wx = wgt * x;
-            if (COUNT) ++wc.samples;
-            s_w += wgt;
-            s_x += wx;
-            s_xx += wx * x;
-            s_y += wgt * y;
-            s_yy += wgt * y * y;
-            s_xy += wx * y;
-        }
-    }
-    if (s_w <= 1e-6f) return 2.0f;
-    const float inv = 1.0f / s_w;
-    const float mx = s_x * inv, my = s_y * inv;
-    const float vx = s_xx * inv - mx * mx;
-    const float vy = s_yy * inv - my * my;
-    if (vx < P.min_var || vy < P.min_var) return 2.0f;
-    const float cov = s_xy * inv - mx * my;
-    const float c = 1.0f - cov / sqrtf(vx * vy);
-    return fminf(2.0f, fmaxf(0.0f, c));
-}
-
 // LDS carve-up shared by the solver kernels.
 struct Lds {
     float* tiles;     // (K+1) slots of (W+1)^2: slot 0 = reference, slot 1+k = neighbour k;
@@ -247,42 +188,10 @@ struct Lds {
     float2* step;     // K: (c_ref - c_nei) / Base
     float2* cdiff;    // K: c_ref - c_nei
     int* nid;         // K
-    uint8_t* wts;     // K*NT view weights (phase kernel)
-    uint8_t* slist;   // K*NT selected-view lists (phase kernel)
     float* btiles;    // blur term: blur values of the same (K+1) slots, FP32 (else null)
     unsigned* next;   // wave engine: next pixel of the current MI to hand out
     int* cur;         // wave engine: the workgroup's current MI (-1: none left)
 };
-
-static size_t lds_bytes(const DevParams& P, int NT, bool phase) {
-    const int n = 2 * P.r + 1;
-    size_t b = 0;
-    b += (((size_t)(P.K + 1) * P.TS * P.tile_bytes) + 15) & ~(size_t)15;
-    b += (size_t)P.WW * 4;
-    b += (size_t)n * n * 4;
-    b += (size_t)P.K * 8 * 2;
-    b += (size_t)P.K * 4;
-    if (phase) b += (size_t)P.K * NT * 2;
-    return (b + 15) & ~(size_t)15;
-}
-
-__device__ __forceinline__ Lds carve(unsigned char* base, const DevParams& P, int NT) {
-    Lds L;
-    const int n = 2 * P.r + 1;
-    unsigned char* q = base;
-    L.tiles = (float*)q;
-    L.htiles = (__half*)q;
-    q += (((size_t)(P.K + 1) * P.TS * P.tile_bytes) + 15) & ~(size_t)15;
-    L.costp = (float*)q; q += (size_t)P.WW * 4;
-    L.spat = (float*)q; q += (size_t)n * n * 4;
-    L.step = (float2*)q; q += (size_t)P.K * 8;
-    L.cdiff = (float2*)q; q += (size_t)P.K * 8;
-    L.nid = (int*)q; q += (size_t)P.K * 4;
-    L.wts = (uint8_t*)q; q += (size_t)P.K * NT;
-    L.slist = (uint8_t*)q;
-    L.btiles = nullptr;
-    return L;
-}
 
 // Tile element access for the two LDS storage modes; arithmetic is FP32 in both.
 template <typename TT> __device__ __forceinline__ const TT* tile_base(const Lds& L);
@@ -353,105 +262,6 @@ __device__ __forceinline__ int stage_mi(Lds& L, const DevParams& P, int mi,
         if (btiles) L.btiles[idx] = btiles[(size_t)src * WW + yy * W + xx];
     }
     return c;
-}
-
-// Bilateral NCC of one hypothesis against up to G views (vk[g] = view index, < 0 = unused) with
-// the window loop outside and the views inside: the reference-side terms of a window sample
-// (intensity, bilateral weight, weighted moments) are computed once and shared by the G views.
-// Same definition as ncc_eval (ComputeBilateralNCC_Frame, AdaptMIPMFrame.cu:73-179).
-constexpr int kGroup = 6;
-
-template <bool COUNT, typename TT>
-__device__ __forceinline__ void ncc_group(const Lds& L, const DevParams& P, int px, int py,
-                                          float rc, float4 h, const int (&vk)[kGroup],
-                                          float (&out)[kGroup], WorkCount& wc) {
-    const int W = P.W, W1 = P.W1, TS = P.TS;
-    const float Wf = (float)W;
-    const bool gate = fabsf(h.z) >= P.cos_max_slant;  // slant gate
-    float sxv[kGroup], syv[kGroup];
-    int base[kGroup];
-    bool act[kGroup];
-    bool any = false;
-#pragma unroll
-    for (int g = 0; g < kGroup; ++g) {
-        out[g] = 2.0f;
-        act[g] = false;
-        sxv[g] = 0.0f; syv[g] = 0.0f; base[g] = 0;
-        if (vk[g] >= 0) {
-            const float2 st = L.step[vk[g]];
-            sxv[g] = st.x; syv[g] = st.y;
-            base[g] = (vk[g] + 1) * TS;
-            const float ax = (float)px + st.x * h.w, ay = (float)py + st.y * h.w;
-            if (ax >= 0.0f && ax < Wf && ay >= 0.0f && ay < Wf && gate) {
-                act[g] = true;
-                if (COUNT) ++wc.triples;
-            }
-        }
-        any |= act[g];
-    }
-    const float A = -h.x / h.z, B = -h.y / h.z;
-    // non-finite slopes make every window sample invalid (the result is 2 for every view)
-    if (!any || !(isfinite(A) && isfinite(B))) return;
-    float acc[kGroup][6];
-#pragma unroll
-    for (int g = 0; g < kGroup; ++g)
-#pragma unroll
-        for (int q = 0; q < 6; ++q) acc[g][q] = 0.0f;
-    const int r = P.r, inc = P.inc, n = 2 * r + 1;
-    const float W1f = (float)W1, Wm1 = (float)(W - 1);
-    const float inv_sc = TileScale<TT>::inv_sc(P), min_var = TileScale<TT>::min_var(P);
-    for (int i = -r; i <= r; i += inc) {
-        const int rx = px + i;
-        if ((unsigned)rx >= (unsigned)W) continue;
-        const float di = h.w + A * (float)i;
-        const float* spat_i = L.spat + (i + r);
-        for (int j = -r; j <= r; j += inc) {
-            const int ry = py + j;
-            if ((unsigned)ry >= (unsigned)W) continue;
-            const float d = di + B * (float)j;
-            const float x = ldt(tile_base<TT>(L) + ry * W1 + rx) - rc;
-            const float wgt = __expf(spat_i[(j + r) * n] - fabsf(x) * inv_sc);
-            const float wx = wgt * x;
-            const float wxx = wx * x;
-            // branchless view block: invalid samples get weight 0 and read a clamped in-crop
-            // address, so the six views' LDS reads can be in flight together
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g) {
-                const float qx = (float)rx + sxv[g] * d, qy = (float)ry + syv[g] * d;
-                const bool ok = act[g] && qx >= 0.0f && qx < Wf && qy >= 0.0f && qy < Wf;
-                const float fx0 = floorf(qx), fy0 = floorf(qy);
-                const float fx = qx - fx0, fy = qy - fy0;
-                const float xa = __builtin_amdgcn_fmed3f(fx0, 0.0f, Wm1);
-                const float ya = __builtin_amdgcn_fmed3f(fy0, 0.0f, Wm1);
-                const TT* t = tile_base<TT>(L) + base[g] + (int)(ya * W1f + xa);
-                const float a = ldt(t), b = ldt(t + 1), c = ldt(t + W1), e = ldt(t + W1 + 1);
-                const float top = a + fx * (b - a);
-                const float bot = c + fx * (e - c);
-                const float y = (top + fy * (bot - top)) - rc;
-                const float m = ok ? 1.0f : 0.0f;
-                const float wm = wgt * m;
-                const float wy = wm * y;
-                if (COUNT && ok) ++wc.samples;
-                acc[g][0] += wm;
-                acc[g][1] += wx * m;
-                acc[g][2] += wxx * m;
-                acc[g][3] += wy;
-                acc[g][4] += wy * y;
-                acc[g][5] += wy * x;
-            }
-        }
-    }
-#pragma unroll
-    for (int g = 0; g < kGroup; ++g) {
-        if (!act[g] || acc[g][0] <= 1e-6f) continue;
-        const float inv = 1.0f / acc[g][0];
-        const float mx = acc[g][1] * inv, my = acc[g][3] * inv;
-        const float vx = acc[g][2] * inv - mx * mx;
-        const float vy = acc[g][4] * inv - my * my;
-        if (vx < min_var || vy < min_var) continue;
-        const float cov = acc[g][5] * inv - mx * my;
-        out[g] = fminf(2.0f, fmaxf(0.0f, 1.0f - cov / sqrtf(vx * vy)));
-    }
 }
 
 __device__ __forceinline__ float4 disp_record(const Lds& L, const DevParams& P, int k, float w) {
@@ -607,81 +417,10 @@ struct FrameDesc {
 struct StageArgs {
     FrameDesc f[kMaxFrames];
     int nf, total;
-    float* scratch;   // [grid][9*K][NT]: 8 candidate cost vectors + sampling probabilities
     unsigned long long* counters;  // counting variant: {samples, triples}
     unsigned* tickets;  // wave engine: 8 zeroed per-XCD MI counters (null: fixed MI stride)
     DevParams P;
 };
-
-// Propagation candidate positions (index into the MI, row-major) — the 8 slots of the
-// reference: 0/2/4/6 = near up/down/left/right, 1/3/5/7 = far up/down/left/right.
-// src[c] = c for an evaluated slot, the slot it duplicates near the border, or -1 if absent.
-__device__ __forceinline__ void select_candidates(const float* __restrict__ costp, int W, int x,
-                                                  int y, int pos[8], int src[8]) {
-    const int H = W, center = y * W + x;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) { pos[i] = center; src[i] = -1; }
-    auto pick = [&](int& b, float& cm, int cand) {
-        const float cc = costp[cand];
-        if (cc < cm) { cm = cc; b = cand; }
-    };
-    if (y > 2) {
-        int b = center - 3 * W; float cm = costp[b];
-        if (y > 3) pick(b, cm, center - 4 * W);
-        pos[1] = b; src[1] = 1;
-    }
-    if (y < H - 3) {
-        int b = center + 3 * W; float cm = costp[b];
-        if (y < H - 4) pick(b, cm, center + 4 * W);
-        pos[3] = b; src[3] = 3;
-    }
-    if (x > 2) {
-        int b = center - 3; float cm = costp[b];
-        if (x > 3) pick(b, cm, center - 4);
-        pos[5] = b; src[5] = 5;
-    }
-    if (x < W - 3) {
-        int b = center + 3; float cm = costp[b];
-        if (x < W - 4) pick(b, cm, center + 4);
-        pos[7] = b; src[7] = 7;
-    }
-    if (y > 0) {
-        const int un = center - W; int b = un; float cm = costp[b];
-        for (int i = 0; i < 2; ++i) {
-            if (y > 1 + i && x > i) pick(b, cm, un - (1 + i) * W - i);
-            if (y > 1 + i && x < W - 1 - i) pick(b, cm, un - (1 + i) * W + i);
-        }
-        pos[0] = b; src[0] = 0;
-    }
-    if (y <= 2 && y > 0) { pos[1] = pos[0]; src[1] = 0; }
-    if (y < H - 1) {
-        const int dn = center + W; int b = dn; float cm = costp[b];
-        for (int i = 0; i < 2; ++i) {
-            if (y < H - 2 - i && x > i) pick(b, cm, dn + (1 + i) * W - i);
-            if (y < H - 2 - i && x < W - 1 - i) pick(b, cm, dn + (1 + i) * W + i);
-        }
-        pos[2] = b; src[2] = 2;
-    }
-    if (y >= H - 3 && y < H - 1) { pos[3] = pos[2]; src[3] = 2; }
-    if (x > 0) {
-        const int ln = center - 1; int b = ln; float cm = costp[b];
-        for (int i = 0; i < 2; ++i) {
-            if (x > 1 + i && y > i) pick(b, cm, ln - (1 + i) - i * W);
-            if (x > 1 + i && y < H - 1 - i) pick(b, cm, ln - (1 + i) + i * W);
-        }
-        pos[4] = b; src[4] = 4;
-    }
-    if (x <= 2 && x > 0) { pos[5] = pos[4]; src[5] = 4; }
-    if (x < W - 1) {
-        const int rn = center + 1; int b = rn; float cm = costp[b];
-        for (int i = 0; i < 2; ++i) {
-            if (x < W - 2 - i && y > i) pick(b, cm, rn + (1 + i) - i * W);
-            if (x < W - 2 - i && y < H - 1 - i) pick(b, cm, rn + (1 + i) + i * W);
-        }
-        pos[6] = b; src[6] = 6;
-    }
-    if (x >= W - 3 && x < W - 1) { pos[7] = pos[6]; src[7] = 6; }
-}
 
 __device__ __forceinline__ void active_pixel(int W, int a, int phase, int& x, int& y) {
     if (W & 1) {
@@ -694,370 +433,6 @@ __device__ __forceinline__ void active_pixel(int W, int a, int phase, int& x, in
         x = 2 * (a - y * half) + ((y + phase) & 1);
     }
 }
-
-
-// Random initialisation of every pixel of MI v (RandomInitializationForFrame :474-518).
-template <bool COUNT, typename TT>
-__device__ __forceinline__ void init_mi(const Lds& L, const DevParams& P, const FrameDesc& F,
-                                        int v, int c, float* __restrict__ scr, WorkCount& wc) {
-    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW;
-    for (int pix = t; pix < WW; pix += NT) {
-        const int x = pix % W, y = pix / W;
-        const size_t idx = (size_t)v * WW + pix;
-        DevStream st((uint32_t)idx, 0u, F.seed, F.uid);
-        float4 h;
-        h.w = u01_dev(st.raw(SLOT_INIT_DEPTH)) * (P.dmax - P.dmin) + P.dmin;
-        random_normal(st, SLOT_INIT_MARSAGLIA, h.x, h.y, h.z);
-        const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
-        int valid = 0, best = -1;
-        float bestc = 2.0f;
-        for (int k0 = 0; k0 < c; k0 += kGroup) {
-            int vk[kGroup];
-            float cg[kGroup];
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g)
-                vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
-            ncc_group<COUNT, TT>(L, P, x, y, rc, h, vk, cg, wc);
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g) {
-                const int k = k0 + g;
-                if (k >= c) break;
-                const float cv = cg[g];
-                scr[(size_t)k * NT + t] = cv;
-                if (cv < 2.0f) {
-                    ++valid;
-                    if (best < 0 || cv < bestc) { best = k; bestc = cv; }
-                }
-            }
-        }
-        float mean = 2.0f;
-        uint32_t mask = 0u;
-        if (valid > 0) {
-            // k smallest costs in ascending order (ComputeSelectedViewsTopK_Frame :227-282)
-            const int kk = min(valid, P.top_k);
-            uint32_t taken = 0u;
-            float sum = 0.0f, thr = 2.0f;
-            for (int m = 0; m < kk; ++m) {
-                int bi = -1;
-                float bv = 0.0f;
-                for (int k = 0; k < c; ++k) {
-                    if ((taken >> k) & 1u) continue;
-                    const float cv = scr[(size_t)k * NT + t];
-                    if (bi < 0 || cv < bv) { bi = k; bv = cv; }
-                }
-                taken |= 1u << bi;
-                sum += bv;
-                thr = bv;
-            }
-            mean = sum / (float)kk;
-            for (int k = 0; k < c; ++k)
-                if (scr[(size_t)k * NT + t] <= thr) mask |= 1u << k;
-        }
-        F.plane_out[idx] = h;
-        F.cost_out[idx] = mean;
-        F.sel_out[idx] = mask;
-    }
-}
-
-// Lowest-cost view record of the final plane over every view (the disp argument of
-// ComputeWeightedCost_Frame, AdaptMIPMFrame.cu:303-306).
-template <bool COUNT, typename TT>
-__device__ __forceinline__ void final_mi(const Lds& L, const DevParams& P, const FrameDesc& F,
-                                         int v, int c, WorkCount& wc) {
-    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW;
-    for (int pix = t; pix < WW; pix += NT) {
-        const int x = pix % W, y = pix / W;
-        const size_t idx = (size_t)v * WW + pix;
-        const float4 h = F.plane_in[idx];
-        const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
-        float mc = 2.0f;
-        int am = -1;
-        for (int k0 = 0; k0 < c; k0 += kGroup) {
-            int vk[kGroup];
-            float cg[kGroup];
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g)
-                vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
-            ncc_group<COUNT, TT>(L, P, x, y, rc, h, vk, cg, wc);
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g)
-                if (k0 + g < c && cg[g] < mc) { mc = cg[g]; am = k0 + g; }
-        }
-        F.disp[idx] = am >= 0 ? disp_record(L, P, am, h.w) : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-}
-
-// One red/black phase of MI v: the active lane t owns one pixel of colour F.phase.
-template <bool COUNT, typename TT>
-__device__ __forceinline__ void phase_mi(const Lds& L, const DevParams& P, const FrameDesc& F,
-                                         int v, int c, float* __restrict__ scr, WorkCount& wc) {
-    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW, K = P.K;
-    const int phase = F.phase;
-    const int nact = (W & 1) ? (phase == 0 ? (WW + 1) / 2 : WW / 2) : WW / 2;
-    const size_t base = (size_t)v * WW;
-    // pass-through of the inactive colour (and of every pixel of an MI without neighbours)
-    for (int i = t; i < WW; i += NT) {
-        const int x = i % W, y = i / W;
-        if (((x + y) & 1) != phase || c <= 0) {
-            F.plane_out[base + i] = F.plane_in[base + i];
-            F.cost_out[base + i] = F.cost_in[base + i];
-            F.sel_out[base + i] = F.sel_in[base + i];
-        }
-    }
-    if (c <= 0 || t >= nact) return;
-    int x, y;
-    active_pixel(W, t, phase, x, y);
-    const int center = y * W + x;
-    const size_t idx = base + center;
-    const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
-    // candidate table packed into registers (no dynamically indexed private arrays):
-    // pos of slot cd in bits 10*cd.. of a 128-bit pair, src+1 in bits 4*cd.. of srcp
-    uint64_t posp[2] = {0ull, 0ull};
-    uint32_t srcp = 0u;
-    {
-        int pos[8], src[8];
-        select_candidates(L.costp, W, x, y, pos, src);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            posp[q / 6] |= (uint64_t)pos[q] << (10 * (q % 6));
-            srcp |= (uint32_t)(src[q] + 1) << (4 * q);
-        }
-    }
-    auto cand_src = [&](int cd) -> int { return (int)((srcp >> (4 * cd)) & 15u) - 1; };
-    auto cand_pos = [&](int cd) -> int {
-        return (int)((cd < 6 ? posp[0] >> (10 * cd) : posp[1] >> (10 * (cd - 6))) & 1023ull);
-    };
-
-    // cost vectors of the propagation candidates over every view
-    for (int cd = 0; cd < 8; ++cd) {
-        if (cand_src(cd) != cd) continue;
-        const float4 h = F.plane_in[base + cand_pos(cd)];
-        for (int k0 = 0; k0 < c; k0 += kGroup) {
-            int vk[kGroup];
-            float cg[kGroup];
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g)
-                vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
-            ncc_group<COUNT, TT>(L, P, x, y, rc, h, vk, cg, wc);
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g)
-                if (k0 + g < c) scr[(size_t)(cd * K + k0 + g) * NT + t] = cg[g];
-        }
-    }
-
-    // view-selection priors from the 4 direct neighbours' previous selections
-    uint32_t dsel[4];
-    bool dval[4];
-    dval[0] = y > 0; dval[1] = y < W - 1; dval[2] = x > 0; dval[3] = x < W - 1;
-    dsel[0] = dval[0] ? F.sel_in[idx - W] : 0u;
-    dsel[1] = dval[1] ? F.sel_in[idx + W] : 0u;
-    dsel[2] = dval[2] ? F.sel_in[idx - 1] : 0u;
-    dsel[3] = dval[3] ? F.sel_in[idx + 1] : 0u;
-
-    // sampling probabilities
-    const float thr = 0.8f * expf((float)(F.iter * F.iter) / (-90.0f));
-    float psum = 0.0f;
-    for (int k = 0; k < c; ++k) {
-        float count = 0.0f, tmpw = 0.0f;
-        int cf = 0;
-        for (int cd = 0; cd < 8; ++cd) {
-            const int sc = cand_src(cd);
-            if (sc < 0) continue;
-            const float cv = scr[(size_t)(sc * K + k) * NT + t];
-            if (cv < thr) { tmpw += expf(div_m018(cv * cv)); count += 1.0f; }
-            if (cv > 1.2f) ++cf;
-        }
-        float pr = 0.0f;
-        if (count > 2.0f && cf < 3) pr = tmpw / count;
-        else if (cf < 3) pr = expf(thr * thr / (-0.32f));
-        float prior = 0.0f;
-        for (int ii = 0; ii < 4; ++ii)
-            if (dval[ii]) prior += ((dsel[ii] >> k) & 1u) ? 0.9f : 0.1f;
-        pr *= prior;
-        scr[(size_t)(8 * K + k) * NT + t] = pr;
-        psum += pr;
-    }
-
-    // 15 draws through the CDF (first view whose cumulative probability exceeds the draw)
-    DevStream st((uint32_t)idx, (uint32_t)(1 + 2 * F.iter + phase), F.seed, F.uid);
-    float draw[VIEW_DRAWS];
-#pragma unroll
-    for (int s = 0; s < (int)VIEW_DRAWS; ++s)
-        draw[s] = u01_dev(st.raw(SLOT_VIEW_DRAW + s)) - 1.1920928955078125e-07f;
-    uint32_t tsel = 0u;
-    int nsel = 0;
-    if (psum > 0.0f) {
-        const float inv = 1.0f / psum;
-        float cum = 0.0f;
-        uint32_t taken = 0u;
-        for (int k = 0; k < c; ++k) {
-            cum += scr[(size_t)(8 * K + k) * NT + t] * inv;
-            int wk = 0;
-#pragma unroll
-            for (int s = 0; s < (int)VIEW_DRAWS; ++s)
-                if (!((taken >> s) & 1u) && cum > draw[s]) { ++wk; taken |= 1u << s; }
-            L.wts[k * NT + t] = (uint8_t)wk;
-            if (wk > 0) { tsel |= 1u << k; L.slist[nsel * NT + t] = (uint8_t)k; ++nsel; }
-        }
-    } else {
-        for (int k = 0; k < c; ++k) L.wts[k * NT + t] = 0;
-    }
-    if (nsel == 0) {
-        for (int k = 0; k < c; ++k) {
-            L.wts[k * NT + t] = 1;
-            L.slist[k * NT + t] = (uint8_t)k;
-            tsel |= 1u << k;
-        }
-        nsel = c;
-    }
-
-    // weighted cost over the selected views (ComputeWeightedCost_Frame :284-317)
-    auto wcost = [&](const float4& h) -> float {
-        float wn = 0.0f, tot = 0.0f;
-        for (int j0 = 0; j0 < nsel; j0 += kGroup) {
-            int vk[kGroup];
-            float cg[kGroup];
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g) {
-                vk[g] = -1;
-                if (j0 + g < nsel) {
-                    const int k = L.slist[(j0 + g) * NT + t];
-                    if (L.nid[k] >= 0) vk[g] = k;
-                }
-            }
-            ncc_group<COUNT, TT>(L, P, x, y, rc, h, vk, cg, wc);
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g) {
-                if (j0 + g >= nsel) break;
-                const int k = L.slist[(j0 + g) * NT + t];
-                const float w = (float)L.wts[k * NT + t];
-                wn += w;
-                tot += w * cg[g];
-            }
-        }
-        return wn <= 1e-6f ? 2.0f : tot / wn;
-    };
-
-    float4 best = F.plane_in[idx];
-    float bcost = wcost(best);
-    for (int cd = 0; cd < 8; ++cd) {
-        const int sc = cand_src(cd);
-        if (sc < 0) continue;
-        float wn = 0.0f, tot = 0.0f;
-        for (int j = 0; j < nsel; ++j) {
-            const int k = L.slist[j * NT + t];
-            const float w = (float)L.wts[k * NT + t];
-            wn += w;
-            tot += w * scr[(size_t)(sc * K + k) * NT + t];
-        }
-        const float cc = wn <= 1e-6f ? 2.0f : tot / wn;
-        if (cc < bcost) { bcost = cc; best = F.plane_in[base + cand_pos(cd)]; }
-    }
-
-    // cross-view proposals through each selected view (BuildCrossViewPlaneProposal :350-398)
-    for (int j = 0; j < nsel; ++j) {
-        const int k = L.slist[j * NT + t];
-        const int n = L.nid[k];
-        if (n < 0) continue;
-        const float2 sk = L.step[k];
-        const float qfx = (float)x + sk.x * best.w, qfy = (float)y + sk.y * best.w;
-        const float fqx = floorf(qfx + 0.5f), fqy = floorf(qfy + 0.5f);
-        if (!(fqx >= 0.0f && fqx < (float)W && fqy >= 0.0f && fqy < (float)W)) continue;
-        const int qx = (int)fqx, qy = (int)fqy;
-        const float4 hn = F.plane_in[(size_t)n * WW + qy * W + qx];
-        const float al = -hn.x / hn.z, be = -hn.y / hn.z;
-        const float ga = hn.w - al * (float)qx - be * (float)qy;
-        const float2 cdv = L.cdiff[k];
-        float den = 1.0f + al * cdv.x + be * cdv.y;
-        if (fabsf(den) < 1e-6f) den = den >= 0.0f ? 1e-6f : -1e-6f;
-        const float aR = al / den, bR = be / den, cR = ga / den;
-        float nx = -aR, ny = -bR, nz = 1.0f;
-        normalize3(nx, ny, nz);
-        const float4 prop = make_float4(nx, ny, nz, aR * (float)x + bR * (float)y + cR);
-        const float cc = wcost(prop);
-        if (cc < bcost) { bcost = cc; best = prop; }
-    }
-
-    // refinement: ACMP depth x normal candidates (AdaptMIPMUtil.cu:181-183)
-    {
-        const float pert = 0.02f;
-        const float dnow = best.w;
-        float drand;
-        if (bcost < 0.1f)
-            drand = (u01_dev(st.raw(SLOT_REF_DEPTH_RAND)) - 0.5f) * 2.0f + dnow;
-        else
-            drand = u01_dev(st.raw(SLOT_REF_DEPTH_RAND)) * (P.dmax - P.dmin) + P.dmin;
-        float rnx, rny, rnz;
-        random_normal(st, SLOT_REF_MARSAGLIA, rnx, rny, rnz);
-        // perturbed normal (GeneratePerturbedNormalLF, CudaUtil.cu:105-142)
-        float vx = (float)x, vy = (float)y, vz = 1.0f;
-        normalize3(vx, vy, vz);
-        const float pamp = pert * 3.14159265358979323846f;
-        const float a1 = (u01_dev(st.raw(SLOT_REF_ANGLE)) - 0.5f) * pamp;
-        const float a2 = (u01_dev(st.raw(SLOT_REF_ANGLE + 1)) - 0.5f) * pamp;
-        const float a3 = (u01_dev(st.raw(SLOT_REF_ANGLE + 2)) - 0.5f) * pamp;
-        const float s1 = sinf(a1), s2 = sinf(a2), s3 = sinf(a3);
-        const float c1 = cosf(a1), c2 = cosf(a2), c3 = cosf(a3);
-        float px_ = (c2 * c3) * best.x + (c3 * s1 * s2 - c1 * s3) * best.y +
-                    (s1 * s3 + c1 * c3 * s2) * best.z;
-        float py_ = (c2 * s3) * best.x + (c1 * c3 + s1 * s2 * s3) * best.y +
-                    (c1 * s2 * s3 - c3 * s1) * best.z;
-        float pz_ = (-s2) * best.x + (c2 * s1) * best.y + (c1 * c2) * best.z;
-        if (px_ * vx + py_ * vy + pz_ * vz >= 0.0f) { px_ = best.x; py_ = best.y; pz_ = best.z; }
-        normalize3(px_, py_, pz_);
-        const float lo = (1.0f - pert) * dnow, hi = (1.0f + pert) * dnow;
-        const float dpert = u01_dev(st.raw(SLOT_REF_DEPTH_PERT)) * (hi - lo) + lo;
-        const float4 cur = best;
-        for (int i = 0; i < 5; ++i) {
-            // (n_cur, d_rand), (n_rand, d_now), (n_rand, d_rand), (n_pert, d_now),
-            // (n_cur, d_pert), selected without a dynamically indexed array
-            float4 hc;
-            hc.x = (i == 0 || i == 4) ? cur.x : (i == 3 ? px_ : rnx);
-            hc.y = (i == 0 || i == 4) ? cur.y : (i == 3 ? py_ : rny);
-            hc.z = (i == 0 || i == 4) ? cur.z : (i == 3 ? pz_ : rnz);
-            hc.w = (i == 0 || i == 2) ? drand : (i == 4 ? dpert : dnow);
-            const float cc = wcost(hc);
-            if (cc < bcost) { bcost = cc; best = hc; }
-        }
-    }
-    F.plane_out[idx] = best;
-    F.cost_out[idx] = bcost;
-    F.sel_out[idx] = tsel;
-}
-
-template <bool COUNT, typename TT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-k_stage(StageArgs a) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const DevParams& P = a.P;
-    const int NT = blockDim.x, t = threadIdx.x, WW = P.WW, K = P.K;
-    Lds L = carve(lds_raw, P, NT);
-    fill_spatial(L, P);
-    float* scr = a.scratch + (size_t)blockIdx.x * 9 * K * NT;
-    // XCD-aware contiguous MI ranges over the concatenated index space of the table
-    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
-    const int lo = (int)(((long long)a.total * xcd) >> 3);
-    const int hi = (int)(((long long)a.total * (xcd + 1)) >> 3);
-    WorkCount wc{0u, 0u};
-    int f = 0;
-    for (int gm = lo + local; gm < hi; gm += P.slots_per_xcd) {
-        while (f + 1 < a.nf && gm >= a.f[f + 1].mi_begin) ++f;
-        const FrameDesc& F = a.f[f];
-        const int stage = F.stage;
-        if (stage == STAGE_IDLE) continue;  // uniform across the workgroup
-        const int v = gm - F.mi_begin;
-        __syncthreads();
-        const int c = stage_mi<TT>(L, P, v, F.tiles, F.centers, F.nbr, F.cnt);
-        if (stage == STAGE_PHASE)
-            for (int i = t; i < WW; i += NT) L.costp[i] = F.cost_in[(size_t)v * WW + i];
-        __syncthreads();
-        if (stage == STAGE_INIT) init_mi<COUNT, TT>(L, P, F, v, c, scr, wc);
-        else if (stage == STAGE_PHASE) phase_mi<COUNT, TT>(L, P, F, v, c, scr, wc);
-        else final_mi<COUNT, TT>(L, P, F, v, c, wc);
-    }
-    flush_counts<COUNT>(a.counters, wc);
-}
-
 // ------------------------------------------------------------------------------------------
 // Wave-per-pixel solver (default).  One wavefront owns one pixel at a time and evaluates a
 // task list of (hypothesis, view) pairs — every view whose anchor stays inside the crop, for
@@ -1065,8 +440,8 @@ k_stage(StageArgs a) {
 // valid window samples (reference intensity, bilateral weight) are compacted once into LDS,
 // so lanes never idle on clipped window positions or on views whose anchor left the crop, and
 // each task's six weighted moments are reduced across its 8 lanes with DPP.  Results follow
-// the same definitions as ncc_group (ComputeBilateralNCC_Frame, AdaptMIPMFrame.cu:73-179)
-// with the moments summed in a different (fixed) order.
+// the definitions of ComputeBilateralNCC_Frame (AdaptMIPMFrame.cu:73-179), with the moments
+// summed in a fixed order.
 // ------------------------------------------------------------------------------------------
 namespace wv {
 
@@ -1108,7 +483,7 @@ __device__ __forceinline__ float red8(float v) {
     return v;
 }
 
-// Window grid of ncc_group: offsets -r, -r+inc, ..., <= r on each axis.
+// Window grid: offsets -r, -r+inc, ..., <= r on each axis (the reference's i/j loops).
 __device__ __forceinline__ int grid_side(const DevParams& P) { return 2 * P.r / P.inc + 1; }
 
 struct WaveLds {
@@ -1164,8 +539,6 @@ __device__ __forceinline__ Lds carve_shared(unsigned char* base, const DevParams
     L.nid = (int*)q;
     L.next = (unsigned*)(base + shared_core_bytes(P) - 16);
     L.cur = (int*)(base + shared_core_bytes(P) - 12);
-    L.wts = nullptr;
-    L.slist = nullptr;
     L.btiles = P.blur ? (float*)(base + shared_core_bytes(P)) : nullptr;
     return L;
 }
@@ -1188,7 +561,7 @@ __device__ __forceinline__ WaveLds carve_wave(unsigned char* base, const DevPara
 }
 
 // Compact the pixel's valid window samples (reference point inside the crop) into S.samp, in
-// the window order of ncc_group (i outer, j inner), zero-padded to a multiple of 8.
+// the reference's window order (i outer, j inner), zero-padded to a multiple of 8.
 template <typename TT>
 __device__ int build_samples(const Lds& L, const DevParams& P, const WaveLds& S, int px, int py,
                              float rc, float rbc) {
@@ -1232,7 +605,7 @@ __device__ int build_samples(const Lds& L, const DevParams& P, const WaveLds& S,
 }
 
 // Append the tasks of hypothesis slot h (plane hv) over the views in vmask (k < c) whose anchor
-// stays inside the crop (the act test of ncc_group); clears the hypothesis' cost row to 2.
+// stays inside the crop (AdaptMIPMFrame.cu:104-108); clears the hypothesis' cost row to 2.
 __device__ __forceinline__ int add_tasks(const Lds& L, const DevParams& P, const WaveLds& S,
                                          int h, float4 hv, uint32_t vmask, int c, int px, int py,
                                          int ntask) {
@@ -2247,49 +1620,6 @@ __global__ void k_segments(const int* __restrict__ hyp, int n, int V, int* __res
     seg[v] = lo;
 }
 
-template <bool COUNT, typename TT>
-__global__ void __launch_bounds__(256) k_eval(EvalArgs a) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const DevParams& P = a.P;
-    const int NT = blockDim.x, t = threadIdx.x, W = P.W, WW = P.WW, K = P.K;
-    Lds L = carve(lds_raw, P, NT);
-    fill_spatial(L, P);
-    int first, end, stride;
-    mi_range(P, first, end, stride);
-    WorkCount wc{0u, 0u};
-    for (int mi = first; mi < end; mi += stride) {
-        const int h0 = a.seg[mi], h1 = a.seg[mi + 1];
-        if (h0 >= h1) continue;  // uniform across the workgroup
-        __syncthreads();
-        const int c = stage_mi<TT>(L, P, mi, a.tiles, a.centers, a.nbr, a.cnt);
-        __syncthreads();
-        for (int hi = h0 + t; hi < h1; hi += NT) {
-            const int* rec = a.hyp + (size_t)hi * 6;
-            const int pix = rec[1];
-            if (pix < 0 || pix >= WW || rec[0] != mi) {  // malformed record: no score
-                for (int k = 0; k < K; ++k) a.out[(size_t)hi * K + k] = 2.0f;
-                continue;
-            }
-            const float4 h = make_float4(__int_as_float(rec[2]), __int_as_float(rec[3]),
-                                         __int_as_float(rec[4]), __int_as_float(rec[5]));
-            const int x = pix % W, y = pix / W;
-            const float rc = ldt(tile_base<TT>(L) + y * P.W1 + x);
-            for (int k0 = 0; k0 < K; k0 += kGroup) {
-                int vk[kGroup];
-                float cg[kGroup];
-#pragma unroll
-                for (int g = 0; g < kGroup; ++g)
-                    vk[g] = (k0 + g < c && L.nid[k0 + g] >= 0) ? k0 + g : -1;
-                ncc_group<COUNT, TT>(L, P, x, y, rc, h, vk, cg, wc);
-#pragma unroll
-                for (int g = 0; g < kGroup; ++g)
-                    if (k0 + g < K) a.out[(size_t)hi * K + k0 + g] = cg[g];
-            }
-        }
-    }
-    flush_counts<COUNT>(a.counters, wc);
-}
-
 // Wave-per-hypothesis form of k_eval (the default): one wavefront scores one record over all
 // of its MI's views with the task engine of wv::run_tasks.
 template <bool COUNT, typename TT, bool BLUR, int W1C>
@@ -2719,11 +2049,6 @@ static int device_cus() {
     return cus;
 }
 
-static int phase_threads(const DevParams& P) {
-    const int nact = (P.WW + 1) / 2;
-    return std::min(256, 64 * ((nact + 63) / 64));
-}
-
 // Persistent grid over `mis` work items: a multiple of 8 workgroups (one share per XCD).
 static int persistent_grid_n(int mis, int per_cu) {
     int g = device_cus() * per_cu;
@@ -2739,20 +2064,19 @@ static thread_local unsigned long long* g_counters = nullptr;
 static const int kPhasePerCu = 8;    // k_stage workgroups per CU (persistent slots), default
 static const int kMaxSlotsPerCu = 8; // upper bound of the autotuner's search
 
-static bool wave_solver();
-// Cost records of V MIs (5 + K floats per pixel; the lane engine keeps none).
+// Cost records of V MIs (5 + K floats per pixel).
 static size_t rec_bytes(const DevParams& P, int V) {
-    return wave_solver() ? (size_t)V * P.WW * (5 + P.K) * 4 : 0;
+    return (size_t)V * P.WW * (5 + P.K) * 4;
 }
-// The wave engine keeps its working state in LDS; its scratch is a 256-byte header followed by
-// the cost records of the P.V MIs.  The lane engine keeps per-lane cost vectors.
+// The solver keeps its working state in LDS; its scratch is a 256-byte header (the per-XCD MI
+// tickets) followed by the cost records of the P.V MIs.
 static size_t stage_scratch_bytes(const DevParams& P, int grid) {
-    if (wave_solver()) return 256 + rec_bytes(P, P.V);
-    return (size_t)grid * 9 * P.K * phase_threads(P) * 4;
+    (void)grid;
+    return 256 + rec_bytes(P, P.V);
 }
 // Records of the MIs from `mi_offset` on in a scratch laid out as above (null: none).
 static float* rec_region(const DevParams& P, void* scratch, int mi_offset) {
-    if (!wave_solver() || !scratch) return nullptr;
+    if (!scratch) return nullptr;
     return (float*)((unsigned char*)scratch + 256) + (size_t)mi_offset * P.WW * (5 + P.K);
 }
 static size_t phase_scratch_bytes(const DevParams& P, int per_cu) {
@@ -2768,17 +2092,6 @@ static FrameDesc frame_desc(const float* tiles, const float* centers, const int*
     return f;
 }
 
-// Solver engine: wave-per-pixel task engine (default) or the lane-per-pixel kernels;
-// LFPM_SOLVER=lane selects the latter (kept for A/B measurements).  Both give results within
-// the parity tolerances; each is deterministic.
-static int g_solver = -1;
-static bool wave_solver() {
-    if (g_solver < 0) {
-        const char* e = std::getenv("LFPM_SOLVER");
-        g_solver = (e && std::strcmp(e, "lane") == 0) ? 0 : 1;
-    }
-    return g_solver == 1;
-}
 // Threads per workgroup of the wave engine: 4 wavefronts, or fewer when the LDS of 4 would pass
 // 64 KiB (the blur term's second set of tiles with many views and a wide window).
 static int wave_threads(const DevParams& P) {
@@ -2825,51 +2138,25 @@ static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_
     }
     a.total = total;
     if (total <= 0) return LFPM_OK;
-    if (wave_solver()) {
-        const int NT = wave_threads(a.P);
-        const size_t lds = wv::lds_total(a.P, NT / 64);
-        if (lds > 65536)
-            return set_error(LFPM_E_UNSUPPORTED, "k_stage: %zu B of LDS per workgroup (crop %d, "
-                             "%d views, radius %d) exceeds 64 KiB", lds, a.P.W, a.P.K, a.P.r);
-        const StageKernel kern = stage_w_kernel(g_counters != nullptr, a.P.tile_bytes == 2,
-                                                a.P.blur != 0, a.P.W);
-        const int fit = fit_per_cu(kern, NT, lds);
-        const int grid = persistent_grid_n(total, std::max(1, std::min(per_cu, fit)));
-        a.P.slots_per_xcd = grid / 8;
-        a.scratch = nullptr;
-        a.counters = g_counters;
-        // MI tickets in the scratch header, cleared by a kernel (not a memset node, so the
-        // launch pair can sit in a captured graph); without a scratch the fixed MI stride
-        a.tickets = nullptr;
-        if (scratch && scratch_bytes >= 256) {
-            a.tickets = (unsigned*)scratch;
-            hipLaunchKernelGGL(k_zero_tickets, dim3(1), dim3(64), 0, s, a.tickets);
-        }
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
-        return launch_check("k_stage");
-    }
-    if (a.P.blur) return set_error(LFPM_E_UNSUPPORTED, "blur term: needs the wave solver");
-    const int grid = persistent_grid_n(total, per_cu);
+    const int NT = wave_threads(a.P);
+    const size_t lds = wv::lds_total(a.P, NT / 64);
+    if (lds > 65536)
+        return set_error(LFPM_E_UNSUPPORTED, "k_stage: %zu B of LDS per workgroup (crop %d, "
+                         "%d views, radius %d) exceeds 64 KiB", lds, a.P.W, a.P.K, a.P.r);
+    const StageKernel kern = stage_w_kernel(g_counters != nullptr, a.P.tile_bytes == 2,
+                                            a.P.blur != 0, a.P.W);
+    const int fit = fit_per_cu(kern, NT, lds);
+    const int grid = persistent_grid_n(total, std::max(1, std::min(per_cu, fit)));
     a.P.slots_per_xcd = grid / 8;
-    const int NT = phase_threads(a.P);
-    if ((a.P.WW + 1) / 2 > NT) return set_error(LFPM_E_INVALID, "crop too large for k_stage");
-    bool needs_scratch = false;  // only init and phase stages keep per-lane cost vectors
-    for (int i = 0; i < a.nf; ++i)
-        needs_scratch |= a.f[i].stage == STAGE_INIT || a.f[i].stage == STAGE_PHASE;
-    const size_t need = stage_scratch_bytes(a.P, grid);
-    if (needs_scratch && (!scratch || scratch_bytes < need))
-        return set_error(LFPM_E_INVALID, "workspace too small (%zu < %zu)", scratch_bytes, need);
-    a.scratch = (float*)scratch;
     a.counters = g_counters;
-    const size_t lds = lds_bytes(a.P, NT, true);
-    const bool fp16 = a.P.tile_bytes == 2;
-    if (g_counters) {
-        if (fp16) hipLaunchKernelGGL((k_stage<true, __half>), dim3(grid), dim3(NT), lds, s, a);
-        else hipLaunchKernelGGL((k_stage<true, float>), dim3(grid), dim3(NT), lds, s, a);
-    } else {
-        if (fp16) hipLaunchKernelGGL((k_stage<false, __half>), dim3(grid), dim3(NT), lds, s, a);
-        else hipLaunchKernelGGL((k_stage<false, float>), dim3(grid), dim3(NT), lds, s, a);
+    // MI tickets in the scratch header, cleared by a kernel (not a memset node, so the
+    // launch pair can sit in a captured graph); without a scratch the fixed MI stride
+    a.tickets = nullptr;
+    if (scratch && scratch_bytes >= 256) {
+        a.tickets = (unsigned*)scratch;
+        hipLaunchKernelGGL(k_zero_tickets, dim3(1), dim3(64), 0, s, a.tickets);
     }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
     return launch_check("k_stage");
 }
 static size_t state_bytes(const DevParams& P) {
@@ -3372,8 +2659,6 @@ int lfpm_patchmatch_frame_acmm(const float* tiles_dev, const float* centers_dev,
         return set_error(LFPM_E_INVALID, "acmm: null output");
     if (p->storage != 0)
         return set_error(LFPM_E_UNSUPPORTED, "acmm: level images are not 8-bit; storage must be 0");
-    if (!wave_solver())
-        return set_error(LFPM_E_UNSUPPORTED, "acmm: needs the wave solver engine");
     const bool blur = p->blur_lambda > 0.0f;
     AcmmLevel lv[2];
     const int L = acmm_levels(p->crop, lv);
@@ -3430,65 +2715,70 @@ int lfpm_patchmatch_frame_acmm(const float* tiles_dev, const float* centers_dev,
                             cinit, sinit, s);
 }
 
-int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,
-                    const int* neighbor_ids_dev, const int* neighbor_counts_dev, int num_mis,
-                    const lfpm_pm_params* p, const void* hyp_dev, int n, float* cost_out_dev,
-                    void* stream) {
-    int rc = check_pm_params(p, num_mis);
-    if (rc) return rc;
-    if ((rc = check_solver_ptrs(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev)))
-        return rc;
-    if (n < 0 || (n > 0 && (!hyp_dev || !cost_out_dev)))
-        return set_error(LFPM_E_INVALID, "eval: bad hypothesis buffer");
-    if (n == 0) return LFPM_OK;
+// Fixed-plane evaluation with the caller's segment table `seg` (num_mis + 1 ints, written
+// here before the evaluation reads it).
+static int eval_costs_with(const float* tiles_dev, const float* centers_dev,
+                           const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                           int num_mis, const lfpm_pm_params* p, const void* hyp_dev, int n,
+                           float* cost_out_dev, int* seg, hipStream_t s) {
     DevParams P = make_dev_params(p, num_mis);
     const int grid = persistent_grid(P, kPhasePerCu);
     P.slots_per_xcd = grid / 8;
-    int* seg = nullptr;
-    hipStream_t s = (hipStream_t)stream;
-    HIP_TRY(hipMallocAsync((void**)&seg, sizeof(int) * (num_mis + 1), s));
+    const int NT = wave_threads(P);
+    const size_t lds = wv::lds_total(P, NT / 64);
+    if (lds > 65536)
+        return set_error(LFPM_E_UNSUPPORTED, "eval: %zu B of LDS per workgroup exceeds 64 KiB",
+                         lds);
     hipLaunchKernelGGL(k_segments, dim3((num_mis + 256) / 256), dim3(256), 0, s,
                        (const int*)hyp_dev, n, num_mis, seg);
     EvalArgs a;
     a.tiles = tiles_dev; a.centers = (const float2*)centers_dev; a.nbr = neighbor_ids_dev;
     a.cnt = neighbor_counts_dev; a.hyp = (const int*)hyp_dev; a.seg = seg; a.n = n;
     a.out = cost_out_dev; a.P = P; a.counters = g_counters;
-    const int NT = wave_solver() ? wave_threads(P) : 256;
     const bool fp16 = P.tile_bytes == 2;
-    if (wave_solver()) {
-        const size_t lds = wv::lds_total(P, NT / 64);
-        if (lds > 65536) {
-            (void)hipFreeAsync(seg, s);
-            return set_error(LFPM_E_UNSUPPORTED, "eval: %zu B of LDS per workgroup exceeds 64 KiB",
-                             lds);
-        }
-        using EvalKernel = void (*)(EvalArgs);
-        const bool blur = P.blur != 0;
-        const EvalKernel kern =
-            g_counters ? (fp16 ? k_eval_w<true, __half, false, 0>
-                               : (blur ? k_eval_w<true, float, true, 0>
-                                       : k_eval_w<true, float, false, 0>))
-                       : (fp16 ? k_eval_w<false, __half, false, 0>
-                               : (blur ? k_eval_w<false, float, true, 0>
-                                       : k_eval_w<false, float, false, 0>));
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
-    } else {
-        if (P.blur) {
-            (void)hipFreeAsync(seg, s);
-            return set_error(LFPM_E_UNSUPPORTED, "blur term: needs the wave solver");
-        }
-        const size_t lds = lds_bytes(P, NT, false);
-        if (g_counters) {
-            if (fp16) hipLaunchKernelGGL((k_eval<true, __half>), dim3(grid), dim3(NT), lds, s, a);
-            else hipLaunchKernelGGL((k_eval<true, float>), dim3(grid), dim3(NT), lds, s, a);
-        } else {
-            if (fp16) hipLaunchKernelGGL((k_eval<false, __half>), dim3(grid), dim3(NT), lds, s, a);
-            else hipLaunchKernelGGL((k_eval<false, float>), dim3(grid), dim3(NT), lds, s, a);
-        }
-    }
-    rc = launch_check("eval");
-    HIP_TRY(hipFreeAsync(seg, s));
-    return rc;
+    using EvalKernel = void (*)(EvalArgs);
+    const bool blur = P.blur != 0;
+    const EvalKernel kern =
+        g_counters ? (fp16 ? k_eval_w<true, __half, false, 0>
+                           : (blur ? k_eval_w<true, float, true, 0>
+                                   : k_eval_w<true, float, false, 0>))
+                   : (fp16 ? k_eval_w<false, __half, false, 0>
+                           : (blur ? k_eval_w<false, float, true, 0>
+                                   : k_eval_w<false, float, false, 0>));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
+    return launch_check("eval");
+}
+
+static int eval_costs_check(const float* tiles_dev, const float* centers_dev,
+                            const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                            int num_mis, const lfpm_pm_params* p, const void* hyp_dev, int n,
+                            float* cost_out_dev) {
+    int rc = check_pm_params(p, num_mis);
+    if (rc) return rc;
+    if ((rc = check_solver_ptrs(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev)))
+        return rc;
+    if (n < 0 || (n > 0 && (!hyp_dev || !cost_out_dev)))
+        return set_error(LFPM_E_INVALID, "eval: bad hypothesis buffer");
+    return LFPM_OK;
+}
+
+int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,
+                    const int* neighbor_ids_dev, const int* neighbor_counts_dev, int num_mis,
+                    const lfpm_pm_params* p, const void* hyp_dev, int n, float* cost_out_dev,
+                    void* stream) {
+    int rc = eval_costs_check(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev,
+                              num_mis, p, hyp_dev, n, cost_out_dev);
+    if (rc || n == 0) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    int* seg = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&seg, sizeof(int) * (num_mis + 1), s));
+    rc = eval_costs_with(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev, num_mis,
+                         p, hyp_dev, n, cost_out_dev, seg, s);
+    const hipError_t fe = hipFreeAsync(seg, s);
+    if (rc) return rc;
+    if (fe != hipSuccess)
+        return set_error(LFPM_E_HIP, "eval: hipFreeAsync: %s", hipGetErrorString(fe));
+    return LFPM_OK;
 }
 
 int lfpm_filter_frame(const float* gray_dev, const float* centers_dev,

@@ -178,9 +178,6 @@ def test_cost_records_keep_results(lfpm, torch_cuda, blur):
     """The frame solve reuses each pixel's recorded all-view costs while its plane is unchanged;
     its outputs equal, bit for bit, those of the step calls without a workspace (no records)."""
     import copy
-    import os
-    if os.environ.get("LFPM_SOLVER") == "lane":
-        pytest.skip("the lane engine needs a workspace for its step calls")
     wl = lfpm.workload_c2(raw_w=800, raw_h=640)
     pm = copy.copy(wl.pm)
     if blur:
