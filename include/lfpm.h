@@ -322,6 +322,19 @@ int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,
                     int num_mis, const lfpm_pm_params* p, const void* hyp_dev, int n,
                     float* cost_out_dev, void* stream);
 
+/* lfpm_eval_costs without any allocation: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, kernels and results as lfpm_eval_costs; the per-MI
+ * segment table lives in a library-owned device array of LFPM_NOALLOC_MAX_MIS + 1 ints (1 MiB
+ * per device, held until the process ends).  num_mis above LFPM_NOALLOC_MAX_MIS is rejected with
+ * LFPM_E_INVALID.  Calls on one device share the array: calls that overlap (other streams or
+ * threads) race and must be serialised by the caller. */
+#define LFPM_NOALLOC_MAX_MIS 262143
+int lfpm_eval_costs_noalloc(const float* tiles_dev, const float* centers_dev,
+                            const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                            int num_mis, const lfpm_pm_params* p, const void* hyp_dev, int n,
+                            float* cost_out_dev, void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Fused CheckModule pass.  Replaces MIDisparityFilterCPU::FilterGlobal/FilterTile
  * (CheckModule/MIDisparityFilter.cpp:423-693, 962-1009) with configuration

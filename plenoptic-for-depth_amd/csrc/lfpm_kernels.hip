@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/lfpm.h"
@@ -2779,6 +2780,51 @@ int lfpm_eval_costs(const float* tiles_dev, const float* centers_dev,
     if (fe != hipSuccess)
         return set_error(LFPM_E_HIP, "eval: hipFreeAsync: %s", hipGetErrorString(fe));
     return LFPM_OK;
+}
+
+// Allocation-free evaluation.  `scratch` holds the segment table (LFPM_NOALLOC_MAX_MIS + 1
+// ints, 1 MiB); each device holds its own copy until the process ends.  Host code never passes
+// the array's name to a kernel: it uses the device address taken once per device with
+// hipGetSymbolAddress.  k_segments rewrites entries 0..num_mis on every call before the
+// evaluation reads them, so nothing has to be cleared.  Calls that overlap on one device share
+// the table and race.
+static __device__ __align__(256) int scratch[LFPM_NOALLOC_MAX_MIS + 1];
+static constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static int* g_scratch_ptr[kMaxScratchDevices] = {};
+
+static int scratch_on_device(int** out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return set_error(LFPM_E_UNSUPPORTED, "eval_noalloc: device %d beyond %d", dev,
+                         kMaxScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_error(LFPM_E_HIP, "eval_noalloc: no address for the scratch table");
+        g_scratch_ptr[dev] = (int*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return LFPM_OK;
+}
+
+int lfpm_eval_costs_noalloc(const float* tiles_dev, const float* centers_dev,
+                            const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                            int num_mis, const lfpm_pm_params* p, const void* hyp_dev, int n,
+                            float* cost_out_dev, void* stream) {
+    int rc = eval_costs_check(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev,
+                              num_mis, p, hyp_dev, n, cost_out_dev);
+    if (rc) return rc;
+    if (num_mis > LFPM_NOALLOC_MAX_MIS)
+        return set_error(LFPM_E_INVALID, "eval_noalloc: %d MIs exceed the scratch table's %d",
+                         num_mis, LFPM_NOALLOC_MAX_MIS);
+    if (n == 0) return LFPM_OK;
+    int* seg = nullptr;
+    if ((rc = scratch_on_device(&seg))) return rc;
+    return eval_costs_with(tiles_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev,
+                           num_mis, p, hyp_dev, n, cost_out_dev, seg, (hipStream_t)stream);
 }
 
 int lfpm_filter_frame(const float* gray_dev, const float* centers_dev,
