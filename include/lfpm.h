@@ -183,7 +183,8 @@ typedef struct lfpm_pm_params {
     uint32_t seed;             /* Philox key word 0 */
     uint32_t frame_uid;        /* Philox key word 1: results depend on (seed, frame_uid) only */
     int storage;               /* 0 = FP32 tiles in LDS; 1 = FP16 tiles in LDS holding the 8-bit
-                                  intensity codes 0..255 exactly (tiles must be k/255, as
+                                  intensity codes 0..255 exactly, each cell with its code and the
+                                  code step to the right (tiles must be k/255, as
                                   lfpm_slice_tiles writes them; other values round to the nearest
                                   1/255), FP32 arithmetic on the 0..255 scale */
     float cos_max_slant;       /* planes with |nz| < cos_max_slant score 2 (0 = no gate, the

@@ -28,7 +28,8 @@ namespace lfpm {
 struct DevParams {
     int W, WW, K, r, inc, top_k, V;
     int W1, TS;          // padded tile row stride (W+1) and slot stride (W+1)^2 in LDS
-    int tile_bytes;      // 4 (FP32 tiles in LDS) or 2 (FP16 tiles in LDS)
+    int tile_bytes;      // bytes per tile cell in LDS (by the tile format)
+    int fmt;             // tile format: 0 FP32 values, 3 FP16 code pairs (storage 1)
     float base;          // Base (disparity unit, lens pitch in px)
     float inv_base;      // 1 / max(Base, 1e-6)
     float inv_ss;        // 1 / (2 sigma_s^2)
@@ -48,7 +49,8 @@ static DevParams make_dev_params(const lfpm_pm_params* p, int V) {
     d.WW = p->crop * p->crop;
     d.W1 = p->crop + 1;
     d.TS = (p->crop + 1) * (p->crop + 1);
-    d.tile_bytes = p->storage == 1 ? 2 : 4;
+    d.fmt = p->storage == 1 ? 3 : 0;
+    d.tile_bytes = 4;
     d.K = p->max_neighbors;
     d.r = p->radius;
     d.inc = std::max(1, p->radius_increment);
@@ -181,9 +183,9 @@ struct WorkCount {
 
 // LDS carve-up shared by the solver kernels.
 struct Lds {
-    float* tiles;     // (K+1) slots of (W+1)^2: slot 0 = reference, slot 1+k = neighbour k;
-    __half* htiles;   // the same region holding FP16 intensities (storage mode 1)
-                      // column W and row W repeat the last column/row (clamp-to-edge for free)
+    float* tiles;     // (K+1) slots of (W+1)^2 cells: slot 0 = reference, slot 1+k = neighbour
+                      // k; column W and row W repeat the last column/row (clamp-to-edge for
+                      // free); FP32 values or PairH cells
     float* costp;     // WW (phase kernel: previous cost of this MI)
     float* spat;      // (2r+1)^2 spatial log-weights -|dxy| / (2 s_s^2)
     float2* step;     // K: (c_ref - c_nei) / Base
@@ -194,26 +196,37 @@ struct Lds {
     int* cur;         // wave engine: the workgroup's current MI (-1: none left)
 };
 
-// Tile element access for the two LDS storage modes; arithmetic is FP32 in both.
-template <typename TT> __device__ __forceinline__ const TT* tile_base(const Lds& L);
-template <> __device__ __forceinline__ const float* tile_base<float>(const Lds& L) { return L.tiles; }
-template <> __device__ __forceinline__ const __half* tile_base<__half>(const Lds& L) {
-    return L.htiles;
+// Tile formats in LDS; arithmetic is FP32 in both.
+//  float  (storage 0): the FP32 tiles as given.
+//  PairH  (storage 1): the 8-bit intensity codes 0..255 (k/255 in the FP32 tiles, recovered
+//         exactly by rint(255 t)) in FP16, each cell holding its code t(x, y) and the step to the
+//         right d = t(x+1, y) - t(x, y) (0 in the last column, whose right neighbour is itself
+//         under clamp-to-edge).  Integers up to 2048 are exact in FP16, so the storage loses
+//         nothing; a row's two bilinear taps are one read and one FMA, t + fx * d, the rounding
+//         of the two-tap form t(x, y) + fx * (t(x+1, y) - t(x, y)).  The NCC then runs on the
+//         0..255 scale: it is scale invariant, and the two scale-dependent constants (colour
+//         weight 1/(2 s_c^2) and the variance gate) are rescaled by TileScale.
+struct __align__(4) PairH { __half t, d; };
+template <typename TT> struct IsPair { static constexpr bool value = false; };
+template <> struct IsPair<PairH> { static constexpr bool value = true; };
+template <typename TT> __device__ __forceinline__ const TT* tile_base(const Lds& L) {
+    return (const TT*)L.tiles;
 }
-// FP16 mode holds the 8-bit intensity codes 0..255 (k/255 in the FP32 tiles, recovered exactly by
-// rint(255 t)); every integer up to 2048 is exact in FP16, so the storage loses nothing and the
-// NCC runs on the 0..255 scale.  NCC is scale invariant; the two scale-dependent constants (colour
-// weight 1/(2 s_c^2) and the variance gate) are rescaled by TileScale.
 __device__ __forceinline__ float ldt(const float* p) { return *p; }
-__device__ __forceinline__ float ldt(const __half* p) { return __half2float(*p); }
+__device__ __forceinline__ float ldt(const PairH* p) { return __half2float(p->t); }
 __device__ __forceinline__ void stt(float* p, float v) { *p = v; }
-__device__ __forceinline__ void stt(__half* p, float v) { *p = __float2half_rn(rintf(v * 255.0f)); }
+__device__ __forceinline__ void stt_pair(PairH* p, float v, float vr) {
+    const float c = rintf(v * 255.0f), cr = rintf(vr * 255.0f);
+    *p = PairH{__float2half_rn(c), __float2half_rn(cr - c)};
+}
+__device__ __forceinline__ float pair_t(const PairH& p) { return __half2float(p.t); }
+__device__ __forceinline__ float pair_d(const PairH& p) { return __half2float(p.d); }
 template <typename TT> struct TileScale;
 template <> struct TileScale<float> {
     __device__ static float inv_sc(const DevParams& P) { return P.inv_sc; }
     __device__ static float min_var(const DevParams& P) { return P.min_var; }
 };
-template <> struct TileScale<__half> {
+template <> struct TileScale<PairH> {
     __device__ static float inv_sc(const DevParams& P) { return P.inv_sc * (1.0f / 255.0f); }
     __device__ static float min_var(const DevParams& P) { return P.min_var * 65025.0f; }
 };
@@ -259,7 +272,11 @@ __device__ __forceinline__ int stage_mi(Lds& L, const DevParams& P, int mi,
             src = nbr[(size_t)mi * K + s - 1];
             if (src < 0) src = mi;  // never read: views with nid < 0 are not scored
         }
-        stt((TT*)tile_base<TT>(L) + idx, tiles[(size_t)src * WW + yy * W + xx]);
+        if constexpr (IsPair<TT>::value)
+            stt_pair((TT*)tile_base<TT>(L) + idx, tiles[(size_t)src * WW + yy * W + xx],
+                     tiles[(size_t)src * WW + yy * W + min(xx + 1, W - 1)]);
+        else
+            stt((TT*)tile_base<TT>(L) + idx, tiles[(size_t)src * WW + yy * W + xx]);
         if (btiles) L.btiles[idx] = btiles[(size_t)src * WW + yy * W + xx];
     }
     return c;
@@ -529,7 +546,6 @@ __device__ __forceinline__ Lds carve_shared(unsigned char* base, const DevParams
     const int n = 2 * P.r + 1;
     unsigned char* q = base;
     L.tiles = (float*)q;
-    L.htiles = (__half*)q;
     q += a16((size_t)(P.K + 1) * P.TS * P.tile_bytes);
     L.costp = (float*)q; q += a16((size_t)P.WW * 4);
     L.spat = (float*)q; q += a16((size_t)n * n * 4);
@@ -734,9 +750,16 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
             m.cell = min(__umul24((unsigned)iy, (unsigned)W1) + (unsigned)ix, maxcell);
             return m;
         };
+        // taps: the 2x2 footprint as (top-left, top-right, bottom-left, bottom-right), or for
+        // pair tiles as (top t, top d, bottom t, bottom d)
         auto taps = [&](const Cell& m, float4& t4, float4& b4) {
             const TT* tp = tsrc + m.cell;
-            t4 = make_float4(ldt(tp), ldt(tp + 1), ldt(tp + W1), ldt(tp + W1 + 1));
+            if constexpr (IsPair<TT>::value) {
+                const TT p0 = tp[0], p1 = tp[W1];
+                t4 = make_float4(pair_t(p0), pair_d(p0), pair_t(p1), pair_d(p1));
+            } else {
+                t4 = make_float4(ldt(tp), ldt(tp + 1), ldt(tp + W1), ldt(tp + W1 + 1));
+            }
             if (BLUR) {
                 const float* bp = L.btiles + base + m.cell;
                 b4 = make_float4(bp[0], bp[1], bp[W1], bp[W1 + 1]);
@@ -746,8 +769,14 @@ __device__ void run_tasks(const Lds& L, const DevParams& P, const WaveLds& S, in
             const float4 rec = m.rec;
             const float fx = m.fx, fy = m.fy;
             const bool ok = m.ok;
-            const float top = t4.x + fx * (t4.y - t4.x);
-            const float bot = t4.z + fx * (t4.w - t4.z);
+            float top, bot;
+            if constexpr (IsPair<TT>::value) {
+                top = fmaf(fx, t4.y, t4.x);
+                bot = fmaf(fx, t4.w, t4.z);
+            } else {
+                top = t4.x + fx * (t4.y - t4.x);
+                bot = t4.z + fx * (t4.w - t4.z);
+            }
             const float y = (top + fy * (bot - top)) - rc;
             if (BLUR) {
                 const float btop = b4.x + fx * (b4.y - b4.x);
@@ -2115,14 +2144,15 @@ static int fit_per_cu(KERN kern, int threads, size_t lds) {
 using StageKernel = void (*)(StageArgs);
 // kSpecialCrop: the crop of the kernel specialised for one tile pitch (FP32, no blur term)
 static const int kSpecialCrop = 16;
-static StageKernel stage_w_kernel(bool count, bool fp16, bool blur, int crop) {
-    if (count)
-        return fp16 ? wv::k_stage_w<true, __half, false, 0>
-                    : (blur ? wv::k_stage_w<true, float, true, 0>
-                            : wv::k_stage_w<true, float, false, 0>);
-    if (!fp16 && !blur && crop == kSpecialCrop) return wv::k_stage_w<false, float, false, kSpecialCrop + 1>;
-    return fp16 ? wv::k_stage_w<false, __half, false, 0>
-                : (blur ? wv::k_stage_w<false, float, true, 0> : wv::k_stage_w<false, float, false, 0>);
+template <typename TT>
+static StageKernel stage_w_fmt(bool count, bool blur, int crop) {
+    if (count) return blur ? wv::k_stage_w<true, TT, true, 0> : wv::k_stage_w<true, TT, false, 0>;
+    if (!blur && crop == kSpecialCrop) return wv::k_stage_w<false, TT, false, kSpecialCrop + 1>;
+    return blur ? wv::k_stage_w<false, TT, true, 0> : wv::k_stage_w<false, TT, false, 0>;
+}
+static StageKernel stage_w_kernel(bool count, int fmt, bool blur, int crop) {
+    if (fmt == 3) return stage_w_fmt<PairH>(count, false, crop);
+    return stage_w_fmt<float>(count, blur, crop);
 }
 
 __global__ void k_zero_tickets(unsigned* __restrict__ t) {
@@ -2144,7 +2174,7 @@ static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_
     if (lds > 65536)
         return set_error(LFPM_E_UNSUPPORTED, "k_stage: %zu B of LDS per workgroup (crop %d, "
                          "%d views, radius %d) exceeds 64 KiB", lds, a.P.W, a.P.K, a.P.r);
-    const StageKernel kern = stage_w_kernel(g_counters != nullptr, a.P.tile_bytes == 2,
+    const StageKernel kern = stage_w_kernel(g_counters != nullptr, a.P.fmt,
                                             a.P.blur != 0, a.P.W);
     const int fit = fit_per_cu(kern, NT, lds);
     const int grid = persistent_grid_n(total, std::max(1, std::min(per_cu, fit)));
@@ -2736,16 +2766,14 @@ static int eval_costs_with(const float* tiles_dev, const float* centers_dev,
     a.tiles = tiles_dev; a.centers = (const float2*)centers_dev; a.nbr = neighbor_ids_dev;
     a.cnt = neighbor_counts_dev; a.hyp = (const int*)hyp_dev; a.seg = seg; a.n = n;
     a.out = cost_out_dev; a.P = P; a.counters = g_counters;
-    const bool fp16 = P.tile_bytes == 2;
     using EvalKernel = void (*)(EvalArgs);
     const bool blur = P.blur != 0;
-    const EvalKernel kern =
-        g_counters ? (fp16 ? k_eval_w<true, __half, false, 0>
-                           : (blur ? k_eval_w<true, float, true, 0>
-                                   : k_eval_w<true, float, false, 0>))
-                   : (fp16 ? k_eval_w<false, __half, false, 0>
-                           : (blur ? k_eval_w<false, float, true, 0>
-                                   : k_eval_w<false, float, false, 0>));
+    EvalKernel kern;
+    if (P.fmt == 3)
+        kern = g_counters ? k_eval_w<true, PairH, false, 0> : k_eval_w<false, PairH, false, 0>;
+    else
+        kern = g_counters ? (blur ? k_eval_w<true, float, true, 0> : k_eval_w<true, float, false, 0>)
+                          : (blur ? k_eval_w<false, float, true, 0> : k_eval_w<false, float, false, 0>);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
     return launch_check("eval");
 }
