@@ -646,13 +646,15 @@ __device__ __forceinline__ int add_tasks(const Lds& L, const DevParams& P, const
 // add_tasks for the rows h0 .. h0 + nh - 1 in `rows` (bit r = row h0 + r; planes already in
 // S.hyp), built side by side: lane (g, k) of a round takes row h0 + r0 + g and view k.  Tasks
 // are appended in (row, view) order, as consecutive add_tasks calls append them; every cost row
-// of the range is cleared to 2.
+// of the range is cleared to 2 (unless `clear` is false: a second round over other views).
 __device__ __forceinline__ int add_tasks_rows(const Lds& L, const DevParams& P,
                                               const WaveLds& S, int h0, int nh, uint32_t rows,
-                                              uint32_t vmask, int c, int px, int py, int ntask) {
+                                              uint32_t vmask, int c, int px, int py, int ntask,
+                                              bool clear = true) {
     const int lane = lane_id();
     const float Wf = (float)P.W;
-    for (int i = lane; i < nh * P.K; i += 64) S.cost[h0 * P.K + i] = 2.0f;
+    if (clear)
+        for (int i = lane; i < nh * P.K; i += 64) S.cost[h0 * P.K + i] = 2.0f;
     // slopes and admissibility of the rows, one row per lane (nh <= kHMax)
     bool rok = false;
     if (lane < nh) {
@@ -904,6 +906,20 @@ __device__ __forceinline__ float wsum(const DevParams& P, const WaveLds& S, cons
     return sl.wn <= 1e-6f ? 2.0f : div_wn(tot, sl.wn, sl.winv);
 }
 
+// wsum over the first `slots` selection slots only (the other views counted as cost 0): the
+// same terms added in the same order, so its value never exceeds the row's wsum (each FMA of a
+// non-negative term rounds to at least the running sum, and the quotient is correctly rounded).
+__device__ __forceinline__ float wsum_prefix(const DevParams& P, const WaveLds& S, const Sel& sl,
+                                             int h, int slots) {
+    float tot = 0.0f;
+    for (int j = 0; j < slots; ++j) {
+        const int k = lane_int(sl.k, j);
+        const float w = lane_value(sl.w, j);
+        tot += w * S.cost[h * P.K + k];
+    }
+    return sl.wn <= 1e-6f ? 2.0f : div_wn(tot, sl.wn, sl.winv);
+}
+
 // wsum of row `lane` for every lane below nrows (lanes at or above it score row 0): the rows
 // of a batch summed side by side, each in selection order as wsum sums it.
 __device__ __forceinline__ float wsum_lanes(const DevParams& P, const WaveLds& S, const Sel& sl,
@@ -912,6 +928,45 @@ __device__ __forceinline__ float wsum_lanes(const DevParams& P, const WaveLds& S
     return wsum(P, S, sl, lane < nrows ? lane : 0);
 }
 
+
+// Rows 0 .. nrows-1 in `rows` (planes in S.hyp) scored over the selected views tsel, for a
+// comparison against a best cost that is at most `bound`.  With 4 or more selected views the
+// first round scores the first half of the selection; a row whose weighted cost over that half
+// (wsum_prefix) already reaches `bound` is settled: its full weighted cost is at least as large,
+// so it cannot win, and its other views are not scored (their entries stay 2, which keeps its
+// wsum at or above `bound`).  The other rows get the remaining views in a second round, so
+// every row that can win is scored on all of them, exactly as in one round.
+template <bool COUNT, typename TT, bool BLUR, int W1C>
+__device__ __forceinline__ void run_rows_bounded(const Lds& L, const DevParams& P,
+                                                 const WaveLds& S, const Sel& sl, int nrows,
+                                                 uint32_t rows, uint32_t tsel, int c, int x,
+                                                 int y, int ns, float rc, float rbc,
+                                                 float bound, WorkCount& wc) {
+    const int lane = lane_id();
+    if (sl.n < 4) {
+        const int nt = add_tasks_rows(L, P, S, 0, nrows, rows, tsel, c, x, y, 0);
+        wave_sync();
+        run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nt, ns, x, y, rc, rbc, wc);
+        return;
+    }
+    const int half = (sl.n + 1) >> 1;
+    // the views of selection slots 0 .. half-1 (slots hold the selected views in increasing
+    // order)
+    uint32_t m1 = 0u, rem = tsel;
+    for (int j = 0; j < half; ++j) {
+        m1 |= rem & (0u - rem);
+        rem &= rem - 1u;
+    }
+    int nt = add_tasks_rows(L, P, S, 0, nrows, rows, m1, c, x, y, 0);
+    wave_sync();
+    run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nt, ns, x, y, rc, rbc, wc);
+    const float part = wsum_prefix(P, S, sl, lane < nrows ? lane : 0, half);
+    const uint32_t open =
+        (uint32_t)__ballot(lane < nrows && ((rows >> lane) & 1u) && part < bound);
+    nt = add_tasks_rows(L, P, S, 0, nrows, open, tsel & ~m1, c, x, y, 0, false);
+    wave_sync();
+    run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nt, ns, x, y, rc, rbc, wc);
+}
 
 // Rounded point of the cross-view proposal through view k at disparity w
 // (BuildCrossViewPlaneProposal :362-376): its pixel index in the neighbour crop, -1 when the
@@ -1386,9 +1441,9 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
         if (lane < jn && qspec >= 0) S.hyp[lane] = pspec;
         wave_sync();
         const uint32_t live = (uint32_t)__ballot(lane < jn && qspec >= 0);
-        const int npt = add_tasks_rows(L, P, S, 0, jn, live, tsel, c, x, y, 0);
-        wave_sync();
-        run_batch<COUNT, TT, BLUR, W1C>(L, P, S, npt, ns, x, y, rc, rbc, wc);
+        // bcost only falls during the replay, so the batch's starting value bounds it
+        run_rows_bounded<COUNT, TT, BLUR, W1C>(L, P, S, sl, jn, live, tsel, c, x, y, ns, rc,
+                                               rbc, bcost, wc);
         const float wrow = wsum_lanes(P, S, sl, jn);
         // lane j's rounded point under the replayed best (recomputed after each acceptance)
         int qcur = qspec;
@@ -1448,9 +1503,8 @@ __device__ void phase_pixel(const Lds& L, const DevParams& P, const FrameDesc& F
             S.hyp[i] = hc;
         }
         wave_sync();
-        const int nrt = add_tasks_rows(L, P, S, 0, 5, 0x1Fu, tsel, c, x, y, 0);
-        wave_sync();
-        run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nrt, ns, x, y, rc, rbc, wc);
+        run_rows_bounded<COUNT, TT, BLUR, W1C>(L, P, S, sl, 5, 0x1Fu, tsel, c, x, y, ns, rc, rbc,
+                                               bcost, wc);
         const float wrow = wsum_lanes(P, S, sl, 5);
         for (int i = 0; i < 5; ++i) {
             const float cc = lane_value(wrow, i);
