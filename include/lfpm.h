@@ -229,6 +229,13 @@ int lfpm_slice_tiles(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pi
 int lfpm_slice_gray(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pitch,
                     const int* origins_dev, int num_mis, int crop, float* gray_dev, void* stream);
 
+/* Colour crops: bytes B, G, R of each MI pixel of a colour frame (bgr_dev: raw_h rows of
+ * bgr_pitch bytes, 3 per pixel) -> bgr_tiles_dev (V*crop*crop*3 bytes), the problem.m_Image_rgb
+ * crops of DepthSolver::Slice_RawMLAImage (DepthSolver.cpp:1951-1997) that colour the VD map. */
+int lfpm_slice_bgr_tiles(const uint8_t* bgr_dev, int raw_w, int raw_h, size_t bgr_pitch,
+                         const int* origins_dev, int num_mis, int crop, uint8_t* bgr_tiles_dev,
+                         void* stream);
+
 /* Blur tiles for the blur term (lfpm_pm_params.blur_lambda): the Laplacian blur score of the
  * gray frame (ImageQualityEstimate::QuantizeBlurByLaplacian, Util/ImageQualityEstimate.cpp:
  * 211-230 — cv::Laplacian with ksize 1, i.e. the 4-neighbour stencil, FP32, border reflected
@@ -420,6 +427,17 @@ int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, con
                            float* vd_dev, uint8_t* color_dev, float* range_dev,
                            void* workspace_dev, size_t workspace_bytes, void* stream);
 
+/* lfpm_virtual_depth_map coloured from the colour crops (lfpm_slice_bgr_tiles): the reference's
+ * real-colour image (DepthSolver.cpp:7895-7993): per raw pixel each of B, G, R sorted on its own
+ * with the colour index rule (color0/1/2, :7919-7927), the fill copying all three channels of
+ * the pixel it finds, ten 5x5 median passes on each channel.  color_dev: raw_h*raw_w*3 bytes
+ * B, G, R (required). */
+int lfpm_virtual_depth_map_bgr(const float* plane_dev, const float* centers_dev,
+                               const uint8_t* bgr_tiles_dev, int num_mis, int crop,
+                               float mla_base, int raw_w, int raw_h, float* vd_dev,
+                               uint8_t* color_dev, float* range_dev, void* workspace_dev,
+                               size_t workspace_bytes, void* stream);
+
 /* VD point cloud of DepthSolver::Virtual_depth_map_TileKeySequence (DepthSolver.cpp:8019-8053),
  * built on the device as the binary payload of StoreColorPlyFileBinaryPointCloud
  * (MVStereo/ACMP.cpp:451-500).  One 27-byte record {x, y, z, nx, ny, nz: float32 LE; red, green,
@@ -440,6 +458,15 @@ int lfpm_vd_point_cloud(const float* vd_dev, const uint8_t* color_dev, int raw_w
                         float B, float bl0, uint8_t* records_dev, size_t records_bytes,
                         unsigned* count_dev, void* workspace_dev, size_t workspace_bytes,
                         void* stream);
+
+/* lfpm_vd_point_cloud with the B, G, R image of lfpm_virtual_depth_map_bgr (raw_h*raw_w*3
+ * bytes): red, green, blue = channels 2, 1, 0, the writer's r/g/b = color.z/.y/.x
+ * (MVStereo/ACMP.cpp:479-481 with point3D.color of DepthSolver.cpp:8044-8046). */
+int lfpm_vd_point_cloud_bgr(const float* vd_dev, const uint8_t* color_bgr_dev, int raw_w,
+                            int raw_h, float origin_x, float origin_y, float pixel_size,
+                            float main_lens_f, float B, float bl0, uint8_t* records_dev,
+                            size_t records_bytes, unsigned* count_dev, void* workspace_dev,
+                            size_t workspace_bytes, void* stream);
 
 /* Host file writers.  lfpm_write_ply_points writes the header of StoreColorPlyFileBinaryPointCloud
  * (MVStereo/ACMP.cpp:458-470) and count records of lfpm_vd_point_cloud (copied to the host).

@@ -49,8 +49,9 @@
 //   disparity_mosaic()      : DepthSolver::WriteDisMap_TileKey_new_AccuSequence,
 //                             DepthSolver.cpp:6716-6897 (float map instead of the renderings).
 //   vd_map()                : DepthSolver::Virtual_depth_map_TileKeySequence,
-//                             DepthSolver.cpp:7832-8000 (median splat, colour fill, 10 x
-//                             medianBlur 5x5 with replicated borders per OpenCV's documentation).
+//                             DepthSolver.cpp:7832-8000 (median splat, per-channel colour
+//                             medians :7919-7927, colour fill, 10 x medianBlur 5x5 with
+//                             replicated borders per OpenCV's documentation, each channel).
 //
 // Semantics the reference leaves undefined and that this restatement fixes (the product does
 // the same): planes whose normal is more than acos(cos_max_slant) from the optical axis score the
@@ -1756,12 +1757,17 @@ static void disparity_mosaic(const float* plane, const float* cost, const float*
     if (range) { range[0] = dmin; range[1] = dmax; }
 }
 
-static void vd_map(const float* plane, const float* centers, const float* gray, int V, int crop,
-                   float mla_base, int raw_w, int raw_h, float* vd_out, uint8_t* color_out,
-                   float* range) {
+// Colour samples come from the gray crops (gray, one channel) or the colour crops (bgr:
+// V x crop^2 x 3 bytes B, G, R, three channels, each sorted and filtered on its own as the
+// reference's color0/1/2 lists and the 3-channel medianBlur are); color_out has N x channels
+// bytes, interleaved.
+static void vd_map(const float* plane, const float* centers, const float* gray, const uint8_t* bgr,
+                   int V, int crop, float mla_base, int raw_w, int raw_h, float* vd_out,
+                   uint8_t* color_out, float* range) {
     const size_t N = (size_t)raw_w * raw_h;
+    const int C = bgr ? 3 : 1;
     std::vector<std::vector<float>> vd(N);
-    std::vector<std::vector<int>> col(N);
+    std::vector<std::vector<int>> col(N * C);
     const int WW = crop * crop;
     for (int v = 0; v < V; ++v)
         for (int y = 0; y < crop; ++y)
@@ -1776,33 +1782,42 @@ static void vd_map(const float* plane, const float* centers, const float* gray, 
                 const float Yv = ym * f_v + centers[2 * v + 1];
                 const int vy = (int)std::round(Yv), vx = (int)std::round(Xv);
                 if (vx > 0 && vx < raw_w && vy > 0 && vy < raw_h) {
-                    vd[(size_t)vy * raw_w + vx].push_back(f_v);
-                    col[(size_t)vy * raw_w + vx].push_back(gray ? (int)(uint8_t)gray[i] : 0);
+                    const size_t q = (size_t)vy * raw_w + vx;
+                    vd[q].push_back(f_v);
+                    for (int c = 0; c < C; ++c)
+                        col[q * C + c].push_back(bgr ? (int)bgr[3 * i + c]
+                                                     : (gray ? (int)(uint8_t)gray[i] : 0));
                 }
             }
     std::vector<float> vdm(N, 0.0f);
-    std::vector<uint8_t> cm(N, 0);
+    std::vector<uint8_t> cm(N * C, 0);
     float vmin = INFINITY, vmax = -INFINITY;
     for (size_t p = 0; p < N; ++p) {
         const int n = (int)vd[p].size();
         if (n == 0) continue;
         std::sort(vd[p].begin(), vd[p].end());
-        std::sort(col[p].begin(), col[p].end());
         const int i1 = std::min(n / 2 + 1, n - 1), i2 = std::min((n + 1) / 2, n - 1);
         vdm[p] = (vd[p][i1] + vd[p][i2]) / 2;
-        cm[p] = (uint8_t)((col[p][n / 2] + col[p][(n + 1) / 2 - 1]) / 2);
+        for (int c = 0; c < C; ++c) {
+            std::vector<int>& cl = col[p * C + c];
+            std::sort(cl.begin(), cl.end());
+            cm[p * C + c] = (uint8_t)((cl[n / 2] + cl[(n + 1) / 2 - 1]) / 2);
+        }
         vmin = std::min(vmin, vdm[p]);
         vmax = std::max(vmax, vdm[p]);
     }
     for (size_t p = 0; p < N; ++p) vd_out[p] = vdm[p];
     if (range) { range[0] = vmin; range[1] = vmax; }
     if (!color_out) return;
-    std::vector<uint8_t> img(N, 0);
+    std::vector<uint8_t> img(N * C, 0);
     const long long rows = raw_h, cols = raw_w;
     for (long long y = 0; y < rows; ++y)
         for (long long x = 0; x < cols; ++x) {
             const long long p = y * cols + x;
-            if (!vd[p].empty()) { img[p] = cm[p]; continue; }
+            if (!vd[p].empty()) {
+                for (int c = 0; c < C; ++c) img[p * C + c] = cm[p * C + c];
+                continue;
+            }
             long long t = 4, k = p;
             while (true) {
                 switch (t % 4) {
@@ -1813,23 +1828,27 @@ static void vd_map(const float* plane, const float* centers, const float* gray, 
                 }
                 ++t;
                 if (k >= rows * cols || k < 0) break;
-                if (!vd[k].empty()) { img[p] = cm[k]; break; }
+                if (!vd[k].empty()) {
+                    for (int c = 0; c < C; ++c) img[p * C + c] = cm[k * C + c];
+                    break;
+                }
             }
         }
-    std::vector<uint8_t> tmp(N);
+    std::vector<uint8_t> tmp(N * C);
     for (int it = 0; it < 10; ++it) {
         for (long long y = 0; y < rows; ++y)
-            for (long long x = 0; x < cols; ++x) {
-                int w[25], n = 0;
-                for (int j = -2; j <= 2; ++j)
-                    for (int i = -2; i <= 2; ++i) {
-                        const long long yy = std::min(std::max(y + j, 0LL), rows - 1);
-                        const long long xx = std::min(std::max(x + i, 0LL), cols - 1);
-                        w[n++] = img[yy * cols + xx];
-                    }
-                std::nth_element(w, w + 12, w + 25);
-                tmp[y * cols + x] = (uint8_t)w[12];
-            }
+            for (long long x = 0; x < cols; ++x)
+                for (int c = 0; c < C; ++c) {
+                    int w[25], n = 0;
+                    for (int j = -2; j <= 2; ++j)
+                        for (int i = -2; i <= 2; ++i) {
+                            const long long yy = std::min(std::max(y + j, 0LL), rows - 1);
+                            const long long xx = std::min(std::max(x + i, 0LL), cols - 1);
+                            w[n++] = img[(yy * cols + xx) * C + c];
+                        }
+                    std::nth_element(w, w + 12, w + 25);
+                    tmp[(y * cols + x) * C + c] = (uint8_t)w[12];
+                }
         img.swap(tmp);
     }
     std::copy(img.begin(), img.end(), color_out);
@@ -1974,7 +1993,14 @@ void orc_disparity_mosaic(const float* plane, const float* cost, const float* ce
 
 void orc_vd_map(const float* plane, const float* centers, const float* gray, int V, int crop,
                 float mla_base, int raw_w, int raw_h, float* vd, uint8_t* color, float* range) {
-    orc::vd_map(plane, centers, gray, V, crop, mla_base, raw_w, raw_h, vd, color, range);
+    orc::vd_map(plane, centers, gray, nullptr, V, crop, mla_base, raw_w, raw_h, vd, color, range);
+}
+
+// VD map with the colour crops (V x crop^2 x 3 bytes B, G, R): color is raw_h x raw_w x 3.
+void orc_vd_map_bgr(const float* plane, const float* centers, const uint8_t* bgr, int V, int crop,
+                    float mla_base, int raw_w, int raw_h, float* vd, uint8_t* color,
+                    float* range) {
+    orc::vd_map(plane, centers, nullptr, bgr, V, crop, mla_base, raw_w, raw_h, vd, color, range);
 }
 
 int orc_fit_v2r(const float* aL, const float* v, int n, double* c) {

@@ -145,7 +145,8 @@ def vd_point_cloud(vd, color, origin, pixel_size, f, B, bl0):
     (MVStereo/ACMP.cpp:451-500): bytes (n, 27) = x, y, z, nx, ny, nz float32 LE + red, green,
     blue, for the pixels with vd > 0 in row-major order.  The image-plane point is FP64, the
     virtual-depth line vdm * B + bl0 FP32 with two roundings, the thin-lens map FP64; non-finite
-    points are written as 0 as the writer does; normals 0; colour = the gray value."""
+    points are written as 0 as the writer does; normals 0; colour = the gray value (color
+    (h, w)) or red, green, blue of a B, G, R colour image (color (h, w, 3))."""
     vd = np.asarray(vd, np.float32)
     ys, xs = np.nonzero(vd > 0)
     vdm = vd[ys, xs]
@@ -167,10 +168,19 @@ def vd_point_cloud(vd, color, origin, pixel_size, f, B, bl0):
     rec = np.zeros((n, 27), np.uint8)
     xyz = np.stack([x, y, z, np.zeros(n, f32), np.zeros(n, f32), np.zeros(n, f32)], axis=1)
     rec[:, :24] = np.ascontiguousarray(xyz.astype("<f4")).view(np.uint8).reshape(n, 24)
-    g = np.zeros(n, np.uint8) if color is None else np.asarray(color, np.uint8)[ys, xs]
-    rec[:, 24] = g
-    rec[:, 25] = g
-    rec[:, 26] = g
+    if color is None:
+        rec[:, 24:27] = 0
+    elif np.asarray(color).ndim == 3:
+        # colour image B, G, R: the writer emits red = channel 2, green = 1, blue = 0
+        c = np.asarray(color, np.uint8)[ys, xs]
+        rec[:, 24] = c[:, 2]
+        rec[:, 25] = c[:, 1]
+        rec[:, 26] = c[:, 0]
+    else:
+        g = np.asarray(color, np.uint8)[ys, xs]
+        rec[:, 24] = g
+        rec[:, 25] = g
+        rec[:, 26] = g
     return rec
 
 
@@ -349,6 +359,29 @@ def vd_map(plane, centers, gray, crop, mla_base, raw_w, raw_h, color=True):
                      C.c_float(mla_base), raw_w, raw_h, _p(vd),
                      _p(col) if col is not None else None, _p(rng))
     return vd, col, rng
+
+
+def vd_map_bgr(plane, centers, bgr_tiles, crop, mla_base, raw_w, raw_h):
+    """VD map with the colour crops bgr_tiles (V, crop, crop, 3) B, G, R: -> (vd (h, w) f32,
+    colour (h, w, 3) u8 B, G, R, range (2,) f32)."""
+    plane = np.ascontiguousarray(plane, np.float32)
+    centers = np.ascontiguousarray(centers, np.float32)
+    V = centers.shape[0]
+    b = np.ascontiguousarray(bgr_tiles, np.uint8)
+    assert b.shape == (V, crop, crop, 3)
+    vd = np.zeros((raw_h, raw_w), np.float32)
+    col = np.zeros((raw_h, raw_w, 3), np.uint8)
+    rng = np.zeros(2, np.float32)
+    lib().orc_vd_map_bgr(_p(plane), _p(centers), _p(b), V, crop, C.c_float(mla_base), raw_w,
+                         raw_h, _p(vd), _p(col), _p(rng))
+    return vd, col, rng
+
+
+def slice_bgr(bgr, origins, crop):
+    """Colour crops (V, crop, crop, 3) of a (H, W, 3) B, G, R frame at the MI origins (the
+    problem.m_Image_rgb crops of DepthSolver::Slice_RawMLAImage, DepthSolver.cpp:1951-1997)."""
+    bgr = np.asarray(bgr, np.uint8)
+    return np.stack([bgr[y:y + crop, x:x + crop] for x, y in np.asarray(origins)])
 
 
 def solve_acmm(tiles_u8, centers, nbr, cnt, pm, threads=None, blur_u8=None, blur=None):

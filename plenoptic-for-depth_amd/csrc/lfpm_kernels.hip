@@ -326,6 +326,25 @@ __global__ void k_slice(const uint8_t* __restrict__ raw, size_t pitch, const int
     }
 }
 
+// Colour crops: pixel (x, y) of MI v = bytes B, G, R of the colour frame at its origin + (x, y)
+// (problem.m_Image_rgb of DepthSolver::Slice_RawMLAImage, DepthSolver.cpp:1951-1997).
+__global__ void k_slice_bgr(const uint8_t* __restrict__ bgr, size_t pitch,
+                            const int2* __restrict__ org, int V, int W, uint8_t* __restrict__ out) {
+    const size_t WW = (size_t)W * W;
+    const size_t total = (size_t)V * WW;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int v = (int)(i / WW);
+        const int o = (int)(i - (size_t)v * WW);
+        const int y = o / W, x = o - y * W;
+        const int2 g = org[v];
+        const uint8_t* src = bgr + (size_t)(g.y + y) * pitch + (size_t)(g.x + x) * 3;
+        out[3 * i] = src[0];
+        out[3 * i + 1] = src[1];
+        out[3 * i + 2] = src[2];
+    }
+}
+
 // Laplacian blur score of the gray frame (ImageQualityEstimate::QuantizeBlurByLaplacian,
 // Util/ImageQualityEstimate.cpp:211-230: cv::Laplacian, ksize 1 = the 4-neighbour stencil, FP32,
 // BORDER_REFLECT_101), gathered straight into the MIs' blur tiles (the crops of
@@ -2340,6 +2359,20 @@ int lfpm_slice_gray(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pit
                     const int* origins_dev, int num_mis, int crop, float* gray_dev, void* stream) {
     return slice_impl(raw_dev, raw_w, raw_h, raw_pitch, origins_dev, num_mis, crop, gray_dev,
                       0.0f, stream);
+}
+
+int lfpm_slice_bgr_tiles(const uint8_t* bgr_dev, int raw_w, int raw_h, size_t bgr_pitch,
+                         const int* origins_dev, int num_mis, int crop, uint8_t* bgr_tiles_dev,
+                         void* stream) {
+    if (!bgr_dev || !origins_dev || !bgr_tiles_dev || num_mis <= 0 || crop < 1 ||
+        raw_w < crop || raw_h < crop || bgr_pitch < (size_t)raw_w * 3)
+        return set_error(LFPM_E_INVALID, "slice_bgr: bad arguments");
+    const size_t total = (size_t)num_mis * crop * crop;
+    const int threads = 256;
+    const int blocks = (int)std::min<size_t>((total + threads - 1) / threads, 65536);
+    hipLaunchKernelGGL(k_slice_bgr, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, bgr_dev,
+                       bgr_pitch, (const int2*)origins_dev, num_mis, crop, bgr_tiles_dev);
+    return launch_check("slice_bgr");
 }
 
 int lfpm_slice_blur_tiles(const uint8_t* raw_dev, int raw_w, int raw_h, size_t raw_pitch,

@@ -2,9 +2,9 @@
 //   * the raw-size disparity mosaic (every MI pixel whose cost passes the gate lands at its raw
 //     position) and the frame's disparity range, and
 //   * the virtual-depth (VD) map: every MI pixel splats f_v = mla_base / d at its projected raw
-//     position; each raw pixel keeps the median of its samples, plus the gray "real colour"
-//     image (median colour, empty pixels filled by the directional search, ten 5x5 median
-//     passes).
+//     position; each raw pixel keeps the median of its samples, plus the "real colour" image,
+//     gray or B, G, R (per channel: median colour, empty pixels filled by the directional
+//     search, ten 5x5 median passes).
 // Both are HBM-bound scatter / gather passes over V*crop^2 MI pixels and raw_w*raw_h raw
 // pixels.  Every result is deterministic: scatters either resolve collisions by an order-free
 // rule (highest MI index wins) or sort what they collected before reducing it.
@@ -252,9 +252,12 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_chunks(const int* __restr
     }
 }
 
+// Samples: sv = f_v, sc = colour; with colour crops (bgr: V*crop^2 pixels B, G, R) sc holds
+// three planes of M bytes (channel c of sample slot s at sc[c*M + s]).
 __global__ void __launch_bounds__(256) k_vd_scatter(const float4* __restrict__ plane,
                                                     const float2* __restrict__ centers,
-                                                    const float* __restrict__ gray, int V,
+                                                    const float* __restrict__ gray,
+                                                    const uint8_t* __restrict__ bgr, int V,
                                                     int crop, float mla_base, int raw_w,
                                                     int raw_h, int* __restrict__ count,
                                                     const int* __restrict__ offs,
@@ -268,14 +271,23 @@ __global__ void __launch_bounds__(256) k_vd_scatter(const float4* __restrict__ p
     if (t < 0) return;
     const int slot = offs[t] + atomicSub(&count[t], 1) - 1;
     sv[slot] = fv;
-    sc[slot] = gray ? (uint8_t)gray[i] : (uint8_t)0;
+    if (bgr) {
+        sc[slot] = bgr[3 * i];
+        sc[M + slot] = bgr[3 * i + 1];
+        sc[2 * M + slot] = bgr[3 * i + 2];
+    } else {
+        sc[slot] = gray ? (uint8_t)gray[i] : (uint8_t)0;
+    }
 }
 
 // Median of each raw pixel's samples (sorted in place).  VD value: the reference's index rule
 // (s[n/2+1] + s[(n+1)/2]) / 2, with indices clamped to n-1 (the rule reads past the end for
-// n <= 2); colour: (c[n/2] + c[(n+1)/2-1]) / 2 in integers.
+// n <= 2); colour, each of the CH channels sorted on its own (color0/1/2 of
+// DepthSolver.cpp:7919-7927): (c[n/2] + c[(n+1)/2-1]) / 2 in integers.  Colour planes: sample
+// planes sc + c*M, median planes col + c*N.
+template <int CH>
 __global__ void __launch_bounds__(256) k_vd_median(const int* __restrict__ offs, long long N,
-                                                   float* __restrict__ sv,
+                                                   long long M, float* __restrict__ sv,
                                                    uint8_t* __restrict__ sc,
                                                    float* __restrict__ vd,
                                                    uint8_t* __restrict__ col,
@@ -285,31 +297,41 @@ __global__ void __launch_bounds__(256) k_vd_median(const int* __restrict__ offs,
     float m = 0.0f;
     if (p < N) {
         const int b = offs[p], n = offs[p + 1] - b;
-        uint8_t cm = 0;
+        uint8_t cm[CH];
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) cm[ch] = 0;
         if (n == 1 || n == 2) {
             // the index rule reads s[n-1] twice for n <= 2 (the largest sample; halving the
             // doubled value is exact), the colour rule averages the two samples
             const float s0 = sv[b], s1 = n == 2 ? sv[b + 1] : s0;
-            const int c0 = sc[b], c1 = n == 2 ? sc[b + 1] : c0;
             m = ((s0 > s1 ? s0 : s1) + (s0 > s1 ? s0 : s1)) / 2.0f;
-            cm = (uint8_t)((c0 + c1) / 2);
+#pragma unroll
+            for (int ch = 0; ch < CH; ++ch) {
+                const uint8_t* c = sc + ch * M;
+                const int c0 = c[b], c1 = n == 2 ? c[b + 1] : c0;
+                cm[ch] = (uint8_t)((c0 + c1) / 2);
+            }
             has = true;
         } else if (n <= 8 && n > 0) {
             // 8-wire sorting network in registers, padding above every sample (+inf, 256)
             float f[8];
-            int q[8];
+            int q[CH][8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 f[k] = k < n ? sv[b + k] : INFINITY;
-                q[k] = k < n ? (int)sc[b + k] : 256;
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch) q[ch][k] = k < n ? (int)sc[ch * M + b + k] : 256;
             }
             auto cx = [&](int i, int j) {
                 const float lo = fminf(f[i], f[j]), hi = fmaxf(f[i], f[j]);
                 f[i] = lo;
                 f[j] = hi;
-                const int a = min(q[i], q[j]), c = max(q[i], q[j]);
-                q[i] = a;
-                q[j] = c;
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch) {
+                    const int a = min(q[ch][i], q[ch][j]), c = max(q[ch][i], q[ch][j]);
+                    q[ch][i] = a;
+                    q[ch][j] = c;
+                }
             };
             // Batcher's odd-even merge sort, 8 inputs (19 comparators)
             cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
@@ -321,37 +343,48 @@ __global__ void __launch_bounds__(256) k_vd_median(const int* __restrict__ offs,
             const int i1 = min(n / 2 + 1, n - 1), i2 = min((n + 1) / 2, n - 1);
             const int j1 = n / 2, j2 = (n + 1) / 2 - 1;
             float a1 = f[0], a2 = f[0];
-            int d1 = q[0], d2 = q[0];
+            int d1[CH], d2[CH];
+#pragma unroll
+            for (int ch = 0; ch < CH; ++ch) d1[ch] = d2[ch] = q[ch][0];
 #pragma unroll
             for (int k = 1; k < 8; ++k) {
                 a1 = k == i1 ? f[k] : a1;
                 a2 = k == i2 ? f[k] : a2;
-                d1 = k == j1 ? q[k] : d1;
-                d2 = k == j2 ? q[k] : d2;
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch) {
+                    d1[ch] = k == j1 ? q[ch][k] : d1[ch];
+                    d2[ch] = k == j2 ? q[ch][k] : d2[ch];
+                }
             }
             m = (a1 + a2) / 2.0f;
-            cm = (uint8_t)((d1 + d2) / 2);
+#pragma unroll
+            for (int ch = 0; ch < CH; ++ch) cm[ch] = (uint8_t)((d1[ch] + d2[ch]) / 2);
             has = true;
         } else if (n > 0) {
             float* s = sv + b;
-            uint8_t* c = sc + b;
             for (int a = 1; a < n; ++a) {
                 const float x = s[a];
                 int j = a - 1;
                 while (j >= 0 && s[j] > x) { s[j + 1] = s[j]; --j; }
                 s[j + 1] = x;
-                const uint8_t y = c[a];
-                j = a - 1;
-                while (j >= 0 && c[j] > y) { c[j + 1] = c[j]; --j; }
-                c[j + 1] = y;
             }
             const int i1 = min(n / 2 + 1, n - 1), i2 = min((n + 1) / 2, n - 1);
             m = (s[i1] + s[i2]) / 2.0f;
-            cm = (uint8_t)(((int)c[n / 2] + (int)c[(n + 1) / 2 - 1]) / 2);
+            for (int ch = 0; ch < CH; ++ch) {
+                uint8_t* c = sc + ch * M + b;
+                for (int a = 1; a < n; ++a) {
+                    const uint8_t y = c[a];
+                    int j = a - 1;
+                    while (j >= 0 && c[j] > y) { c[j + 1] = c[j]; --j; }
+                    c[j + 1] = y;
+                }
+                cm[ch] = (uint8_t)(((int)c[n / 2] + (int)c[(n + 1) / 2 - 1]) / 2);
+            }
             has = true;
         }
         vd[p] = m;
-        col[p] = cm;
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) col[ch * N + p] = cm[ch];
     }
     block_minmax(range, has, m);
 }
@@ -559,11 +592,15 @@ __global__ void __launch_bounds__(256) k_fill_resolve(const int* __restrict__ of
                                                       const int* __restrict__ nd,
                                                       const int* __restrict__ pu,
                                                       const uint8_t* __restrict__ colm,
-                                                      uint8_t* __restrict__ out) {
+                                                      int nch, uint8_t* __restrict__ out) {
+    // colm and out: nch planes of N bytes; every channel takes the colour of the same pixel
     const long long N = (long long)raw_w * raw_h;
     const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (p >= N) return;
-    if (offs[p + 1] > offs[p]) { out[p] = colm[p]; return; }
+    if (offs[p + 1] > offs[p]) {
+        for (int ch = 0; ch < nch; ++ch) out[ch * N + p] = colm[ch * N + p];
+        return;
+    }
     const int y = (int)(p / raw_w);
     const long long INF = 0x7fffffffffffffffLL;
     long long best_t = INF;
@@ -592,7 +629,7 @@ __global__ void __launch_bounds__(256) k_fill_resolve(const int* __restrict__ of
         const long long k = pu[p];
         consider(3, k >= 0 ? (p - k) / raw_w : INF, k, (long long)y + 1);
     }
-    out[p] = best_k >= 0 ? colm[best_k] : (uint8_t)0;
+    for (int ch = 0; ch < nch; ++ch) out[ch * N + p] = best_k >= 0 ? colm[ch * N + best_k] : (uint8_t)0;
 }
 
 // 5x5 median with replicated borders (cv::medianBlur's documented border rule).  One lane per
@@ -600,8 +637,11 @@ __global__ void __launch_bounds__(256) k_fill_resolve(const int* __restrict__ of
 // one word, and the median-of-25 selection network runs on packed 16-bit min / max.
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
+// out: pixel (x, y) at out[(y * raw_w + x) * ostride] (ostride 3: one channel of an
+// interleaved B, G, R image).
 __global__ void __launch_bounds__(256) k_median5(const uint8_t* __restrict__ in, int raw_w,
-                                                 int raw_h, uint8_t* __restrict__ out) {
+                                                 int raw_h, uint8_t* __restrict__ out,
+                                                 int ostride) {
     const int pairs = (raw_w + 1) >> 1;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= (long long)pairs * raw_h) return;
@@ -623,9 +663,9 @@ __global__ void __launch_bounds__(256) k_median5(const uint8_t* __restrict__ in,
     const u16x2 m = median25(
         v, [](u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); },
         [](u16x2 a, u16x2 b) { return __builtin_elementwise_max(a, b); });
-    uint8_t* o = out + (long long)y * raw_w + x0;
+    uint8_t* o = out + ((long long)y * raw_w + x0) * ostride;
     o[0] = (uint8_t)m.x;
-    if (x0 + 1 < raw_w) o[1] = (uint8_t)m.y;
+    if (x0 + 1 < raw_w) o[ostride] = (uint8_t)m.y;
 }
 
 int grid_for(long long n, int threads) {
@@ -651,7 +691,7 @@ __global__ void __launch_bounds__(256) k_pt_flag(const float* __restrict__ vd, l
 __device__ __forceinline__ bool ply_open(float v) { return v < FLT_MAX && v > -FLT_MAX; }
 
 __global__ void __launch_bounds__(256) k_pt_emit(const float* __restrict__ vd,
-                                                 const uint8_t* __restrict__ color,
+                                                 const uint8_t* __restrict__ color, int nch,
                                                  const int* __restrict__ offs, int raw_w,
                                                  long long N, float ox, float oy, float pix,
                                                  float f, float B, float bl0,
@@ -679,13 +719,20 @@ __global__ void __launch_bounds__(256) k_pt_emit(const float* __restrict__ vd,
     float Xy = (float)(-fd / (tz - fd) * ty);
     // the writer's finiteness test (z admits -FLT_MAX, x and y do not)
     if (!ply_open(Xx) || !ply_open(Xy) || !(X < FLT_MAX && X >= -FLT_MAX)) Xx = Xy = X = 0.0f;
-    const uint8_t g = color ? color[i] : (uint8_t)0;
     uint8_t r[27];
     const float v[6] = {Xx, Xy, X, 0.0f, 0.0f, 0.0f};
     __builtin_memcpy(r, v, 24);
-    r[24] = g;  // red, green, blue of the gray colour image
-    r[25] = g;
-    r[26] = g;
+    if (nch == 3) {
+        // red, green, blue of a B, G, R image: the writer emits color.z, .y, .x
+        r[24] = color[3 * i + 2];
+        r[25] = color[3 * i + 1];
+        r[26] = color[3 * i];
+    } else {
+        const uint8_t g = color ? color[i] : (uint8_t)0;  // the gray colour image
+        r[24] = g;
+        r[25] = g;
+        r[26] = g;
+    }
     uint8_t* dst = rec + (size_t)offs[i] * 27;
 #pragma unroll
     for (int k = 0; k < 27; ++k) dst[k] = r[k];
@@ -705,7 +752,7 @@ VdLayout vd_layout(int V, int crop, int raw_w, int raw_h) {
     L.offs = o; o += al((N + 1) * 4);
     L.sums = o; o += al(nb * 4);
     L.sv = o; o += al(M * 4);
-    L.sc = o; o += al(M);
+    L.sc = o; o += al(3 * M);
     L.nr = o; o += al(N * 4);
     L.pr = o; o += al(N * 4);
     L.nd = o; o += al(N * 4);
@@ -714,7 +761,7 @@ VdLayout vd_layout(int V, int crop, int raw_w, int raw_h) {
     L.rlast = o; o += al((size_t)raw_h * 4);
     L.nrf = o; o += al((size_t)raw_h * 4);
     L.prl = o; o += al((size_t)raw_h * 4);
-    L.col = o; o += al(N);
+    L.col = o; o += al(3 * N);
     L.range = o; o += al(kRangeBytes);
     // column segments (raw_w per segment) and diagonal segments (raw_w - 1 chains per segment)
     const size_t ncs = ((size_t)raw_h + kFillSeg - 1) / kFillSeg;
@@ -783,10 +830,16 @@ int lfpm_disparity_mosaic(const float* plane_dev, const float* cost_dev, const f
     return check_launch("lfpm_disparity_mosaic");
 }
 
-int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, const float* gray_dev,
-                           int num_mis, int crop, float mla_base, int raw_w, int raw_h,
-                           float* vd_dev, uint8_t* color_dev, float* range_dev,
-                           void* workspace_dev, size_t workspace_bytes, void* stream) {
+}  // extern "C"
+
+namespace lfpm {
+namespace {
+// The VD map with its colour image from the gray crops (gray_dev, one channel) or the colour
+// crops (bgr_dev, B, G, R: color_dev is then raw_h x raw_w x 3, interleaved).
+int vd_map_impl(const float* plane_dev, const float* centers_dev, const float* gray_dev,
+                const uint8_t* bgr_dev, int num_mis, int crop, float mla_base, int raw_w,
+                int raw_h, float* vd_dev, uint8_t* color_dev, float* range_dev,
+                void* workspace_dev, size_t workspace_bytes, void* stream) {
     if (int e = check_frame(num_mis, crop, raw_w, raw_h)) return e;
     if (!plane_dev || !centers_dev || !vd_dev || !workspace_dev)
         return set_error(LFPM_E_INVALID, "lfpm_virtual_depth_map: null pointer");
@@ -816,10 +869,15 @@ int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, con
     k_scan_sums<<<nb, kScanThreads, 0, s>>>(count, N + 1, sums);
     k_scan_sums_scan<<<1, kScanThreads, 0, s>>>(sums, nb);
     k_scan_chunks<<<nb, kScanThreads, 0, s>>>(count, N + 1, sums, offs);
-    k_vd_scatter<<<grid_for(M, 256), 256, 0, s>>>(plane, centers, gray_dev, num_mis, crop,
-                                                   mla_base, raw_w, raw_h, count, offs, sv, sc);
+    k_vd_scatter<<<grid_for(M, 256), 256, 0, s>>>(plane, centers, gray_dev, bgr_dev, num_mis,
+                                                   crop, mla_base, raw_w, raw_h, count, offs, sv,
+                                                   sc);
     uint8_t* colm = ws + L.col;
-    k_vd_median<<<grid_for(N, 256), 256, 0, s>>>(offs, N, sv, sc, vd_dev, colm, range);
+    const int nch = bgr_dev ? 3 : 1;
+    if (bgr_dev)
+        k_vd_median<3><<<grid_for(N, 256), 256, 0, s>>>(offs, N, M, sv, sc, vd_dev, colm, range);
+    else
+        k_vd_median<1><<<grid_for(N, 256), 256, 0, s>>>(offs, N, M, sv, sc, vd_dev, colm, range);
     if (range_dev) k_range_out<<<1, 1, 0, s>>>(range, range_dev);
     if (color_dev) {
         int* nr = (int*)(ws + L.nr);
@@ -849,21 +907,59 @@ int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, con
             k_cols_carry<<<grid_for(raw_w, 256), 256, 0, s>>>(raw_w, ncs, segc);
             k_cols_links<<<g, 256, 0, s>>>(offs, raw_w, raw_h, segc, pu);
         }
-        // filled colour -> color_dev, then ten 5x5 median passes ping-ponging with a scratch
-        // image in the fill tables' memory (no longer needed); an even pass count ends in
-        // color_dev
+        // filled colour planes -> color_dev, then ten 5x5 median passes per channel
+        // ping-ponging with scratch planes in the fill tables' memory (no longer needed); the
+        // tenth pass writes each channel into its place in color_dev (interleaved for B, G, R)
         k_fill_resolve<<<grid_for(N, 256), 256, 0, s>>>(offs, raw_w, raw_h, nr, pr, nrf, prl, nd,
-                                                        pu, colm, color_dev);
-        uint8_t* tmp = (uint8_t*)nr;
+                                                        pu, colm, nch, color_dev);
+        uint8_t* tmp = (uint8_t*)nr;  // N * 4 bytes >= nch planes
         const long long npairs = (long long)((raw_w + 1) >> 1) * raw_h;
-        for (int it = 0; it < 10; ++it) {
-            if (it & 1)
-                k_median5<<<grid_for(npairs, 256), 256, 0, s>>>(tmp, raw_w, raw_h, color_dev);
-            else
-                k_median5<<<grid_for(npairs, 256), 256, 0, s>>>(color_dev, raw_w, raw_h, tmp);
+        for (int ch = 0; ch < nch; ++ch) {
+            uint8_t* a = color_dev + ch * N;
+            uint8_t* b = tmp + ch * N;
+            for (int it = 0; it < 9; ++it) {
+                if (it & 1)
+                    k_median5<<<grid_for(npairs, 256), 256, 0, s>>>(b, raw_w, raw_h, a, 1);
+                else
+                    k_median5<<<grid_for(npairs, 256), 256, 0, s>>>(a, raw_w, raw_h, b, 1);
+            }
         }
+        for (int ch = 0; ch < nch; ++ch)
+            k_median5<<<grid_for(npairs, 256), 256, 0, s>>>(tmp + ch * N, raw_w, raw_h,
+                                                            color_dev + ch, nch);
     }
     return check_launch("lfpm_virtual_depth_map");
+}
+
+int point_cloud_impl(const float* vd_dev, const uint8_t* color_dev, int nch, int raw_w,
+                     int raw_h, float origin_x, float origin_y, float pixel_size,
+                     float main_lens_f, float B, float bl0, uint8_t* records_dev,
+                     size_t records_bytes, unsigned* count_dev, void* workspace_dev,
+                     size_t workspace_bytes, void* stream);
+}  // namespace
+}  // namespace lfpm
+
+extern "C" {
+
+int lfpm_virtual_depth_map(const float* plane_dev, const float* centers_dev, const float* gray_dev,
+                           int num_mis, int crop, float mla_base, int raw_w, int raw_h,
+                           float* vd_dev, uint8_t* color_dev, float* range_dev,
+                           void* workspace_dev, size_t workspace_bytes, void* stream) {
+    return vd_map_impl(plane_dev, centers_dev, gray_dev, nullptr, num_mis, crop, mla_base, raw_w,
+                       raw_h, vd_dev, color_dev, range_dev, workspace_dev, workspace_bytes,
+                       stream);
+}
+
+int lfpm_virtual_depth_map_bgr(const float* plane_dev, const float* centers_dev,
+                               const uint8_t* bgr_tiles_dev, int num_mis, int crop,
+                               float mla_base, int raw_w, int raw_h, float* vd_dev,
+                               uint8_t* color_dev, float* range_dev, void* workspace_dev,
+                               size_t workspace_bytes, void* stream) {
+    if (!bgr_tiles_dev)
+        return set_error(LFPM_E_INVALID, "lfpm_virtual_depth_map_bgr: null colour crops");
+    return vd_map_impl(plane_dev, centers_dev, nullptr, bgr_tiles_dev, num_mis, crop, mla_base,
+                       raw_w, raw_h, vd_dev, color_dev, range_dev, workspace_dev,
+                       workspace_bytes, stream);
 }
 
 int lfpm_vd_point_cloud(const float* vd_dev, const uint8_t* color_dev, int raw_w, int raw_h,
@@ -871,6 +967,32 @@ int lfpm_vd_point_cloud(const float* vd_dev, const uint8_t* color_dev, int raw_w
                         float B, float bl0, uint8_t* records_dev, size_t records_bytes,
                         unsigned* count_dev, void* workspace_dev, size_t workspace_bytes,
                         void* stream) {
+    return point_cloud_impl(vd_dev, color_dev, 1, raw_w, raw_h, origin_x, origin_y, pixel_size,
+                            main_lens_f, B, bl0, records_dev, records_bytes, count_dev,
+                            workspace_dev, workspace_bytes, stream);
+}
+
+int lfpm_vd_point_cloud_bgr(const float* vd_dev, const uint8_t* color_bgr_dev, int raw_w,
+                            int raw_h, float origin_x, float origin_y, float pixel_size,
+                            float main_lens_f, float B, float bl0, uint8_t* records_dev,
+                            size_t records_bytes, unsigned* count_dev, void* workspace_dev,
+                            size_t workspace_bytes, void* stream) {
+    if (!color_bgr_dev)
+        return set_error(LFPM_E_INVALID, "lfpm_vd_point_cloud_bgr: null colour image");
+    return point_cloud_impl(vd_dev, color_bgr_dev, 3, raw_w, raw_h, origin_x, origin_y,
+                            pixel_size, main_lens_f, B, bl0, records_dev, records_bytes,
+                            count_dev, workspace_dev, workspace_bytes, stream);
+}
+
+}  // extern "C"
+
+namespace lfpm {
+namespace {
+int point_cloud_impl(const float* vd_dev, const uint8_t* color_dev, int nch, int raw_w,
+                     int raw_h, float origin_x, float origin_y, float pixel_size,
+                     float main_lens_f, float B, float bl0, uint8_t* records_dev,
+                     size_t records_bytes, unsigned* count_dev, void* workspace_dev,
+                     size_t workspace_bytes, void* stream) {
     if (raw_w <= 0 || raw_h <= 0 || (long long)raw_w * raw_h + 1 >= 0x7fffffffLL)
         return set_error(LFPM_E_INVALID, "lfpm_vd_point_cloud: bad frame %dx%d", raw_w, raw_h);
     if (!vd_dev || !records_dev || !count_dev || !workspace_dev)
@@ -893,10 +1015,10 @@ int lfpm_vd_point_cloud(const float* vd_dev, const uint8_t* color_dev, int raw_w
     k_scan_sums<<<(int)nb, kScanThreads, 0, s>>>(flag, N + 1, sums);
     k_scan_sums_scan<<<1, kScanThreads, 0, s>>>(sums, (int)nb);
     k_scan_chunks<<<(int)nb, kScanThreads, 0, s>>>(flag, N + 1, sums, offs);
-    k_pt_emit<<<grid_for(N, 256), 256, 0, s>>>(vd_dev, color_dev, offs, raw_w, N, origin_x,
+    k_pt_emit<<<grid_for(N, 256), 256, 0, s>>>(vd_dev, color_dev, nch, offs, raw_w, N, origin_x,
                                                 origin_y, pixel_size, main_lens_f, B, bl0,
                                                 records_dev, count_dev);
     return check_launch("lfpm_vd_point_cloud");
 }
-
-}  // extern "C"
+}  // namespace
+}  // namespace lfpm

@@ -316,8 +316,8 @@ def bgr_to_gray(bgr, out=None, stream=None):
 def vd_point_cloud(vd, color=None, origin=(8.0, 8.0), pixel_size=1.0, main_lens_f=1.0, B=1.0,
                    bl0=0.0, ws=None, stream=None):
     """VD point cloud on the device (lfpm_vd_point_cloud): (records uint8 (n, 27), n).  vd and
-    color are the (raw_h, raw_w) outputs of virtual_depth_map; origin is the reference's
-    (crop / 2, crop / 2)."""
+    color are outputs of virtual_depth_map (color (raw_h, raw_w) gray or (raw_h, raw_w, 3)
+    B, G, R: lfpm_vd_point_cloud_bgr); origin is the reference's (crop / 2, crop / 2)."""
     import torch
     h, w = vd.shape
     recs = torch.empty((h * w, 27), dtype=torch.uint8, device=vd.device)
@@ -326,7 +326,9 @@ def vd_point_cloud(vd, color=None, origin=(8.0, 8.0), pixel_size=1.0, main_lens_
         n = h * w + 1
         ws = torch.empty(2 * ((n * 4 + 255) // 256 * 256) + (n // 4096 + 2) * 4 + 512,
                          dtype=torch.uint8, device=vd.device)
-    _check(lib().lfpm_vd_point_cloud(
+    bgr = color is not None and color.dim() == 3
+    fn = lib().lfpm_vd_point_cloud_bgr if bgr else lib().lfpm_vd_point_cloud
+    _check(fn(
         _dp(vd), _dp(color) if color is not None else None, w, h, C.c_float(origin[0]),
         C.c_float(origin[1]), C.c_float(pixel_size), C.c_float(main_lens_f), C.c_float(B),
         C.c_float(bl0), _dp(recs), C.c_size_t(recs.numel()), _dp(cnt), _dp(ws),
@@ -526,17 +528,40 @@ def disparity_mosaic(df, plane, cost, max_cost=0.2, ws=None, stream=None):
     return out, rng
 
 
-def virtual_depth_map(df, plane, mla_base=None, color=True, ws=None, stream=None):
+def slice_bgr_tiles(df, bgr, stream=None):
+    """Colour crops (V, crop, crop, 3) B, G, R of a device colour frame bgr (h, w, 3) uint8 at
+    the frame's MI origins (lfpm_slice_bgr_tiles)."""
+    import torch
+    wl = df.wl
+    h, w = bgr.shape[:2]
+    assert bgr.is_contiguous() and bgr.shape[2] == 3
+    out = torch.empty((wl.V, wl.crop, wl.crop, 3), dtype=torch.uint8, device=bgr.device)
+    _check(lib().lfpm_slice_bgr_tiles(_dp(bgr), w, h, C.c_size_t(bgr.stride(0)),
+                                      _dp(df.origins), wl.V, wl.crop, _dp(out), _stream(stream)))
+    return out
+
+
+def virtual_depth_map(df, plane, mla_base=None, color=True, ws=None, stream=None,
+                      bgr_tiles=None):
     """Virtual-depth median splat (Virtual_depth_map_TileKeySequence, DepthSolver.cpp:7832-8000).
-    Returns (vd float32 (h, w), colour uint8 (h, w) or None, range float32 (2,))."""
+    Returns (vd float32 (h, w), colour or None, range float32 (2,)); the colour image is uint8
+    (h, w) from the gray crops, or (h, w, 3) B, G, R from the colour crops bgr_tiles
+    (slice_bgr_tiles; lfpm_virtual_depth_map_bgr)."""
     import torch
     wl = df.wl
     h, w = df.raw.shape
     mla_base = wl.L["pitch"] if mla_base is None else mla_base
     ws = products_workspace(df) if ws is None else ws
     vd = torch.empty((h, w), dtype=torch.float32, device=plane.device)
-    col = torch.empty((h, w), dtype=torch.uint8, device=plane.device) if color else None
     rng = torch.empty(2, dtype=torch.float32, device=plane.device)
+    if bgr_tiles is not None:
+        col = torch.empty((h, w, 3), dtype=torch.uint8, device=plane.device)
+        _check(lib().lfpm_virtual_depth_map_bgr(_dp(plane), _dp(df.centers), _dp(bgr_tiles),
+                                                wl.V, wl.crop, C.c_float(mla_base), w, h,
+                                                _dp(vd), _dp(col), _dp(rng), _dp(ws),
+                                                C.c_size_t(ws.numel()), _stream(stream)))
+        return vd, col, rng
+    col = torch.empty((h, w), dtype=torch.uint8, device=plane.device) if color else None
     _check(lib().lfpm_virtual_depth_map(_dp(plane), _dp(df.centers), _dp(df.gray), wl.V,
                                         wl.crop, C.c_float(mla_base), w, h, _dp(vd),
                                         _dp(col) if col is not None else None, _dp(rng), _dp(ws),

@@ -75,3 +75,36 @@ def test_products_edge_cases(lfpm, orc, torch_cuda):
     cost[:] = 0.0
     _check(lfpm, orc, torch, df, torch.from_numpy(plane).cuda(), torch.from_numpy(cost).cuda(),
            "dense")
+
+
+def test_products_bgr_colour(lfpm, orc, torch_cuda):
+    """Colour VD map (lfpm_slice_bgr_tiles, lfpm_virtual_depth_map_bgr) and its coloured point
+    cloud (lfpm_vd_point_cloud_bgr) on a solved frame with a colour raw image, bit for bit vs the
+    oracle: VD values, the B, G, R image (per-channel medians, fill, ten 5x5 passes per channel)
+    and the 27-byte records with red, green, blue from channels 2, 1, 0."""
+    torch = torch_cuda
+    wl = lfpm.workload_c1(raw_w=640, raw_h=512)
+    _say("case products bgr V=%d" % wl.V)
+    df = lfpm.DeviceFrame(wl)
+    bufs = lfpm.patchmatch_frame(df)
+    raw = wl.raw().astype(np.int64)
+    rng = np.random.default_rng(21)
+    bgr = np.stack([raw, (raw * 7 + 11) % 256, rng.integers(0, 256, raw.shape)], -1).astype(np.uint8)
+    bgr_dev = torch.from_numpy(bgr).cuda()
+    tiles = lfpm.slice_bgr_tiles(df, bgr_dev)
+    vd, col, vr = lfpm.virtual_depth_map(df, bufs.plane, bgr_tiles=tiles)
+    recs, n = lfpm.vd_point_cloud(vd, col, origin=(8.0, 8.0), pixel_size=0.0055,
+                                  main_lens_f=50.0, B=1.3, bl0=70.0)
+    torch.cuda.synchronize()
+    p = bufs.plane.cpu().numpy()
+    otiles = orc.slice_bgr(bgr, wl.L["origins"], wl.crop)
+    assert np.array_equal(tiles.cpu().numpy(), otiles)
+    ovd, ocol, ovr = orc.vd_map_bgr(p, wl.L["centers"], otiles, wl.crop, wl.L["pitch"], 640, 512)
+    g_vd, g_col = vd.cpu().numpy(), col.cpu().numpy()
+    assert np.array_equal(g_vd.view(np.uint32), ovd.view(np.uint32))
+    assert np.array_equal(vr.cpu().numpy().view(np.uint32), ovr.view(np.uint32))
+    assert np.array_equal(g_col, ocol)
+    orec = orc.vd_point_cloud(ovd, ocol, (8.0, 8.0), 0.0055, 50.0, 1.3, 70.0)
+    assert n == orec.shape[0] > 1000
+    assert np.array_equal(recs.cpu().numpy(), orec)
+    assert (ocol[..., 0] != ocol[..., 2]).mean() > 0.3

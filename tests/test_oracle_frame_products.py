@@ -124,3 +124,50 @@ def test_vd_fill_probes_right_before_left(orc):
     gap = blk0[-1] + 1
     assert blk1[0] == gap + 1 and np.count_nonzero(vd[40, :]) == 30
     assert col[40, gap] == 200 and col[40, gap - 1] == 50 and col[40, gap + 1] == 200
+
+
+def _solved_like_plane(V, crop, seed):
+    rng = np.random.default_rng(seed)
+    n = V * crop * crop
+    p = np.zeros((n, 4), np.float32)
+    p[:, 2] = 1.0
+    p[:, 3] = rng.uniform(2.5, 9.0, n).astype(np.float32)
+    return p
+
+
+def test_vd_bgr_channels_are_independent(orc, lfpm):
+    """The colour VD map treats B, G and R as three independent gray images (the reference sorts
+    color0/1/2 separately, DepthSolver.cpp:7919-7927, and medianBlur filters each channel): its
+    channel c equals the gray map run on channel c alone, and the VD values are the gray run's.
+    The channels are chosen so that sorting them together (one sample's colour for all three)
+    would differ."""
+    wl = lfpm.workload_c1(raw_w=256, raw_h=200)
+    crop, V = wl.crop, wl.V
+    plane = _solved_like_plane(V, crop, 5)
+    rng = np.random.default_rng(9)
+    bgr = rng.integers(0, 256, (V, crop, crop, 3)).astype(np.uint8)
+    vd, col, r = orc.vd_map_bgr(plane, wl.L["centers"], bgr, crop, wl.L["pitch"], 256, 200)
+    assert col.shape == (200, 256, 3)
+    for c in range(3):
+        vdc, colc, rc = orc.vd_map(plane, wl.L["centers"], bgr[..., c].astype(np.float32).ravel(),
+                                   crop, wl.L["pitch"], 256, 200)
+        assert np.array_equal(vdc.view(np.uint32), vd.view(np.uint32))
+        assert np.array_equal(rc, r)
+        assert np.array_equal(colc, col[..., c]), c
+    # co-sorted colours would make the three channels one sample's colour; here they differ
+    assert (col[..., 0] != col[..., 1]).mean() > 0.5
+
+
+def test_vd_bgr_point_colours(orc):
+    """Point records take red, green, blue from channels 2, 1, 0 of the B, G, R image (the
+    writer's r/g/b = color.z/.y/.x, MVStereo/ACMP.cpp:479-481)."""
+    vd = np.zeros((4, 5), np.float32)
+    vd[1, 2] = 3.0
+    vd[3, 4] = 5.0
+    col = np.zeros((4, 5, 3), np.uint8)
+    col[1, 2] = (10, 20, 30)
+    col[3, 4] = (200, 150, 100)
+    rec = orc.vd_point_cloud(vd, col, (2.0, 2.0), 1.0, 50.0, 1.0, 0.0)
+    assert rec.shape == (2, 27)
+    assert rec[0, 24:].tolist() == [30, 20, 10]
+    assert rec[1, 24:].tolist() == [100, 150, 200]
