@@ -108,8 +108,13 @@ int lfpm_layout_fixed_neighbors(const lfpm_layout_params* p, int* neighbor_ids,
  * "Calib"; BlurInfo/MLAType id="t" -> blur_radius[t] (t in 0..2, 0 when absent); MLAInfo/MLA
  * {key "row,col" (%i: a leading 0 reads octal, 0x hex), center "x,y", MLAType, aera} per lens,
  * returned in (row, col) order (the MLA map's QuadTreeTileKey order; a repeated key keeps its
- * first entry).  tilekeys: 2*count ints (col, row) as lfpm_layout_build writes them.  With every
- * array NULL only count (and blur_radius) is returned; otherwise capacity must hold count.
+ * last entry, as the map assignment of :1242 does).  tilekeys: 2*count ints (col, row) as
+ * lfpm_layout_build writes them.  With every array NULL only count (and blur_radius) is
+ * returned; otherwise capacity must hold count.  blur_radius holds the BlurInfo values wherever
+ * the group sits in the document; the reference gives each lens the radius its type has when
+ * that MLA element is parsed, so a BlurInfo group placed after MLAInfo leaves its lenses at 0
+ * (a lens's radius is blur_radius[type] only when BlurInfo comes first, as in the files the
+ * reference's calibration writes).
  *
  * lfpm_table_fixed_neighbors is lfpm_layout_fixed_neighbors over such a table: the lens
  * lattice is lattice_cols x lattice_rows keys (the intrinsics' mla_u_size x mla_v_size); keys

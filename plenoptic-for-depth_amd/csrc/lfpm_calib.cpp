@@ -285,14 +285,21 @@ int lfpm_read_calib_xml(const char* path, int capacity, int* tilekeys, float* ce
             }
         }
     }
-    // the MLA map is keyed by (row, col) (QuadTreeTileKey order, Common/QuadTree.h:137-148); a
-    // repeated key keeps its first entry, as std::map::insert does
+    // the MLA map is keyed by (row, col) (QuadTreeTileKey order, Common/QuadTree.h:137-148,
+    // compared by value through QuadTreeTileKeyMapCmpLess); a repeated key keeps its last entry,
+    // as the reference's m_MLA_info_map[ptrKey] = ptrInfo assignment does (DepthSolver.cpp:1242):
+    // stable sort, then the last of each run of equal keys
     std::stable_sort(mla.begin(), mla.end(), [](const Mla& a, const Mla& b) {
         return a.row != b.row ? a.row < b.row : a.col < b.col;
     });
-    mla.erase(std::unique(mla.begin(), mla.end(),
-                          [](const Mla& a, const Mla& b) { return a.row == b.row && a.col == b.col; }),
-              mla.end());
+    {
+        std::vector<Mla> last;
+        last.reserve(mla.size());
+        for (size_t i = 0; i < mla.size(); ++i)
+            if (i + 1 == mla.size() || mla[i + 1].row != mla[i].row || mla[i + 1].col != mla[i].col)
+                last.push_back(mla[i]);
+        mla.swap(last);
+    }
     const int n = (int)mla.size();
     *count = n;
     if (blur_radius)

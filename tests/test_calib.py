@@ -7,7 +7,7 @@ element walk (tag names, "_" wrappers, the order of values, sscanf formats); the
 walk's rules: the rotation's "_" groups are columns, the last value under mia/mesh/pitch wins,
 keys are "row,col" read with %i (a leading 0 is octal), a blank before a comma ends a %f,%f
 read, lenses come back in (row, col) order
-with the first of a repeated key kept.  The table-driven neighbour builder must reproduce the
+with the last of a repeated key kept (the map assignment of DepthSolver.cpp:1242).  The table-driven neighbour builder must reproduce the
 layout-driven one (lfpm_layout_fixed_neighbors) on the same lenses."""
 import numpy as np
 import pytest
@@ -94,12 +94,12 @@ def test_calib_table(lfpm, tmp_path):
     p = tmp_path / "CalibInfo.xml"
     p.write_text(CALIB)
     c = lfpm.read_calib_xml(p)
-    # (row, col) order: (0,1), (0,7), (1,2) [first of the repeated key], (8,3) [octal 010]
+    # (row, col) order: (0,1), (0,7), (1,2) [last of the repeated key], (8,3) [octal 010]
     assert c["tilekeys"].tolist() == [[1, 0], [7, 0], [2, 1], [3, 8]]
     # "7 , 8": %f,%f stops at the blank before the comma, y keeps the initial 0
-    assert c["centers"].tolist() == [[7, 0], [30, 40], [100.5, 200.25], [5.5, 6.5]]
-    assert c["lens_type"].tolist() == [0, 2, 1, 0]
-    assert c["area"].tolist() == [3, 400, 415.5, 100]
+    assert c["centers"].tolist() == [[7, 0], [30, 40], [-1, -1], [5.5, 6.5]]
+    assert c["lens_type"].tolist() == [0, 2, 0, 0]
+    assert c["area"].tolist() == [3, 400, 0, 100]
     assert c["blur_radius"].tolist() == [2.5, 0.0, 4.25]
 
 
