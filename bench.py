@@ -207,6 +207,17 @@ def measured_traffic(row):
     return None
 
 
+def cpu_model():
+    """The host CPU's model name (first processor in /proc/cpuinfo), or None."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(lfpm):
     """FP32 CPU restatement (oracle, OpenMP) of the same solve on a bounded sample: the same
     parameters on an 800x640 sub-frame (1088 MIs, about 10 s on 16 cores), scaled to a 4080x3068
@@ -224,6 +235,7 @@ def cpu_baseline(lfpm):
     full = lfpm.workload_c2()
     s_per_frame = dt * full.V / wl.V
     return {"value": 1.0 / s_per_frame, "unit": "frames/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
             "sample": "FP32 oracle solve (init + 6 iterations, 18 neighbours, 15x15) of an 800x640 "
                       "sub-frame (%d MIs) in %.1f s, scaled by MI count to the %d-MI frame"
                       % (wl.V, dt, full.V)}
@@ -269,11 +281,15 @@ def plain_section(args, lfpm, torch, wl):
         achieved = flop_per_launch / (kt["phase"] * 1e-3) / 1e12
         out["work"] = work
         out["roofline"] = {
-            "bound": "mfma", "pipe": "FP32 VALU (compute-bound class; the NCC is not a "
-                                     "contraction, so no MFMA instructions are issued)",
+            "bound": "valu", "pipe": "FP32 VALU (compute-bound; the NCC is not a contraction, "
+                                     "so no MFMA instructions are issued; peak = MI355X FP32 "
+                                     "vector)",
             "kernel": "k_stage (red/black phase)", "achieved": achieved,
             "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / (PEAK_FP32 / 1e12),
-            "traffic": measured_traffic("phase"), "avg_launch_ms": kt["phase"],
+            "traffic": measured_traffic("phase"),
+            "traffic_source": "profiles/hbm_traffic_plain_4080x3068.md (rocprofv3 FETCH_SIZE / "
+                              "WRITE_SIZE, committed summary)",
+            "avg_launch_ms": kt["phase"],
             "flop_per_launch": flop_per_launch,
             "per_unit": "22 FLOP per evaluated window sample + 16 per evaluated "
                         "(pixel, view, hypothesis) triple"}
@@ -503,12 +519,15 @@ def products_section(args, lfpm, torch, wl):
             "records_equal_oracle": same,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
                          "frac": gbs / 8000.0, "traffic": point_cloud_traffic(),
+                         "traffic_source": "profiles/hbm_traffic_products_4080x3068.md "
+                                           "(committed rocprofv3 FETCH_SIZE / WRITE_SIZE "
+                                           "summary, not measured in this run)",
                          "kernel": "lfpm_vd_point_cloud (flag, scan, emit)",
                          "avg_launch_ms": ms["point_cloud"],
                          "per_unit": "5 B per raw pixel read (vd, colour) + 27 B per point "
                                      "written"},
             "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "frames/s", "cores": 1,
-                             "kind": "port",
+                             "kind": "port", "cpu": cpu_model(),
                              "sample": "oracle mosaic + VD map + point cloud of the same solved "
                                        "%dx%d frame, one frame in %.1f s" % (w, h, cpu_s)}}
 
@@ -590,6 +609,7 @@ def ingest_section(args, lfpm, torch, rw, rh):
                          "avg_launch_ms": ms,
                          "per_unit": "4 B per pixel (3 B of B, G, R read, 1 B gray written)"},
             "cpu_baseline": {"value": cpu_fps, "unit": "frames/s", "cores": 1, "kind": "port",
+                             "cpu": cpu_model(),
                              "sample": "oracle rule (numpy, int64) on one whole %dx%d frame, "
                                        "%d frames in %.0f s" % (rw, rh, reps, 10.0)}}
 
