@@ -568,6 +568,68 @@ int lfpm_set_work_counters(unsigned long long* counters_dev);
  * Replaces AdaptMIPMFrame::{Initialize, InitializeGPU, RunPatchMatchCUDAForFrame,
  * WriteBackResults, ReleaseMemory} (MIStereo/AdaptMIPMFrame.cpp:48-124, 278-384).
  * ------------------------------------------------------------------------------------- */
+/* ---------------------------------------------------------------------------------------
+ * Served form: a continuous-batching frame server.  Replaces the per-frame loop of
+ * DepthSolver::ProcessProblemsImpSequence (DepthSolver.cpp:2897-2901) under
+ * DepthSolverManager::DepthSolverPipelineBySequence / ByBatch (DepthSolverManager.cpp:41-135),
+ * which solves one frame at a time and resets the device after each (DepthSolver.cpp:368-374).
+ *
+ * lfpm_frame_submit copies the host raw frame (gray, raw_h rows of raw_pitch bytes; 0 = raw_w)
+ * and returns at once with a ticket; a server thread admits queued frames into device slots
+ * while their bytes fit hbm_budget_bytes (0 = half the free device memory) and max_slots
+ * (0 = 16), keeps each admitted frame's tiles, state and cost records resident, and advances all
+ * of them one stage per step (init, one red/black phase, final record + filter + refocus).
+ * Frames of one parameter class (equal lfpm_pm_params but seed / frame_uid) share one launch per
+ * step; classes run on their own streams (`streams`, 0 = 4) and overlap.  lfpm_frame_wait blocks
+ * until that frame is done and copies its results into the caller's buffers (NULL = not wanted):
+ * plane V*crop^2*4 floats, cost, disp *4, sel, reason (with a filter), refocus slices*raw_h*raw_w
+ * floats; with a filter plane/cost/disp/sel are the filtered copies (as lfpm_ctx_solve_frame
+ * returns them).  Results equal the same frame solved alone, bit for bit.  latency_ms = submit to
+ * completion (queueing included), queue_ms = submit to admission.  Plain variant, no blur term.
+ * A server is used from any number of host threads; lfpm_server_destroy finishes every
+ * submitted frame first.
+ * ------------------------------------------------------------------------------------- */
+typedef struct lfpm_server lfpm_server;
+
+typedef struct lfpm_serve_opts {
+    size_t hbm_budget_bytes;
+    int max_slots;
+    int streams;
+} lfpm_serve_opts;
+
+typedef struct lfpm_frame_request {
+    const lfpm_layout_params* layout;
+    const uint8_t* raw_host;
+    size_t raw_pitch;
+    const lfpm_pm_params* params;      /* crop and max_neighbors must match the layout */
+    const lfpm_filter_params* filter;  /* NULL: no CheckModule pass */
+    int refocus_slices;                /* 0: no refocus stack */
+    float refocus_vmin, refocus_vmax;
+} lfpm_frame_request;
+
+typedef struct lfpm_frame_result {
+    float* plane;
+    float* cost;
+    float* disp;
+    uint32_t* sel;
+    uint8_t* reason;
+    float* refocus;
+    int num_mis, crop;                 /* out */
+    double latency_ms, queue_ms;       /* out */
+} lfpm_frame_result;
+
+typedef struct lfpm_server_stats {
+    long long submitted, completed, steps, launches;
+    int max_active, classes;
+    size_t in_use_bytes, budget_bytes;
+} lfpm_server_stats;
+
+int lfpm_server_create(int device, const lfpm_serve_opts* opts, lfpm_server** out);
+void lfpm_server_destroy(lfpm_server* server);
+int lfpm_frame_submit(lfpm_server* server, const lfpm_frame_request* request, uint64_t* ticket);
+int lfpm_frame_wait(lfpm_server* server, uint64_t ticket, lfpm_frame_result* out);
+int lfpm_server_get_stats(lfpm_server* server, lfpm_server_stats* out);
+
 typedef struct lfpm_ctx lfpm_ctx;
 
 int lfpm_create(int device, lfpm_ctx** out);

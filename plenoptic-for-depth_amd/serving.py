@@ -1,14 +1,14 @@
-"""Continuous-batching depth server over the lfpm C ABI.
+"""Served form: Poisson request streams through the continuous-batching frame server of the C ABI
+(lfpm_server_create / lfpm_frame_submit / lfpm_frame_wait).
 
-Requests (one raw frame each) arrive on a wall clock.  A request is admitted into a device slot
-when its bytes fit the HBM budget and a slot is free; its raw frame is uploaded and its plane /
-cost / selection state stays device-resident until it completes.  Every server step advances all
-in-flight frames by one stage (init, one red/black phase, or final record + filter + refocus)
-through one `lfpm_batch_step` call, which groups frames of the same parameter class (same lens
-pitch / window / neighbours) into one persistent kernel launch — requests that arrive while
-others are on the GPU join at the next step.  A frame's results do not depend on what it is
-batched with.  Latency = completion time - arrival time (queueing included).
+The server (csrc/lfpm_serve.hip) admits submitted frames by HBM budget, keeps their state
+resident, advances every admitted frame one stage per step with one launch per parameter class
+(classes on their own streams, overlapping), and returns each frame's results on wait.  This
+module submits requests at their arrival times and collects the results; latency is measured by
+the server from submit to completion (queueing included).
 """
+import ctypes as C
+import threading
 import time
 
 import numpy as np
@@ -16,113 +16,151 @@ import numpy as np
 import lfpm
 
 
+class ServeOpts(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_slots", C.c_int), ("streams", C.c_int)]
+
+
+class FrameRequest(C.Structure):
+    _fields_ = [("layout", C.POINTER(lfpm.LayoutParams)), ("raw_host", C.c_void_p),
+                ("raw_pitch", C.c_size_t), ("params", C.POINTER(lfpm.PmParams)),
+                ("filter", C.POINTER(lfpm.FilterParams)), ("refocus_slices", C.c_int),
+                ("refocus_vmin", C.c_float), ("refocus_vmax", C.c_float)]
+
+
+class FrameResult(C.Structure):
+    _fields_ = [("plane", C.c_void_p), ("cost", C.c_void_p), ("disp", C.c_void_p),
+                ("sel", C.c_void_p), ("reason", C.c_void_p), ("refocus", C.c_void_p),
+                ("num_mis", C.c_int), ("crop", C.c_int), ("latency_ms", C.c_double),
+                ("queue_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong),
+                ("steps", C.c_longlong), ("launches", C.c_longlong), ("max_active", C.c_int),
+                ("classes", C.c_int), ("in_use_bytes", C.c_size_t),
+                ("budget_bytes", C.c_size_t)]
+
+
 class Request:
-    def __init__(self, rid, arrival_s, wl):
+    """One frame to serve: a workload (layout, raw frame, solve / filter / refocus settings)
+    and its arrival time in seconds from the start of the stream."""
+
+    def __init__(self, rid, arrival_s, wl, filt=True, refocus=True):
         self.rid = rid
         self.arrival = arrival_s
         self.wl = wl
-        self.slot = None
-        self.schedule = None
-        self.pos = 0
-        self.done_s = None
-        self.admit_s = None
-
-
-def schedule_for(pm):
-    q = [(lfpm.STAGE_INIT, 0, 0)]
-    for it in range(pm.iterations):
-        q += [(lfpm.STAGE_PHASE, it, 0), (lfpm.STAGE_PHASE, it, 1)]
-    q.append((lfpm.STAGE_FINAL, 0, 0))
-    return q
-
-
-def slot_bytes(wl, slices):
-    """Device bytes one frame occupies while in flight (inputs, state, products)."""
-    n = wl.V * wl.crop ** 2
-    h, w = wl.sp.raw_h, wl.sp.raw_w
-    return (h * w + n * (4 + 4 + 16 + 4 + 4 + 16 + 16 + 4 + 4 + 16 + 4 + 16 + 4 + 1)
-            + slices * h * w * 4 + wl.L["V"] * (8 + 8 + 4 * wl.pm.max_neighbors + 4))
+        self.filt = filt
+        self.refocus = refocus
+        self.ticket = None
+        self.result = None
+        self.latency_ms = None
+        self.queue_ms = None
 
 
 class Server:
-    def __init__(self, hbm_budget_bytes, max_slots=16, device="cuda"):
-        self.budget = int(hbm_budget_bytes)
-        self.max_slots = max_slots
-        self.device = device
-        self.in_use = 0
-        self.active = []
-        self.queue = []
-        self.done = []
-        self.steps = 0
-        self.batch_sizes = []
+    """A frame server on one device (lfpm_server): submit() returns at once, wait() blocks for
+    that request's results."""
 
-    def _admit(self, now):
-        while self.queue and len(self.active) < self.max_slots:
-            r = self.queue[0]
-            need = slot_bytes(r.wl, r.wl.slices)
-            if self.in_use + need > self.budget:
-                if not self.active:
-                    raise lfpm.LfpmError(-3, "request %d needs %d B > budget %d B"
-                                         % (r.rid, need, self.budget))
-                break
-            self.queue.pop(0)
-            r.slot = lfpm.FrameSlot(r.wl, device=self.device)
-            r.schedule = schedule_for(r.wl.pm)
-            r.pos = 0
-            r.admit_s = now
-            r.bytes = need
-            self.in_use += need
-            self.active.append(r)
+    def __init__(self, hbm_budget_bytes=0, max_slots=16, streams=4, device=0, keep_results=True):
+        L = lfpm.lib()
+        self._L = L
+        self.keep_results = keep_results
+        h = C.c_void_p()
+        opts = ServeOpts(int(hbm_budget_bytes), int(max_slots), int(streams))
+        lfpm._check(L.lfpm_server_create(int(device), C.byref(opts), C.byref(h)))
+        self._h = h
+        self.done = []
+
+    def close(self):
+        if self._h:
+            self._L.lfpm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, r):
+        wl = r.wl
+        raw = np.ascontiguousarray(wl.raw())
+        r._raw = raw
+        r._req = FrameRequest(C.pointer(wl.lp), raw.ctypes.data_as(C.c_void_p), raw.shape[1],
+                              C.pointer(wl.pm), C.pointer(wl.fp) if r.filt else None,
+                              wl.slices if r.refocus else 0, wl.vmin, wl.vmax)
+        t = C.c_uint64()
+        lfpm._check(self._L.lfpm_frame_submit(self._h, C.byref(r._req), C.byref(t)))
+        r.ticket = t.value
+        return r.ticket
+
+    def wait(self, r):
+        wl = r.wl
+        out = FrameResult()
+        res = None
+        if self.keep_results:
+            n = wl.V * wl.crop ** 2
+            h, w = wl.raw().shape
+            res = dict(plane=np.zeros((n, 4), np.float32), cost=np.zeros(n, np.float32),
+                       disp=np.zeros((n, 4), np.float32), sel=np.zeros(n, np.uint32))
+            if r.filt:
+                res["reason"] = np.zeros(n, np.uint8)
+            if r.refocus and wl.slices > 0:
+                res["refocus"] = np.zeros((wl.slices, h, w), np.float32)
+            for k, a in res.items():
+                setattr(out, k, a.ctypes.data_as(C.c_void_p))
+        lfpm._check(self._L.lfpm_frame_wait(self._h, C.c_uint64(r.ticket), C.byref(out)))
+        r.result = res
+        r.latency_ms = out.latency_ms
+        r.queue_ms = out.queue_ms
+        return r
+
+    def stats(self):
+        st = ServerStats()
+        lfpm._check(self._L.lfpm_server_get_stats(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
 
     def run(self, requests, clock=time.perf_counter, log=None):
-        """Serve `requests` (sorted by arrival, seconds from start); returns the done list."""
-        import torch
-        pending = sorted(requests, key=lambda r: r.arrival)
+        """Submit `requests` at their arrival times (seconds from now) while a waiter thread
+        collects them in submit order; returns the requests, each with its result and the
+        server's latency."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        submitted = []
+        cv = threading.Condition()
+        errors = []
+
+        def waiter():
+            i = 0
+            while i < len(reqs):
+                with cv:
+                    cv.wait_for(lambda: len(submitted) > i)
+                    r = submitted[i]
+                try:
+                    self.wait(r)
+                except Exception as e:  # noqa: BLE001 - reported after the run
+                    errors.append(e)
+                if log:
+                    log("serve: request %d done (%dx%d pitch %g) latency %.1f ms"
+                        % (r.rid, r.wl.sp.raw_w, r.wl.sp.raw_h, r.wl.sp.pitch,
+                           r.latency_ms or -1.0))
+                i += 1
+
+        th = threading.Thread(target=waiter)
+        th.start()
         t0 = clock()
-        batch = None
-        batch_key = None
-        while pending or self.queue or self.active:
-            now = clock() - t0
-            while pending and pending[0].arrival <= now:
-                self.queue.append(pending.pop(0))
-            self._admit(now)
-            if not self.active:
-                wait = pending[0].arrival - (clock() - t0)
-                if wait > 0:
-                    time.sleep(wait)
-                continue
-            key = tuple(id(r) for r in self.active)
-            if key != batch_key:
-                batch = lfpm.Batch([r.slot for r in self.active], device=self.device)
-                batch_key = key
-            st, it, ph = [], [], []
-            for r in self.active:
-                s = r.schedule[r.pos]
-                st.append(s[0]); it.append(s[1]); ph.append(s[2])
-            batch.step(st, it, ph)
-            torch.cuda.current_stream().synchronize()
-            self.steps += 1
-            self.batch_sizes.append(len(self.active))
-            now = clock() - t0
-            still = []
-            for r in self.active:
-                r.pos += 1
-                if r.pos == len(r.schedule):
-                    r.done_s = now
-                    self.in_use -= r.bytes
-                    self.done.append(r)
-                    if log:
-                        log("serve: request %d done (%dx%d pitch %g) latency %.1f ms"
-                            % (r.rid, r.wl.sp.raw_w, r.wl.sp.raw_h, r.wl.sp.pitch,
-                               1e3 * (r.done_s - r.arrival)))
-                else:
-                    still.append(r)
-            if len(still) != len(self.active):
-                for r in self.active:
-                    if r.pos == len(r.schedule):
-                        r.slot = None  # release device buffers
-                self.active = still
-        return self.done
+        for r in reqs:
+            wait = r.arrival - (clock() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            self.submit(r)
+            with cv:
+                submitted.append(r)
+                cv.notify()
+        th.join()
+        if errors:
+            raise errors[0]
+        self.done = reqs
+        return reqs
 
 
 def poisson_requests(n, rate_hz, seed, sizes, pitches, make_workload):
@@ -140,6 +178,6 @@ def poisson_requests(n, rate_hz, seed, sizes, pitches, make_workload):
 
 
 def latency_stats(done):
-    lat = np.array([1e3 * (r.done_s - r.arrival) for r in done])
+    lat = np.array([r.latency_ms for r in done])
     return dict(p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                 mean_ms=float(lat.mean()), max_ms=float(lat.max()))
