@@ -323,26 +323,45 @@ def batched_section(args, lfpm, torch, wls):
                 lat=lat, frames=len(slots), result=res, mis=sum(w.V for w in wls))
 
 
+def plain_frames_section(args, lfpm, torch, wls):
+    """The same frames solved one at a time by the plain path (launch per phase, sync after
+    each, then filter and refocus), all frames' buffers resident: frames/s over args.steps
+    passes."""
+    frames = []
+    for wl in wls:
+        rw, rh = wl.sp.raw_w, wl.sp.raw_h
+        df = lfpm.DeviceFrame(wl)
+        bufs = lfpm.SolveBuffers(wl.V, wl.crop, wl.pm)
+        n = wl.V * wl.crop ** 2
+        fo = dict(plane=torch.empty((n, 4), device="cuda"), cost=torch.empty(n, device="cuda"),
+                  disp=torch.empty((n, 4), device="cuda"),
+                  sel=torch.empty(n, dtype=torch.int32, device="cuda"),
+                  reason=torch.empty(n, dtype=torch.uint8, device="cuda"))
+        rf = torch.empty((wl.slices, rh, rw), device="cuda")
+        frames.append((df, bufs, fo, rf))
+    for i in range(args.warmup):
+        for f in frames:
+            run_step(lfpm, *f, Timer(torch), torch)
+        log("bench: plain-frames warmup %d done" % i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        for f in frames:
+            run_step(lfpm, *f, Timer(torch), torch)
+        torch.cuda.synchronize()
+        log("bench: plain-frames pass %d done" % i)
+    t = time.perf_counter() - t0
+    del frames
+    return dict(fps=len(wls) * args.steps / t, ms_per_pass=1e3 * t / args.steps)
+
+
 def same_results(a, b, torch):
     return all(torch.equal(a[k], b[k]) for k in a)
 
 
 def mixed_workloads(lfpm, n, seed=11):
-    """The uneven batch: raw size from {1024x800, 2048x1536, 4080x3068} and lens pitch from
-    {15, 23, 31} per frame, multi-focus 3-type hex grid with 6/12/18 neighbours by lens type,
-    6-12 seeded planes with virtual depth in [2, 8], 6 iterations."""
-    rng = np.random.default_rng(seed)
-    sizes = [(1024, 800), (2048, 1536), (4080, 3068)]
-    radius = {15.0: 5, 23.0: 7, 31.0: 9}
-    wls = []
-    for i in range(n):
-        w, h = sizes[int(rng.integers(3))]
-        p = [15.0, 23.0, 31.0][int(rng.integers(3))]
-        wls.append(lfpm.workload_c2(raw_w=w, raw_h=h, pitch=p, radius=radius[p],
-                                    planes=int(rng.integers(6, 13)), lens_types=3,
-                                    per_type=(6, 12, 18), scene_seed=1000 + i,
-                                    noise_seed=2000 + i, seed=1000 + i, frame_uid=i))
-    return wls
+    """The uneven batch of lfpm.mixed_workloads."""
+    return lfpm.mixed_workloads(n, seed)
 
 
 def serve_section(args, lfpm, torch):
@@ -724,6 +743,10 @@ def main():
                                "launches over all frames' MIs) + per-frame CheckModule filter + "
                                "%d-slice refocus" % (args.batch, rw, rh, wl.slices))
         bat = batched_section(args, lfpm, torch, wls)
+        if args.mode == "mixed":
+            pf = plain_frames_section(args, lfpm, torch, wls)
+            line.update(plain_same_frames_fps=pf["fps"],
+                        speedup_vs_plain_same_frames=bat["fps"] / pf["fps"])
         line.update(value=bat["fps"], ms_per_step=bat["ms_per_step"],
                     p50_ms=float(np.percentile(bat["lat"], 50)),
                     p99_ms=float(np.percentile(bat["lat"], 99)),

@@ -246,6 +246,24 @@ def workload_c2(**kw):
     return Workload(**args)
 
 
+def mixed_workloads(n, seed=11):
+    """The uneven batch: raw size from {1024x800, 2048x1536, 4080x3068} and lens pitch from
+    {15, 23, 31} per frame, multi-focus 3-type hex grid with 6/12/18 neighbours by lens type,
+    6-12 seeded planes with virtual depth in [2, 8], 6 iterations."""
+    rng = np.random.default_rng(seed)
+    sizes = [(1024, 800), (2048, 1536), (4080, 3068)]
+    radius = {15.0: 5, 23.0: 7, 31.0: 9}
+    wls = []
+    for i in range(n):
+        w, h = sizes[int(rng.integers(3))]
+        p = [15.0, 23.0, 31.0][int(rng.integers(3))]
+        wls.append(workload_c2(raw_w=w, raw_h=h, pitch=p, radius=radius[p],
+                               planes=int(rng.integers(6, 13)), lens_types=3,
+                               per_type=(6, 12, 18), scene_seed=1000 + i,
+                               noise_seed=2000 + i, seed=1000 + i, frame_uid=i))
+    return wls
+
+
 # --------------------------------------------------------------------------------------
 # Device entry points (torch tensors on the current device)
 # --------------------------------------------------------------------------------------
