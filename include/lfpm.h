@@ -139,6 +139,20 @@ int lfpm_table_fixed_neighbors(const int* tilekeys, const float* centers, int nu
                                int lattice_cols, int lattice_rows, int max_neighbors,
                                int* neighbor_ids, int* neighbor_counts, int* garbage);
 
+/* Matching neighbour table of a read lens table (SelectNeighbors::CollectMIANeighImagesForMatch
+ * -> ComputeNeighborScoresForMatch -> MLA_Problem::SortNeighScoreForMatch,
+ * MINeighbor/SelectNeighbors.cpp:113-193,770-830, MVStereo/LFDepthInfo.cpp:214-241): the keys
+ * on hex rings 1..rings around each lens (cube distance in the odd-r offset lattice, odd key rows
+ * half a pitch to the right), ordered by the reference's score 1 - normalised baseline (weights
+ * 1, 0, 0), i.e. by the centre distance (float, as the reference computes it); equal distances by
+ * angle in [0, 2 pi), then collection order; capped at max_neighbors (-1 padded).  rings in
+ * 1..5 (2 = ring-1 + ring-2, 18 neighbours in the interior).  The reference also drops
+ * candidates by a RANSAC test on image-similarity scores (Hu moments,
+ * MLA_Problem::OutlierImpByRANSAC); this table keeps every lens of the rings. */
+int lfpm_table_ring_neighbors(const int* tilekeys, const float* centers, int num_mis,
+                              int lattice_cols, int lattice_rows, int rings, int max_neighbors,
+                              int* neighbor_ids, int* neighbor_counts);
+
 /* Crop origins of a read lens table (DepthSolver::CreateProblemsAndDisNormals_Frame,
  * DepthSolver.cpp:1894-1905, and the slicing rectangle of :1977): corner = centre - (crop-1)/2
  * in FP32; kept[v] = 0 (origin -1, -1) when the crop leaves mla_u_size*baseline x

@@ -499,3 +499,34 @@ def refine_candidates(pm, x, y, best, bcost, words, L=None):
                                 _p(np.ascontiguousarray(best, np.float64)), C.c_double(bcost),
                                 _p(np.ascontiguousarray(words, np.uint32)), _p(out))
     return out
+
+
+def ring_neighbors(tilekeys, centers, rings, K):
+    """Matching neighbour table of a lens table by brute force over all lens pairs: the lenses at
+    hex cube distance 1..rings of each lens (keys (col, row), odd rows half a pitch right),
+    ordered by the float centre distance sqrt(double dx^2 + double dy^2), then angle in [0, 2 pi);
+    capped at K, -1 padded (SelectNeighbors::CollectMIANeighImagesForMatch with the score of
+    ComputeNeighborScoresForMatch, MINeighbor/SelectNeighbors.cpp:113-193,770-830, and
+    MLA_Problem::SortNeighScoreForMatch, MVStereo/LFDepthInfo.cpp:214-241)."""
+    keys = np.asarray(tilekeys, np.int64)
+    c = np.asarray(centers, np.float32)
+    V = keys.shape[0]
+    X, Y = keys[:, 0], keys[:, 1]
+    q = X - (Y - (Y & 1)) // 2
+    nbr = np.full((V, K), -1, np.int32)
+    cnt = np.zeros(V, np.int32)
+    for v in range(V):
+        dq = q - q[v]
+        dr = Y - Y[v]
+        d = np.maximum(np.maximum(np.abs(dq), np.abs(dr)), np.abs(dq + dr))
+        u = np.nonzero((d >= 1) & (d <= rings))[0]
+        dx = (c[u, 0] - c[v, 0]).astype(np.float32).astype(np.float64)
+        dy = (c[u, 1] - c[v, 1]).astype(np.float32).astype(np.float64)
+        b = np.sqrt(dx * dx + dy * dy).astype(np.float32)
+        a = np.arctan2(dy, dx)
+        a = np.where(a < 0, a + 2 * np.pi, a)
+        order = np.lexsort((a, b))
+        sel = u[order][:K]
+        nbr[v, :len(sel)] = sel
+        cnt[v] = len(sel)
+    return nbr, cnt

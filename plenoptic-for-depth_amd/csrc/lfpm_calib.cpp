@@ -3,6 +3,7 @@
 // 1112-1251) over a small XML reader of this file, and the FixedPosition neighbour table of a
 // micro-lens table read that way.  Numbers are parsed with the reference's sscanf formats.
 #include <algorithm>
+#include <cmath>
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
@@ -340,6 +341,65 @@ int lfpm_table_fixed_neighbors(const int* tilekeys, const float* centers, int nu
     return lfpm::fixed_neighbors_core(num_mis, mi_i.data(), mi_j.data(), cx.data(), cy.data(),
                                       cell.data(), lattice_cols, lattice_rows, max_neighbors,
                                       neighbor_ids, neighbor_counts, garbage);
+}
+
+int lfpm_table_ring_neighbors(const int* tilekeys, const float* centers, int num_mis,
+                              int lattice_cols, int lattice_rows, int rings, int max_neighbors,
+                              int* neighbor_ids, int* neighbor_counts) {
+    if (!tilekeys || !centers || num_mis < 0 || lattice_cols <= 0 || lattice_rows <= 0 ||
+        rings < 1 || rings > 5 || max_neighbors < 1 || max_neighbors > LFPM_MAX_NEIGHBORS)
+        return lfpm::set_error(LFPM_E_INVALID, "ring neighbours: bad arguments");
+    std::vector<int> cell((size_t)lattice_cols * lattice_rows, -1);
+    for (int v = 0; v < num_mis; ++v) {
+        const int X = tilekeys[2 * v], Y = tilekeys[2 * v + 1];
+        if (X < 0 || Y < 0 || X >= lattice_cols || Y >= lattice_rows)
+            return lfpm::set_error(LFPM_E_INVALID, "ring neighbours: key (%d, %d) outside the "
+                                   "%d x %d lattice", X, Y, lattice_cols, lattice_rows);
+        int& c = cell[(size_t)Y * lattice_cols + X];
+        if (c >= 0)
+            return lfpm::set_error(LFPM_E_INVALID, "ring neighbours: key (%d, %d) repeated", X, Y);
+        c = v;
+    }
+    // ring walk (SelectNeighbors::CollectMIANeighImagesForMatch, MINeighbor/SelectNeighbors.cpp:
+    // 113-193, circles 1..rings of the hex lattice): keys at cube distance 1..rings in the
+    // odd-r offset lattice (odd key rows sit half a pitch to the right); the score of
+    // ComputeNeighborScoresForMatch (:770-830, weights a = 1, b = c = 0) is 1 - the normalised
+    // baseline, so SortNeighScoreForMatch (MVStereo/LFDepthInfo.cpp:214-241) orders them by
+    // baseline; equal baselines by angle in [0, 2 pi), then collection order.
+    struct Cand { int u; float base; double ang; int order; };
+    std::vector<Cand> cand;
+    for (int v = 0; v < num_mis; ++v) {
+        const int X = tilekeys[2 * v], Y = tilekeys[2 * v + 1];
+        const int q = X - (Y - (Y & 1)) / 2;
+        cand.clear();
+        for (int dr = -rings; dr <= rings; ++dr)
+            for (int dq = -rings; dq <= rings; ++dq) {
+                const int ring = std::max(std::abs(dq), std::max(std::abs(dr), std::abs(dq + dr)));
+                if (ring == 0 || ring > rings) continue;
+                const int r2 = Y + dr, q2 = q + dq;
+                const int X2 = q2 + (r2 - (r2 & 1)) / 2;
+                if (X2 < 0 || r2 < 0 || X2 >= lattice_cols || r2 >= lattice_rows) continue;
+                const int u = cell[(size_t)r2 * lattice_cols + X2];
+                if (u < 0) continue;
+                const float dx = centers[2 * u] - centers[2 * v];
+                const float dy = centers[2 * u + 1] - centers[2 * v + 1];
+                // the reference's float baseline sqrt(pow(dx, 2) + pow(dy, 2)) (pow in double)
+                const float b = (float)std::sqrt((double)dx * dx + (double)dy * dy);
+                double a = std::atan2((double)dy, (double)dx);
+                if (a < 0) a += 2.0 * M_PI;
+                cand.push_back(Cand{u, b, a, (int)cand.size()});
+            }
+        std::sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) {
+            if (a.base != b.base) return a.base < b.base;
+            if (a.ang != b.ang) return a.ang < b.ang;
+            return a.order < b.order;
+        });
+        const int n = std::min((int)cand.size(), max_neighbors);
+        for (int k = 0; k < max_neighbors; ++k)
+            if (neighbor_ids) neighbor_ids[(size_t)v * max_neighbors + k] = k < n ? cand[k].u : -1;
+        if (neighbor_counts) neighbor_counts[v] = n;
+    }
+    return LFPM_OK;
 }
 
 int lfpm_table_origins(const float* centers, int num_mis, int crop, int mla_u_size,

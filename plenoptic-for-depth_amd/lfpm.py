@@ -424,6 +424,19 @@ def table_origins(centers, crop, mla_u_size, mla_v_size, baseline, raw_w, raw_h)
     return o, k
 
 
+def table_ring_neighbors(tilekeys, centers, cols, rows, rings, max_neighbors):
+    """Matching neighbour table of a read lens table (lfpm_table_ring_neighbors): the lenses on
+    hex rings 1..rings ordered by centre distance, then angle: (nbr (V, K), cnt (V,))."""
+    keys = np.ascontiguousarray(tilekeys, np.int32)
+    cen = np.ascontiguousarray(centers, np.float32)
+    V = keys.shape[0]
+    nbr = np.zeros((V, max_neighbors), np.int32)
+    cnt = np.zeros(V, np.int32)
+    _check(lib().lfpm_table_ring_neighbors(_np(keys), _np(cen), V, int(cols), int(rows),
+                                           int(rings), int(max_neighbors), _np(nbr), _np(cnt)))
+    return nbr, cnt
+
+
 def table_fixed_neighbors(tilekeys, centers, cols, rows, max_neighbors):
     """lfpm_table_fixed_neighbors: FixedPosition tables of a lens table -> (nbr, cnt, garbage)."""
     k = np.ascontiguousarray(tilekeys, np.int32)

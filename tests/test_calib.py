@@ -187,3 +187,70 @@ def test_calib_file_drives_the_solver(lfpm, tmp_path):
         outs.append((b.plane.cpu().numpy(), b.cost.cpu().numpy()))
     assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
     assert np.array_equal(outs[0][1].view(np.uint32), outs[1][1].view(np.uint32))
+
+
+@pytest.mark.parametrize("pitch,w,h,K,rings", [(23.0, 1024, 800, 18, 2), (15.0, 640, 480, 6, 1),
+                                               (31.0, 900, 700, 24, 3)])
+def test_table_ring_neighbours_equal_brute_force(lfpm, orc, pitch, w, h, K, rings):
+    """lfpm_table_ring_neighbors (ring walk over a key -> lens map) equals the oracle's brute
+    force over all lens pairs, on the layout's centres and on calibrated-like centres (jittered by
+    up to 0.5 px, lenses in shuffled order); interior lenses get the 6 / 18 / 36 lenses of rings
+    1 / 2 / 3 (capped at K)."""
+    L = lfpm.layout(lfpm.layout_params(pitch, w, h, 18))
+    keys, cen = L["tilekeys"], L["centers"]
+    rng = np.random.default_rng(int(pitch))
+    for jitter in (False, True):
+        c = cen.copy()
+        k = keys.copy()
+        if jitter:
+            c = (c + rng.uniform(-0.5, 0.5, c.shape)).astype(np.float32)
+            perm = rng.permutation(len(k))
+            k, c = k[perm], c[perm]
+        nb, ct = lfpm.table_ring_neighbors(k, c, L["cols"], L["rows"], rings, K)
+        onb, oct_ = orc.ring_neighbors(k, c, rings, K)
+        assert np.array_equal(nb, onb) and np.array_equal(ct, oct_)
+        ring_size = 3 * rings * (rings + 1)
+        assert ct.max() == min(K, ring_size)
+        # nearest first: distances never decrease along a row
+        V = len(k)
+        for v in range(0, V, max(1, V // 50)):
+            d = np.hypot(*(c[nb[v, :ct[v]]] - c[v]).T)
+            assert np.all(np.diff(d) >= -1e-4)
+
+
+@pytest.mark.gpu
+def test_calib_file_ring_table_solve_vs_fp64(lfpm, orc, tmp_path):
+    """A frame driven by its CalibInfo.xml (keys and centres read back, crop origins and the
+    ring-walk matching table of rings 1-2 built from the read table) solved on the GPU: within
+    0.05 px of the FP64 oracle solving the same tables on >= 99.5 % of valid pixels."""
+    import torch
+    from _helpers import pm_to_oracle
+    wl = lfpm.workload_c2(raw_w=640, raw_h=512)
+    L = wl.L
+    print("calib ring chain V", wl.V, flush=True)
+    lines = ["<?xml version=\"1.0\"?>", "<Calib>", "<MLAInfo>"]
+    for (X, Y), (cx, cy) in zip(L["tilekeys"], L["centers"]):
+        lines.append("<MLA><key>%d,%d</key><center>%.9g,%.9g</center><MLAType>0</MLAType>"
+                     "<aera>1</aera></MLA>" % (Y, X, cx, cy))
+    lines += ["</MLAInfo>", "</Calib>"]
+    path = tmp_path / "CalibInfo.xml"
+    path.write_text("\n".join(lines))
+    c = lfpm.read_calib_xml(path)
+    h, w = wl.raw().shape
+    o, k = lfpm.table_origins(c["centers"], L["crop"], L["cols"], L["rows"], 1e6, w, h)
+    assert k.all()
+    K = wl.pm.max_neighbors
+    nbr, cnt = lfpm.table_ring_neighbors(c["tilekeys"], c["centers"], L["cols"], L["rows"], 2, K)
+    assert cnt.max() == 18
+    df = lfpm.DeviceFrame(wl)
+    df.nbr = torch.from_numpy(nbr).cuda()
+    df.cnt = torch.from_numpy(cnt).cuda()
+    b = lfpm.patchmatch_frame(df)
+    torch.cuda.synchronize()
+    d = b.plane.cpu().numpy()[:, 3].astype(np.float64)
+    tiles = orc.slice_u8(wl.raw(), o, L["crop"])
+    ores = orc.solve(tiles, c["centers"], nbr, cnt, pm_to_oracle(orc, wl.pm))
+    valid = ores["cost"] <= 1.8
+    frac = float((np.abs(d - ores["plane"][:, 3])[valid] <= 0.05).mean())
+    print("calib ring solve frac", frac, flush=True)
+    assert frac >= 0.995
