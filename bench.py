@@ -284,7 +284,7 @@ def plain_section(args, lfpm, torch, wl):
             "bound": "valu", "pipe": "FP32 VALU (compute-bound; the NCC is not a contraction, "
                                      "so no MFMA instructions are issued; peak = MI355X FP32 "
                                      "vector)",
-            "kernel": "k_stage (red/black phase)", "achieved": achieved,
+            "kernel": "k_stage_w (red/black phase)", "achieved": achieved,
             "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / (PEAK_FP32 / 1e12),
             "traffic": measured_traffic("phase"),
             "traffic_source": "profiles/hbm_traffic_plain_4080x3068.md (rocprofv3 FETCH_SIZE / "
@@ -731,7 +731,7 @@ def main():
             log("bench: synthesising %d mixed frames" % args.batch)
             wls = mixed_workloads(lfpm, args.batch)
             cfg.update(raw="mixed {1024x800, 2048x1536, 4080x3068}", pitch="mixed {15, 23, 31}",
-                       neighbors="6/12/18 by lens type")
+                       neighbors="6/12/18 by lens type", mis=sum(w.V for w in wls))
             cfg["workload"] = ("%d uneven frames in one captured hipGraph: solve + filter + "
                                "refocus per frame" % args.batch)
         else:
@@ -739,7 +739,7 @@ def main():
             wls = [wl] + [lfpm.workload_c2(raw_w=rw, raw_h=rh, scene_seed=4321 + 17 * i,
                                            noise_seed=4322 + 17 * i, frame_uid=i)
                           for i in range(1, args.batch)]
-            cfg["workload"] = ("%d frames of %dx%d in one captured hipGraph (persistent k_stage "
+            cfg["workload"] = ("%d frames of %dx%d in one captured hipGraph (persistent k_stage_w "
                                "launches over all frames' MIs) + per-frame CheckModule filter + "
                                "%d-slice refocus" % (args.batch, rw, rh, wl.slices))
         bat = batched_section(args, lfpm, torch, wls)

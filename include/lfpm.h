@@ -487,6 +487,87 @@ int lfpm_vd_point_cloud_bgr(const float* vd_dev, const uint8_t* color_bgr_dev, i
                             size_t records_bytes, unsigned* count_dev, void* workspace_dev,
                             size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Refocus-side composites of a solved frame (device pointers, asynchronous on `stream`).
+ *
+ * lfpm_aif_composite replaces LFRefocus::AIFImageCompositeForMIA (FocalStack/LFRefocus.cpp:
+ * 183-425) with FilterMatchingPoints (:2283-2301), GenerateVirtualImage (:2303-2328) and
+ * HoleFilling (:2362-2498).  For every MI v not flagged in garbage_dev (V bytes, may be NULL)
+ * and every crop pixel (x, y) with d = plane.w > 0: vd = baseline / d (FP32; the reference passes
+ * the MI-image baseline here), (X, Y) = round(((x, y) - (crop - 1) / 2) * vd + centre_v) with
+ * round half away from zero; positions inside the raw frame are candidates of that virtual
+ * pixel.  Each virtual pixel keeps its first candidate in collection order (MI, row, column)
+ * among those with the highest score rich / 255 * 0.9 + blur / 255 * 0.1 (integer quotients:
+ * 1 only for a saturated 255), the score maps read at row = source column, column = source row
+ * as the reference indexes them.  richness_tiles_dev: the binarised texture-richness crops
+ * (V*crop^2 bytes, 255 = rich; NULL = all 0); blur_tiles_dev: the blur-score crops (NULL = all
+ * 0).  The reference's std::sort leaves the order of equal scores unspecified; below 17
+ * candidates its implementation keeps collection order, as this one always does.  Outputs per
+ * raw pixel (0 where no candidate): vd_dev (vd), inv_vd_dev (1 / vd), real_mm_dev (f * t / (t -
+ * f) with t = vd * B + bl0, f = main_lens_f; metres = this / 1000), aif_dev (raw_h*raw_w*3 bytes
+ * B, G, R of the kept source pixel), filled_dev (the composite after HoleFilling: raster order,
+ * a hole takes the source pixel next to its right, left, lower or upper neighbour's source, in
+ * that order of preference, the right and lower neighbours as they were when the pass started;
+ * a filled pixel's source is attributed to the MI of the last candidate collected, as the
+ * reference's member m_vIPoint does; passes until >= 98 % of pixels are counted, a pass fills
+ * nothing or more than 30000 fills were made), denoised_dev (five 5x5 median passes with
+ * replicated borders of aif_dev, each channel on its own).  stats_dev (8 ints): collected
+ * candidates, kept pixels, hole-filling passes, fills, most candidates at one pixel, the
+ * attribution MI (-1: none), pixels with more than 16 candidates, pixels with every channel >= 1
+ * before filling.  Limits: crop <= 127, raw_w <= 8192, V*crop^2 < 2^30.
+ * Display renderings (MAD clamp, normalisation, gamma, colour maps) are not produced.
+ * ------------------------------------------------------------------------------------- */
+size_t lfpm_aif_workspace_size(int num_mis, int crop, int raw_w, int raw_h);
+
+int lfpm_aif_composite(const float* plane_dev, const float* centers_dev,
+                       const uint8_t* bgr_tiles_dev, const uint8_t* richness_tiles_dev,
+                       const uint8_t* blur_tiles_dev, const uint8_t* garbage_dev, int num_mis,
+                       int crop, float baseline, float B, float bl0, float main_lens_f,
+                       int raw_w, int raw_h, float* vd_dev, float* inv_vd_dev,
+                       float* real_mm_dev, uint8_t* aif_dev, uint8_t* filled_dev,
+                       uint8_t* denoised_dev, int* stats_dev, void* workspace_dev,
+                       size_t workspace_bytes, void* stream);
+
+/* lfpm_fuse_backproject replaces LFRefocus::FuseVirtualDepth_BackProject (FocalStack/
+ * LFRefocus.cpp:1553-1793) with ComputeVirtualDepth, ComputeVirtual3D and BackProject2MI
+ * (:2086-2122).  The reference walks the MIs in order and each MI's pixels in raster order; a
+ * pixel p of MI a not masked, with d >= 1, has the virtual point P = ((c, r) - ((crop - 1) / 2 +
+ * 1)) * vd + centre_a, vd = mla_base / d (FP64 quotient, FP32 result), mla_base = baseline * bl0
+ * / (bl0 + B).  For each matching neighbour b (nbr_dev / cnt_dev, the solve's table, -1 and
+ * garbage entries skipped) P is projected into b, rounded by +0.5 and truncation; an unmasked
+ * pixel there with d > 0 whose own virtual point projects back into a within 1 px, with virtual
+ * depths within 5 % of vd, is consistent.  With >= 2 consistent neighbours the point, normal and
+ * colour are averaged over the pixel and them (a product with the float reciprocal of count + 1,
+ * as cv::Vec3f /= float is); the point splats at cvRound(x, y) (ties to even) into vd_dev (z),
+ * aif_dev (B, G, R, rounded and saturated) and conf_dev (min(255, (count + 1) * 60)), later
+ * points overwriting earlier ones, and the neighbour pixels used are masked.  Outputs are zero
+ * where nothing splats.  points_dev (optional, points_bytes >= V*crop^2*36): the fused points
+ * inside the frame in walk order, 9 floats {x, y, z, nx, ny, nz, b, g, r} each.  counts_dev (2
+ * uint64): points inside the frame, fused points.  The walk runs as a schedule from
+ * lfpm_fuse_schedule (order_dev on the device, level_start on the host): MIs of one level touch
+ * no common mask, so they run together, and every MI whose walk touches a mask another one
+ * touches runs in a later level than it when it comes later in MI order — the outcome equals
+ * the sequential walk.  Limits: max_neighbors <= 31, crop <= 127.  A garbage neighbour is
+ * skipped (the reference indexes an empty mask for it).  Display renderings are not produced. */
+size_t lfpm_fuse_workspace_size(int num_mis, int crop, int raw_w, int raw_h);
+
+/* Host: the level schedule of a neighbour table (nbr: V*max_neighbors, cnt: V, garbage: V bytes
+ * or NULL).  order: V ints (the non-garbage MIs grouped by level, ascending within a level);
+ * level_start: V + 1 ints, level l = order[level_start[l] .. level_start[l + 1]); num_levels. */
+int lfpm_fuse_schedule(const int32_t* nbr, const int32_t* cnt, const uint8_t* garbage,
+                       int num_mis, int max_neighbors, int32_t* order, int32_t* level_start,
+                       int* num_levels);
+
+int lfpm_fuse_backproject(const float* plane_dev, const float* centers_dev,
+                          const uint8_t* bgr_tiles_dev, const int32_t* nbr_dev,
+                          const int32_t* cnt_dev, int max_neighbors, const uint8_t* garbage_dev,
+                          const int32_t* order_dev, const int32_t* level_start, int num_levels,
+                          int num_mis, int crop, float baseline, float B, float bl0, int raw_w,
+                          int raw_h, float* vd_dev, uint8_t* aif_dev, uint8_t* conf_dev,
+                          float* points_dev, size_t points_bytes,
+                          unsigned long long* counts_dev, void* workspace_dev,
+                          size_t workspace_bytes, void* stream);
+
 /* Host file writers.  lfpm_write_ply_points writes the header of StoreColorPlyFileBinaryPointCloud
  * (MVStereo/ACMP.cpp:458-470) and count records of lfpm_vd_point_cloud (copied to the host).
  * lfpm_write_dmb / lfpm_read_dmb are writeDepthDmb / readDepthDmb (MVStereo/ACMP.cpp:338-390):

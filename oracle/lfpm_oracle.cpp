@@ -1854,6 +1854,312 @@ static void vd_map(const float* plane, const float* centers, const float* gray, 
     std::copy(img.begin(), img.end(), color_out);
 }
 
+
+// ------------------------------------------------------------------------------------------
+// All-in-focus composite: LFRefocus::AIFImageCompositeForMIA (FocalStack/LFRefocus.cpp:183-425)
+// with FilterMatchingPoints (:2283-2301), GenerateVirtualImage (:2303-2328) and HoleFilling
+// (:2362-2498).  Renderings (MAD clamp, normalisation, gamma, colour maps, binary image) are
+// display-only and not restated.
+
+// Five-by-five median passes with replicated borders (cv::medianBlur, ksize 5) on an
+// interleaved image of C channels, each channel on its own.
+static void median5_passes(std::vector<uint8_t>& img, long long rows, long long cols, int C,
+                           int passes) {
+    std::vector<uint8_t> tmp(img.size());
+    for (int it = 0; it < passes; ++it) {
+        for (long long y = 0; y < rows; ++y)
+            for (long long x = 0; x < cols; ++x)
+                for (int c = 0; c < C; ++c) {
+                    int w[25], n = 0;
+                    for (int j = -2; j <= 2; ++j)
+                        for (int i = -2; i <= 2; ++i) {
+                            const long long yy = std::min(std::max(y + j, 0LL), rows - 1);
+                            const long long xx = std::min(std::max(x + i, 0LL), cols - 1);
+                            w[n++] = img[(yy * cols + xx) * C + c];
+                        }
+                    std::nth_element(w, w + 12, w + 25);
+                    tmp[(y * cols + x) * C + c] = (uint8_t)w[12];
+                }
+        img.swap(tmp);
+    }
+}
+
+struct VIPoint {
+    int mi, sx, sy;   // source MI (layout index), source pixel column / row
+    float vd;         // virtual depth baseline / disparity
+    float score;      // FilterMatchingPoints score
+};
+
+// stats (8 ints): collected points, chosen pixels, hole-filling passes, fills, largest number of
+// candidates at one pixel, the MI filled points are attributed to, pixels with more than 16
+// candidates, initial count of pixels with every channel >= 1.
+static void aif_composite(const float* plane, const float* centers, const uint8_t* bgr,
+                          const uint8_t* rich, const uint8_t* blur, const uint8_t* garbage, int V,
+                          int crop, float baseline, float B, float bl0, float f_main, int raw_w,
+                          int raw_h, float* vd_out, float* inv_out, float* real_mm_out,
+                          uint8_t* aif_out, uint8_t* filled_out, uint8_t* denoise_out,
+                          int* stats) {
+    const int W = crop, WW = crop * crop;
+    const long long rows = raw_h, cols = raw_w, N = rows * cols;
+    // Step 1 (:207-282): every pixel with d > 0 maps to round(offset * vd + centre); positions
+    // in [0, cols] x [0, rows] are collected (the last row and column past the image are never
+    // visited by step 2).  Collection order: MI (key) order, then pixel rows, then columns.
+    std::vector<std::vector<VIPoint>> co((size_t)(rows + 1) * (cols + 1));
+    VIPoint last{-1, 0, 0, 0.0f, 0.0f};
+    long long ncollected = 0;
+    for (int v = 0; v < V; ++v) {
+        if (garbage && garbage[v]) continue;
+        const float cx = centers[2 * v], cy = centers[2 * v + 1];
+        for (int y = 0; y < W; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t i = (size_t)v * WW + (size_t)y * W + x;
+                const float disp = plane[4 * i + 3];
+                if (!(disp > 0)) continue;
+                const float vdp = baseline / disp;
+                const float centerX = (W - 1) * 0.5f, centerY = (W - 1) * 0.5f;
+                const float offX = x - centerX, offY = y - centerY;
+                const int X = (int)std::round(offX * vdp + cx);
+                const int Y = (int)std::round(offY * vdp + cy);
+                if (X <= cols && X >= 0 && Y <= rows && Y >= 0) {
+                    const VIPoint pt{v, x, y, vdp, 0.0f};
+                    co[(size_t)Y * (cols + 1) + X].push_back(pt);
+                    last = pt;
+                    ++ncollected;
+                }
+            }
+    }
+    // Step 2 (:285-307) with FilterMatchingPoints: score = rich/255 * 0.9 + blur/255 * 0.1 with
+    // integer quotients, read at [srcImageX][srcImageY] (row = source column), std::sort by
+    // descending score, first element kept.
+    struct Chosen { int mi, sx, sy; float vd; bool has; };
+    std::vector<Chosen> ch((size_t)N, Chosen{-1, 0, 0, 0.0f, false});
+    int maxc = 0, over16 = 0;
+    long long nchosen = 0;
+    for (long long y = 0; y < rows; ++y)
+        for (long long x = 0; x < cols; ++x) {
+            std::vector<VIPoint> vec = co[(size_t)y * (cols + 1) + x];
+            if (vec.empty()) continue;
+            maxc = std::max(maxc, (int)vec.size());
+            if (vec.size() > 16) ++over16;
+            if (vec.size() > 1) {
+                for (VIPoint& q : vec) {
+                    const size_t t = (size_t)q.mi * WW + (size_t)q.sx * W + q.sy;
+                    const float similarityScore = (float)((rich ? rich[t] : 0) / 255);
+                    const float blureScore = (float)((blur ? blur[t] : 0) / 255);
+                    const float a = 0.90f, b = 0.10f;
+                    q.score = similarityScore * a + blureScore * b;
+                }
+                std::sort(vec.begin(), vec.end(),
+                          [](const VIPoint& p, const VIPoint& q) { return p.score > q.score; });
+            }
+            const VIPoint& w = vec[0];
+            ch[(size_t)(y * cols + x)] = Chosen{w.mi, w.sx, w.sy, w.vd, true};
+            ++nchosen;
+        }
+    // Step 3: GenerateVirtualImage.
+    std::vector<uint8_t> img((size_t)N * 3, 0);
+    for (long long p = 0; p < N; ++p) {
+        vd_out[p] = 0.0f; inv_out[p] = 0.0f; real_mm_out[p] = 0.0f;
+        const Chosen& c = ch[(size_t)p];
+        if (!c.has) continue;
+        const float tmp = c.vd * B + bl0;
+        const float real_d = (f_main * tmp) / (tmp - f_main);
+        real_mm_out[p] = real_d;
+        inv_out[p] = 1 / c.vd;
+        vd_out[p] = c.vd;
+        const uint8_t* s = bgr + ((size_t)c.mi * WW + (size_t)c.sy * W + c.sx) * 3;
+        for (int k = 0; k < 3; ++k) img[(size_t)p * 3 + k] = s[k];
+    }
+    std::copy(img.begin(), img.end(), aif_out);
+    // Step 4: HoleFilling on a copy, in place, raster order.  A filled point is recorded under
+    // the MI of the last point collected in step 1 (the member m_vIPoint), with the source
+    // pixel stepped away from the neighbour it came from.
+    const int kstar = last.mi;
+    auto hole = [&](long long p) {
+        return img[p * 3] == 0 && img[p * 3 + 1] == 0 && img[p * 3 + 2] == 0;
+    };
+    int pixelCount = 0;
+    for (long long p = 0; p < N; ++p)
+        if (img[p * 3] >= 1 && img[p * 3 + 1] >= 1 && img[p * 3 + 2] >= 1) ++pixelCount;
+    const int pixelCount0 = pixelCount;
+    int counter = 0, passes = 0;
+    const float targetFillRatio = 0.98f;
+    bool bstop = true;
+    while (bstop) {
+        ++passes;
+        int filledCounter = 0;
+        for (long long y = 0; y < rows; ++y)
+            for (long long x = 0; x < cols; ++x) {
+                const long long p = y * cols + x;
+                if (!hole(p)) continue;
+                long long q;
+                int nsx, nsy;
+                if (x + 1 < cols && !hole(p + 1)) {
+                    q = p + 1;
+                    if (!ch[q].has) continue;
+                    if (!(ch[q].sx - 1 >= 0)) continue;
+                    nsx = ch[q].sx - 1; nsy = ch[q].sy;
+                } else if (x - 1 >= 0 && !hole(p - 1)) {
+                    q = p - 1;
+                    if (!ch[q].has) continue;
+                    if (!(ch[q].sx + 1 < W)) continue;
+                    nsx = ch[q].sx + 1; nsy = ch[q].sy;
+                } else if (y + 1 < rows && !hole(p + cols)) {
+                    q = p + cols;
+                    if (!ch[q].has) continue;
+                    if (!(ch[q].sy - 1 >= 0)) continue;
+                    nsx = ch[q].sx; nsy = ch[q].sy - 1;
+                } else if (y - 1 >= 0 && !hole(p - cols)) {
+                    q = p - cols;
+                    if (!ch[q].has) continue;
+                    if (!(ch[q].sy + 1 < W)) continue;
+                    nsx = ch[q].sx; nsy = ch[q].sy + 1;
+                } else {
+                    continue;
+                }
+                const uint8_t* s = bgr + ((size_t)ch[q].mi * WW + (size_t)nsy * W + nsx) * 3;
+                for (int k = 0; k < 3; ++k) img[p * 3 + k] = s[k];
+                ++counter; ++filledCounter; ++pixelCount;
+                ch[p] = Chosen{kstar, nsx, nsy, ch[p].vd, true};
+            }
+        const float FillRatio = static_cast<float>(pixelCount) / static_cast<float>(rows * cols);
+        if (FillRatio >= targetFillRatio) bstop = false;
+        else if (filledCounter == 0) bstop = false;
+        else if (counter > 30000) bstop = false;
+    }
+    std::copy(img.begin(), img.end(), filled_out);
+    // Denoise (:364-368): five 5x5 median passes of the composite (not of the filled copy).
+    std::vector<uint8_t> dn(aif_out, aif_out + N * 3);
+    median5_passes(dn, rows, cols, 3, 5);
+    std::copy(dn.begin(), dn.end(), denoise_out);
+    if (stats) {
+        stats[0] = (int)ncollected; stats[1] = (int)nchosen; stats[2] = passes;
+        stats[3] = counter; stats[4] = maxc; stats[5] = kstar; stats[6] = over16;
+        stats[7] = pixelCount0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Back-projection consistency fusion: LFRefocus::FuseVirtualDepth_BackProject
+// (FocalStack/LFRefocus.cpp:1553-1793) with ComputeVirtualDepth (:2086-2096), ComputeVirtual3D
+// (:2098-2114) and BackProject2MI (:2116-2122).  Visualisation (quantile stretch, colour maps,
+// HSV boost, confidence binarisation) is display-only and not restated.
+
+static float bp_virtual_depth(double disparity, float mla_base) {
+    if (disparity <= 0) return -9999.0f;  // g_Invalid_image (Common/Common.cpp:53)
+    return (float)(mla_base / disparity);
+}
+
+// points (optional): 9 floats per fused point {x, y, z, nx, ny, nz, b, g, r} in the order the
+// reference pushes them; npts receives their number (points inside the frame).
+static void fuse_backproject(const float* plane, const float* centers, const uint8_t* bgr,
+                             const int* nbr, const int* cnt, int K, const uint8_t* garbage, int V,
+                             int crop, float baseline, float B, float bl0, int raw_w, int raw_h,
+                             float* vd_out, uint8_t* aif_out, uint8_t* conf_out, float* points,
+                             long long* npts, long long* nfused) {
+    const int W = crop, WW = crop * crop;
+    const long long N = (long long)raw_w * raw_h;
+    for (long long p = 0; p < N; ++p) {
+        vd_out[p] = 0.0f; conf_out[p] = 0;
+        aif_out[p * 3] = aif_out[p * 3 + 1] = aif_out[p * 3 + 2] = 0;
+    }
+    std::vector<uint8_t> masks((size_t)V * WW, 0);
+    const float mla_base = baseline * bl0 / (bl0 + B);
+    const float cmi = (W - 1) * 0.5f + 1.0f;  // center_mi, both axes
+    auto v3d = [&](float vd, int pc, int pr, int v, float* P) {
+        const float ox = pc - cmi, oy = pr - cmi;
+        P[0] = ox * vd + centers[2 * v];
+        P[1] = oy * vd + centers[2 * v + 1];
+        P[2] = vd;
+    };
+    auto back = [&](const float* P, int v, float* q) {
+        q[0] = ((P[0] - centers[2 * v]) / P[2]) + cmi;
+        q[1] = ((P[1] - centers[2 * v + 1]) / P[2]) + cmi;
+    };
+    auto bad = [](float vd) { return std::fabs(vd - -9999.0f) < 1.0e-6f || !std::isfinite(vd); };
+    long long np = 0, nf = 0;
+    for (int a = 0; a < V; ++a) {
+        if (garbage && garbage[a]) continue;
+        const int n_ngb = cnt[a];
+        std::vector<int> used(n_ngb);
+        for (int r = 0; r < W; ++r)
+            for (int c = 0; c < W; ++c) {
+                if (masks[(size_t)a * WW + r * W + c] == 1) continue;
+                const size_t ir = (size_t)a * WW + r * W + c;
+                const float disp_ref = plane[4 * ir + 3];
+                if (disp_ref < 1.0f) continue;
+                const float vd_ref = bp_virtual_depth(disp_ref, mla_base);
+                if (bad(vd_ref)) continue;
+                float nrm[3] = {plane[4 * ir], plane[4 * ir + 1], plane[4 * ir + 2]};
+                float P[3];
+                v3d(vd_ref, c, r, a, P);
+                const float Pref[3] = {P[0], P[1], P[2]};
+                float col[3] = {(float)bgr[ir * 3], (float)bgr[ir * 3 + 1], (float)bgr[ir * 3 + 2]};
+                std::fill(used.begin(), used.end(), -1);
+                int num = 0;
+                for (int j = 0; j < n_ngb; ++j) {
+                    const int b = nbr[(size_t)a * K + j];
+                    if (b < 0 || (garbage && garbage[b])) continue;
+                    float q[2];
+                    back(Pref, b, q);
+                    const int sr = static_cast<int>(q[1] + 0.5f);
+                    const int sc = static_cast<int>(q[0] + 0.5f);
+                    if (sc < 0 || sc >= W || sr < 0 || sr >= W) continue;
+                    if (masks[(size_t)b * WW + sr * W + sc] == 1) continue;
+                    const size_t is = (size_t)b * WW + sr * W + sc;
+                    const float disp_src = plane[4 * is + 3];
+                    if (disp_src <= 0.0f) continue;
+                    const float vd_src = bp_virtual_depth(disp_src, mla_base);
+                    if (bad(vd_src)) continue;
+                    float Ps[3], qr[2];
+                    v3d(vd_src, sc, sr, b, Ps);
+                    back(Ps, a, qr);
+                    const float err = std::sqrt((c - qr[0]) * (c - qr[0]) + (r - qr[1]) * (r - qr[1]));
+                    const float rel = std::fabs(vd_src - vd_ref) / std::max(vd_ref, 1.0e-6f);
+                    if (err < 1.0f && rel < 0.05f) {
+                        for (int k = 0; k < 3; ++k) {
+                            P[k] += Ps[k];
+                            nrm[k] += plane[4 * is + k];
+                            col[k] += (float)bgr[is * 3 + k];
+                        }
+                        used[j] = sr * W + sc;
+                        ++num;
+                    }
+                }
+                if (num >= 2) {
+                    // cv::Vec3f /= float: product with the float reciprocal (OpenCV matx.hpp)
+                    const float denom = static_cast<float>(num) + 1.0f;
+                    const float ial = 1.f / denom;
+                    for (int k = 0; k < 3; ++k) { P[k] *= ial; nrm[k] *= ial; col[k] *= ial; }
+                    ++nf;
+                    const int vx = (int)std::nearbyint(P[0]), vy = (int)std::nearbyint(P[1]);
+                    if (vy >= 0 && vy < raw_h && vx >= 0 && vx < raw_w) {
+                        const long long o = (long long)vy * raw_w + vx;
+                        vd_out[o] = P[2];
+                        conf_out[o] = (uint8_t)std::min(255, (num + 1) * 60);
+                        for (int k = 0; k < 3; ++k) {
+                            const int iv = (int)std::nearbyint(col[k]);
+                            aif_out[o * 3 + k] = (uint8_t)std::min(std::max(iv, 0), 255);
+                        }
+                        if (points) {
+                            float* o9 = points + np * 9;
+                            for (int k = 0; k < 3; ++k) {
+                                o9[k] = P[k]; o9[3 + k] = nrm[k]; o9[6 + k] = col[k];
+                            }
+                        }
+                        ++np;
+                    }
+                    for (int j = 0; j < n_ngb; ++j) {
+                        if (used[j] == -1) continue;
+                        masks[(size_t)nbr[(size_t)a * K + j] * WW + used[j]] = 1;
+                    }
+                }
+            }
+    }
+    if (npts) *npts = np;
+    if (nfused) *nfused = nf;
+}
+
 } // namespace orc
 
 // ------------------------------------------------------------------------------------------
@@ -2063,6 +2369,24 @@ void orc_refine_candidates_f64(const orc::PmParams* p, int x, int y, const doubl
         cand[4 * i] = c[i].x; cand[4 * i + 1] = c[i].y;
         cand[4 * i + 2] = c[i].z; cand[4 * i + 3] = c[i].w;
     }
+}
+
+void orc_aif_composite(const float* plane, const float* centers, const uint8_t* bgr,
+                       const uint8_t* rich, const uint8_t* blur, const uint8_t* garbage, int V,
+                       int crop, float baseline, float B, float bl0, float f_main, int raw_w,
+                       int raw_h, float* vd, float* inv, float* real_mm, uint8_t* aif,
+                       uint8_t* filled, uint8_t* denoise, int* stats) {
+    orc::aif_composite(plane, centers, bgr, rich, blur, garbage, V, crop, baseline, B, bl0,
+                       f_main, raw_w, raw_h, vd, inv, real_mm, aif, filled, denoise, stats);
+}
+
+void orc_fuse_backproject(const float* plane, const float* centers, const uint8_t* bgr,
+                          const int* nbr, const int* cnt, int K, const uint8_t* garbage, int V,
+                          int crop, float baseline, float B, float bl0, int raw_w, int raw_h,
+                          float* vd, uint8_t* aif, uint8_t* conf, float* points, long long* npts,
+                          long long* nfused) {
+    orc::fuse_backproject(plane, centers, bgr, nbr, cnt, K, garbage, V, crop, baseline, B, bl0,
+                          raw_w, raw_h, vd, aif, conf, points, npts, nfused);
 }
 
 } // extern "C"

@@ -377,6 +377,58 @@ def vd_map_bgr(plane, centers, bgr_tiles, crop, mla_base, raw_w, raw_h):
     return vd, col, rng
 
 
+def aif_composite(plane, centers, bgr_tiles, crop, baseline, B, bl0, f_main, raw_w, raw_h,
+                  rich=None, blur=None, garbage=None, L=None):
+    """All-in-focus composite (AIFImageCompositeForMIA): dict vd, inv_vd, real_mm (h, w) f32,
+    aif, filled, denoised (h, w, 3) u8, stats (8,) int32."""
+    plane = np.ascontiguousarray(plane, np.float32)
+    centers = np.ascontiguousarray(centers, np.float32)
+    V = centers.shape[0]
+    b = np.ascontiguousarray(bgr_tiles, np.uint8)
+    assert b.shape == (V, crop, crop, 3)
+    r = dict(vd=np.zeros((raw_h, raw_w), np.float32), inv_vd=np.zeros((raw_h, raw_w), np.float32),
+             real_mm=np.zeros((raw_h, raw_w), np.float32),
+             aif=np.zeros((raw_h, raw_w, 3), np.uint8), filled=np.zeros((raw_h, raw_w, 3), np.uint8),
+             denoised=np.zeros((raw_h, raw_w, 3), np.uint8), stats=np.zeros(8, np.int32))
+    ri = np.ascontiguousarray(rich, np.uint8) if rich is not None else None
+    bl = np.ascontiguousarray(blur, np.uint8) if blur is not None else None
+    ga = np.ascontiguousarray(garbage, np.uint8) if garbage is not None else None
+    (L or lib()).orc_aif_composite(_p(plane), _p(centers), _p(b), _p(ri) if ri is not None else None,
+                            _p(bl) if bl is not None else None, _p(ga) if ga is not None else None,
+                            V, crop, C.c_float(baseline), C.c_float(B), C.c_float(bl0),
+                            C.c_float(f_main), raw_w, raw_h, _p(r["vd"]), _p(r["inv_vd"]),
+                            _p(r["real_mm"]), _p(r["aif"]), _p(r["filled"]), _p(r["denoised"]),
+                            _p(r["stats"]))
+    return r
+
+
+def fuse_backproject(plane, centers, bgr_tiles, nbr, cnt, crop, baseline, B, bl0, raw_w, raw_h,
+                     garbage=None, L=None):
+    """Back-projection consistency fusion (FuseVirtualDepth_BackProject): dict vd (h, w) f32,
+    aif (h, w, 3) u8, conf (h, w) u8, points (n, 9) f32, nfused."""
+    plane = np.ascontiguousarray(plane, np.float32)
+    centers = np.ascontiguousarray(centers, np.float32)
+    V = centers.shape[0]
+    b = np.ascontiguousarray(bgr_tiles, np.uint8)
+    nbr = np.ascontiguousarray(nbr, np.int32)
+    cnt = np.ascontiguousarray(cnt, np.int32)
+    K = nbr.size // V
+    ga = np.ascontiguousarray(garbage, np.uint8) if garbage is not None else None
+    r = dict(vd=np.zeros((raw_h, raw_w), np.float32), aif=np.zeros((raw_h, raw_w, 3), np.uint8),
+             conf=np.zeros((raw_h, raw_w), np.uint8))
+    pts = np.zeros((V * crop * crop, 9), np.float32)
+    npts = C.c_longlong()
+    nf = C.c_longlong()
+    (L or lib()).orc_fuse_backproject(_p(plane), _p(centers), _p(b), _p(nbr), _p(cnt), K,
+                               _p(ga) if ga is not None else None, V, crop, C.c_float(baseline),
+                               C.c_float(B), C.c_float(bl0), raw_w, raw_h, _p(r["vd"]),
+                               _p(r["aif"]), _p(r["conf"]), _p(pts), C.byref(npts),
+                               C.byref(nf))
+    r["points"] = pts[:npts.value].copy()
+    r["nfused"] = nf.value
+    return r
+
+
 def slice_bgr(bgr, origins, crop):
     """Colour crops (V, crop, crop, 3) of a (H, W, 3) B, G, R frame at the MI origins (the
     problem.m_Image_rgb crops of DepthSolver::Slice_RawMLAImage, DepthSolver.cpp:1951-1997)."""

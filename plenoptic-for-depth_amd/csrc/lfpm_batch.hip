@@ -1,4 +1,4 @@
-// lfpm_batch.hip — batched and served forms over the frame-table kernel (k_stage):
+// lfpm_batch.hip — batched and served forms over the frame-table kernel (k_stage_w):
 // class grouping, the stream-ordered batch schedule, its hipGraph capture, the continuous-
 // batching step and the occupancy autotuner.
 #include <hip/hip_runtime.h>

@@ -37,4 +37,14 @@ int stage_launch(const lfpm_pm_params* p, const StageFrame* frames, int n, void*
 // the setter the autotuner uses.
 int stage_slots_per_cu(const lfpm_pm_params* p);
 void set_stage_slots_per_cu(const lfpm_pm_params* p, int per_cu);
+
+// Device helpers of lfpm_products.hip.  exclusive_scan: out[i] = sum of in[0..i) for i < n, on
+// `stream`, with `sums` of scan_sums_bytes(n) bytes.  median5_passes: `passes` (>= 1) 5x5
+// median passes with replicated borders over the raw_w x raw_h plane `in`, ping-ponging through
+// t0 / t1 (raw_w*raw_h bytes each; `in` is not written), the last one into out at
+// out[(y*raw_w + x) * ostride].
+size_t scan_sums_bytes(long long n);
+void exclusive_scan(const int* in, long long n, int* out, int* sums, void* stream);
+void median5_passes(const uint8_t* in, uint8_t* t0, uint8_t* t1, int raw_w, int raw_h, int passes,
+                    uint8_t* out, int ostride, void* stream);
 }  // namespace lfpm

@@ -1723,7 +1723,7 @@ __global__ void k_segments(const int* __restrict__ hyp, int n, int V, int* __res
     seg[v] = lo;
 }
 
-// Wave-per-hypothesis form of k_eval (the default): one wavefront scores one record over all
+// Wave-per-hypothesis cost evaluation: one wavefront scores one record over all
 // of its MI's views with the task engine of wv::run_tasks.
 template <bool COUNT, typename TT, bool BLUR, int W1C>
 __global__ void __launch_bounds__(256) k_eval_w(EvalArgs a) {
@@ -2164,7 +2164,7 @@ static int persistent_grid(const DevParams& P, int per_cu) { return persistent_g
 
 static thread_local unsigned long long* g_counters = nullptr;
 
-static const int kPhasePerCu = 8;    // k_stage workgroups per CU (persistent slots), default
+static const int kPhasePerCu = 8;    // k_stage_w workgroups per CU (persistent slots), default
 static const int kMaxSlotsPerCu = 8; // upper bound of the autotuner's search
 
 // Cost records of V MIs (5 + K floats per pixel).
@@ -2232,7 +2232,7 @@ __global__ void k_zero_tickets(unsigned* __restrict__ t) {
     if (threadIdx.x < 8) t[threadIdx.x] = 0u;
 }
 
-// Launch k_stage over the frames of `a` (mi_begin/total filled here).
+// Launch k_stage_w over the frames of `a` (mi_begin/total filled here).
 static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_bytes,
                         hipStream_t s) {
     int total = 0;
@@ -2245,7 +2245,7 @@ static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_
     const int NT = wave_threads(a.P);
     const size_t lds = wv::lds_total(a.P, NT / 64);
     if (lds > 65536)
-        return set_error(LFPM_E_UNSUPPORTED, "k_stage: %zu B of LDS per workgroup (crop %d, "
+        return set_error(LFPM_E_UNSUPPORTED, "k_stage_w: %zu B of LDS per workgroup (crop %d, "
                          "%d views, radius %d) exceeds 64 KiB", lds, a.P.W, a.P.K, a.P.r);
     const StageKernel kern = stage_w_kernel(g_counters != nullptr, a.P.fmt,
                                             a.P.blur != 0, a.P.W);
@@ -2261,7 +2261,7 @@ static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_
         hipLaunchKernelGGL(k_zero_tickets, dim3(1), dim3(64), 0, s, a.tickets);
     }
     hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
-    return launch_check("k_stage");
+    return launch_check("k_stage_w");
 }
 static size_t state_bytes(const DevParams& P) {
     return (size_t)P.V * P.WW * (16 + 4 + 4);
@@ -2269,7 +2269,7 @@ static size_t state_bytes(const DevParams& P) {
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Autotuned k_stage occupancy per parameter class (crop, neighbours, radius).
+// Autotuned k_stage_w occupancy per parameter class (crop, neighbours, radius).
 struct TuneKey {
     int crop, K, r;
     bool operator==(const TuneKey& o) const { return crop == o.crop && K == o.K && r == o.r; }

@@ -1022,3 +1022,33 @@ int point_cloud_impl(const float* vd_dev, const uint8_t* color_dev, int nch, int
 }
 }  // namespace
 }  // namespace lfpm
+
+// Helpers shared with the composites of lfpm_fusion.hip (declared in lfpm_internal.h).
+namespace lfpm {
+
+size_t scan_sums_bytes(long long n) {
+    return al((size_t)((n + kScanChunk - 1) / kScanChunk) * 4);
+}
+
+void exclusive_scan(const int* in, long long n, int* out, int* sums, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const int nb = (int)((n + kScanChunk - 1) / kScanChunk);
+    k_scan_sums<<<nb, kScanThreads, 0, s>>>(in, n, sums);
+    k_scan_sums_scan<<<1, kScanThreads, 0, s>>>(sums, nb);
+    k_scan_chunks<<<nb, kScanThreads, 0, s>>>(in, n, sums, out);
+}
+
+void median5_passes(const uint8_t* in, uint8_t* t0, uint8_t* t1, int raw_w, int raw_h, int passes,
+                    uint8_t* out, int ostride, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const long long npairs = (long long)((raw_w + 1) >> 1) * raw_h;
+    const uint8_t* src = in;
+    for (int it = 0; it + 1 < passes; ++it) {
+        uint8_t* dst = (it & 1) ? t1 : t0;
+        k_median5<<<grid_for(npairs, 256), 256, 0, s>>>(src, raw_w, raw_h, dst, 1);
+        src = dst;
+    }
+    k_median5<<<grid_for(npairs, 256), 256, 0, s>>>(src, raw_w, raw_h, out, ostride);
+}
+
+}  // namespace lfpm

@@ -75,6 +75,8 @@ def lib():
         L.lfpm_products_workspace_size.restype = C.c_size_t
         L.lfpm_patchmatch_acmm_workspace_size.restype = C.c_size_t
         L.lfpm_batch_workspace_size.restype = C.c_size_t
+        L.lfpm_aif_workspace_size.restype = C.c_size_t
+        L.lfpm_fuse_workspace_size.restype = C.c_size_t
         _lib = L
     return _lib
 
@@ -598,6 +600,94 @@ def virtual_depth_map(df, plane, mla_base=None, color=True, ws=None, stream=None
                                         _dp(col) if col is not None else None, _dp(rng), _dp(ws),
                                         C.c_size_t(ws.numel()), _stream(stream)))
     return vd, col, rng
+
+
+def aif_composite(df, plane, bgr_tiles, richness=None, blur=None, garbage=None, baseline=None,
+                  B=1.0, bl0=0.0, main_lens_f=105.0, stream=None):
+    """All-in-focus composite of a solved frame (lfpm_aif_composite: AIFImageCompositeForMIA,
+    FocalStack/LFRefocus.cpp:183-425).  bgr_tiles: colour crops (V, crop, crop, 3); richness /
+    blur: optional uint8 crops (V, crop, crop) of the binarised richness and the blur score;
+    garbage: optional uint8 (V,).  baseline defaults to the lens pitch.  Returns a dict of
+    device tensors vd, inv_vd, real_mm (h, w) float32, aif, filled, denoised (h, w, 3) uint8 and
+    stats (8,) int32."""
+    import torch
+    wl = df.wl
+    h, w = df.raw.shape
+    dev = plane.device
+    baseline = wl.L["pitch"] if baseline is None else baseline
+    n = lib().lfpm_aif_workspace_size(wl.V, wl.crop, w, h)
+    ws = torch.empty(n, dtype=torch.uint8, device=dev)
+    out = dict(vd=torch.empty((h, w), dtype=torch.float32, device=dev),
+               inv_vd=torch.empty((h, w), dtype=torch.float32, device=dev),
+               real_mm=torch.empty((h, w), dtype=torch.float32, device=dev),
+               aif=torch.empty((h, w, 3), dtype=torch.uint8, device=dev),
+               filled=torch.empty((h, w, 3), dtype=torch.uint8, device=dev),
+               denoised=torch.empty((h, w, 3), dtype=torch.uint8, device=dev),
+               stats=torch.empty(8, dtype=torch.int32, device=dev))
+    opt = lambda t: _dp(t) if t is not None else None  # noqa: E731
+    _check(lib().lfpm_aif_composite(
+        _dp(plane), _dp(df.centers), _dp(bgr_tiles), opt(richness), opt(blur), opt(garbage),
+        wl.V, wl.crop, C.c_float(baseline), C.c_float(B), C.c_float(bl0),
+        C.c_float(main_lens_f), w, h, _dp(out["vd"]), _dp(out["inv_vd"]), _dp(out["real_mm"]),
+        _dp(out["aif"]), _dp(out["filled"]), _dp(out["denoised"]), _dp(out["stats"]), _dp(ws),
+        C.c_size_t(n), _stream(stream)))
+    return out
+
+
+def fuse_schedule(nbr, cnt, garbage=None):
+    """Level schedule of a neighbour table (lfpm_fuse_schedule, host): (order int32 (V,),
+    level_start int32 (levels + 1,), levels)."""
+    nbr = np.ascontiguousarray(nbr, np.int32)
+    cnt = np.ascontiguousarray(cnt, np.int32)
+    V = cnt.shape[0]
+    K = nbr.size // V
+    g = None if garbage is None else np.ascontiguousarray(garbage, np.uint8)
+    order = np.zeros(V, np.int32)
+    start = np.zeros(V + 1, np.int32)
+    nl = C.c_int()
+    _check(lib().lfpm_fuse_schedule(_np(nbr), _np(cnt), _np(g) if g is not None else None, V, K,
+                                    _np(order), _np(start), C.byref(nl)))
+    return order, start[:nl.value + 1].copy(), nl.value
+
+
+def fuse_backproject(df, plane, bgr_tiles, garbage=None, baseline=None, B=1.0, bl0=0.0,
+                     points=True, schedule=None, stream=None):
+    """Back-projection consistency fusion of a solved frame (lfpm_fuse_backproject:
+    FuseVirtualDepth_BackProject, FocalStack/LFRefocus.cpp:1553-1793) over the frame's matching
+    table.  Returns a dict of device tensors vd (h, w) float32, aif (h, w, 3) uint8, conf (h, w)
+    uint8, points (n, 9) float32 (or None), and ints npoints, nfused, levels."""
+    import torch
+    wl = df.wl
+    h, w = df.raw.shape
+    dev = plane.device
+    baseline = wl.L["pitch"] if baseline is None else baseline
+    K = df.nbr.numel() // wl.V
+    gh = None if garbage is None else np.ascontiguousarray(
+        garbage.cpu().numpy() if hasattr(garbage, "cpu") else garbage, np.uint8)
+    if schedule is None:
+        schedule = fuse_schedule(df.nbr.cpu().numpy(), df.cnt.cpu().numpy(), gh)
+    order, start, nl = schedule
+    order_d = torch.from_numpy(order).to(dev)
+    gd = None if gh is None else torch.from_numpy(gh).to(dev)
+    n = lib().lfpm_fuse_workspace_size(wl.V, wl.crop, w, h)
+    ws = torch.empty(n, dtype=torch.uint8, device=dev)
+    M = wl.V * wl.crop * wl.crop
+    pts = torch.empty((M, 9), dtype=torch.float32, device=dev) if points else None
+    out = dict(vd=torch.empty((h, w), dtype=torch.float32, device=dev),
+               aif=torch.empty((h, w, 3), dtype=torch.uint8, device=dev),
+               conf=torch.empty((h, w), dtype=torch.uint8, device=dev))
+    counts = torch.zeros(2, dtype=torch.int64, device=dev)
+    _check(lib().lfpm_fuse_backproject(
+        _dp(plane), _dp(df.centers), _dp(bgr_tiles), _dp(df.nbr), _dp(df.cnt), K,
+        _dp(gd) if gd is not None else None, _dp(order_d), _np(start), nl, wl.V, wl.crop,
+        C.c_float(baseline), C.c_float(B), C.c_float(bl0), w, h, _dp(out["vd"]),
+        _dp(out["aif"]), _dp(out["conf"]), _dp(pts) if pts is not None else None,
+        C.c_size_t(pts.numel() * 4 if pts is not None else 0), _dp(counts), _dp(ws),
+        C.c_size_t(n), _stream(stream)))
+    c = counts.cpu().numpy()
+    out.update(npoints=int(c[0]), nfused=int(c[1]), levels=nl,
+               points=pts[:int(c[0])] if pts is not None else None)
+    return out
 
 
 class Context:

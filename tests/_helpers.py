@@ -1,4 +1,6 @@
 """Helpers shared by the test modules (conversion between product and oracle structs)."""
+import os
+
 import numpy as np
 
 
@@ -54,3 +56,24 @@ def cost_bar(out, ref, geo, gate, ferr, tol, floor=1e-3):
                 frac_le_tol=float((e <= tol).mean()), max_rel=float(e.max()),
                 max_rel_unexcused=float((err / np.maximum(np.abs(ref), floor))[
                     scored & ~straddle & ~ill].max()))
+
+
+ORACLE_SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                          "oracle", "lfpm_oracle.cpp")
+
+
+def build_oracle_mutant(i, edits, tmp):
+    """Compile the oracle source with textual edits [(old, new), ...] (each old text must occur
+    exactly once) into tmp/mut<i>.so and return its path."""
+    import subprocess
+    s = open(ORACLE_SRC).read()
+    for old, new in edits:
+        assert s.count(old) == 1, (i, old, s.count(old))
+        s = s.replace(old, new)
+    src = os.path.join(tmp, "mut%d.cpp" % i)
+    so = os.path.join(tmp, "mut%d.so" % i)
+    with open(src, "w") as f:
+        f.write(s)
+    subprocess.check_call(["g++", "-O0", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off",
+                           "-shared", "-o", so, src])
+    return so

@@ -7,15 +7,12 @@ of them is kept."""
 import concurrent.futures as cf
 import ctypes as C
 import os
-import subprocess
 
 import pytest
 
 import oracle as O
+from _helpers import build_oracle_mutant as _build
 from _update_pins import all_checks
-
-HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(os.path.dirname(HERE), "oracle", "lfpm_oracle.cpp")
 
 
 @pytest.fixture(scope="module")
@@ -62,20 +59,6 @@ MUTATIONS = [
                                   "if (y <= 1 && y > 0) { flag[1] = true;")]),
     ("far extra step dropped", [("if (y > 3) pick(b, cm, center - 4 * W);", "")]),
 ]
-
-
-def _build(i, edits, tmp):
-    s = open(SRC).read()
-    for old, new in edits:
-        assert s.count(old) == 1, (i, old, s.count(old))
-        s = s.replace(old, new)
-    src = os.path.join(tmp, "mut%d.cpp" % i)
-    so = os.path.join(tmp, "mut%d.so" % i)
-    with open(src, "w") as f:
-        f.write(s)
-    subprocess.check_call(["g++", "-O0", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off",
-                           "-shared", "-o", so, src])
-    return so
 
 
 def test_mutations_are_caught(tmp_path, pm):
