@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="batched",
                     choices=["plain", "batched", "mixed", "serve", "precision", "ingest",
-                             "products"])
+                             "products", "composites"])
     ap.add_argument("--hypotheses", type=int, default=1000000, help="precision: hypotheses")
     ap.add_argument("--batch", type=int, default=4, help="frames per batch (batched / mixed)")
     ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
@@ -566,6 +566,122 @@ def products_section(args, lfpm, torch, wl):
                                        "%dx%d frame, one frame in %.1f s" % (w, h, cpu_s)}}
 
 
+def composites_section(args, lfpm, torch, wl):
+    """Refocus-side composites of one solved frame with a colour raw image: the all-in-focus
+    composite with its depth maps, hole filling and denoising (lfpm_aif_composite) and the
+    back-projection consistency fusion with its point records (lfpm_fuse_backproject).  The
+    frame is solved once and the fusion's level schedule built once (host work per lens
+    layout); each step runs both entry points on the launch stream, timed with HIP events."""
+    import ctypes as C
+    df = lfpm.DeviceFrame(wl)
+    bufs = lfpm.patchmatch_frame(df)
+    h, w = df.raw.shape
+    V, W = wl.V, wl.crop
+    raw = wl.raw().astype(np.int64)
+    rng = np.random.default_rng(35)
+    bgr = np.stack([raw, (raw * 7 + 11) % 256, rng.integers(0, 256, raw.shape)],
+                   -1).astype(np.uint8)
+    tiles = lfpm.slice_bgr_tiles(df, torch.from_numpy(bgr).cuda())
+    rich = np.where(rng.random((V, W, W)) < 0.4, 255, 0).astype(np.uint8)
+    blur = rng.integers(0, 256, (V, W, W)).astype(np.uint8)
+    blur[rng.random((V, W, W)) < 0.25] = 255
+    rich_d, blur_d = torch.from_numpy(rich).cuda(), torch.from_numpy(blur).cuda()
+    K = wl.L["nbr"].size // V
+    B, bl0, f = 0.9, 42.0, 30.0
+    baseline_bp = wl.L["pitch"] * (bl0 + B) / bl0
+    order, start, nl = lfpm.fuse_schedule(wl.L["nbr"].reshape(V, K), wl.L["cnt"])
+    order_d = torch.from_numpy(order).cuda()
+    L, st, dp = lfpm.lib(), lfpm._stream(), lfpm._dp
+    na = L.lfpm_aif_workspace_size(V, W, w, h)
+    nf = L.lfpm_fuse_workspace_size(V, W, w, h)
+    ws_a = torch.empty(na, dtype=torch.uint8, device="cuda")
+    ws_f = torch.empty(nf, dtype=torch.uint8, device="cuda")
+    f32 = lambda *s: torch.empty(s, dtype=torch.float32, device="cuda")  # noqa: E731
+    u8 = lambda *s: torch.empty(s, dtype=torch.uint8, device="cuda")  # noqa: E731
+    a_out = [f32(h, w), f32(h, w), f32(h, w), u8(h, w, 3), u8(h, w, 3), u8(h, w, 3)]
+    a_stats = torch.empty(8, dtype=torch.int32, device="cuda")
+    f_vd, f_aif, f_conf = f32(h, w), u8(h, w, 3), u8(h, w)
+    pts = f32(V * W * W, 9)
+    counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    def step():
+        e0 = ev()
+        lfpm._check(L.lfpm_aif_composite(
+            dp(bufs.plane), dp(df.centers), dp(tiles), dp(rich_d), dp(blur_d), None, V, W,
+            C.c_float(wl.L["pitch"]), C.c_float(B), C.c_float(bl0), C.c_float(f), w, h,
+            *[dp(t) for t in a_out], dp(a_stats), dp(ws_a), C.c_size_t(na), st))
+        e1 = ev()
+        lfpm._check(L.lfpm_fuse_backproject(
+            dp(bufs.plane), dp(df.centers), dp(tiles), dp(df.nbr), dp(df.cnt), K, None,
+            dp(order_d), lfpm._np(start), nl, V, W, C.c_float(baseline_bp), C.c_float(B),
+            C.c_float(bl0), w, h, dp(f_vd), dp(f_aif), dp(f_conf), dp(pts),
+            C.c_size_t(pts.numel() * 4), dp(counts), dp(ws_f), C.c_size_t(nf), st))
+        e2 = ev()
+        return e0, e1, e2
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    marks = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        marks.append(step())
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    ms = {k: float(np.mean([m[i].elapsed_time(m[i + 1]) for m in marks]))
+          for i, k in enumerate(("aif_composite", "fuse_backproject"))}
+    ast = a_stats.cpu().numpy()
+    c = counts.cpu().numpy()
+    # algorithmic HBM bytes of the two calls: inputs read once (plane 16 B + colour 3 B per MI
+    # pixel; richness + blur 2 B per MI pixel for the composite), outputs written once (21 B per
+    # raw pixel for the composite's maps and images, 8 B for the fusion's, 36 B per point)
+    M, N = V * W * W, w * h
+    alg = 2 * 19.0 * M + 2.0 * M + 29.0 * N + 36.0 * int(c[0])
+    gbs = alg / (sum(ms.values()) * 1e-3) / 1e9
+    res = {"fps": 1e3 / sum(ms.values()), "ms": ms, "wall_ms": 1e3 * wall,
+           "levels": nl, "fused": int(c[1]), "points": int(c[0]),
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
+                        "frac": gbs / 8000.0, "traffic": None,
+                        "kernel": "lfpm_aif_composite + lfpm_fuse_backproject (all launches)",
+                        "avg_launch_ms": sum(ms.values()),
+                        "per_unit": "19 B per MI pixel per call + 2 B (score maps) + 29 B per "
+                                    "raw pixel + 36 B per fused point",
+                        "note": "dependency-bound, not bandwidth-bound: the reference's fusion "
+                                "walk is sequential within each MI (one wavefront per MI) and "
+                                "across %d schedule levels; hole filling iterates rows to a "
+                                "fixed point" % nl},
+           "aif_stats": {"collected": int(ast[0]), "kept": int(ast[1]), "passes": int(ast[2]),
+                         "fills": int(ast[3])}, "cpu_baseline": None}
+    if args.no_cpu_baseline:
+        return res
+    # CPU baseline: the oracle's two composites of the same frame
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    p = bufs.plane.cpu().numpy()
+    tl = tiles.cpu().numpy()
+    t1 = time.perf_counter()
+    oa = oracle.aif_composite(p, wl.L["centers"], tl, W, wl.L["pitch"], B, bl0, f, w, h,
+                              rich=rich, blur=blur)
+    of = oracle.fuse_backproject(p, wl.L["centers"], tl, wl.L["nbr"].reshape(V, K), wl.L["cnt"],
+                                 W, baseline_bp, B, bl0, w, h)
+    cpu_s = time.perf_counter() - t1
+    same = bool(np.array_equal(a_out[4].cpu().numpy(), oa["filled"]) and
+                np.array_equal(f_vd.cpu().numpy().view(np.uint32), of["vd"].view(np.uint32)))
+    res.update(equal_oracle=same,
+               cpu_baseline={"value": 1.0 / cpu_s, "unit": "frames/s", "cores": 1,
+                             "kind": "port", "cpu": cpu_model(),
+                             "sample": "oracle AIF composite + back-projection fusion of the "
+                                       "same solved %dx%d frame, one frame in %.1f s"
+                                       % (w, h, cpu_s)})
+    return res
+
+
 def calib_copy(torch):
     """One 1 GiB device-to-device copy: the known-size kernel that scales the rocprofv3
     FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -705,6 +821,21 @@ def main():
                     config={"workload": "disparity mosaic + VD median splat with gray colour "
                                         "image + VD point cloud of one solved %dx%d frame "
                                         "(%d MIs)" % (rw, rh, wl.V)})
+        print(json.dumps(line), flush=True)
+        return
+    if args.mode == "composites":
+        wl = lfpm.workload_c2(raw_w=rw, raw_h=rh)
+        r = composites_section(args, lfpm, torch, wl)
+        if args.calib_copy:
+            calib_copy(torch)
+        line.update(metric="refocus composites per s (%dx%d: AIF composite + back-projection "
+                           "fusion)" % (rw, rh), value=r["fps"], ms_per_step=sum(r["ms"].values()),
+                    dtype="fp32", roofline=r["roofline"], cpu_baseline=r["cpu_baseline"],
+                    composites=r,
+                    config={"workload": "all-in-focus composite (depth maps, hole filling, "
+                                        "denoising) + back-projection consistency fusion (18 "
+                                        "neighbours) of one solved %dx%d frame (%d MIs)"
+                                        % (rw, rh, wl.V)})
         print(json.dumps(line), flush=True)
         return
     log("bench: synthesising %dx%d frame" % (rw, rh))

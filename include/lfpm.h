@@ -514,7 +514,9 @@ int lfpm_vd_point_cloud_bgr(const float* vd_dev, const uint8_t* color_bgr_dev, i
  * replicated borders of aif_dev, each channel on its own).  stats_dev (8 ints): collected
  * candidates, kept pixels, hole-filling passes, fills, most candidates at one pixel, the
  * attribution MI (-1: none), pixels with more than 16 candidates, pixels with every channel >= 1
- * before filling.  Limits: crop <= 127, raw_w <= 8192, V*crop^2 < 2^30.
+ * before filling.  Limits: crop <= 127, raw_w <= 8192, V*crop^2 < 2^30.  The hole filling runs
+ * as row iterations whose number depends on the data: the call synchronises `stream` to read
+ * their progress and returns when all outputs are written.
  * Display renderings (MAD clamp, normalisation, gamma, colour maps) are not produced.
  * ------------------------------------------------------------------------------------- */
 size_t lfpm_aif_workspace_size(int num_mis, int crop, int raw_w, int raw_h);

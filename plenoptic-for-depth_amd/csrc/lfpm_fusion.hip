@@ -11,9 +11,9 @@
 //   k_aif_collect   one lane per MI pixel: projection, per-pixel atomic max of (score class,
 //                   -collection index) — the first best candidate in collection order
 //   k_aif_resolve   one lane per virtual pixel: depth maps, colour, hole-filling state
-//   k_aif_holefill  one workgroup: the raster recurrence of HoleFilling, row by row (pixels whose
-//                   outcome does not depend on their left neighbour in parallel, the runs that
-//                   do walked by one lane each), passes until the reference's stop rule holds
+//   k_hole_row      one workgroup per row of a HoleFilling pass, iterated to the fixed point of
+//                   the rows' dependence on the row above (the raster result); passes until the
+//                   reference's stop rule holds
 //
 // Back-projection consistency fusion, LFRefocus::FuseVirtualDepth_BackProject (:1553-1793).
 // Every MI pixel's virtual point is back-projected into its matching neighbours; a neighbour
@@ -42,7 +42,6 @@ namespace lfpm {
 namespace {
 
 constexpr uint32_t kSeqMask = 0x3FFFFFFFu;   // collection index field of a candidate key
-constexpr int kHoleThreads = 1024;
 // cc word flags of the hole-filling row state (colour in bits 0..23)
 constexpr uint32_t kDep = 1u << 31;      // outcome depends on the left neighbour
 constexpr uint32_t kDFill = 1u << 30;    // the down / up alternative fills
@@ -130,12 +129,21 @@ __global__ void __launch_bounds__(256) k_aif_collect(
             }
         }
     }
+    // block totals: candidates collected, and the last one collected (highest index) for the
+    // attribution MI
+    __shared__ int s_n, s_last;
+    if (threadIdx.x == 0) { s_n = 0; s_last = -1; }
+    __syncthreads();
     const unsigned long long m = __ballot(collected);
     const int lane = threadIdx.x & 63;
     if (m) {
-        if (lane == __ffsll((long long)m) - 1) atomicAdd(&stats[0], __popcll(m));
-        // the last collected point of the wave (highest index) for the attribution MI
-        if (lane == 63 - __clzll((long long)m)) atomicMax(&stats[5], (int)i);
+        if (lane == __ffsll((long long)m) - 1) atomicAdd(&s_n, __popcll(m));
+        if (lane == 63 - __clzll((long long)m)) atomicMax(&s_last, (int)i);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n) {
+        atomicAdd(&stats[0], s_n);
+        atomicMax(&stats[5], s_last);
     }
 }
 
@@ -175,15 +183,24 @@ __global__ void __launch_bounds__(256) k_aif_resolve(
             map32[p] = map_word(v, pix % W, pix / W);
         }
     }
+    __shared__ int s_c, s_o, s_m;
+    if (threadIdx.x == 0) { s_c = 0; s_o = 0; s_m = 0; }
+    __syncthreads();
     const unsigned long long mc = __ballot(chosen), mo = __ballot(over16);
     const int lane = threadIdx.x & 63;
     // wave maximum of the candidate counts
     int mx = nc;
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
     if (lane == 0) {
-        if (mc) atomicAdd(&stats[1], __popcll(mc));
-        if (mo) atomicAdd(&stats[6], __popcll(mo));
-        if (mx > 0) atomicMax(&stats[4], mx);
+        if (mc) atomicAdd(&s_c, __popcll(mc));
+        if (mo) atomicAdd(&s_o, __popcll(mo));
+        if (mx > 0) atomicMax(&s_m, mx);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_c) atomicAdd(&stats[1], s_c);
+        if (s_o) atomicAdd(&stats[6], s_o);
+        if (s_m) atomicMax(&stats[4], s_m);
     }
 }
 
@@ -203,134 +220,272 @@ __device__ __forceinline__ bool fill_from(uint32_t m, int dsx, int dsy, int W,
     return true;
 }
 
-// HoleFilling (FocalStack/LFRefocus.cpp:2362-2498) on col32 / map32 in place.  One workgroup.
-// Raster order; the pixel to the right and the one below are read as they were when the pass
-// started, the one to the left and the one above as already updated.  A hole pixel's outcome
-// depends on its left neighbour only when the right neighbour is a hole; those pixels form runs
-// that one lane walks, everything else is decided in parallel.  Stop rule per pass: fill ratio
-// >= 0.98, no fill in the pass, or more than 30000 fills in all.
-__global__ void __launch_bounds__(kHoleThreads) k_aif_holefill(
-    uint32_t* __restrict__ col32, uint32_t* __restrict__ map32, const uint8_t* __restrict__ bgr,
-    int W, int cols, int rows, int stage_kcrop, int* __restrict__ stats) {
+// HoleFilling (FocalStack/LFRefocus.cpp:2362-2498).  Raster order within a pass: a pixel reads
+// its right and lower neighbours as they were when the pass started (A) and its left and upper
+// neighbours as already updated.  Row y therefore depends on the pass-start state and on the
+// updated row y - 1, and on the latter only through the "up" option (a hole whose right, left-at-
+// visit and lower neighbours are holes).  A pass runs as a fixed-point iteration over rows:
+// iteration 1 computes every row against A for the row above; iteration t recomputes each row
+// that consulted the row above and whose row above changed since, against that row's latest
+// result.  Row 0 is final after iteration 1 and row y after iteration y + 1 at the latest; the
+// fixed point is the sequential raster result.  Each row keeps two result slots (the slot its
+// lower neighbour reads in an iteration is never the one written), a version counting its
+// changes, and the version of the row above it was computed against.  Within a row, pixels whose
+// outcome does not depend on the left neighbour are decided in parallel; the runs that do (holes
+// with a hole to the right) are walked by one lane each.
+struct HoleState {
+    const uint32_t* a_col;  // pass-start colours (B, G, R packed; 0 = hole)
+    const uint32_t* a_map;  // pass-start source words
+    uint32_t* b_col0; uint32_t* b_map0; uint32_t* b_col1; uint32_t* b_map1;  // result slots
+    int* slot;   // [2][rows] by iteration parity: 0 / 1 = result slot, 2 = the pass-start row
+    int* ver;    // [2][rows] versions
+    int* dep;    // [rows] version of the row above used; -1 = the row above was not consulted
+    int* fills;  // [rows] fills of the row's latest result
+    int* dirty;  // [iterations + 1] rows recomputed per iteration
+};
+
+constexpr int kRowThreads = 256;
+
+__global__ void __launch_bounds__(kRowThreads) k_hole_row(HoleState h,
+                                                          const uint8_t* __restrict__ bgr,
+                                                          int W, int cols, int rows,
+                                                          const int* __restrict__ stats,
+                                                          int stage_kcrop, int t) {
     extern __shared__ __align__(16) uint32_t lds[];
     uint32_t* cc = lds;             // row state: colour | flags
-    uint32_t* cm = lds + cols;      // row state: map word
+    uint32_t* cm = lds + cols;      // row state: source word
     uint32_t* kl = lds + 2 * cols;  // colours of the attribution MI's crop (when staged)
-    __shared__ int s_count, s_stop;
-    const int tid = threadIdx.x;
-    const long long N = (long long)cols * rows;
-    const int kstar = stats[5] >= 0 ? stats[5] / (W * W) : -1;
+    uint8_t* kz = (uint8_t*)(kl + W * W);  // [sy][s]: first column >= s of row sy that is black
+    __shared__ int s_go, s_changed, s_fills, s_consult;
+    const int y = blockIdx.x, tid = threadIdx.x;
+    const int rd = (t + 1) & 1, wr = t & 1;
+    const int sp = h.slot[rd * rows + y], vp = h.ver[rd * rows + y];
+    if (tid == 0) {
+        int go = t == 1;
+        if (!go && y > 0 && h.dep[y] >= 0 && h.ver[rd * rows + y - 1] != h.dep[y]) go = 1;
+        s_go = go;
+        s_changed = 0;
+        s_fills = 0;
+        s_consult = 0;
+        if (!go) {
+            h.slot[wr * rows + y] = sp;
+            h.ver[wr * rows + y] = vp;
+        }
+    }
+    __syncthreads();
+    if (!s_go) return;
+    const int kstar = stats[5];
     const uint32_t* kcrop = nullptr;
     if (stage_kcrop && kstar >= 0) {
-        for (int t = tid; t < W * W; t += kHoleThreads)
-            kl[t] = rgb_of(bgr, (long long)kstar * W * W + t);
+        for (int i = tid; i < W * W; i += kRowThreads)
+            kl[i] = rgb_of(bgr, (long long)kstar * W * W + i);
         kcrop = kl;
     }
-    if (tid == 0) s_count = 0;
     __syncthreads();
-    if (tid == 0) stats[5] = kstar;  // report the attribution MI (collection index before)
-    int local = 0;
-    for (long long p = tid; p < N; p += kHoleThreads) {
+    if (kcrop) {
+        for (int r = tid; r < W; r += kRowThreads) {
+            int z = W;
+            for (int c = W - 1; c >= 0; --c) {
+                if (kl[r * W + c] == 0) z = c;
+                kz[r * W + c] = (uint8_t)z;
+            }
+        }
+    }
+    const int su = y > 0 ? h.slot[rd * rows + y - 1] : 2;
+    const int vu = y > 0 ? h.ver[rd * rows + y - 1] : 0;
+    const uint32_t* up_col = su == 2 ? h.a_col : (su == 0 ? h.b_col0 : h.b_col1);
+    const uint32_t* up_map = su == 2 ? h.a_map : (su == 0 ? h.b_map0 : h.b_map1);
+    const long long row = (long long)y * cols;
+    __syncthreads();
+    bool consult = false;
+    for (int x = tid; x < cols; x += kRowThreads) {
+        const long long p = row + x;
+        const uint32_t c0 = h.a_col[p];
+        uint32_t oc, om = 0;
+        if (c0 != 0) {
+            oc = c0;
+            om = h.a_map[p];
+        } else if (x + 1 < cols && h.a_col[p + 1] != 0) {
+            uint32_t c, m;
+            if (fill_from(h.a_map[p + 1], -1, 0, W, bgr, kcrop, kstar, c, m)) {
+                oc = c | kChanged;
+                om = m;
+            } else {
+                oc = 0;
+            }
+        } else {
+            uint32_t c = 0, m = 0;
+            bool f = false;
+            if (y + 1 < rows && h.a_col[p + cols] != 0) {
+                f = fill_from(h.a_map[p + cols], 0, -1, W, bgr, kcrop, kstar, c, m);
+            } else if (y >= 1) {
+                consult = true;
+                if (up_col[p - cols] != 0)
+                    f = fill_from(up_map[p - cols], 0, 1, W, bgr, kcrop, kstar, c, m);
+            }
+            oc = kDep | (f ? (kDFill | c) : 0u);
+            om = f ? m : 0u;
+        }
+        cc[x] = oc;
+        cm[x] = om;
+    }
+    if (consult) s_consult = 1;
+    __syncthreads();
+    int fills = 0;
+    // runs of left-dependent pixels, one lane each.  A pixel with a non-hole left neighbour
+    // fills from the left (the source one column right of the neighbour's); a chain of such
+    // fills inside the attribution MI's crop runs along one crop row until the crop edge or a
+    // black source pixel, so it is written in one stretch from the first-black table.
+    for (int x = tid; x < cols; x += kRowThreads) {
+        if (!(cc[x] & kDep) || (x > 0 && (cc[x - 1] & kDep))) continue;
+        int x1 = x + 1;
+        while (x1 < cols && (cc[x1] & kDep)) ++x1;
+        bool lnon = x > 0 && (cc[x - 1] & kRgb) != 0;
+        uint32_t lm = x > 0 ? cm[x - 1] : 0u;
+        int xx = x;
+        while (xx < x1) {
+            if (lnon && kcrop && lm != 0 && map_mi(lm) == kstar) {
+                const int sx = map_sx(lm), sy = map_sy(lm);
+                if (sx + 1 >= W) {  // no source to the right: the pixel stays a hole
+                    cc[xx] = kDep;
+                    lnon = false;
+                    ++xx;
+                    continue;
+                }
+                const int z = kz[sy * W + sx + 1];  // first black column >= sx + 1 (W: none)
+                const int last = min(z, W - 1);     // last source column of the stretch
+                const int n = min(last - sx, x1 - xx);
+                for (int k = 0; k < n; ++k) {
+                    const int col = sx + 1 + k;
+                    cc[xx + k] = kDep | kChanged | kl[sy * W + col];
+                    cm[xx + k] = map_word(kstar, col, sy);
+                }
+                xx += n;
+                fills += n;
+                lm = map_word(kstar, sx + n, sy);
+                lnon = kl[sy * W + sx + n] != 0;
+                continue;
+            }
+            const uint32_t d = cc[xx];
+            uint32_t c = 0, m = 0;
+            bool f;
+            if (lnon) {
+                f = fill_from(lm, 1, 0, W, bgr, kcrop, kstar, c, m);
+            } else if (d & kDFill) {
+                f = true;
+                c = d & kRgb;
+                m = cm[xx];
+            } else {
+                f = false;
+            }
+            if (f) {
+                cc[xx] = kDep | kChanged | c;
+                cm[xx] = m;
+                ++fills;
+            } else {
+                cc[xx] = kDep;
+            }
+            lnon = f && c != 0;
+            lm = m;
+            ++xx;
+        }
+    }
+    __syncthreads();
+    const int ns = sp == 0 ? 1 : 0;
+    uint32_t* out_col = ns == 0 ? h.b_col0 : h.b_col1;
+    uint32_t* out_map = ns == 0 ? h.b_map0 : h.b_map1;
+    const uint32_t* prev_col = sp == 2 ? h.a_col : (sp == 0 ? h.b_col0 : h.b_col1);
+    const uint32_t* prev_map = sp == 2 ? h.a_map : (sp == 0 ? h.b_map0 : h.b_map1);
+    bool changed = false;
+    for (int x = tid; x < cols; x += kRowThreads) {
+        const long long p = row + x;
+        const uint32_t d = cc[x];
+        const bool f = (d & kChanged) != 0;
+        const uint32_t nc = f ? (d & kRgb) : h.a_col[p];
+        const uint32_t nm = f ? cm[x] : h.a_map[p];
+        out_col[p] = nc;
+        out_map[p] = nm;
+        fills += f && !(d & kDep);
+        changed |= nc != prev_col[p] || nm != prev_map[p];
+    }
+    if (changed) s_changed = 1;
+    atomicAdd(&s_fills, fills);
+    __syncthreads();
+    if (tid == 0) {
+        h.slot[wr * rows + y] = ns;
+        h.ver[wr * rows + y] = vp + (s_changed ? 1 : 0);
+        h.dep[y] = s_consult ? vu : -1;
+        h.fills[y] = s_fills;
+        atomicAdd(&h.dirty[t], 1);
+    }
+}
+
+// End of a pass: the rows' latest results become the pass-start state of the next pass.
+__global__ void __launch_bounds__(256) k_hole_gather(HoleState h, int cols, int rows, int par,
+                                                     uint32_t* __restrict__ col32,
+                                                     uint32_t* __restrict__ map32) {
+    const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (p >= (long long)cols * rows) return;
+    const int y = (int)(p / cols);
+    const int sl = h.slot[par * rows + y];
+    if (sl == 2) return;
+    col32[p] = sl == 0 ? h.b_col0[p] : h.b_col1[p];
+    map32[p] = sl == 0 ? h.b_map0[p] : h.b_map1[p];
+}
+
+// pass state: [0] pixel count, [1] fills in all passes, [2] passes, [3] stop
+__global__ void __launch_bounds__(256) k_hole_count(const uint32_t* __restrict__ col32, long long N,
+                                                    int* __restrict__ pstate) {
+    const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    bool in = false;
+    if (p < N) {
         const uint32_t c = col32[p];
-        local += ((c & 0xFFu) >= 1 && ((c >> 8) & 0xFFu) >= 1 && ((c >> 16) & 0xFFu) >= 1);
+        in = (c & 0xFFu) >= 1 && ((c >> 8) & 0xFFu) >= 1 && ((c >> 16) & 0xFFu) >= 1;
     }
-    atomicAdd(&s_count, local);
+    __shared__ int s_n;
+    if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    int pixelCount = s_count;  // thread 0's copy is the one that counts
-    if (tid == 0) stats[7] = pixelCount;
-    int counter = 0, passes = 0;
-    for (;;) {
-        if (tid == 0) s_count = 0;
-        __syncthreads();
-        int fills = 0;
-        for (int y = 0; y < rows; ++y) {
-            const long long row = (long long)y * cols;
-            // pixels decided without the left neighbour; the down / up alternative of the rest
-            for (int x = tid; x < cols; x += kHoleThreads) {
-                const long long p = row + x;
-                const uint32_t c0 = col32[p];
-                uint32_t oc, om = 0;
-                if (c0 != 0) {
-                    oc = c0;
-                    om = map32[p];
-                } else if (x + 1 < cols && col32[p + 1] != 0) {
-                    uint32_t c, m;
-                    if (fill_from(map32[p + 1], -1, 0, W, bgr, kcrop, kstar, c, m)) {
-                        oc = c | kChanged;
-                        om = m;
-                    } else {
-                        oc = 0;
-                    }
-                } else {
-                    uint32_t c = 0, m = 0;
-                    bool f = false;
-                    if (y + 1 < rows && col32[p + cols] != 0)
-                        f = fill_from(map32[p + cols], 0, -1, W, bgr, kcrop, kstar, c, m);
-                    else if (y >= 1 && col32[p - cols] != 0)
-                        f = fill_from(map32[p - cols], 0, 1, W, bgr, kcrop, kstar, c, m);
-                    oc = kDep | (f ? (kDFill | c) : 0u);
-                    om = f ? m : 0u;
-                }
-                cc[x] = oc;
-                cm[x] = om;
-            }
-            __syncthreads();
-            // runs of left-dependent pixels, one lane each
-            for (int x = tid; x < cols; x += kHoleThreads) {
-                if (!(cc[x] & kDep) || (x > 0 && (cc[x - 1] & kDep))) continue;
-                bool lnon = x > 0 && (cc[x - 1] & kRgb) != 0;
-                uint32_t lm = x > 0 ? cm[x - 1] : 0u;
-                for (int xx = x; xx < cols; ++xx) {
-                    const uint32_t d = cc[xx];
-                    if (!(d & kDep)) break;
-                    uint32_t c = 0, m = 0;
-                    bool f;
-                    if (lnon)
-                        f = fill_from(lm, 1, 0, W, bgr, kcrop, kstar, c, m);
-                    else if (d & kDFill) {
-                        f = true;
-                        c = d & kRgb;
-                        m = cm[xx];
-                    } else {
-                        f = false;
-                    }
-                    if (f) {
-                        cc[xx] = kDep | kChanged | c;
-                        cm[xx] = m;
-                        ++fills;
-                    } else {
-                        cc[xx] = kDep;
-                    }
-                    lnon = f && c != 0;
-                    lm = m;
-                }
-            }
-            __syncthreads();
-            for (int x = tid; x < cols; x += kHoleThreads) {
-                const uint32_t d = cc[x];
-                if (d & kChanged) {
-                    col32[row + x] = d & kRgb;
-                    map32[row + x] = cm[x];
-                    if (!(d & kDep)) ++fills;
-                }
-            }
-            __syncthreads();
-        }
-        atomicAdd(&s_count, fills);
-        __syncthreads();
-        if (tid == 0) {
-            const int f = s_count;
-            ++passes;
-            counter += f;
-            pixelCount += f;
-            const float ratio = static_cast<float>(pixelCount) / static_cast<float>(rows * cols);
-            s_stop = (ratio >= 0.98f) || f == 0 || counter > 30000;
-            stats[2] = passes;
-            stats[3] = counter;
-        }
-        __syncthreads();
-        if (s_stop) break;
+    const unsigned long long m = __ballot(in);
+    if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicAdd(&s_n, __popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n) atomicAdd(&pstate[0], s_n);
+}
+
+__global__ void k_hole_pass_end(const int* __restrict__ fills, int rows, int cols,
+                                int* __restrict__ pstate, int* __restrict__ stats) {
+    __shared__ int s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    int f = 0;
+    for (int y = threadIdx.x; y < rows; y += blockDim.x) f += fills[y];
+    atomicAdd(&s, f);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int fl = s;
+        pstate[0] += fl;
+        pstate[1] += fl;
+        pstate[2] += 1;
+        const float ratio = static_cast<float>(pstate[0]) / static_cast<float>(rows * cols);
+        pstate[3] = (ratio >= 0.98f) || fl == 0 || pstate[1] > 30000;
+        stats[2] = pstate[2];
+        stats[3] = pstate[1];
     }
+}
+
+__global__ void k_hole_init(int* __restrict__ slot0, int* __restrict__ ver0, int rows,
+                            int* __restrict__ dirty, int niter) {
+    for (int y = threadIdx.x; y < rows; y += blockDim.x) {
+        slot0[y] = 2;
+        ver0[y] = 0;
+    }
+    for (int i = threadIdx.x; i < niter; i += blockDim.x) dirty[i] = 0;
+}
+
+// The attribution MI (the collection index of the last candidate collected -> its MI) and the
+// pixel count before filling.
+__global__ void k_aif_kstar(int* __restrict__ stats, int WW, int* __restrict__ pstate) {
+    if (stats[5] >= 0) stats[5] /= WW;
+    stats[7] = pstate[0];
 }
 
 __global__ void __launch_bounds__(256) k_unpack_rgb(const uint32_t* __restrict__ col32, long long N,
@@ -349,11 +504,14 @@ __global__ void __launch_bounds__(256) k_split3(const uint8_t* __restrict__ in, 
 }
 
 struct AifLayout {
-    size_t win, cand, col32, map32, planes, tmp0, tmp1, total;
+    size_t win, cand, col32, map32, planes, tmp0, tmp1, b[4], slot, ver, dep, fills, dirty,
+        pstate, total;
 };
 
+constexpr int kHoleGroup = 8;  // iterations launched between two reads of their dirty counts
+
 AifLayout aif_layout(int raw_w, int raw_h) {
-    const size_t N = (size_t)raw_w * raw_h;
+    const size_t N = (size_t)raw_w * raw_h, R = (size_t)raw_h;
     AifLayout L;
     size_t o = 0;
     L.win = o; o += al(N * 4);
@@ -363,6 +521,13 @@ AifLayout aif_layout(int raw_w, int raw_h) {
     L.planes = o; o += al(N * 3);
     L.tmp0 = o; o += al(N);
     L.tmp1 = o; o += al(N);
+    for (int i = 0; i < 4; ++i) { L.b[i] = o; o += al(N * 4); }
+    L.slot = o; o += al(2 * R * 4);
+    L.ver = o; o += al(2 * R * 4);
+    L.dep = o; o += al(R * 4);
+    L.fills = o; o += al(R * 4);
+    L.dirty = o; o += al((R + 2 + kHoleGroup) * 4);
+    L.pstate = o; o += al(4 * 4);
     L.total = o;
     return L;
 }
@@ -404,7 +569,8 @@ __device__ __forceinline__ bool bp_source(float vd, int c, int r, float2 ca, flo
 __global__ void __launch_bounds__(256) k_bp_consistency(
     const float4* __restrict__ plane, const float2* __restrict__ centers,
     const int* __restrict__ nbr, const int* __restrict__ cnt, int K,
-    const uint8_t* __restrict__ garbage, int V, BpGeom g, uint32_t* __restrict__ cons) {
+    const uint8_t* __restrict__ garbage, int V, BpGeom g, uint32_t* __restrict__ cons,
+    unsigned short* __restrict__ srcs) {
 #pragma clang fp contract(off)
     const int W = g.W, WW = W * W;
     const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -436,7 +602,10 @@ __global__ void __launch_bounds__(256) k_bp_consistency(
                 const float e0 = (float)c - r0, e1 = (float)r - r1;
                 const float err = sqrtf(e0 * e0 + e1 * e1);
                 const float rel = fabsf(vd_src - vd_ref) / fmaxf(vd_ref, 1.0e-6f);
-                if (err < 1.0f && rel < 0.05f) bits |= 1u << j;
+                if (err < 1.0f && rel < 0.05f) {
+                    bits |= 1u << j;
+                    srcs[i * K + j] = (unsigned short)(sr * W + sc);
+                }
             }
         }
     }
@@ -445,16 +614,22 @@ __global__ void __launch_bounds__(256) k_bp_consistency(
 
 // One wavefront per MI of one schedule level: the reference's masked walk over the MI's pixels.
 // masks: per MI ceil(WW/32) words.  incl[a*WW + p] = the neighbours fused at pixel p (0: none).
+// Masks only remove candidates, so pixels with fewer than two consistent neighbours never fuse:
+// the walk visits only the others (in pixel order), with their source pixels computed in
+// parallel beforehand.
 __global__ void __launch_bounds__(64) k_bp_level(
     const int* __restrict__ order, int begin, const float4* __restrict__ plane,
     const float2* __restrict__ centers, const int* __restrict__ nbr, const int* __restrict__ cnt,
     int K, const uint8_t* __restrict__ garbage, BpGeom g, const uint32_t* __restrict__ cons,
-    uint32_t* __restrict__ masks, uint32_t* __restrict__ incl) {
+    const unsigned short* __restrict__ srcs, uint32_t* __restrict__ masks,
+    uint32_t* __restrict__ incl) {
     extern __shared__ __align__(16) uint32_t lds[];
     const int W = g.W, WW = W * W, MW = (WW + 31) >> 5;
-    uint32_t* cl = lds;               // cons of the MI's pixels
-    float* dl = (float*)(lds + WW);   // disparity of the MI's pixels
-    uint32_t* ml = lds + 2 * WW;      // masks: own, then neighbour j at (1 + j) * MW
+    uint32_t* ml = lds;                                  // masks: own, then entry j at (1 + j)*MW
+    int* pl = (int*)(lds + (1 + K) * MW);                // walk list: pixel indices
+    uint32_t* cl = (uint32_t*)(pl + WW);                 // consistency bits by pixel
+    unsigned short* sl = (unsigned short*)(cl + WW);     // source pixel of pixel p, neighbour j
+    __shared__ int s_nb[32], s_slot[32];
     const int a = order[begin + blockIdx.x];
     const int lane = threadIdx.x;
     const int n = min(cnt[a], K);
@@ -469,44 +644,125 @@ __global__ void __launch_bounds__(64) k_bp_level(
         const int bj = __shfl(b, j);
         if (j < slot && bj == b && b >= 0) slot = j;
     }
-    for (int t = lane; t < WW; t += 64) {
-        cl[t] = cons[(long long)a * WW + t];
-        dl[t] = plane[(long long)a * WW + t].w;
+    if (lane < n) {
+        s_nb[lane] = b;
+        s_slot[lane] = slot;
     }
-    for (int t = lane; t < MW; t += 64) ml[t] = masks[(long long)a * MW + t];
-    for (int j = 0; j < n; ++j) {
-        const int bj = __shfl(b, j), sj = __shfl(slot, j);
-        if (bj < 0 || sj != j) continue;
-        for (int t = lane; t < MW; t += 64) ml[(1 + j) * MW + t] = masks[(long long)bj * MW + t];
+    const long long base = (long long)a * WW;
+#pragma unroll 4
+    for (int t = lane; t < WW; t += 64) cl[t] = cons[base + t];
+    __syncthreads();
+    for (int q = lane; q < (1 + n) * MW; q += 64) {
+        const int e = q / MW, t = q - e * MW;
+        const int mi = e == 0 ? a : s_nb[e - 1];
+        if (mi < 0 || (e > 0 && s_slot[e - 1] != e - 1)) continue;
+        ml[q] = masks[(long long)mi * MW + t];
+    }
+    // walk list: pixels entering the walk with at least two consistent neighbours, in order
+    int len = 0;
+    const uint32_t nmask = (1u << n) - 1u;
+    for (int c0 = 0; c0 < WW; c0 += 64) {
+        const int p = c0 + lane;
+        const uint32_t cb = p < WW ? cl[p] : 0u;
+        const bool take = (cb >> 31) && __popc(cb & nmask) >= 2;
+        const unsigned long long m = __ballot(take);
+        if (take) pl[len + __popcll(m & ((1ull << lane) - 1ull))] = p;
+        len += __popcll(m);
+    }
+    // source pixels of every (pixel, consistent neighbour) from the consistency pass: the MI's
+    // block of WW * K entries, copied with 16 loads in flight per lane
+    {
+        const int total = WW * K;
+        const unsigned short* src = srcs + base * K;
+        for (int q0 = lane; q0 < total; q0 += 64 * 16) {
+            unsigned short v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int q = q0 + 64 * u;
+                v[u] = q < total ? src[q] : (unsigned short)0;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int q = q0 + 64 * u;
+                if (q < total) sl[q] = v[u];
+            }
+        }
     }
     __syncthreads();
-    const float2 ca = centers[a];
-    const float2 cb = b >= 0 ? centers[b] : make_float2(0.0f, 0.0f);
-    for (int p = 0; p < WW; ++p) {
-        const uint32_t cbits = cl[p];
-        if (!(cbits >> 31)) continue;
-        if ((ml[p >> 5] >> (p & 31)) & 1u) continue;
-        bool inc = false;
-        int s = 0;
-        if (lane < n && ((cbits >> lane) & 1u)) {
-            const float vd_ref = bp_vd(dl[p], g.mla_base);
-            int sr, sc;
-            bp_source(vd_ref, p % W, p / W, ca, cb, g, sr, sc);
-            s = sr * W + sc;
-            inc = !((ml[(1 + slot) * MW + (s >> 5)] >> (s & 31)) & 1u);
+    bool dup = false;
+    for (int j = 0; j < n; ++j) dup |= s_slot[j] != j && s_nb[j] >= 0;
+    if (MW <= 8 && !dup) {
+        // masks in registers: every lane the MI's own, lane j neighbour j's (the own mask does
+        // not change during the MI's walk); the next entry is read while this one is decided
+        uint32_t own[8], nm[8];
+#pragma unroll
+        for (int w = 0; w < 8; ++w) {
+            own[w] = w < MW ? ml[w] : 0u;
+            nm[w] = (w < MW && lane < n && b >= 0) ? ml[(1 + lane) * MW + w] : 0u;
         }
-        const unsigned long long m = __ballot(inc);
-        if (__popcll(m) >= 2) {
-            if (lane == 0) incl[(long long)a * WW + p] = (uint32_t)m;
-            if (inc) atomicOr(&ml[(1 + slot) * MW + (s >> 5)], 1u << (s & 31));
-            __syncthreads();
+        int pn = len > 0 ? pl[0] : 0;
+        uint32_t cn = len > 0 ? cl[pn] : 0u;
+        int sn = (len > 0 && lane < n) ? sl[pn * K + lane] : 0;
+        for (int i = 0; i < len; ++i) {
+            const int p = pn;
+            const uint32_t cbits = cn;
+            const int sidx = sn;
+            if (i + 1 < len) {
+                pn = pl[i + 1];
+                cn = cl[pn];
+                sn = lane < n ? sl[pn * K + lane] : 0;
+            }
+            uint32_t ow = 0;
+#pragma unroll
+            for (int w = 0; w < 8; ++w) ow = (p >> 5) == w ? own[w] : ow;
+            if ((ow >> (p & 31)) & 1u) continue;
+            bool inc = false;
+            if (lane < n && ((cbits >> lane) & 1u)) {
+                uint32_t mw = 0;
+#pragma unroll
+                for (int w = 0; w < 8; ++w) mw = (sidx >> 5) == w ? nm[w] : mw;
+                inc = !((mw >> (sidx & 31)) & 1u);
+            }
+            const unsigned long long m = __ballot(inc);
+            if (__popcll(m) >= 2) {
+                if (lane == 0) incl[base + p] = (uint32_t)m;
+                if (inc) {
+#pragma unroll
+                    for (int w = 0; w < 8; ++w)
+                        if ((sidx >> 5) == w) nm[w] |= 1u << (sidx & 31);
+                }
+            }
+        }
+        if (lane < n && b >= 0) {
+#pragma unroll
+            for (int w = 0; w < 8; ++w)
+                if (w < MW) ml[(1 + lane) * MW + w] = nm[w];
+        }
+    } else {
+        for (int i = 0; i < len; ++i) {
+            const int p = pl[i];
+            if ((ml[p >> 5] >> (p & 31)) & 1u) continue;
+            const uint32_t cbits = cl[p];
+            bool inc = false;
+            int sidx = 0;
+            if (lane < n && ((cbits >> lane) & 1u)) {
+                sidx = sl[p * K + lane];
+                inc = !((ml[(1 + slot) * MW + (sidx >> 5)] >> (sidx & 31)) & 1u);
+            }
+            const unsigned long long m = __ballot(inc);
+            if (__popcll(m) >= 2) {
+                if (lane == 0) incl[base + p] = (uint32_t)m;
+                if (inc) atomicOr(&ml[(1 + slot) * MW + (sidx >> 5)], 1u << (sidx & 31));
+                __syncthreads();
+            }
         }
     }
     __syncthreads();
-    for (int j = 0; j < n; ++j) {
-        const int bj = __shfl(b, j), sj = __shfl(slot, j);
-        if (bj < 0 || sj != j) continue;
-        for (int t = lane; t < MW; t += 64) masks[(long long)bj * MW + t] = ml[(1 + j) * MW + t];
+    for (int q = lane + MW; q < (1 + n) * MW; q += 64) {
+        const int e = q / MW, t = q - e * MW;
+        const int mi = s_nb[e - 1];
+        if (mi < 0 || s_slot[e - 1] != e - 1) continue;
+        masks[(long long)mi * MW + t] = ml[q];
     }
 }
 
@@ -591,9 +847,13 @@ __global__ void __launch_bounds__(256) k_bp_emit(
         }
     }
     if (!POINTS) {
+        __shared__ int s_n;
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
         const unsigned long long b = __ballot(fused);
-        if (b && (threadIdx.x & 63) == __ffsll((long long)b) - 1)
-            atomicAdd(&counts[1], (unsigned long long)__popcll(b));
+        if (b && (threadIdx.x & 63) == __ffsll((long long)b) - 1) atomicAdd(&s_n, __popcll(b));
+        __syncthreads();
+        if (threadIdx.x == 0 && s_n) atomicAdd(&counts[1], (unsigned long long)s_n);
     }
 }
 
@@ -625,7 +885,7 @@ __global__ void k_bp_count(const int* __restrict__ offs, const int* __restrict__
 }
 
 struct BpLayout {
-    size_t cons, masks, incl, pos, flag, offs, sums, winner, total;
+    size_t cons, srcs, masks, incl, pos, flag, offs, sums, winner, total;
 };
 
 BpLayout bp_layout(int V, int crop, int raw_w, int raw_h) {
@@ -635,6 +895,7 @@ BpLayout bp_layout(int V, int crop, int raw_w, int raw_h) {
     BpLayout L;
     size_t o = 0;
     L.cons = o; o += al(M * 4);
+    L.srcs = o; o += al(M * 31 * 2);
     L.masks = o; o += al((size_t)V * MW * 4);
     L.incl = o; o += al(M * 4);
     L.pos = o; o += al(M * 4);
@@ -693,7 +954,7 @@ int lfpm_aif_composite(const float* plane_dev, const float* centers_dev,
     if (lds_rows > 65536)
         return set_error(LFPM_E_UNSUPPORTED, "lfpm_aif_composite: frame width %d > 8192",
                          raw_w);
-    const int stage_kcrop = lds_rows + (size_t)crop * crop * 4 <= 65536 ? 1 : 0;
+    const int stage_kcrop = lds_rows + (size_t)crop * crop * 5 <= 65536 ? 1 : 0;
     hipStream_t s = (hipStream_t)stream;
     unsigned char* ws = (unsigned char*)workspace_dev;
     uint32_t* win = (uint32_t*)(ws + L.win);
@@ -713,9 +974,46 @@ int lfpm_aif_composite(const float* plane_dev, const float* centers_dev,
     k_aif_resolve<<<grid_for(N, 256), 256, 0, s>>>(
         (const float4*)plane_dev, bgr_tiles_dev, win, cand, crop, baseline, B, bl0, main_lens_f,
         N, vd_dev, inv_vd_dev, real_mm_dev, aif_dev, col32, map32, stats_dev);
-    const size_t lds = lds_rows + (stage_kcrop ? (size_t)crop * crop * 4 : 0);
-    k_aif_holefill<<<1, kHoleThreads, lds, s>>>(col32, map32, bgr_tiles_dev, crop, raw_w, raw_h,
-                                                stage_kcrop, stats_dev);
+    // HoleFilling: passes of row fixed-point iterations; the stream is synchronised to read the
+    // iterations' dirty counts and the pass state
+    const size_t lds = lds_rows + (stage_kcrop ? (size_t)crop * crop * 5 : 0);
+    HoleState h;
+    h.a_col = col32; h.a_map = map32;
+    h.b_col0 = (uint32_t*)(ws + L.b[0]); h.b_map0 = (uint32_t*)(ws + L.b[1]);
+    h.b_col1 = (uint32_t*)(ws + L.b[2]); h.b_map1 = (uint32_t*)(ws + L.b[3]);
+    h.slot = (int*)(ws + L.slot); h.ver = (int*)(ws + L.ver); h.dep = (int*)(ws + L.dep);
+    h.fills = (int*)(ws + L.fills); h.dirty = (int*)(ws + L.dirty);
+    int* pstate = (int*)(ws + L.pstate);
+    k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)pstate, 4, 0u);
+    k_hole_count<<<grid_for(N, 256), 256, 0, s>>>(col32, N, pstate);
+    k_aif_kstar<<<1, 1, 0, s>>>(stats_dev, crop * crop, pstate);
+    const int niter = raw_h + 2 + kHoleGroup;
+    std::vector<int> dirty(kHoleGroup);
+    int pst[4] = {0, 0, 0, 0};
+    for (;;) {
+        k_hole_init<<<1, 256, 0, s>>>(h.slot, h.ver, raw_h, h.dirty, niter);
+        int t = 1, fin = -1;
+        while (fin < 0) {
+            for (int g = 0; g < kHoleGroup; ++g)
+                k_hole_row<<<raw_h, kRowThreads, lds, s>>>(h, bgr_tiles_dev, crop, raw_w, raw_h,
+                                                           stats_dev, stage_kcrop, t + g);
+            if (hipMemcpyAsync(dirty.data(), h.dirty + t, kHoleGroup * sizeof(int),
+                               hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                return check_launch("lfpm_aif_composite (hole filling)");
+            for (int g = 0; g < kHoleGroup && fin < 0; ++g)
+                if (dirty[g] == 0) fin = t + g;
+            t += kHoleGroup;
+            if (fin < 0 && t > raw_h + 1)
+                return set_error(LFPM_E_HIP, "lfpm_aif_composite: hole filling did not settle");
+        }
+        k_hole_gather<<<grid_for(N, 256), 256, 0, s>>>(h, raw_w, raw_h, fin & 1, col32, map32);
+        k_hole_pass_end<<<1, 256, 0, s>>>(h.fills, raw_h, raw_w, pstate, stats_dev);
+        if (hipMemcpyAsync(pst, pstate, sizeof(pst), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return check_launch("lfpm_aif_composite (hole filling)");
+        if (pst[3]) break;
+    }
     k_unpack_rgb<<<grid_for(N, 256), 256, 0, s>>>(col32, N, filled_dev);
     // denoise: five 5x5 median passes of the composite, per channel
     k_split3<<<grid_for(N, 256), 256, 0, s>>>(aif_dev, N, planes);
@@ -807,12 +1105,14 @@ int lfpm_fuse_backproject(const float* plane_dev, const float* centers_dev,
         return set_error(LFPM_E_INVALID, "lfpm_fuse_backproject: workspace %zu < %zu bytes",
                          workspace_bytes, L.total);
     const int WW = crop * crop, MW = (WW + 31) / 32;
-    const size_t lds = (size_t)(2 * WW + (1 + max_neighbors) * MW) * 4;
+    const size_t lds = (size_t)((1 + max_neighbors) * MW + 2 * WW) * 4 +
+                       (size_t)WW * max_neighbors * 2;
     if (lds > 65536)
         return set_error(LFPM_E_UNSUPPORTED, "lfpm_fuse_backproject: %zu B of LDS", lds);
     hipStream_t s = (hipStream_t)stream;
     unsigned char* ws = (unsigned char*)workspace_dev;
     uint32_t* cons = (uint32_t*)(ws + L.cons);
+    unsigned short* srcs = (unsigned short*)(ws + L.srcs);
     uint32_t* masks = (uint32_t*)(ws + L.masks);
     uint32_t* incl = (uint32_t*)(ws + L.incl);
     int* pos = (int*)(ws + L.pos);
@@ -837,12 +1137,12 @@ int lfpm_fuse_backproject(const float* plane_dev, const float* centers_dev,
     k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)counts_dev, 4, 0u);
     k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)(flag + M), 1, 0u);
     k_bp_consistency<<<grid_for(M, 256), 256, 0, s>>>(pl, ce, nbr_dev, cnt_dev, max_neighbors,
-                                                      garbage_dev, num_mis, g, cons);
+                                                      garbage_dev, num_mis, g, cons, srcs);
     for (int l = 0; l < num_levels; ++l) {
         const int b0 = level_start[l], nb = level_start[l + 1] - b0;
         if (nb <= 0) continue;
         k_bp_level<<<nb, 64, lds, s>>>(order_dev, b0, pl, ce, nbr_dev, cnt_dev, max_neighbors,
-                                       garbage_dev, g, cons, masks, incl);
+                                       garbage_dev, g, cons, srcs, masks, incl);
     }
     k_bp_emit<false><<<grid_for(M, 256), 256, 0, s>>>(pl, ce, bgr_tiles_dev, nbr_dev,
                                                      max_neighbors, g, M, raw_w, raw_h, incl, pos,

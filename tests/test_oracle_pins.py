@@ -257,3 +257,42 @@ def test_oracle_solve_recovers_ground_truth(lfpm, orc):
     err = np.abs(o["plane"][:, 3] - wl.L["pitch"] / gtv)
     assert np.median(err) < 0.2
     assert (err < 0.5).mean() > 0.9
+
+
+@pytest.mark.parametrize("shift,pix,d,want", [
+    ((23.0, 0.0), (8, 8), 0.25, 4.75),     # samples x 5..11 -> 5.25..11.25: nearest edge 16
+    ((23.0, 0.0), (2, 8), 0.25, 0.25),     # window clipped at x = 0: sample 0.25 from the edge
+    ((23.0, 0.0), (2, 8), -0.5, 0.5),      # sample at -0.5 (outside by 0.5), next at 0.5
+    ((0.0, 23.0), (8, 8), 0.25, 4.75),     # vertical step: x is exact, y decides
+    ((0.0, 23.0), (2, 13), 0.25, 0.75),    # y samples 10..15 -> 10.25..15.25; x exact
+])
+def test_eval_margin_geo_closed_form(orc, lfpm, shift, pix, d, want):
+    """The straddle margin of a fronto-parallel hypothesis: the smallest distance of the anchor
+    or a window sample's source point (x + step * d) from a change of its inside-the-crop test,
+    axes with a zero step component not counted."""
+    rng = np.random.default_rng(3)
+    W = 16
+    ref = rng.integers(20, 200, (W, W))
+    pm = _pm(orc, W)
+    h = _hyp(lfpm, pix[1] * W + pix[0], [0, 0, 1, d])
+    _, geo, _, _ = orc.eval_margins(*_two_mi_frame(orc, ref, ref, shift=shift), pm, h)
+    assert abs(geo[0, 0] - want) < 1e-12
+
+
+def test_eval_margin_ferr_bounds_fp32(orc, lfpm):
+    """The FP32 error estimate bounds the FP32 twin's deviation from the FP64 cost on every
+    evaluation that is not a straddle, and is not vacuous (the median ratio is above 1e-4)."""
+    wl = lfpm.workload_c2(raw_w=320, raw_h=256)
+    from _helpers import cone_hypotheses
+    hyp = cone_hypotheses(lfpm, wl, 4000, 5)
+    pm = pm_to_oracle(orc, wl.pm)
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    L = wl.L
+    c64, geo, gate, ferr = orc.eval_margins(tiles, L["centers"], L["nbr"], L["cnt"], pm, hyp)
+    c32 = orc.eval_costs(tiles, L["centers"], L["nbr"], L["cnt"], pm, hyp, dtype=np.float32)
+    e = np.abs(c32.astype(np.float64) - c64)
+    ok = (geo >= 1e-4) & (gate >= 1) & (c64 < 2)
+    assert ok.sum() > 10000
+    assert np.all(e[ok] <= ferr[ok]), float(np.max(e[ok] / np.maximum(ferr[ok], 1e-300)))
+    ratio = e[ok & (e > 0)] / ferr[ok & (e > 0)]
+    assert np.median(ratio) > 1e-4
