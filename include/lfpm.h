@@ -644,15 +644,23 @@ int lfpm_graph_launch(lfpm_graph* g, void* stream);
 void lfpm_graph_destroy(lfpm_graph* g);
 
 /* One continuous-batching step: every frame advances by its own stage (stages[i], iters[i],
- * phases[i]); LFPM_STAGE_PHASE reads A and writes B for phase 0, B to A for phase 1. */
+ * phases[i]); LFPM_STAGE_PHASE reads A and writes B for phase 0, B to A for phase 1.  With
+ * several parameter classes, each class runs on a stream of its own (forked from `stream` and
+ * joined back into it before the call returns), so their launches overlap. */
 int lfpm_batch_step(const lfpm_batch_frame* frames, int n, const int* stages, const int* iters,
                     const int* phases, void* workspace_dev, size_t workspace_bytes, void* stream);
 
-/* Autotuner: time k_stage phase launches of this frame's parameter class at each persistent
- * occupancy in {2,4,6,8} workgroups per CU, keep the fastest for the class (process-wide), and
- * return it.  Results do not depend on the choice.  Uses and overwrites B state of `frame`. */
+/* Autotuner (the reference fixes one 16x16 block shape, MIStereo/AdaptMIPMFrame.cu:957-966):
+ * time phase launches of this frame's parameter class (crop, neighbours, radius, tile storage,
+ * blur term) at every launch shape of {1, 2, 4} wavefronts per workgroup (the threads that share
+ * one MI's staged tiles, which sets the workgroup's LDS footprint) x {2, 4, 6, 8} persistent
+ * workgroups per CU, keep the fastest for the class (process-wide), and return its workgroups
+ * per CU.  Results do not depend on the choice.  Uses and overwrites B state of `frame`.
+ * lfpm_autotune_get returns the launch shape a class uses (the defaults, 8 and 4, until tuned);
+ * it has no reference counterpart. */
 int lfpm_autotune_class(const lfpm_batch_frame* frame, void* workspace_dev,
                         size_t workspace_bytes, int* best_per_cu, float* best_ms);
+int lfpm_autotune_get(const lfpm_pm_params* params, int* per_cu, int* waves_per_mi);
 
 /* Work accounting for roofline reports (no reference counterpart).  While counters_dev (two
  * device uint64: evaluated NCC window samples, evaluated (pixel, view, hypothesis) triples) is

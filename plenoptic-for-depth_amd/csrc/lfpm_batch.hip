@@ -162,6 +162,40 @@ int run_class(const lfpm_batch_frame* fr, const ClassPlan& c, unsigned char* ws,
     return LFPM_OK;
 }
 
+// Streams on which lfpm_batch_step runs the parameter classes of a step side by side, one pool
+// per host thread and device (default priority, non-blocking), with the fork and join events.
+constexpr int kStepStreams = 4;
+struct StepStreams {
+    int device = -1;
+    hipStream_t s[kStepStreams] = {};
+    hipEvent_t done[kStepStreams] = {};
+    hipEvent_t fork = nullptr;
+    ~StepStreams() {
+        for (int i = 0; i < kStepStreams; ++i) {
+            if (done[i]) (void)hipEventDestroy(done[i]);
+            if (s[i]) (void)hipStreamDestroy(s[i]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+    }
+};
+thread_local std::vector<StepStreams> t_step_streams;
+
+StepStreams* step_streams() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    for (auto& p : t_step_streams)
+        if (p.device == dev) return &p;
+    t_step_streams.emplace_back();
+    StepStreams& p = t_step_streams.back();
+    p.device = dev;
+    if (hipEventCreateWithFlags(&p.fork, hipEventDisableTiming) != hipSuccess) return nullptr;
+    for (int i = 0; i < kStepStreams; ++i)
+        if (hipStreamCreateWithFlags(&p.s[i], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&p.done[i], hipEventDisableTiming) != hipSuccess)
+            return nullptr;
+    return &p;
+}
+
 }  // namespace
 
 struct lfpm_graph {
@@ -285,7 +319,20 @@ int lfpm_batch_step(const lfpm_batch_frame* frames, int n, const int* stages, co
         if (stages[i] == LFPM_STAGE_PHASE && (phases[i] < 0 || phases[i] > 1 || iters[i] < 0))
             return set_error(LFPM_E_INVALID, "step: frame %d bad iter/phase", i);
     }
-    for (auto& c : classes) {
+    // several parameter classes: each on a stream of its own after a fork from `stream`, joined
+    // back into `stream` at the end, so their launches overlap on the device
+    StepStreams* ps = classes.size() > 1 ? step_streams() : nullptr;
+    const int nstreams = ps ? std::min((int)classes.size(), kStepStreams) : 0;
+    if (ps) {
+        if (hipEventRecord(ps->fork, (hipStream_t)stream) != hipSuccess)
+            return set_error(LFPM_E_HIP, "step: fork event");
+        for (int i = 0; i < nstreams; ++i)
+            if (hipStreamWaitEvent(ps->s[i], ps->fork, 0) != hipSuccess)
+                return set_error(LFPM_E_HIP, "step: fork wait");
+    }
+    for (size_t ci = 0; ci < classes.size(); ++ci) {
+        auto& c = classes[ci];
+        void* cs = ps ? (void*)ps->s[ci % kStepStreams] : stream;
         std::vector<StageFrame> sf;
         for (int i : c.frames) {
             const int st = stages[i] == LFPM_STAGE_FINAL && !frames[i].disp_dev ? LFPM_STAGE_IDLE
@@ -293,11 +340,17 @@ int lfpm_batch_step(const lfpm_batch_frame* frames, int n, const int* stages, co
             sf.push_back(stage_frame(frames[i], st, iters[i], phases[i]));
         }
         if ((rc = lfpm::stage_launch(c.p, sf.data(), (int)sf.size(),
-                                     (unsigned char*)ws + c.scratch_off, c.scratch_bytes, stream)))
+                                     (unsigned char*)ws + c.scratch_off, c.scratch_bytes, cs)))
             return rc;
         for (int i : c.frames)
-            if (stages[i] == LFPM_STAGE_FINAL && (rc = frame_products(frames[i], stream)))
+            if (stages[i] == LFPM_STAGE_FINAL && (rc = frame_products(frames[i], cs)))
                 return rc;
+    }
+    if (ps) {
+        for (int i = 0; i < nstreams; ++i)
+            if (hipEventRecord(ps->done[i], ps->s[i]) != hipSuccess ||
+                hipStreamWaitEvent((hipStream_t)stream, ps->done[i], 0) != hipSuccess)
+                return set_error(LFPM_E_HIP, "step: join");
     }
     return LFPM_OK;
 }
@@ -306,44 +359,61 @@ int lfpm_autotune_class(const lfpm_batch_frame* frame, void* ws, size_t ws_bytes
                         int* best_per_cu, float* best_ms) {
     if (!frame) return set_error(LFPM_E_INVALID, "tune: null frame");
     const lfpm_pm_params* p = frame->params;
-    const int choices[4] = {2, 4, 6, 8};
+    // launch shapes: wavefronts per workgroup (threads sharing one MI's staged tiles, and with
+    // them the workgroup's LDS footprint) x persistent workgroups per CU
+    const int waves[3] = {1, 2, 4};
+    const int per_cu[4] = {2, 4, 6, 8};
     hipStream_t s;
     B_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     hipEvent_t e0, e1;
     B_TRY(hipEventCreate(&e0));
     B_TRY(hipEventCreate(&e1));
-    int best = -1, rc = LFPM_OK;
+    int best = -1, bestw = -1, rc = LFPM_OK;
     float bms = 0.0f;
-    const int prev = lfpm::stage_slots_per_cu(p);
-    for (int ci = 0; ci < 4 && rc == LFPM_OK; ++ci) {
-        lfpm::set_stage_slots_per_cu(p, choices[ci]);
-        const size_t need = lfpm::stage_scratch_size(p, frame->num_mis);
-        if (!ws || ws_bytes < need) {
-            rc = set_error(LFPM_E_INVALID, "tune: workspace too small (%zu < %zu)", ws_bytes, need);
-            break;
+    const int prev = lfpm::stage_slots_per_cu(p), prevw = lfpm::stage_waves(p);
+    for (int wi = 0; wi < 3 && rc == LFPM_OK; ++wi)
+        for (int ci = 0; ci < 4 && rc == LFPM_OK; ++ci) {
+            lfpm::set_stage_config(p, per_cu[ci], waves[wi]);
+            const size_t need = lfpm::stage_scratch_size(p, frame->num_mis);
+            if (!ws || ws_bytes < need) {
+                rc = set_error(LFPM_E_INVALID, "tune: workspace too small (%zu < %zu)", ws_bytes,
+                               need);
+                break;
+            }
+            StageFrame sf = stage_frame(*frame, lfpm::kStagePhase, 1, 0);
+            // warm-up launch, then one timed launch
+            for (int rep = 0; rep < 2 && rc == LFPM_OK; ++rep) {
+                if (rep == 1 && hipEventRecord(e0, s) != hipSuccess) rc = LFPM_E_HIP;
+                if (rc == LFPM_OK) rc = lfpm::stage_launch(p, &sf, 1, ws, ws_bytes, s);
+                if (rep == 1 && rc == LFPM_OK && hipEventRecord(e1, s) != hipSuccess)
+                    rc = LFPM_E_HIP;
+            }
+            if (rc == LFPM_OK && hipEventSynchronize(e1) != hipSuccess) rc = LFPM_E_HIP;
+            float ms = 0.0f;
+            if (rc == LFPM_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = LFPM_E_HIP;
+            if (rc == LFPM_OK && (best < 0 || ms < bms)) {
+                best = per_cu[ci];
+                bestw = waves[wi];
+                bms = ms;
+            }
         }
-        StageFrame sf = stage_frame(*frame, lfpm::kStagePhase, 1, 0);
-        // warm-up launch, then one timed launch
-        for (int rep = 0; rep < 2 && rc == LFPM_OK; ++rep) {
-            if (rep == 1 && hipEventRecord(e0, s) != hipSuccess) rc = LFPM_E_HIP;
-            if (rc == LFPM_OK) rc = lfpm::stage_launch(p, &sf, 1, ws, ws_bytes, s);
-            if (rep == 1 && rc == LFPM_OK && hipEventRecord(e1, s) != hipSuccess) rc = LFPM_E_HIP;
-        }
-        if (rc == LFPM_OK && hipEventSynchronize(e1) != hipSuccess) rc = LFPM_E_HIP;
-        float ms = 0.0f;
-        if (rc == LFPM_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = LFPM_E_HIP;
-        if (rc == LFPM_OK && (best < 0 || ms < bms)) { best = choices[ci]; bms = ms; }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(s);
     if (rc) {
-        lfpm::set_stage_slots_per_cu(p, prev);
+        lfpm::set_stage_config(p, prev, prevw);
         return rc == LFPM_E_HIP ? set_error(LFPM_E_HIP, "tune: HIP event failure") : rc;
     }
-    lfpm::set_stage_slots_per_cu(p, best);
+    lfpm::set_stage_config(p, best, bestw);
     if (best_per_cu) *best_per_cu = best;
     if (best_ms) *best_ms = bms;
+    return LFPM_OK;
+}
+
+int lfpm_autotune_get(const lfpm_pm_params* params, int* per_cu, int* waves_per_mi) {
+    if (!params) return set_error(LFPM_E_INVALID, "tune: null params");
+    if (per_cu) *per_cu = lfpm::stage_slots_per_cu(params);
+    if (waves_per_mi) *waves_per_mi = lfpm::stage_waves(params);
     return LFPM_OK;
 }
 

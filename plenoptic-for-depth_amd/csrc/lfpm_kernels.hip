@@ -2195,10 +2195,11 @@ static FrameDesc frame_desc(const float* tiles, const float* centers, const int*
     return f;
 }
 
-// Threads per workgroup of the wave engine: 4 wavefronts, or fewer when the LDS of 4 would pass
-// 64 KiB (the blur term's second set of tiles with many views and a wide window).
-static int wave_threads(const DevParams& P) {
-    for (int nw = 4; nw > 1; nw /= 2)
+// Threads per workgroup of the wave engine: `waves` wavefronts (1, 2 or 4; the autotuned value
+// of the parameter class), or fewer when their LDS would pass 64 KiB (the blur term's second
+// set of tiles with many views and a wide window).
+static int wave_threads(const DevParams& P, int waves) {
+    for (int nw = std::min(std::max(waves, 1), 4); nw > 1; nw /= 2)
         if (wv::lds_total(P, nw) <= 65536) return 64 * nw;
     return 64;
 }
@@ -2233,8 +2234,8 @@ __global__ void k_zero_tickets(unsigned* __restrict__ t) {
 }
 
 // Launch k_stage_w over the frames of `a` (mi_begin/total filled here).
-static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_bytes,
-                        hipStream_t s) {
+static int launch_stage(StageArgs& a, int per_cu, int waves, void* scratch,
+                        size_t scratch_bytes, hipStream_t s) {
     int total = 0;
     for (int i = 0; i < a.nf; ++i) {
         a.f[i].mi_begin = total;
@@ -2242,7 +2243,7 @@ static int launch_stage(StageArgs& a, int per_cu, void* scratch, size_t scratch_
     }
     a.total = total;
     if (total <= 0) return LFPM_OK;
-    const int NT = wave_threads(a.P);
+    const int NT = wave_threads(a.P, waves);
     const size_t lds = wv::lds_total(a.P, NT / 64);
     if (lds > 65536)
         return set_error(LFPM_E_UNSUPPORTED, "k_stage_w: %zu B of LDS per workgroup (crop %d, "
@@ -2269,25 +2270,47 @@ static size_t state_bytes(const DevParams& P) {
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Autotuned k_stage_w occupancy per parameter class (crop, neighbours, radius).
+// Autotuned k_stage_w launch shape per parameter class (crop, neighbours, radius, tile format,
+// blur term): persistent workgroups per CU and wavefronts per workgroup (the threads that share
+// one MI's staged tiles).  Process-wide, guarded for callers on several host threads.
 struct TuneKey {
-    int crop, K, r;
-    bool operator==(const TuneKey& o) const { return crop == o.crop && K == o.K && r == o.r; }
+    int crop, K, r, storage, blur;
+    bool operator==(const TuneKey& o) const {
+        return crop == o.crop && K == o.K && r == o.r && storage == o.storage && blur == o.blur;
+    }
 };
-static std::vector<std::pair<TuneKey, int>> g_tuned;
+struct TuneVal {
+    int per_cu, waves;
+};
+static std::vector<std::pair<TuneKey, TuneVal>> g_tuned;
+static std::mutex g_tuned_mu;
+static const int kDefaultWaves = 4;
 
-int stage_slots_per_cu(const lfpm_pm_params* p) {
-    const TuneKey k{p->crop, p->max_neighbors, p->radius};
+static TuneKey tune_key(const lfpm_pm_params* p) {
+    return TuneKey{p->crop, p->max_neighbors, p->radius, p->storage, p->blur_lambda > 0.0f};
+}
+
+static TuneVal stage_config(const lfpm_pm_params* p) {
+    const TuneKey k = tune_key(p);
+    std::lock_guard<std::mutex> g(g_tuned_mu);
     for (auto& e : g_tuned)
         if (e.first == k) return e.second;
-    return kPhasePerCu;
+    return TuneVal{kPhasePerCu, kDefaultWaves};
+}
+
+int stage_slots_per_cu(const lfpm_pm_params* p) { return stage_config(p).per_cu; }
+int stage_waves(const lfpm_pm_params* p) { return stage_config(p).waves; }
+
+void set_stage_config(const lfpm_pm_params* p, int per_cu, int waves) {
+    const TuneKey k = tune_key(p);
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    for (auto& e : g_tuned)
+        if (e.first == k) { e.second = TuneVal{per_cu, waves}; return; }
+    g_tuned.push_back({k, TuneVal{per_cu, waves}});
 }
 
 void set_stage_slots_per_cu(const lfpm_pm_params* p, int per_cu) {
-    const TuneKey k{p->crop, p->max_neighbors, p->radius};
-    for (auto& e : g_tuned)
-        if (e.first == k) { e.second = per_cu; return; }
-    g_tuned.push_back({k, per_cu});
+    set_stage_config(p, per_cu, stage_waves(p));
 }
 
 float* stage_records(const lfpm_pm_params* p, void* scratch, int mi_offset) {
@@ -2322,7 +2345,7 @@ int stage_launch(const lfpm_pm_params* p, const StageFrame* frames, int n, void*
             total += s.V;
         }
         a.P = make_dev_params(p, total);
-        if ((rc = launch_stage(a, stage_slots_per_cu(p), scratch, scratch_bytes,
+        if ((rc = launch_stage(a, stage_slots_per_cu(p), stage_waves(p), scratch, scratch_bytes,
                                (hipStream_t)stream)))
             return rc;
     }
@@ -2440,7 +2463,7 @@ int lfpm_patchmatch_init(const float* tiles_dev, const float* centers_dev,
     a.f[0].stage = STAGE_INIT;
     a.f[0].plane_out = (float4*)plane_dev; a.f[0].cost_out = cost_dev; a.f[0].sel_out = sel_dev;
     a.f[0].cache = step_records(a.P, workspace_dev, workspace_bytes);
-    return launch_stage(a, stage_slots_per_cu(p), workspace_dev, workspace_bytes,
+    return launch_stage(a, stage_slots_per_cu(p), stage_waves(p), workspace_dev, workspace_bytes,
                         (hipStream_t)stream);
 }
 
@@ -2470,7 +2493,7 @@ int lfpm_patchmatch_phase(const float* tiles_dev, const float* centers_dev,
     f.plane_in = (const float4*)plane_in_dev; f.cost_in = cost_in_dev; f.sel_in = sel_in_dev;
     f.plane_out = (float4*)plane_out_dev; f.cost_out = cost_out_dev; f.sel_out = sel_out_dev;
     f.cache = step_records(a.P, workspace_dev, workspace_bytes);
-    return launch_stage(a, stage_slots_per_cu(p), workspace_dev, workspace_bytes,
+    return launch_stage(a, stage_slots_per_cu(p), stage_waves(p), workspace_dev, workspace_bytes,
                         (hipStream_t)stream);
 }
 
@@ -2493,7 +2516,8 @@ static int finalize_impl(const float* tiles_dev, const float* centers_dev,
     a.f[0].stage = STAGE_FINAL;
     a.f[0].plane_in = (const float4*)plane_dev; a.f[0].disp = (float4*)disp_dev;
     a.f[0].cache = step_records(a.P, scratch, scratch_bytes);
-    return launch_stage(a, stage_slots_per_cu(p), scratch, scratch_bytes, (hipStream_t)stream);
+    return launch_stage(a, stage_slots_per_cu(p), stage_waves(p), scratch, scratch_bytes,
+                        (hipStream_t)stream);
 }
 
 int lfpm_patchmatch_finalize(const float* tiles_dev, const float* centers_dev,
@@ -2711,7 +2735,7 @@ static int acmm_level_solve(const float* tiles, const float* centers, const int*
     {
         StageArgs a = base_desc(STAGE_INIT);
         a.f[0].plane_out = (float4*)plane; a.f[0].cost_out = cost; a.f[0].sel_out = sel;
-        if ((rc = launch_stage(a, stage_slots_per_cu(q), nullptr, 0, s))) return rc;
+        if ((rc = launch_stage(a, stage_slots_per_cu(q), stage_waves(q), nullptr, 0, s))) return rc;
     }
     for (int it = 0; it < q->iterations; ++it)
         for (int ph = 0; ph < 2; ++ph) {
@@ -2724,14 +2748,14 @@ static int acmm_level_solve(const float* tiles, const float* centers, const int*
             f.plane_out = (float4*)(ph == 0 ? plane_b : plane);
             f.cost_out = ph == 0 ? cost_b : cost;
             f.sel_out = ph == 0 ? sel_b : sel;
-            if ((rc = launch_stage(a, stage_slots_per_cu(q), nullptr, 0, s))) return rc;
+            if ((rc = launch_stage(a, stage_slots_per_cu(q), stage_waves(q), nullptr, 0, s))) return rc;
         }
     StageArgs a = base_desc(STAGE_FINAL);
     a.f[0].plane_in = (const float4*)plane;
     a.f[0].disp = (float4*)disp;
     a.f[0].cost_out = cost;
     a.f[0].sel_out = sel;
-    return launch_stage(a, stage_slots_per_cu(q), nullptr, 0, s);
+    return launch_stage(a, stage_slots_per_cu(q), stage_waves(q), nullptr, 0, s);
 }
 
 struct AcmmWs {
@@ -2842,7 +2866,7 @@ static int eval_costs_with(const float* tiles_dev, const float* centers_dev,
     DevParams P = make_dev_params(p, num_mis);
     const int grid = persistent_grid(P, kPhasePerCu);
     P.slots_per_xcd = grid / 8;
-    const int NT = wave_threads(P);
+    const int NT = wave_threads(P, kDefaultWaves);
     const size_t lds = wv::lds_total(P, NT / 64);
     if (lds > 65536)
         return set_error(LFPM_E_UNSUPPORTED, "eval: %zu B of LDS per workgroup exceeds 64 KiB",

@@ -962,3 +962,11 @@ class Batch:
         _check(lib().lfpm_autotune_class(a1, _dp(self.ws), C.c_size_t(self.ws_bytes),
                                          C.byref(best), C.byref(ms)))
         return best.value, ms.value
+
+
+def autotune_get(pm):
+    """Launch shape the solver uses for pm's parameter class: (workgroups per CU, wavefronts
+    per workgroup) (lfpm_autotune_get)."""
+    per_cu, waves = C.c_int(), C.c_int()
+    _check(lib().lfpm_autotune_get(C.byref(pm), C.byref(per_cu), C.byref(waves)))
+    return per_cu.value, waves.value

@@ -116,11 +116,15 @@ def test_serving_steps_staggered(lfpm, torch_cuda, setup):
 
 
 def test_autotune_keeps_results(lfpm, torch_cuda, setup):
+    """The autotuner searches wavefronts per workgroup x workgroups per CU; the solve under the
+    chosen launch shape equals the one under the default shape bit for bit."""
     slots, ref = setup
     b = lfpm.Batch(slots[:1])
     best, ms = b.autotune(0)
-    _say("autotune", best, ms)
+    shape = lfpm.autotune_get(slots[0].wl.pm)
+    _say("autotune", best, ms, "shape", shape)
     assert best in (2, 4, 6, 8) and ms > 0
+    assert shape[0] == best and shape[1] in (1, 2, 4)
     again = _plain(lfpm, torch_cuda, slots[0])
     _same(ref[0], again, "after tune")
 
