@@ -2066,6 +2066,167 @@ __global__ void __launch_bounds__(256) k_refocus(const float* __restrict__ tiles
     out[((size_t)k * out_h + y) * out_w + x] = nacc ? acc / (float)nacc : 0.0f;
 }
 
+// Refocus, one workgroup per 16 x 16 output tile and all slices.  The candidate MIs of the tile
+// (the lattice cells whose centre can reach it at the largest slice) are listed once, in
+// lattice order.  Per slice the workgroup keeps the candidates whose projected crop can touch the
+// tile and tabulates, for each of them, the projection of the tile's 16 columns and 16 rows
+// (the inclusion test and the bilinear cell and weight per axis, from the same correctly rounded
+// quotient as k_refocus); every pixel then sums its candidates in lattice order, so the output
+// equals k_refocus's.  The host bounds the candidates of a tile (at most kRefocusCands) and
+// sizes the LDS by that bound.
+constexpr int kRefocusCands = 160;
+
+struct RefocusAxis {
+    int cell;    // floor(u), -1 when u is outside [0, W - 1]
+    float frac;  // u - floor(u)
+};
+
+__global__ void __launch_bounds__(256) k_refocus_tile(const float* __restrict__ tiles,
+                                                      const float2* __restrict__ centers,
+                                                      const int* __restrict__ l2m, int cols,
+                                                      int rows, float pitch, int W, int out_w,
+                                                      int out_h, int S, float vmin, float vmax,
+                                                      int cap, float* __restrict__ out) {
+    // LDS sized by the host's bound `cap` on the candidates of a tile
+    extern __shared__ __align__(16) unsigned char rf_lds[];
+    RefocusAxis* s_tab = (RefocusAxis*)rf_lds;              // [cap][32]
+    float2* s_c = (float2*)(s_tab + (size_t)cap * 32);      // [cap]
+    int* s_id = (int*)(s_c + cap);                          // [cap]
+    int* s_live = s_id + cap;                               // [cap]
+    __shared__ int s_n, s_nl;
+    // workgroups are dealt to the 8 XCDs in turn: give each XCD a contiguous run of tiles (row
+    // major), so that the MI tiles its workgroups read stay in that XCD's L2
+    const int tiles_x = (out_w + 15) / 16, ntiles = tiles_x * ((out_h + 15) / 16);
+    const int per_xcd = (ntiles + 7) / 8;
+    const int tile = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (tile >= ntiles) return;  // uniform across the workgroup, before any barrier
+    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    const int X0 = (tile % tiles_x) * 16, Y0 = (tile / tiles_x) * 16;
+    const int x = X0 + tx, y = Y0 + ty;
+    const float halff = (float)(W - 1) * 0.5f;
+    auto quot = [](float a, float b, float rb) {
+        const float q0 = a * rb;
+        return fmaf(fmaf(-q0, b, a), rb, q0);
+    };
+    auto slice_v = [&](int k) {
+        return S > 1 ? vmin + (vmax - vmin) * (float)k / (float)(S - 1) : vmin;
+    };
+    const float rs = pitch * 0.8660254037844386f;
+    const float rrs = 1.0f / rs, rpitch = 1.0f / pitch;
+    // candidates: every lattice cell a pixel of the tile enumerates at any slice (the reach
+    // grows with the slice's virtual depth), in lattice order; the row ranges first, then one
+    // wavefront reads the cells' MIs and compacts them in order
+    __shared__ int s_j0, s_nrow, s_i0[32], s_cnt[33];
+    if (t == 0) {
+        float vtop = vmin;
+        for (int k = 0; k < S; ++k) vtop = fmaxf(vtop, slice_v(k));
+        const float reach = vtop * halff + 1.0f;
+        const int j0 = max(0, (int)floorf(quot((float)Y0 - reach - 0.5f * pitch, rs, rrs)));
+        const int j1 = min(rows - 1,
+                           (int)ceilf(quot((float)(Y0 + 15) + reach - 0.5f * pitch, rs, rrs)));
+        const int nrow = min(max(j1 - j0 + 1, 0), 32);
+        s_j0 = j0;
+        s_nrow = nrow;
+        s_cnt[0] = 0;
+        for (int r = 0; r < nrow; ++r) {
+            const int j = j0 + r;
+            const float off = 0.5f * pitch * (float)(1 + (j & 1));
+            const int i0 = max(0, (int)floorf(quot((float)X0 - reach - off, pitch, rpitch)));
+            const int i1 = min(cols - 1,
+                               (int)ceilf(quot((float)(X0 + 15) + reach - off, pitch, rpitch)));
+            s_i0[r] = i0;
+            s_cnt[r + 1] = s_cnt[r] + max(i1 - i0 + 1, 0);
+        }
+    }
+    __syncthreads();
+    if (t < 64) {
+        const int nrow = s_nrow, ncell = s_cnt[nrow];
+        int nn = 0;
+        for (int q0 = 0; q0 < ncell; q0 += 64) {
+            const int q = q0 + t;
+            int v = -1;
+            if (q < ncell) {
+                int r = 0;
+                while (s_cnt[r + 1] <= q) ++r;
+                v = l2m[(s_j0 + r) * cols + s_i0[r] + (q - s_cnt[r])];
+            }
+            const bool keep = v >= 0;
+            const uint64_t bal = __ballot(keep);
+            const int at = nn + __popcll(bal & ((1ull << t) - 1ull));
+            if (keep && at < cap) {
+                s_id[at] = v;
+                s_c[at] = centers[v];
+            }
+            nn += __popcll(bal);
+        }
+        if (t == 0) s_n = min(nn, cap);
+    }
+    __syncthreads();
+    const int n = s_n;
+    const int WW = W * W;
+    for (int k = 0; k < S; ++k) {
+        const float vk = slice_v(k);
+        const float rvk = 1.0f / vk;
+        // candidates whose projected crop [c - halff vk, c + halff vk] (with 1 px of slack) meets
+        // the tile on both axes, kept in lattice order
+        if (t < 64) {
+            int nl = 0;
+            const float ext = halff * vk + 1.0f;
+            for (int c0 = 0; c0 < n; c0 += 64) {
+                const int c = c0 + t;
+                bool keep = false;
+                if (c < n) {
+                    const float2 cc = s_c[c];
+                    keep = cc.x + ext >= (float)X0 && cc.x - ext <= (float)(X0 + 15) &&
+                           cc.y + ext >= (float)Y0 && cc.y - ext <= (float)(Y0 + 15);
+                }
+                const uint64_t bal = __ballot(keep);
+                if (keep) s_live[nl + __popcll(bal & ((1ull << t) - 1ull))] = c;
+                nl += __popcll(bal);
+            }
+            if (t == 0) s_nl = nl;
+        }
+        __syncthreads();
+        const int nl = s_nl;
+        // projection tables of the kept candidates: entry r < 16 is column X0 + r, else row
+        // Y0 + r - 16
+        for (int e = t; e < nl * 32; e += 256) {
+            const int li = e >> 5, r = e & 31;
+            const float2 cc = s_c[s_live[li]];
+            const float a = r < 16 ? (float)(X0 + r) - cc.x : (float)(Y0 + r - 16) - cc.y;
+            const float u = quot(a, vk, rvk) + halff;
+            RefocusAxis ax;
+            if (u >= 0.0f && u <= (float)(W - 1)) {
+                const float f0 = floorf(u);
+                ax.cell = (int)f0;
+                ax.frac = u - f0;
+            } else {
+                ax.cell = -1;
+                ax.frac = 0.0f;
+            }
+            s_tab[li * 32 + r] = ax;
+        }
+        __syncthreads();
+        float acc = 0.0f;
+        int nacc = 0;
+        for (int li = 0; li < nl; ++li) {
+            const RefocusAxis ax = s_tab[li * 32 + tx], ay = s_tab[li * 32 + 16 + ty];
+            if (ax.cell < 0 || ay.cell < 0) continue;
+            const int x0 = ax.cell, y0 = ay.cell;
+            const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, W - 1);
+            const float* tl = tiles + (size_t)s_id[s_live[li]] * WW;
+            const float a = tl[y0 * W + x0], b = tl[y0 * W + x1];
+            const float c2 = tl[y1 * W + x0], d = tl[y1 * W + x1];
+            const float top = fmaf(ax.frac, b - a, a), bot = fmaf(ax.frac, d - c2, c2);
+            acc += fmaf(ay.frac, bot - top, top);
+            ++nacc;
+        }
+        if (x < out_w && y < out_h)
+            out[((size_t)k * out_h + y) * out_w + x] = nacc ? acc / (float)nacc : 0.0f;
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // V2R behavioural model fit, FP64: Gram-matrix partials, then a fixed-order sum and a
 // column-scaled 3x3 solve with partial pivoting on one lane.
@@ -3002,6 +3163,24 @@ int lfpm_refocus(const float* tiles_dev, const float* centers_dev, const int* la
         lattice_rows <= 0 || !(pitch > 0.0f) || crop < 2 || out_w <= 0 || out_h <= 0 ||
         num_slices <= 0 || !(vmin > 0.0f) || !(vmax >= vmin))
         return set_error(LFPM_E_INVALID, "refocus: bad arguments");
+    // the tile kernel when every tile's candidate list fits: at most (rows spanned) x (columns
+    // spanned) lattice cells for the largest reach
+    const float vtop = std::max(vmin, vmax);
+    const double reach = (double)vtop * (crop - 1) * 0.5 + 1.0;
+    const double span = 16.0 + 2.0 * reach;
+    const double nrows = std::ceil(span / (pitch * 0.8660254037844386)) + 2.0;
+    const double ncand = nrows * (std::ceil(span / pitch) + 2.0);
+    if (ncand <= kRefocusCands && nrows <= 32) {
+        const int cap = (int)ncand;
+        const size_t lds = (size_t)cap * (32 * sizeof(RefocusAxis) + sizeof(float2) + 8);
+        const int ntiles = ((out_w + 15) / 16) * ((out_h + 15) / 16);
+        dim3 grid(8 * ((ntiles + 7) / 8));
+        hipLaunchKernelGGL(k_refocus_tile, grid, dim3(256), lds, (hipStream_t)stream, tiles_dev,
+                           (const float2*)centers_dev, lattice_to_mi_dev, lattice_cols,
+                           lattice_rows, pitch, crop, out_w, out_h, num_slices, vmin, vmax, cap,
+                           out_dev);
+        return launch_check("refocus");
+    }
     dim3 grid((out_w + 15) / 16, (out_h + 15) / 16, num_slices);
     hipLaunchKernelGGL(k_refocus, grid, dim3(256), 0, (hipStream_t)stream, tiles_dev,
                        (const float2*)centers_dev, lattice_to_mi_dev, lattice_cols, lattice_rows,
