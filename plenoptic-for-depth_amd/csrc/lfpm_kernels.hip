@@ -1874,7 +1874,11 @@ __device__ float f_photo(const FilterArgs& a, int v, int n, float px, float py, 
 __global__ void __launch_bounds__(1024) k_filter(FilterArgs a) {
 #pragma clang fp contract(off)
     __shared__ uint8_t keep1[LFPM_MAX_CROP * LFPM_MAX_CROP];
-    const int v = blockIdx.x;
+    // workgroups are dealt to the 8 XCDs in turn: each XCD takes a contiguous MI range, so the
+    // neighbour states an MI reads are mostly read by the same XCD (one L2)
+    const int per_xcd = (a.V + 7) / 8;
+    const int v = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (v >= a.V) return;  // uniform across the workgroup, before any barrier
     const int W = a.W, WW = W * W, t = threadIdx.x;
     const lfpm_filter_params& fp = a.f;
     const int cntv = min(a.cnt[v], a.K);
@@ -3152,7 +3156,8 @@ int lfpm_filter_frame(const float* gray_dev, const float* centers_dev,
     a.plane_out = (float4*)plane_out_dev; a.cost_out = cost_out_dev;
     a.disp_out = (float4*)disp_out_dev; a.sel_out = sel_out_dev; a.reason = reason_dev;
     const int NT = 64 * ((crop * crop + 63) / 64);
-    hipLaunchKernelGGL(k_filter, dim3(num_mis), dim3(NT), 0, (hipStream_t)stream, a);
+    hipLaunchKernelGGL(k_filter, dim3(8 * ((num_mis + 7) / 8)), dim3(NT), 0,
+                       (hipStream_t)stream, a);
     return launch_check("filter");
 }
 
