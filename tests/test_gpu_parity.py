@@ -236,6 +236,32 @@ def test_refocus_vs_fp64(lfpm, orc, torch_cuda):
     assert err.max() <= 1e-5
 
 
+@pytest.mark.parametrize("w,h,pitch,slices,vmin,vmax", [
+    (330, 290, 23.0, 6, 1.5, 9.0),    # partial 16x16 output tiles on both axes
+    (400, 300, 31.0, 4, 2.0, 8.0),    # crop 21
+    (300, 250, 15.0, 4, 2.0, 8.0),    # crop 10
+    (256, 208, 23.0, 3, 2.0, 30.0),   # a reach beyond the tile kernel's candidate bound:
+                                      # the per-pixel kernel
+])
+def test_refocus_shapes_vs_fp64(lfpm, orc, torch_cuda, w, h, pitch, slices, vmin, vmax):
+    """Refocus stacks on frames whose size is not a multiple of the 16x16 output tile, at three
+    lens pitches, and with a slice range whose reach sends the call to the per-pixel kernel:
+    within 1e-5 of the FP64 oracle everywhere."""
+    radius = {15.0: 5, 23.0: 7, 31.0: 9}[pitch]
+    wl = lfpm.workload_c2(raw_w=w, raw_h=h, pitch=pitch, radius=radius)
+    _say("case refocus %dx%d pitch %g slices %d v %g..%g" % (w, h, pitch, slices, vmin, vmax))
+    df = lfpm.DeviceFrame(wl)
+    out = lfpm.refocus(df, slices=slices, vmin=vmin, vmax=vmax)
+    torch_cuda.cuda.synchronize()
+    g = out.cpu().numpy().astype(np.float64)
+    tiles = orc.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    o = orc.refocus(tiles, wl.L["centers"], wl.L["pitch"], wl.crop, w, h, slices, vmin, vmax)
+    err = np.abs(g - o)
+    _say("refocus shapes max abs err", float(err.max()), "covered", float((o > 0).mean()))
+    assert (o > 0).mean() > 0.3
+    assert err.max() <= 1e-5
+
+
 def test_fit_v2r_fp64(lfpm, orc, torch_cuda):
     rng = np.random.default_rng(3)
     c_true = np.array([0.05, 1.7, -0.4])
