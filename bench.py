@@ -653,9 +653,9 @@ def composites_section(args, lfpm, torch, wl):
                         "per_unit": "19 B per MI pixel per call + 2 B (score maps) + 29 B per "
                                     "raw pixel + 36 B per fused point",
                         "note": "dependency-bound, not bandwidth-bound: the reference's fusion "
-                                "walk is sequential within each MI (one wavefront per MI) and "
-                                "across %d schedule levels; hole filling iterates rows to a "
-                                "fixed point" % nl},
+                                "walk is sequential within each MI (solved here as a fixed "
+                                "point, one workgroup per MI) and across %d schedule levels; "
+                                "hole filling iterates rows to a fixed point" % nl},
            "aif_stats": {"collected": int(ast[0]), "kept": int(ast[1]), "passes": int(ast[2]),
                          "fills": int(ast[3])}, "cpu_baseline": None}
     if args.no_cpu_baseline:

@@ -23,9 +23,11 @@
 // sequential in the reference (MI order, pixel order); here
 //   k_bp_consistency   one lane per MI pixel: the consistency bits of all neighbours (masks
 //                      only remove candidates, so they are applied afterwards)
-//   k_bp_level         one wavefront per MI, level by level of a schedule in which MIs of one
+//   k_bp_level_wg      one workgroup per MI, level by level of a schedule in which MIs of one
 //                      level share no mask (lfpm_fuse_schedule): the masked walk over the MI's
-//                      pixels, lane = neighbour
+//                      pixels as the fixed point of its triangular system, thread = pixel
+//   k_bp_level         the same walk step by step, one wavefront per MI, lane = neighbour
+//                      (crops above 32)
 //   k_bp_emit          one lane per fused pixel: the averages in neighbour order, splat by the
 //                      highest walk index (the reference's last write), point records in order
 #include <hip/hip_runtime.h>
@@ -632,7 +634,7 @@ __global__ void __launch_bounds__(64) k_bp_level(
     __shared__ int s_nb[32], s_slot[32];
     const int a = order[begin + blockIdx.x];
     const int lane = threadIdx.x;
-    const int n = min(cnt[a], K);
+    const int n = max(0, min(cnt[a], K));
     int b = -1;
     if (lane < n) {
         b = nbr[(long long)a * K + lane];
@@ -759,6 +761,150 @@ __global__ void __launch_bounds__(64) k_bp_level(
     }
     __syncthreads();
     for (int q = lane + MW; q < (1 + n) * MW; q += 64) {
+        const int e = q / MW, t = q - e * MW;
+        const int mi = s_nb[e - 1];
+        if (mi < 0 || s_slot[e - 1] != e - 1) continue;
+        masks[(long long)mi * MW + t] = ml[q];
+    }
+}
+
+constexpr int kBpMaxPix = 1024;  // crop^2 bound of the workgroup walk: one thread per pixel
+constexpr int kBpMaxNb = 31;     // neighbour bits of a consistency word (bit 31: in the walk)
+
+// One workgroup per MI of one schedule level, one thread per MI pixel: the masked walk decided
+// for all of the MI's pixels at once.  A pixel's outcome depends on earlier pixels only through
+// the neighbour pixels their fusions mark, so the walk is the unique solution of the triangular
+// system
+//   inc(p)   = inc0(p) minus {j : a fused q < p marks the source pixel of p in neighbour j},
+//   fused(p) = |inc(p)| >= 2,
+// where inc0(p) holds the consistent neighbours whose source pixel is unmarked when the MI's walk
+// starts.  Rounds of Jacobi iteration from fused = |inc0| >= 2 reach it (after k rounds the first
+// k walk pixels are final) and a round that changes nothing ends them, since a fixed point is
+// that solution.  Only (mask copy, source pixel) pairs that two walk entries target can change
+// an outcome, so the rounds visit only those.
+// A pair is the bit index slot * MW * 32 + source of the neighbour masks after the MI's own
+// (each copy padded to whole words).  claim[pair]: the number of entries targeting it, then per
+// round r the key r << 16 | (0xFFFF - q) of the first fused q that targets it (atomic max: the
+// newest round, then the smallest q).
+__global__ void __launch_bounds__(kBpMaxPix) k_bp_level_wg(
+    const int* __restrict__ order, int begin, const int* __restrict__ nbr,
+    const int* __restrict__ cnt, int K, const uint8_t* __restrict__ garbage, int W,
+    const uint32_t* __restrict__ cons, const unsigned short* __restrict__ srcs,
+    uint32_t* __restrict__ masks, uint32_t* __restrict__ incl) {
+    extern __shared__ __align__(16) uint32_t lds[];
+    const int WW = W * W, MW = (WW + 31) >> 5;
+    uint32_t* ml = lds;                    // masks: own, then entry j at (1 + j) * MW
+    uint32_t* nm = lds + MW;               // the neighbour copies as one bit array
+    uint32_t* claim = lds + (1 + K) * MW;  // K * MW * 32
+    __shared__ int s_nb[32], s_slot[32];
+    __shared__ int s_any, s_chg[2];
+    const int a = order[begin + blockIdx.x];
+    const int p = threadIdx.x;
+    const int n = max(0, min(cnt[a], K));
+    if (p < 64) {
+        int b = -1;
+        if (p < n) {
+            b = nbr[(long long)a * K + p];
+            if (b >= 0 && garbage && garbage[b]) b = -1;
+        }
+        // a neighbour listed twice shares the mask copy of its first entry
+        int slot = p;
+        for (int j = 0; j < n; ++j) {
+            const int bj = __shfl(b, j);
+            if (j < slot && bj == b && b >= 0) slot = j;
+        }
+        if (p < n) {
+            s_nb[p] = b;
+            s_slot[p] = slot;
+        }
+        if (p == 0) {
+            s_any = 0;
+            s_chg[0] = 0;
+            s_chg[1] = 0;
+        }
+    }
+    const long long base = (long long)a * WW;
+    const uint32_t cb = p < WW ? cons[base + p] : 0u;
+    const uint32_t nmask = (1u << n) - 1u;
+    const bool walk = (cb >> 31) && __popc(cb & nmask) >= 2;
+    // the pixel's source in every consistent neighbour, straight into registers
+    int ci[kBpMaxNb];
+#pragma unroll
+    for (int j = 0; j < kBpMaxNb; ++j) ci[j] = 0;
+    if (walk) {
+        const unsigned short* sp = srcs + (base + p) * K;
+#pragma unroll
+        for (int j = 0; j < kBpMaxNb; ++j)
+            if (j < n) ci[j] = sp[j];
+    }
+    __syncthreads();
+    for (int q = p; q < (1 + n) * MW; q += blockDim.x) {
+        const int e = q / MW, t = q - e * MW;
+        const int mi = e == 0 ? a : s_nb[e - 1];
+        if (mi < 0 || (e > 0 && s_slot[e - 1] != e - 1)) continue;
+        ml[q] = masks[(long long)mi * MW + t];
+    }
+    for (int q = p; q < n * MW * 32; q += blockDim.x) claim[q] = 0u;
+    // an inconsistent neighbour has no source from the consistency pass: it points at pixel 0 of
+    // its copy, which it neither reads nor marks
+#pragma unroll
+    for (int j = 0; j < kBpMaxNb; ++j)
+        if (j < n) ci[j] = ((cb >> j) & 1u ? ci[j] : 0) + s_slot[j] * MW * 32;
+    __syncthreads();
+    // the MI's own mask does not change during its walk
+    const bool act = walk && !((ml[p >> 5] >> (p & 31)) & 1u);
+    uint32_t inc0 = 0;
+    if (act) {
+#pragma unroll
+        for (int j = 0; j < kBpMaxNb; ++j)
+            if (j < n) inc0 |= ((cb >> j) & ~(nm[ci[j] >> 5] >> (ci[j] & 31)) & 1u) << j;
+#pragma unroll
+        for (int j = 0; j < kBpMaxNb; ++j)
+            if ((inc0 >> j) & 1u) atomicAdd(&claim[ci[j]], 1u);
+    }
+    __syncthreads();
+    uint32_t cm = 0;
+#pragma unroll
+    for (int j = 0; j < kBpMaxNb; ++j)
+        if (((inc0 >> j) & 1u) && claim[ci[j]] > 1u) cm |= 1u << j;
+    if (cm) s_any = 1;
+    bool fz = __popc(inc0) >= 2;
+    uint32_t inc = inc0;
+    __syncthreads();
+    if (s_any) {
+        for (uint32_t r = 1;; ++r) {
+            if (fz) {
+                const uint32_t key = (r << 16) | (0xFFFFu - (uint32_t)p);
+#pragma unroll
+                for (int j = 0; j < kBpMaxNb; ++j)
+                    if ((cm >> j) & 1u) atomicMax(&claim[ci[j]], key);
+            }
+            __syncthreads();
+            if (p == 0) s_chg[(r + 1) & 1] = 0;
+            uint32_t ni = inc0;
+#pragma unroll
+            for (int j = 0; j < kBpMaxNb; ++j) {
+                if ((cm >> j) & 1u) {
+                    const uint32_t v = claim[ci[j]];
+                    if ((v >> 16) == r && 0xFFFFu - (v & 0xFFFFu) < (uint32_t)p) ni &= ~(1u << j);
+                }
+            }
+            const bool nf = act && __popc(ni) >= 2;
+            if (nf != fz) s_chg[r & 1] = 1;
+            fz = nf;
+            inc = ni;
+            __syncthreads();
+            if (!s_chg[r & 1]) break;
+        }
+    }
+    if (fz) {
+        incl[base + p] = inc;
+#pragma unroll
+        for (int j = 0; j < kBpMaxNb; ++j)
+            if ((inc >> j) & 1u) atomicOr(&nm[ci[j] >> 5], 1u << (ci[j] & 31));
+    }
+    __syncthreads();
+    for (int q = p + MW; q < (1 + n) * MW; q += blockDim.x) {
         const int e = q / MW, t = q - e * MW;
         const int mi = s_nb[e - 1];
         if (mi < 0 || s_slot[e - 1] != e - 1) continue;
@@ -1105,9 +1251,12 @@ int lfpm_fuse_backproject(const float* plane_dev, const float* centers_dev,
         return set_error(LFPM_E_INVALID, "lfpm_fuse_backproject: workspace %zu < %zu bytes",
                          workspace_bytes, L.total);
     const int WW = crop * crop, MW = (WW + 31) / 32;
+    // the workgroup walk up to 1024 pixels per MI, the wavefront walk beyond
+    const size_t lds_wg = (size_t)((1 + max_neighbors) * MW + max_neighbors * MW * 32) * 4;
+    const bool wg_walk = WW <= kBpMaxPix && lds_wg <= 65536;
     const size_t lds = (size_t)((1 + max_neighbors) * MW + 2 * WW) * 4 +
                        (size_t)WW * max_neighbors * 2;
-    if (lds > 65536)
+    if (!wg_walk && lds > 65536)
         return set_error(LFPM_E_UNSUPPORTED, "lfpm_fuse_backproject: %zu B of LDS", lds);
     hipStream_t s = (hipStream_t)stream;
     unsigned char* ws = (unsigned char*)workspace_dev;
@@ -1141,8 +1290,13 @@ int lfpm_fuse_backproject(const float* plane_dev, const float* centers_dev,
     for (int l = 0; l < num_levels; ++l) {
         const int b0 = level_start[l], nb = level_start[l + 1] - b0;
         if (nb <= 0) continue;
-        k_bp_level<<<nb, 64, lds, s>>>(order_dev, b0, pl, ce, nbr_dev, cnt_dev, max_neighbors,
-                                       garbage_dev, g, cons, srcs, masks, incl);
+        if (wg_walk)
+            k_bp_level_wg<<<nb, (WW + 63) / 64 * 64, lds_wg, s>>>(
+                order_dev, b0, nbr_dev, cnt_dev, max_neighbors, garbage_dev, crop, cons, srcs,
+                masks, incl);
+        else
+            k_bp_level<<<nb, 64, lds, s>>>(order_dev, b0, pl, ce, nbr_dev, cnt_dev,
+                                           max_neighbors, garbage_dev, g, cons, srcs, masks, incl);
     }
     k_bp_emit<false><<<grid_for(M, 256), 256, 0, s>>>(pl, ce, bgr_tiles_dev, nbr_dev,
                                                      max_neighbors, g, M, raw_w, raw_h, incl, pos,
