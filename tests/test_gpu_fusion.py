@@ -23,9 +23,31 @@ def torch_cuda():
     return torch
 
 
-def _inputs(lfpm, torch, wl, seed):
+class _Planes:
+    def __init__(self, plane):
+        self.plane = plane
+
+
+def _gt_planes(torch, wl, seed):
+    """Planes from the scene's ground-truth virtual depth (disparity pitch / v, 0 outside the
+    discs) with 2 % multiplicative noise and random unit normals: a frame for crops the solve's
+    LDS budget does not take."""
+    rng = np.random.default_rng(seed)
+    v = wl.gt_virtual_depth().astype(np.float64)
+    o, W = wl.L["origins"].reshape(-1, 2), wl.crop
+    t = np.stack([v[y:y + W, x:x + W] for x, y in o])
+    d = np.where(t > 0, wl.L["pitch"] / np.where(t > 0, t, 1.0), 0.0)
+    d *= 1.0 + 0.02 * rng.standard_normal(d.shape)
+    n = rng.standard_normal(d.shape + (3,))
+    n[..., 2] = -np.abs(n[..., 2]) - 1.0
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    p = np.concatenate([n, d[..., None]], -1).astype(np.float32).reshape(len(o), W * W, 4)
+    return _Planes(torch.from_numpy(p).cuda())
+
+
+def _inputs(lfpm, torch, wl, seed, gt=False):
     df = lfpm.DeviceFrame(wl)
-    bufs = lfpm.patchmatch_frame(df)
+    bufs = _gt_planes(torch, wl, seed) if gt else lfpm.patchmatch_frame(df)
     raw = wl.raw().astype(np.int64)
     rng = np.random.default_rng(seed)
     bgr = np.stack([raw, (raw * 7 + 11) % 256, rng.integers(0, 256, raw.shape)],
@@ -41,8 +63,8 @@ def _inputs(lfpm, torch, wl, seed):
     return df, bufs, tiles, rich, blur, garbage
 
 
-def _aif_check(lfpm, orc, torch, wl, seed, tag, B=0.9, bl0=42.0, f=30.0):
-    df, bufs, tiles, rich, blur, garbage = _inputs(lfpm, torch, wl, seed)
+def _aif_check(lfpm, orc, torch, wl, seed, tag, B=0.9, bl0=42.0, f=30.0, gt=False):
+    df, bufs, tiles, rich, blur, garbage = _inputs(lfpm, torch, wl, seed, gt)
     h, w = df.raw.shape
     _say("case aif %s V=%d crop=%d %dx%d" % (tag, wl.V, wl.crop, w, h))
     dev = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
@@ -68,8 +90,8 @@ def _aif_check(lfpm, orc, torch, wl, seed, tag, B=0.9, bl0=42.0, f=30.0):
     assert st["kept"] > 0.2 * w * h and st["fills"] > 0
 
 
-def _fuse_check(lfpm, orc, torch, wl, seed, tag, B=1.2, bl0=60.0):
-    df, bufs, tiles, _, _, garbage = _inputs(lfpm, torch, wl, seed)
+def _fuse_check(lfpm, orc, torch, wl, seed, tag, B=1.2, bl0=60.0, gt=False):
+    df, bufs, tiles, _, _, garbage = _inputs(lfpm, torch, wl, seed, gt)
     h, w = df.raw.shape
     # baseline * bl0 / (bl0 + B) = pitch: the disparities' own baseline
     baseline = wl.L["pitch"] * (bl0 + B) / bl0
@@ -113,3 +135,17 @@ def test_aif_fuse_full_frame(lfpm, orc, torch_cuda):
     wl = lfpm.workload_c2(iterations=2)
     _aif_check(lfpm, orc, torch_cuda, wl, 35, "c2full")
     _fuse_check(lfpm, orc, torch_cuda, wl, 36, "c2full")
+
+
+@pytest.mark.parametrize("pitch", [40.0, 43.0])
+def test_aif_fuse_large_crops(lfpm, orc, torch_cuda, pitch):
+    """Crops 28 and 30 with 18 neighbours on ground-truth planes: the workgroup walk at 59.5 KB
+    of LDS, and the step-by-step wavefront walk that takes over where the workgroup walk's claim
+    table would pass 64 KB."""
+    wl = lfpm.workload_c2(raw_w=720, raw_h=560, pitch=pitch, radius=9, iterations=2)
+    W, K = wl.crop, wl.L["nbr"].size // wl.V
+    MW = (W * W + 31) // 32
+    wg_lds = ((1 + K) * MW + K * MW * 32) * 4
+    assert (wg_lds <= 65536) == (pitch == 40.0), wg_lds
+    _aif_check(lfpm, orc, torch_cuda, wl, 37, "crop%d" % W, gt=True)
+    _fuse_check(lfpm, orc, torch_cuda, wl, 38, "crop%d" % W, gt=True)
