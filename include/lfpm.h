@@ -662,10 +662,13 @@ int lfpm_autotune_class(const lfpm_batch_frame* frame, void* workspace_dev,
                         size_t workspace_bytes, int* best_per_cu, float* best_ms);
 int lfpm_autotune_get(const lfpm_pm_params* params, int* per_cu, int* waves_per_mi);
 
-/* Work accounting for roofline reports (no reference counterpart).  While counters_dev (two
- * device uint64: evaluated NCC window samples, evaluated (pixel, view, hypothesis) triples) is
- * set, solver launches from the calling thread use counting kernel variants that accumulate into
- * it; results are identical to the non-counting kernels.  NULL switches counting off. */
+/* Work accounting for roofline reports (no reference counterpart).  While counters_dev (four
+ * device uint64: evaluated NCC window samples, evaluated (pixel, view, hypothesis) triples, and
+ * the samples and triples the two-round candidate scoring leaves out, i.e. the views of rows
+ * whose cost over the first half of the selection already reaches the best) is set, solver
+ * launches from the calling thread use counting kernel variants that accumulate into it; they
+ * score the left-out views too (into the last two counters only) and their results are identical
+ * to the non-counting kernels.  NULL switches counting off. */
 int lfpm_set_work_counters(unsigned long long* counters_dev);
 
 /* ---------------------------------------------------------------------------------------

@@ -179,6 +179,8 @@ __device__ void random_normal(DevStream& st, uint32_t base, float& nx, float& ny
 // (pixel, view, hypothesis) triples — the units of the roofline FLOP count.
 struct WorkCount {
     unsigned int samples, triples;
+    // the samples and triples a single-round scoring would add (rows settled by the bound)
+    unsigned int samples_skipped, triples_skipped;
 };
 
 // LDS carve-up shared by the solver kernels.
@@ -294,6 +296,8 @@ __device__ __forceinline__ void flush_counts(unsigned long long* counters, const
     if (COUNT && counters) {
         atomicAdd(&counters[0], (unsigned long long)wc.samples);
         atomicAdd(&counters[1], (unsigned long long)wc.triples);
+        atomicAdd(&counters[2], (unsigned long long)wc.samples_skipped);
+        atomicAdd(&counters[3], (unsigned long long)wc.triples_skipped);
     }
 }
 
@@ -985,6 +989,23 @@ __device__ __forceinline__ void run_rows_bounded(const Lds& L, const DevParams& 
     nt = add_tasks_rows(L, P, S, 0, nrows, open, tsel & ~m1, c, x, y, 0, false);
     wave_sync();
     run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nt, ns, x, y, rc, rbc, wc);
+    if (COUNT) {
+        // counting only: score the settled rows' other views as well, into the skipped counts,
+        // and put their entries back to 2
+        const uint32_t settled = (uint32_t)__ballot(lane < nrows && ((rows >> lane) & 1u)) & ~open;
+        nt = add_tasks_rows(L, P, S, 0, nrows, settled, tsel & ~m1, c, x, y, 0, false);
+        wave_sync();
+        WorkCount sk{0u, 0u, 0u, 0u};
+        run_batch<COUNT, TT, BLUR, W1C>(L, P, S, nt, ns, x, y, rc, rbc, sk);
+        wc.samples_skipped += sk.samples;
+        wc.triples_skipped += sk.triples;
+        const uint32_t vrest = tsel & ~m1;
+        for (int i = lane; i < nrows * P.K; i += 64) {
+            const int r = i / P.K, k = i - r * P.K;
+            if (((settled >> r) & 1u) && ((vrest >> k) & 1u)) S.cost[i] = 2.0f;
+        }
+        wave_sync();
+    }
 }
 
 // Rounded point of the cross-view proposal through view k at disparity w
@@ -1636,7 +1657,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     unsigned* const tickets = a.tickets;
     const int total = a.total, stride = P.slots_per_xcd;
     int turn = 0, prev = -1;  // wavefront 0's hand-out state
-    WorkCount wc{0u, 0u};
+    WorkCount wc{0u, 0u, 0u, 0u};
     int f = 0;  // frame of the current MI
     const uint32_t ctab = kCandLanes[t & 63];
     if (wave == 0) next_mi(L, tickets, total, stride, turn, prev);
@@ -1737,7 +1758,7 @@ __global__ void __launch_bounds__(256) k_eval_w(EvalArgs a) {
     fill_spatial(L, P);
     int first, end, stride;
     mi_range(P, first, end, stride);
-    WorkCount wc{0u, 0u};
+    WorkCount wc{0u, 0u, 0u, 0u};
     for (int mi = first; mi < end; mi += stride) {
         const int h0 = a.seg[mi], h1 = a.seg[mi + 1];
         if (h0 >= h1) continue;  // uniform across the workgroup

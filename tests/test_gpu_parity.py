@@ -476,3 +476,31 @@ def test_ctx_blur_term_matches_device_path(lfpm, torch_cuda):
     for out in (o, outs[0]):
         for k in ("plane", "cost", "sel", "disp"):
             assert np.array_equal(np.asarray(out[k]).view(np.uint32), g[k].view(np.uint32)), k
+
+
+def test_counting_kernels_identical(lfpm, torch_cuda):
+    """The counting kernel variants (work counters attached) solve a C2-parameter frame to the
+    same planes, costs and selections as the plain kernels, and count the views the two-round
+    candidate scoring leaves out."""
+    torch = torch_cuda
+    wl = lfpm.workload_c2(raw_w=1024, raw_h=800, iterations=3)
+    df = lfpm.DeviceFrame(wl)
+    _say("case counting kernels V=%d" % wl.V)
+    plain = lfpm.patchmatch_frame(df)
+    torch.cuda.synchronize()
+    cnt = torch.zeros(4, dtype=torch.int64, device="cuda")
+    L = lfpm.lib()
+    L.lfpm_set_work_counters(C.c_void_p(cnt.data_ptr()))
+    try:
+        counted = lfpm.patchmatch_frame(df)
+        torch.cuda.synchronize()
+    finally:
+        L.lfpm_set_work_counters(None)
+    for k in ("plane", "cost", "disp", "sel"):
+        a, b = getattr(plain, k).cpu().numpy(), getattr(counted, k).cpu().numpy()
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), k
+    c = cnt.cpu().numpy()
+    _say("work", c.tolist(), "left out %.1f %% of samples" % (100.0 * c[2] / (c[0] + c[2])))
+    assert c[0] > 0 and c[1] > 0 and c[2] > 0 and c[3] > 0
+    # a left-out triple has at most the full window of samples
+    assert c[2] <= c[3] * (2 * wl.pm.radius + 1) ** 2
