@@ -2338,6 +2338,20 @@ static int device_cus() {
     return cus;
 }
 
+// LDS one workgroup may allocate on the current device (160 KiB on MI355X: a workgroup can hold
+// the whole of a CU's LDS), read once per process.
+static size_t g_lds_per_block = 0;
+static size_t device_lds_per_block() {
+    if (g_lds_per_block > 0) return g_lds_per_block;
+    int dev = 0, b = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        b < 65536)
+        b = 65536;
+    g_lds_per_block = (size_t)b;
+    return g_lds_per_block;
+}
+
 // Persistent grid over `mis` work items: a multiple of 8 workgroups (one share per XCD).
 static int persistent_grid_n(int mis, int per_cu) {
     int g = device_cus() * per_cu;
@@ -2382,11 +2396,11 @@ static FrameDesc frame_desc(const float* tiles, const float* centers, const int*
 }
 
 // Threads per workgroup of the wave engine: `waves` wavefronts (1, 2 or 4; the autotuned value
-// of the parameter class), or fewer when their LDS would pass 64 KiB (the blur term's second
-// set of tiles with many views and a wide window).
+// of the parameter class), or fewer when their LDS would pass what one workgroup may allocate
+// (wide crops with many views and a wide window, or the blur term's second set of tiles).
 static int wave_threads(const DevParams& P, int waves) {
     for (int nw = std::min(std::max(waves, 1), 4); nw > 1; nw /= 2)
-        if (wv::lds_total(P, nw) <= 65536) return 64 * nw;
+        if (wv::lds_total(P, nw) <= device_lds_per_block()) return 64 * nw;
     return 64;
 }
 
@@ -2431,9 +2445,10 @@ static int launch_stage(StageArgs& a, int per_cu, int waves, void* scratch,
     if (total <= 0) return LFPM_OK;
     const int NT = wave_threads(a.P, waves);
     const size_t lds = wv::lds_total(a.P, NT / 64);
-    if (lds > 65536)
+    if (lds > device_lds_per_block())
         return set_error(LFPM_E_UNSUPPORTED, "k_stage_w: %zu B of LDS per workgroup (crop %d, "
-                         "%d views, radius %d) exceeds 64 KiB", lds, a.P.W, a.P.K, a.P.r);
+                         "%d views, radius %d) exceeds the device's %zu B", lds, a.P.W, a.P.K,
+                         a.P.r, device_lds_per_block());
     const StageKernel kern = stage_w_kernel(g_counters != nullptr, a.P.fmt,
                                             a.P.blur != 0, a.P.W);
     const int fit = fit_per_cu(kern, NT, lds);
@@ -3054,9 +3069,9 @@ static int eval_costs_with(const float* tiles_dev, const float* centers_dev,
     P.slots_per_xcd = grid / 8;
     const int NT = wave_threads(P, kDefaultWaves);
     const size_t lds = wv::lds_total(P, NT / 64);
-    if (lds > 65536)
-        return set_error(LFPM_E_UNSUPPORTED, "eval: %zu B of LDS per workgroup exceeds 64 KiB",
-                         lds);
+    if (lds > device_lds_per_block())
+        return set_error(LFPM_E_UNSUPPORTED, "eval: %zu B of LDS per workgroup exceeds the "
+                         "device's %zu B", lds, device_lds_per_block());
     hipLaunchKernelGGL(k_segments, dim3((num_mis + 256) / 256), dim3(256), 0, s,
                        (const int*)hyp_dev, n, num_mis, seg);
     EvalArgs a;
