@@ -652,12 +652,14 @@ int lfpm_batch_step(const lfpm_batch_frame* frames, int n, const int* stages, co
 
 /* Autotuner (the reference fixes one 16x16 block shape, MIStereo/AdaptMIPMFrame.cu:957-966):
  * time phase launches of this frame's parameter class (crop, neighbours, radius, tile storage,
- * blur term) at every launch shape of {1, 2, 4} wavefronts per workgroup (the threads that share
- * one MI's staged tiles, which sets the workgroup's LDS footprint) x {2, 4, 6, 8} persistent
- * workgroups per CU, keep the fastest for the class (process-wide), and return its workgroups
- * per CU.  Results do not depend on the choice.  Uses and overwrites B state of `frame`.
- * lfpm_autotune_get returns the launch shape a class uses (the defaults, 8 and 4, until tuned);
- * it has no reference counterpart. */
+ * blur term) at every launch shape of {1, 2, 4, 8, 12} wavefronts per workgroup (the threads that
+ * share one MI's staged tiles, which sets the workgroup's LDS footprint) x {2, 4, 6, 8}
+ * persistent workgroups per CU, keep the fastest for the class (process-wide), and return its
+ * workgroups per CU.  Results do not depend on the choice.  Uses and overwrites B state of
+ * `frame`.  lfpm_autotune_get returns the launch shape a class uses: until tuned, 8 workgroups
+ * per CU (as many as fit) of whichever of 4, 8 or 12 wavefronts keeps the most wavefronts per
+ * CU (4 for crops up to 16 with 18 views; 12 for the crop-21 class); it has no reference
+ * counterpart. */
 int lfpm_autotune_class(const lfpm_batch_frame* frame, void* workspace_dev,
                         size_t workspace_bytes, int* best_per_cu, float* best_ms);
 int lfpm_autotune_get(const lfpm_pm_params* params, int* per_cu, int* waves_per_mi);

@@ -361,7 +361,7 @@ int lfpm_autotune_class(const lfpm_batch_frame* frame, void* ws, size_t ws_bytes
     const lfpm_pm_params* p = frame->params;
     // launch shapes: wavefronts per workgroup (threads sharing one MI's staged tiles, and with
     // them the workgroup's LDS footprint) x persistent workgroups per CU
-    const int waves[3] = {1, 2, 4};
+    const int waves[5] = {1, 2, 4, 8, 12};
     const int per_cu[4] = {2, 4, 6, 8};
     hipStream_t s;
     B_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -371,7 +371,7 @@ int lfpm_autotune_class(const lfpm_batch_frame* frame, void* ws, size_t ws_bytes
     int best = -1, bestw = -1, rc = LFPM_OK;
     float bms = 0.0f;
     const int prev = lfpm::stage_slots_per_cu(p), prevw = lfpm::stage_waves(p);
-    for (int wi = 0; wi < 3 && rc == LFPM_OK; ++wi)
+    for (int wi = 0; wi < 5 && rc == LFPM_OK; ++wi)
         for (int ci = 0; ci < 4 && rc == LFPM_OK; ++ci) {
             lfpm::set_stage_config(p, per_cu[ci], waves[wi]);
             const size_t need = lfpm::stage_scratch_size(p, frame->num_mis);
@@ -413,7 +413,7 @@ int lfpm_autotune_class(const lfpm_batch_frame* frame, void* ws, size_t ws_bytes
 int lfpm_autotune_get(const lfpm_pm_params* params, int* per_cu, int* waves_per_mi) {
     if (!params) return set_error(LFPM_E_INVALID, "tune: null params");
     if (per_cu) *per_cu = lfpm::stage_slots_per_cu(params);
-    if (waves_per_mi) *waves_per_mi = lfpm::stage_waves(params);
+    if (waves_per_mi) *waves_per_mi = lfpm::stage_waves_used(params);
     return LFPM_OK;
 }
 

@@ -37,8 +37,11 @@ int stage_launch(const lfpm_pm_params* p, const StageFrame* frames, int n, void*
 // the setter the autotuner uses.
 int stage_slots_per_cu(const lfpm_pm_params* p);
 void set_stage_slots_per_cu(const lfpm_pm_params* p, int per_cu);
-// Wavefronts per k_stage workgroup (1, 2 or 4) for a parameter class, and the setter of both.
+// Wavefronts per k_stage workgroup for a parameter class as configured (0 = chosen at launch:
+// the shape that keeps the most wavefronts per CU), the number a launch uses, and the setter of
+// both values.
 int stage_waves(const lfpm_pm_params* p);
+int stage_waves_used(const lfpm_pm_params* p);
 void set_stage_config(const lfpm_pm_params* p, int per_cu, int waves);
 
 // Device helpers of lfpm_products.hip.  exclusive_scan: out[i] = sum of in[0..i) for i < n, on

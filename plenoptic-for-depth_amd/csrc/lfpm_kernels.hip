@@ -488,6 +488,8 @@ namespace wv {
 
 constexpr int kSlots = 8;      // tasks per pass
 constexpr int kHMax = 16;      // hypotheses per batch
+// k_stage_w workgroups: up to 12 wavefronts (3 per SIMD, the occupancy its registers allow)
+constexpr int kStageMaxThreads = 768;
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1643,7 +1645,8 @@ __device__ __forceinline__ int next_pixel(const Lds& L) {
 }
 
 template <bool COUNT, typename TT, bool BLUR, int W1C>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_stage_w(StageArgs a) {
+__global__ void __launch_bounds__(kStageMaxThreads) __attribute__((amdgpu_waves_per_eu(3)))
+k_stage_w(StageArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const DevParams& P = a.P;
     const int NT = blockDim.x, t = threadIdx.x, WW = P.WW, W = P.W;
@@ -2395,12 +2398,14 @@ static FrameDesc frame_desc(const float* tiles, const float* centers, const int*
     return f;
 }
 
-// Threads per workgroup of the wave engine: `waves` wavefronts (1, 2 or 4; the autotuned value
-// of the parameter class), or fewer when their LDS would pass what one workgroup may allocate
-// (wide crops with many views and a wide window, or the blur term's second set of tiles).
-static int wave_threads(const DevParams& P, int waves) {
-    for (int nw = std::min(std::max(waves, 1), 4); nw > 1; nw /= 2)
-        if (wv::lds_total(P, nw) <= device_lds_per_block()) return 64 * nw;
+// Threads per workgroup of the wave engine: `waves` wavefronts (at most max_waves), or the
+// largest of 12, 8, 6, 4, 2, 1 below it whose LDS fits what one workgroup may allocate (wide
+// crops with many views and a wide window, or the blur term's second set of tiles).
+static int wave_threads(const DevParams& P, int waves, int max_waves) {
+    const int steps[6] = {12, 8, 6, 4, 2, 1};
+    const int want = std::min(std::max(waves, 1), max_waves);
+    for (int nw : steps)
+        if (nw <= want && wv::lds_total(P, nw) <= device_lds_per_block()) return 64 * nw;
     return 64;
 }
 
@@ -2433,6 +2438,25 @@ __global__ void k_zero_tickets(unsigned* __restrict__ t) {
     if (threadIdx.x < 8) t[threadIdx.x] = 0u;
 }
 
+// Default wavefronts per k_stage_w workgroup: of 4, 8 and 12, the count that keeps the most
+// wavefronts per CU (workgroups that fit by LDS and registers, at most kPhasePerCu, times their
+// wavefronts), the smaller on ties.  Narrow crops fit three 4-wavefront workgroups per CU
+// (12 wavefronts, the register limit); wide crops, whose tiles fill the LDS, get one or two
+// larger workgroups instead.
+static int auto_waves(const DevParams& P, StageKernel kern) {
+    int best = 4, best_wpc = 0;
+    for (int nw : {4, 8, 12}) {
+        const int NT = wave_threads(P, nw, wv::kStageMaxThreads / 64);
+        const int fit = std::min(fit_per_cu(kern, NT, wv::lds_total(P, NT / 64)), kPhasePerCu);
+        const int wpc = fit * (NT / 64);
+        if (wpc > best_wpc) {
+            best_wpc = wpc;
+            best = NT / 64;
+        }
+    }
+    return best;
+}
+
 // Launch k_stage_w over the frames of `a` (mi_begin/total filled here).
 static int launch_stage(StageArgs& a, int per_cu, int waves, void* scratch,
                         size_t scratch_bytes, hipStream_t s) {
@@ -2443,14 +2467,15 @@ static int launch_stage(StageArgs& a, int per_cu, int waves, void* scratch,
     }
     a.total = total;
     if (total <= 0) return LFPM_OK;
-    const int NT = wave_threads(a.P, waves);
+    const StageKernel kern = stage_w_kernel(g_counters != nullptr, a.P.fmt,
+                                            a.P.blur != 0, a.P.W);
+    const int NT = wave_threads(a.P, waves > 0 ? waves : auto_waves(a.P, kern),
+                                wv::kStageMaxThreads / 64);
     const size_t lds = wv::lds_total(a.P, NT / 64);
     if (lds > device_lds_per_block())
         return set_error(LFPM_E_UNSUPPORTED, "k_stage_w: %zu B of LDS per workgroup (crop %d, "
                          "%d views, radius %d) exceeds the device's %zu B", lds, a.P.W, a.P.K,
                          a.P.r, device_lds_per_block());
-    const StageKernel kern = stage_w_kernel(g_counters != nullptr, a.P.fmt,
-                                            a.P.blur != 0, a.P.W);
     const int fit = fit_per_cu(kern, NT, lds);
     const int grid = persistent_grid_n(total, std::max(1, std::min(per_cu, fit)));
     a.P.slots_per_xcd = grid / 8;
@@ -2485,7 +2510,7 @@ struct TuneVal {
 };
 static std::vector<std::pair<TuneKey, TuneVal>> g_tuned;
 static std::mutex g_tuned_mu;
-static const int kDefaultWaves = 4;
+static const int kDefaultWaves = 4;  // k_eval_w wavefronts per workgroup
 
 static TuneKey tune_key(const lfpm_pm_params* p) {
     return TuneKey{p->crop, p->max_neighbors, p->radius, p->storage, p->blur_lambda > 0.0f};
@@ -2496,11 +2521,18 @@ static TuneVal stage_config(const lfpm_pm_params* p) {
     std::lock_guard<std::mutex> g(g_tuned_mu);
     for (auto& e : g_tuned)
         if (e.first == k) return e.second;
-    return TuneVal{kPhasePerCu, kDefaultWaves};
+    return TuneVal{kPhasePerCu, 0};
 }
 
 int stage_slots_per_cu(const lfpm_pm_params* p) { return stage_config(p).per_cu; }
 int stage_waves(const lfpm_pm_params* p) { return stage_config(p).waves; }
+
+int stage_waves_used(const lfpm_pm_params* p) {
+    const DevParams P = make_dev_params(p, 1);
+    const int w = stage_waves(p);
+    const StageKernel kern = stage_w_kernel(false, P.fmt, P.blur != 0, P.W);
+    return wave_threads(P, w > 0 ? w : auto_waves(P, kern), wv::kStageMaxThreads / 64) / 64;
+}
 
 void set_stage_config(const lfpm_pm_params* p, int per_cu, int waves) {
     const TuneKey k = tune_key(p);
@@ -3067,7 +3099,7 @@ static int eval_costs_with(const float* tiles_dev, const float* centers_dev,
     DevParams P = make_dev_params(p, num_mis);
     const int grid = persistent_grid(P, kPhasePerCu);
     P.slots_per_xcd = grid / 8;
-    const int NT = wave_threads(P, kDefaultWaves);
+    const int NT = wave_threads(P, kDefaultWaves, 4);
     const size_t lds = wv::lds_total(P, NT / 64);
     if (lds > device_lds_per_block())
         return set_error(LFPM_E_UNSUPPORTED, "eval: %zu B of LDS per workgroup exceeds the "

@@ -124,7 +124,8 @@ def test_autotune_keeps_results(lfpm, torch_cuda, setup):
     shape = lfpm.autotune_get(slots[0].wl.pm)
     _say("autotune", best, ms, "shape", shape)
     assert best in (2, 4, 6, 8) and ms > 0
-    assert shape[0] == best and shape[1] in (1, 2, 4)
+    # wavefronts per workgroup: the searched 1, 2, 4, 8, 12, or 6 where 8 do not fit the LDS
+    assert shape[0] == best and shape[1] in (1, 2, 4, 6, 8, 12)
     again = _plain(lfpm, torch_cuda, slots[0])
     _same(ref[0], again, "after tune")
 
