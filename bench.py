@@ -43,8 +43,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="batched",
-                    choices=["plain", "batched", "mixed", "serve", "precision", "ingest",
-                             "products", "composites"])
+                    choices=["plain", "batched", "mixed", "serve", "precision", "parity",
+                             "ingest", "products", "composites"])
     ap.add_argument("--hypotheses", type=int, default=1000000, help="precision: hypotheses")
     ap.add_argument("--batch", type=int, default=4, help="frames per batch (batched / mixed)")
     ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
@@ -411,6 +411,42 @@ def serve_section(args, lfpm, torch):
     # the first stream at), where latency is service time rather than queueing
     r["sustainable"] = one_run(0.7 * r["fps"], 8)
     return r
+
+
+def parity_section(args, lfpm, torch, rw, rh):
+    """The whole frame at the full-frame parameters solved on the GPU and by the FP64 oracle on
+    the host: the fraction of valid pixels (oracle cost <= 1.8) whose disparity is within
+    0.05 px, the bar the solve is held to; the oracle prints a line per stage."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    wl = lfpm.workload_c2(raw_w=rw, raw_h=rh)
+    log("bench: parity %dx%d, V=%d, K=%d, window %d, %d iterations"
+        % (rw, rh, wl.V, wl.pm.max_neighbors, 2 * wl.pm.radius + 1, wl.pm.iterations))
+    df = lfpm.DeviceFrame(wl)
+    t0 = time.perf_counter()
+    bufs = lfpm.patchmatch_frame(df)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    d_g = bufs.plane.cpu().numpy()[:, 3].astype(np.float64)
+    sel_g = bufs.sel.cpu().numpy().view(np.uint32)
+    log("bench: GPU solve %.2f s" % gpu_s)
+    tiles = oracle.slice_u8(wl.raw(), wl.L["origins"], wl.crop)
+    pm = oracle.PmParams(*[getattr(wl.pm, f[0]) for f in oracle.PmParams._fields_])
+    threads = max(1, min(16, os.cpu_count() or 1))
+    oracle.set_progress(True)
+    t0 = time.perf_counter()
+    o = oracle.solve(tiles, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], pm, threads=threads)
+    orc_s = time.perf_counter() - t0
+    oracle.set_progress(False)
+    log("bench: FP64 oracle solve %.1f s on %d threads" % (orc_s, threads))
+    valid = o["cost"] <= 1.8
+    err = np.abs(d_g - o["plane"][:, 3])
+    return {"pixels": int(err.size), "valid_frac": float(valid.mean()),
+            "frac_le_005": float((err[valid] <= 0.05).mean()),
+            "frac_all_le_005": float((err <= 0.05).mean()),
+            "max_err_px": float(err[valid].max()),
+            "sel_equal": float((sel_g == o["sel"]).mean()),
+            "gpu_s": gpu_s, "oracle_s": orc_s, "oracle_threads": threads, "cpu": cpu_model()}
 
 
 def precision_section(args, lfpm, torch):
@@ -797,6 +833,16 @@ def main():
                                         "15x15 window), FP32 and FP16 LDS tiles vs the FP64 "
                                         "oracle; full-solve agreement on a 1024x800 frame"})
         line["dtype"] = "fp16 storage / fp32 arithmetic"
+        print(json.dumps(line), flush=True)
+        return
+    if args.mode == "parity":
+        r = parity_section(args, lfpm, torch, rw, rh)
+        line.update(metric="valid pixels within 0.05 px of the FP64 oracle (%dx%d, full solve)"
+                    % (rw, rh), value=r["frac_le_005"], unit="fraction", steps=1,
+                    ms_per_step=1e3 * r["gpu_s"], parity=r,
+                    config={"workload": "one %dx%d frame at the full-frame parameters (18 views, "
+                                        "15x15 window, 6 iterations) solved on the GPU and by the "
+                                        "FP64 oracle on the host" % (rw, rh)})
         print(json.dumps(line), flush=True)
         return
     if args.mode == "serve":

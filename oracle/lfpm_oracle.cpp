@@ -66,6 +66,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 #ifdef _OPENMP
@@ -956,6 +957,10 @@ static void build_frame(Frame<T>& f, const uint8_t* tiles_u8, const double* blur
     }
 }
 
+// Stage lines on stderr ("init done", "iteration i phase p done") from solve, for callers that
+// run a long solve under a watchdog of output silence.
+static int g_progress = 0;
+
 template <typename T>
 static void solve(const uint8_t* tiles_u8, const double* blur, const float* centers,
                   const int* nbr, const int* cnt, int V, const PmParams& p, T* plane_out,
@@ -974,6 +979,7 @@ static void solve(const uint8_t* tiles_u8, const double* blur, const float* cent
     for (int v = 0; v < V; ++v)
         for (int y = 0; y < W; ++y)
             for (int x = 0; x < W; ++x) init_pixel(f, A, v, x, y);
+    if (g_progress) fprintf(stderr, "oracle: init done (%d MIs)\n", V);
     for (int it = 0; it < p.iterations; ++it) {
         for (int ph = 0; ph < 2; ++ph) {
             const State<T>& src = ph == 0 ? A : B;
@@ -982,6 +988,7 @@ static void solve(const uint8_t* tiles_u8, const double* blur, const float* cent
             for (int v = 0; v < V; ++v)
                 for (int y = 0; y < W; ++y)
                     for (int x = 0; x < W; ++x) update_pixel(f, src, dst, v, x, y, it, ph);
+            if (g_progress) fprintf(stderr, "oracle: iteration %d phase %d done\n", it, ph);
         }
     }
     for (size_t i = 0; i < n; ++i) {
@@ -2196,6 +2203,8 @@ void orc_slice_u8(const uint8_t* raw, int raw_w, int raw_h, const int* origins, 
                 tiles[((size_t)v * crop + y) * crop + x] =
                     raw[(size_t)(origins[2 * v + 1] + y) * raw_w + origins[2 * v] + x];
 }
+
+void orc_set_progress(int on) { orc::g_progress = on; }
 
 void orc_solve_f64(const uint8_t* tiles, const double* blur, const float* centers,
                    const int* nbr, const int* cnt, int V, const orc::PmParams* p, double* plane,

@@ -65,6 +65,11 @@ def nthreads():
     return max(1, min(16, os.cpu_count() or 1))
 
 
+def set_progress(on):
+    """Stage lines on stderr from solve (init, then each iteration's two phases)."""
+    lib().orc_set_progress(1 if on else 0)
+
+
 def philox(ctr, key):
     c = (C.c_uint32 * 4)(*ctr)
     k = (C.c_uint32 * 2)(*key)
