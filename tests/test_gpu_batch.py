@@ -130,6 +130,17 @@ def test_autotune_keeps_results(lfpm, torch_cuda, setup):
     _same(ref[0], again, "after tune")
 
 
+def test_default_launch_shape_keeps_wavefronts(lfpm, torch_cuda):
+    """Until tuned, a class runs the workgroup size that keeps the most wavefronts per CU: three
+    4-wavefront workgroups for crop 16 (48 KB of LDS each), one 12-wavefront workgroup for
+    crop 21 with a 19 x 19 window (138 KB: a workgroup may use the device's whole LDS)."""
+    c16 = lfpm.workload_c2(raw_w=320, raw_h=256)
+    c21 = lfpm.workload_c2(raw_w=330, raw_h=280, pitch=31.0, radius=9)
+    _say("case default launch shapes: crop %d, crop %d" % (c16.crop, c21.crop))
+    assert lfpm.autotune_get(c16.pm)[1] == 4
+    assert lfpm.autotune_get(c21.pm)[1] == 12
+
+
 def test_batch_blur_term_matches_plain(lfpm, torch_cuda):
     """The blur term in a batch (Laplacian blur tiles after the intensity tiles): the batched
     solve equals the single-frame solve bit for bit, and both differ from the solve without it."""
