@@ -33,6 +33,9 @@
 //   filter_f32()            : MIDisparityFilterCPU::FilterTile, CheckModule/MIDisparityFilter.cpp:
 //                             423-693 (+ helpers :25-174) with snapshot reads (every read sees
 //                             the pre-filter state), configuration MIStereoMatch.cpp:759-778.
+//   orc_filter_f32_inplace(): the same filter in FilterGlobal's order (MIDisparityFilter.cpp:
+//                             973-985): MIs in map order, each invalidating its rejects before
+//                             the next one reads them.
 //   refocus()               : projection u = (X - c)/v + c_mi of LFRefocus::BackProject2MI,
 //                             FocalStack/LFRefocus.cpp:2100-2125 and :244-281.
 //   fit_v2r()/v2r_convert() : VirtualToRealDepthFunc::BehavioralModel / convertVirtualToRealDepth,
@@ -2270,6 +2273,29 @@ void orc_filter_f32(const float* gray, const float* centers, const int* nbr, con
     c.V = V; c.W = crop; c.K = K; c.gray = gray; c.centers = centers; c.nbr = nbr; c.cnt = cnt;
     c.base = base; c.plane = plane_in; c.cost = cost_in; c.fp = *fp;
     orc::filter_f32(c, disp_in, sel_in, plane_out, cost_out, disp_out, sel_out, reason, nthreads);
+}
+
+// The filter in the reference's own order (MIDisparityFilterCPU::FilterGlobal,
+// CheckModule/MIDisparityFilter.cpp:973-985, with FilterTile's invalidation at :671-686): MIs
+// one after another in map order (tile row, then column: the MI index order of the lens lattice
+// here), each reading its neighbours' disparities and costs as the MIs
+// before it left them and invalidating its rejected pixels before the next MI runs.  The fused
+// GPU pass and orc_filter_f32 read the pre-filter snapshot instead; this order-dependent form
+// measures how far the two differ.
+void orc_filter_f32_inplace(const float* gray, const float* centers, const int* nbr,
+                            const int* cnt, int V, int crop, int K, float base,
+                            const orc::FilterParams* fp, const float* plane_in,
+                            const float* cost_in, const float* disp_in, const uint32_t* sel_in,
+                            float* plane_out, float* cost_out, float* disp_out, uint32_t* sel_out,
+                            uint8_t* reason) {
+    const size_t n = (size_t)V * crop * crop;
+    std::memcpy(plane_out, plane_in, n * 4 * sizeof(float));
+    std::memcpy(cost_out, cost_in, n * sizeof(float));
+    orc::FilterCtx c;
+    c.V = V; c.W = crop; c.K = K; c.gray = gray; c.centers = centers; c.nbr = nbr; c.cnt = cnt;
+    c.base = base; c.plane = plane_out; c.cost = cost_out; c.fp = *fp;
+    // one thread: MI v is filtered, and written back in place, before MI v + 1 reads it
+    orc::filter_f32(c, disp_in, sel_in, plane_out, cost_out, disp_out, sel_out, reason, 1);
 }
 
 void orc_refocus_f64(const uint8_t* tiles, const float* centers, int V, float pitch, int crop,

@@ -275,6 +275,25 @@ def filter_f32(gray, centers, nbr, cnt, crop, K, base, fp, plane, cost, disp, se
     return dict(plane=po, cost=co, disp=do, sel=so, reason=reason)
 
 
+def filter_f32_inplace(gray, centers, nbr, cnt, crop, K, base, fp, plane, cost, disp, sel):
+    """filter_f32 in the reference's order: MI after MI, each reading the disparities and
+    costs the MIs before it left (its rejected pixels invalidated in place)."""
+    V = centers.shape[0]
+    n = V * crop * crop
+    po = np.zeros((n, 4), np.float32)
+    co = np.zeros(n, np.float32)
+    do = np.zeros((n, 4), np.float32)
+    so = np.zeros(n, np.uint32)
+    reason = np.zeros(n, np.uint8)
+    lib().orc_filter_f32_inplace(
+        _p(np.ascontiguousarray(gray, np.float32)), _p(np.ascontiguousarray(centers, np.float32)),
+        _p(np.ascontiguousarray(nbr, np.int32)), _p(np.ascontiguousarray(cnt, np.int32)), V,
+        crop, K, C.c_float(base), C.byref(fp), _p(np.ascontiguousarray(plane, np.float32)),
+        _p(np.ascontiguousarray(cost, np.float32)), _p(np.ascontiguousarray(disp, np.float32)),
+        _p(np.ascontiguousarray(sel, np.uint32)), _p(po), _p(co), _p(do), _p(so), _p(reason))
+    return dict(plane=po, cost=co, disp=do, sel=so, reason=reason)
+
+
 def refocus(tiles_u8, centers, pitch, crop, out_w, out_h, S, vmin, vmax, threads=None):
     V = centers.shape[0]
     out = np.zeros((S, out_h, out_w), np.float64)

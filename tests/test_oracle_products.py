@@ -191,3 +191,48 @@ def test_v2r_convert_clamps(orc):
     assert z[2] == 0.0          # negative -> 0
     assert z[3] == 0.0          # nan -> 0
     assert z[1] > 0
+
+
+def _run_inplace(orc, wl, gray, fp, plane, costs, disp, sel):
+    return orc.filter_f32_inplace(gray, wl.L["centers"], wl.L["nbr"], wl.L["cnt"], wl.crop,
+                                  wl.pm.max_neighbors, wl.pm.base, fp, plane, costs, disp, sel)
+
+
+def test_inplace_order_equals_snapshot_when_nothing_is_dropped(orc, scene):
+    """With every gate open (no pixel invalidated) the reference's in-place order
+    (FilterGlobal, CheckModule/MIDisparityFilter.cpp:973-985) and the snapshot agree exactly."""
+    wl, gray = scene
+    fp = _fp(orc, wl, enable_cycle_photo_filter=0, enable_cycle_geo_filter=0,
+             enable_spike_filter=0, min_good_neighbors=0)
+    args = _inputs(wl)
+    a = _run(orc, wl, gray, fp, *args)
+    b = _run_inplace(orc, wl, gray, fp, *args)
+    assert np.all(a["reason"] == PR_KEEP)
+    for k in a:
+        assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+
+
+def test_inplace_order_only_later_mis_change(orc, scene):
+    """The in-place order differs from the snapshot only in MIs that read a pixel an earlier MI
+    invalidated: MI 0 is identical, and every MI whose reasons differ has a neighbour with a
+    smaller index that dropped a pixel."""
+    wl, gray = scene
+    fp = _fp(orc, wl)
+    plane, costs, disp, sel = _inputs(wl)
+    rng = np.random.default_rng(9)
+    W, V, K = wl.crop, wl.V, wl.pm.max_neighbors
+    # spikes and high costs in a tenth of the MIs invalidate pixels their neighbours read
+    for v in rng.choice(V, V // 10, replace=False):
+        idx = v * W * W + rng.integers(0, W * W, 12)
+        plane[idx[:6], 3] += 1.5
+        costs[idx[6:]] = 1.9
+    a = _run(orc, wl, gray, fp, plane, costs, disp, sel)["reason"].reshape(V, W * W)
+    b = _run_inplace(orc, wl, gray, fp, plane, costs, disp, sel)["reason"].reshape(V, W * W)
+    assert np.array_equal(a[0], b[0])
+    dropped = (b != PR_KEEP).any(axis=1)
+    diff = np.flatnonzero((a != b).any(axis=1))
+    assert diff.size > 0, "the scene must exercise the order dependence"
+    for u in diff:
+        nb = wl.L["nbr"][u * K:(u + 1) * K] if wl.L["nbr"].ndim == 1 else wl.L["nbr"][u]
+        earlier = [n for n in nb[:int(wl.L["cnt"][u])] if 0 <= n < u]
+        assert any(dropped[n] for n in earlier), u
