@@ -152,6 +152,11 @@ struct LightFieldParams {
     // CheckModule filter after the solve (MIStereoMatch.cpp:759-778); on by default
     lfpm_filter_params filter_params{};
     bool run_filter = true;
+    // true: MIs filtered in the reference's map order, each reading the filtered state its
+    // smaller-index neighbours left (MIDisparityFilterCPU::FilterGlobal,
+    // CheckModule/MIDisparityFilter.cpp:973-985), the reference's result; false: every MI reads
+    // the pre-filter snapshot (order-independent, one launch)
+    bool filter_reference_order = true;
     // disparity-mosaic cost gate (DepthSolver.cpp:6802-6805)
     float mosaic_max_cost = 0.2f;
 };

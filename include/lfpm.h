@@ -365,8 +365,10 @@ int lfpm_eval_costs_noalloc(const float* tiles_dev, const float* centers_dev,
 /* ---------------------------------------------------------------------------------------
  * Fused CheckModule pass.  Replaces MIDisparityFilterCPU::FilterGlobal/FilterTile
  * (CheckModule/MIDisparityFilter.cpp:423-693, 962-1009) with configuration
- * MIStereoMatch::FilterAndStatic (MIStereo/MIStereoMatch.cpp:759-778).  Reads a snapshot of
- * the solve (all reads see pre-filter values) and writes filtered copies + reason mask.
+ * MIStereoMatch::FilterAndStatic (MIStereo/MIStereoMatch.cpp:759-778).  lfpm_filter_frame
+ * reads a snapshot of the solve (all reads see pre-filter values, independent of MI order) and
+ * writes filtered copies + reason mask; lfpm_filter_frame_ordered below keeps the reference's
+ * MI order.
  * gray_dev: V*crop^2 floats in 0..255 (the uint8 crops as float).
  * ------------------------------------------------------------------------------------- */
 typedef struct lfpm_filter_params {
@@ -396,6 +398,33 @@ int lfpm_filter_frame(const float* gray_dev, const float* centers_dev,
                       const float* disp_in_dev, const uint32_t* sel_in_dev,
                       float* plane_out_dev, float* cost_out_dev, float* disp_out_dev,
                       uint32_t* sel_out_dev, uint8_t* reason_dev, void* stream);
+
+/* The same pass in FilterGlobal's own order (CheckModule/MIDisparityFilter.cpp:973-985, with
+ * FilterTile's invalidation at :671-686): MIs in index order (the reference's tile-key map
+ * order, tile row then column), each reading the filtered disparities and costs its neighbours
+ * with a smaller index left and the pre-filter values of the others.  The result is the
+ * reference's order-dependent one (rejections cascade to later MIs).
+ *
+ * lfpm_filter_order_levels (host memory, once per layout) groups the MIs into levels: an MI's
+ * level is one more than the largest level of its smaller-index neighbours (0 without any), so
+ * the MIs of one level read no filtered state of each other.  mi_order (num_mis ints) lists
+ * the MIs level by level, level l at mi_order[level_start[l] .. level_start[l+1]);
+ * level_start holds num_levels + 1 ints (at most num_mis + 1).
+ * lfpm_filter_frame_ordered runs one launch per level: mi_order_dev is mi_order copied to the
+ * device, level_start stays in host memory.  Outputs must not alias the inputs. */
+int lfpm_filter_order_levels(const int* neighbor_ids, const int* neighbor_counts, int num_mis,
+                             int max_neighbors, int* mi_order, int* level_start,
+                             int* num_levels);
+int lfpm_filter_frame_ordered(const float* gray_dev, const float* centers_dev,
+                              const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                              int num_mis, int crop, int max_neighbors, float base,
+                              const lfpm_filter_params* f,
+                              const float* plane_in_dev, const float* cost_in_dev,
+                              const float* disp_in_dev, const uint32_t* sel_in_dev,
+                              float* plane_out_dev, float* cost_out_dev, float* disp_out_dev,
+                              uint32_t* sel_out_dev, uint8_t* reason_dev,
+                              const int* mi_order_dev, const int* level_start, int num_levels,
+                              void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * Focal-stack refocus: S slices at virtual depths v_k = vmin + (vmax-vmin)*k/(S-1).
@@ -626,6 +655,12 @@ typedef struct lfpm_batch_frame {
     int refocus_slices;
     float refocus_vmin, refocus_vmax;
     float* refocus_dev;               /* slices*raw_h*raw_w */
+    /* filter order: filter_num_levels == 0 runs lfpm_filter_frame (snapshot); > 0 runs
+     * lfpm_filter_frame_ordered on this schedule (lfpm_filter_order_levels; mi_order on the
+     * device, level_start on the host) */
+    const int* filter_mi_order_dev;
+    const int* filter_level_start;
+    int filter_num_levels;
 } lfpm_batch_frame;
 
 /* Workspace for lfpm_batch_run / lfpm_batch_graph_create / lfpm_batch_step on these frames. */
@@ -752,6 +787,15 @@ const char* lfpm_last_error(void);
 #define LFPM_VARIANT_PLAIN 0
 #define LFPM_VARIANT_ACMM 1
 int lfpm_ctx_set_variant(lfpm_ctx* ctx, int variant);
+
+/* Order of the context's CheckModule pass (every solve call that filters):
+ * LFPM_FILTER_SNAPSHOT = lfpm_filter_frame (the default), LFPM_FILTER_REFERENCE_ORDER =
+ * lfpm_filter_frame_ordered on the schedule the context builds with each MI set, the result of
+ * the reference's MIDisparityFilterCPU::FilterGlobal (CheckModule/MIDisparityFilter.cpp:
+ * 973-985). */
+#define LFPM_FILTER_SNAPSHOT 0
+#define LFPM_FILTER_REFERENCE_ORDER 1
+int lfpm_ctx_set_filter_order(lfpm_ctx* ctx, int order);
 
 /* Upload a layout and a raw host frame, solve it, filter it, and copy results to host.
  * Host outputs (any may be NULL): plane 4*V*crop^2, cost, disp 4*V*crop^2, sel, reason

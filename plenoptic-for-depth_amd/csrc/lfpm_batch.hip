@@ -101,13 +101,20 @@ int frame_products(const lfpm_batch_frame& f, void* stream) {
         if (!f.gray_dev || !f.f_plane_dev || !f.f_cost_dev || !f.f_disp_dev || !f.f_sel_dev ||
             !f.reason_dev)
             return set_error(LFPM_E_INVALID, "batch: filter requested without buffers");
-        if ((rc = lfpm_filter_frame(f.gray_dev, f.centers_dev, f.neighbor_ids_dev,
-                                    f.neighbor_counts_dev, f.num_mis, f.params->crop,
-                                    f.params->max_neighbors, f.params->base, f.filter,
-                                    f.plane_dev, f.cost_dev, f.disp_dev, f.sel_dev, f.f_plane_dev,
-                                    f.f_cost_dev, f.f_disp_dev, f.f_sel_dev, f.reason_dev,
-                                    stream)))
-            return rc;
+        if (f.filter_num_levels > 0)
+            rc = lfpm_filter_frame_ordered(
+                f.gray_dev, f.centers_dev, f.neighbor_ids_dev, f.neighbor_counts_dev, f.num_mis,
+                f.params->crop, f.params->max_neighbors, f.params->base, f.filter, f.plane_dev,
+                f.cost_dev, f.disp_dev, f.sel_dev, f.f_plane_dev, f.f_cost_dev, f.f_disp_dev,
+                f.f_sel_dev, f.reason_dev, f.filter_mi_order_dev, f.filter_level_start,
+                f.filter_num_levels, stream);
+        else
+            rc = lfpm_filter_frame(f.gray_dev, f.centers_dev, f.neighbor_ids_dev,
+                                   f.neighbor_counts_dev, f.num_mis, f.params->crop,
+                                   f.params->max_neighbors, f.params->base, f.filter,
+                                   f.plane_dev, f.cost_dev, f.disp_dev, f.sel_dev, f.f_plane_dev,
+                                   f.f_cost_dev, f.f_disp_dev, f.f_sel_dev, f.reason_dev, stream);
+        if (rc) return rc;
     }
     if (f.refocus_slices > 0) {
         if (!f.refocus_dev || !f.lattice_to_mi_dev)

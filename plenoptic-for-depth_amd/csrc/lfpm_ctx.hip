@@ -66,6 +66,10 @@ struct lfpm_ctx {
     size_t ws_bytes = 0;
     int solved = 0;               // frames of the last solve call
     int variant = 0;              // LFPM_VARIANT_* of single-frame solves
+    int filter_order = LFPM_FILTER_SNAPSHOT;
+    // level schedule of the ordered filter over the current MI set (lfpm_filter_order_levels)
+    int* lvl_order = nullptr;     // device, V ints
+    std::vector<int> lvl_start;   // host, levels + 1 ints
     // per-frame products (sized for the last raw frame)
     void* pws = nullptr;
     size_t pws_bytes = 0;
@@ -92,10 +96,12 @@ static void free_frames(lfpm_ctx* c) {
 
 static void free_tables(lfpm_ctx* c) {
     free_frames(c);
-    void* ps[] = {c->centers, c->origins, c->nbr, c->cnt};
+    void* ps[] = {c->centers, c->origins, c->nbr, c->cnt, c->lvl_order};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     c->centers = nullptr; c->origins = nullptr; c->nbr = nullptr; c->cnt = nullptr;
+    c->lvl_order = nullptr;
+    c->lvl_start.clear();
     c->have_layout = false;
     c->V = c->crop = c->K = 0;
 }
@@ -112,8 +118,36 @@ static int alloc_tables(lfpm_ctx* c, int V, int W, int K) {
     CTX_TRY(hipMalloc((void**)&c->origins, 8 * (size_t)V));
     CTX_TRY(hipMalloc((void**)&c->nbr, 4 * (size_t)V * K));
     CTX_TRY(hipMalloc((void**)&c->cnt, 4 * (size_t)V));
+    CTX_TRY(hipMalloc((void**)&c->lvl_order, 4 * (size_t)V));
     c->V = V; c->crop = W; c->K = K;
     return LFPM_OK;
+}
+
+// Level schedule of the ordered filter from the host neighbour tables of the current MI set;
+// the device copy is ordered on the context's stream before any filter launch.
+static int build_levels(lfpm_ctx* c, const int* nbr, const int* cnt) {
+    std::vector<int> order((size_t)c->V);
+    c->lvl_start.assign((size_t)c->V + 1, 0);
+    int levels = 0;
+    const int rc = lfpm_filter_order_levels(nbr, cnt, c->V, c->K, order.data(),
+                                            c->lvl_start.data(), &levels);
+    if (rc) return rc;
+    c->lvl_start.resize((size_t)levels + 1);
+    CTX_TRY(hipMemcpy(c->lvl_order, order.data(), 4 * (size_t)c->V, hipMemcpyHostToDevice));
+    return LFPM_OK;
+}
+
+static int run_filter(lfpm_ctx* c, FrameBufs& f, const lfpm_pm_params* p,
+                      const lfpm_filter_params* flt) {
+    if (c->filter_order == LFPM_FILTER_REFERENCE_ORDER)
+        return lfpm_filter_frame_ordered(f.gray, c->centers, c->nbr, c->cnt, c->V, c->crop, c->K,
+                                         p->base, flt, f.plane, f.cost, f.disp, f.sel, f.fplane,
+                                         f.fcost, f.fdisp, f.fsel, f.reason, c->lvl_order,
+                                         c->lvl_start.data(), (int)c->lvl_start.size() - 1,
+                                         c->stream);
+    return lfpm_filter_frame(f.gray, c->centers, c->nbr, c->cnt, c->V, c->crop, c->K, p->base,
+                             flt, f.plane, f.cost, f.disp, f.sel, f.fplane, f.fcost, f.fdisp,
+                             f.fsel, f.reason, c->stream);
 }
 
 // Device buffers for n frames (kept when the count and batching match) and the workspace.
@@ -204,6 +238,7 @@ static int ensure_layout(lfpm_ctx* c, const lfpm_layout_params* lp, const lfpm_p
     CTX_TRY(hipMemcpy(c->origins, origins.data(), origins.size() * 4, hipMemcpyHostToDevice));
     CTX_TRY(hipMemcpy(c->nbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice));
     CTX_TRY(hipMemcpy(c->cnt, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice));
+    if ((rc = build_levels(c, nbr.data(), cnt.data()))) return rc;
     c->lp = *lp;
     c->raw_w = lp->raw_w;
     c->raw_h = lp->raw_h;
@@ -237,10 +272,7 @@ static int solve_one(lfpm_ctx* c, FrameBufs& f, const lfpm_pm_params* p,
                                   c->crop, f.gray, c->stream)))
             return rc;
         f.gray_ok = true;
-        if ((rc = lfpm_filter_frame(f.gray, c->centers, c->nbr, c->cnt, c->V, c->crop, c->K,
-                                    p->base, flt, f.plane, f.cost, f.disp, f.sel, f.fplane,
-                                    f.fcost, f.fdisp, f.fsel, f.reason, c->stream)))
-            return rc;
+        if ((rc = run_filter(c, f, p, flt))) return rc;
         f.out_plane = f.fplane;
         f.out_cost = f.fcost;
     }
@@ -299,6 +331,14 @@ int lfpm_ctx_set_variant(lfpm_ctx* c, int variant) {
     if (variant != LFPM_VARIANT_PLAIN && variant != LFPM_VARIANT_ACMM)
         return set_error(LFPM_E_INVALID, "set_variant: unknown variant %d", variant);
     c->variant = variant;  // the next solve re-sizes the workspace for it
+    return LFPM_OK;
+}
+
+int lfpm_ctx_set_filter_order(lfpm_ctx* c, int order) {
+    if (!c) return set_error(LFPM_E_INVALID, "set_filter_order: null ctx");
+    if (order != LFPM_FILTER_SNAPSHOT && order != LFPM_FILTER_REFERENCE_ORDER)
+        return set_error(LFPM_E_INVALID, "set_filter_order: unknown order %d", order);
+    c->filter_order = order;
     return LFPM_OK;
 }
 
@@ -372,6 +412,11 @@ int lfpm_ctx_solve_frames(lfpm_ctx* c, const lfpm_layout_params* lp, int n,
         b.plane_b_dev = fb.plane_b; b.cost_b_dev = fb.cost_b; b.sel_b_dev = fb.sel_b;
         b.f_plane_dev = fb.fplane; b.f_cost_dev = fb.fcost; b.f_disp_dev = fb.fdisp;
         b.f_sel_dev = fb.fsel; b.reason_dev = fb.reason;
+        if (flt && c->filter_order == LFPM_FILTER_REFERENCE_ORDER) {
+            b.filter_mi_order_dev = c->lvl_order;
+            b.filter_level_start = c->lvl_start.data();
+            b.filter_num_levels = (int)c->lvl_start.size() - 1;
+        }
         fb.out_plane = flt ? fb.fplane : fb.plane;
         fb.out_cost = flt ? fb.fcost : fb.cost;
     }
@@ -434,6 +479,7 @@ int lfpm_ctx_solve_tiles(lfpm_ctx* c, const uint8_t* tiles_u8, const float* cent
                            c->stream));
     CTX_TRY(hipMemcpyAsync(c->cnt, neighbor_counts, 4 * (size_t)V, hipMemcpyHostToDevice,
                            c->stream));
+    if ((rc = build_levels(c, neighbor_ids, neighbor_counts))) return rc;
     if ((rc = ensure_frames(c, 1, false, p, false))) return rc;
     FrameBufs& fb = c->frames[0];
     CTX_TRY(hipMemcpyAsync(fb.raw, tiles_u8, (size_t)V * W * W, hipMemcpyHostToDevice,

@@ -536,4 +536,35 @@ int lfpm_read_dmb(const char* path, float* data, size_t capacity, int* height, i
     return LFPM_OK;
 }
 
+// Level schedule of the ordered CheckModule pass: MI v reads the filtered state of every
+// neighbour n < v, so its level is one more than the largest level among those neighbours
+// (0 without any).  MIs of one level read no filtered state of each other; levels run in order.
+int lfpm_filter_order_levels(const int* neighbor_ids, const int* neighbor_counts, int num_mis,
+                             int max_neighbors, int* mi_order, int* level_start,
+                             int* num_levels) {
+    if (!neighbor_ids || !neighbor_counts || !mi_order || !level_start || !num_levels ||
+        num_mis <= 0 || max_neighbors < 1 || max_neighbors > LFPM_MAX_NEIGHBORS)
+        return lfpm::set_error(LFPM_E_INVALID, "filter_order_levels: bad arguments");
+    std::vector<int> level((size_t)num_mis, 0);
+    int top = 0;
+    for (int v = 0; v < num_mis; ++v) {
+        const int c = std::min(neighbor_counts[v], max_neighbors);
+        int l = 0;
+        for (int b = 0; b < c; ++b) {
+            const int n = neighbor_ids[(size_t)v * max_neighbors + b];
+            if (n >= 0 && n < v) l = std::max(l, level[(size_t)n] + 1);
+        }
+        level[(size_t)v] = l;
+        top = std::max(top, l);
+    }
+    const int L = top + 1;
+    std::vector<int> count((size_t)L + 1, 0);
+    for (int v = 0; v < num_mis; ++v) ++count[(size_t)level[(size_t)v] + 1];
+    for (int l = 0; l < L; ++l) count[(size_t)l + 1] += count[(size_t)l];
+    for (int l = 0; l <= L; ++l) level_start[l] = count[(size_t)l];
+    for (int v = 0; v < num_mis; ++v) mi_order[count[(size_t)level[(size_t)v]]++] = v;
+    *num_levels = L;
+    return LFPM_OK;
+}
+
 } // extern "C"

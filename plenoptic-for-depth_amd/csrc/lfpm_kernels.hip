@@ -1797,7 +1797,9 @@ __global__ void __launch_bounds__(256) k_eval_w(EvalArgs a) {
 // ------------------------------------------------------------------------------------------
 // Fused CheckModule pass (MIDisparityFilterCPU::FilterTile, MIDisparityFilter.cpp:423-693).
 // FP32 with a fixed operation order and no contraction, so the reason mask is bit-identical to
-// the CPU twin given identical inputs.  All reads see the pre-filter snapshot.
+// the CPU twin given identical inputs.  Snapshot form: all reads see the pre-filter state.
+// Ordered form (FilterGlobal, MIDisparityFilter.cpp:973-985): an MI reads the filtered states of
+// its neighbours with a smaller index and the pre-filter states of the others.
 // ------------------------------------------------------------------------------------------
 struct FilterArgs {
     const float* gray; const float2* centers; const int* nbr; const int* cnt;
@@ -1806,9 +1808,11 @@ struct FilterArgs {
     lfpm_filter_params f;
     const float4* plane_in; const float* cost_in; const float4* disp_in; const uint32_t* sel_in;
     float4* plane_out; float* cost_out; float4* disp_out; uint32_t* sel_out; uint8_t* reason;
+    // ordered form: the MIs of the level this launch runs (mi_order[level_begin + block])
+    const int* mi_order; int level_begin;
 };
 
-__device__ float f_bilin_disp(const FilterArgs& a, int n, float x, float y) {
+__device__ float f_bilin_disp(const FilterArgs& a, const float4* pl, int n, float x, float y) {
 #pragma clang fp contract(off)
     const int W = a.W;
     if (x < 0 || y < 0 || x > W - 1 || y > W - 1) return __int_as_float(0x7fc00000);
@@ -1819,7 +1823,7 @@ __device__ float f_bilin_disp(const FilterArgs& a, int n, float x, float y) {
     float d[4];
     const int xi[4] = {x0, x0, x1, x1}, yi[4] = {y0, y1, y0, y1};
     for (int q = 0; q < 4; ++q) {
-        float v = a.plane_in[b + yi[q] * W + xi[q]].w;
+        float v = pl[b + yi[q] * W + xi[q]].w;
         if (!isfinite(v) || v <= a.f.min_valid_disp) return __int_as_float(0x7fc00000);
         d[q] = v;
     }
@@ -1827,7 +1831,7 @@ __device__ float f_bilin_disp(const FilterArgs& a, int n, float x, float y) {
            ax * ay * d[3];
 }
 
-__device__ float f_bilin_cost(const FilterArgs& a, int n, float x, float y) {
+__device__ float f_bilin_cost(const FilterArgs& a, const float* co, int n, float x, float y) {
 #pragma clang fp contract(off)
     const int W = a.W;
     if (x < 0 || y < 0 || x > W - 1 || y > W - 1) return __int_as_float(0x7fc00000);
@@ -1835,8 +1839,8 @@ __device__ float f_bilin_cost(const FilterArgs& a, int n, float x, float y) {
     const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, W - 1);
     const float ax = x - (float)x0, ay = y - (float)y0;
     const size_t b = (size_t)n * W * W;
-    const float c00 = a.cost_in[b + y0 * W + x0], c01 = a.cost_in[b + y1 * W + x0];
-    const float c10 = a.cost_in[b + y0 * W + x1], c11 = a.cost_in[b + y1 * W + x1];
+    const float c00 = co[b + y0 * W + x0], c01 = co[b + y1 * W + x0];
+    const float c10 = co[b + y0 * W + x1], c11 = co[b + y1 * W + x1];
     if (!isfinite(c00) || !isfinite(c01) || !isfinite(c10) || !isfinite(c11))
         return __int_as_float(0x7fc00000);
     return (1.0f - ax) * (1.0f - ay) * c00 + (1.0f - ax) * ay * c01 + ax * (1.0f - ay) * c10 +
@@ -1895,15 +1899,25 @@ __device__ float f_photo(const FilterArgs& a, int v, int n, float px, float py, 
     return fabsf(In - Ir);
 }
 
+template <bool ORDERED>
 __global__ void __launch_bounds__(1024) k_filter(FilterArgs a) {
 #pragma clang fp contract(off)
     __shared__ uint8_t keep1[LFPM_MAX_CROP * LFPM_MAX_CROP];
-    // workgroups are dealt to the 8 XCDs in turn: each XCD takes a contiguous MI range, so the
-    // neighbour states an MI reads are mostly read by the same XCD (one L2)
-    const int per_xcd = (a.V + 7) / 8;
-    const int v = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-    if (v >= a.V) return;  // uniform across the workgroup, before any barrier
-    const int W = a.W, WW = W * W, t = threadIdx.x;
+    const int t = threadIdx.x;
+    int v;
+    if (ORDERED) {
+        // one MI of the current level; the levels before it were completed by earlier
+        // launches
+        v = a.mi_order[a.level_begin + (int)blockIdx.x];
+        if (v < 0 || v >= a.V) return;  // uniform across the workgroup, before any barrier
+    } else {
+        // workgroups are dealt to the 8 XCDs in turn: each XCD takes a contiguous MI range, so
+        // the neighbour states an MI reads are mostly read by the same XCD (one L2)
+        const int per_xcd = (a.V + 7) / 8;
+        v = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+        if (v >= a.V) return;  // uniform across the workgroup, before any barrier
+    }
+    const int W = a.W, WW = W * W;
     const lfpm_filter_params& fp = a.f;
     const int cntv = min(a.cnt[v], a.K);
     const size_t base = (size_t)v * WW;
@@ -1953,10 +1967,12 @@ __global__ void __launch_bounds__(1024) k_filter(FilterArgs a) {
                 const float qx = px + dx * d * scale;
                 const float qy = py + dy * d * scale;
                 if (qx < 0 || qy < 0 || qx > W - 1 || qy > W - 1) continue;
-                const float dn = f_bilin_disp(a, n, qx, qy);
+                const bool filtered = ORDERED && n < v;
+                const float dn = f_bilin_disp(a, filtered ? a.plane_out : a.plane_in, n, qx, qy);
                 if (!isfinite(dn)) continue;
                 if (fp.enable_cost_filter) {
-                    const float nc = f_bilin_cost(a, n, qx, qy);
+                    const float nc =
+                        f_bilin_cost(a, filtered ? a.cost_out : a.cost_in, n, qx, qy);
                     const bool inv = !isfinite(nc) || nc < 0.0f;
                     const bool hi = isfinite(nc) && nc > fp.max_cost;
                     if (inv || hi) continue;
@@ -1983,53 +1999,54 @@ __global__ void __launch_bounds__(1024) k_filter(FilterArgs a) {
         keep1[t] = k1 ? 1 : 0;
     }
     __syncthreads();
-    if (t >= WW) return;
-    bool kf = k1;
-    if (k1 && fp.enable_spike_filter) {
-        const int x = t % W, y = t / W;
-        float vals[8];
-        int nvals = 0;
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                if (dx == 0 && dy == 0) continue;
-                const int xn = x + dx, yn = y + dy;
-                if (xn < 0 || yn < 0 || xn >= W || yn >= W) continue;
-                const int ni = yn * W + xn;
-                if (!keep1[ni]) continue;
-                const float dv = a.plane_in[base + ni].w;
-                if (!isfinite(dv) || dv <= fp.min_valid_disp) continue;
-                vals[nvals++] = dv;
-            }
-        if (nvals >= fp.spike_min_neighbors) {
-            for (int i = 1; i < nvals; ++i) {  // insertion sort (<= 8 values)
-                const float key = vals[i];
-                int j = i - 1;
-                while (j >= 0 && vals[j] > key) { vals[j + 1] = vals[j]; --j; }
-                vals[j + 1] = key;
-            }
-            const float med = vals[nvals / 2];
-            if (isfinite(d) && fabsf(d - med) > fp.spike_abs_diff) {
-                kf = false;
-                reason = LFPM_PR_SPIKE;
+    if (t < WW) {
+        bool kf = k1;
+        if (k1 && fp.enable_spike_filter) {
+            const int x = t % W, y = t / W;
+            float vals[8];
+            int nvals = 0;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (dx == 0 && dy == 0) continue;
+                    const int xn = x + dx, yn = y + dy;
+                    if (xn < 0 || yn < 0 || xn >= W || yn >= W) continue;
+                    const int ni = yn * W + xn;
+                    if (!keep1[ni]) continue;
+                    const float dv = a.plane_in[base + ni].w;
+                    if (!isfinite(dv) || dv <= fp.min_valid_disp) continue;
+                    vals[nvals++] = dv;
+                }
+            if (nvals >= fp.spike_min_neighbors) {
+                for (int i = 1; i < nvals; ++i) {  // insertion sort (<= 8 values)
+                    const float key = vals[i];
+                    int j = i - 1;
+                    while (j >= 0 && vals[j] > key) { vals[j + 1] = vals[j]; --j; }
+                    vals[j + 1] = key;
+                }
+                const float med = vals[nvals / 2];
+                if (isfinite(d) && fabsf(d - med) > fp.spike_abs_diff) {
+                    kf = false;
+                    reason = LFPM_PR_SPIKE;
+                }
             }
         }
+        const size_t g = base + t;
+        float4 po = a.plane_in[g];
+        float co = a.cost_in[g];
+        float4 dout = a.disp_in[g];
+        uint32_t so = a.sel_in[g];
+        if (isfinite(d) && d > fp.min_valid_disp && !kf) {
+            po = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+            co = 2.0f;
+            dout = make_float4(-1.0f, -1.0f, 0.0f, 0.0f);
+            if (fp.clear_selected_views_when_invalid) so = 0u;
+        }
+        a.plane_out[g] = po;
+        a.cost_out[g] = co;
+        a.disp_out[g] = dout;
+        a.sel_out[g] = so;
+        a.reason[g] = reason;
     }
-    const size_t g = base + t;
-    float4 po = a.plane_in[g];
-    float co = a.cost_in[g];
-    float4 dout = a.disp_in[g];
-    uint32_t so = a.sel_in[g];
-    if (isfinite(d) && d > fp.min_valid_disp && !kf) {
-        po = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-        co = 2.0f;
-        dout = make_float4(-1.0f, -1.0f, 0.0f, 0.0f);
-        if (fp.clear_selected_views_when_invalid) so = 0u;
-    }
-    a.plane_out[g] = po;
-    a.cost_out[g] = co;
-    a.disp_out[g] = dout;
-    a.sel_out[g] = so;
-    a.reason[g] = reason;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -3199,13 +3216,14 @@ int lfpm_eval_costs_noalloc(const float* tiles_dev, const float* centers_dev,
                            num_mis, p, hyp_dev, n, cost_out_dev, seg, (hipStream_t)stream);
 }
 
-int lfpm_filter_frame(const float* gray_dev, const float* centers_dev,
-                      const int* neighbor_ids_dev, const int* neighbor_counts_dev, int num_mis,
-                      int crop, int max_neighbors, float base, const lfpm_filter_params* f,
-                      const float* plane_in_dev, const float* cost_in_dev,
-                      const float* disp_in_dev, const uint32_t* sel_in_dev, float* plane_out_dev,
-                      float* cost_out_dev, float* disp_out_dev, uint32_t* sel_out_dev,
-                      uint8_t* reason_dev, void* stream) {
+static int filter_impl(bool ordered, const float* gray_dev, const float* centers_dev,
+                       const int* neighbor_ids_dev, const int* neighbor_counts_dev, int num_mis,
+                       int crop, int max_neighbors, float base, const lfpm_filter_params* f,
+                       const float* plane_in_dev, const float* cost_in_dev,
+                       const float* disp_in_dev, const uint32_t* sel_in_dev,
+                       float* plane_out_dev, float* cost_out_dev, float* disp_out_dev,
+                       uint32_t* sel_out_dev, uint8_t* reason_dev, const int* mi_order_dev,
+                       const int* level_start, int num_levels, hipStream_t s) {
     if (!f || num_mis <= 0 || crop < 3 || crop > LFPM_MAX_CROP || max_neighbors < 1 ||
         max_neighbors > LFPM_MAX_NEIGHBORS || !(base > 0.0f))
         return set_error(LFPM_E_INVALID, "filter: bad parameters");
@@ -3223,10 +3241,59 @@ int lfpm_filter_frame(const float* gray_dev, const float* centers_dev,
     a.disp_in = (const float4*)disp_in_dev; a.sel_in = sel_in_dev;
     a.plane_out = (float4*)plane_out_dev; a.cost_out = cost_out_dev;
     a.disp_out = (float4*)disp_out_dev; a.sel_out = sel_out_dev; a.reason = reason_dev;
+    a.mi_order = mi_order_dev; a.level_begin = 0;
     const int NT = 64 * ((crop * crop + 63) / 64);
-    hipLaunchKernelGGL(k_filter, dim3(8 * ((num_mis + 7) / 8)), dim3(NT), 0,
-                       (hipStream_t)stream, a);
-    return launch_check("filter");
+    if (!ordered) {
+        hipLaunchKernelGGL(k_filter<false>, dim3(8 * ((num_mis + 7) / 8)), dim3(NT), 0, s, a);
+        return launch_check("filter");
+    }
+    // one launch per level, in order: the kernel boundary makes a level's outputs visible to
+    // the next
+    for (int l = 0; l < num_levels; ++l) {
+        a.level_begin = level_start[l];
+        const int n = level_start[l + 1] - level_start[l];
+        hipLaunchKernelGGL(k_filter<true>, dim3(n), dim3(NT), 0, s, a);
+    }
+    return launch_check("filter_ordered");
+}
+
+int lfpm_filter_frame(const float* gray_dev, const float* centers_dev,
+                      const int* neighbor_ids_dev, const int* neighbor_counts_dev, int num_mis,
+                      int crop, int max_neighbors, float base, const lfpm_filter_params* f,
+                      const float* plane_in_dev, const float* cost_in_dev,
+                      const float* disp_in_dev, const uint32_t* sel_in_dev, float* plane_out_dev,
+                      float* cost_out_dev, float* disp_out_dev, uint32_t* sel_out_dev,
+                      uint8_t* reason_dev, void* stream) {
+    return filter_impl(false, gray_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev,
+                       num_mis, crop, max_neighbors, base, f, plane_in_dev, cost_in_dev,
+                       disp_in_dev, sel_in_dev, plane_out_dev, cost_out_dev, disp_out_dev,
+                       sel_out_dev, reason_dev, nullptr, nullptr, 0, (hipStream_t)stream);
+}
+
+int lfpm_filter_frame_ordered(const float* gray_dev, const float* centers_dev,
+                              const int* neighbor_ids_dev, const int* neighbor_counts_dev,
+                              int num_mis, int crop, int max_neighbors, float base,
+                              const lfpm_filter_params* f, const float* plane_in_dev,
+                              const float* cost_in_dev, const float* disp_in_dev,
+                              const uint32_t* sel_in_dev, float* plane_out_dev,
+                              float* cost_out_dev, float* disp_out_dev, uint32_t* sel_out_dev,
+                              uint8_t* reason_dev, const int* mi_order_dev,
+                              const int* level_start, int num_levels, void* stream) {
+    if (!mi_order_dev || !level_start || num_levels < 1 || num_levels > num_mis)
+        return set_error(LFPM_E_INVALID, "filter_ordered: no level schedule");
+    if (level_start[0] != 0 || level_start[num_levels] != num_mis)
+        return set_error(LFPM_E_INVALID, "filter_ordered: the levels must cover MIs 0..%d",
+                         num_mis - 1);
+    for (int l = 0; l < num_levels; ++l)
+        if (level_start[l + 1] <= level_start[l])
+            return set_error(LFPM_E_INVALID, "filter_ordered: level %d is empty or reversed", l);
+    if (plane_out_dev == plane_in_dev || cost_out_dev == cost_in_dev)
+        return set_error(LFPM_E_INVALID, "filter_ordered: outputs must not alias the inputs");
+    return filter_impl(true, gray_dev, centers_dev, neighbor_ids_dev, neighbor_counts_dev,
+                       num_mis, crop, max_neighbors, base, f, plane_in_dev, cost_in_dev,
+                       disp_in_dev, sel_in_dev, plane_out_dev, cost_out_dev, disp_out_dev,
+                       sel_out_dev, reason_dev, mi_order_dev, level_start, num_levels,
+                       (hipStream_t)stream);
 }
 
 int lfpm_refocus(const float* tiles_dev, const float* centers_dev, const int* lattice_to_mi_dev,

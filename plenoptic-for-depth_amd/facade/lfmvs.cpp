@@ -185,6 +185,10 @@ void AdaptMIPMFrame::Solve(int variant) {
     if (m_bReleased || m_num_views <= 0)
         throw std::runtime_error("AdaptMIPMFrame: RunPatchMatchCUDAForFrame before Initialize");
     check(lfpm_ctx_set_variant(Context(), variant), "AdaptMIPMFrame: solver variant");
+    check(lfpm_ctx_set_filter_order(Context(), m_Params.filter_reference_order
+                                                   ? LFPM_FILTER_REFERENCE_ORDER
+                                                   : LFPM_FILTER_SNAPSHOT),
+          "AdaptMIPMFrame: filter order");
     const size_t n = (size_t)m_num_views * m_crop * m_crop;
     plane_final_host.assign(4 * n, 0.0f);
     cost_final_host.assign(n, 0.0f);
@@ -532,6 +536,10 @@ int DepthSolver::ProcessAllFramesBatched() {
     }
     if (!m_frame) m_frame.reset(new AdaptMIPMFrame(m_Params));
     check(lfpm_ctx_set_variant(m_frame->Context(), LFPM_VARIANT_PLAIN),
+          "DepthSolver::ProcessAllFramesBatched");
+    check(lfpm_ctx_set_filter_order(m_frame->Context(), m_Params.filter_reference_order
+                                                            ? LFPM_FILTER_REFERENCE_ORDER
+                                                            : LFPM_FILTER_SNAPSHOT),
           "DepthSolver::ProcessAllFramesBatched");
     const bool filt = m_Params.run_filter;
     check(lfpm_ctx_solve_frames(m_frame->Context(), &m_lp, n, raws.data(), &p,

@@ -504,8 +504,14 @@ def eval_costs(df, hyp, pm=None, stream=None):
     return out
 
 
-def filter_frame(df, bufs, fp=None, stream=None):
+def filter_frame(df, bufs, fp=None, stream=None, order="snapshot"):
+    """CheckModule pass over a solve (lfpm_filter_frame).  order="reference" runs
+    lfpm_filter_frame_ordered: MIs in the reference's map order, rejections cascading to later
+    MIs (MIDisparityFilterCPU::FilterGlobal, CheckModule/MIDisparityFilter.cpp:973-985), on the
+    device frame's level schedule (built once, see filter_levels)."""
     import torch
+    if order not in ("snapshot", "reference"):
+        raise ValueError("filter_frame: order must be 'snapshot' or 'reference'")
     fp = fp or df.wl.fp
     V, W = df.wl.V, df.wl.crop
     n = V * W * W
@@ -515,12 +521,48 @@ def filter_frame(df, bufs, fp=None, stream=None):
                disp=torch.empty((n, 4), dtype=torch.float32, device=dev),
                sel=torch.empty(n, dtype=torch.int32, device=dev),
                reason=torch.empty(n, dtype=torch.uint8, device=dev))
-    _check(lib().lfpm_filter_frame(
-        _dp(df.gray), _dp(df.centers), _dp(df.nbr), _dp(df.cnt), V, W, df.wl.pm.max_neighbors,
-        C.c_float(df.wl.pm.base), C.byref(fp), _dp(bufs.plane), _dp(bufs.cost), _dp(bufs.disp),
-        _dp(bufs.sel), _dp(out["plane"]), _dp(out["cost"]), _dp(out["disp"]), _dp(out["sel"]),
-        _dp(out["reason"]), _stream(stream)))
+    args = (_dp(df.gray), _dp(df.centers), _dp(df.nbr), _dp(df.cnt), V, W,
+            df.wl.pm.max_neighbors, C.c_float(df.wl.pm.base), C.byref(fp), _dp(bufs.plane),
+            _dp(bufs.cost), _dp(bufs.disp), _dp(bufs.sel), _dp(out["plane"]), _dp(out["cost"]),
+            _dp(out["disp"]), _dp(out["sel"]), _dp(out["reason"]))
+    if order == "snapshot":
+        _check(lib().lfpm_filter_frame(*args, _stream(stream)))
+        return out
+    order_d, start_h = _df_levels(df)
+    _check(lib().lfpm_filter_frame_ordered(
+        *args, _dp(order_d), start_h.ctypes.data_as(C.POINTER(C.c_int)), len(start_h) - 1,
+        _stream(stream)))
     return out
+
+
+def _df_levels(df):
+    """The ordered filter's level schedule of a device frame's layout, built once: (MI order on
+    the device, level boundaries on the host)."""
+    import torch
+    sched = getattr(df, "_filter_levels", None)
+    if sched is None:
+        order_h, start_h = filter_levels(df.wl.L["nbr"], df.wl.L["cnt"], df.wl.pm.max_neighbors)
+        sched = (torch.from_numpy(order_h).to(df.nbr.device), start_h)
+        df._filter_levels = sched
+    return sched
+
+
+def filter_levels(nbr, cnt, K):
+    """Level schedule of the ordered filter (lfpm_filter_order_levels, host only): the MIs level
+    by level (int32 [V]) and the level boundaries (int32 [levels + 1])."""
+    nbr = np.ascontiguousarray(nbr, np.int32).reshape(-1)
+    cnt = np.ascontiguousarray(cnt, np.int32)
+    V = cnt.shape[0]
+    if nbr.shape[0] != V * K:
+        raise ValueError("filter_levels: neighbour table is not V x K")
+    order = np.zeros(V, np.int32)
+    start = np.zeros(V + 1, np.int32)
+    nl = C.c_int(0)
+    P = C.POINTER(C.c_int)
+    _check(lib().lfpm_filter_order_levels(nbr.ctypes.data_as(P), cnt.ctypes.data_as(P), V, K,
+                                          order.ctypes.data_as(P), start.ctypes.data_as(P),
+                                          C.byref(nl)))
+    return order, start[:nl.value + 1].copy()
 
 
 def refocus(df, slices=None, vmin=None, vmax=None, out=None, stream=None):
@@ -715,6 +757,14 @@ class Context:
         """Solver used by solve_frame / solve_tiles: Context.PLAIN or Context.ACMM."""
         _check(lib().lfpm_ctx_set_variant(self.h, int(variant)))
 
+    def set_filter_order(self, order):
+        """'snapshot' (LFPM_FILTER_SNAPSHOT, the default) or 'reference'
+        (LFPM_FILTER_REFERENCE_ORDER): the order of the filter pass of every later solve."""
+        codes = {"snapshot": 0, "reference": 1}
+        if order not in codes:
+            raise ValueError("set_filter_order: order must be 'snapshot' or 'reference'")
+        _check(lib().lfpm_ctx_set_filter_order(self.h, codes[order]))
+
     @staticmethod
     def _outs(n):
         return dict(plane=np.zeros((n, 4), np.float32), cost=np.zeros(n, np.float32),
@@ -839,15 +889,21 @@ class BatchFrame(C.Structure):
                 ("sel_b_dev", C.c_void_p), ("f_plane_dev", C.c_void_p), ("f_cost_dev", C.c_void_p),
                 ("f_disp_dev", C.c_void_p), ("f_sel_dev", C.c_void_p), ("reason_dev", C.c_void_p),
                 ("refocus_slices", C.c_int), ("refocus_vmin", C.c_float),
-                ("refocus_vmax", C.c_float), ("refocus_dev", C.c_void_p)]
+                ("refocus_vmax", C.c_float), ("refocus_dev", C.c_void_p),
+                ("filter_mi_order_dev", C.c_void_p),
+                ("filter_level_start", C.POINTER(C.c_int)), ("filter_num_levels", C.c_int)]
 
 
 class FrameSlot:
     """Device-resident inputs + state + products of one frame in a batch (one serving slot)."""
 
-    def __init__(self, wl, device="cuda", filt=True, refocus=True, blur_raw=None, pm=None):
+    def __init__(self, wl, device="cuda", filt=True, refocus=True, blur_raw=None, pm=None,
+                 filter_order="snapshot"):
         import torch
+        if filter_order not in ("snapshot", "reference"):
+            raise ValueError("FrameSlot: filter_order must be 'snapshot' or 'reference'")
         self.wl = wl
+        self.filter_order = filter_order
         self.df = DeviceFrame(wl, device=device, blur_raw=blur_raw)
         V, W = wl.V, wl.crop
         n = V * W * W
@@ -899,6 +955,11 @@ class FrameSlot:
             d.f_plane_dev, d.f_cost_dev, d.f_disp_dev = (P(self.f_plane), P(self.f_cost),
                                                          P(self.f_disp))
             d.f_sel_dev, d.reason_dev = P(self.f_sel), P(self.reason)
+            if self.filter_order == "reference":
+                order_d, start_h = _df_levels(df)
+                d.filter_mi_order_dev = P(order_d)
+                d.filter_level_start = start_h.ctypes.data_as(C.POINTER(C.c_int))
+                d.filter_num_levels = len(start_h) - 1
         d.refocus_slices = self.slices
         d.refocus_vmin, d.refocus_vmax = self.wl.vmin, self.wl.vmax
         d.refocus_dev = P(self.refocus)

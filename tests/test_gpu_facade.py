@@ -1,7 +1,7 @@
 """C++ facades (namespace LFMVS, include/lfmvs.hpp) on the GPU, driven like the reference's
 main() drives DepthSolverManager: DepthSolverPipelineBySequence and DepthSolverPipelineByBatch
 over two synthetic frames give, per frame, the C-ABI path's filtered planes, costs, selections
-and CheckModule reasons bit for bit, and the oracle's disparity mosaic, virtual-depth map and
+and CheckModule reasons (the reference's MI order, the facades' default) bit for bit, and the oracle's disparity mosaic, virtual-depth map and
 colour image; AdaptMIPM on one micro-image equals lfpm_ctx_solve_tiles on the same work list and
 agrees with the FP64 oracle."""
 import copy
@@ -49,6 +49,7 @@ def expected(lfpm, orc):
     if not os.path.exists(DRIVER):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp")])
     ctx = lfpm.Context()
+    ctx.set_filter_order("reference")  # the facades' default (LightFieldParams)
     exp = []
     try:
         for i in range(2):
@@ -125,6 +126,7 @@ def test_facade_acmm_matcher(lfpm, tmp_path):
     ctx = lfpm.Context()
     try:
         ctx.set_variant(lfpm.Context.ACMM)
+        ctx.set_filter_order("reference")
         exp = ctx.solve_frame(wl)
     finally:
         ctx.close()

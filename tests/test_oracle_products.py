@@ -236,3 +236,73 @@ def test_inplace_order_only_later_mis_change(orc, scene):
         nb = wl.L["nbr"][u * K:(u + 1) * K] if wl.L["nbr"].ndim == 1 else wl.L["nbr"][u]
         earlier = [n for n in nb[:int(wl.L["cnt"][u])] if 0 <= n < u]
         assert any(dropped[n] for n in earlier), u
+
+
+def _levels_brute(nbr, cnt, K):
+    """Longest chain of smaller-index neighbours ending at each MI, by plain recursion."""
+    V = len(cnt)
+    nbr = np.asarray(nbr).reshape(V, K)
+    memo = {}
+
+    def lev(v):
+        if v not in memo:
+            ds = [int(n) for n in nbr[v, :min(int(cnt[v]), K)] if 0 <= n < v]
+            memo[v] = 1 + max(lev(n) for n in ds) if ds else 0
+        return memo[v]
+
+    return np.array([lev(v) for v in range(V)])
+
+
+@pytest.mark.parametrize("which", ["c1", "c2"])
+def test_filter_levels_schedule(lfpm, which):
+    """lfpm_filter_order_levels: a permutation of the MIs, level by level in index order within
+    a level; each MI's level is one more than the largest level of its smaller-index neighbours
+    (brute force), so no MI reads the filtered state of an MI of its own or a later level."""
+    wl = (lfpm.workload_c1(raw_w=400, raw_h=320) if which == "c1"
+          else lfpm.workload_c2(raw_w=1024, raw_h=800))
+    K = wl.pm.max_neighbors
+    order, start = lfpm.filter_levels(wl.L["nbr"], wl.L["cnt"], K)
+    want = _levels_brute(wl.L["nbr"], wl.L["cnt"], K)
+    assert np.array_equal(np.sort(order), np.arange(wl.V))
+    assert start[0] == 0 and start[-1] == wl.V and np.all(np.diff(start) > 0)
+    got = np.empty(wl.V, np.int64)
+    for l in range(len(start) - 1):
+        members = order[start[l]:start[l + 1]]
+        assert np.all(np.diff(members) > 0)
+        got[members] = l
+    assert np.array_equal(got, want)
+    assert len(start) - 1 == want.max() + 1
+
+
+def test_filter_levels_small_cases(lfpm):
+    """A chain 0 <- 1 <- 2 gives three levels; MIs without smaller-index neighbours share
+    level 0; -1 padding and counts below K are respected."""
+    K = 2
+    nbr = np.array([[1, -1], [0, 2], [1, -1]], np.int32)
+    order, start = lfpm.filter_levels(nbr, np.array([1, 2, 1], np.int32), K)
+    assert list(order) == [0, 1, 2] and list(start) == [0, 1, 2, 3]
+    # MI 2's only listed neighbour is beyond its count: it joins level 0
+    order, start = lfpm.filter_levels(nbr, np.array([1, 1, 0], np.int32), K)
+    assert list(order) == [0, 2, 1] and list(start) == [0, 2, 3]
+
+
+def test_filter_ordered_rejects_bad_schedules(lfpm):
+    """Host-side validation of lfpm_filter_frame_ordered, before any device work."""
+    import ctypes as C
+    fp = lfpm.filter_default()
+    fake = C.c_void_p(256)
+
+    def call(start, nl, order=fake, out_plane=C.c_void_p(512)):
+        s = np.asarray(start, np.int32)
+        return lfpm.lib().lfpm_filter_frame_ordered(
+            fake, fake, fake, fake, 3, 16, 2, C.c_float(23.0), C.byref(fp), fake, fake, fake,
+            fake, out_plane, C.c_void_p(1024), fake, fake, fake, order,
+            s.ctypes.data_as(C.POINTER(C.c_int)), nl, None)
+
+    assert call([0, 2, 3], 2, order=None) == -1
+    assert call([0, 2, 4], 2) == -1          # does not end at num_mis
+    assert b"cover" in lfpm.lib().lfpm_last_error()
+    assert call([0, 2, 2, 3], 3) == -1       # empty level
+    assert call([0, 3], 0) == -1
+    assert call([0, 2, 3], 2, out_plane=fake) == -1   # output aliases the input
+    assert b"alias" in lfpm.lib().lfpm_last_error()
