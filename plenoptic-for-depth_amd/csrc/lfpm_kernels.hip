@@ -1899,6 +1899,10 @@ __device__ float f_photo(const FilterArgs& a, int v, int n, float px, float py, 
     return fabsf(In - Ir);
 }
 
+// Views whose loads a pixel issues together before deciding on them (one memory round per
+// group instead of one per view and function).
+constexpr int kFilterViewGroup = 2;
+
 template <bool ORDERED>
 __global__ void __launch_bounds__(1024) k_filter(FilterArgs a) {
 #pragma clang fp contract(off)
@@ -1921,76 +1925,194 @@ __global__ void __launch_bounds__(1024) k_filter(FilterArgs a) {
     const lfpm_filter_params& fp = a.f;
     const int cntv = min(a.cnt[v], a.K);
     const size_t base = (size_t)v * WW;
+    // the MI's neighbour ids and centres, shared by its pixels; the pixel's own state is loaded
+    // before the barrier so both sets of loads are in flight together
+    __shared__ int s_nb[LFPM_MAX_NEIGHBORS];
+    __shared__ float2 s_cn[LFPM_MAX_NEIGHBORS];
+    if (t < cntv) {
+        const int n = a.nbr[(size_t)v * a.K + t];
+        s_nb[t] = n;
+        s_cn[t] = n >= 0 ? a.centers[n] : make_float2(0.0f, 0.0f);
+    }
+    float d = 0.0f, cost0 = 0.0f;
+    uint32_t sel0 = 0u;
+    // the pixel's own photometric terms, the same for every view
+    float Ir = 0.0f, mr = 0.0f, sr = 0.0f;
+    bool own3 = false;
+    if (t < WW) {
+        d = a.plane_in[base + t].w;
+        cost0 = a.cost_in[base + t];
+        sel0 = a.sel_in[base + t];
+        if (fp.enable_cycle_photo_filter) {
+            const float px = (float)(t % W), py = (float)(t / W);
+            f_bilin_gray(a, v, px, py, Ir);
+            own3 = f_mean_std3(a, v, px, py, mr, sr);
+        }
+    }
+    const float2 cv = a.centers[v];
+    __syncthreads();
     uint8_t reason = LFPM_PR_KEEP;
     bool k1 = false;
-    float d = 0.0f;
     if (t < WW) {
         const int x = t % W, y = t / W;
-        d = a.plane_in[base + t].w;
         bool done = !isfinite(d) || d <= fp.min_valid_disp;
         if (!done && fp.enable_cost_filter) {
-            const float cost = a.cost_in[base + t];
-            const bool invalid = !isfinite(cost) || cost < 0.0f;
-            const bool high = isfinite(cost) && cost > fp.max_cost;
+            const bool invalid = !isfinite(cost0) || cost0 < 0.0f;
+            const bool high = isfinite(cost0) && cost0 > fp.max_cost;
             if (invalid || high) { reason = LFPM_PR_COST; done = true; }
         }
-        int vids[32];
+        // views in order: the first max_triplet set bits of the selection, or the first
+        // max_triplet neighbours (bit b of m = neighbour slot b)
+        uint32_t m = 0u;
         int nv = 0;
         if (!done) {
             if (fp.use_selected_views_only) {
-                uint32_t m = a.sel_in[base + t];
-                for (int b = 0; m && nv < fp.max_triplet; ++b) {
-                    if (m & 1u) vids[nv++] = b;
-                    m >>= 1u;
-                }
+                m = sel0;
+                nv = min(__popc(m), fp.max_triplet);
             } else {
-                const int lim = min(cntv, fp.max_triplet);
-                for (int b = 0; b < lim; ++b) vids[nv++] = b;
+                nv = min(cntv, fp.max_triplet);
+                m = nv >= 32 ? 0xFFFFFFFFu : ((1u << nv) - 1u);
             }
             if (nv == 0) { reason = LFPM_PR_NOSUPPORT; done = true; }
         }
         if (!done) {
             const float px = (float)x, py = (float)y;
             int support = 0, good = 0;
-            const float2 cv = a.centers[v];
-            for (int vi = 0; vi < nv; ++vi) {
-                const int b = vids[vi];
-                if (b < 0 || b >= cntv) continue;
-                const int n = a.nbr[(size_t)v * a.K + b];
-                if (n < 0) continue;
-                const float2 cn = a.centers[n];
-                const float Bx = cv.x - cn.x, By = cv.y - cn.y;
-                const float bl = sqrtf(Bx * Bx + By * By);
-                if (bl < 1.0e-6f) continue;
-                const float dx = Bx / bl, dy = By / bl;
-                const float scale = bl / a.base;
-                const float qx = px + dx * d * scale;
-                const float qy = py + dy * d * scale;
-                if (qx < 0 || qy < 0 || qx > W - 1 || qy > W - 1) continue;
-                const bool filtered = ORDERED && n < v;
-                const float dn = f_bilin_disp(a, filtered ? a.plane_out : a.plane_in, n, qx, qy);
-                if (!isfinite(dn)) continue;
-                if (fp.enable_cost_filter) {
-                    const float nc =
-                        f_bilin_cost(a, filtered ? a.cost_out : a.cost_in, n, qx, qy);
-                    const bool inv = !isfinite(nc) || nc < 0.0f;
-                    const bool hi = isfinite(nc) && nc > fp.max_cost;
-                    if (inv || hi) continue;
+            constexpr int G = kFilterViewGroup;
+            for (int g0 = 0; g0 < nv; g0 += G) {
+                // phase 1: geometry of up to G views and every load they need, all in flight
+                bool ok[G], ok3[G];
+                int nn[G];
+                float qxs[G], qys[G], dxs[G], dys[G], scs[G], axs[G], ays[G];
+                float td[G][4], tc[G][4], tg[G][4], t9[G][9];
+#pragma unroll
+                for (int j = 0; j < G; ++j) {
+                    ok[j] = false;
+                    ok3[j] = false;
+                    nn[j] = -1;
+                    if (g0 + j < nv) {
+                        const int b = __ffs(m) - 1;
+                        m &= m - 1u;
+                        const int n = b < cntv ? s_nb[b] : -1;
+                        if (n >= 0) {
+                            const float2 cn = s_cn[b];
+                            const float Bx = cv.x - cn.x, By = cv.y - cn.y;
+                            const float bl = sqrtf(Bx * Bx + By * By);
+                            if (!(bl < 1.0e-6f)) {
+                                const float dx = Bx / bl, dy = By / bl;
+                                const float scale = bl / a.base;
+                                const float qx = px + dx * d * scale;
+                                const float qy = py + dy * d * scale;
+                                if (!(qx < 0 || qy < 0 || qx > W - 1 || qy > W - 1)) {
+                                    ok[j] = true;
+                                    nn[j] = n;
+                                    qxs[j] = qx; qys[j] = qy;
+                                    dxs[j] = dx; dys[j] = dy; scs[j] = scale;
+                                }
+                            }
+                        }
+                    }
+                    if (ok[j]) {
+                        const int n = nn[j];
+                        const int x0 = (int)floorf(qxs[j]), y0 = (int)floorf(qys[j]);
+                        const int x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, W - 1);
+                        axs[j] = qxs[j] - (float)x0;
+                        ays[j] = qys[j] - (float)y0;
+                        const int o[4] = {y0 * W + x0, y1 * W + x0, y0 * W + x1, y1 * W + x1};
+                        const size_t nb0 = (size_t)n * WW;
+                        const bool filtered = ORDERED && n < v;
+                        const float4* pl = filtered ? a.plane_out : a.plane_in;
+                        const float* co = filtered ? a.cost_out : a.cost_in;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) td[j][q] = pl[nb0 + o[q]].w;
+                        if (fp.enable_cost_filter) {
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) tc[j][q] = co[nb0 + o[q]];
+                        }
+                        if (fp.enable_cycle_photo_filter) {
+                            const float* gn = a.gray + nb0;
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) tg[j][q] = gn[o[q]];
+                            const int cx = (int)roundf(qxs[j]), cy = (int)roundf(qys[j]);
+                            ok3[j] = !(cx < 1 || cy < 1 || cx >= W - 1 || cy >= W - 1);
+                            if (ok3[j]) {
+#pragma unroll
+                                for (int q = 0; q < 9; ++q)
+                                    t9[j][q] = gn[(cy - 1 + q / 3) * W + cx - 1 + q % 3];
+                            }
+                        }
+                    }
                 }
-                ++support;
-                bool pg = true, pp = true;
-                if (fp.enable_cycle_geo_filter) {
-                    const float phx = qx - dx * dn * scale;
-                    const float phy = qy - dy * dn * scale;
-                    const float ex = phx - px, ey = phy - py;
-                    const float ge = sqrtf(ex * ex + ey * ey);
-                    pg = isfinite(ge) && ge <= fp.max_geo_err_px;
+                // phase 2: the decisions, view by view in order, with the operations and
+                // their order of the per-view functions above
+#pragma unroll
+                for (int j = 0; j < G; ++j) {
+                    if (!ok[j]) continue;
+                    const float ax = axs[j], ay = ays[j];
+                    bool dvalid = true;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        dvalid = dvalid && isfinite(td[j][q]) && !(td[j][q] <= fp.min_valid_disp);
+                    if (!dvalid) continue;
+                    const float dn = (1.0f - ax) * (1.0f - ay) * td[j][0] +
+                                     (1.0f - ax) * ay * td[j][1] + ax * (1.0f - ay) * td[j][2] +
+                                     ax * ay * td[j][3];
+                    if (!isfinite(dn)) continue;
+                    if (fp.enable_cost_filter) {
+                        float nc;
+                        if (!isfinite(tc[j][0]) || !isfinite(tc[j][1]) || !isfinite(tc[j][2]) ||
+                            !isfinite(tc[j][3]))
+                            nc = __int_as_float(0x7fc00000);
+                        else
+                            nc = (1.0f - ax) * (1.0f - ay) * tc[j][0] +
+                                 (1.0f - ax) * ay * tc[j][1] + ax * (1.0f - ay) * tc[j][2] +
+                                 ax * ay * tc[j][3];
+                        const bool inv = !isfinite(nc) || nc < 0.0f;
+                        const bool hi = isfinite(nc) && nc > fp.max_cost;
+                        if (inv || hi) continue;
+                    }
+                    ++support;
+                    bool pg = true, pp = true;
+                    if (fp.enable_cycle_geo_filter) {
+                        const float phx = qxs[j] - dxs[j] * dn * scs[j];
+                        const float phy = qys[j] - dys[j] * dn * scs[j];
+                        const float ex = phx - px, ey = phy - py;
+                        const float ge = sqrtf(ex * ex + ey * ey);
+                        pg = isfinite(ge) && ge <= fp.max_geo_err_px;
+                    }
+                    if (fp.enable_cycle_photo_filter) {
+                        const float In = (1.0f - ax) * (1.0f - ay) * tg[j][0] +
+                                         (1.0f - ax) * ay * tg[j][1] +
+                                         ax * (1.0f - ay) * tg[j][2] + ax * ay * tg[j][3];
+                        float pe;
+                        bool nb3 = false;
+                        float mn = 0.0f, sn = 0.0f;
+                        if (own3 && ok3[j]) {
+                            double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                            for (int q = 0; q < 9; ++q) {
+                                const double g = (double)t9[j][q];
+                                s1 += g;
+                                s2 += g * g;
+                            }
+                            const double scale = 1.0 / 9.0;
+                            const double mm = s1 * scale;
+                            const double var = fmax(s2 * scale - mm * mm, 0.0);
+                            mn = (float)mm;
+                            sn = (float)sqrt(var);
+                            nb3 = isfinite(mn) && isfinite(sn) && sn > 1.0e-3f;
+                        }
+                        if (own3 && nb3) {
+                            const float ra = (Ir - mr) / sr;
+                            const float rb = (In - mn) / sn;
+                            pe = fabsf(ra - rb);
+                        } else {
+                            pe = fabsf(In - Ir);
+                        }
+                        pp = isfinite(pe) && pe <= fp.max_photo_err_u;
+                    }
+                    if (pg && pp) ++good;
                 }
-                if (fp.enable_cycle_photo_filter) {
-                    const float pe = f_photo(a, v, n, px, py, qx, qy);
-                    pp = isfinite(pe) && pe <= fp.max_photo_err_u;
-                }
-                if (pg && pp) ++good;
             }
             if (support < fp.min_good_neighbors) reason = LFPM_PR_NOSUPPORT;
             else if (good < fp.min_good_neighbors) reason = LFPM_PR_CYCLE;
