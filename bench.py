@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="batched",
                     choices=["plain", "batched", "mixed", "serve", "precision", "parity",
-                             "ingest", "products", "composites"])
+                             "ingest", "products", "composites", "filter"])
     ap.add_argument("--hypotheses", type=int, default=1000000, help="precision: hypotheses")
     ap.add_argument("--batch", type=int, default=4, help="frames per batch (batched / mixed)")
     ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
@@ -615,6 +615,77 @@ def products_section(args, lfpm, torch, wl):
                                        "%dx%d frame, one frame in %.1f s" % (w, h, cpu_s)}}
 
 
+def filter_section(args, lfpm, torch, wl):
+    """The CheckModule pass over one solved frame in both orders: lfpm_filter_frame (snapshot,
+    one launch) and lfpm_filter_frame_ordered (the reference's MI order, one launch per level of
+    the schedule), timed with HIP events on the launch stream, with the share of pixels whose
+    reason differs between the two.  CPU baseline: the oracle's FP32 twins of both orders on
+    the same solved frame."""
+    df = lfpm.DeviceFrame(wl)
+    bufs = lfpm.patchmatch_frame(df)
+    torch.cuda.synchronize()
+    V, W = wl.V, wl.crop
+    px = V * W * W
+    levels = len(lfpm._df_levels(df)[1]) - 1
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    out = {}
+    ms = {}
+    for order in ("snapshot", "reference"):
+        for _ in range(args.warmup):
+            lfpm.filter_frame(df, bufs, order=order)
+        torch.cuda.synchronize()
+        marks = []
+        for _ in range(args.steps):
+            e0 = ev()
+            out[order] = lfpm.filter_frame(df, bufs, order=order)
+            marks.append((e0, ev()))
+        torch.cuda.synchronize()
+        ms[order] = float(np.mean([a.elapsed_time(b) for a, b in marks]))
+    rs = out["snapshot"]["reason"].cpu().numpy()
+    rr = out["reference"]["reason"].cpu().numpy()
+    # compulsory bytes per pixel: plane 16 + cost 4 + disparity record 16 + selection 4 + gray 4
+    # read, the same four outputs (40 B) and the reason byte written
+    per_px = 44.0 + 41.0
+    gbs = per_px * px / (ms["snapshot"] * 1e-3) / 1e9
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    host = dict(gray=df.gray.cpu().numpy(), plane=bufs.plane.cpu().numpy(),
+                cost=bufs.cost.cpu().numpy(), disp=bufs.disp.cpu().numpy(),
+                sel=bufs.sel.cpu().numpy().view(np.uint32))
+    args_o = (host["gray"], wl.L["centers"], wl.L["nbr"], wl.L["cnt"], W, wl.pm.max_neighbors,
+              wl.pm.base, oracle.FilterParams(*[getattr(wl.fp, f[0]) for f in
+                                                oracle.FilterParams._fields_]),
+              host["plane"], host["cost"],
+              host["disp"], host["sel"])
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    os_ = oracle.filter_f32(*args_o, threads=threads)
+    snap_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    or_ = oracle.filter_f32_inplace(*args_o)
+    ord_s = time.perf_counter() - t0
+    same = bool(np.array_equal(os_["reason"], rs) and np.array_equal(or_["reason"], rr))
+    return {"ms": ms, "levels": levels, "us_per_level": 1e3 * ms["reference"] / levels,
+            "pixels": px, "kept_snapshot": float((rs == 0).mean()),
+            "kept_reference": float((rr == 0).mean()), "reason_differs": float((rr != rs).mean()),
+            "reasons_equal_oracle": same,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
+                         "frac": gbs / 8000.0, "traffic": None,
+                         "kernel": "k_filter<false> (lfpm_filter_frame)",
+                         "avg_launch_ms": ms["snapshot"],
+                         "per_unit": "85 B per MI pixel compulsory (44 B read, 41 B written)"},
+            "cpu_baseline": {"value": 1.0 / ord_s, "unit": "frames/s", "cores": 1,
+                             "kind": "port", "cpu": cpu_model(),
+                             "sample": "oracle FP32 twin in the reference's order (one thread, "
+                                       "%.2f s) on the same solved frame; the snapshot twin on "
+                                       "%d threads took %.2f s" % (ord_s, threads, snap_s)}}
+
+
 def composites_section(args, lfpm, torch, wl):
     """Refocus-side composites of one solved frame with a colour raw image: the all-in-focus
     composite with its depth maps, hole filling and denoising (lfpm_aif_composite) and the
@@ -880,6 +951,20 @@ def main():
                     config={"workload": "disparity mosaic + VD median splat with gray colour "
                                         "image + VD point cloud of one solved %dx%d frame "
                                         "(%d MIs)" % (rw, rh, wl.V)})
+        print(json.dumps(line), flush=True)
+        return
+    if args.mode == "filter":
+        wl = lfpm.workload_c2(raw_w=rw, raw_h=rh)
+        r = filter_section(args, lfpm, torch, wl)
+        roof, cpu = r.pop("roofline"), r.pop("cpu_baseline")
+        line.update(metric="CheckModule passes per s (%dx%d, the reference's MI order)"
+                    % (rw, rh), value=1e3 / r["ms"]["reference"],
+                    ms_per_step=r["ms"]["reference"], dtype="fp32", roofline=roof,
+                    cpu_baseline=cpu, filter=r,
+                    config={"workload": "cycle-consistency / photometric / spike filter of one "
+                                        "solved %dx%d frame (%d MIs, 18 neighbours, at most 4 "
+                                        "views per pixel), snapshot and reference order"
+                                        % (rw, rh, wl.V)})
         print(json.dumps(line), flush=True)
         return
     if args.mode == "composites":
