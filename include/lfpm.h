@@ -740,6 +740,8 @@ typedef struct lfpm_serve_opts {
     size_t hbm_budget_bytes;
     int max_slots;
     int streams;
+    int filter_order;  /* LFPM_FILTER_SNAPSHOT (0) or LFPM_FILTER_REFERENCE_ORDER: order of every
+                          request's CheckModule pass (lfpm_ctx_set_filter_order) */
 } lfpm_serve_opts;
 
 typedef struct lfpm_frame_request {

@@ -60,12 +60,14 @@ struct Job {
     lfpm_pm_params pm{};
     lfpm_filter_params fp{};
     bool filt = false;
+    bool ordered = false;                // filter in the reference's MI order
     int slices = 0;
     float vmin = 0.0f, vmax = 0.0f;
     int raw_w = 0, raw_h = 0, V = 0, crop = 0, K = 0, cols = 0, rows = 0;
     // host tables of the layout
     std::vector<float> centers;
     std::vector<int> origins, nbr, cnt, l2m;
+    std::vector<int> lvl_order, lvl_start;  // level schedule of the ordered filter
     uint8_t* raw_pinned = nullptr;       // the raw frame, copied at submit
     // device block and its buffers
     void* dev = nullptr;
@@ -73,6 +75,7 @@ struct Job {
     uint8_t* raw_d = nullptr;
     float *tiles = nullptr, *gray = nullptr, *centers_d = nullptr;
     int *origins_d = nullptr, *nbr_d = nullptr, *cnt_d = nullptr, *l2m_d = nullptr;
+    int* lvl_order_d = nullptr;
     float *plane = nullptr, *cost = nullptr, *disp = nullptr, *rec = nullptr;
     uint32_t* sel = nullptr;
     float *plane_b = nullptr, *cost_b = nullptr;
@@ -117,6 +120,7 @@ size_t slot_bytes(const Job& j) {
     b += a256(n * 16);                                     // disp
     b += a256(n * (5 + (size_t)j.K) * 4);                  // cost records
     if (j.filt) b += a256(n * 16) * 2 + a256(n * 4) * 2 + a256(n);
+    if (j.ordered) b += a256((size_t)j.V * 4);
     b += a256((size_t)j.slices * j.raw_w * j.raw_h * 4);
     return b;
 }
@@ -132,6 +136,7 @@ struct lfpm_server {
     int device = 0;
     size_t budget = 0;
     int max_slots = 16;
+    int filter_order = LFPM_FILTER_SNAPSHOT;
     std::vector<hipStream_t> streams;
     std::thread worker;
     std::mutex mu;
@@ -206,6 +211,7 @@ int admit(Job* j, hipStream_t st) {
         j->fsel = (uint32_t*)take(n * 4);
         j->reason = (uint8_t*)take(n);
     }
+    if (j->ordered) j->lvl_order_d = (int*)take((size_t)j->V * 4);
     if (j->slices > 0) j->refocus = (float*)take((size_t)j->slices * j->raw_w * j->raw_h * 4);
     S_TRY(hipMemcpyAsync(j->raw_d, j->raw_pinned, (size_t)j->raw_w * j->raw_h,
                          hipMemcpyHostToDevice, st));
@@ -216,6 +222,9 @@ int admit(Job* j, hipStream_t st) {
     S_TRY(hipMemcpyAsync(j->nbr_d, j->nbr.data(), j->nbr.size() * 4, hipMemcpyHostToDevice, st));
     S_TRY(hipMemcpyAsync(j->cnt_d, j->cnt.data(), j->cnt.size() * 4, hipMemcpyHostToDevice, st));
     S_TRY(hipMemcpyAsync(j->l2m_d, j->l2m.data(), j->l2m.size() * 4, hipMemcpyHostToDevice, st));
+    if (j->ordered)
+        S_TRY(hipMemcpyAsync(j->lvl_order_d, j->lvl_order.data(), j->lvl_order.size() * 4,
+                             hipMemcpyHostToDevice, st));
     int rc;
     if ((rc = lfpm_slice_tiles(j->raw_d, j->raw_w, j->raw_h, (size_t)j->raw_w, j->origins_d, j->V,
                                j->crop, j->tiles, st)) ||
@@ -266,12 +275,21 @@ lfpm::StageFrame stage_of(const Job* j) {
 int finish(Job* j, hipStream_t st) {
     int rc;
     const size_t n = (size_t)j->V * j->crop * j->crop;
-    if (j->filt &&
-        (rc = lfpm_filter_frame(j->gray, j->centers_d, j->nbr_d, j->cnt_d, j->V, j->crop, j->K,
-                                j->pm.base, &j->fp, j->plane, j->cost, j->disp, j->sel,
-                                j->fplane, j->fcost, j->fdisp, j->fsel, j->reason, st))) {
-        j->err = lfpm_last_error();
-        return rc;
+    if (j->filt) {
+        if (j->ordered)
+            rc = lfpm_filter_frame_ordered(j->gray, j->centers_d, j->nbr_d, j->cnt_d, j->V,
+                                           j->crop, j->K, j->pm.base, &j->fp, j->plane, j->cost,
+                                           j->disp, j->sel, j->fplane, j->fcost, j->fdisp,
+                                           j->fsel, j->reason, j->lvl_order_d,
+                                           j->lvl_start.data(), (int)j->lvl_start.size() - 1, st);
+        else
+            rc = lfpm_filter_frame(j->gray, j->centers_d, j->nbr_d, j->cnt_d, j->V, j->crop,
+                                   j->K, j->pm.base, &j->fp, j->plane, j->cost, j->disp, j->sel,
+                                   j->fplane, j->fcost, j->fdisp, j->fsel, j->reason, st);
+        if (rc) {
+            j->err = lfpm_last_error();
+            return rc;
+        }
     }
     if (j->slices > 0 &&
         (rc = lfpm_refocus(j->tiles, j->centers_d, j->l2m_d, j->cols, j->rows, j->lp.pitch,
@@ -464,6 +482,12 @@ int lfpm_server_create(int device, const lfpm_serve_opts* opts, lfpm_server** ou
     s->budget = opts && opts->hbm_budget_bytes ? opts->hbm_budget_bytes : freeb / 2;
     s->max_slots = opts && opts->max_slots > 0 ? opts->max_slots : 16;
     const int ns = opts && opts->streams > 0 ? opts->streams : 4;
+    if (opts && opts->filter_order != LFPM_FILTER_SNAPSHOT &&
+        opts->filter_order != LFPM_FILTER_REFERENCE_ORDER) {
+        delete s;
+        return set_error(LFPM_E_INVALID, "server: unknown filter order %d", opts->filter_order);
+    }
+    s->filter_order = opts ? opts->filter_order : LFPM_FILTER_SNAPSHOT;
     for (int i = 0; i < ns; ++i) {
         hipStream_t st;
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
@@ -544,6 +568,19 @@ int lfpm_frame_submit(lfpm_server* s, const lfpm_frame_request* r, uint64_t* tic
                                 j->nbr.data(), j->cnt.data(), j->l2m.data()))) {
         delete j;
         return rc;
+    }
+    j->ordered = j->filt && s->filter_order == LFPM_FILTER_REFERENCE_ORDER;
+    if (j->ordered) {
+        j->lvl_order.resize(j->V);
+        j->lvl_start.resize((size_t)j->V + 1);
+        int levels = 0;
+        if ((rc = lfpm_filter_order_levels(j->nbr.data(), j->cnt.data(), j->V, j->K,
+                                           j->lvl_order.data(), j->lvl_start.data(),
+                                           &levels))) {
+            delete j;
+            return rc;
+        }
+        j->lvl_start.resize((size_t)levels + 1);
     }
     if (hipSetDevice(s->device) != hipSuccess ||
         hipHostMalloc((void**)&j->raw_pinned, (size_t)j->raw_w * j->raw_h) != hipSuccess) {

@@ -17,7 +17,8 @@ import lfpm
 
 
 class ServeOpts(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_slots", C.c_int), ("streams", C.c_int)]
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_slots", C.c_int), ("streams", C.c_int),
+                ("filter_order", C.c_int)]
 
 
 class FrameRequest(C.Structure):
@@ -61,12 +62,17 @@ class Server:
     """A frame server on one device (lfpm_server): submit() returns at once, wait() blocks for
     that request's results."""
 
-    def __init__(self, hbm_budget_bytes=0, max_slots=16, streams=4, device=0, keep_results=True):
+    def __init__(self, hbm_budget_bytes=0, max_slots=16, streams=4, device=0, keep_results=True,
+                 filter_order="snapshot"):
+        codes = {"snapshot": 0, "reference": 1}
+        if filter_order not in codes:
+            raise ValueError("Server: filter_order must be 'snapshot' or 'reference'")
         L = lfpm.lib()
         self._L = L
         self.keep_results = keep_results
         h = C.c_void_p()
-        opts = ServeOpts(int(hbm_budget_bytes), int(max_slots), int(streams))
+        opts = ServeOpts(int(hbm_budget_bytes), int(max_slots), int(streams),
+                         codes[filter_order])
         lfpm._check(L.lfpm_server_create(int(device), C.byref(opts), C.byref(h)))
         self._h = h
         self.done = []

@@ -35,10 +35,10 @@ def _workloads(lfpm):
     ]
 
 
-def _solo(lfpm, torch, wl):
+def _solo(lfpm, torch, wl, order="snapshot"):
     df = lfpm.DeviceFrame(wl)
     bufs = lfpm.patchmatch_frame(df)
-    fo = lfpm.filter_frame(df, bufs)
+    fo = lfpm.filter_frame(df, bufs, order=order)
     rf = lfpm.refocus(df, slices=wl.slices)
     torch.cuda.synchronize()
     return dict(plane=fo["plane"].cpu().numpy(), cost=fo["cost"].cpu().numpy(),
@@ -46,12 +46,12 @@ def _solo(lfpm, torch, wl):
                 reason=fo["reason"].cpu().numpy(), refocus=rf.cpu().numpy())
 
 
-def _serve(lfpm, torch, arrivals):
+def _serve(lfpm, torch, arrivals, order="snapshot"):
     import serving
     wls = _workloads(lfpm)
-    solo = [_solo(lfpm, torch, wl) for wl in wls]
+    solo = [_solo(lfpm, torch, wl, order) for wl in wls]
     reqs = [serving.Request(i, float(arrivals[i]), wl) for i, wl in enumerate(wls)]
-    srv = serving.Server(hbm_budget_bytes=8 << 30, max_slots=8, streams=4)
+    srv = serving.Server(hbm_budget_bytes=8 << 30, max_slots=8, streams=4, filter_order=order)
     try:
         _say("case serve: %d requests, MIs %s" % (len(reqs), [wl.V for wl in wls]))
         done = srv.run(reqs)
@@ -83,6 +83,12 @@ def test_served_burst_shares_launches(lfpm, torch_cuda):
     st, stages = _serve(lfpm, torch_cuda, np.zeros(6))
     assert st["max_active"] == 6
     assert st["launches"] < stages
+
+
+def test_served_reference_order_equals_solo(lfpm, torch_cuda):
+    """A server created with the reference filter order (lfpm_serve_opts.filter_order): every
+    request's reject mask and filtered outputs equal its solo solve + ordered pass."""
+    _serve(lfpm, torch_cuda, np.zeros(6), order="reference")
 
 
 def test_served_frame_over_budget_is_refused(lfpm, torch_cuda):
