@@ -6,6 +6,7 @@ FilterGlobal, CheckModule/MIDisparityFilter.cpp:973-985, invalidation :671-686).
 - the 320 x 256 frame at the small-frame parameters and the 4080 x 3068 frame at the full-frame
   parameters, one launch per level of the schedule: reason masks and outputs bit-exact vs the
   twin, and the result differs from the snapshot pass (rejections cascade);
+- a three-lens-type layout (asymmetric neighbour table) likewise;
 - two calls on the full frame are identical.
 """
 import time
@@ -71,6 +72,15 @@ def test_filter_ordered_small_frame(lfpm, orc, torch_cuda):
     _check_ordered(lfpm, orc, torch_cuda, wl, "ordered c1")
 
 
+def test_filter_ordered_three_lens_types(lfpm, orc, torch_cuda):
+    """Per-type neighbour caps (6, 12, 18) make the neighbour table asymmetric: an MI can read a
+    neighbour that does not read it back."""
+    wl = lfpm.workload_c2(raw_w=300, raw_h=260, pitch=15.0, radius=5, lens_types=3,
+                          per_type=(6, 12, 18), iterations=2)
+    _say("case filter ordered, 3 lens types")
+    _check_ordered(lfpm, orc, torch_cuda, wl, "ordered 3 types")
+
+
 def test_filter_ordered_full_frame(lfpm, orc, torch_cuda):
     wl = lfpm.workload_c2()
     _say("case filter ordered 4080x3068")
@@ -114,3 +124,31 @@ def test_ctx_and_batch_reference_order(lfpm, torch_cuda):
     got = dict(plane=slot.f_plane, cost=slot.f_cost, sel=slot.f_sel, reason=slot.reason)
     for k, v in got.items():
         assert np.array_equal(v.cpu().numpy().view(np.uint8), ref[k].view(np.uint8)), k
+
+
+def test_filter_ordered_higher_index_neighbour_earlier(lfpm, orc, torch_cuda):
+    """An edited neighbour table in which odd MIs keep only their higher-index neighbours: those
+    MIs sit in level 0, so an even MI reads higher-index neighbours that the schedule filters
+    before it (or beside it).  The pass must still give them their pre-filter values, as the
+    reference's map order does: bit-exact vs the in-place twin on the same table."""
+    wl = lfpm.workload_c1(raw_w=320, raw_h=256, frame_uid=21)
+    V, K = wl.V, wl.pm.max_neighbors
+    nbr = np.asarray(wl.L["nbr"]).reshape(V, K).copy()
+    cnt = np.asarray(wl.L["cnt"]).copy()
+    for v in range(1, V, 2):
+        keep = [int(n) for n in nbr[v, :cnt[v]] if n > v]
+        if keep:
+            nbr[v] = -1
+            nbr[v, :len(keep)] = keep
+            cnt[v] = len(keep)
+    wl.L["nbr"] = nbr.reshape(np.asarray(wl.L["nbr"]).shape)
+    wl.L["cnt"] = cnt.astype(np.asarray(wl.L["cnt"]).dtype)
+    order, start = lfpm.filter_levels(wl.L["nbr"], wl.L["cnt"], K)
+    lev = np.empty(V, np.int64)
+    for l in range(len(start) - 1):
+        lev[order[start[l]:start[l + 1]]] = l
+    early = sum(1 for v in range(V) for n in nbr[v, :cnt[v]] if n > v and lev[n] <= lev[v])
+    _say("case filter ordered, edited table: %d reads of a higher-index MI filtered no later"
+         % early)
+    assert early > 0
+    _check_ordered(lfpm, orc, torch_cuda, wl, "ordered edited table")
