@@ -1934,13 +1934,19 @@ __global__ void __launch_bounds__(1024) k_filter(FilterArgs a) {
         s_nb[t] = n;
         s_cn[t] = n >= 0 ? a.centers[n] : make_float2(0.0f, 0.0f);
     }
+    // the pixel's own state (kept for the outputs) and its disparity in LDS for the spike test
+    __shared__ float s_d[LFPM_MAX_CROP * LFPM_MAX_CROP];
+    float4 p0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), r0 = p0;
     float d = 0.0f, cost0 = 0.0f;
     uint32_t sel0 = 0u;
     // the pixel's own photometric terms, the same for every view
     float Ir = 0.0f, mr = 0.0f, sr = 0.0f;
     bool own3 = false;
     if (t < WW) {
-        d = a.plane_in[base + t].w;
+        p0 = a.plane_in[base + t];
+        r0 = a.disp_in[base + t];
+        d = p0.w;
+        s_d[t] = d;
         cost0 = a.cost_in[base + t];
         sel0 = a.sel_in[base + t];
         if (fp.enable_cycle_photo_filter) {
@@ -2125,27 +2131,40 @@ __global__ void __launch_bounds__(1024) k_filter(FilterArgs a) {
         bool kf = k1;
         if (k1 && fp.enable_spike_filter) {
             const int x = t % W, y = t / W;
+            // the kept 8-neighbours' disparities in fixed slots (+inf where none), sorted by an
+            // 19-comparator network in registers: the sorted values, and so the median
+            // vals[nvals / 2], are those of sorting the kept values alone
             float vals[8];
             int nvals = 0;
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    if (dx == 0 && dy == 0) continue;
-                    const int xn = x + dx, yn = y + dy;
-                    if (xn < 0 || yn < 0 || xn >= W || yn >= W) continue;
-                    const int ni = yn * W + xn;
-                    if (!keep1[ni]) continue;
-                    const float dv = a.plane_in[base + ni].w;
-                    if (!isfinite(dv) || dv <= fp.min_valid_disp) continue;
-                    vals[nvals++] = dv;
-                }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int dy = (q < 3) ? -1 : (q < 5 ? 0 : 1);
+                const int dx = (q < 3) ? q - 1 : (q < 5 ? (q == 3 ? -1 : 1) : q - 6);
+                const int xn = x + dx, yn = y + dy;
+                vals[q] = __int_as_float(0x7f800000);
+                if (xn < 0 || yn < 0 || xn >= W || yn >= W) continue;
+                const int ni = yn * W + xn;
+                if (!keep1[ni]) continue;
+                const float dv = s_d[ni];
+                if (!isfinite(dv) || dv <= fp.min_valid_disp) continue;
+                vals[q] = dv;
+                ++nvals;
+            }
             if (nvals >= fp.spike_min_neighbors) {
-                for (int i = 1; i < nvals; ++i) {  // insertion sort (<= 8 values)
-                    const float key = vals[i];
-                    int j = i - 1;
-                    while (j >= 0 && vals[j] > key) { vals[j + 1] = vals[j]; --j; }
-                    vals[j + 1] = key;
+                constexpr int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6},
+                                            {5, 7}, {1, 2}, {5, 6}, {0, 4}, {3, 7}, {1, 5}, {2, 6},
+                                            {1, 4}, {3, 6}, {2, 4}, {3, 5}, {3, 4}};
+#pragma unroll
+                for (int c = 0; c < 19; ++c) {
+                    const float lo = fminf(vals[net[c][0]], vals[net[c][1]]);
+                    const float hi = fmaxf(vals[net[c][0]], vals[net[c][1]]);
+                    vals[net[c][0]] = lo;
+                    vals[net[c][1]] = hi;
                 }
-                const float med = vals[nvals / 2];
+                float med = vals[0];
+#pragma unroll
+                for (int q = 1; q < 8; ++q)
+                    if (q == nvals / 2) med = vals[q];
                 if (isfinite(d) && fabsf(d - med) > fp.spike_abs_diff) {
                     kf = false;
                     reason = LFPM_PR_SPIKE;
@@ -2153,10 +2172,10 @@ __global__ void __launch_bounds__(1024) k_filter(FilterArgs a) {
             }
         }
         const size_t g = base + t;
-        float4 po = a.plane_in[g];
-        float co = a.cost_in[g];
-        float4 dout = a.disp_in[g];
-        uint32_t so = a.sel_in[g];
+        float4 po = p0;
+        float co = cost0;
+        float4 dout = r0;
+        uint32_t so = sel0;
         if (isfinite(d) && d > fp.min_valid_disp && !kf) {
             po = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
             co = 2.0f;
