@@ -1929,10 +1929,10 @@ __global__ void __launch_bounds__(1024) k_filter(FilterArgs a) {
     // before the barrier so both sets of loads are in flight together
     __shared__ int s_nb[LFPM_MAX_NEIGHBORS];
     __shared__ float2 s_cn[LFPM_MAX_NEIGHBORS];
-    if (t < cntv) {
+    if (t < a.K) {  // the whole row, loaded beside the count (slots past it are never read)
         const int n = a.nbr[(size_t)v * a.K + t];
         s_nb[t] = n;
-        s_cn[t] = n >= 0 ? a.centers[n] : make_float2(0.0f, 0.0f);
+        s_cn[t] = (n >= 0 && n < a.V) ? a.centers[n] : make_float2(0.0f, 0.0f);
     }
     // the pixel's own state (kept for the outputs) and its disparity in LDS for the spike test
     __shared__ float s_d[LFPM_MAX_CROP * LFPM_MAX_CROP];
