@@ -209,6 +209,22 @@ def measured_traffic(row):
     return None
 
 
+def filter_traffic():
+    """HBM bytes per lfpm_filter_frame dispatch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated), or None if absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_plain_4080x3068.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == "lfpm::k_filter<false>":
+            try:
+                return float(cells[4])
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_model():
     """The host CPU's model name (first processor in /proc/cpuinfo), or None."""
     try:
@@ -675,7 +691,10 @@ def filter_section(args, lfpm, torch, wl):
             "kept_reference": float((rr == 0).mean()), "reason_differs": float((rr != rs).mean()),
             "reasons_equal_oracle": same,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
-                         "frac": gbs / 8000.0, "traffic": None,
+                         "frac": gbs / 8000.0, "traffic": filter_traffic(),
+                         "traffic_source": "profiles/hbm_traffic_plain_4080x3068.md "
+                                           "(committed rocprofv3 FETCH_SIZE / WRITE_SIZE "
+                                           "summary, not measured in this run)",
                          "kernel": "k_filter<false> (lfpm_filter_frame)",
                          "avg_launch_ms": ms["snapshot"],
                          "per_unit": "85 B per MI pixel compulsory (44 B read, 41 B written)"},
