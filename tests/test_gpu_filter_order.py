@@ -6,7 +6,8 @@ FilterGlobal, CheckModule/MIDisparityFilter.cpp:973-985, invalidation :671-686).
 - the 320 x 256 frame at the small-frame parameters and the 4080 x 3068 frame at the full-frame
   parameters, one launch per level of the schedule: reason masks and outputs bit-exact vs the
   twin, and the result differs from the snapshot pass (rejections cascade);
-- a three-lens-type layout (asymmetric neighbour table) likewise;
+- a three-lens-type layout (asymmetric neighbour table), a crop-21 frame and eight checked views
+  per pixel likewise, the snapshot pass against its own twin each time;
 - two calls on the full frame are identical.
 """
 import time
@@ -38,10 +39,12 @@ def _check_ordered(lfpm, orc, torch, wl, tag):
     df = lfpm.DeviceFrame(wl)
     bufs = lfpm.patchmatch_frame(df)
     snap = lfpm.filter_frame(df, bufs)
+    lfpm.filter_levels(wl.L["nbr"], wl.L["cnt"], wl.pm.max_neighbors)
+    torch.cuda.synchronize()
     t0 = time.time()
     ordr = lfpm.filter_frame(df, bufs, order="reference")
     torch.cuda.synchronize()
-    _say("%s: V=%d, %d levels, ordered pass + sync %.1f ms"
+    _say("%s: V=%d, %d levels, ordered pass (first call, schedule built) + sync %.1f ms"
          % (tag, wl.V, len(df._filter_levels[1]) - 1, 1e3 * (time.time() - t0)))
     g = _host(ordr)
     s = _host(snap)
@@ -52,6 +55,11 @@ def _check_ordered(lfpm, orc, torch, wl, tag):
     o = orc.filter_f32_inplace(df.gray.cpu().numpy(), wl.L["centers"], wl.L["nbr"], wl.L["cnt"],
                                wl.crop, wl.pm.max_neighbors, wl.pm.base,
                                fp_to_oracle(orc, wl.fp), plane, cost, disp, sel)
+    os_ = orc.filter_f32(df.gray.cpu().numpy(), wl.L["centers"], wl.L["nbr"], wl.L["cnt"],
+                         wl.crop, wl.pm.max_neighbors, wl.pm.base, fp_to_oracle(orc, wl.fp),
+                         plane, cost, disp, sel)
+    for k in ("reason", "plane", "cost", "disp", "sel"):
+        assert np.array_equal(s[k].view(np.uint8), os_[k].view(np.uint8)), ("snapshot", k)
     reasons = {int(r): int((g["reason"] == r).sum()) for r in (0, 80, 140, 200, 255)}
     differ = float((g["reason"] != s["reason"]).mean())
     _say("%s reasons" % tag, reasons, "differ from snapshot %.4f" % differ)
@@ -152,3 +160,18 @@ def test_filter_ordered_higher_index_neighbour_earlier(lfpm, orc, torch_cuda):
          % early)
     assert early > 0
     _check_ordered(lfpm, orc, torch_cuda, wl, "ordered edited table")
+
+
+@pytest.mark.parametrize("case", ["crop21", "eight_views"])
+def test_filter_orders_other_shapes(lfpm, orc, torch_cuda, case):
+    """Both orders vs their twins on a crop-21 frame (448-thread workgroups, 19 x 19 window) and
+    with every one of the first 8 neighbours checked (max_triplet 8, selection ignored: four
+    rounds of paired views per pixel)."""
+    if case == "crop21":
+        wl = lfpm.workload_c2(raw_w=330, raw_h=280, pitch=31.0, radius=9, iterations=2)
+    else:
+        wl = lfpm.workload_c2(raw_w=400, raw_h=320, iterations=2)
+        wl.fp.max_triplet = 8
+        wl.fp.use_selected_views_only = 0
+    _say("case filter orders, %s: crop %d, V=%d" % (case, wl.crop, wl.V))
+    _check_ordered(lfpm, orc, torch_cuda, wl, "ordered " + case)
