@@ -103,3 +103,17 @@ def test_facade_library_exports_reference_entry_points():
                  "LFMVS::AdaptMIPM::RunPatchMatchCUDAForMI()",
                  "LFMVS::AdaptMIPM::Initialize("):
         assert name in syms, name
+
+
+def test_filter_order_arguments_checked_on_the_host(lfpm):
+    """The filter-order switches reject unknown orders before any device work."""
+    import sys
+    import pytest
+    sys.path.insert(0, PKG)
+    import serving
+    for call in (lambda: lfpm.filter_frame(None, None, order="inplace"),
+                 lambda: lfpm.FrameSlot(None, filter_order="inplace"),
+                 lambda: serving.Server(filter_order="inplace")):
+        with pytest.raises(ValueError):
+            call()
+    assert lfpm.lib().lfpm_ctx_set_filter_order(None, 1) == -1
