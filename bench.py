@@ -1018,7 +1018,7 @@ def main():
         line.update(value=plain["fps"], ms_per_step=plain["ms_per_step"],
                     p50_ms=line["plain_p50_ms"], p99_ms=line["plain_p99_ms"])
         cfg["workload"] = ("plain GPU single-frame solve (init + %d red-black iterations + disp "
-                           "record) + CheckModule filter + %d-slice refocus, one launch per "
+                           "record) + CheckModule filter (snapshot order) + %d-slice refocus, one launch per "
                            "phase with a sync after each" % (wl.pm.iterations, wl.slices))
     else:
         if args.mode == "mixed":
@@ -1034,7 +1034,8 @@ def main():
                                            noise_seed=4322 + 17 * i, frame_uid=i)
                           for i in range(1, args.batch)]
             cfg["workload"] = ("%d frames of %dx%d in one captured hipGraph (persistent k_stage_w "
-                               "launches over all frames' MIs) + per-frame CheckModule filter + "
+                               "launches over all frames' MIs) + per-frame CheckModule filter (snapshot "
+                               "order) + "
                                "%d-slice refocus" % (args.batch, rw, rh, wl.slices))
         bat = batched_section(args, lfpm, torch, wls)
         if args.mode == "mixed":
