@@ -1034,9 +1034,9 @@ def main():
                                            noise_seed=4322 + 17 * i, frame_uid=i)
                           for i in range(1, args.batch)]
             cfg["workload"] = ("%d frames of %dx%d in one captured hipGraph (persistent k_stage_w "
-                               "launches over all frames' MIs) + per-frame CheckModule filter (snapshot "
-                               "order) + "
-                               "%d-slice refocus" % (args.batch, rw, rh, wl.slices))
+                               "launches over all frames' MIs) + per-frame CheckModule filter "
+                               "(snapshot order) + %d-slice refocus"
+                               % (args.batch, rw, rh, wl.slices))
         bat = batched_section(args, lfpm, torch, wls)
         if args.mode == "mixed":
             pf = plain_frames_section(args, lfpm, torch, wls)
