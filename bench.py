@@ -1018,8 +1018,9 @@ def main():
         line.update(value=plain["fps"], ms_per_step=plain["ms_per_step"],
                     p50_ms=line["plain_p50_ms"], p99_ms=line["plain_p99_ms"])
         cfg["workload"] = ("plain GPU single-frame solve (init + %d red-black iterations + disp "
-                           "record) + CheckModule filter (snapshot order) + %d-slice refocus, one launch per "
-                           "phase with a sync after each" % (wl.pm.iterations, wl.slices))
+                           "record) + CheckModule filter (snapshot order) + %d-slice refocus, one "
+                           "launch per phase with a sync after each"
+                           % (wl.pm.iterations, wl.slices))
     else:
         if args.mode == "mixed":
             log("bench: synthesising %d mixed frames" % args.batch)
