@@ -306,3 +306,35 @@ def test_filter_ordered_rejects_bad_schedules(lfpm):
     assert call([0, 3], 0) == -1
     assert call([0, 2, 3], 2, out_plane=fake) == -1   # output aliases the input
     assert b"alias" in lfpm.lib().lfpm_last_error()
+
+
+def test_inplace_order_equals_iterated_snapshot(orc, lfpm):
+    """The reference's order restated from the snapshot form: MI v's result is the snapshot
+    filter of the state that MIs 0..v-1 left, restricted to MI v (one full snapshot call per MI
+    on a small frame).  The in-place twin must equal it bit for bit."""
+    wl = lfpm.workload_c1(raw_w=256, raw_h=200)
+    V, W = wl.V, wl.crop
+    WW = W * W
+    rng = np.random.default_rng(17)
+    gray = rng.integers(0, 256, (V, W, W)).astype(np.float32)
+    plane, costs, disp, sel = _inputs(wl)
+    # a varied state: disparities around 5 px with spikes, some high costs
+    plane[:, 3] = (5.0 + rng.normal(0.0, 0.3, plane.shape[0])).astype(np.float32)
+    plane[rng.random(plane.shape[0]) < 0.05, 3] += 2.0
+    costs[rng.random(costs.shape[0]) < 0.1] = 1.95
+    fp = _fp(orc, wl, enable_cycle_photo_filter=0)
+    want = {k: np.empty_like(a) for k, a in
+            dict(plane=plane, cost=costs, disp=disp, sel=sel).items()}
+    want["reason"] = np.empty(V * WW, np.uint8)
+    cur_plane, cur_cost = plane.copy(), costs.copy()
+    for v in range(V):
+        o = _run(orc, wl, gray, fp, cur_plane, cur_cost, disp, sel)
+        sl = slice(v * WW, (v + 1) * WW)
+        for k in ("plane", "cost", "disp", "sel", "reason"):
+            want[k][sl] = o[k][sl]
+        cur_plane[sl] = o["plane"][sl]
+        cur_cost[sl] = o["cost"][sl]
+    got = _run_inplace(orc, wl, gray, fp, plane, costs, disp, sel)
+    assert (want["reason"] != PR_KEEP).any() and (want["reason"] == PR_KEEP).any()
+    for k in want:
+        assert np.array_equal(got[k].view(np.uint8), want[k].view(np.uint8)), k
