@@ -8,7 +8,8 @@ FilterGlobal, CheckModule/MIDisparityFilter.cpp:973-985, invalidation :671-686).
   twin, and the result differs from the snapshot pass (rejections cascade);
 - a three-lens-type layout (asymmetric neighbour table), a crop-21 frame and eight checked views
   per pixel likewise, the snapshot pass against its own twin each time;
-- two calls on the full frame are identical.
+- two calls on the full frame are identical;
+- the ordered pass captured in a batch graph replays bit-exactly.
 """
 import time
 
@@ -175,3 +176,32 @@ def test_filter_orders_other_shapes(lfpm, orc, torch_cuda, case):
         wl.fp.use_selected_views_only = 0
     _say("case filter orders, %s: crop %d, V=%d" % (case, wl.crop, wl.V))
     _check_ordered(lfpm, orc, torch_cuda, wl, "ordered " + case)
+
+
+def test_batch_graph_reference_order_replays(lfpm, torch_cuda):
+    """The ordered pass inside a captured batch graph (one kernel node per level): three
+    replays, with the outputs scribbled over between them, each equal the plain solve + ordered
+    pass bit for bit."""
+    wl = lfpm.workload_c1(raw_w=320, raw_h=256, frame_uid=31)
+    _say("case batch graph, reference order: V=%d" % wl.V)
+    df = lfpm.DeviceFrame(wl)
+    bufs = lfpm.patchmatch_frame(df)
+    ref = _host(lfpm.filter_frame(df, bufs, order="reference"))
+    torch_cuda.cuda.synchronize()
+    slot = lfpm.FrameSlot(wl, refocus=False, filter_order="reference")
+    b = lfpm.Batch([slot])
+    b.capture()
+    try:
+        for rep in range(3):
+            for t in (slot.f_plane, slot.f_cost, slot.plane, slot.cost):
+                t.fill_(-7.0)
+            slot.reason.fill_(9)
+            torch_cuda.cuda.synchronize()
+            b.replay()
+            torch_cuda.cuda.synchronize()
+            got = dict(plane=slot.f_plane, cost=slot.f_cost, sel=slot.f_sel, reason=slot.reason)
+            for k, v in got.items():
+                assert np.array_equal(v.cpu().numpy().view(np.uint8),
+                                      ref[k].view(np.uint8)), (rep, k)
+    finally:
+        b.close()
